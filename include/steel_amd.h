@@ -1,0 +1,103 @@
+/* C ABI of the MI355X Steel step library (libsteel_amd.so).
+ *
+ * The drop-in boundary for the reference's step path: the JS `Sim` (packages/steel-webgpu/src/sim.js:63-835,
+ * typed in src/sim.d.ts:53-92) keeps its API and scene packing and calls these functions where it used
+ * WebGPU.  Plain pointers and sizes only; every call returns 0 (STEEL_OK) or a negative status, with the
+ * message in steel_last_error().  The library owns all device memory: create copies the caller's arrays,
+ * read copies into caller memory, destroy frees the world's memory but never the device.
+ *
+ * A "batch" is N independent worlds stepped together (one launch per kernel for all of them); a plain
+ * per-world Sim is a batch of one.
+ */
+#ifndef STEEL_AMD_H
+#define STEEL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "steel_world.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STEEL_OK 0
+#define STEEL_ERROR -1
+#define STEEL_UNSUPPORTED -2   /* no usable GPU; the JS side maps it to UnsupportedError (src/device.js:1-9) */
+#define STEEL_BAD_ARGUMENT -3
+#define STEEL_OUT_OF_MEMORY -4
+
+/* Buffer ids for steel_batch_read / steel_batch_write (names of sim.js:503-519). */
+enum steel_buffer {
+  STEEL_BUF_POSES = 0,     /* float4[W_NPOSES]: body COM pose, shape poses, AABBs, previous poses */
+  STEEL_BUF_VEL = 1,       /* float4[2*NB] */
+  STEEL_BUF_ART = 2,       /* float[W_NART] */
+  STEEL_BUF_ANCHOR_KEYS = 3,  /* uint32[2*AH] */
+  STEEL_BUF_ANCHOR_VALS = 4,  /* uint32[2*AH*STEEL_AVN] */
+  STEEL_BUF_PAIRS = 5,     /* uint32[2*MAXP], the last substep's broadphase pairs (unordered) */
+  STEEL_BUF_CONTACTS = 6,  /* float4[6*MAXC], the last substep's contacts in canonical order */
+  STEEL_BUF_ROWS = 7,      /* float4[MAXC*ROWS] */
+  STEEL_BUF_SCHED = 8,     /* uint32 schedule tables */
+  STEEL_BUF_COUNTERS = 9,  /* uint32[4]: pairs, contacts, substep count, flags */
+  STEEL_BUF_STATS = 10,    /* uint32[16], the reference's stats words (src/wgsl.js:3879-3890) */
+  STEEL_BUF_COUNT = 11
+};
+
+/* Stage mask for steel_batch_substep_stages (stage-isolated parity runs). */
+#define STEEL_STAGE_COLLIDE 1u
+#define STEEL_STAGE_NARROW 2u
+#define STEEL_STAGE_PREPARE 4u
+#define STEEL_STAGE_SOLVE 8u
+#define STEEL_STAGE_ALL 15u
+
+typedef struct steel_device steel_device;
+typedef struct steel_batch steel_batch;
+
+/* One packed world (include/steel_world.h): the arrays sim.js:95-611 builds, plus the initial state. */
+typedef struct steel_world_desc {
+  const uint32_t* header;  /* STEEL_WORLD_HEADER_WORDS words */
+  const float* statF;      /* 4 * header[W_NSTATF] floats */
+  const uint32_t* statU;   /* header[W_NSTATU] words (at least 1) */
+  const float* poses0;     /* 4 * header[W_NPOSES] floats */
+  const float* vel0;       /* 8 * header[W_NB] floats */
+  const float* art0;       /* header[W_NART] floats */
+} steel_world_desc;
+
+/* Thread-local message of the last failed call. */
+const char* steel_last_error(void);
+
+/* Device acquisition (src/device.js:1-27 initDevice). Returns STEEL_UNSUPPORTED without a GPU. */
+int steel_device_open(int ordinal, steel_device** out);
+int steel_device_close(steel_device* dev);
+/* Name and memory of the device, for logs. */
+int steel_device_info(steel_device* dev, char* name, size_t name_len, size_t* total_mem, uint32_t* num_cus);
+
+/* Batch of worlds: new Sim(...) for each (sim.js:64-611); arrays are copied. */
+int steel_batch_create(steel_device* dev, uint32_t nworlds, const steel_world_desc* worlds, steel_batch** out);
+int steel_batch_destroy(steel_batch* b);
+/* Sim.reset (sim.js:613-623): initial poses/vel/art, zero anchors, counters and stats. world < 0: all. */
+int steel_batch_reset(steel_batch* b, int32_t world);
+/* Sim.step(frames) (sim.js:692-701): frames x substeps x (collide, prepare, solve) on the batch stream,
+ * asynchronous.  With graphs enabled the frame is a captured hipGraph replayed per frame. */
+int steel_batch_step(steel_batch* b, uint32_t frames);
+/* One substep with only the given stages (STEEL_STAGE_*), for stage-isolated parity checks. */
+int steel_batch_substep_stages(steel_batch* b, uint32_t stages);
+/* Wait for the batch's stream. */
+int steel_batch_sync(steel_batch* b);
+/* Sim.read / queue.writeBuffer on one world's buffer (sim.js:716-802); read synchronises the stream. */
+int steel_batch_read(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_off, size_t bytes, void* dst);
+int steel_batch_write(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_off, size_t bytes, const void* src);
+/* Whole-buffer gather of one buffer for all worlds into dst (worlds back to back, each its own size). */
+int steel_batch_buffer_size(steel_batch* b, uint32_t world, uint32_t buf, size_t* bytes);
+/* Sim.stats (sim.js:822-834): the 16 stats words of one world and the GPU time of the last step. */
+int steel_batch_stats(steel_batch* b, uint32_t world, uint32_t out[16], double* gpu_ms);
+/* Launch options: workgroup size of prepare/solve (64, 128 or 256) and hipGraph capture on/off. */
+int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph);
+/* Device bytes the batch holds (admission by HBM budget). */
+int steel_batch_device_bytes(steel_batch* b, size_t* bytes);
+/* Device bytes a batch of these worlds would hold, without creating it. */
+int steel_estimate_bytes(uint32_t nworlds, const steel_world_desc* worlds, size_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

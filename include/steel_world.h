@@ -1,0 +1,55 @@
+/* Packed world format: the data contract between the host packer and the step kernels.
+ *
+ * One world is a set of flat arrays with the record layouts of the reference's buffers
+ * (packages/steel-webgpu/src/sim.js:212-237 statF/statU sections, :438 poses, :452-455 art, :5-22 names):
+ *   statF  float4[W_NSTATF]   body / shape / hull / link / dof records (read only)
+ *   statU  uint32[W_NSTATU]   hull topology, articulation table, collision-filter bitset (read only)
+ *   poses  float4[W_NPOSES]   body COM pose, shape poses, shape AABBs, previous body poses
+ *   vel    float4[2*NB]       free-body COM twists (v, 0), (w, 0)
+ *   art    float [W_NART]     articulation state and per-step factor blocks
+ *   warm   uint32[W_NWARM]    friction-anchor tables (two parities)
+ * plus a header of uint32 words, indexed by the enum below (float fields hold float bits).
+ * Capacities (MAXP, MAXC, ...) are per world; a world that overflows one drops work and reports it in
+ * its stats, as the reference does (sim.js:826-828).
+ */
+#ifndef STEEL_WORLD_H
+#define STEEL_WORLD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum steel_world_field {
+  /* counts */
+  W_NB = 0, W_NS, W_NH, W_NA, W_NL, W_ND, W_NQ, W_NDA, W_JM, W_MAXPATH, W_MAXLP,
+  /* capacities */
+  W_MAXP, W_MAXC, W_MAXU, W_UH, W_AH, W_MAXDENSE, W_NCOLW,
+  /* row record shape and switches */
+  W_ROWS, W_PO, W_PATCH, W_EXTRA, W_FRICTION_COMBINE, W_RESTITUTION_COMBINE, W_REST,
+  /* statF sections (float4 index) */
+  W_SF_BODY, W_SF_SHAPE, W_SF_HAABB, W_SF_VERTS, W_SF_FPLANE, W_SF_LINK, W_SF_DOF, W_NSTATF,
+  /* statU sections (uint32 index) */
+  W_SU_HULL, W_SU_FACE, W_SU_FV, W_SU_HEDGE, W_SU_EDGE, W_SU_ART, W_SU_FILTER, W_NSTATU,
+  /* poses sections (float4 index) */
+  W_P_BODY, W_P_SHAPE, W_P_AABB, W_P_PREV, W_NPOSES,
+  /* art sections (float index) */
+  W_A_Q, W_A_QD, W_A_QDH, W_A_TGT, W_A_TGTV, W_A_LIM, W_A_DRV, W_A_JF, W_A_BF, W_A_FRIC, W_A_HINV,
+  W_A_LINK, W_A_LG, W_A_RL, W_A_RLN, W_A_ULOAD, W_A_DENSE, W_NART,
+  W_DHDR, W_DPS, W_NWARM,
+  /* solver constants (float bits) */
+  W_F_CFM, W_F_OMEGA, W_F_SPEC, W_F_GAP_GATE, W_F_FGAP, W_F_FSCALE, W_F_ANCHOR_BETA, W_F_BP_CELL,
+  W_F_REST_SPEED, W_F_JL_GAP, W_F_JL_SPEED,
+  /* per-substep parameters (float bits unless noted) */
+  W_DT, W_BETA, W_MAX_DEPEN, W_GX, W_GY, W_GZ, W_WARM_SCALE, W_REG, W_ITERS /* u32 */, W_SUBSTEPS /* u32 */,
+  W_FRAME_DT,
+  W_FIELD_COUNT
+};
+
+#define STEEL_WORLD_HEADER_WORDS 128
+#define STEEL_LB 268      /* floats per link block in art (artwgsl.js:8) */
+#define STEEL_AVN 14      /* u32 per friction-anchor value record (wgsl.js:96) */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

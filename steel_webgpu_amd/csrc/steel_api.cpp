@@ -1,0 +1,395 @@
+// Host runtime and C ABI of the Steel step library (declared in include/steel_amd.h).
+//
+// A batch keeps every per-world buffer in one device arena per buffer kind (one hipMalloc each, 256-byte
+// aligned slices), a device table of per-world pointers, a copy of the initial state for reset, and one
+// HIP stream.  A frame is S substeps x (k_collide, k_prepare, k_solve); with graphs on, the frame is
+// captured once and replayed, since nothing in it depends on the host (the substep parity lives in each
+// world's device counter).
+#include <hip/hip_runtime.h>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "steel_dev.h"
+#include "steel_launch.h"
+#include "../../include/steel_amd.h"
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(STEEL_ERROR, "%s: %s", #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+enum Kind { K_HDR, K_STATF, K_STATU, K_POSES, K_VEL, K_ART, K_AKEY, K_AVAL, K_CTR, K_STATS, K_PAIRS, K_CRAW, K_CONTACTS, K_ROWS, K_SCHED,
+            K_SKEY, K_SVAL, K_TW, K_IMP, K_AQD, K_POSES0, K_VEL0, K_ART0, K_COUNT };
+
+struct Sizes { size_t b[K_COUNT]; };
+
+int world_sizes(const uint32_t* h, Sizes& s) {
+  uint32_t NB = h[W_NB], NS = h[W_NS], ND = h[W_ND], MAXC = h[W_MAXC], MAXP = h[W_MAXP], AH = h[W_AH], ROWS = h[W_ROWS];
+  if (MAXC < NS || MAXC == 0) return fail(STEEL_BAD_ARGUMENT, "world capacity MAXC=%u is below its shape count %u", MAXC, NS);
+  if (h[W_NDA] > 64 || h[W_MAXLP] > 64) return fail(STEEL_BAD_ARGUMENT, "articulations of more than 64 dofs or links are not supported");
+  if (h[W_MAXPATH] > 64 || h[W_MAXPATH] * h[W_JM] > 192) return fail(STEEL_BAD_ARGUMENT, "articulation root paths deeper than 64 links are not supported");
+  if (h[W_UH] > MAXC || h[W_MAXU] > MAXC) return fail(STEEL_BAD_ARGUMENT, "unit capacities exceed the contact capacity");
+  SchedLayout L = sched_layout(MAXC, h[W_MAXU], h[W_UH], h[W_NCOLW] * 32, NB, h[W_NCOLW], h[W_NL]);
+  s.b[K_HDR] = STEEL_WORLD_HEADER_WORDS * 4;
+  s.b[K_STATF] = (size_t)h[W_NSTATF] * 16;
+  s.b[K_STATU] = (size_t)std::max(h[W_NSTATU], 1u) * 4;
+  s.b[K_POSES] = (size_t)h[W_NPOSES] * 16;
+  s.b[K_VEL] = (size_t)std::max(NB, 1u) * 32;
+  s.b[K_ART] = (size_t)h[W_NART] * 4;
+  s.b[K_AKEY] = (size_t)2 * AH * 4;
+  s.b[K_AVAL] = (size_t)2 * AH * STEEL_AVN * 4;
+  s.b[K_CTR] = 16;
+  s.b[K_STATS] = 64;
+  s.b[K_PAIRS] = (size_t)2 * MAXP * 4;
+  s.b[K_CRAW] = (size_t)6 * MAXC * 16;
+  s.b[K_CONTACTS] = (size_t)6 * MAXC * 16;
+  s.b[K_ROWS] = (size_t)MAXC * ROWS * 16;
+  s.b[K_SCHED] = (size_t)L.TOTAL * 4;
+  s.b[K_SKEY] = (size_t)2 * MAXC * 8;
+  s.b[K_SVAL] = (size_t)2 * MAXC * 4;
+  s.b[K_TW] = (size_t)std::max(NB, 1u) * 32;
+  s.b[K_IMP] = (size_t)std::max(NB, 1u) * 32;
+  s.b[K_AQD] = (size_t)std::max(ND, 1u) * 4;
+  s.b[K_POSES0] = s.b[K_POSES];
+  s.b[K_VEL0] = s.b[K_VEL];
+  s.b[K_ART0] = s.b[K_ART];
+  return STEEL_OK;
+}
+}  // namespace
+
+struct steel_device {
+  int ordinal;
+  hipDeviceProp_t prop;
+};
+
+struct steel_batch {
+  steel_device* dev;
+  uint32_t n;
+  uint32_t substeps;
+  uint32_t block = 256;
+  bool use_graph = true;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  char* arena[K_COUNT] = {};
+  std::vector<size_t> off[K_COUNT];
+  std::vector<size_t> size[K_COUNT];
+  size_t total[K_COUNT] = {};
+  WorldDev* d_worlds = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint32_t> headers;  // n * 128
+};
+
+extern "C" {
+
+const char* steel_last_error(void) { return g_err.c_str(); }
+
+int steel_device_open(int ordinal, steel_device** out) {
+  if (!out) return fail(STEEL_BAD_ARGUMENT, "null output");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(STEEL_UNSUPPORTED, "no HIP device is available");
+  if (ordinal < 0 || ordinal >= count) return fail(STEEL_BAD_ARGUMENT, "device ordinal %d out of range (%d devices)", ordinal, count);
+  steel_device* d = new steel_device();
+  d->ordinal = ordinal;
+  if (hipSetDevice(ordinal) != hipSuccess || hipGetDeviceProperties(&d->prop, ordinal) != hipSuccess) {
+    delete d;
+    return fail(STEEL_UNSUPPORTED, "cannot open HIP device %d", ordinal);
+  }
+  if (std::strncmp(d->prop.gcnArchName, "gfx950", 6) != 0) {
+    std::string arch = d->prop.gcnArchName;
+    delete d;
+    return fail(STEEL_UNSUPPORTED, "device %d is %s; this library is built for gfx950", ordinal, arch.c_str());
+  }
+  *out = d;
+  return STEEL_OK;
+}
+
+int steel_device_close(steel_device* dev) {
+  delete dev;
+  return STEEL_OK;
+}
+
+int steel_device_info(steel_device* dev, char* name, size_t name_len, size_t* total_mem, uint32_t* num_cus) {
+  if (!dev) return fail(STEEL_BAD_ARGUMENT, "null device");
+  if (name && name_len) { std::snprintf(name, name_len, "%s (%s)", dev->prop.name, dev->prop.gcnArchName); }
+  if (total_mem) *total_mem = dev->prop.totalGlobalMem;
+  if (num_cus) *num_cus = (uint32_t)dev->prop.multiProcessorCount;
+  return STEEL_OK;
+}
+
+int steel_estimate_bytes(uint32_t nworlds, const steel_world_desc* worlds, size_t* bytes) {
+  if (!bytes || (nworlds && !worlds)) return fail(STEEL_BAD_ARGUMENT, "null argument");
+  size_t tot = 0;
+  for (uint32_t i = 0; i < nworlds; i++) {
+    Sizes s;
+    int rc = world_sizes(worlds[i].header, s);
+    if (rc) return rc;
+    for (int k = 0; k < K_COUNT; k++) tot += align256(s.b[k]);
+  }
+  tot += align256((size_t)nworlds * sizeof(WorldDev));
+  *bytes = tot;
+  return STEEL_OK;
+}
+
+static void batch_free(steel_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (int k = 0; k < K_COUNT; k++) if (b->arena[k]) (void)hipFree(b->arena[k]);
+  if (b->d_worlds) (void)hipFree(b->d_worlds);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+int steel_batch_reset(steel_batch* b, int32_t world) {
+  if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  uint32_t lo = 0, hi = b->n;
+  if (world >= 0) {
+    if ((uint32_t)world >= b->n) return fail(STEEL_BAD_ARGUMENT, "world %d out of range", world);
+    lo = (uint32_t)world; hi = lo + 1;
+  }
+  if (world < 0) {
+    HIPCHK(hipMemcpyAsync(b->arena[K_POSES], b->arena[K_POSES0], b->total[K_POSES], hipMemcpyDeviceToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->arena[K_VEL], b->arena[K_VEL0], b->total[K_VEL], hipMemcpyDeviceToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->arena[K_ART], b->arena[K_ART0], b->total[K_ART], hipMemcpyDeviceToDevice, b->stream));
+    for (int k : {K_AKEY, K_AVAL, K_CTR, K_STATS, K_TW, K_IMP, K_AQD, K_ROWS, K_SCHED}) HIPCHK(hipMemsetAsync(b->arena[k], 0, b->total[k], b->stream));
+  } else {
+    for (uint32_t i = lo; i < hi; i++) {
+      HIPCHK(hipMemcpyAsync(b->arena[K_POSES] + b->off[K_POSES][i], b->arena[K_POSES0] + b->off[K_POSES0][i], b->size[K_POSES][i], hipMemcpyDeviceToDevice, b->stream));
+      HIPCHK(hipMemcpyAsync(b->arena[K_VEL] + b->off[K_VEL][i], b->arena[K_VEL0] + b->off[K_VEL0][i], b->size[K_VEL][i], hipMemcpyDeviceToDevice, b->stream));
+      HIPCHK(hipMemcpyAsync(b->arena[K_ART] + b->off[K_ART][i], b->arena[K_ART0] + b->off[K_ART0][i], b->size[K_ART][i], hipMemcpyDeviceToDevice, b->stream));
+      for (int k : {K_AKEY, K_AVAL, K_CTR, K_STATS, K_TW, K_IMP, K_AQD, K_ROWS, K_SCHED})
+        HIPCHK(hipMemsetAsync(b->arena[k] + b->off[k][i], 0, b->size[k][i], b->stream));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return STEEL_OK;
+}
+
+int steel_batch_create(steel_device* dev, uint32_t nworlds, const steel_world_desc* worlds, steel_batch** out) {
+  if (!dev || !out || nworlds == 0 || !worlds) return fail(STEEL_BAD_ARGUMENT, "bad batch arguments");
+  HIPCHK(hipSetDevice(dev->ordinal));
+  steel_batch* b = new steel_batch();
+  b->dev = dev;
+  b->n = nworlds;
+  b->substeps = worlds[0].header[W_SUBSTEPS];
+  b->headers.resize((size_t)nworlds * STEEL_WORLD_HEADER_WORDS);
+  for (int k = 0; k < K_COUNT; k++) { b->off[k].resize(nworlds); b->size[k].resize(nworlds); }
+  for (uint32_t i = 0; i < nworlds; i++) {
+    const steel_world_desc& w = worlds[i];
+    if (!w.header || !w.statF || !w.statU || !w.poses0 || !w.vel0 || !w.art0) { delete b; return fail(STEEL_BAD_ARGUMENT, "world %u: null array", i); }
+    if (w.header[W_SUBSTEPS] != b->substeps) { delete b; return fail(STEEL_BAD_ARGUMENT, "world %u: all worlds of a batch step the same substep count", i); }
+    Sizes s;
+    int rc = world_sizes(w.header, s);
+    if (rc) { delete b; return rc; }
+    for (int k = 0; k < K_COUNT; k++) { b->off[k][i] = b->total[k]; b->size[k][i] = s.b[k]; b->total[k] += align256(s.b[k]); }
+    std::memcpy(&b->headers[(size_t)i * STEEL_WORLD_HEADER_WORDS], w.header, STEEL_WORLD_HEADER_WORDS * 4);
+  }
+  for (int k = 0; k < K_COUNT; k++) {
+    if (hipMalloc(&b->arena[k], std::max<size_t>(b->total[k], 256)) != hipSuccess) {
+      batch_free(b);
+      delete b;
+      return fail(STEEL_OUT_OF_MEMORY, "device allocation of %zu bytes failed", b->total[k]);
+    }
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+      hipEventCreate(&b->ev1) != hipSuccess) {
+    batch_free(b);
+    delete b;
+    return fail(STEEL_ERROR, "stream/event creation failed");
+  }
+  // Upload static arrays and initial state through one pinned staging buffer per kind.
+  std::vector<char> host;
+  auto upload = [&](int k, const void* (*get)(const steel_world_desc&), const steel_world_desc* ws) -> int {
+    host.assign(b->total[k], 0);
+    for (uint32_t i = 0; i < nworlds; i++) std::memcpy(host.data() + b->off[k][i], get(ws[i]), b->size[k][i]);
+    HIPCHK(hipMemcpy(b->arena[k], host.data(), b->total[k], hipMemcpyHostToDevice));
+    return STEEL_OK;
+  };
+  int rc = 0;
+  rc |= upload(K_HDR, [](const steel_world_desc& w) -> const void* { return w.header; }, worlds);
+  rc |= upload(K_STATF, [](const steel_world_desc& w) -> const void* { return w.statF; }, worlds);
+  rc |= upload(K_STATU, [](const steel_world_desc& w) -> const void* { return w.statU; }, worlds);
+  rc |= upload(K_POSES0, [](const steel_world_desc& w) -> const void* { return w.poses0; }, worlds);
+  rc |= upload(K_VEL0, [](const steel_world_desc& w) -> const void* { return w.vel0; }, worlds);
+  rc |= upload(K_ART0, [](const steel_world_desc& w) -> const void* { return w.art0; }, worlds);
+  if (rc) { batch_free(b); delete b; return STEEL_ERROR; }
+  std::vector<WorldDev> wd(nworlds);
+  for (uint32_t i = 0; i < nworlds; i++) {
+    auto P = [&](int k) { return b->arena[k] + b->off[k][i]; };
+    WorldDev& x = wd[i];
+    x.hdr = (const uint32_t*)P(K_HDR);
+    x.statF = (const float4*)P(K_STATF);
+    x.statU = (const uint32_t*)P(K_STATU);
+    x.poses = (float4*)P(K_POSES);
+    x.vel = (float4*)P(K_VEL);
+    x.art = (float*)P(K_ART);
+    x.akey = (uint32_t*)P(K_AKEY);
+    x.aval = (uint32_t*)P(K_AVAL);
+    x.ctr = (uint32_t*)P(K_CTR);
+    x.stats = (uint32_t*)P(K_STATS);
+    x.pairs = (uint32_t*)P(K_PAIRS);
+    x.craw = (float4*)P(K_CRAW);
+    x.contacts = (float4*)P(K_CONTACTS);
+    x.rows = (float4*)P(K_ROWS);
+    x.sched = (uint32_t*)P(K_SCHED);
+    x.skey = (uint64_t*)P(K_SKEY);
+    x.sval = (uint32_t*)P(K_SVAL);
+    x.tw = (float4*)P(K_TW);
+    x.imp = (float4*)P(K_IMP);
+    x.aqd = (float*)P(K_AQD);
+  }
+  if (hipMalloc(&b->d_worlds, sizeof(WorldDev) * nworlds) != hipSuccess ||
+      hipMemcpy(b->d_worlds, wd.data(), sizeof(WorldDev) * nworlds, hipMemcpyHostToDevice) != hipSuccess) {
+    batch_free(b);
+    delete b;
+    return fail(STEEL_OUT_OF_MEMORY, "world table allocation failed");
+  }
+  rc = steel_batch_reset(b, -1);
+  if (rc) { batch_free(b); delete b; return rc; }
+  *out = b;
+  return STEEL_OK;
+}
+
+int steel_batch_destroy(steel_batch* b) {
+  if (!b) return STEEL_OK;
+  (void)hipSetDevice(b->dev->ordinal);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+  delete b;
+  return STEEL_OK;
+}
+
+int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph) {
+  if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
+  if (block != 64 && block != 128 && block != 256) return fail(STEEL_BAD_ARGUMENT, "block must be 64, 128 or 256");
+  if (b->gexec && (block != b->block || (use_graph != 0) != b->use_graph)) {
+    (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr;
+    (void)hipGraphDestroy(b->graph); b->graph = nullptr;
+  }
+  b->block = block;
+  b->use_graph = use_graph != 0;
+  return STEEL_OK;
+}
+
+static int launch_frame(steel_batch* b) {
+  for (uint32_t s = 0; s < b->substeps; s++) HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->stream, STAGE_ALL));
+  return STEEL_OK;
+}
+
+int steel_batch_step(steel_batch* b, uint32_t frames) {
+  if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  if (frames == 0) return STEEL_OK;
+  if (b->use_graph && !b->gexec) {
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_frame(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(STEEL_ERROR, "graph capture failed: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+  }
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  for (uint32_t f = 0; f < frames; f++) {
+    if (b->use_graph) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    else { int rc = launch_frame(b); if (rc) return rc; }
+  }
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  b->timed = true;
+  return STEEL_OK;
+}
+
+int steel_batch_substep_stages(steel_batch* b, uint32_t stages) {
+  if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->stream, stages));
+  return STEEL_OK;
+}
+
+int steel_batch_sync(steel_batch* b) {
+  if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return STEEL_OK;
+}
+
+static int buf_kind(uint32_t buf) {
+  static const int map[STEEL_BUF_COUNT] = {K_POSES, K_VEL, K_ART, K_AKEY, K_AVAL, K_PAIRS, K_CONTACTS, K_ROWS, K_SCHED, K_CTR, K_STATS};
+  return buf < STEEL_BUF_COUNT ? map[buf] : -1;
+}
+
+int steel_batch_buffer_size(steel_batch* b, uint32_t world, uint32_t buf, size_t* bytes) {
+  if (!b || !bytes) return fail(STEEL_BAD_ARGUMENT, "null argument");
+  int k = buf_kind(buf);
+  if (k < 0 || world >= b->n) return fail(STEEL_BAD_ARGUMENT, "bad buffer %u or world %u", buf, world);
+  *bytes = b->size[k][world];
+  return STEEL_OK;
+}
+
+int steel_batch_read(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_off, size_t bytes, void* dst) {
+  if (!b || (!dst && bytes)) return fail(STEEL_BAD_ARGUMENT, "null argument");
+  int k = buf_kind(buf);
+  if (k < 0 || world >= b->n) return fail(STEEL_BAD_ARGUMENT, "bad buffer %u or world %u", buf, world);
+  if (byte_off + bytes > b->size[k][world]) return fail(STEEL_BAD_ARGUMENT, "read of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, b->size[k][world]);
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(dst, b->arena[k] + b->off[k][world] + byte_off, bytes, hipMemcpyDeviceToHost));
+  return STEEL_OK;
+}
+
+int steel_batch_write(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_off, size_t bytes, const void* src) {
+  if (!b || (!src && bytes)) return fail(STEEL_BAD_ARGUMENT, "null argument");
+  int k = buf_kind(buf);
+  if (k < 0 || world >= b->n) return fail(STEEL_BAD_ARGUMENT, "bad buffer %u or world %u", buf, world);
+  if (byte_off + bytes > b->size[k][world]) return fail(STEEL_BAD_ARGUMENT, "write of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, b->size[k][world]);
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  HIPCHK(hipMemcpyAsync(b->arena[k] + b->off[k][world] + byte_off, src, bytes, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return STEEL_OK;
+}
+
+int steel_batch_stats(steel_batch* b, uint32_t world, uint32_t out[16], double* gpu_ms) {
+  if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
+  if (out) { int rc = steel_batch_read(b, world, STEEL_BUF_STATS, 0, 64, out); if (rc) return rc; }
+  if (gpu_ms) {
+    *gpu_ms = 0.0;
+    if (b->timed) {
+      HIPCHK(hipEventSynchronize(b->ev1));
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+      *gpu_ms = ms;
+    }
+  }
+  return STEEL_OK;
+}
+
+int steel_batch_device_bytes(steel_batch* b, size_t* bytes) {
+  if (!b || !bytes) return fail(STEEL_BAD_ARGUMENT, "null argument");
+  size_t t = 0;
+  for (int k = 0; k < K_COUNT; k++) t += b->total[k];
+  *bytes = t + sizeof(WorldDev) * b->n;
+  return STEEL_OK;
+}
+
+}  // extern "C"

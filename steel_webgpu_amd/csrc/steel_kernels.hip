@@ -1,0 +1,1444 @@
+// Step kernels for gfx950.  Every kernel runs one world per workgroup over an array of worlds, so the
+// plain per-world step is a one-block launch and a batch of thousands of uneven worlds is one launch of
+// the same code.  Per substep: k_collide (shapes, broadphase, narrowphase, contact sort), k_prepare
+// (articulation factor, grouping, reduction, rows, coloring, dense rows), k_solve (colored PGS with
+// articulation propagation, integration, friction anchors), following the reference's kernels
+// shapes/broad/narrow/csort (wgsl.js:117-1215, 3895-3980), prepare (1787-2494) and solve (3633-3891).
+#include "steel_collide.cuh"
+#include "steel_launch.h"
+
+// ------------------------------------------------------------------------------------------------
+// Block helpers
+// ------------------------------------------------------------------------------------------------
+// Exclusive scan of one value per thread; returns the exclusive prefix, writes the block total.
+template <int BS>
+DEV uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t& total) {
+  uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+  for (int k = 0; k < BS / 64; k++) { uint32_t t = s_w[k]; if (k < (int)wave) pre += t; tot += t; }
+  __syncthreads();
+  total = tot;
+  return pre + x - v;
+}
+// In-place exclusive scan of a global array a[0..n); returns the total.
+template <int BS>
+DEV uint32_t block_scan_array(uint32_t* a, uint32_t n, uint32_t* s_w) {
+  uint32_t per = (n + BS - 1) / BS;
+  uint32_t lo_ = min(threadIdx.x * per, n), hi = min(lo_ + per, n);
+  uint32_t s = 0;
+  for (uint32_t i = lo_; i < hi; i++) s += a[i];
+  uint32_t total;
+  uint32_t off = block_excl_scan<BS>(s, s_w, total);
+  for (uint32_t i = lo_; i < hi; i++) { uint32_t v = a[i]; a[i] = off; off += v; }
+  __syncthreads();
+  return total;
+}
+
+// Stable LSD radix sort of n (key, value) pairs, 8-bit digits, digits that no key varies in skipped.
+// Ranks within a wave come from ballots (a peer mask per digit value), across waves from LDS counts.
+// Sorted output lands in (k0, v0).  LDS: s_hist[256], s_cnt[BS/64][256], s_red[2*BS/64].
+template <int BS>
+DEV void block_radix_sort(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint32_t n, uint32_t* s_hist, uint32_t* s_cnt,
+                          uint64_t* s_red) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr uint32_t NW = BS / 64;
+  // bits any two keys differ in
+  uint64_t kor = 0, kand = ~0ull;
+  for (uint32_t i = tid; i < n; i += BS) { kor |= k0[i]; kand &= k0[i]; }
+  for (int off = 32; off >= 1; off >>= 1) {
+    kor |= (uint64_t)__shfl_xor((unsigned long long)kor, off, 64);
+    kand &= (uint64_t)__shfl_xor((unsigned long long)kand, off, 64);
+  }
+  if (lane == 0) { s_red[wave] = kor; s_red[NW + wave] = kand; }
+  __syncthreads();
+  kor = 0; kand = ~0ull;
+  for (uint32_t k = 0; k < NW; k++) { kor |= s_red[k]; kand &= s_red[NW + k]; }
+  __syncthreads();
+  uint64_t vary = kor ^ kand;
+  uint64_t* src = k0; uint32_t* srcv = v0; uint64_t* dst = k1; uint32_t* dstv = v1;
+  for (uint32_t sh = 0; sh < 64; sh += 8) {
+    if (((vary >> sh) & 0xffull) == 0) continue;
+    for (uint32_t d = tid; d < 256; d += BS) s_hist[d] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += BS) atomicAdd(&s_hist[(uint32_t)(src[i] >> sh) & 0xffu], 1u);
+    __syncthreads();
+    if (tid == 0) { uint32_t acc = 0; for (uint32_t d = 0; d < 256; d++) { uint32_t c = s_hist[d]; s_hist[d] = acc; acc += c; } }
+    for (uint32_t i = tid; i < NW * 256; i += BS) s_cnt[i] = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += BS) {
+      uint32_t i = base + tid;
+      bool valid = i < n;
+      uint64_t key = valid ? src[i] : 0;
+      uint32_t val = valid ? srcv[i] : 0;
+      uint32_t dg = (uint32_t)(key >> sh) & 0xffu;
+      unsigned long long peers = __ballot(valid);
+      for (uint32_t b = 0; b < 8; b++) {
+        unsigned long long m = __ballot((dg >> b) & 1u);
+        peers &= ((dg >> b) & 1u) ? m : ~m;
+      }
+      uint32_t rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+      if (valid && rank == 0) s_cnt[wave * 256 + dg] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      if (valid) {
+        uint32_t pos = s_hist[dg] + rank;
+        for (uint32_t w2 = 0; w2 < wave; w2++) pos += s_cnt[w2 * 256 + dg];
+        dst[pos] = key; dstv[pos] = val;
+      }
+      __syncthreads();
+      for (uint32_t d = tid; d < 256; d += BS) {
+        uint32_t s = 0;
+        for (uint32_t w2 = 0; w2 < NW; w2++) { s += s_cnt[w2 * 256 + d]; s_cnt[w2 * 256 + d] = 0; }
+        s_hist[d] += s;
+      }
+      __syncthreads();
+    }
+    uint64_t* tk = src; src = dst; dst = tk;
+    uint32_t* tv = srcv; srcv = dstv; dstv = tv;
+  }
+  if (src != k0) {
+    for (uint32_t i = tid; i < n; i += BS) { k0[i] = src[i]; v0[i] = srcv[i]; }
+    __syncthreads();
+  }
+}
+
+DEV void load_header(uint32_t* Hs, const WorldDev& wd) {
+  for (uint32_t i = threadIdx.x; i < STEEL_WORLD_HEADER_WORDS; i += blockDim.x) Hs[i] = wd.hdr[i];
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_collide
+// ------------------------------------------------------------------------------------------------
+DEV void shape_pose(const W& w, uint32_t s) {  // wgsl.js:117-150
+  uint32_t o = w.so(s);
+  float4 r0 = w.sf(o);
+  int body = ti(r0.x);
+  uint32_t hull = tu(r0.y);
+  f3 bp = mk3(0, 0, 0);
+  q4 bq = mk4(0, 0, 0, 1);
+  if (body >= 0) {
+    f3 c = w.pose_c(body);
+    bq = w.pose_q(body);
+    bp = c - qrot(bq, w.body_com_local(body));
+  }
+  f3 sp = bp + qrot(bq, xyz(w.sf(o + 1)));
+  q4 sq = nrm4(qmul(bq, Q(w.sf(o + 2))));
+  uint32_t PS = w.h(W_P_SHAPE), PA = w.h(W_P_AABB);
+  w.d.poses[PS + s * 2] = F4(sp, 0.0f);
+  w.d.poses[PS + s * 2 + 1] = F4(sq);
+  if (w.sf(o + 3).x == 4.0f) {
+    w.d.poses[PA + s * 2] = make_float4(-1e6f, -1e6f, -1e6f, 0.0f);
+    w.d.poses[PA + s * 2 + 1] = make_float4(1e6f, 1e6f, 1e6f, 0.0f);
+    return;
+  }
+  f3 hc = xyz(w.sf(w.h(W_SF_HAABB) + hull * 2));
+  f3 hh = xyz(w.sf(w.h(W_SF_HAABB) + hull * 2 + 1));
+  m3 m = qmat(sq);
+  f3 c = sp + mul(m, hc);
+  f3 e = vabs(m.c[0]) * hh.x + vabs(m.c[1]) * hh.y + vabs(m.c[2]) * hh.z + mk3(r0.w, r0.w, r0.w);
+  w.d.poses[PA + s * 2] = F4(c - e, 0.0f);
+  w.d.poses[PA + s * 2 + 1] = F4(c + e, 0.0f);
+}
+
+// The reference's pair predicate (wgsl.js:158-172) minus the x-overlap the sweep already holds.
+DEV bool pair_kept(const W& w, uint32_t i, uint32_t j) {
+  int bi = ti(w.sf(w.so(i)).x), bj = ti(w.sf(w.so(j)).x);
+  if (bi == bj) return false;
+  if (bi < 0 && bj < 0) return false;
+  bool fi = bi < 0 || w.body_kind(bi) == 3 || (w.body_flags(bi) & 1) != 0;
+  bool fj = bj < 0 || w.body_kind(bj) == 3 || (w.body_flags(bj) & 1) != 0;
+  if (fi && fj) return false;
+  uint32_t NS = w.NS();
+  uint32_t k = i * NS + j;
+  if ((w.su(w.h(W_SU_FILTER) + (k >> 5)) & (1u << (k & 31))) != 0) return false;
+  uint32_t PA = w.h(W_P_AABB);
+  float4 lo_i = w.d.poses[PA + i * 2], hi_i = w.d.poses[PA + i * 2 + 1];
+  float4 lo_j = w.d.poses[PA + j * 2], hi_j = w.d.poses[PA + j * 2 + 1];
+  if (lo_i.x > hi_j.x || lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.x > hi_i.x || lo_j.y > hi_i.y || lo_j.z > hi_i.z) return false;
+  return true;
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_collide(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t stage_mask) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ uint32_t s_hist[256];
+  __shared__ uint32_t s_cnt[(BS / 64) * 256];
+  __shared__ uint64_t s_red[2 * (BS / 64)];
+  __shared__ uint32_t s_np, s_nc, s_tie;
+  uint32_t wid = blockIdx.x;
+  if (wid >= nworlds) return;
+  const WorldDev wd = worlds[wid];
+  load_header(Hs, wd);
+  W w = make_world(Hs, wd);
+  const uint32_t tid = threadIdx.x, NS = w.NS();
+  if (tid == 0) { s_np = 0; s_nc = 0; s_tie = 0; }
+  for (uint32_t s = tid; s < NS; s += BS) shape_pose(w, s);
+  __syncthreads();
+  // Broadphase: sort shapes by the order-preserving integer of AABB min x, then sweep the sorted list;
+  // pair (i, j) survives iff it passes the reference's all-pairs predicate.
+  uint32_t PA = w.h(W_P_AABB);
+  for (uint32_t s = tid; s < NS; s += BS) { wd.skey[s] = fflip_u(fbits(wd.poses[PA + s * 2].x)); wd.sval[s] = s; }
+  __syncthreads();
+  block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), NS, s_hist, s_cnt, s_red);
+  uint32_t MAXP = w.MAXP();
+  for (uint32_t i = tid; i < NS; i += BS) {
+    uint32_t si = wd.sval[i];
+    float hi = wd.poses[PA + si * 2 + 1].x;
+    for (uint32_t j = i + 1; j < NS; j++) {
+      uint32_t sj = wd.sval[j];
+      if (wd.poses[PA + sj * 2].x > hi) break;
+      uint32_t a = min(si, sj), b = max(si, sj);
+      if (!pair_kept(w, a, b)) continue;
+      uint32_t slot = atomicAdd(&s_np, 1u);
+      if (slot < MAXP) { wd.pairs[slot * 2] = a; wd.pairs[slot * 2 + 1] = b; }
+    }
+  }
+  __syncthreads();
+  uint32_t np = min(s_np, MAXP);
+  if (stage_mask & 1u) {
+    // Narrowphase: one wave64 per candidate pair.
+    Collider col;
+    col.w = w;
+    col.s_nc = &s_nc;
+    uint32_t lane = tid & 63, wave = tid >> 6;
+    for (uint32_t p = wave; p < np; p += BS / 64) col.narrow_pair(p, lane);
+    __syncthreads();
+    // Canonical order: (body pair, feature key) by the radix sort; equal pairs of those (a feature-hash
+    // collision) are ordered by (point bits, emission index) afterwards, as the reference's key does.
+    uint32_t nc = min(s_nc, w.MAXC());
+    for (uint32_t c = tid; c < nc; c += BS) {
+      float4 m = wd.craw[c * 6 + 2];
+      uint32_t pair = (uint32_t)(ti(m.x) + 1) | ((uint32_t)(ti(m.y) + 1) << 16);
+      uint32_t feat = tu(m.z) | (tu(m.w) << 16);
+      wd.skey[c] = ((uint64_t)pair << 32) | feat;
+      wd.sval[c] = c;
+    }
+    __syncthreads();
+    block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), nc, s_hist, s_cnt, s_red);
+    for (uint32_t i = tid; i + 1 < nc; i += BS) if (wd.skey[i] == wd.skey[i + 1]) s_tie = 1;
+    __syncthreads();
+    if (s_tie && tid == 0) {
+      auto bits_of = [&](uint32_t c) {
+        float4 p = wd.craw[c * 6];
+        return hash_u32(hash_u32(hash_u32(fbits(p.x)) ^ fbits(p.y)) ^ fbits(p.z)) ^ fbits(p.w);
+      };
+      for (uint32_t i = 1; i < nc; i++) {
+        uint64_t k = wd.skey[i];
+        uint32_t v = wd.sval[i];
+        uint32_t bv = bits_of(v);
+        int j = (int)i - 1;
+        while (j >= 0 && wd.skey[j] == k && (bits_of(wd.sval[j]) > bv || (bits_of(wd.sval[j]) == bv && wd.sval[j] > v))) {
+          wd.skey[j + 1] = wd.skey[j]; wd.sval[j + 1] = wd.sval[j]; j--;
+        }
+        wd.skey[j + 1] = k; wd.sval[j + 1] = v;
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nc * 6; i += BS) {
+      uint32_t r = i / 6, q = i % 6;
+      wd.contacts[r * 6 + q] = wd.craw[wd.sval[r] * 6 + q];
+    }
+  }
+  if (tid == 0) { wd.ctr[0] = s_np; wd.ctr[1] = s_nc; }
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_prepare (wgsl.js:1787-2494)
+// ------------------------------------------------------------------------------------------------
+struct Prep {
+  W w;
+  uint32_t parity;
+  DEV float4 con(uint32_t c, uint32_t q) const { return w.d.contacts[c * 6 + q]; }
+  DEV uint32_t contact_pair(uint32_t c) const { float4 m = con(c, 2); return (uint32_t)(ti(m.x) + 1) | ((uint32_t)(ti(m.y) + 1) << 16); }
+  DEV uint32_t unit_of(uint32_t p, uint32_t nu) const {
+    const uint32_t* US = w.sch(w.S.USTART);
+    uint32_t lo_ = 0, hi = nu;
+    for (uint32_t it = 0; it < 32 && hi - lo_ > 1; it++) { uint32_t mid = (lo_ + hi) / 2; if (US[mid] <= p) lo_ = mid; else hi = mid; }
+    return lo_;
+  }
+  struct BPC { uint32_t k0, k1, k2; f2 pos; float gap; uint32_t pk; };
+  DEV BPC bp_contact(uint32_t c) const {  // wgsl.js:1475-1495
+    const float BINS[20][3] = {
+        {-0.57095543f, 0.75026579f, 0.33333337f}, {-0.64404845f, 0.17218052f, 0.74535593f}, {-0.86918069f, -0.36525859f, 0.33333330f},
+        {-0.93522697f, -0.11932883f, -0.33333339f}, {-0.75091368f, 0.57010308f, -0.33333336f}, {0.36427171f, 0.86959472f, -0.33333337f},
+        {-0.17291149f, 0.64385240f, -0.74535609f}, {0.47113684f, 0.47167205f, 0.74535599f}, {0.11826704f, 0.93536185f, 0.33333335f},
+        {0.17291155f, -0.64385245f, 0.74535603f}, {0.0f, 0.0f, 1.0f}, {-0.11826707f, -0.93536184f, -0.33333335f},
+        {-0.36427169f, -0.86959473f, 0.33333337f}, {0.0f, 0.0f, -1.0f}, {-0.47113677f, -0.47167196f, -0.74535609f},
+        {0.93522695f, 0.11932898f, 0.33333339f}, {0.75091367f, -0.57010306f, 0.33333340f}, {0.57095545f, -0.75026579f, -0.33333334f},
+        {0.64404840f, -0.17218051f, -0.74535597f}, {0.86918063f, 0.36525872f, -0.33333332f}};
+    float4 c0 = con(c, 0), c1 = con(c, 1), cm = con(c, 2), c4 = con(c, 4);
+    int a = ti(cm.x), b = ti(cm.y);
+    uint32_t ga = tu(c4.z), gb = tu(c4.w);
+    f3 ng = gb < ga ? -xyz(c1) : xyz(c1);
+    int rb = a >= 0 ? a : b;
+    f3 org = mk3(0, 0, 0);
+    if (rb >= 0) org = w.pose_c(rb) - qrot(w.pose_q(rb), w.body_com_local(rb));
+    uint32_t bin = 0;
+    float bd = -2.0f;
+    for (uint32_t i = 0; i < 20; i++) { float dd = dot(ng, mk3(BINS[i][0], BINS[i][1], BINS[i][2])); if (dd > bd) { bd = dd; bin = i; } }
+    f3 fn = mk3(BINS[bin][0], BINS[bin][1], BINS[bin][2]);
+    f3 rv = fabsf(fn.y) < 0.9f ? mk3(0, 1, 0) : mk3(1, 0, 0);
+    f3 u = nrm(rv - dot(rv, fn) * fn);
+    f3 v = cross(fn, u);
+    f3 x = xyz(c0) - org;
+    f2 pos = mk2(dot(x, u), dot(x, v));
+    float cell = w.hf(W_F_BP_CELL);
+    uint32_t cx = (uint32_t)((int32_t)clampf_(floorf(pos.x / cell + 0.5f), -31.0f, 31.0f) + 31);
+    uint32_t cy = (uint32_t)((int32_t)clampf_(floorf(pos.y / cell + 0.5f), -31.0f, 31.0f) + 31);
+    uint32_t qx = (uint32_t)((int32_t)clampf_(floorf(pos.x * 2000.0f), -32768.0f, 32767.0f) + 32768);
+    uint32_t qy = (uint32_t)((int32_t)clampf_(floorf(pos.y * 2000.0f), -8192.0f, 8191.0f) + 8192);
+    BPC r;
+    r.k0 = min(ga, gb); r.k1 = max(ga, gb); r.k2 = bin | (cx << 5) | (cy << 11);
+    r.pos = pos; r.gap = c0.w; r.pk = (qx << 15) | (qy << 1) | 1u;
+    return r;
+  }
+  DEV uint2 bp_value(const BPC& g, uint32_t q) const {  // wgsl.js:1498-1503
+    if (q == 6) return make_uint2(fflip_u(fbits(-g.gap)), g.pk);
+    if (q > 6 && g.gap > 0.0f) return make_uint2(0, 0);
+    uint32_t k = q > 6 ? q - 7 : q;
+    float ang = (float)k * 1.0471975511965976f;
+    return make_uint2(fflip_u(fbits(dot(g.pos, mk2(cos_c(ang), sin_c(ang))))), g.pk);
+  }
+  DEV uint32_t first_free_color(int a, int b) const {  // wgsl.js:1233-1246
+    uint32_t NCOLW = w.h(W_NCOLW);
+    uint32_t* M = w.sch(w.S.WMASK);
+    for (uint32_t k = 0; k < NCOLW; k++) {
+      uint32_t used = 0;
+      if (a >= 0) used |= M[(uint32_t)a * NCOLW + k];
+      if (b >= 0) used |= M[(uint32_t)b * NCOLW + k];
+      if (used != 0xffffffffu) {
+        uint32_t bit = (uint32_t)__builtin_ctz(~used);
+        if (a >= 0) M[(uint32_t)a * NCOLW + k] |= 1u << bit;
+        if (b >= 0) M[(uint32_t)b * NCOLW + k] |= 1u << bit;
+        return k * 32 + bit;
+      }
+    }
+    return w.NCOL();
+  }
+  DEV uint32_t anchor_hash(uint32_t pair, uint32_t k) const { return max(hash_u32(pair * 2u + k), 1u); }
+  DEV float4 row(uint32_t r, uint32_t k) const { return w.d.rows[r * w.ROWS() + k]; }
+  DEV void set_row(uint32_t r, uint32_t k, float4 v) const { w.d.rows[r * w.ROWS() + k] = v; }
+  DEVN void build_patch(uint32_t sp) const {  // wgsl.js:1562-1725 (warm start of anchors off: warm scale 0)
+    const uint32_t* US = w.sch(w.S.USTART);
+    uint32_t MAXC = w.MAXC(), PO = w.PO(), AH = w.AH();
+    uint32_t st = US[sp];
+    uint32_t n = min(w.sch(w.S.USIZE)[sp], MAXC - min(st, MAXC));
+    if (n == 0) return;
+    uint32_t pair = w.sch(w.S.UKEY)[sp];
+    int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+    f3 ca = mk3(0, 0, 0), cb = mk3(0, 0, 0);
+    q4 qa = mk4(0, 0, 0, 1);
+    if (a >= 0) { ca = w.pose_c(a); qa = w.pose_q(a); }
+    if (b >= 0) cb = w.pose_c(b);
+    float r = mx(mn(w.body_radius(a), w.body_radius(b)), 1e-4f);
+    uint32_t seed = st;
+    for (uint32_t i = st; i < st + n; i++) if (row(i, 1).w < row(seed, 1).w) seed = i;
+    f3 ns = xyz(row(seed, 0));
+    uint32_t kept = w.sch(w.S.OSTART)[sp] - st;
+    const uint32_t* KEPT = w.sch(w.S.KEPT);
+    float4 ms = con(KEPT[kept + seed], 5);
+    uint32_t first = seed;
+    for (uint32_t i = st; i < st + n; i++) {
+      f3 dseed = xyz(row(i, PO + 2)) - xyz(row(seed, PO + 2));
+      float4 mi = con(KEPT[kept + i], 5);
+      if (dot(xyz(row(i, 0)), ns) < 0.995f || mi.x != ms.x || mi.y != ms.y || fabsf(dot(dseed, ns)) > 0.02f * r || len(dseed) > 2.0f * r) continue;
+      float4 rp = row(i, PO); rp.w = 0.0f; set_row(i, PO, rp);
+      f3 p = xyz(row(i, PO + 2)), pf = xyz(row(first, PO + 2));
+      if (p.x < pf.x || (p.x == pf.x && (p.y < pf.y || (p.y == pf.y && p.z < pf.z)))) first = i;
+    }
+    f3 pf = xyz(row(first, PO + 2));
+    uint32_t second = first;
+    float far = 1e-8f * r * r;
+    for (uint32_t i = st; i < st + n; i++) {
+      if (row(i, PO).w != 0.0f) continue;
+      f3 dd = xyz(row(i, PO + 2)) - pf;
+      if (dot(dd, dd) > far) { far = dot(dd, dd); second = i; }
+    }
+    uint32_t na = second != first ? 2 : 1;
+    f3 loc0 = xyz(row(first, PO + 2)), loc1 = xyz(row(second, PO + 2));
+    if (na == 2) {
+      f3 t0 = cross(ns, mk3(1, 0, 0));
+      if (dot(t0, t0) < 1e-6f) t0 = cross(ns, mk3(0, 1, 0));
+      t0 = nrm(t0);
+      f3 t1 = nrm(cross(ns, t0));
+      float sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0, cnt = 0;
+      f3 cen = mk3(0, 0, 0);
+      for (uint32_t i = st; i < st + n; i++) {
+        if (row(i, PO).w != 0.0f) continue;
+        f3 pc = xyz(row(i, PO + 2));
+        f3 dd = pc - pf;
+        float x = dot(dd, t0), y = dot(dd, t1);
+        sx += x; sy += y; sxx += x * x; sxy += x * y; syy += y * y; cen += pc; cnt += 1.0f;
+      }
+      float xx = sxx - sx * sx / cnt, xy = sxy - sx * sy / cnt, yy = syy - sy * sy / cnt;
+      float spread = sqrtf((xx - yy) * (xx - yy) + 4.0f * xy * xy);
+      if (spread > 1e-6f * (xx + yy)) {
+        float big = 0.5f * (xx + yy + spread);
+        f2 a2 = mk2(xy, big - xx);
+        if (xx >= yy) a2 = mk2(big - yy, xy);
+        a2 = nrm(a2);
+        f3 ax = a2.x * t0 + a2.y * t1;
+        float lo_ = 1e30f, hi = -1e30f;
+        for (uint32_t i = st; i < st + n; i++) {
+          if (row(i, PO).w != 0.0f) continue;
+          float pr = dot(xyz(row(i, PO + 2)) - pf, ax);
+          if (pr < lo_) { lo_ = pr; first = i; }
+          if (pr > hi) { hi = pr; second = i; }
+        }
+        f3 centroid = cen / cnt;
+        loc0 = centroid + dot(xyz(row(first, PO + 2)) - centroid, ax) * ax;
+        loc1 = centroid + dot(xyz(row(second, PO + 2)) - centroid, ax) * ax;
+        f3 across = nrm(cross(ns, ax));
+        float tol = mx(1e-5f * r, 0.05f * (hi - lo_));
+        f3 e0 = mk3(0, 0, 0), e1 = mk3(0, 0, 0);
+        float c0 = 0, c1 = 0;
+        for (uint32_t i = st; i < st + n; i++) {
+          if (row(i, PO).w != 0.0f) continue;
+          f3 pc = xyz(row(i, PO + 2));
+          float pr = dot(pc - pf, ax);
+          if (pr <= lo_ + tol) { e0 += pc; c0 += 1.0f; }
+          if (pr >= hi - tol) { e1 += pc; c1 += 1.0f; }
+        }
+        e0 = e0 / mx(c0, 1.0f); e1 = e1 / mx(c1, 1.0f);
+        loc0 += dot(e0 - centroid, across) * across + dot(e0 - centroid, ns) * ns;
+        loc1 += dot(e1 - centroid, across) * across + dot(e1 - centroid, ns) * ns;
+      }
+    }
+    for (uint32_t i = st; i < st + n; i++) {
+      if (row(i, PO).w != 0.0f) continue;
+      if (i != first && !(na == 2 && i == second)) { float4 l6 = row(i, 6); set_row(i, 6, make_float4(l6.x, 0.0f, 0.0f, l6.w)); }
+    }
+    uint32_t rbase = parity * AH;
+    for (uint32_t k = 0; k < na; k++) {
+      uint32_t i = k == 1 ? second : first;
+      f3 loc = k == 1 ? loc1 : loc0;
+      f3 nn = xyz(row(i, 0)), t1 = xyz(row(i, 1)), t2 = xyz(row(i, 2));
+      f3 fra = loc - ca, frb = loc - cb;
+      f3 Ma[2][2], Mb[2][2];
+      f3 dirs[2] = {t1, t2};
+      for (uint32_t q = 0; q < 2; q++) {
+        w.body_response(a, -dirs[q], -cross(fra, dirs[q]), Ma[q]);
+        w.body_response(b, dirs[q], cross(frb, dirs[q]), Mb[q]);
+        set_row(i, 11 + q * 4, F4(Ma[q][0], 0.0f)); set_row(i, 12 + q * 4, F4(Ma[q][1], 0.0f));
+        set_row(i, 13 + q * 4, F4(Mb[q][0], 0.0f)); set_row(i, 14 + q * 4, F4(Mb[q][1], 0.0f));
+      }
+      float K[4];
+      for (uint32_t x = 0; x < 2; x++)
+        for (uint32_t y = 0; y < 2; y++) {
+          f3 dd = dirs[x];
+          K[x * 2 + y] = dot(-dd, Ma[y][0]) + dot(-cross(fra, dd), Ma[y][1]) + dot(dd, Mb[y][0]) + dot(cross(frb, dd), Mb[y][1]);
+        }
+      float ks = mx(K[0] + K[3], 1e-30f);
+      float k00 = K[0] + 1e-6f * ks, k01 = 0.5f * (K[1] + K[2]), k11 = K[3] + 1e-6f * ks;
+      float det = k00 * k11 - k01 * k01;
+      set_row(i, 5, make_float4(k11 / det, -k01 / det, k00 / det, row(i, 5).w));
+      set_row(i, 2, F4(t2, row(i, 2).w / (float)na));
+      f3 e = mk3(0, 0, 0);
+      float best = 4.0f * r * r;
+      for (uint32_t kk = 0; kk < 2; kk++) {
+        uint32_t hk = anchor_hash(pair, kk);
+        uint32_t h = hk % AH;
+        for (uint32_t probe = 0; probe < 32; probe++) {
+          uint32_t key = w.d.akey[rbase + h];
+          if (key == 0) break;
+          const uint32_t* av = w.d.aval + (size_t)(rbase + h) * AVN;
+          if (key == hk && av[0] == pair && av[1] == kk) {
+            f3 ll = mk3(ubits(av[2]), ubits(av[3]), ubits(av[4]));
+            f3 el = mk3(ubits(av[5]), ubits(av[6]), ubits(av[7]));
+            f3 prev = ca + qrot(qa, ll);
+            f3 dl = prev - loc;
+            if (dot(dl, dl) <= best && fabsf(dot(dl, nn)) <= 0.05f * r) {
+              f3 ew = qrot(qa, el);
+              f3 on = qrot(qa, mk3(ubits(av[11]), ubits(av[12]), ubits(av[13])));
+              float cs = dot(on, nn);
+              if (cs > 0.0f) { f3 ax = cross(on, nn); f3 ce = cross(ax, ew); ew += ce + cross(ax, ce) / (1.0f + cs); }
+              ew -= nn * dot(nn, ew);
+              if (dot(ew, ew) <= 0.01f * r * r) { e = ew; best = dot(dl, dl); }
+            }
+            break;
+          }
+          h = (h + 1) % AH;
+        }
+      }
+      set_row(i, PO, F4(fra, 2.0f));
+      set_row(i, PO + 1, F4(frb, (float)k));
+      set_row(i, PO + 2, F4(e, 0.0f));
+      set_row(i, PO + 3, F4(qinvrot(qa, fra), ubits(pair)));
+    }
+  }
+  DEVN void build_row(uint32_t idx, uint32_t nu) const {  // wgsl.js:2106-2269
+    uint32_t sp = unit_of(idx, nu);
+    uint32_t c = w.sch(w.S.KEPT)[w.sch(w.S.OSTART)[sp] + idx - w.sch(w.S.USTART)[sp]];
+    float4 c0 = con(c, 0), c1 = con(c, 1), cmeta = con(c, 2);
+    int a = ti(cmeta.x), b = ti(cmeta.y);
+    uint32_t key = max(tu(cmeta.z) | (tu(cmeta.w) << 16), 1u);
+    f3 p = xyz(c0);
+    float phi = c0.w;
+    f3 n = xyz(c1);
+    float mu = phi <= w.hf(W_F_FGAP) ? c1.w * w.hf(W_F_FSCALE) : 0.0f;
+    f3 t1 = cross(n, mk3(1, 0, 0));
+    if (dot(t1, t1) < 0.1f) t1 = cross(n, mk3(0, 1, 0));
+    t1 = nrm(t1);
+    f3 t2 = cross(n, t1);
+    f3 ca = mk3(0, 0, 0), cb = mk3(0, 0, 0);
+    if (a >= 0) ca = w.pose_c(a);
+    if (b >= 0) cb = w.pose_c(b);
+    float half = 0.5f * phi;
+    f3 ra = p - n * half - ca, rb = p + n * half - cb;
+    f3 fa = ra, fb = rb;
+    f3 dirs[3] = {n, t1, t2};
+    f3 Ma[3][2], Mb[3][2];
+    for (uint32_t r = 0; r < 3; r++) {
+      f3 dd = dirs[r];
+      f3 aa = r == 0 ? ra : fa, ab = r == 0 ? rb : fb;
+      w.body_response(a, -dd, -cross(aa, dd), Ma[r]);
+      w.body_response(b, dd, cross(ab, dd), Mb[r]);
+      set_row(idx, 7 + r * 4, F4(Ma[r][0], 0.0f)); set_row(idx, 8 + r * 4, F4(Ma[r][1], 0.0f));
+      set_row(idx, 9 + r * 4, F4(Mb[r][0], 0.0f)); set_row(idx, 10 + r * 4, F4(Mb[r][1], 0.0f));
+    }
+    float K[9];
+    for (uint32_t r = 0; r < 3; r++)
+      for (uint32_t s = 0; s < 3; s++) {
+        f3 dd = dirs[r];
+        f3 aa = r == 0 ? ra : fa, ab = r == 0 ? rb : fb;
+        K[r * 3 + s] = dot(-dd, Ma[s][0]) + dot(-cross(aa, dd), Ma[s][1]) + dot(dd, Mb[s][0]) + dot(cross(ab, dd), Mb[s][1]);
+      }
+    uint32_t PO = w.PO();
+    if (w.h(W_EXTRA)) { set_row(idx, PO, F4(fa, 1.0f)); set_row(idx, PO + 1, F4(fb, 0.0f)); set_row(idx, PO + 2, F4(p, 0.0f)); }
+    float cfm = w.hf(W_F_CFM), dt = w.hf(W_DT);
+    float kscale = mx(K[0] + K[4] + K[8], 1e-30f);
+    float effn = 0.0f;
+    if (K[0] > 1e-6f * kscale && K[0] > 1e-12f) effn = 1.0f / (K[0] + cfm);
+    float k00 = K[4] + 1e-6f * kscale + cfm, k01 = 0.5f * (K[5] + K[7]), k11 = K[8] + 1e-6f * kscale + cfm;
+    float det = k00 * k11 - k01 * k01;
+    float rhs;
+    if (phi < 0.0f) rhs = mx(w.hf(W_BETA) * phi / dt, -w.depen_cap(a, b)); else rhs = w.hf(W_F_SPEC) * phi / dt;
+    float e = con(c, 3).x;
+    bool bounce = false;
+    if (e > 0.0f) {
+      f3 ta0[2], tb0[2];
+      pre_twist(a, ta0); pre_twist(b, tb0);
+      float vn0 = dot(n, (tb0[0] + cross(tb0[1], rb)) - (ta0[0] + cross(ta0[1], ra)));
+      bounce = restitution_fires(e, phi, vn0);
+      if (bounce) rhs = e * vn0;
+    }
+    set_row(idx, 0, F4(n, effn));
+    set_row(idx, 1, F4(t1, rhs));
+    set_row(idx, 2, F4(t2, mu));
+    set_row(idx, 3, F4(ra, (float)(key & 0xffffu)));
+    set_row(idx, 4, F4(rb, (float)(key >> 16)));
+    float wrow = row_weight(phi);
+    if (wrow < 1.0f && ((a >= 0 && w.body_kind(a) == 2) || (b >= 0 && w.body_kind(b) == 2))) {
+      f3 ua[2], ub[2];
+      w.body_response_unsplit(a, -n, -cross(ra, n), ua);
+      w.body_response_unsplit(b, n, cross(rb, n), ub);
+      float du = dot(-n, ua[0]) + dot(-cross(ra, n), ua[1]) + dot(n, ub[0]) + dot(cross(rb, n), ub[1]);
+      if (du > 0.0f && K[0] != du) wrow = wrow * K[0] / (wrow * K[0] + (1.0f - wrow) * du);
+    }
+    set_row(idx, 5, make_float4(k11 / det, -k01 / det, k00 / det, bounce ? 1.0f : wrow));
+    set_row(idx, 6, make_float4(0.0f, 0.0f, 0.0f, bounce ? 1e30f : phi));
+  }
+  DEV float row_weight(float phi) const { float reg = w.hf(W_REG); if (phi > 0.0f || reg <= 0.0f) return 1.0f; return 1.0f / (1.0f + reg); }
+  DEV bool restitution_fires(float e, float phi, float vn) const {
+    if (w.h(W_REST) == 0 || e <= 0.0f || vn >= -w.hf(W_F_REST_SPEED)) return false;
+    return phi <= 1e-6f || phi + w.hf(W_DT) * vn <= 1e-6f;
+  }
+  DEV void pre_twist(int b, f3 out[2]) const {
+    out[0] = mk3(0, 0, 0); out[1] = mk3(0, 0, 0);
+    if (b < 0) return;
+    if (w.body_kind(b) == 1) { out[0] = xyz(w.d.vel[(uint32_t)b * 2]); out[1] = xyz(w.d.vel[(uint32_t)b * 2 + 1]); return; }
+    if (w.body_kind(b) == 2) { uint32_t o = w.lo((uint32_t)w.body_link(b)) + L_V; out[0] = w.ld3(o + 3); out[1] = w.ld3(o); }
+  }
+  DEVN void dense_row(uint32_t c, uint32_t slot) const {  // wgsl.js:2370-2444
+    float4 c0 = con(c, 0), c1 = con(c, 1), cmeta = con(c, 2);
+    int a = ti(cmeta.x), b = ti(cmeta.y);
+    uint32_t key = max(tu(cmeta.z) | (tu(cmeta.w) << 16), 1u);
+    f3 p = xyz(c0);
+    float phi = c0.w;
+    f3 n = xyz(c1);
+    float mu = phi <= w.hf(W_F_FGAP) ? c1.w * w.hf(W_F_FSCALE) : 0.0f;
+    f3 t1 = cross(n, mk3(1, 0, 0));
+    if (dot(t1, t1) < 0.1f) t1 = cross(n, mk3(0, 1, 0));
+    t1 = nrm(t1);
+    f3 t2 = cross(n, t1);
+    uint32_t art_id = (uint32_t)w.body_art(a);
+    uint4 ai = w.art_info(art_id);
+    uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR);
+    float J[3 * 64];
+    for (uint32_t k = 0; k < 3 * NDA; k++) J[k] = 0.0f;
+    w.add_gen_jac((uint32_t)w.body_link(b), p, 1.0f, n, t1, t2, ai.z, J);
+    w.add_gen_jac((uint32_t)w.body_link(a), p, -1.0f, n, t1, t2, ai.z, J);
+    float* art_ = w.d.art;
+    uint32_t o = w.h(W_A_DENSE) + slot * w.h(W_DPS);
+    float K[9];
+    SVd ps[64], dvs[64];
+    float uu[6 * 64];
+    for (uint32_t r = 0; r < 3; r++) {
+      float tau[64], y[64];
+      for (uint32_t k = 0; k < NDA; k++) { tau[k] = k < ai.w ? J[r * NDA + k] : 0.0f; y[k] = 0.0f; }
+      w.art_solve_gen(art_id, tau, y, ps, dvs, uu);
+      for (uint32_t k = 0; k < ai.w; k++) { art_[o + DHDR + r * NDA + k] = J[r * NDA + k]; art_[o + DHDR + 3 * NDA + r * NDA + k] = y[k]; }
+      for (uint32_t s2 = 0; s2 < 3; s2++) { float acc = 0.0f; for (uint32_t k = 0; k < ai.w; k++) acc += J[s2 * NDA + k] * y[k]; K[s2 * 3 + r] = acc; }
+    }
+    float cfm = w.hf(W_F_CFM), dt = w.hf(W_DT);
+    float kscale = mx(K[0] + K[4] + K[8], 1e-30f);
+    float effn = 0.0f;
+    if (K[0] > 1e-6f * kscale && K[0] > 1e-12f) effn = 1.0f / (K[0] + cfm);
+    float k00 = K[4] + 1e-6f * kscale + cfm, k01 = 0.5f * (K[5] + K[7]), k11 = K[8] + 1e-6f * kscale + cfm;
+    float det = k00 * k11 - k01 * k01;
+    float rhs;
+    if (phi < 0.0f) rhs = mx(w.hf(W_BETA) * phi / dt, -w.depen_cap(a, b)); else rhs = w.hf(W_F_SPEC) * phi / dt;
+    float e = con(c, 3).x;
+    bool bounce = false;
+    if (e > 0.0f) {
+      float vn0 = 0.0f;
+      for (uint32_t k = 0; k < ai.w; k++) vn0 += J[k] * art_[w.h(W_A_QD) + ai.z + k];
+      bounce = restitution_fires(e, phi, vn0);
+      if (bounce) rhs = e * vn0;
+    }
+    art_[o] = (float)art_id; art_[o + 1] = (float)(key & 0xffffu); art_[o + 2] = (float)(key >> 16); art_[o + 3] = mu;
+    art_[o + 4] = rhs; art_[o + 5] = effn;
+    art_[o + 6] = k11 / det; art_[o + 7] = -k01 / det; art_[o + 8] = k00 / det;
+    art_[o + 9] = 0.0f; art_[o + 10] = 0.0f; art_[o + 11] = 0.0f;
+    w.st3(o + 12, t1); w.st3(o + 15, t2); w.st3(o + 18, n);
+    art_[o + 21] = 0.0f;
+    art_[o + 24] = bounce ? 1.0f : row_weight(phi);
+    art_[o + 22] = bounce ? 1e30f : phi;
+  }
+};
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ uint32_t s_w[BS / 64];
+  uint32_t wid = blockIdx.x;
+  if (wid >= nworlds) return;
+  const WorldDev wd = worlds[wid];
+  load_header(Hs, wd);
+  Prep P;
+  P.w = make_world(Hs, wd);
+  const W& w = P.w;
+  const SchedLayout& S = w.S;
+  const uint32_t tid = threadIdx.x;
+  P.parity = wd.ctr[2] & 1u;
+  const uint32_t NB = w.NB(), NA = w.NA(), NL = w.NL(), ND = w.ND(), MAXC = w.MAXC(), MAXU = w.MAXU(), UH = w.UH(), AH = w.AH();
+  const uint32_t NCOL = w.NCOL(), NCOLW = w.h(W_NCOLW), PATCH = w.h(W_PATCH);
+  uint32_t* sched = wd.sched;
+  float* art_ = wd.art;
+  // ---- Phase 0
+  for (uint32_t i = tid; i < UH; i += BS) { sched[S.HKEY + i] = 0; sched[S.HCNT + i] = 0; }
+  if (PATCH) for (uint32_t i = tid; i < AH; i += BS) wd.akey[(1 - P.parity) * AH + i] = 0;
+  for (uint32_t i = tid; i < max(NB, 1u); i += BS) sched[S.LINKHIT + i] = 0;
+  for (uint32_t i = tid; i < max(NB, 1u) * NCOLW; i += BS) sched[S.WMASK + i] = 0;
+  for (uint32_t i = tid; i < NCOL + 2; i += BS) sched[S.WCOUNT + i] = 0;
+  for (uint32_t i = tid; i < max(NL, 1u); i += BS) sched[S.JSPLIT + i] = 0;
+  if (tid == 0) sched[S.META + 3] = 0;
+  const float dt = w.hf(W_DT);
+  for (uint32_t a = tid; a < NA; a += BS) {
+    uint4 ai = w.art_info(a);
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < ai.w; k++) {
+      uint32_t dd = ai.z + k;
+      float4 lim = w.dof_limit(dd);
+      bool on = false;
+      if (lim.z > 0.5f) {
+        uint32_t l = tu(lim.w);
+        float q = art_[w.h(W_A_Q) + w.link_qs(l) + dd - w.link_dof(l)];
+        float gg = w.hf(W_F_JL_GAP) + w.hf(W_F_JL_SPEED) * dt * fabsf(art_[w.h(W_A_QD) + dd]);
+        on = q <= lim.x + gg || q >= lim.y - gg;
+      }
+      art_[w.h(W_A_LG) + dd] = on ? 1.0f : 0.0f;
+      bool rowf = on || w.dof_passive(dd).y > 0.0f;
+      if (rowf && w.scalar_dof(dd)) { art_[w.h(W_A_RL) + ai.z + n] = (float)dd; n++; }
+    }
+    art_[w.h(W_A_RLN) + a] = (float)n;
+    w.art_fk(a);
+    w.art_factor(a);
+  }
+  __syncthreads();
+  // ---- Phase 1: classes, run heads and dense slots
+  const uint32_t nc = min(wd.ctr[1], MAXC);
+  for (uint32_t c = tid; c < nc; c += BS) {
+    float4 m = P.con(c, 2);
+    int a = ti(m.x), b = ti(m.y);
+    uint32_t pk = P.contact_pair(c);
+    uint32_t cls, head = 0, dense = 0;
+    if (w.same_art(a, b)) { cls = 0xffffffffu; dense = 1; sched[S.CLOCAL + c] = 1; }
+    else { cls = (c == 0 || P.contact_pair(c - 1) != pk) ? 1u : 0u; head = cls; }
+    sched[S.CSLOT + c] = cls;
+    sched[S.HEADS + c] = head;
+    sched[S.DSLOT + c] = dense;
+  }
+  __syncthreads();
+  const uint32_t nruns = block_scan_array<BS>(sched + S.HEADS, nc, s_w);
+  const uint32_t ndc = block_scan_array<BS>(sched + S.DSLOT, nc, s_w);
+  for (uint32_t c = tid; c < nc; c += BS) {
+    uint32_t cls = sched[S.CSLOT + c];
+    if (cls >= 0xfffffffeu) continue;
+    uint32_t unit = sched[S.HEADS + c] + cls;  // inclusive count of heads
+    uint32_t h = unit - 1;
+    if (h >= UH) { sched[S.CSLOT + c] = 0xfffffffeu; continue; }
+    uint32_t pk = P.contact_pair(c);
+    uint32_t run0 = c;
+    while (run0 > 0 && P.contact_pair(run0 - 1) == pk) run0--;
+    if (c == run0) sched[S.HKEY + h] = pk;
+    atomicAdd(&sched[S.HCNT + h], 1u);
+    sched[S.CSLOT + c] = h;
+    sched[S.CLOCAL + c] = c - run0;
+    float4 m = P.con(c, 2);
+    int a = ti(m.x), b = ti(m.y);
+    if (a >= 0 && w.body_kind(a) == 2) sched[S.LINKHIT + (uint32_t)a] = 1;
+    if (b >= 0 && w.body_kind(b) == 2) sched[S.LINKHIT + (uint32_t)b] = 1;
+  }
+  __syncthreads();
+  // ---- Phase 2: units are the run slots in order (unit u = slot h); walk offsets
+  const uint32_t nh = min(nruns, UH);
+  for (uint32_t h = tid; h < nh; h += BS) sched[S.UWOFF + h] = sched[S.HCNT + h];
+  __syncthreads();
+  const uint32_t nwalk = block_scan_array<BS>(sched + S.UWOFF, nh, s_w);
+  const uint32_t nu = min(nh, MAXU);
+  for (uint32_t h = tid; h < nu; h += BS) { sched[S.UKEY + h] = sched[S.HKEY + h]; sched[S.USIZE + h] = sched[S.HCNT + h]; }
+  for (uint32_t l = tid; l < NL; l += BS) {
+    if (sched[S.LINKHIT + (uint32_t)w.link_body(l)] == 0) continue;
+    int j = (int)l;
+    while (j >= 0) { atomicAdd(&sched[S.JSPLIT + (uint32_t)j], 1u); j = w.link_parent((uint32_t)j); }
+  }
+  __syncthreads();
+  for (uint32_t l = tid; l < NL; l += BS) {
+    if (sched[S.LINKHIT + (uint32_t)w.link_body(l)] == 0) continue;
+    w.link_response(l, sched + S.JSPLIT);
+  }
+  for (uint32_t dd = tid; dd < ND; dd += BS) {
+    float4 lim = w.dof_limit(dd);
+    if (lim.z > 0.5f || w.dof_passive(dd).y > 0.0f || w.dof_passive(dd).z >= 0.0f || w.sf(w.h(W_SF_DOF) + dd * 5 + 3).w > 0.0f) {
+      uint32_t l = tu(lim.w);
+      art_[w.h(W_A_HINV) + dd] = w.dof_hinv(l, dd - w.link_dof(l));
+    }
+  }
+  for (uint32_t c = tid; c < nc; c += BS) {
+    uint32_t h = sched[S.CSLOT + c];
+    if (h >= UH) continue;
+    sched[S.WALK + sched[S.UWOFF + h] + sched[S.CLOCAL + c]] = h >= MAXU ? 0xffffffffu : c;
+  }
+  if (tid == 0) { sched[S.META + 0] = nu; sched[S.META + 2] = nc; }
+  for (uint32_t i = tid; i < nu; i += BS) sched[S.USTART + i] = sched[S.USIZE + i];
+  __syncthreads();
+  uint32_t tot = block_scan_array<BS>(sched + S.USTART, nu, s_w);
+  if (tid == 0) sched[S.USTART + nu] = tot;
+  __syncthreads();
+  // ---- Phase 5: listing and body-pair reduction
+  for (uint32_t c = tid; c < nc; c += BS) {
+    uint32_t slot = sched[S.CSLOT + c];
+    if (slot >= 0xfffffffeu || slot >= MAXU) continue;
+    sched[S.CORDER + sched[S.USTART + slot] + sched[S.CLOCAL + c]] = c;
+  }
+  __syncthreads();
+  const uint32_t nl = min(tot, MAXC);
+  for (uint32_t p = tid; p < nl; p += BS) {
+    Prep::BPC g = P.bp_contact(sched[S.CORDER + p]);
+    sched[S.RK0 + p] = g.k0; sched[S.RK1 + p] = g.k1; sched[S.RK2 + p] = g.k2;
+  }
+  for (uint32_t sp = tid; sp < nu; sp += BS) sched[S.OSTART + sp] = sched[S.USTART + sp];
+  __syncthreads();
+  for (uint32_t p = tid; p < nl; p += BS) {
+    uint32_t st = sched[S.USTART + P.unit_of(p, nu)];
+    uint32_t k0 = sched[S.RK0 + p], k1 = sched[S.RK1 + p], k2 = sched[S.RK2 + p];
+    uint32_t lead = p;
+    for (uint32_t m = st; m < p; m++)
+      if (sched[S.RK2 + m] == k2 && sched[S.RK0 + m] == k0 && sched[S.RK1 + m] == k1) { lead = m; break; }
+    sched[S.LEAD + p] = lead;
+    for (uint32_t q = 0; q < 26; q++) sched[S.SLOT + p * 26 + q] = 0;
+  }
+  __syncthreads();
+  for (uint32_t p = tid; p < nl; p += BS) {
+    Prep::BPC g = P.bp_contact(sched[S.CORDER + p]);
+    uint32_t sl = S.SLOT + sched[S.LEAD + p] * 26;
+    for (uint32_t q = 0; q < 13; q++) { uint2 v = P.bp_value(g, q); if (v.y != 0) atomicMax(&sched[sl + q * 2], v.x); }
+  }
+  __syncthreads();
+  for (uint32_t p = tid; p < nl; p += BS) {
+    Prep::BPC g = P.bp_contact(sched[S.CORDER + p]);
+    uint32_t sl = S.SLOT + sched[S.LEAD + p] * 26;
+    for (uint32_t q = 0; q < 13; q++) { uint2 v = P.bp_value(g, q); if (v.y != 0 && v.x == sched[sl + q * 2]) atomicMax(&sched[sl + q * 2 + 1], v.y); }
+  }
+  __syncthreads();
+  for (uint32_t p = tid; p < nl; p += BS) {
+    Prep::BPC g = P.bp_contact(sched[S.CORDER + p]);
+    uint32_t sl = S.SLOT + sched[S.LEAD + p] * 26;
+    bool win = false;
+    for (uint32_t q = 0; q < 13; q++) {
+      uint2 v = P.bp_value(g, q);
+      if (v.y != 0 && v.x == sched[sl + q * 2] && v.y == sched[sl + q * 2 + 1]) win = true;
+    }
+    sched[S.HEADS + p] = win ? 1u : 0u;
+  }
+  __syncthreads();
+  for (uint32_t sp = tid; sp < nu; sp += BS) {
+    uint32_t n = sched[S.USIZE + sp], st = sched[S.USTART + sp], kept = 0;
+    for (uint32_t j = 0; j < n; j++) {
+      if (st + j >= MAXC) break;
+      if (sched[S.HEADS + st + j] != 0) { sched[S.KEPT + st + kept] = sched[S.CORDER + st + j]; kept++; }
+    }
+    sched[S.USIZE + sp] = kept;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < nu; i += BS) sched[S.USTART + i] = sched[S.USIZE + i];
+  __syncthreads();
+  tot = block_scan_array<BS>(sched + S.USTART, nu, s_w);
+  if (tid == 0) sched[S.USTART + nu] = tot;
+  __syncthreads();
+  // ---- Phase 6: rows, friction patches
+  const uint32_t total = min(tot, MAXC);
+  for (uint32_t idx = tid; idx < total; idx += BS) P.build_row(idx, nu);
+  __syncthreads();
+  if (PATCH) {
+    for (uint32_t sp = tid; sp < nu; sp += BS) P.build_patch(sp);
+    __syncthreads();
+  }
+  // ---- Phase 6c: first-fit coloring in walk order (serial by definition, wgsl.js:2282-2363)
+  const uint32_t totp = total;
+  for (uint32_t c = tid; c < nc; c += BS) sched[S.ROWOF + c] = 0xffffffffu;
+  for (uint32_t sp = tid; sp < nu; sp += BS) { sched[S.UTC + sp] = 0; sched[S.UTP + sp] = 0; }
+  __syncthreads();
+  for (uint32_t r = tid; r < totp; r += BS) {
+    uint32_t sp = P.unit_of(r, nu);
+    uint32_t c = sched[S.KEPT + sched[S.OSTART + sp] + r - sched[S.USTART + sp]];
+    sched[S.RUNIT + r] = sp; sched[S.RCON + r] = c;
+    if (c < nc) sched[S.ROWOF + c] = r;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t ncu = min(nwalk, MAXC);
+    uint32_t* WC = sched + S.WCOUNT;
+    for (uint32_t i = 0; i < ncu; i++) {
+      uint32_t wc = sched[S.WALK + i];
+      uint32_t r = wc < MAXC ? sched[S.ROWOF + min(wc, MAXC - 1)] : 0xffffffffu;
+      if (r >= totp) continue;
+      uint32_t key = sched[S.UKEY + sched[S.RUNIT + min(r, MAXC - 1)]];
+      uint32_t col = P.first_free_color((int)(key & 0xffffu) - 1, (int)(key >> 16) - 1);
+      sched[S.PCOL + r] = col;
+      WC[col]++;
+      uint32_t& utp = sched[S.UTP + sched[S.RUNIT + r]];
+      utp = max(utp, ((col + 1) << 16) | wc);
+    }
+    for (uint32_t sp = 0; sp < nu; sp++) {
+      uint32_t packed = sched[S.UTP + sp];
+      sched[S.UTC + sp] = packed >> 16;
+      sched[S.UTP + sp] = packed != 0 ? sched[S.ROWOF + (packed & 0xffffu)] : 0xffffffffu;
+    }
+    uint32_t acc = 0, ncol = 0;
+    for (uint32_t c = 0; c <= NCOL; c++) {
+      uint32_t n = WC[c];
+      sched[S.CSTART + c] = acc; WC[c] = acc; acc += n;
+      if (n > 0) ncol = c + 1;
+    }
+    sched[S.CSTART + NCOL + 1] = acc;
+    sched[S.META + 1] = ncol;
+    for (uint32_t r = 0; r < totp; r++) {
+      uint32_t slot = WC[sched[S.PCOL + r]]++;
+      if (slot < MAXC) sched[S.PORD + slot] = r;
+    }
+    // overflow color by contact index (wgsl.js:1740-1780)
+    uint32_t o0 = min(sched[S.CSTART + NCOL], MAXC), m = min(sched[S.CSTART + NCOL + 1], MAXC) - o0;
+    for (uint32_t i = 1; i < m; i++) {
+      uint32_t r = sched[S.PORD + o0 + i];
+      uint32_t kr = sched[S.RCON + r];
+      int j = (int)i - 1;
+      while (j >= 0) {
+        uint32_t rj = sched[S.PORD + o0 + j];
+        uint32_t kj = sched[S.RCON + rj];
+        if (kj < kr || (kj == kr && rj < r)) break;
+        sched[S.PORD + o0 + j + 1] = rj;
+        j--;
+      }
+      sched[S.PORD + o0 + j + 1] = r;
+    }
+    sched[S.META + 3] = ndc;
+  }
+  // ---- Phase 7: dense rows of same-articulation contacts, in contact order
+  const uint32_t MAXDENSE = w.h(W_MAXDENSE);
+  for (uint32_t c = tid; c < nc; c += BS) {
+    if (sched[S.CSLOT + c] != 0xffffffffu || sched[S.CLOCAL + c] == 0) continue;
+    uint32_t slot = sched[S.DSLOT + c];
+    if (slot >= MAXDENSE) continue;
+    P.dense_row(c, slot);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_solve (wgsl.js:2501-3891)
+// ------------------------------------------------------------------------------------------------
+struct Solver {
+  W w;
+  uint32_t parity;
+  DEV float4 row(uint32_t r, uint32_t k) const { return w.d.rows[r * w.ROWS() + k]; }
+  DEV void load_tw(int b, f3 out[2]) const {
+    if (b < 0) { out[0] = mk3(0, 0, 0); out[1] = mk3(0, 0, 0); return; }
+    out[0] = xyz(w.d.tw[(uint32_t)b * 2]); out[1] = xyz(w.d.tw[(uint32_t)b * 2 + 1]);
+  }
+  DEV void add_imp(int b, f3 f, f3 t) const {
+    if (b < 0) return;
+    if (w.body_kind(b) != 2) return;
+    float4 i0 = w.d.imp[(uint32_t)b * 2], i1 = w.d.imp[(uint32_t)b * 2 + 1];
+    w.d.imp[(uint32_t)b * 2] = make_float4(i0.x + f.x, i0.y + f.y, i0.z + f.z, i0.w + 0.0f);
+    w.d.imp[(uint32_t)b * 2 + 1] = make_float4(i1.x + t.x, i1.y + t.y, i1.z + t.z, i1.w + 0.0f);
+  }
+  static DEV f2 friction_pair_solve(float a0, float c0, float d0, float r0, float r1, float old0, float old1, float radius, float omega) {
+    if (radius <= 0.0f) return mk2(0, 0);
+    float scale = mx(mx(a0, d0), 1e-20f);
+    float a = a0 / scale, c = c0 / scale, d = d0 / scale;
+    float largest = 0.5f * (a + d + sqrtf((a - d) * (a - d) + 4.0f * c * c));
+    float smallest = largest > 0.0f ? mx((a * d - c * c) / largest, 0.0f) : 0.0f;
+    float vx = c, vy = largest - a;
+    if (a >= d) { vx = largest - d; vy = c; }
+    float vn = sqrtf(vx * vx + vy * vy);
+    if (vn > 0.0f) { vx /= vn; vy /= vn; } else { vx = 1.0f; vy = 0.0f; }
+    float r0r = (vx * r0 + vy * r1) / scale;
+    float r1r = (-vy * r0 + vx * r1) / scale;
+    f2 res = mk2(old0, old1);
+    bool sticking = false;
+    if (largest > 0.0f && (smallest > 0.0f || r1r == 0.0f)) {
+      float dx = r0r / largest;
+      float dy = smallest > 0.0f ? r1r / smallest : 0.0f;
+      res = mk2(old0 - (vx * dx - vy * dy), old1 - (vy * dx + vx * dy));
+      sticking = len(res) <= radius;
+    }
+    if (!sticking) {
+      float on = len(mk2(old0, old1)), rn = len(mk2(r0, r1));
+      if (on > 0.0f && on <= radius && radius - on <= 2e-7f * radius && r0 * old0 + r1 * old1 <= 0.0f &&
+          fabsf(r0 * old1 - r1 * old0) <= 2e-7f * on * rn)
+        return mk2(old0, old1);
+      float ox = vx * old0 + vy * old1, oy = -vy * old0 + vx * old1;
+      float b0 = largest * ox - r0r, b1 = smallest * oy - r1r;
+      float x = 0.0f, y = 0.0f;
+      float hi = sqrtf(b0 * b0 + b1 * b1) / radius;
+      if (hi > 0.0f) {
+        float lo_ = mx(mx(hi - largest, fabsf(b1) / radius - smallest), 0.0f);
+        float alpha = lo_;
+        bool done = false;
+        for (uint32_t it = 0; it < 8; it++) {
+          float i0 = 1.0f / (largest + alpha);
+          float i1 = smallest + alpha > 0.0f ? 1.0f / (smallest + alpha) : 0.0f;
+          float tx = ox - (r0r + alpha * ox) * i0, ty = oy - (r1r + alpha * oy) * i1;
+          float nrmv = sqrtf(tx * tx + ty * ty);
+          if (fabsf(nrmv - radius) <= 2e-7f * radius || (alpha == 0.0f && nrmv <= radius)) { x = tx; y = ty; done = true; break; }
+          if (nrmv > radius) lo_ = alpha; else hi = alpha;
+          float slope = tx * tx * i0 + ty * ty * i1;
+          float cand = alpha;
+          if (slope > 0.0f) cand = alpha + (nrmv / radius - 1.0f) * nrmv * nrmv / slope;
+          alpha = (cand > lo_ && cand < hi) ? cand : 0.5f * (lo_ + hi);
+        }
+        if (!done) {
+          for (uint32_t it = 0; it < 24; it++) {
+            alpha = 0.5f * (lo_ + hi);
+            if (len(mk2(b0 / (largest + alpha), b1 / (smallest + alpha))) > radius) lo_ = alpha; else hi = alpha;
+          }
+          x = b0 / (largest + hi); y = b1 / (smallest + hi);
+        }
+      }
+      res = mk2(old0 + vx * (x - ox) - vy * (y - oy), old1 + vy * (x - ox) + vx * (y - oy));
+    }
+    f2 oldv = mk2(old0, old1);
+    f2 relaxed = oldv + omega * (res - oldv);
+    float m = len(relaxed);
+    return m > radius ? relaxed * (radius / m) : relaxed;
+  }
+  DEVN void solve_unit(uint32_t us, uint32_t r, int a, int b, uint32_t mode) const {  // wgsl.js:3033-3231
+    const uint32_t PO = w.PO();
+    const bool EXTRA = w.h(W_EXTRA) != 0, PATCH = w.h(W_PATCH) != 0;
+    float* art_ = w.d.art;
+    f3 ta[2], tb[2];
+    load_tw(a, ta); load_tw(b, tb);
+    float4 r0 = row(r, 0), r1 = row(r, 1), r2 = row(r, 2);
+    f3 ra = xyz(row(r, 3)), rb = xyz(row(r, 4));
+    f3 n = xyz(r0), t1 = xyz(r1), t2 = xyz(r2);
+    float4 lam = row(r, 6);
+    float dn = 0.0f, dt1 = 0.0f, dt2 = 0.0f;
+    float4* R = w.d.rows + (size_t)r * w.ROWS();
+    if (mode == 0) {
+      dn = lam.x; dt1 = lam.y; dt2 = lam.z;
+      for (uint32_t k = 0; k < 3; k++) {
+        float dd = k == 0 ? dn : (k == 1 ? dt1 : dt2);
+        ta[0] += xyz(R[7 + k * 4]) * dd; ta[1] += xyz(R[8 + k * 4]) * dd;
+        tb[0] += xyz(R[9 + k * 4]) * dd; tb[1] += xyz(R[10 + k * 4]) * dd;
+      }
+      f3 fra = ra, frb = rb;
+      if (EXTRA) { fra = xyz(R[PO]); frb = xyz(R[PO + 1]); }
+      f3 fn = n * dn, ft = t1 * dt1 + t2 * dt2;
+      add_imp(a, -fn - ft, -cross(ra, fn) - cross(fra, ft));
+      add_imp(b, fn + ft, cross(rb, fn) + cross(frb, ft));
+    } else {
+      f3 vrel = (tb[0] + cross(tb[1], rb)) - (ta[0] + cross(ta[1], ra));
+      float jv = dot(n, vrel);
+      float wgt = R[5].w;
+      float res = jv + r1.w;
+      float omega = w.hf(W_F_OMEGA);
+      float nl = mx(lam.x + omega * (-res * r0.w * wgt - (1.0f - wgt) * lam.x), 0.0f);
+      dn = nl - lam.x;
+      lam.x = nl;
+      ta[0] += xyz(R[7]) * dn; ta[1] += xyz(R[8]) * dn;
+      tb[0] += xyz(R[9]) * dn; tb[1] += xyz(R[10]) * dn;
+      add_imp(a, -n * dn, -cross(ra, n * dn));
+      add_imp(b, n * dn, cross(rb, n * dn));
+      f3 fra = ra, frb = rb;
+      float lim = r2.w * nl;
+      f2 bias = mk2(0, 0);
+      float role = 1.0f;
+      if (EXTRA) {
+        role = R[PO].w;
+        if (PATCH && role != 1.0f) art_[w.h(W_A_ULOAD) + us] += dn;
+        fra = xyz(R[PO]); frb = xyz(R[PO + 1]);
+        if (PATCH && role == 2.0f) {
+          float load = art_[w.h(W_A_ULOAD) + us];
+          lim = r2.w * load;
+          f3 e = xyz(R[PO + 2]);
+          float beta = w.hf(W_F_ANCHOR_BETA), dt = w.hf(W_DT);
+          bias = mk2(beta * dot(t1, e) / dt, beta * dot(t2, e) / dt);
+          float4 ex = R[PO + 2]; ex.w = lim; R[PO + 2] = ex;
+        }
+      }
+      if (role != 0.0f) {
+        f3 vr2 = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
+        float4 ki = R[5];
+        float v1 = dot(t1, vr2) + bias.x, v2 = dot(t2, vr2) + bias.y;
+        float kd = ki.x * ki.z - ki.y * ki.y;
+        f2 lt = friction_pair_solve(ki.z / kd, -ki.y / kd, ki.x / kd, v1, v2, lam.y, lam.z, lim, omega);
+        dt1 = lt.x - lam.y; dt2 = lt.y - lam.z;
+        lam.y = lt.x; lam.z = lt.y;
+        ta[0] += xyz(R[11]) * dt1 + xyz(R[15]) * dt2;
+        ta[1] += xyz(R[12]) * dt1 + xyz(R[16]) * dt2;
+        tb[0] += xyz(R[13]) * dt1 + xyz(R[17]) * dt2;
+        tb[1] += xyz(R[14]) * dt1 + xyz(R[18]) * dt2;
+        f3 f = t1 * dt1 + t2 * dt2;
+        add_imp(a, -f, -cross(fra, f));
+        add_imp(b, f, cross(frb, f));
+      }
+      R[6] = lam;
+    }
+    if (a >= 0) { w.d.tw[(uint32_t)a * 2] = F4(ta[0], 0.0f); w.d.tw[(uint32_t)a * 2 + 1] = F4(ta[1], 0.0f); }
+    if (b >= 0) { w.d.tw[(uint32_t)b * 2] = F4(tb[0], 0.0f); w.d.tw[(uint32_t)b * 2 + 1] = F4(tb[1], 0.0f); }
+  }
+  DEV void set_link_twist(uint32_t l, SVd v) const {
+    uint32_t b = (uint32_t)w.link_body(l);
+    w.d.tw[b * 2] = F4(v.l, 0.0f); w.d.tw[b * 2 + 1] = F4(v.a, 0.0f);
+  }
+  DEVN void art_twists(uint32_t a) const {
+    uint4 ai = w.art_info(a);
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k;
+      int par = w.link_parent(l);
+      SVd v = sv0();
+      if (par >= 0) v = mshift(w.ldsv(w.lo((uint32_t)par) + L_V), w.link_r(l));
+      for (uint32_t r = 0; r < w.link_ndof(l); r++) v = sadd(v, sscale(w.ldS(l, r), w.d.aqd[w.link_dof(l) + r]));
+      w.stsv(w.lo(l) + L_V, v);
+      set_link_twist(l, v);
+    }
+  }
+  DEVN void art_apply(uint32_t a) const {  // wgsl.js:3254-3299
+    uint4 ai = w.art_info(a);
+    float* art_ = w.d.art;
+    for (uint32_t kk = 0; kk < ai.y; kk++) {
+      uint32_t l = ai.x + ai.y - 1 - kk, L = w.lo(l);
+      uint32_t n = w.link_ndof(l);
+      SVd p = w.ldsv(L + L_P);
+      float u[6];
+      for (uint32_t r = 0; r < n; r++) { u[r] = art_[L + L_TAU + r] - sdot(w.ldS(l, r), p); art_[L + L_UU + r] = u[r]; }
+      int par = w.link_parent(l);
+      if (par >= 0) {
+        SVd pp = p;
+        for (uint32_t r = 0; r < n; r++) {
+          float du = 0.0f;
+          for (uint32_t c = 0; c < n; c++) du += w.ldDI(l, r, c) * u[c];
+          pp = sadd(pp, sscale(w.ldU(l, r), du));
+        }
+        uint32_t P0 = w.lo((uint32_t)par) + L_P;
+        w.stsv(P0, sadd(w.ldsv(P0), fshift(pp, w.link_r(l))));
+      }
+    }
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k, L = w.lo(l);
+      int par = w.link_parent(l);
+      SVd dv = sv0();
+      if (par >= 0) dv = mshift(w.ldsv(w.lo((uint32_t)par) + L_DV), w.link_r(l));
+      uint32_t n = w.link_ndof(l);
+      float dq[6];
+      for (uint32_t r = 0; r < n; r++) {
+        dq[r] = 0.0f;
+        for (uint32_t c = 0; c < n; c++) dq[r] += w.ldDI(l, r, c) * (art_[L + L_UU + c] - sdot(w.ldU(l, c), dv));
+      }
+      for (uint32_t r = 0; r < n; r++) { dv = sadd(dv, sscale(w.ldS(l, r), dq[r])); w.d.aqd[w.link_dof(l) + r] += dq[r]; }
+      w.stsv(L + L_DV, dv);
+      SVd v = sadd(w.ldsv(L + L_V), dv);
+      w.stsv(L + L_V, v);
+      set_link_twist(l, v);
+      w.stsv(L + L_P, sv0());
+      for (uint32_t r = 0; r < w.JM(); r++) art_[L + L_TAU + r] = 0.0f;
+    }
+  }
+  DEVN bool art_dense(uint32_t a, uint32_t mode, uint32_t nd) const {  // wgsl.js:3306-3364, serial lane
+    uint4 ai = w.art_info(a);
+    float* art_ = w.d.art;
+    float* aqd = w.d.aqd;
+    uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR), DPS = w.h(W_DPS), AD = w.h(W_A_DENSE);
+    float omega = w.hf(W_F_OMEGA);
+    bool touched = false;
+    for (uint32_t i = 0; i < nd; i++) {
+      uint32_t o = AD + i * DPS;
+      if (tu(art_[o]) != a) continue;
+      touched = true;
+      uint32_t jo = o + DHDR, yo = o + DHDR + 3 * NDA;
+      f3 lam = mk3(art_[o + 9], art_[o + 10], art_[o + 11]);
+      if (mode == 0) {
+        for (uint32_t k = 0; k < ai.w; k++) aqd[ai.z + k] += art_[yo + k] * lam.x + art_[yo + NDA + k] * lam.y + art_[yo + 2 * NDA + k] * lam.z;
+        continue;
+      }
+      float part = 0.0f;
+      for (uint32_t k = 0; k < ai.w; k++) part += art_[jo + k] * aqd[ai.z + k];
+      float jv = part;
+      float wgt = art_[o + 24];
+      float nl = mx(lam.x + omega * (-(jv + art_[o + 4]) * art_[o + 5] * wgt - (1.0f - wgt) * lam.x), 0.0f);
+      float dn = nl - lam.x;
+      float p1 = 0.0f, p2 = 0.0f;
+      for (uint32_t k = 0; k < ai.w; k++) {
+        aqd[ai.z + k] += art_[yo + k] * dn;
+        p1 += art_[jo + NDA + k] * aqd[ai.z + k];
+        p2 += art_[jo + 2 * NDA + k] * aqd[ai.z + k];
+      }
+      float lim = art_[o + 3] * nl;
+      float kd = art_[o + 6] * art_[o + 8] - art_[o + 7] * art_[o + 7];
+      f2 lt = friction_pair_solve(art_[o + 8] / kd, -art_[o + 7] / kd, art_[o + 6] / kd, p1, p2, lam.y, lam.z, lim, omega);
+      float d1 = lt.x - lam.y, d2 = lt.y - lam.z;
+      for (uint32_t k = 0; k < ai.w; k++) aqd[ai.z + k] += art_[yo + NDA + k] * d1 + art_[yo + 2 * NDA + k] * d2;
+      art_[o + 9] = nl; art_[o + 10] = lt.x; art_[o + 11] = lt.y;
+    }
+    return touched;
+  }
+  DEVN void art_contact_impulses(uint32_t a) const {
+    uint4 ai = w.art_info(a);
+    bool hit = false;
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k, b = (uint32_t)w.link_body(l);
+      f3 f = xyz(w.d.imp[b * 2]), t = xyz(w.d.imp[b * 2 + 1]);
+      if (vne0(f) || vne0(t)) {
+        hit = true;
+        w.stsv(w.lo(l) + L_P, mksv(-t, -f));
+        w.d.imp[b * 2] = make_float4(0, 0, 0, 0); w.d.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
+      }
+    }
+    if (hit) art_apply(a);
+  }
+  DEVN void art_propagate_rows(uint32_t a) const {  // joint-limit rows (wgsl.js:3404-3461)
+    uint4 ai = w.art_info(a);
+    float* art_ = w.d.art;
+    uint32_t n = tu(art_[w.h(W_A_RLN) + a]);
+    float dt = w.hf(W_DT), beta = w.hf(W_BETA), maxd = w.hf(W_MAX_DEPEN), omega = w.hf(W_F_OMEGA);
+    for (uint32_t i = 0; i < n; i++) {
+      uint32_t dd = tu(art_[w.h(W_A_RL) + ai.z + i]);
+      float4 lim = w.dof_limit(dd);
+      uint32_t l = tu(lim.w), r = dd - w.link_dof(l);
+      float hinv = mx(art_[w.h(W_A_HINV) + dd], 1e-12f);
+      if (art_[w.h(W_A_LG) + dd] > 0.5f) {
+        float q = art_[w.h(W_A_Q) + w.link_qs(l) + r];
+        for (uint32_t side = 0; side < 2; side++) {
+          float sg = side == 0 ? 1.0f : -1.0f;
+          float gap = side == 0 ? q - lim.x : lim.y - q;
+          float lam = art_[w.h(W_A_LIM) + 2 * dd + side];
+          float rhs = gap / dt;
+          if (gap < 0.0f) rhs = mx(beta * gap / dt, -maxd);
+          float nl = mx(lam - omega * (sg * w.d.aqd[dd] + rhs) / hinv, 0.0f);
+          if (nl == lam) continue;
+          art_[w.h(W_A_LIM) + 2 * dd + side] = nl;
+          art_[w.lo(l) + L_TAU + r] = sg * (nl - lam);
+          art_apply(a);
+        }
+      }
+    }
+  }
+  DEVN f3 gyroscopic_velocity(const m3& inertia, f3 u0, float h0) const {  // wgsl.js:2543-2562
+    auto inverse3 = [](const m3& m) {
+      f3 a = m.c[0], b = m.c[1], c = m.c[2];
+      f3 r0 = cross(b, c), r1 = cross(c, a), r2 = cross(a, b);
+      float det = dot(a, r0);
+      m3 t; t.c[0] = r0; t.c[1] = r1; t.c[2] = r2;
+      return mscale(tr(t), 1.0f / det);
+    };
+    m3 a = mscale(inertia, 1.0f / mx(inertia.c[0].x, mx(inertia.c[1].y, inertia.c[2].z)));
+    m3 inv = inverse3(a);
+    float speed = sqrtf(mx(dot(u0, mul(a, u0)) * (inv.c[0].x + inv.c[1].y + inv.c[2].z), 0.0f));
+    uint32_t n = (uint32_t)clampf_(ceilf(2.0f * fabsf(h0) * speed), 1.0f, 32.0f);
+    float h = h0 / (float)n;
+    f3 wv = u0;
+    for (uint32_t m = 0; m < n; m++) {
+      f3 u = wv;
+      float energy = dot(u, mul(a, u));
+      for (uint32_t it = 0; it < 8; it++) {
+        m3 Sm = mscale(0.5f * h, skew3(mul(a, 0.5f * (u + wv))));
+        f3 c = mul(inverse3(msub(a, Sm)), mul(madd(a, Sm), u));
+        f3 dd = c - wv;
+        wv = c;
+        if (dot(dd, mul(a, dd)) <= 1e-12f * energy) break;
+      }
+    }
+    return wv;
+  }
+  DEV f3 body_motion(int b, f3 x) const {  // wgsl.js:3472-3489
+    if (b < 0) return mk3(0, 0, 0);
+    uint32_t PP = w.h(W_P_PREV), PB = w.h(W_P_BODY);
+    f3 p0 = xyz(w.d.poses[PP + (uint32_t)b * 2]);
+    q4 q0 = Q(w.d.poses[PP + (uint32_t)b * 2 + 1]);
+    f3 p1 = xyz(w.d.poses[PB + (uint32_t)b * 2]);
+    q4 q1 = Q(w.d.poses[PB + (uint32_t)b * 2 + 1]);
+    q4 r = qmul(q1, qconj(q0));
+    if (r.w < 0.0f) r = mk4(-r.x, -r.y, -r.z, -r.w);
+    float sn = len(xyz(r));
+    f3 ang = 2.0f * xyz(r);
+    f3 trv = p1 - p0;
+    f3 lin = trv;
+    if (sn > 1e-6f) {
+      float half = atan2_c(sn, r.w);
+      f3 axis = xyz(r) / sn;
+      ang = (2.0f * half) * axis;
+      float sc = half * r.w / sn;
+      lin = sc * trv + ((1.0f - sc) * dot(axis, trv)) * axis;
+    }
+    return lin + cross(ang, x - 0.5f * (p0 + p1));
+  }
+};
+
+template <int BS>
+DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint32_t nd) {
+  const W& w = sv.w;
+  const uint32_t tid = threadIdx.x, NA = w.NA(), NCOL = w.NCOL(), PO = w.PO();
+  const uint32_t* sched = w.d.sched;
+  const SchedLayout& S = w.S;
+  for (uint32_t a = tid; a < NA; a += BS) sv.art_propagate_rows(a);
+  if (w.h(W_PATCH) && mode == 1) {
+    for (uint32_t us = tid; us < nu; us += BS) {
+      uint32_t s0 = sched[S.USTART + us];
+      float load = 0.0f;
+      for (uint32_t m = s0; m < s0 + sched[S.USIZE + us]; m++) if (sv.row(m, PO).w != 1.0f) load += sv.row(m, 6).x;
+      w.d.art[w.h(W_A_ULOAD) + us] = load;
+    }
+  }
+  __syncthreads();
+  for (uint32_t a = tid; a < NA; a += BS) if (sv.art_dense(a, 1, nd)) sv.art_twists(a);
+  __syncthreads();
+  const uint4* slots = (const uint4*)(sched + S.SLOTT);
+  for (uint32_t c = 0; c < NCOL && c < ncol; c++) {
+    uint32_t s = sched[S.CSTART + c], e = sched[S.CSTART + c + 1];
+    for (uint32_t u = s + tid; u < e; u += BS) { uint4 sl = slots[u]; sv.solve_unit(sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    uint32_t MAXC = w.MAXC();
+    for (uint32_t u = sched[S.CSTART + NCOL]; u < sched[S.CSTART + NCOL + 1]; u++) {
+      uint4 sl = slots[min(u, MAXC - 1)];
+      sv.solve_unit(sl.y, sl.x, (int)sl.z, (int)sl.w, mode);
+    }
+  }
+  __syncthreads();
+  for (uint32_t a = tid; a < NA; a += BS) sv.art_contact_impulses(a);
+  __syncthreads();
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  uint32_t wid = blockIdx.x;
+  if (wid >= nworlds) return;
+  const WorldDev wd = worlds[wid];
+  load_header(Hs, wd);
+  Solver sv;
+  sv.w = make_world(Hs, wd);
+  const W& w = sv.w;
+  const SchedLayout& S = w.S;
+  const uint32_t tid = threadIdx.x;
+  sv.parity = wd.ctr[2] & 1u;
+  const uint32_t NB = w.NB(), NA = w.NA(), NL = w.NL(), ND = w.ND(), MAXC = w.MAXC(), NCOL = w.NCOL(), AH = w.AH(), PO = w.PO();
+  uint32_t* sched = wd.sched;
+  float* art_ = wd.art;
+  const float dt = w.hf(W_DT);
+  const f3 g = mk3(w.hf(W_GX), w.hf(W_GY), w.hf(W_GZ));
+  const uint32_t nu = sched[S.META], ncol = min(sched[S.META + 1], NCOL);
+  const uint32_t nd = min(sched[S.META + 3], w.h(W_MAXDENSE));
+  for (uint32_t i = tid; i < ND; i += BS) wd.aqd[i] = art_[w.h(W_A_QDH) + i];
+  for (uint32_t b = tid; b < NB; b += BS) {
+    uint32_t kind = w.body_kind((int)b);
+    wd.imp[b * 2] = make_float4(0, 0, 0, 0); wd.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
+    if (kind != 1) continue;
+    uint32_t fl = w.body_flags((int)b);
+    f3 v = xyz(wd.vel[b * 2]) + (fl != 0 ? mk3(0, 0, 0) : g * dt);
+    f3 wv = xyz(wd.vel[b * 2 + 1]);
+    uint32_t bfo = w.h(W_A_BF) + b * 8;
+    if ((fl & 1) == 0 && (vne0(w.ld3(bfo)) || vne0(w.ld3(bfo + 4)))) {
+      m3 R = qmat(w.pose_q((int)b));
+      v += w.ld3(bfo) * w.body_inv_mass((int)b) * dt;
+      wv += mul(R, mul(w.body_inv_inertia((int)b), mul(tr(R), w.ld3(bfo + 4)))) * dt;
+    }
+    m3 I = w.body_inertia((int)b);
+    bool iso = I.c[0].x == I.c[1].y && I.c[1].y == I.c[2].z && I.c[0].y == 0.0f && I.c[0].z == 0.0f && I.c[1].z == 0.0f;
+    if ((fl & 1) == 0 && !iso && vne0(wv)) {
+      m3 R = qmat(w.pose_q((int)b));
+      wv = mul(R, sv.gyroscopic_velocity(I, mul(tr(R), wv), dt));
+    }
+    wd.tw[b * 2] = F4(v, 0.0f); wd.tw[b * 2 + 1] = F4(wv, 0.0f);
+  }
+  for (uint32_t i = tid; i < 2 * ND; i += BS) art_[w.h(W_A_LIM) + i] = 0.0f;
+  for (uint32_t i = tid; i < ND; i += BS) { art_[w.h(W_A_FRIC) + i] = 0.0f; art_[w.h(W_A_DRV) + i] = 0.0f; }
+  const uint32_t nslot = min(sched[S.CSTART + NCOL + 1], MAXC);
+  uint4* slots = (uint4*)(sched + S.SLOTT);
+  for (uint32_t u = tid; u < nslot; u += BS) {
+    uint32_t r = sched[S.PORD + u], us = sched[S.RUNIT + r], key = sched[S.UKEY + us];
+    slots[u] = make_uint4(r, us, (uint32_t)((int)(key & 0xffffu) - 1), (uint32_t)((int)(key >> 16) - 1));
+  }
+  __syncthreads();
+  for (uint32_t a = tid; a < NA; a += BS) sv.art_twists(a);
+  __syncthreads();
+  for (uint32_t a = tid; a < NA; a += BS) if (sv.art_dense(a, 0, nd)) sv.art_twists(a);
+  __syncthreads();
+  sweep<BS>(sv, 0, ncol, nu, nd);
+  const uint32_t iters = w.h(W_ITERS);
+  for (uint32_t it = 0; it < iters; it++) sweep<BS>(sv, 1, ncol, nu, nd);
+  const uint32_t nrows = sched[S.USTART + nu];
+  const bool PATCH = w.h(W_PATCH) != 0;
+  const uint32_t PB = w.h(W_P_BODY), PP = w.h(W_P_PREV);
+  if (PATCH) {
+    for (uint32_t b = tid; b < NB * 2; b += BS) wd.poses[PP + b] = wd.poses[PB + b];
+    __syncthreads();
+  }
+  for (uint32_t b = tid; b < NB; b += BS) {
+    if (w.body_kind((int)b) != 1) continue;
+    f3 v = xyz(wd.tw[b * 2]), wv = xyz(wd.tw[b * 2 + 1]);
+    float damp = (w.body_flags((int)b) & 1) != 0 ? 1.0f : 1.0f - w.body_limits((int)b).x * dt;
+    wd.vel[b * 2] = F4(v, 0.0f); wd.vel[b * 2 + 1] = F4(wv * damp, 0.0f);
+    f3 c = xyz(wd.poses[PB + b * 2]) + v * dt;
+    q4 q = Q(wd.poses[PB + b * 2 + 1]);
+    q = nrm4(q + (0.5f * dt) * qmul(mk4(wv, 0.0f), q));
+    wd.poses[PB + b * 2] = F4(c, 0.0f);
+    wd.poses[PB + b * 2 + 1] = F4(q);
+  }
+  for (uint32_t i = tid; i < ND; i += BS) art_[w.h(W_A_QD) + i] = wd.aqd[i];
+  __syncthreads();
+  for (uint32_t l = tid; l < NL; l += BS) {
+    if (w.link_type(l) != JT_FREE || w.link_parent(l) >= 0) continue;
+    float k = 1.0f - w.body_limits(w.link_body(l)).x * dt;
+    uint32_t dd = w.h(W_A_QD) + w.link_dof(l);
+    art_[dd + 3] *= k; art_[dd + 4] *= k; art_[dd + 5] *= k;
+  }
+  for (uint32_t a = tid; a < NA; a += BS) {
+    uint4 ai = w.art_info(a);
+    uint32_t AQ = w.h(W_A_Q);
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k, qs = w.link_qs(l), dd = w.link_dof(l), t = w.link_type(l);
+      if (t == JT_REVOLUTE || t == JT_PRISMATIC || t == JT_D6)
+        for (uint32_t r = 0; r < w.link_ndof(l); r++) art_[AQ + qs + r] += wd.aqd[dd + r] * dt;
+      if (t == JT_BALL) {
+        f3 wv = mk3(wd.aqd[dd], wd.aqd[dd + 1], wd.aqd[dd + 2]);
+        q4 q = mk4(art_[AQ + qs], art_[AQ + qs + 1], art_[AQ + qs + 2], art_[AQ + qs + 3]);
+        q = nrm4(q + (0.5f * dt) * qmul(mk4(wv, 0.0f), q));
+        art_[AQ + qs] = q.x; art_[AQ + qs + 1] = q.y; art_[AQ + qs + 2] = q.z; art_[AQ + qs + 3] = q.w;
+      }
+      if (t == JT_FREE) {
+        for (uint32_t c = 0; c < 3; c++) art_[AQ + qs + c] += wd.aqd[dd + c] * dt;
+        f3 wv = mk3(wd.aqd[dd + 3], wd.aqd[dd + 4], wd.aqd[dd + 5]);
+        q4 q = mk4(art_[AQ + qs + 3], art_[AQ + qs + 4], art_[AQ + qs + 5], art_[AQ + qs + 6]);
+        q = nrm4(q + (0.5f * dt) * qmul(mk4(wv, 0.0f), q));
+        art_[AQ + qs + 3] = q.x; art_[AQ + qs + 4] = q.y; art_[AQ + qs + 5] = q.z; art_[AQ + qs + 6] = q.w;
+      }
+    }
+    w.art_fk(a);
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k, b = (uint32_t)w.link_body(l);
+      wd.poses[PB + b * 2] = F4(w.ld3(w.lo(l) + L_COM), 0.0f);
+      wd.poses[PB + b * 2 + 1] = F4(w.link_rot(l));
+    }
+  }
+  if (PATCH) {
+    __syncthreads();
+    uint32_t ab = (1 - sv.parity) * AH;
+    for (uint32_t i = tid; i < nrows; i += BS) {
+      if (sv.row(i, PO).w != 2.0f) continue;
+      uint32_t pair = fbits(sv.row(i, PO + 3).w);
+      int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+      f3 fra = xyz(sv.row(i, PO)), frb = xyz(sv.row(i, PO + 1));
+      uint32_t k = tu(sv.row(i, PO + 1).w);
+      float4 ex = sv.row(i, PO + 2);
+      float lim = ex.w;
+      if (lim <= 0.0f) continue;
+      f3 n = xyz(sv.row(i, 0)), t1 = xyz(sv.row(i, 1)), t2 = xyz(sv.row(i, 2));
+      f3 ta[2], tb[2];
+      sv.load_tw(a, ta); sv.load_tw(b, tb);
+      f3 vrel = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
+      float4 lam = sv.row(i, 6);
+      f3 lt = t1 * lam.y + t2 * lam.z;
+      float r = mx(mn(w.body_radius(a), w.body_radius(b)), 1e-4f);
+      if (len(lt) >= (1.0f - 1e-5f) * lim && dot(vrel, lt) < -lim * 1e-5f * r / dt) continue;
+      q4 qa = mk4(0, 0, 0, 1);
+      f3 ca = mk3(0, 0, 0);
+      if (a >= 0) { qa = w.pose_q(a); ca = w.pose_c(a); }
+      f3 ll = xyz(sv.row(i, PO + 3));
+      f3 x = ca + qrot(qa, ll);
+      f3 e = xyz(ex) + sv.body_motion(b, x) - sv.body_motion(a, x);
+      e -= n * dot(n, e);
+      f3 el = qinvrot(qa, e);
+      uint32_t hk = max(hash_u32(pair * 2u + k), 1u);
+      uint32_t h = hk % AH;
+      for (uint32_t probe = 0; probe < 32; probe++) {
+        if (atomicCAS(&wd.akey[ab + h], 0u, hk) == 0u) {
+          uint32_t* av = wd.aval + (size_t)(ab + h) * AVN;
+          f3 il = qinvrot(qa, lt), nl_ = qinvrot(qa, n);
+          av[0] = pair; av[1] = k;
+          av[2] = fbits(ll.x); av[3] = fbits(ll.y); av[4] = fbits(ll.z);
+          av[5] = fbits(el.x); av[6] = fbits(el.y); av[7] = fbits(el.z);
+          av[8] = fbits(il.x); av[9] = fbits(il.y); av[10] = fbits(il.z);
+          av[11] = fbits(nl_.x); av[12] = fbits(nl_.y); av[13] = fbits(nl_.z);
+          break;
+        }
+        h = (h + 1) % AH;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t* st = wd.stats;
+    st[1] = max(st[1], wd.ctr[1]);
+    st[2] = max(st[2], ncol);
+    st[3] = max(st[3], nu);
+    st[4] = max(st[4], wd.ctr[0]);
+    st[5] = wd.ctr[1];
+    st[6] = ncol;
+    st[8] = 0;
+    st[7] = max(st[7], sched[S.META + 3]);
+    wd.ctr[0] = 0; wd.ctr[1] = 0;
+    wd.ctr[2] += 1;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------------------------------
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, hipStream_t stream, uint32_t stages) {
+  if (nworlds == 0) return hipSuccess;
+  dim3 grid(nworlds);
+  if (stages & STAGE_COLLIDE) {
+    hipLaunchKernelGGL((k_collide<256>), grid, dim3(256), 0, stream, worlds, nworlds, (stages & STAGE_NARROW) ? 1u : 0u);
+  }
+  if (stages & STAGE_PREPARE) {
+    if (block == 64) hipLaunchKernelGGL((k_prepare<64>), grid, dim3(64), 0, stream, worlds, nworlds);
+    else if (block == 128) hipLaunchKernelGGL((k_prepare<128>), grid, dim3(128), 0, stream, worlds, nworlds);
+    else hipLaunchKernelGGL((k_prepare<256>), grid, dim3(256), 0, stream, worlds, nworlds);
+  }
+  if (stages & STAGE_SOLVE) {
+    if (block == 64) hipLaunchKernelGGL((k_solve<64>), grid, dim3(64), 0, stream, worlds, nworlds);
+    else if (block == 128) hipLaunchKernelGGL((k_solve<128>), grid, dim3(128), 0, stream, worlds, nworlds);
+    else hipLaunchKernelGGL((k_solve<256>), grid, dim3(256), 0, stream, worlds, nworlds);
+  }
+  return hipGetLastError();
+}

@@ -1,0 +1,508 @@
+// Per-world accessors shared by the step kernels: record readers for the packed world arrays
+// (packages/steel-webgpu/src/wgsl.js:70-105, 185-195; artwgsl.js:29-95) and the articulated-body
+// algorithm (artwgsl.js:98-596), with the reference's operation order.
+#pragma once
+#include "vmath.h"
+#include "steel_dev.h"
+#include "../../include/steel_world.h"
+
+#define DEV __device__ __forceinline__
+#define DEVN __device__ __noinline__
+
+static constexpr uint32_t NLANE = 32;   // the reference's lane stride; tie order of every SAT search
+static constexpr uint32_t GT_PLANE = 1, GT_SPHERE = 3, GT_CAPSULE = 4, GT_CYL = 6, GT_BOX = 7, GT_CONE = 9, GT_CONVEX = 10;
+static constexpr uint32_t JT_PRISMATIC = 0, JT_REVOLUTE = 1, JT_BALL = 2, JT_FREE = 4, JT_D6 = 6;
+static constexpr uint32_t LB = 268, L_S = 0, L_U = 36, L_DI = 72, L_IA = 108, L_C = 135, L_PA = 141, L_V = 147, L_W = 153, L_A = 189,
+                          L_UU = 195, L_P = 201, L_DV = 207, L_POS = 213, L_ROT = 216, L_COM = 220, L_TAU = 223, L_JR = 229, L_W2 = 232;
+static constexpr uint32_t AVN = 14;
+static constexpr uint32_t MAXPATH_DEV = 64;  // deepest articulation root path the kernels take
+static constexpr uint32_t MAXJM = 6;
+static constexpr uint32_t MAX_PATHDOF = 192;  // MAXPATH * JM bound of the link-response scratch
+
+struct SVd { f3 a, l; };
+DEV SVd sv0() { SVd s; s.a = mk3(0, 0, 0); s.l = mk3(0, 0, 0); return s; }
+DEV SVd mksv(f3 a, f3 l) { SVd s; s.a = a; s.l = l; return s; }
+DEV SVd sadd(SVd x, SVd y) { return mksv(x.a + y.a, x.l + y.l); }
+DEV SVd sscale(SVd x, float s) { return mksv(x.a * s, x.l * s); }
+DEV float sdot(SVd m, SVd f) { return dot(m.a, f.a) + dot(m.l, f.l); }
+DEV SVd crm(SVd v, SVd m) { return mksv(cross(v.a, m.a), cross(v.a, m.l) + cross(v.l, m.a)); }
+DEV SVd crf(SVd v, SVd f) { return mksv(cross(v.a, f.a) + cross(v.l, f.l), cross(v.a, f.l)); }
+DEV SVd mshift(SVd m, f3 r) { return mksv(m.a, m.l + cross(m.a, r)); }
+DEV SVd fshift(SVd f, f3 r) { return mksv(f.a + cross(r, f.l), f.l); }
+
+struct W {
+  const uint32_t* H;  // header copy in LDS
+  WorldDev d;
+  SchedLayout S;
+
+  DEV uint32_t h(uint32_t k) const { return H[k]; }
+  DEV float hf(uint32_t k) const { return ubits(H[k]); }
+  DEV uint32_t NB() const { return H[W_NB]; }
+  DEV uint32_t NS() const { return H[W_NS]; }
+  DEV uint32_t NA() const { return H[W_NA]; }
+  DEV uint32_t NL() const { return H[W_NL]; }
+  DEV uint32_t ND() const { return H[W_ND]; }
+  DEV uint32_t NDA() const { return H[W_NDA]; }
+  DEV uint32_t JM() const { return H[W_JM]; }
+  DEV uint32_t MAXC() const { return H[W_MAXC]; }
+  DEV uint32_t MAXP() const { return H[W_MAXP]; }
+  DEV uint32_t MAXU() const { return H[W_MAXU]; }
+  DEV uint32_t UH() const { return H[W_UH]; }
+  DEV uint32_t AH() const { return H[W_AH]; }
+  DEV uint32_t NCOL() const { return H[W_NCOLW] * 32; }
+  DEV uint32_t ROWS() const { return H[W_ROWS]; }
+  DEV uint32_t PO() const { return H[W_PO]; }
+  DEV float4 sf(uint32_t i) const { return d.statF[i]; }
+  DEV uint32_t su(uint32_t i) const { return d.statU[i]; }
+  DEV uint32_t* sch(uint32_t off) const { return d.sched + off; }
+
+  // ---- body records
+  DEV uint32_t bo(int b) const { return H[W_SF_BODY] + (uint32_t)b * 8; }
+  DEV float body_inv_mass(int b) const { return sf(bo(b)).x; }
+  DEV uint32_t body_kind(int b) const { return tu(sf(bo(b)).y); }
+  DEV int body_art(int b) const { return ti(sf(bo(b)).z); }
+  DEV int body_link(int b) const { return ti(sf(bo(b)).w); }
+  DEV f3 body_com_local(int b) const { return xyz(sf(bo(b) + 1)); }
+  DEV bool same_art(int a, int b) const { return a >= 0 && b >= 0 && body_kind(a) == 2 && body_kind(b) == 2 && body_art(a) == body_art(b); }
+  DEV float body_mass(int b) const { return sf(bo(b) + 1).w; }
+  DEV q4 body_limits(int b) const { uint32_t o = bo(b); return mk4(sf(o + 3).w, sf(o + 4).w, sf(o + 5).w, sf(o + 6).w); }
+  DEV float depen_cap(int a, int b) const {
+    float m = 1e20f;
+    if (a >= 0) { float l = body_limits(a).w; if (l > 0.0f) m = mn(m, l); }
+    if (b >= 0) { float l = body_limits(b).w; if (l > 0.0f) m = mn(m, l); }
+    return m;
+  }
+  DEV uint32_t body_flags(int b) const { if (b < 0) return 0; return tu(sf(bo(b) + 7).w); }
+  DEV float body_radius(int b) const { if (b < 0) return 1e30f; return sf(bo(b) + 2).w; }
+  DEV m3 rows_T(uint32_t o) const { m3 m; m.c[0] = xyz(sf(o)); m.c[1] = xyz(sf(o + 1)); m.c[2] = xyz(sf(o + 2)); return tr(m); }
+  DEV m3 body_inertia(int b) const { return rows_T(bo(b) + 2); }
+  DEV m3 body_inv_inertia(int b) const { return rows_T(bo(b) + 5); }
+  // ---- hull records
+  DEV f3 hv(uint32_t h_, uint32_t i) const { return xyz(sf(H[W_SF_VERTS] + su(H[W_SU_HULL] + h_ * 4) + i)); }
+  DEV uint32_t hnv(uint32_t h_) const { return su(H[W_SU_HULL] + h_ * 4 + 1); }
+  DEV uint32_t hfo(uint32_t h_) const { return su(H[W_SU_HULL] + h_ * 4 + 2); }
+  DEV uint32_t hnf(uint32_t h_) const { return su(H[W_SU_HULL] + h_ * 4 + 3); }
+  DEV q4 fplane(uint32_t f) const { return Q(sf(H[W_SF_FPLANE] + f)); }
+  DEV uint32_t fstart(uint32_t f) const { return su(H[W_SU_FACE] + f * 2); }
+  DEV uint32_t fcount(uint32_t f) const { return su(H[W_SU_FACE] + f * 2 + 1); }
+  DEV uint32_t fvert(uint32_t k) const { return su(H[W_SU_FV] + k); }
+  DEV uint32_t hne(uint32_t h_) const { return su(H[W_SU_HEDGE] + h_ * 2 + 1); }
+  DEV uint4 hedge(uint32_t h_, uint32_t k) const {
+    uint32_t o = H[W_SU_EDGE] + (su(H[W_SU_HEDGE] + h_ * 2) + k) * 4;
+    return make_uint4(su(o), su(o + 1), su(o + 2), su(o + 3));
+  }
+  DEV uint32_t so(uint32_t s) const { return H[W_SF_SHAPE] + s * 7; }
+  DEV q4 shape_prim(uint32_t s) const { return Q(sf(so(s) + 3)); }
+  DEV uint32_t geo_type(uint32_t s) const { return tu(sf(so(s) + 6).x); }
+  // ---- articulation records
+  DEV uint32_t lk(uint32_t l) const { return H[W_SF_LINK] + l * 6; }
+  DEV uint32_t link_type(uint32_t l) const { return tu(sf(lk(l)).x); }
+  DEV int link_parent(uint32_t l) const { return ti(sf(lk(l)).y); }
+  DEV uint32_t link_dof(uint32_t l) const { return tu(sf(lk(l)).z); }
+  DEV int link_body(uint32_t l) const { return ti(sf(lk(l)).w); }
+  DEV uint32_t link_ndof(uint32_t l) const { return tu(sf(lk(l) + 1).w); }
+  DEV uint32_t link_qs(uint32_t l) const { return tu(sf(lk(l) + 3).w); }
+  DEV float4 dof_drive(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5); }
+  DEV float4 dof_limit(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 1); }
+  DEV float4 dof_passive(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 2); }
+  DEV bool scalar_dof(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 2).w > 0.5f; }
+  DEV uint4 art_info(uint32_t a) const { uint32_t o = H[W_SU_ART] + a * 4; return make_uint4(su(o), su(o + 1), su(o + 2), su(o + 3)); }
+  DEV uint32_t lo(uint32_t l) const { return H[W_A_LINK] + l * LB; }
+  DEV float* art() const { return d.art; }
+  DEV f3 ld3(uint32_t o) const { return mk3(d.art[o], d.art[o + 1], d.art[o + 2]); }
+  DEV void st3(uint32_t o, f3 v) const { d.art[o] = v.x; d.art[o + 1] = v.y; d.art[o + 2] = v.z; }
+  DEV SVd ldsv(uint32_t o) const { return mksv(ld3(o), ld3(o + 3)); }
+  DEV void stsv(uint32_t o, SVd s) const { st3(o, s.a); st3(o + 3, s.l); }
+  DEV m3 ldm(uint32_t o) const { m3 m; m.c[0] = ld3(o); m.c[1] = ld3(o + 3); m.c[2] = ld3(o + 6); return m; }
+  DEV void stm(uint32_t o, const m3& m) const { st3(o, m.c[0]); st3(o + 3, m.c[1]); st3(o + 6, m.c[2]); }
+  DEV SVd imul(uint32_t o, SVd m) const {
+    m3 A = ldm(o), B = ldm(o + 9), C = ldm(o + 18);
+    return mksv(mul(A, m.a) + mul(B, m.l), mul(tr(B), m.a) + mul(C, m.l));
+  }
+  DEV SVd ldS(uint32_t l, uint32_t k) const { return ldsv(lo(l) + L_S + k * 6); }
+  DEV SVd ldU(uint32_t l, uint32_t k) const { return ldsv(lo(l) + L_U + k * 6); }
+  DEV float ldDI(uint32_t l, uint32_t r, uint32_t c) const { return d.art[lo(l) + L_DI + r * JM() + c]; }
+  DEV f3 link_r(uint32_t l) const {
+    int par = link_parent(l);
+    if (par < 0) return mk3(0, 0, 0);
+    return ld3(lo(l) + L_COM) - ld3(lo((uint32_t)par) + L_COM);
+  }
+  DEV q4 link_rot(uint32_t l) const { f3 r = ld3(lo(l) + L_ROT); return mk4(r, d.art[lo(l) + L_ROT + 3]); }
+  DEV q4 pose_q(int b) const { return Q(d.poses[H[W_P_BODY] + (uint32_t)b * 2 + 1]); }
+  DEV f3 pose_c(int b) const { return xyz(d.poses[H[W_P_BODY] + (uint32_t)b * 2]); }
+
+  // ---- Featherstone (artwgsl.js), one thread per articulation
+  DEVN void art_fk(uint32_t a) const {
+    uint4 ai = art_info(a);
+    uint32_t JMv = JM();
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k, o = lk(l);
+      f3 pp = mk3(0, 0, 0);
+      q4 pq = mk4(0, 0, 0, 1);
+      int par = link_parent(l);
+      if (par >= 0) { pp = ld3(lo((uint32_t)par) + L_POS); pq = link_rot((uint32_t)par); }
+      f3 jp = pp + qrot(pq, xyz(sf(o + 1)));
+      q4 jq = qmul(pq, Q(sf(o + 2)));
+      f3 axis = xyz(sf(o + 5));
+      q4 mq = mk4(0, 0, 0, 1);
+      f3 mp = mk3(0, 0, 0);
+      uint32_t t = link_type(l), qs = link_qs(l);
+      uint32_t AQ = H[W_A_Q];
+      float* art_ = d.art;
+      if (t == JT_REVOLUTE) mq = qaxis(axis, art_[AQ + qs]);
+      if (t == JT_PRISMATIC) mp = axis * art_[AQ + qs];
+      if (t == JT_BALL) mq = nrm4(mk4(art_[AQ + qs], art_[AQ + qs + 1], art_[AQ + qs + 2], art_[AQ + qs + 3]));
+      if (t == JT_FREE) {
+        mp = mk3(art_[AQ + qs], art_[AQ + qs + 1], art_[AQ + qs + 2]);
+        mq = nrm4(mk4(art_[AQ + qs + 3], art_[AQ + qs + 4], art_[AQ + qs + 5], art_[AQ + qs + 6]));
+      }
+      q4 cq = qmul(jq, mq);
+      f3 cp = jp + qrot(jq, mp);
+      q4 iq = qconj(Q(sf(o + 4)));
+      q4 lq = nrm4(qmul(cq, iq));
+      f3 lp = cp + qrot(cq, -qrot(iq, xyz(sf(o + 3))));
+      uint32_t L = lo(l);
+      st3(L + L_POS, lp); st3(L + L_ROT, xyz(lq)); art_[L + L_ROT + 3] = lq.w;
+      f3 com = lp + qrot(lq, body_com_local(link_body(l)));
+      st3(L + L_COM, com);
+      f3 aw = qrot(jq, axis);
+      f3 rj = com - jp;
+      for (uint32_t c = 0; c < JMv; c++) stsv(L + L_S + c * 6, sv0());
+      if (t == JT_REVOLUTE) stsv(L + L_S, mksv(aw, cross(aw, rj)));
+      if (t == JT_PRISMATIC) stsv(L + L_S, mksv(mk3(0, 0, 0), aw));
+      if (t == JT_BALL)
+        for (uint32_t c = 0; c < 3; c++) {
+          f3 e = mk3(0, 0, 0); setc(e, c, 1.0f);
+          f3 ew = qrot(jq, e);
+          stsv(L + L_S + c * 6, mksv(ew, cross(ew, rj)));
+        }
+      f3 rf = com - cp;
+      st3(L + L_JR, rf);
+      if (t == JT_FREE)
+        for (uint32_t c = 0; c < 3; c++) {
+          f3 e = mk3(0, 0, 0); setc(e, c, 1.0f);
+          f3 ew = qrot(jq, e);
+          stsv(L + L_S + c * 6, mksv(mk3(0, 0, 0), ew));
+          stsv(L + L_S + (c + 3) * 6, mksv(ew, cross(ew, rf)));
+        }
+    }
+  }
+  DEV void inv_spd(float* D, uint32_t n, float* Di) const {
+    uint32_t JMv = JM();
+    for (uint32_t i = 0; i < JMv * JMv; i++) Di[i] = 0.0f;
+    for (uint32_t i = 0; i < n; i++) Di[i * JMv + i] = 1.0f;
+    for (uint32_t c = 0; c < n; c++) {
+      float pv = 1.0f / mx(D[c * JMv + c], 1e-12f);
+      for (uint32_t k = 0; k < n; k++) { D[c * JMv + k] *= pv; Di[c * JMv + k] *= pv; }
+      for (uint32_t r = 0; r < n; r++) {
+        if (r == c) continue;
+        float f = D[r * JMv + c];
+        if (f == 0.0f) continue;
+        for (uint32_t k = 0; k < n; k++) { D[r * JMv + k] -= f * D[c * JMv + k]; Di[r * JMv + k] -= f * Di[c * JMv + k]; }
+      }
+    }
+  }
+  DEV void add_shifted_inertia(uint32_t o, const m3& A, const m3& B, const m3& C, f3 r) const {
+    m3 R = skew3(r);
+    stm(o, msub(madd(msub(madd(ldm(o), A), mul(B, R)), mul(R, tr(B))), mul(mul(R, C), R)));
+    stm(o + 9, madd(madd(ldm(o + 9), B), mul(R, C)));
+    stm(o + 18, madd(ldm(o + 18), C));
+  }
+  DEVN void art_factor(uint32_t a) const {
+    uint4 ai = art_info(a);
+    float* art_ = d.art;
+    uint32_t JMv = JM(), AQD = H[W_A_QD], AJF = H[W_A_JF], ABF = H[W_A_BF], AQDH = H[W_A_QDH];
+    float dt = hf(W_DT);
+    f3 g = mk3(hf(W_GX), hf(W_GY), hf(W_GZ));
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k, L = lo(l);
+      int par = link_parent(l);
+      SVd vp = sv0();
+      if (par >= 0) vp = mshift(ldsv(lo((uint32_t)par) + L_V), link_r(l));
+      SVd sq = sv0();
+      for (uint32_t c = 0; c < link_ndof(l); c++) sq = sadd(sq, sscale(ldS(l, c), art_[AQD + link_dof(l) + c]));
+      SVd v = sadd(vp, sq);
+      stsv(L + L_V, v);
+      SVd cb = crm(vp, sq);
+      if (link_type(l) == JT_FREE) {
+        f3 vo = mk3(0, 0, 0), wj = mk3(0, 0, 0);
+        for (uint32_t c = 0; c < link_ndof(l); c++) {
+          SVd sc = ldS(l, c);
+          float q = art_[AQD + link_dof(l) + c];
+          if (c < 3) vo += sc.l * q; else wj += sc.a * q;
+        }
+        cb = sadd(cb, mksv(mk3(0, 0, 0), -cross(wj, vo)));
+      }
+      stsv(L + L_C, cb);
+      int b = link_body(l);
+      float m = body_mass(b);
+      m3 R = qmat(link_rot(l));
+      m3 Ic = mul(mul(R, body_inertia(b)), tr(R));
+      stm(L + L_IA, Ic);
+      stm(L + L_IA + 9, m3zero());
+      m3 Mm; Mm.c[0] = mk3(m, 0, 0); Mm.c[1] = mk3(0, m, 0); Mm.c[2] = mk3(0, 0, m);
+      stm(L + L_IA + 18, Mm);
+      SVd pa0 = crf(v, imul(L + L_IA, v));
+      if ((tu(sf(bo(b) + 7).w) & 2u) != 0) pa0 = sadd(pa0, mksv(mk3(0, 0, 0), m * g));
+      pa0 = sadd(pa0, mksv(-ld3(ABF + (uint32_t)b * 8 + 4), -ld3(ABF + (uint32_t)b * 8)));
+      stsv(L + L_PA, pa0);
+    }
+    for (uint32_t kk = 0; kk < ai.y; kk++) {
+      uint32_t l = ai.x + ai.y - 1 - kk, L = lo(l);
+      uint32_t n = link_ndof(l);
+      SVd pA = ldsv(L + L_PA);
+      float D[36], u[6] = {0, 0, 0, 0, 0, 0};
+      for (uint32_t i = 0; i < JMv * JMv; i++) D[i] = 0.0f;
+      for (uint32_t c = 0; c < n; c++) stsv(L + L_U + c * 6, imul(L + L_IA, ldS(l, c)));
+      for (uint32_t r = 0; r < n; r++) {
+        uint32_t dd = link_dof(l) + r;
+        for (uint32_t c = 0; c < n; c++) D[r * JMv + c] = sdot(ldS(l, r), ldU(l, c));
+        float4 dr = dof_drive(dd);
+        D[r * JMv + r] += dr.x;
+        float damp = dof_passive(dd).x;
+        D[r * JMv + r] += dt * damp;
+        float tau = -damp * art_[AQD + dd] + art_[AJF + dd];
+        u[r] = tau - sdot(ldS(l, r), pA);
+      }
+      float Di[36];
+      inv_spd(D, n, Di);
+      for (uint32_t i = 0; i < JMv * JMv; i++) art_[L + L_DI + i] = Di[i];
+      for (uint32_t r = 0; r < JMv; r++) art_[L + L_UU + r] = u[r];
+      int par = link_parent(l);
+      if (par < 0) continue;
+      m3 A = ldm(L + L_IA), B = ldm(L + L_IA + 9), C = ldm(L + L_IA + 18);
+      SVd Duu = sv0();
+      for (uint32_t r = 0; r < n; r++) {
+        SVd Ur = ldU(l, r);
+        float du = 0.0f;
+        for (uint32_t c = 0; c < n; c++) {
+          SVd Uc = ldU(l, c);
+          float w = ldDI(l, r, c);
+          A = msub(A, mscale(w, outer3(Ur.a, Uc.a)));
+          B = msub(B, mscale(w, outer3(Ur.a, Uc.l)));
+          C = msub(C, mscale(w, outer3(Ur.l, Uc.l)));
+          du += w * u[c];
+        }
+        Duu = sadd(Duu, sscale(Ur, du));
+      }
+      stm(L + L_IA, A); stm(L + L_IA + 9, B); stm(L + L_IA + 18, C);
+      SVd cbias = ldsv(L + L_C);
+      SVd pa = sadd(sadd(pA, imul(L + L_IA, cbias)), Duu);
+      uint32_t P0 = lo((uint32_t)par);
+      f3 r = link_r(l);
+      add_shifted_inertia(P0 + L_IA, A, B, C, r);
+      stsv(P0 + L_PA, sadd(ldsv(P0 + L_PA), fshift(pa, r)));
+    }
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k, L = lo(l);
+      int par = link_parent(l);
+      SVd ap = mksv(mk3(0, 0, 0), -g);
+      if (par >= 0) ap = mshift(ldsv(lo((uint32_t)par) + L_A), link_r(l));
+      SVd a1 = sadd(ap, ldsv(L + L_C));
+      uint32_t n = link_ndof(l);
+      SVd acc = a1;
+      for (uint32_t r = 0; r < n; r++) {
+        float qdd = 0.0f;
+        for (uint32_t c = 0; c < n; c++) qdd += ldDI(l, r, c) * (art_[L + L_UU + c] - sdot(ldU(l, c), a1));
+        uint32_t dd = link_dof(l) + r;
+        art_[AQDH + dd] = art_[AQD + dd] + dt * qdd;
+        acc = sadd(acc, sscale(ldS(l, r), qdd));
+      }
+      stsv(L + L_A, acc);
+    }
+  }
+  DEV uint32_t link_path(uint32_t l0, uint32_t* path) const {
+    uint32_t n = 0, MP = H[W_MAXPATH];
+    int l = (int)l0;
+    while (!(l < 0 || n >= MP)) { path[n++] = (uint32_t)l; l = link_parent((uint32_t)l); }
+    return n;
+  }
+  DEVN float dof_hinv(uint32_t tl, uint32_t td) const {  // artwgsl.js:513-518 via path_test 375-414
+    uint32_t path[MAXPATH_DEV];
+    uint32_t np = link_path(tl, path);
+    uint32_t JMv = JM();
+    float ub[MAXPATH_DEV * MAXJM];
+    SVd p = sscale(sv0(), -1.0f);
+    for (uint32_t i = 0; i < np; i++) {
+      uint32_t l = path[i], n = link_ndof(l);
+      float u[6] = {0, 0, 0, 0, 0, 0};
+      for (uint32_t r = 0; r < n; r++) {
+        float t = 0.0f;
+        if (l == tl && r == td) t = 1.0f;
+        u[r] = t - sdot(ldS(l, r), p);
+        ub[i * JMv + r] = u[r];
+      }
+      for (uint32_t r = 0; r < n; r++) {
+        float du = 0.0f;
+        for (uint32_t c = 0; c < n; c++) du += ldDI(l, r, c) * u[c];
+        p = sadd(p, sscale(ldU(l, r), du));
+      }
+      p = fshift(p, link_r(l));
+    }
+    SVd dv = sv0();
+    float out = 0.0f;
+    for (uint32_t ii = 0; ii < np; ii++) {
+      uint32_t i = np - 1 - ii, l = path[i], n = link_ndof(l);
+      dv = mshift(dv, link_r(l));
+      float dq[6];
+      for (uint32_t r = 0; r < n; r++) {
+        dq[r] = 0.0f;
+        for (uint32_t c = 0; c < n; c++) dq[r] += ldDI(l, r, c) * (ub[i * JMv + c] - sdot(ldU(l, c), dv));
+      }
+      for (uint32_t r = 0; r < n; r++) {
+        dv = sadd(dv, sscale(ldS(l, r), dq[r]));
+        if (l == tl && r == td) out = dq[r];
+      }
+    }
+    return out;
+  }
+  DEVN void link_response(uint32_t l, const uint32_t* jsplit) const {  // artwgsl.js:431-510
+    uint32_t path[MAXPATH_DEV];
+    uint32_t np = link_path(l, path);
+    uint32_t JMv = JM();
+    const uint32_t PD = np * JMv;  // compact stride (indexes only path dofs)
+    float ub[6 * MAX_PATHDOF];
+    SVd p[6];
+    for (uint32_t e = 0; e < 6; e++) {
+      f3 f = mk3(0, 0, 0), t = mk3(0, 0, 0);
+      if (e < 3) setc(f, e, 1.0f); else setc(t, e - 3, 1.0f);
+      p[e] = mksv(-t, -f);
+    }
+    for (uint32_t i = 0; i < np; i++) {
+      uint32_t pl = path[i], n = link_ndof(pl);
+      f3 r = link_r(pl);
+      for (uint32_t e = 0; e < 6; e++) {
+        float u[6];
+        for (uint32_t c = 0; c < n; c++) { u[c] = 0.0f - sdot(ldS(pl, c), p[e]); ub[e * PD + i * JMv + c] = u[c]; }
+        SVd pe = p[e];
+        for (uint32_t rr = 0; rr < n; rr++) {
+          float du = 0.0f;
+          for (uint32_t c = 0; c < n; c++) du += ldDI(pl, rr, c) * u[c];
+          pe = sadd(pe, sscale(ldU(pl, rr), du));
+        }
+        p[e] = fshift(pe, r);
+      }
+    }
+    SVd dv[6];
+    for (uint32_t e = 0; e < 6; e++) dv[e] = sv0();
+    for (uint32_t ii = 0; ii < np; ii++) {
+      uint32_t i = np - 1 - ii, pl = path[i], n = link_ndof(pl);
+      f3 r = link_r(pl);
+      for (uint32_t e = 0; e < 6; e++) {
+        SVd dd = mshift(dv[e], r);
+        float dq[6];
+        for (uint32_t rr = 0; rr < n; rr++) {
+          dq[rr] = 0.0f;
+          for (uint32_t c = 0; c < n; c++) dq[rr] += ldDI(pl, rr, c) * (ub[e * PD + i * JMv + c] - sdot(ldU(pl, c), dd));
+        }
+        for (uint32_t rr = 0; rr < n; rr++) dd = sadd(dd, sscale(ldS(pl, rr), dq[rr]));
+        dv[e] = dd;
+      }
+    }
+    float Wm[36];
+    for (uint32_t e = 0; e < 6; e++) {
+      Wm[0 * 6 + e] = dv[e].l.x; Wm[1 * 6 + e] = dv[e].l.y; Wm[2 * 6 + e] = dv[e].l.z;
+      Wm[3 * 6 + e] = dv[e].a.x; Wm[4 * 6 + e] = dv[e].a.y; Wm[5 * 6 + e] = dv[e].a.z;
+    }
+    float* art_ = d.art;
+    for (uint32_t k = 0; k < 36; k++) art_[lo(l) + L_W2 + k] = Wm[k];
+    for (uint32_t i = 0; i < np; i++) {
+      float extra = (float)jsplit[path[i]] - 1.0f;
+      if (extra <= 0.0f) continue;
+      uint32_t pl = path[i], n = link_ndof(pl);
+      float t[36];
+      for (uint32_t cc = 0; cc < 6; cc++)
+        for (uint32_t x = 0; x < n; x++) {
+          float v = 0.0f;
+          for (uint32_t y = 0; y < n; y++) v += ldDI(pl, x, y) * ub[cc * PD + i * JMv + y];
+          t[cc * 6 + x] = v;
+        }
+      for (uint32_t rr = 0; rr < 6; rr++)
+        for (uint32_t cc = 0; cc < 6; cc++) {
+          float sv = 0.0f;
+          for (uint32_t x = 0; x < n; x++) sv += ub[rr * PD + i * JMv + x] * t[cc * 6 + x];
+          Wm[rr * 6 + cc] += extra * sv;
+        }
+    }
+    for (uint32_t k = 0; k < 36; k++) art_[lo(l) + L_W + k] = Wm[k];
+  }
+  DEV void add_gen_jac(uint32_t x, f3 p, float sgn, f3 d0, f3 d1, f3 d2, uint32_t ds, float* J) const {
+    int l = (int)x;
+    uint32_t NDAv = NDA();
+    while (l >= 0) {
+      uint32_t k = (uint32_t)l;
+      f3 c = ld3(lo(k) + L_COM);
+      for (uint32_t r = 0; r < link_ndof(k); r++) {
+        SVd Sv = ldS(k, r);
+        f3 v = Sv.l + cross(Sv.a, p - c);
+        uint32_t i = link_dof(k) + r - ds;
+        J[i] += sgn * dot(d0, v); J[NDAv + i] += sgn * dot(d1, v); J[2 * NDAv + i] += sgn * dot(d2, v);
+      }
+      l = link_parent(k);
+    }
+  }
+  // qd change of articulation a for generalized impulse tau (artwgsl.js:557-596); scratch: p, dvs in SVd[MAXLP], uu float[6*MAXLP]
+  DEVN void art_solve_gen(uint32_t a, const float* tau, float* out, SVd* p, SVd* dvs, float* uu) const {
+    uint4 ai = art_info(a);
+    for (uint32_t k = 0; k < ai.y; k++) p[k] = sv0();
+    for (uint32_t kk = 0; kk < ai.y; kk++) {
+      uint32_t k = ai.y - 1 - kk, l = ai.x + k, n = link_ndof(l);
+      float u[6] = {0, 0, 0, 0, 0, 0};
+      for (uint32_t r = 0; r < n; r++) u[r] = tau[link_dof(l) + r - ai.z] - sdot(ldS(l, r), p[k]);
+      for (uint32_t r = 0; r < 6; r++) uu[k * 6 + r] = u[r];
+      int par = link_parent(l);
+      if (par >= 0) {
+        SVd pp = p[k];
+        for (uint32_t r = 0; r < n; r++) {
+          float du = 0.0f;
+          for (uint32_t c = 0; c < n; c++) du += ldDI(l, r, c) * u[c];
+          pp = sadd(pp, sscale(ldU(l, r), du));
+        }
+        uint32_t pk = (uint32_t)par - ai.x;
+        p[pk] = sadd(p[pk], fshift(pp, link_r(l)));
+      }
+    }
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k;
+      int par = link_parent(l);
+      SVd dv = sv0();
+      if (par >= 0) dv = mshift(dvs[(uint32_t)par - ai.x], link_r(l));
+      uint32_t n = link_ndof(l);
+      float dq[6];
+      for (uint32_t r = 0; r < n; r++) {
+        dq[r] = 0.0f;
+        for (uint32_t c = 0; c < n; c++) dq[r] += ldDI(l, r, c) * (uu[k * 6 + c] - sdot(ldU(l, c), dv));
+      }
+      for (uint32_t r = 0; r < n; r++) { dv = sadd(dv, sscale(ldS(l, r), dq[r])); out[link_dof(l) + r - ai.z] = dq[r]; }
+      dvs[k] = dv;
+    }
+  }
+  DEV void body_response(int b, f3 f, f3 t, f3 out[2]) const {  // wgsl.js:1312-1328
+    if (b < 0) { out[0] = mk3(0, 0, 0); out[1] = mk3(0, 0, 0); return; }
+    if (body_kind(b) == 1) {
+      m3 R = qmat(pose_q(b));
+      out[0] = f * body_inv_mass(b);
+      out[1] = mul(R, mul(body_inv_inertia(b), mul(tr(R), t)));
+      return;
+    }
+    uint32_t wo = lo((uint32_t)body_link(b)) + L_W;
+    float x[6] = {f.x, f.y, f.z, t.x, t.y, t.z}, o[6];
+    for (uint32_t r = 0; r < 6; r++) { float s = 0.0f; for (uint32_t c = 0; c < 6; c++) s += d.art[wo + r * 6 + c] * x[c]; o[r] = s; }
+    out[0] = mk3(o[0], o[1], o[2]); out[1] = mk3(o[3], o[4], o[5]);
+  }
+  DEV void body_response_unsplit(int b, f3 f, f3 t, f3 out[2]) const {
+    if (b < 0 || body_kind(b) != 2) { body_response(b, f, t, out); return; }
+    uint32_t wo = lo((uint32_t)body_link(b)) + L_W2;
+    float x[6] = {f.x, f.y, f.z, t.x, t.y, t.z}, o[6];
+    for (uint32_t r = 0; r < 6; r++) { float s = 0.0f; for (uint32_t c = 0; c < 6; c++) s += d.art[wo + r * 6 + c] * x[c]; o[r] = s; }
+    out[0] = mk3(o[0], o[1], o[2]); out[1] = mk3(o[3], o[4], o[5]);
+  }
+};
+
+DEV W make_world(const uint32_t* Hs, const WorldDev& wd) {
+  W w;
+  w.H = Hs;
+  w.d = wd;
+  w.S = sched_layout(Hs[W_MAXC], Hs[W_MAXU], Hs[W_UH], Hs[W_NCOLW] * 32, Hs[W_NB], Hs[W_NCOLW], Hs[W_NL]);
+  return w;
+}

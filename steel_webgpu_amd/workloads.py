@@ -1,0 +1,193 @@
+"""Synthetic workloads: the benchmark scenes, built with SceneBuilder from a seeded PCG32 stream.
+
+Scenes (fps / substeps / steps as in the benchmark definitions):
+  boxes64        ground plane + 64 boxes, half extents U[0.05,0.2]^3, density 1000, mu 0.6; 240 / 4 / 500
+  arm_hulls      32-link fixed-base revolute capsule chain among 256 convex hulls in a walled 2x2 m bin; 240 / 8 / 1000
+  quadruped(i)   floating-base 12-joint quadruped + K~Geom(0.1)<=64 boxes, world i of a batch; 200 / 4 / 1000
+Gravity is -z and the ground is z = 0, so the vertical range of each spawn box is its z range.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .scene import SceneBuilder
+
+
+class PCG32:
+    """PCG-XSH-RR 64/32 (O'Neill 2014): seeded, stream-selectable, platform independent."""
+
+    MULT = 6364136223846793005
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed, stream=0):
+        self.inc = ((stream << 1) | 1) & self.MASK
+        self.state = 0
+        self.next_u32()
+        self.state = (self.state + seed) & self.MASK
+        self.next_u32()
+
+    def next_u32(self):
+        old = self.state
+        self.state = (old * self.MULT + self.inc) & self.MASK
+        xorshifted = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xorshifted >> rot) | (xorshifted << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * (self.next_u32() / 4294967296.0)
+
+    def geometric(self, p):
+        """Failures before the first success, P(k) = (1-p)^k p."""
+        u = 1.0 - self.uniform()
+        return int(math.floor(math.log(u) / math.log(1.0 - p)))
+
+    def unit_quat(self):
+        """Uniform random rotation (Shoemake, Graphics Gems III), as x, y, z, w."""
+        u1, u2, u3 = self.uniform(), self.uniform(), self.uniform()
+        a, b = math.sqrt(1 - u1), math.sqrt(u1)
+        return [a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2), b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3)]
+
+    def unit_vec(self):
+        z = self.uniform(-1, 1)
+        t = self.uniform(0, 2 * math.pi)
+        r = math.sqrt(max(0.0, 1 - z * z))
+        return [r * math.cos(t), r * math.sin(t), z]
+
+
+SEED = 0x57EE1000
+
+
+def _place(rng, lo, hi, radius, placed, tries=1000):
+    """Uniform position in the box [lo, hi], re-drawn while its bounding sphere overlaps a placed one or
+    reaches below the ground (z = 0)."""
+    for _ in range(tries):
+        p = [rng.uniform(lo[k], hi[k]) for k in range(3)]
+        if p[2] >= radius and all((p[0] - q[0]) ** 2 + (p[1] - q[1]) ** 2 + (p[2] - q[2]) ** 2 >= (radius + r) ** 2 for q, r in placed):
+            placed.append((p, radius))
+            return p
+    raise RuntimeError("no free spawn position")
+
+
+def _qmat(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def _obb_overlap(c1, R1, h1, c2, R2, h2):
+    """Exact separating-axis test of two oriented boxes (15 axes)."""
+    d = c2 - c1
+    axes = [R1[:, i] for i in range(3)] + [R2[:, i] for i in range(3)]
+    axes += [np.cross(R1[:, i], R2[:, j]) for i in range(3) for j in range(3)]
+    for a in axes:
+        n = np.linalg.norm(a)
+        if n < 1e-9:
+            continue
+        a = a / n
+        r1 = sum(h1[i] * abs(np.dot(R1[:, i], a)) for i in range(3))
+        r2 = sum(h2[i] * abs(np.dot(R2[:, i], a)) for i in range(3))
+        if abs(np.dot(d, a)) > r1 + r2:
+            return False
+    return True
+
+
+def _place_box(rng, lo, hi, h, q, boxes, tries=20000):
+    """Position in [lo, hi] for a box (half extents h, rotation q), re-drawn while it overlaps a placed
+    box (exact oriented-box test) or reaches below the ground."""
+    R = _qmat(q)
+    hz = sum(h[i] * abs(R[2, i]) for i in range(3))
+    hv = np.asarray(h, float)
+    rad = float(np.linalg.norm(hv))
+    for _ in range(tries):
+        c = np.array([rng.uniform(lo[k], hi[k]) for k in range(3)])
+        if c[2] - hz < 0.0:
+            continue
+        ok = True
+        for c2, R2, h2, r2 in boxes:
+            if np.sum((c - c2) ** 2) < (rad + r2) ** 2 and _obb_overlap(c, R, hv, c2, R2, h2):
+                ok = False
+                break
+        if ok:
+            boxes.append((c, R, hv, rad))
+            return [float(v) for v in c]
+    raise RuntimeError("no free spawn position")
+
+
+def _add_box(b, rng, placed, lo, hi, mu):
+    h = [rng.uniform(0.05, 0.2) for _ in range(3)]
+    q = rng.unit_quat()
+    p = _place_box(rng, lo, hi, h, q, placed)
+    body = b.addBody(xform=p + q)
+    b.addShapeBox(body, hx=h[0], hy=h[1], hz=h[2], mu=mu, density=1000.0)
+    return body
+
+
+def boxes64(seed=SEED + 0, n=64, fps=240, substeps=4, iterations=12):
+    """Ground plane + n random boxes (the plain single-world scene)."""
+    rng = PCG32(seed, 0)
+    b = SceneBuilder(fps=fps)
+    b.addGroundPlane(mu=0.6)
+    placed = []
+    for _ in range(n):
+        _add_box(b, rng, placed, (-1, -1, 0.2), (1, 1, 2.2), 0.6)
+    return b.finalize(solver=dict(substeps=substeps, iterations=iterations))
+
+
+def arm_hulls(seed=SEED + 1, nlinks=32, nhulls=256, fps=240, substeps=8, iterations=12):
+    """32-link fixed-base revolute capsule chain among convex hulls in a walled 2x2 m bin."""
+    rng = PCG32(seed, 0)
+    b = SceneBuilder(fps=fps)
+    b.addGroundPlane(mu=0.6)
+    for sx, sy, hx, hy in ((1.05, 0, 0.05, 1.1), (-1.05, 0, 0.05, 1.1), (0, 1.05, 1.1, 0.05), (0, -1.05, 1.1, 0.05)):
+        b.addShapeBox(-1, hx=hx, hy=hy, hz=0.5, mu=0.6, xform=[sx, sy, 0.5, 0, 0, 0, 1])
+    parent, prev_len, z = -1, 0.0, 0.0
+    for i in range(nlinks):
+        L = rng.uniform(0.1, 0.3)
+        body = b.addBody(xform=[0, 0, z, 0, 0, 0, 1])
+        b.addShapeCapsule(body, radius=0.03, halfHeight=L / 2, mu=0.6, xform=[0, 0, L / 2, 0, 0, 0, 1])
+        axis = rng.unit_vec()
+        b.addJointRevolute(parent, body, axis=axis, parentXform=[0, 0, prev_len if parent >= 0 else 0.0, 0, 0, 0, 1],
+                           limitLower=-math.pi / 2, limitUpper=math.pi / 2)
+        parent, prev_len, z = body, L, z + L
+    placed = [([0.0, 0.0, 0.5 * z], 0.25)]
+    for _ in range(nhulls):
+        r = rng.uniform(0.04, 0.12)
+        npts = 8 + int(rng.uniform(0, 9)) if True else 8
+        npts = min(npts, 16)
+        pts = []
+        for _ in range(npts):
+            v = rng.unit_vec()
+            pts.append([r * v[0], r * v[1], r * v[2]])
+        p = _place(rng, (-0.9, -0.9, 0.15), (0.9, 0.9, 2.5), r, placed)
+        body = b.addBody(xform=p + rng.unit_quat())
+        b.addShapeConvexHull(body, vertices=pts, mu=0.6, density=1000.0)
+    return b.finalize(solver=dict(substeps=substeps, iterations=iterations))
+
+
+def quadruped_world(index, seed=SEED + 2, fps=200, substeps=4, iterations=12, kmax=64):
+    """One world of the batch: floating-base quadruped (box base, 4 legs x 3 revolute capsule links) + K boxes."""
+    rng = PCG32(seed, index)
+    b = SceneBuilder(fps=fps)
+    b.addGroundPlane(mu=0.6)
+    base_z = 0.55
+    base = b.addBody(xform=[0, 0, base_z, 0, 0, 0, 1])
+    b.addShapeBox(base, hx=0.2, hy=0.05, hz=0.1, mu=0.6, density=1000.0)
+    axes = ([1, 0, 0], [0, 1, 0], [0, 1, 0])
+    for sx in (1, -1):
+        for sy in (1, -1):
+            parent = base
+            pxf = [sx * 0.2, sy * 0.05, -0.1, 0, 0, 0, 1]
+            for k in range(3):
+                body = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1])
+                b.addShapeCapsule(body, radius=0.02, halfHeight=0.075, mu=0.6, xform=[0, 0, -0.075, 0, 0, 0, 1])
+                b.addJointRevolute(parent, body, axis=axes[k], parentXform=pxf, q=rng.uniform(-0.3, 0.3))
+                parent, pxf = body, [0, 0, -0.15, 0, 0, 0, 1]
+    K = min(rng.geometric(0.1), kmax)
+    # keep-out box around the quadruped
+    placed = [(np.array([0.0, 0.0, base_z - 0.2]), np.eye(3), np.array([0.45, 0.3, 0.45]), 0.71)]
+    for _ in range(K):
+        _add_box(b, rng, placed, (-1.5, -1.5, 0.2), (1.5, 1.5, 2.2), 0.6)
+    return b.finalize(solver=dict(substeps=substeps, iterations=iterations))

@@ -1,0 +1,82 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so); test infrastructure."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+class Dbg(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("pairs", "npairs", "contacts", "ncontacts", "rowinfo", "nrows", "meta")]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE):
+            import subprocess
+            subprocess.check_call(["make", "-s", "-C", os.path.dirname(ORACLE)])
+        L = ctypes.CDLL(ORACLE)
+        L.oracle_anchor_val_size_f32.restype = ctypes.c_size_t
+        L.oracle_anchor_val_size_f64.restype = ctypes.c_size_t
+        L.oracle_hash_u32.restype = ctypes.c_uint32
+        L.oracle_hash_u32.argtypes = [ctypes.c_uint32]
+        for f in ("oracle_sinf", "oracle_cosf"):
+            getattr(L, f).restype = ctypes.c_float
+            getattr(L, f).argtypes = [ctypes.c_float]
+        L.oracle_atan2f.restype = ctypes.c_float
+        L.oracle_atan2f.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.oracle_friction_pair_solve.argtypes = [ctypes.c_float] * 9 + [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def P(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleWorld:
+    """One packed world stepped by the CPU oracle (float = parity oracle, double = error reference)."""
+
+    def __init__(self, pw, f64=False):
+        L = lib()
+        self.pw = pw
+        self.f64 = f64
+        dt = np.float64 if f64 else np.float32
+        self.poses = pw.poses0.astype(dt)
+        self.vel = (pw.vel0 if pw.vel0.size else np.zeros(8, np.float32)).astype(dt)
+        self.art = pw.art0.astype(dt)
+        AH = pw.h("W_AH")
+        self.akey = np.zeros(2 * AH, np.uint32)
+        vs = L.oracle_anchor_val_size_f64() if f64 else L.oracle_anchor_val_size_f32()
+        self.aval = np.zeros(2 * AH * vs, np.uint8)
+        self.count = np.zeros(1, np.uint32)
+        self.stats = np.zeros(16, np.uint32)
+        self.statU = pw.statU if pw.statU.size else np.zeros(1, np.uint32)
+        MAXC, MAXP = pw.h("W_MAXC"), pw.h("W_MAXP")
+        self.dbg_bufs = dict(pairs=np.zeros(2 * MAXP, np.uint32), npairs=np.zeros(1, np.uint32), contacts=np.zeros(MAXC * 24, np.float32),
+                             ncontacts=np.zeros(1, np.uint32), rowinfo=np.zeros(MAXC * 4, np.uint32), nrows=np.zeros(1, np.uint32),
+                             meta=np.zeros(8, np.uint32))
+        self.dbg = Dbg(**{k: P(v).value for k, v in self.dbg_bufs.items()})
+
+    def substeps(self, n, debug=False):
+        L = lib()
+        fn = L.oracle_substeps_f64 if self.f64 else L.oracle_substeps_f32
+        rc = fn(P(self.pw.header), P(self.pw.statF), P(self.statU), P(self.poses), P(self.vel), P(self.art), P(self.akey), P(self.aval),
+                P(self.count), ctypes.c_uint32(n), P(self.stats), ctypes.byref(self.dbg) if debug else None)
+        assert rc == 0
+
+    def frames(self, n):
+        self.substeps(n * self.pw.h("W_SUBSTEPS"))
+
+    def anchors_u32(self):
+        assert not self.f64
+        AH = self.pw.h("W_AH")
+        out = np.zeros(2 * AH + 2 * AH * 14, np.uint32)
+        lib().oracle_anchor_export_f32(ctypes.c_uint32(AH), P(self.akey), P(self.aval), P(out))
+        return out
