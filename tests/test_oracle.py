@@ -1,0 +1,354 @@
+"""Pinning tests of the CPU oracle (oracle/steel_oracle.cpp) against facts that do not come from its own code:
+published inverses, libm, closed forms, brute force, definitions (first-fit coloring, KKT conditions),
+invariants (momentum, energy).  The reference ships no physics fixtures or golden vectors, so these are
+what pins the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle_bind import OracleWorld, lib
+from steel_webgpu_amd import SceneBuilder, pack_scene, workloads
+
+
+def lowbias32_inverse(x):
+    """Published inverse of lowbias32 (C. Wellons, hash-prospector, "lowbias32_r")."""
+    M = 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x43021123) & M
+    x ^= (x >> 15) ^ (x >> 30)
+    x = (x * 0x1D69E2A5) & M
+    x ^= x >> 16
+    return x
+
+
+def test_hash_u32_inverts_with_published_inverse():
+    L = lib()
+    rng = np.random.default_rng(1)
+    for x in list(rng.integers(0, 2**32, 2000, dtype=np.uint64)) + [0, 1, 0xFFFFFFFF, 0x80000000]:
+        x = int(x)
+        assert L.oracle_hash_u32(lowbias32_inverse(x)) == x
+    assert L.oracle_hash_u32(0) == 0
+
+
+def test_sin_cos_atan2_within_two_ulp_of_libm():
+    L = lib()
+    xs = np.concatenate([np.linspace(-20, 20, 4001), np.linspace(-1e-3, 1e-3, 101)]).astype(np.float32)
+    for fn, ref in ((L.oracle_sinf, np.sin), (L.oracle_cosf, np.cos)):
+        got = np.array([fn(float(x)) for x in xs], np.float32)
+        want = ref(xs.astype(np.float64))
+        ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+        err = np.abs(got.astype(np.float64) - want)
+        assert np.all(err <= 2 * ulp + 1e-9), float((err / ulp).max())
+    ys = np.linspace(1e-5, 3, 301).astype(np.float32)
+    for x in (-2.0, -0.5, 1e-3, 0.7, 3.0):
+        got = np.array([L.oracle_atan2f(float(y), x) for y in ys])
+        want = np.arctan2(ys.astype(np.float64), x)
+        assert np.all(np.abs(got - want) <= 3 * np.spacing(np.abs(want).astype(np.float32)) + 1e-9)
+
+
+def test_friction_pair_solve_meets_kkt_of_the_disk_problem():
+    """min 1/2 (l-lo)^T K (l-lo) + v^T (l-lo) s.t. |l| <= R (omega 1): stationarity with a radial multiplier."""
+    L = lib()
+    rng = np.random.default_rng(2)
+    out = np.zeros(2, np.float32)
+    from oracle_bind import P
+    for _ in range(400):
+        A = rng.normal(size=(2, 2))
+        K = A @ A.T + 0.05 * np.eye(2)
+        v = rng.normal(size=2) * rng.choice([0.01, 1, 10])
+        lo = rng.normal(size=2) * 0.3
+        R = abs(rng.normal()) + 0.05
+        if np.linalg.norm(lo) > R:
+            lo = lo * (R / np.linalg.norm(lo)) * 0.99
+        L.oracle_friction_pair_solve(*(float(x) for x in (K[0, 0], K[0, 1], K[1, 1], v[0], v[1], lo[0], lo[1], R, 1.0)), P(out))
+        lam = out.astype(np.float64)
+        g = K @ (lam - lo) + v  # gradient; KKT: g = -mu * lam, mu >= 0, or g = 0 inside
+        nrm = np.linalg.norm(lam)
+        assert nrm <= R * (1 + 1e-5)
+        scale = np.linalg.norm(K) * max(np.linalg.norm(lam - lo), 1e-3) + np.linalg.norm(v)
+        if nrm < R * (1 - 1e-4):
+            assert np.linalg.norm(g) <= 2e-4 * scale
+        else:
+            # g antiparallel to lam
+            cross = abs(g[0] * lam[1] - g[1] * lam[0])
+            assert cross <= 2e-4 * scale * nrm
+            assert g @ lam <= 2e-4 * scale * nrm
+
+
+def _one_substep(scene, debug=True, f64=False):
+    pw = pack_scene(scene)
+    o = OracleWorld(pw, f64=f64)
+    o.substeps(1, debug=debug)
+    return pw, o
+
+
+def _contacts(o):
+    n = int(o.dbg_bufs["ncontacts"][0])
+    return o.dbg_bufs["contacts"][:n * 24].reshape(n, 24)
+
+
+def test_broadphase_equals_brute_force_all_pairs():
+    scene = workloads.boxes64()
+    pw, o = _one_substep(scene)
+    NS = pw.NS
+    aabb = o.poses[pw.h("W_P_AABB") * 4:(pw.h("W_P_AABB") + 2 * NS) * 4].reshape(NS, 2, 4)
+    body = [int(pw.statF[(pw.h("W_SF_SHAPE") + s * 7) * 4]) for s in range(NS)]
+    want = set()
+    for i in range(NS):
+        for j in range(i + 1, NS):
+            if body[i] == body[j] or (body[i] < 0 and body[j] < 0):
+                continue
+            lo_i, hi_i, lo_j, hi_j = aabb[i, 0, :3], aabb[i, 1, :3], aabb[j, 0, :3], aabb[j, 1, :3]
+            if np.any(lo_i > hi_j) or np.any(lo_j > hi_i):
+                continue
+            want.add((i, j))
+    n = int(o.dbg_bufs["npairs"][0])
+    got = set(map(tuple, o.dbg_bufs["pairs"][:2 * n].reshape(-1, 2).tolist()))
+    assert got == want and len(want) > 0
+
+
+def test_box_on_plane_four_corner_contacts():
+    """A box resting axis-aligned at height z over the plane: 4 bottom corners, phi = z - hz, n = +z."""
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane()
+    h = (0.1, 0.15, 0.05)
+    z = 0.08
+    body = b.addBody(xform=[0.3, -0.2, z, 0, 0, 0, 1])
+    b.addShapeBox(body, hx=h[0], hy=h[1], hz=h[2])
+    pw, o = _one_substep(b.finalize(solver=dict(substeps=1)))
+    C = _contacts(o)
+    assert len(C) == 4
+    phi = C[:, 3]
+    np.testing.assert_allclose(phi, z - h[2], atol=2e-6)
+    np.testing.assert_allclose(C[:, 4:7], np.tile([0, 0, 1], (4, 1)), atol=1e-6)
+    xs = sorted(map(tuple, np.round(C[:, :2], 5).tolist()))
+    want = sorted((0.3 + sx * h[0], -0.2 + sy * h[1]) for sx in (-1, 1) for sy in (-1, 1))
+    np.testing.assert_allclose(np.array(xs), np.array(want), atol=2e-5)
+    # point midway between the surfaces
+    np.testing.assert_allclose(C[:, 2], 0.5 * (z - h[2]), atol=2e-6)
+
+
+def test_sphere_sphere_closest_points():
+    b = SceneBuilder(fps=240)
+    p1, p2, r1, r2 = np.array([0.0, 0.0, 1.0]), np.array([0.05, 0.12, 1.09]), 0.07, 0.05
+    for p, r in ((p1, r1), (p2, r2)):
+        bd = b.addBody(xform=list(p) + [0, 0, 0, 1], gravity=False)
+        b.addShapeSphere(bd, radius=r)
+    pw, o = _one_substep(b.finalize(solver=dict(substeps=1)))
+    C = _contacts(o)
+    assert len(C) == 1
+    d = np.linalg.norm(p2 - p1)
+    n = (p2 - p1) / d
+    np.testing.assert_allclose(C[0, 3], d - r1 - r2, atol=2e-6)
+    np.testing.assert_allclose(C[0, 4:7], n, atol=2e-6)
+    np.testing.assert_allclose(C[0, :3], p1 + n * (r1 + 0.5 * (d - r1 - r2)), atol=2e-6)
+
+
+def test_stacked_boxes_face_manifold_on_overlap():
+    """A small box centred on a larger one: four contacts at the small box's bottom corners."""
+    b = SceneBuilder(fps=240)
+    lower = b.addBody(xform=[0, 0, 1.0, 0, 0, 0, 1], gravity=False)
+    b.addShapeBox(lower, hx=0.3, hy=0.3, hz=0.1)
+    gap = 0.01
+    upper = b.addBody(xform=[0.02, -0.03, 1.0 + 0.1 + 0.05 + gap, 0, 0, 0, 1], gravity=False)
+    b.addShapeBox(upper, hx=0.1, hy=0.08, hz=0.05)
+    pw, o = _one_substep(b.finalize(solver=dict(substeps=1)))
+    C = _contacts(o)
+    assert len(C) == 4
+    np.testing.assert_allclose(C[:, 3], gap, atol=3e-6)
+    np.testing.assert_allclose(np.abs(C[:, 6]), 1.0, atol=1e-6)
+    xs = sorted(map(tuple, np.round(C[:, :2], 5).tolist()))
+    want = sorted((0.02 + sx * 0.1, -0.03 + sy * 0.08) for sx in (-1, 1) for sy in (-1, 1))
+    np.testing.assert_allclose(np.array(xs), np.array(want), atol=3e-5)
+
+
+def test_contacts_sorted_by_canonical_key():
+    pw, o = _one_substep(workloads.boxes64())
+    C = _contacts(o)
+    pair = (C[:, 8].astype(np.int64) + 1) | ((C[:, 9].astype(np.int64) + 1) << 16)
+    feat = C[:, 10].astype(np.int64) | (C[:, 11].astype(np.int64) << 16)
+    key = (pair << 32) | feat
+    assert np.all(np.diff(key) > 0)
+
+
+def test_coloring_is_first_fit_in_walk_order():
+    """Each row takes the lowest color no earlier row of either of its bodies holds (the world never
+    conflicts); colors past 63 fold into the serial overflow color 64.  Rows are in walk order."""
+    for scene in (workloads.boxes64(), workloads.quadruped_world(3)):
+        pw = pack_scene(scene)
+        o = OracleWorld(pw)
+        o.substeps(600, debug=True)
+        nr = int(o.dbg_bufs["nrows"][0])
+        info = o.dbg_bufs["rowinfo"][:nr * 4].reshape(nr, 4)
+        used = {}
+        ncol = 64
+        for r in range(nr):
+            key = int(info[r, 3])
+            a, b = (key & 0xFFFF) - 1, (key >> 16) - 1
+            busy = used.get(a, set()) | used.get(b, set())
+            c = next((k for k in range(ncol) if k not in busy), ncol)
+            assert info[r, 1] == c
+            if c < ncol:
+                for x in (a, b):
+                    if x >= 0:
+                        used.setdefault(x, set()).add(c)
+        assert nr > 10
+
+
+def _free_box_scene(z=5.0, gravity=True, twist=(0, 0, 0, 0, 0, 0)):
+    b = SceneBuilder(fps=240)
+    bd = b.addBody(xform=[0.1, 0.2, z, 0, 0, 0, 1], gravity=gravity, twist=twist)
+    b.addShapeBox(bd, hx=0.1, hy=0.2, hz=0.3)
+    return b.finalize(solver=dict(substeps=4))
+
+
+def test_ballistic_free_fall_closed_form_fp64():
+    pw = pack_scene(_free_box_scene())
+    o = OracleWorld(pw, f64=True)
+    n = 40
+    o.substeps(n)
+    dt = pw.hf("W_DT")
+    g = -9.81
+    gz = float(np.float32(g))
+    assert o.vel[2] == pytest.approx(n * dt * gz, rel=1e-12)
+    zexp = 5.0 + dt * dt * gz * n * (n + 1) / 2
+    assert o.poses[2] == pytest.approx(zexp, rel=1e-12)
+    assert o.poses[0] == float(np.float32(0.1)) and o.poses[1] == float(np.float32(0.2))
+
+
+def test_gyroscopic_step_conserves_kinetic_energy():
+    """Free asymmetric box spinning with no damping and no gravity: Cayley microsteps keep w^T I w."""
+    scene = _free_box_scene(gravity=False, twist=(0, 0, 0, 3.0, -2.0, 5.0))
+    scene.json["bodies"]["angular_damping"] = [0.0]
+    pw = pack_scene(scene)
+    o = OracleWorld(pw, f64=True)
+    I = np.array(scene.json["bodies"]["inertia"][0], np.float32).astype(np.float64).reshape(3, 3)
+
+    def energy():
+        q = o.poses[4:8]
+        x, y, z, w = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        wv = o.vel[4:7]
+        return wv @ (R @ I @ R.T) @ wv
+
+    e0 = energy()
+    o.substeps(200)
+    # explicit Euler on the same spin drifts ~1e-2 here; the Cayley microsteps stay within the
+    # fixed-point tolerance of each microstep
+    assert energy() == pytest.approx(e0, rel=1e-6)
+
+
+def test_collision_conserves_linear_momentum():
+    b = SceneBuilder(fps=240)
+    b1 = b.addBody(xform=[0, 0, 1, 0, 0, 0, 1], gravity=False, twist=(1.0, 0.2, 0, 0, 0, 0))
+    b.addShapeBox(b1, hx=0.1, hy=0.1, hz=0.1, mu=0.5)
+    b2 = b.addBody(xform=[0.25, 0.05, 1.02, 0, 0, 0, 1], gravity=False, twist=(-1.5, 0, 0.1, 0, 0, 0))
+    b.addShapeBox(b2, hx=0.08, hy=0.12, hz=0.1, mu=0.5)
+    scene = b.finalize(solver=dict(substeps=4))
+    pw = pack_scene(scene)
+    o = OracleWorld(pw, f64=True)
+    inv_m = np.array([pw.statF[(pw.h("W_SF_BODY") + b * 8) * 4] for b in range(pw.NB)], np.float64)
+    m = 1.0 / inv_m  # the masses the step integrates with
+
+    def mom():
+        v = o.vel.reshape(-1, 8)[:, :3]
+        return (m[:, None] * v).sum(0)
+
+    p0 = mom()
+    o.substeps(120)
+    assert o.stats[1] > 0  # they touched
+    np.testing.assert_allclose(mom(), p0, atol=1e-9 * np.abs(p0).max())
+
+
+def test_floating_base_free_fall_has_zero_joint_acceleration():
+    """Articulated-body free velocity (qd_hat) of an untouched floating quadruped at rest: the whole tree falls,
+    base acceleration g, joint accelerations 0 (exact for any configuration)."""
+    scene = workloads.quadruped_world(7)
+    for p in scene.json["bodies"]["q"]:
+        p[2] += 3.0
+    for j, t in enumerate(scene.json["joints"]["type"]):
+        if t == 4:
+            scene.json["joints"]["q"][scene.json["joints"]["q_start"][j] + 2] += 3.0
+    pw = pack_scene(scene)
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    A = pw.A
+    dt = pw.hf("W_DT")
+    qdh = o.art[A["A_QDH"]:A["A_QDH"] + pw.ND]
+    g = np.array([0, 0, np.float32(-9.81)], np.float64)
+    np.testing.assert_allclose(qdh[0:3], dt * g, atol=1e-12)
+    np.testing.assert_allclose(qdh[3:], 0.0, atol=1e-12)
+
+
+def _rot(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def test_fixed_base_chain_free_acceleration_matches_crba():
+    """qd_hat of a fixed-base revolute chain at rest = q_dot + dt * H^-1 tau_gravity, with H and tau from the
+    composite rigid-body formula (sum over links of COM Jacobians) and forward kinematics computed here from the
+    scene's joint arrays."""
+    from scipy.spatial.transform import Rotation as Rot
+    rng = np.random.default_rng(5)
+    b = SceneBuilder(fps=240)
+    parent, prev = -1, 0.0
+    for i in range(7):
+        l = float(rng.uniform(0.1, 0.3))
+        bd = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1])
+        b.addShapeCapsule(bd, radius=0.03, halfHeight=l / 2, xform=[0, 0, l / 2, 0, 0, 0, 1])
+        ax = rng.normal(size=3)
+        b.addJointRevolute(parent, bd, axis=list(ax / np.linalg.norm(ax)), parentXform=[0, 0, prev, 0, 0, 0, 1], q=float(rng.uniform(-1, 1)))
+        parent, prev = bd, l
+    scene = b.finalize(solver=dict(substeps=1))
+    S, J = scene.json, scene.json["joints"]
+    n = J["count"]
+    # forward kinematics (body frame = joint child frame; X_c identity, X_p translation only here)
+    R_prev, p_prev = np.eye(3), np.zeros(3)
+    axes_w, orig_w, coms, Rs = [], [], [], []
+    for j in range(n):
+        xp = np.array(J["X_p"][j][:3])
+        a = np.array(J["axis"][J["qd_start"][j]])
+        o = p_prev + R_prev @ xp
+        aw = R_prev @ a
+        R = R_prev @ Rot.from_rotvec(a / np.linalg.norm(a) * J["q"][J["q_start"][j]]).as_matrix()
+        axes_w.append(aw); orig_w.append(o); Rs.append(R)
+        coms.append(o + R @ np.array(S["bodies"]["com"][J["child"][j]]))
+        R_prev, p_prev = R, o
+    H = np.zeros((n, n))
+    tau = np.zeros(n)
+    g = np.array([0, 0, float(np.float32(-9.81))])
+    for l in range(n):
+        bd = J["child"][l]
+        m = S["bodies"]["mass"][bd]
+        Iw = Rs[l] @ np.array(S["bodies"]["inertia"][bd]).reshape(3, 3) @ Rs[l].T
+        Jv, Jw = np.zeros((3, n)), np.zeros((3, n))
+        for k in range(l + 1):
+            Jw[:, k] = axes_w[k]
+            Jv[:, k] = np.cross(axes_w[k], coms[l] - orig_w[k])
+        H += m * Jv.T @ Jv + Jw.T @ Iw @ Jw
+        tau += Jv.T @ (m * g)
+    qdd = np.linalg.solve(H, tau)
+    pw = pack_scene(scene)
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    dt = pw.hf("W_DT")
+    qdh = o.art[pw.A["A_QDH"]:pw.A["A_QDH"] + n]
+    np.testing.assert_allclose(qdh / dt, qdd, rtol=2e-6)
+
+
+def test_fp32_oracle_tracks_fp64_for_one_step():
+    """Error of one float step against the double reimplementation of the same step (reported figure)."""
+    for scene in (workloads.boxes64(), workloads.quadruped_world(1)):
+        pw = pack_scene(scene)
+        o32, o64 = OracleWorld(pw), OracleWorld(pw, f64=True)
+        o32.frames(1)
+        o64.frames(1)
+        NB = pw.NB
+        p32, p64 = o32.poses[:NB * 8].astype(np.float64), o64.poses[:NB * 8]
+        assert np.abs(p32 - p64).max() <= 1e-5 * max(1.0, np.abs(p64).max())
