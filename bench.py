@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Benchmark of the Steel step path on one MI355X.
+
+Workload (the batching configuration): 4096 worlds of uneven size, each a floating-base 12-joint quadruped plus
+K ~ Geometric(0.1) <= 64 boxes on a ground plane, FP32, dt = 1/200 s, 4 substeps, 12 iterations, all worlds in
+one launch per kernel with the frame's substep loop captured in a hipGraph.
+
+value       batched world-steps/s (worlds x frames / wall time of the timed frames, inputs resident in HBM)
+plain       world-steps/s of the plain per-world step (one world per launch, per-substep launches, as the
+            reference dispatches), measured on a sample of the same worlds
+latency     p50/p99 of the batched frame time (HIP events per frame)
+roofline    dominant kernel: algorithmic bytes per launch / average launch time (HIP events on the library's
+            stream) against 8 TB/s
+cpu_baseline  the FP32 CPU oracle (port of the reference step), one world per process on the host cores
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "world-steps/s batched vs plain per-world (FP32) + p50/p99 step latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def _pack_world(i):
+    from steel_webgpu_amd import pack, workloads
+    pw = pack.pack_scene(workloads.quadruped_world(i), capacity_policy="compact")
+    pw.scene = None
+    return pw
+
+
+def build_worlds(n, workers):
+    if workers <= 1:
+        return [_pack_world(i) for i in range(n)]
+    ctx = mp.get_context("fork")
+    with ctx.Pool(workers) as pool:
+        return pool.map(_pack_world, range(n), chunksize=max(1, n // (workers * 8)))
+
+
+def hull_bytes(pw):
+    """16 nv + 24 nf + 4 sum|face| + 16 ne per hull (the per-pair hull traffic of a SAT/manifold query)."""
+    out = {}
+    NH = pw.h("W_NH")
+    SU = pw.statU
+    for h in range(NH):
+        nv, nf = int(SU[pw.h("W_SU_HULL") + h * 4 + 1]), int(SU[pw.h("W_SU_HULL") + h * 4 + 3])
+        fo = int(SU[pw.h("W_SU_HULL") + h * 4 + 2])
+        fsum = sum(int(SU[pw.h("W_SU_FACE") + (fo + f) * 2 + 1]) for f in range(nf))
+        ne = int(SU[pw.h("W_SU_HEDGE") + h * 2 + 1])
+        out[h] = 16 * nv + 24 * nf + 4 * fsum + 16 * ne
+    return out
+
+
+def algorithmic_bytes(batch, worlds):
+    """Per-substep compulsory bytes of each kernel, summed over worlds, from each world's last-substep counts.
+
+    collide  288 NS + 296 NP + Hp + 304 NC      (shape pose/AABB, pair record + two shape records, hull data of
+                                                 hull pairs, contact write + sort read/key/write)
+    prepare  96 NC + 432 NR + 1200 NL           (sorted contact read, row build, articulation factor/response)
+    solve    (It+1)(384 NR + 576 NL) + 200 NB + 8 ND   (row read + impulse write per sweep, link propagation,
+                                                 predictor/integrator, joint state)
+    """
+    tot = np.zeros(3)
+    for i, pw in enumerate(worlds):
+        st, _ = batch.stats_raw(i)
+        NP, NC, NR = int(st[9]), int(st[5]), int(st[10])
+        NS, NB, NL, ND = pw.NS, pw.NB, pw.NL, pw.ND
+        It = pw.h("W_ITERS")
+        hb = hull_bytes(pw)
+        pairs = batch.read(i, "pairs", 8 * min(NP, pw.h("W_MAXP")), dtype=np.uint32).reshape(-1, 2) if NP else np.zeros((0, 2), np.int64)
+        Hp = 0
+        SF = pw.statF.reshape(-1, 4)
+        for a, b in pairs:
+            ka, kb = SF[pw.h("W_SF_SHAPE") + a * 7 + 3][0], SF[pw.h("W_SF_SHAPE") + b * 7 + 3][0]
+            if ka == 4 or kb == 4:
+                if ka == 0 or kb == 0:  # plane against a convex hull
+                    s = b if ka == 4 else a
+                    Hp += hb[int(SF[pw.h("W_SF_SHAPE") + s * 7][1])]
+                continue
+            for s, k in ((a, ka), (b, kb)):
+                if k == 0:
+                    Hp += hb[int(SF[pw.h("W_SF_SHAPE") + s * 7][1])]
+        tot += [288 * NS + 296 * NP + Hp + 304 * NC, 96 * NC + 432 * NR + 1200 * NL,
+                (It + 1) * (384 * NR + 576 * NL) + 200 * NB + 8 * ND]
+    return tot
+
+
+def _cpu_worker(args):
+    idx, frames = args
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import OracleWorld
+    n = 0
+    t = 0.0
+    for i in idx:
+        pw = _pack_world(i)
+        o = OracleWorld(pw)
+        t0 = time.perf_counter()
+        o.frames(frames)
+        t += time.perf_counter() - t0
+        n += frames
+    return n, t
+
+
+def cpu_baseline(nworlds, frames, procs):
+    """FP32 CPU oracle (the port of the reference step) on `procs` processes; returns the aggregate rate
+    (all world-steps / the busiest process's stepping time), the step count and the wall time."""
+    chunks = [list(range(k, nworlds, procs)) for k in range(procs)]
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_worker, [(c, frames) for c in chunks])
+    wall = time.perf_counter() - t0
+    steps = sum(r[0] for r in res)
+    busy = max(r[1] for r in res)
+    return steps / busy, steps, wall
+
+
+def traffic_from_profile(kernel):
+    path = os.path.join(ROOT, "profiles", f"{kernel}_quadruped4096_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--plain-worlds", type=int, default=8)
+    ap.add_argument("--plain-steps", type=int, default=40)
+    ap.add_argument("--cpu-worlds", type=int, default=64)
+    ap.add_argument("--cpu-frames", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0, help="prepare/solve workgroup size (0: autotune)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (worlds shard whole across GPUs with no "
+              f"per-step exchange, so N GPUs would run N independent batches).", file=sys.stderr)
+        sys.exit(2)
+    procs = max(1, min(16, os.cpu_count() or 1))
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    # Host-side work that uses worker processes runs before anything touches the GPU.
+    t0 = time.time()
+    worlds = build_worlds(args.worlds, procs)
+    log(f"built {len(worlds)} worlds in {time.time() - t0:.1f}s; bodies/world mean {np.mean([w.NB for w in worlds]):.1f} "
+        f"max {max(w.NB for w in worlds)}")
+    cpu = None
+    if not args.no_cpu:
+        rate, steps, cwall = cpu_baseline(args.cpu_worlds, args.cpu_frames, procs)
+        cpu = {"value": round(rate, 1), "unit": "world-steps/s", "cores": procs, "kind": "port",
+               "sample": f"worlds 0..{args.cpu_worlds - 1} of the {args.worlds}-world quadruped batch x {args.cpu_frames} frames from "
+                         f"rest, FP32 oracle, one world at a time per process ({steps} world-steps, {cwall:.1f}s wall)"}
+        log(f"cpu baseline: {cpu['value']} world-steps/s on {procs} processes")
+    from steel_webgpu_amd import sim
+    dev = sim.Device(0)
+    log(f"device {dev.name}, {dev.num_cus} CUs, {dev.total_mem / 2**30:.0f} GiB")
+    batch = sim.Batch(worlds, dev, block=256, graph=True)
+    log(f"batch resident: {batch.device_bytes() / 2**30:.2f} GiB")
+    # Autotuner: prepare/solve workgroup size by measured frame time.
+    block = args.block
+    if block == 0:
+        best = None
+        for bs in (64, 128, 256):
+            batch.configure(bs, True)
+            batch.reset()
+            batch.step(3)
+            ms = float(np.median(batch.step_timed(4)))
+            log(f"  autotune block {bs}: {ms:.2f} ms/frame")
+            if best is None or ms < best[1]:
+                best = (bs, ms)
+        block = best[0]
+    batch.configure(block, True)
+    batch.reset()
+    batch.step(args.warmup)
+    batch.sync()
+    t0 = time.perf_counter()
+    frame_ms = batch.step_timed(args.steps)
+    batch.sync()
+    wall = time.perf_counter() - t0
+    nw = len(worlds)
+    value = nw * args.steps / wall
+    ms_per_step = wall / args.steps * 1e3
+    log(f"batched: {value:.0f} world-steps/s, {ms_per_step:.3f} ms/frame (block {block})")
+    # Roofline of the dominant kernel.
+    S = worlds[0].h("W_SUBSTEPS")
+    kms = batch.profile_kernels(2 * S)
+    algo = algorithmic_bytes(batch, worlds)
+    names = ["collide", "prepare", "solve"]
+    k = int(np.argmax(kms))
+    achieved = algo[k] / (kms[k] * 1e-3) / 1e9
+    overflow = sum(1 for i in range(nw) if batch.stats_raw(i)[0][1] >= worlds[i].h("W_MAXC"))
+    log(f"kernels ms/launch collide {kms[0]:.3f} prepare {kms[1]:.3f} solve {kms[2]:.3f}; algorithmic MB/launch "
+        f"{algo[0] / 1e6:.1f} {algo[1] / 1e6:.1f} {algo[2] / 1e6:.1f}; worlds at contact capacity: {overflow}")
+    batch.destroy()
+    # Plain per-world step: one world per batch, one launch per kernel per substep (no graph).
+    plain_rates, plain_ms = [], []
+    for i in range(min(args.plain_worlds, nw)):
+        b1 = sim.Batch([worlds[i * (nw // max(1, args.plain_worlds))]], dev, block=64, graph=False)
+        b1.step(5)
+        b1.sync()
+        t1 = time.perf_counter()
+        fm = b1.step_timed(args.plain_steps)
+        b1.sync()
+        dtp = time.perf_counter() - t1
+        plain_rates.append(args.plain_steps / dtp)
+        plain_ms.extend(fm.tolist())
+        b1.destroy()
+    plain = float(np.mean(plain_rates))
+    log(f"plain per-world: {plain:.1f} world-steps/s ({np.median(plain_ms):.3f} ms/frame GPU)")
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "world-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded PCG32 scenes)",
+        "config": {"workload": "quadruped_batch", "worlds": nw, "fps": 200, "substeps": S, "iterations": worlds[0].h("W_ITERS"),
+                   "block": block, "graph": True},
+        "plain_world_steps_per_s": round(plain, 2),
+        "speedup_vs_plain": round(value / plain, 1),
+        "latency_ms": {"p50": round(float(np.percentile(frame_ms, 50)), 4), "p99": round(float(np.percentile(frame_ms, 99)), 4),
+                       "plain_p50": round(float(np.percentile(plain_ms, 50)), 4), "plain_p99": round(float(np.percentile(plain_ms, 99)), 4)},
+        "roofline": {"bound": "hbm", "kernel": names[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(names[k]),
+                     "kernel_ms": [round(x, 4) for x in kms], "algorithmic_bytes_per_launch": [int(x) for x in algo]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -38,7 +38,8 @@ enum steel_buffer {
   STEEL_BUF_ROWS = 7,      /* float4[MAXC*ROWS] */
   STEEL_BUF_SCHED = 8,     /* uint32 schedule tables */
   STEEL_BUF_COUNTERS = 9,  /* uint32[4]: pairs, contacts, substep count, flags */
-  STEEL_BUF_STATS = 10,    /* uint32[16], the reference's stats words (src/wgsl.js:3879-3890) */
+  STEEL_BUF_STATS = 10,    /* uint32[16]: the reference's stats words 0-8 (src/wgsl.js:3879-3890); 9-11 the last
+                              substep's candidate pairs, rows and units */
   STEEL_BUF_COUNT = 11
 };
 
@@ -81,12 +82,17 @@ int steel_batch_reset(steel_batch* b, int32_t world);
 int steel_batch_step(steel_batch* b, uint32_t frames);
 /* One substep with only the given stages (STEEL_STAGE_*), for stage-isolated parity checks. */
 int steel_batch_substep_stages(steel_batch* b, uint32_t stages);
+/* steel_batch_step one frame at a time with a HIP event after each; frame_ms[f] is frame f's GPU time. */
+int steel_batch_step_timed(steel_batch* b, uint32_t frames, float* frame_ms);
+/* Average GPU time per launch of k_collide, k_prepare and k_solve over `substeps` substeps launched one
+ * kernel at a time with HIP events between them on the batch's stream (the state advances). */
+int steel_batch_profile_kernels(steel_batch* b, uint32_t substeps, double kernel_ms[3]);
 /* Wait for the batch's stream. */
 int steel_batch_sync(steel_batch* b);
 /* Sim.read / queue.writeBuffer on one world's buffer (sim.js:716-802); read synchronises the stream. */
 int steel_batch_read(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_off, size_t bytes, void* dst);
 int steel_batch_write(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_off, size_t bytes, const void* src);
-/* Whole-buffer gather of one buffer for all worlds into dst (worlds back to back, each its own size). */
+/* Size in bytes of one world's buffer `buf` (worlds of a batch may differ in size). */
 int steel_batch_buffer_size(steel_batch* b, uint32_t world, uint32_t buf, size_t* bytes);
 /* Sim.stats (sim.js:822-834): the 16 stats words of one world and the GPU time of the last step. */
 int steel_batch_stats(steel_batch* b, uint32_t world, uint32_t out[16], double* gpu_ms);

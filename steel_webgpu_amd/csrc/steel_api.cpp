@@ -327,6 +327,45 @@ int steel_batch_substep_stages(steel_batch* b, uint32_t stages) {
   return STEEL_OK;
 }
 
+int steel_batch_step_timed(steel_batch* b, uint32_t frames, float* frame_ms) {
+  if (!b || (frames && !frame_ms)) return fail(STEEL_BAD_ARGUMENT, "null argument");
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  std::vector<hipEvent_t> ev(frames + 1);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  int rc = STEEL_OK;
+  HIPCHK(hipEventRecord(ev[0], b->stream));
+  for (uint32_t f = 0; f < frames && rc == STEEL_OK; f++) {
+    rc = steel_batch_step(b, 1);
+    if (rc == STEEL_OK && hipEventRecord(ev[f + 1], b->stream) != hipSuccess) rc = fail(STEEL_ERROR, "event record failed");
+  }
+  if (rc == STEEL_OK && hipEventSynchronize(ev[frames]) != hipSuccess) rc = fail(STEEL_ERROR, "event sync failed");
+  for (uint32_t f = 0; f < frames && rc == STEEL_OK; f++) {
+    if (hipEventElapsedTime(&frame_ms[f], ev[f], ev[f + 1]) != hipSuccess) rc = fail(STEEL_ERROR, "event timing failed");
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int steel_batch_profile_kernels(steel_batch* b, uint32_t substeps, double kernel_ms[3]) {
+  if (!b || !kernel_ms) return fail(STEEL_BAD_ARGUMENT, "null argument");
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  std::vector<hipEvent_t> ev(4 * substeps);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  int rc = STEEL_OK;
+  for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)
+    if (steel_launch_substep(b->d_worlds, b->n, b->block, b->stream, STAGE_ALL, &ev[4 * s]) != hipSuccess) rc = fail(STEEL_ERROR, "launch failed");
+  if (rc == STEEL_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = fail(STEEL_ERROR, "sync failed");
+  for (int k = 0; k < 3; k++) kernel_ms[k] = 0.0;
+  for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)
+    for (int k = 0; k < 3; k++) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev[4 * s + k], ev[4 * s + k + 1]) != hipSuccess) { rc = fail(STEEL_ERROR, "event timing failed"); break; }
+      kernel_ms[k] += ms / substeps;
+    }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
 int steel_batch_sync(steel_batch* b) {
   if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
   HIPCHK(hipSetDevice(b->dev->ordinal));

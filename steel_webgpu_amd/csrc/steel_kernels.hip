@@ -1416,6 +1416,9 @@ __global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ world
     st[6] = ncol;
     st[8] = 0;
     st[7] = max(st[7], sched[S.META + 3]);
+    st[9] = wd.ctr[0];   // last substep: candidate pairs, rows, units (per-step traffic accounting)
+    st[10] = nrows;
+    st[11] = nu;
     wd.ctr[0] = 0; wd.ctr[1] = 0;
     wd.ctr[2] += 1;
   }
@@ -1424,21 +1427,26 @@ __global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ world
 // ------------------------------------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------------------------------------
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, hipStream_t stream, uint32_t stages) {
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, hipStream_t stream, uint32_t stages,
+                                hipEvent_t* marks) {
   if (nworlds == 0) return hipSuccess;
   dim3 grid(nworlds);
+  if (marks) (void)hipEventRecord(marks[0], stream);
   if (stages & STAGE_COLLIDE) {
     hipLaunchKernelGGL((k_collide<256>), grid, dim3(256), 0, stream, worlds, nworlds, (stages & STAGE_NARROW) ? 1u : 0u);
   }
+  if (marks) (void)hipEventRecord(marks[1], stream);
   if (stages & STAGE_PREPARE) {
     if (block == 64) hipLaunchKernelGGL((k_prepare<64>), grid, dim3(64), 0, stream, worlds, nworlds);
     else if (block == 128) hipLaunchKernelGGL((k_prepare<128>), grid, dim3(128), 0, stream, worlds, nworlds);
     else hipLaunchKernelGGL((k_prepare<256>), grid, dim3(256), 0, stream, worlds, nworlds);
   }
+  if (marks) (void)hipEventRecord(marks[2], stream);
   if (stages & STAGE_SOLVE) {
     if (block == 64) hipLaunchKernelGGL((k_solve<64>), grid, dim3(64), 0, stream, worlds, nworlds);
     else if (block == 128) hipLaunchKernelGGL((k_solve<128>), grid, dim3(128), 0, stream, worlds, nworlds);
     else hipLaunchKernelGGL((k_solve<256>), grid, dim3(256), 0, stream, worlds, nworlds);
   }
+  if (marks) (void)hipEventRecord(marks[3], stream);
   return hipGetLastError();
 }

@@ -11,4 +11,6 @@ enum : uint32_t {
   STAGE_ALL = 15u,
 };
 
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, hipStream_t stream, uint32_t stages);
+// marks (optional): four events recorded before collide, prepare, solve and after solve.
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, hipStream_t stream, uint32_t stages,
+                                hipEvent_t* marks = nullptr);

@@ -110,8 +110,30 @@ def capacities(NS, NB, NA, policy="reference"):
     return maxp, maxc, maxu, uh, ah, maxdense
 
 
-def pack_scene(scene, opts=None, capacity_policy="reference"):
-    """Pack a LoadedScene into a PackedWorld (sim.js:64-75 options, :95-611 build)."""
+def sched_layout(MAXC, MAXU, UH, NCOL, NB, NCOLW, NL):
+    """Word offsets of the kernels' schedule tables (mirror of sched_layout in csrc/steel_dev.h)."""
+    names = [("HKEY", UH), ("HCNT", UH), ("CSLOT", MAXC), ("CLOCAL", MAXC), ("UKEY", MAXU), ("USIZE", MAXU), ("USTART", MAXU + 1),
+             ("CSTART", NCOL + 2), ("META", 8), ("CORDER", MAXC), ("RK0", MAXC), ("RK1", MAXC), ("RK2", MAXC), ("KEPT", MAXC),
+             ("OSTART", MAXU), ("LEAD", MAXC), ("SLOT", MAXC * 26), ("ROWOF", MAXC), ("RUNIT", MAXC), ("PCOL", MAXC), ("PORD", MAXC),
+             ("UTC", MAXU), ("UTP", MAXU), ("RCON", MAXC), ("UWOFF", UH), ("WALK", MAXC), ("HEADS", MAXC + 1), ("DSLOT", MAXC + 1),
+             ("WCOL", MAXC), ("WMASK", max(NB, 1) * NCOLW), ("WCOUNT", NCOL + 2), ("LINKHIT", max(NB, 1)), ("JSPLIT", max(NL, 1)),
+             ("SLOTT", MAXC * 4), ("NPAIRL", 4)]
+    out, o = {}, 0
+    for k, n in names:
+        out[k] = o
+        o += n
+    out["TOTAL"] = o
+    return out
+
+
+def world_sched_layout(pw):
+    return sched_layout(pw.h("W_MAXC"), pw.h("W_MAXU"), pw.h("W_UH"), pw.h("W_NCOLW") * 32, pw.NB, pw.h("W_NCOLW"), pw.NL)
+
+
+def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=None):
+    """Pack a LoadedScene into a PackedWorld (sim.js:64-75 options, :95-611 build).
+
+    capacity_override: {"MAXP"|"MAXC"|"MAXU"|"UH"|"AH"|"MAXDENSE": n} to force small capacities (overflow tests)."""
     S = scene.json
     f32, u32 = scene.f32, scene.u32
     opts = dict(opts or {})
@@ -479,6 +501,11 @@ def pack_scene(scene, opts=None, capacity_policy="reference"):
     ROWS = FMO
     P = dict(P_BODY=0, P_SHAPE=NB * 2, P_AABB=NB * 2 + NS * 2, P_PREV=NB * 2 + NS * 4)
     maxp, maxc, maxu, uh, ah, maxdense = capacities(NS, NB, NA, capacity_policy)
+    if capacity_override:
+        co = capacity_override
+        maxp, maxc = co.get("MAXP", maxp), co.get("MAXC", maxc)
+        maxu, uh = min(co.get("MAXU", maxu), maxc), min(co.get("UH", uh), maxc)
+        ah, maxdense = co.get("AH", ah), co.get("MAXDENSE", maxdense)
     NQ = len(q0)
     NDA = max([1] + [a["dofCount"] for a in arts])
     DHDR = 25

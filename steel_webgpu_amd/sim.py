@@ -25,7 +25,7 @@ STAGE_COLLIDE, STAGE_NARROW, STAGE_PREPARE, STAGE_SOLVE, STAGE_ALL = 1, 2, 4, 8,
 EXPORTS = ["steel_last_error", "steel_device_open", "steel_device_close", "steel_device_info", "steel_batch_create",
            "steel_batch_destroy", "steel_batch_reset", "steel_batch_step", "steel_batch_substep_stages", "steel_batch_sync",
            "steel_batch_read", "steel_batch_write", "steel_batch_buffer_size", "steel_batch_stats", "steel_batch_configure",
-           "steel_batch_device_bytes", "steel_estimate_bytes"]
+           "steel_batch_device_bytes", "steel_estimate_bytes", "steel_batch_step_timed", "steel_batch_profile_kernels"]
 
 
 class UnsupportedError(RuntimeError):
@@ -71,6 +71,8 @@ def load_library(path=LIB_PATH):
         "steel_batch_configure": ([V, U32, ctypes.c_int], ctypes.c_int),
         "steel_batch_device_bytes": ([V, ctypes.POINTER(SZ)], ctypes.c_int),
         "steel_estimate_bytes": ([U32, ctypes.POINTER(WorldDesc), ctypes.POINTER(SZ)], ctypes.c_int),
+        "steel_batch_step_timed": ([V, U32, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "steel_batch_profile_kernels": ([V, U32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -156,6 +158,18 @@ class Batch:
 
     def step(self, frames=1):
         _check(self.lib.steel_batch_step(self.handle, frames))
+
+    def step_timed(self, frames):
+        """Step `frames` frames; per-frame GPU milliseconds from HIP events."""
+        out = np.zeros(frames, np.float32)
+        _check(self.lib.steel_batch_step_timed(self.handle, frames, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def profile_kernels(self, substeps):
+        """Average GPU milliseconds per launch of (collide, prepare, solve), HIP events on the batch stream."""
+        out = (ctypes.c_double * 3)()
+        _check(self.lib.steel_batch_profile_kernels(self.handle, substeps, out))
+        return list(out)
 
     def substep_stages(self, stages):
         _check(self.lib.steel_batch_substep_stages(self.handle, stages))

@@ -1,0 +1,138 @@
+"""CPU-side checks of the boundary: the C ABI library loads and exports every declared symbol, the host
+packing follows the reference's rules, unsupported options raise."""
+import re
+import os
+
+import numpy as np
+import pytest
+
+from steel_webgpu_amd import SceneBuilder, pack, workloads, scene as scene_mod
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    from steel_webgpu_amd import sim
+    lib = sim.load_library()
+    hdr = open(os.path.join(ROOT, "include", "steel_amd.h")).read()
+    declared = set(re.findall(r"^(?:int|const char\*)\s+(steel_\w+)\(", hdr, re.M))
+    assert declared and declared == set(sim.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_device_open_without_gpu_is_unsupported():
+    from steel_webgpu_amd import sim
+    import ctypes
+    lib = sim.load_library()
+    h = ctypes.c_void_p()
+    rc = lib.steel_device_open(0, ctypes.byref(h))
+    if rc == 0:
+        pytest.skip("a GPU is present")
+    assert rc == sim.STEEL_UNSUPPORTED
+    assert b"device" in lib.steel_last_error()
+
+
+def test_header_fields_match_c_enum():
+    """The packer reads the header field order from include/steel_world.h; check it against an independent parse."""
+    src = open(os.path.join(ROOT, "include", "steel_world.h")).read()
+    body = src[src.index("enum steel_world_field"):]
+    body = body[body.index("{") + 1:body.index("}")]
+    names = [t.split("=")[0].strip() for t in re.sub(r"/\*.*?\*/|//[^\n]*", "", body, flags=re.S).split(",") if t.strip()]
+    assert names[-1] == "W_FIELD_COUNT" and len(names) <= pack.HEADER_WORDS
+    for i, n in enumerate(names[:-1]):
+        assert pack.FIELDS[n] == i, n
+    assert len(pack.FIELDS) == len(names) - 1
+
+
+def test_sched_layout_mirror_matches_device_header():
+    src = open(os.path.join(ROOT, "steel_webgpu_amd", "csrc", "steel_dev.h")).read()
+    steps = re.findall(r"s\.(\w+) = o; o \+= ([^;]+);", src)
+    L = pack.sched_layout(1000, 300, 256, 64, 40, 2, 12)
+    env = dict(MAXC=1000, MAXU=300, UH=256, NCOL=64, NB=40, NCOLW=2, NL=12)
+    o = 0
+    for name, expr in steps:
+        assert L[name] == o, name
+        o += eval(expr.replace("(NB > 0 ? NB : 1)", "max(NB,1)").replace("NB > 0 ? NB : 1", "max(NB,1)")
+                  .replace("NL > 0 ? NL : 1", "max(NL,1)"), {"max": max}, env)
+    assert L["TOTAL"] == o
+
+
+def test_filter_bitset_holds_joint_parent_child_and_explicit_filters():
+    b = SceneBuilder()
+    b.addGroundPlane()
+    p = b.addBody(xform=[0, 0, 1, 0, 0, 0, 1])
+    sp = b.addShapeBox(p, hx=0.1, hy=0.1, hz=0.1)
+    c = b.addBody(xform=[0, 0, 1.3, 0, 0, 0, 1])
+    sc = b.addShapeBox(c, hx=0.1, hy=0.1, hz=0.1)
+    d = b.addBody(xform=[1, 0, 1, 0, 0, 0, 1])
+    sd = b.addShapeSphere(d, radius=0.1)
+    b.addJointRevolute(p, c, axis=[0, 1, 0])
+    b.addCollisionFilter(sp, sd)
+    pw = pack.pack_scene(b.finalize())
+    NS = pw.NS
+    bits = pw.statU[pw.h("W_SU_FILTER"):]
+
+    def filtered(i, j):
+        k = i * NS + j
+        return bool(bits[k >> 5] & (1 << (k & 31)))
+
+    assert filtered(sp, sc) and filtered(sc, sp)
+    assert filtered(sp, sd) and filtered(sd, sp)
+    assert not filtered(sc, sd) and not filtered(0, sp)
+
+
+def test_capacities_are_exact_bounds():
+    maxp, maxc, maxu, uh, ah, md = pack.capacities(65, 64, 0)
+    assert maxp == 65 * 64 // 2 and maxc == min(8192, 5 * maxp)
+    assert pack.capacities(300, 300, 1)[0] == 16384  # the reference's MAXP
+    c = pack.capacities(25, 24, 1, "compact")
+    assert c[1] == 400 and c[3] <= c[1]
+
+
+def test_unsupported_options_raise():
+    with pytest.raises(pack.UnsupportedError):
+        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"enable_sleeping": True}})
+    with pytest.raises(ValueError):
+        pack.pack_scene(workloads.boxes64(n=4), opts={"substeps": 64})
+    b = SceneBuilder()
+    bd = b.addBody()
+    b.addShapeCylinder(bd)
+    with pytest.raises(pack.UnsupportedError):
+        pack.pack_scene(b.finalize())
+
+
+def test_convex_hull_faces_are_outward_ccw_planes():
+    rng = np.random.default_rng(3)
+    pts = rng.normal(size=(14, 3))
+    pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+    h = scene_mod.convexHull(pts.tolist())
+    V = np.array(h["verts"])
+    for f in h["faces"]:
+        n, d = np.array(f["n"]), f["d"]
+        assert abs(np.linalg.norm(n) - 1) < 1e-9
+        assert np.all(V @ n - d <= 1e-9)  # all vertices behind the plane
+        ring = V[f["v"]]
+        np.testing.assert_allclose(ring @ n, d, atol=1e-9)
+        area = sum(np.cross(ring[i] - ring[0], ring[i + 1] - ring[0]) for i in range(1, len(ring) - 1))
+        assert area @ n > 0  # counter-clockwise about the outward normal
+
+
+def test_scene_folder_round_trip(tmp_path):
+    sc = workloads.boxes64(n=5)
+    sc.save(str(tmp_path))
+    back = scene_mod.loadScene(str(tmp_path))
+    a, b = pack.pack_scene(sc), pack.pack_scene(back)
+    assert np.array_equal(a.statF, b.statF) and np.array_equal(a.statU, b.statU) and np.array_equal(a.header, b.header)
+
+
+def test_box_mass_properties_closed_form():
+    mp = scene_mod.massProps(scene_mod.GT["BOX"], [0.1, 0.2, 0.3], None)
+    m = 8 * 0.1 * 0.2 * 0.3
+    assert mp["m"] == pytest.approx(m)
+    assert mp["I"][0] == pytest.approx(m / 3 * (0.04 + 0.09))
+    h = scene_mod.convexHull(scene_mod.shapePoints(scene_mod.GT["BOX"], [0.1, 0.2, 0.3]))
+    mh = scene_mod.massProps(scene_mod.GT["CONVEX_MESH"], None, h)
+    assert mh["m"] == pytest.approx(m)
+    np.testing.assert_allclose(mh["I"], mp["I"], atol=1e-12)
+    assert len(h["faces"]) == 6 and len(h["verts"]) == 8
