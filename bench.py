@@ -17,6 +17,7 @@ import argparse
 import json
 import multiprocessing as mp
 import os
+import pickle
 import sys
 import time
 
@@ -36,12 +37,35 @@ def _pack_world(i):
     return pw
 
 
-def build_worlds(n, workers):
+def build_worlds(n, workers, cache=None):
+    """Pack worlds 0..n-1 (worker processes); with `cache`, load them from / save them to that file."""
+    if cache and os.path.exists(cache):
+        with open(cache, "rb") as f:
+            worlds = pickle.load(f)  # written by this function
+        if len(worlds) >= n:
+            return worlds[:n]
     if workers <= 1:
-        return [_pack_world(i) for i in range(n)]
-    ctx = mp.get_context("fork")
-    with ctx.Pool(workers) as pool:
-        return pool.map(_pack_world, range(n), chunksize=max(1, n // (workers * 8)))
+        worlds = [_pack_world(i) for i in range(n)]
+    else:
+        ctx = mp.get_context("fork")
+        with ctx.Pool(workers) as pool:
+            worlds = pool.map(_pack_world, range(n), chunksize=max(1, n // (workers * 8)))
+    if cache:
+        with open(cache, "wb") as f:
+            pickle.dump(worlds, f, protocol=pickle.HIGHEST_PROTOCOL)
+    return worlds
+
+
+def copy_calibration(nbytes=1 << 30, reps=4):
+    """A plain device copy of known size (k_calib_copy, libsteel_calib.so), for scaling the profiler's
+    FETCH_SIZE / WRITE_SIZE counters: each launch reads and writes `nbytes`."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "steel_webgpu_amd", "libsteel_calib.so"))
+    lib.steel_calib_copy.argtypes = [ctypes.c_size_t, ctypes.c_int]
+    rc = lib.steel_calib_copy(nbytes, reps)
+    if rc != 0:
+        raise RuntimeError(f"calibration copy failed ({rc})")
+    return nbytes
 
 
 def hull_bytes(pw):
@@ -123,11 +147,16 @@ def cpu_baseline(nworlds, frames, procs):
 
 
 def traffic_from_profile(kernel):
-    path = os.path.join(ROOT, "profiles", f"{kernel}_quadruped4096_traffic.json")
+    """HBM bytes per launch of k_<kernel> on this workload, from the committed counter summary (its
+    `bytes_per_launch:` line: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", f"k_{kernel}_quadruped4096_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        for line in f:
+            if line.startswith("bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def main():
@@ -142,7 +171,16 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="prepare/solve workgroup size (0: autotune)")
+    ap.add_argument("--worlds-cache", default=None, help="pickle file to load packed worlds from / save them to")
+    ap.add_argument("--pack-only", action="store_true", help="pack the worlds into --worlds-cache and exit")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiler run: batched leg only (no autotune, plain or CPU legs), then a 1 GiB copy for counter scaling")
+    ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
+    if args.profile:
+        args.no_cpu = True
+        args.plain_worlds = 0
+        args.block = args.block or 64
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (worlds shard whole across GPUs with no "
               f"per-step exchange, so N GPUs would run N independent batches).", file=sys.stderr)
@@ -151,7 +189,9 @@ def main():
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     # Host-side work that uses worker processes runs before anything touches the GPU.
     t0 = time.time()
-    worlds = build_worlds(args.worlds, procs)
+    worlds = build_worlds(args.worlds, procs, args.worlds_cache)
+    if args.pack_only:
+        return
     log(f"built {len(worlds)} worlds in {time.time() - t0:.1f}s; bodies/world mean {np.mean([w.NB for w in worlds]):.1f} "
         f"max {max(w.NB for w in worlds)}")
     cpu = None
@@ -164,14 +204,15 @@ def main():
     from steel_webgpu_amd import sim
     dev = sim.Device(0)
     log(f"device {dev.name}, {dev.num_cus} CUs, {dev.total_mem / 2**30:.0f} GiB")
-    batch = sim.Batch(worlds, dev, block=256, graph=True)
+    graph = not args.no_graph
+    batch = sim.Batch(worlds, dev, block=256, graph=graph)
     log(f"batch resident: {batch.device_bytes() / 2**30:.2f} GiB")
     # Autotuner: prepare/solve workgroup size by measured frame time.
     block = args.block
     if block == 0:
         best = None
         for bs in (64, 128, 256):
-            batch.configure(bs, True)
+            batch.configure(bs, graph)
             batch.reset()
             batch.step(3)
             ms = float(np.median(batch.step_timed(4)))
@@ -179,7 +220,7 @@ def main():
             if best is None or ms < best[1]:
                 best = (bs, ms)
         block = best[0]
-    batch.configure(block, True)
+    batch.configure(block, graph)
     batch.reset()
     batch.step(args.warmup)
     batch.sync()
@@ -191,9 +232,9 @@ def main():
     value = nw * args.steps / wall
     ms_per_step = wall / args.steps * 1e3
     log(f"batched: {value:.0f} world-steps/s, {ms_per_step:.3f} ms/frame (block {block})")
-    # Roofline of the dominant kernel.
+    # Roofline of the dominant kernel: the timed region's frame count again, each kernel bracketed by HIP events.
     S = worlds[0].h("W_SUBSTEPS")
-    kms = batch.profile_kernels(2 * S)
+    kms = batch.profile_kernels(args.steps * S)
     algo = algorithmic_bytes(batch, worlds)
     names = ["collide", "prepare", "solve"]
     k = int(np.argmax(kms))
@@ -215,6 +256,10 @@ def main():
         plain_rates.append(args.plain_steps / dtp)
         plain_ms.extend(fm.tolist())
         b1.destroy()
+    if args.profile:
+        copy_calibration()
+        log("profile run done")
+        return
     plain = float(np.mean(plain_rates))
     log(f"plain per-world: {plain:.1f} world-steps/s ({np.median(plain_ms):.3f} ms/frame GPU)")
     out = {
@@ -222,7 +267,7 @@ def main():
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded PCG32 scenes)",
         "config": {"workload": "quadruped_batch", "worlds": nw, "fps": 200, "substeps": S, "iterations": worlds[0].h("W_ITERS"),
-                   "block": block, "graph": True},
+                   "block": block, "graph": graph},
         "plain_world_steps_per_s": round(plain, 2),
         "speedup_vs_plain": round(value / plain, 1),
         "latency_ms": {"p50": round(float(np.percentile(frame_ms, 50)), 4), "p99": round(float(np.percentile(frame_ms, 99)), 4),
