@@ -332,6 +332,10 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     bodyPose = np.zeros(NB * 8, np.float32)
     pw = PackedWorld()
     pw.bodyKind = list(bodyKind)
+    # Bodies the mouse spring may pull (sim.js:176-178); drives are refused above, so no articulation is driven.
+    kinem = (S["bodies"].get("kinematic") or [0] * NB)
+    pw.draggable = [S.get("mouse_drag") is not False and not kinem[b] and
+                    (k == 1 or (k == 2 and links[bodyLink[b]]["movable"])) for b, k in enumerate(bodyKind)]
     for b in range(NB):
         o4 = (SF["SF_BODY"] + b * 8) * 4
         m = S["bodies"]["mass"][b]

@@ -234,6 +234,8 @@ class Sim:
         self.initial = {"bodyPose": pw.poses0[:pw.NB * 8].copy()}
         self.hasTimestamps = True
         self.pick = None
+        self.userTargets = False
+        self.draggable = list(pw.draggable)
         self.batch = Batch([pw], self.device)
         self.time = 0.0
         self.frame = 0
@@ -299,6 +301,24 @@ class Sim:
             jointQ[J["q_start"][j]:J["q_start"][j] + 7] = bodyQ[b * 7:b * 7 + 7]
             jointQd[J["qd_start"][j]:J["qd_start"][j] + 6] = bodyQd[b * 6:b * 6 + 6]
         return dict(bodyQ=bodyQ, bodyQd=bodyQd, jointQ=jointQ, jointQd=jointQd)
+
+    def setJointTarget(self, newtonDof, pos, vel=0.0):
+        """Joint position/velocity target of one Newton dof (sim.js:716-723): art[A_TGT + d], art[A_TGTV + d]."""
+        d = self.packed.dofMap.get(newtonDof)
+        if d is None:
+            raise ValueError(f"dof {newtonDof} is not simulated as an articulation dof")
+        A = self.packed.A
+        self.userTargets = True
+        self.batch.write(0, "art", np.asarray([pos], np.float32), (A["A_TGT"] + d) * 4)
+        self.batch.write(0, "art", np.asarray([vel], np.float32), (A["A_TGTV"] + d) * 4)
+
+    def setJointTargets(self, pos, vel=None, first=0):
+        """Targets for consecutive Newton dofs from `first` (sim.js:726-728)."""
+        for k in range(len(pos)):
+            self.setJointTarget(first + k, pos[k], vel[k] if vel is not None else 0.0)
+
+    def readPosesPooled(self):
+        return self.poses()
 
     def setJointForce(self, newtonDof, value):
         d = self.packed.dofMap.get(newtonDof)

@@ -155,8 +155,7 @@ def arm_hulls(seed=SEED + 1, nlinks=32, nhulls=256, fps=240, substeps=8, iterati
     placed = [([0.0, 0.0, 0.5 * z], 0.25)]
     for _ in range(nhulls):
         r = rng.uniform(0.04, 0.12)
-        npts = 8 + int(rng.uniform(0, 9)) if True else 8
-        npts = min(npts, 16)
+        npts = min(8 + int(rng.uniform(0, 9)), 16)
         pts = []
         for _ in range(npts):
             v = rng.unit_vec()
