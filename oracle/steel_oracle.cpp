@@ -2129,11 +2129,24 @@ template <class T> struct World {
       for (uint32_t r = 0; r < JM; r++) art[L + L_TAU + r] = 0;
     }
   }
-  bool art_dense(uint32_t a, uint32_t mode, uint32_t it) {  // wgsl.js:3306-3364, serial (lane 0) form
+  // subgroupAdd of the reference's 32-lane subgroup path (wgsl.js:3305); WGSL leaves its order open, this
+  // build fixes it as an xor butterfly over strides 16, 8, 4, 2, 1 (lane i adds lane i^s's value).
+  static T lane_sum32(T (&v)[32]) {
+    for (uint32_t s = 16; s >= 1; s >>= 1) {
+      T nv[32];
+      for (uint32_t i = 0; i < 32; i++) nv[i] = v[i] + v[i ^ s];
+      for (uint32_t i = 0; i < 32; i++) v[i] = nv[i];
+    }
+    return v[0];
+  }
+  // Dense same-articulation rows, subgroup form (wgsl.js:3306-3364, SG = 1): lane l owns dofs l, l+32, ...
+  // and sums its own products in dof order before lane_sum32.
+  bool art_dense(uint32_t a, uint32_t mode, uint32_t it) {
     (void)it;
     uint32_t nd = std::min(S_META[3], MAXDENSE);
     uint32_t ai[4];
     art_info(a, ai);
+    const uint32_t n = ai[3];
     bool touched = false;
     for (uint32_t i = 0; i < nd; i++) {
       uint32_t o = A_DENSE + i * DPS;
@@ -2142,26 +2155,33 @@ template <class T> struct World {
       uint32_t jo = o + DHDR, yo = o + DHDR + 3 * NDA;
       V3<T> lam{art[o + 9], art[o + 10], art[o + 11]};
       if (mode == 0) {
-        for (uint32_t k = 0; k < ai[3]; k++) aqd[ai[2] + k] += art[yo + k] * lam.x + art[yo + NDA + k] * lam.y + art[yo + 2 * NDA + k] * lam.z;
+        for (uint32_t k = 0; k < n; k++) aqd[ai[2] + k] += art[yo + k] * lam.x + art[yo + NDA + k] * lam.y + art[yo + 2 * NDA + k] * lam.z;
         continue;
       }
-      T part = 0;
-      for (uint32_t k = 0; k < ai[3]; k++) part += art[jo + k] * aqd[ai[2] + k];
-      T jv = part;
+      T part[32];
+      for (uint32_t l = 0; l < 32; l++) {
+        part[l] = T(0);
+        for (uint32_t k = l; k < n; k += 32) part[l] += art[jo + k] * aqd[ai[2] + k];
+      }
+      T jv = lane_sum32(part);
       T w = art[o + 24];
       T nl = mx(lam.x + F_OMEGA * (-(jv + art[o + 4]) * art[o + 5] * w - (T(1) - w) * lam.x), T(0));
       T dn = nl - lam.x;
-      T p1 = 0, p2 = 0;
-      for (uint32_t k = 0; k < ai[3]; k++) {
-        aqd[ai[2] + k] += art[yo + k] * dn;
-        p1 += art[jo + NDA + k] * aqd[ai[2] + k];
-        p2 += art[jo + 2 * NDA + k] * aqd[ai[2] + k];
+      T p1[32], p2[32];
+      for (uint32_t l = 0; l < 32; l++) {
+        p1[l] = T(0); p2[l] = T(0);
+        for (uint32_t k = l; k < n; k += 32) {
+          aqd[ai[2] + k] += art[yo + k] * dn;
+          p1[l] += art[jo + NDA + k] * aqd[ai[2] + k];
+          p2[l] += art[jo + 2 * NDA + k] * aqd[ai[2] + k];
+        }
       }
+      T v1 = lane_sum32(p1), v2 = lane_sum32(p2);
       T lim = art[o + 3] * nl;
       T kd = art[o + 6] * art[o + 8] - art[o + 7] * art[o + 7];
-      V2<T> lt = friction_pair_solve(art[o + 8] / kd, -art[o + 7] / kd, art[o + 6] / kd, p1, p2, lam.y, lam.z, lim, F_OMEGA);
+      V2<T> lt = friction_pair_solve(art[o + 8] / kd, -art[o + 7] / kd, art[o + 6] / kd, v1, v2, lam.y, lam.z, lim, F_OMEGA);
       T d1 = lt.x - lam.y, d2 = lt.y - lam.z;
-      for (uint32_t k = 0; k < ai[3]; k++) aqd[ai[2] + k] += art[yo + NDA + k] * d1 + art[yo + 2 * NDA + k] * d2;
+      for (uint32_t k = 0; k < n; k++) aqd[ai[2] + k] += art[yo + NDA + k] * d1 + art[yo + 2 * NDA + k] * d2;
       art[o + 9] = nl; art[o + 10] = lt.x; art[o + 11] = lt.y;
     }
     return touched;

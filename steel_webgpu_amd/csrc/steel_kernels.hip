@@ -1082,41 +1082,72 @@ struct Solver {
       for (uint32_t r = 0; r < w.JM(); r++) art_[L + L_TAU + r] = 0.0f;
     }
   }
-  DEVN bool art_dense(uint32_t a, uint32_t mode, uint32_t nd) const {  // wgsl.js:3306-3364, serial lane
+  // subgroupAdd of the reference's 32-lane path (wgsl.js:3305), order fixed as an xor butterfly over strides
+  // 16, 8, 4, 2, 1 within each 32-lane half of the wave; every lane ends with the same bits (x + y == y + x).
+  static DEV float lane_sum32(float x) {
+    x = x + __shfl_xor(x, 16, 32);
+    x = x + __shfl_xor(x, 8, 32);
+    x = x + __shfl_xor(x, 4, 32);
+    x = x + __shfl_xor(x, 2, 32);
+    x = x + __shfl_xor(x, 1, 32);
+    return x;
+  }
+  // Dense same-articulation rows, one wave per articulation (wgsl.js:3306-3364, subgroup form): lane l < 32
+  // owns dofs l and l + 32 and keeps their joint velocities in registers for the whole pass; rows run in
+  // order (Gauss-Seidel).  Called by all 64 lanes of the wave with the same (a, mode, nd).
+  DEVN bool art_dense(uint32_t a, uint32_t mode, uint32_t nd, uint32_t lane) const {
     uint4 ai = w.art_info(a);
     float* art_ = w.d.art;
     float* aqd = w.d.aqd;
-    uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR), DPS = w.h(W_DPS), AD = w.h(W_A_DENSE);
-    float omega = w.hf(W_F_OMEGA);
+    const uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR), DPS = w.h(W_DPS), AD = w.h(W_A_DENSE);
+    const float omega = w.hf(W_F_OMEGA);
+    const uint32_t n = ai.w;
+    const bool own0 = lane < 32 && lane < n, own1 = lane < 32 && lane + 32 < n;
+    float q0 = own0 ? aqd[ai.z + lane] : 0.0f, q1 = own1 ? aqd[ai.z + lane + 32] : 0.0f;
     bool touched = false;
     for (uint32_t i = 0; i < nd; i++) {
-      uint32_t o = AD + i * DPS;
+      const uint32_t o = AD + i * DPS;
       if (tu(art_[o]) != a) continue;
       touched = true;
-      uint32_t jo = o + DHDR, yo = o + DHDR + 3 * NDA;
-      f3 lam = mk3(art_[o + 9], art_[o + 10], art_[o + 11]);
+      const uint32_t jo = o + DHDR, yo = o + DHDR + 3 * NDA;
+      const float lx = art_[o + 9], ly = art_[o + 10], lz = art_[o + 11];
+      float ja0 = 0, jb0 = 0, jc0 = 0, ya0 = 0, yb0 = 0, yc0 = 0, ja1 = 0, jb1 = 0, jc1 = 0, ya1 = 0, yb1 = 0, yc1 = 0;
+      if (own0) {
+        ja0 = art_[jo + lane]; jb0 = art_[jo + NDA + lane]; jc0 = art_[jo + 2 * NDA + lane];
+        ya0 = art_[yo + lane]; yb0 = art_[yo + NDA + lane]; yc0 = art_[yo + 2 * NDA + lane];
+      }
+      if (own1) {
+        ja1 = art_[jo + lane + 32]; jb1 = art_[jo + NDA + lane + 32]; jc1 = art_[jo + 2 * NDA + lane + 32];
+        ya1 = art_[yo + lane + 32]; yb1 = art_[yo + NDA + lane + 32]; yc1 = art_[yo + 2 * NDA + lane + 32];
+      }
       if (mode == 0) {
-        for (uint32_t k = 0; k < ai.w; k++) aqd[ai.z + k] += art_[yo + k] * lam.x + art_[yo + NDA + k] * lam.y + art_[yo + 2 * NDA + k] * lam.z;
+        if (own0) q0 += ya0 * lx + yb0 * ly + yc0 * lz;
+        if (own1) q1 += ya1 * lx + yb1 * ly + yc1 * lz;
         continue;
       }
+      const float rhs = art_[o + 4], meff = art_[o + 5], wgt = art_[o + 24], mu = art_[o + 3];
+      const float k6 = art_[o + 6], k7 = art_[o + 7], k8 = art_[o + 8];
       float part = 0.0f;
-      for (uint32_t k = 0; k < ai.w; k++) part += art_[jo + k] * aqd[ai.z + k];
-      float jv = part;
-      float wgt = art_[o + 24];
-      float nl = mx(lam.x + omega * (-(jv + art_[o + 4]) * art_[o + 5] * wgt - (1.0f - wgt) * lam.x), 0.0f);
-      float dn = nl - lam.x;
+      if (own0) part += ja0 * q0;
+      if (own1) part += ja1 * q1;
+      const float jv = lane_sum32(part);
+      const float nl = mx(lx + omega * (-(jv + rhs) * meff * wgt - (1.0f - wgt) * lx), 0.0f);
+      const float dn = nl - lx;
       float p1 = 0.0f, p2 = 0.0f;
-      for (uint32_t k = 0; k < ai.w; k++) {
-        aqd[ai.z + k] += art_[yo + k] * dn;
-        p1 += art_[jo + NDA + k] * aqd[ai.z + k];
-        p2 += art_[jo + 2 * NDA + k] * aqd[ai.z + k];
-      }
-      float lim = art_[o + 3] * nl;
-      float kd = art_[o + 6] * art_[o + 8] - art_[o + 7] * art_[o + 7];
-      f2 lt = friction_pair_solve(art_[o + 8] / kd, -art_[o + 7] / kd, art_[o + 6] / kd, p1, p2, lam.y, lam.z, lim, omega);
-      float d1 = lt.x - lam.y, d2 = lt.y - lam.z;
-      for (uint32_t k = 0; k < ai.w; k++) aqd[ai.z + k] += art_[yo + NDA + k] * d1 + art_[yo + 2 * NDA + k] * d2;
-      art_[o + 9] = nl; art_[o + 10] = lt.x; art_[o + 11] = lt.y;
+      if (own0) { q0 += ya0 * dn; p1 += jb0 * q0; p2 += jc0 * q0; }
+      if (own1) { q1 += ya1 * dn; p1 += jb1 * q1; p2 += jc1 * q1; }
+      const float v1 = lane_sum32(p1), v2 = lane_sum32(p2);
+      const float lim = mu * nl;
+      const float kd = k6 * k8 - k7 * k7;
+      f2 lt = friction_pair_solve(k8 / kd, -k7 / kd, k6 / kd, v1, v2, ly, lz, lim, omega);
+      const float d1 = lt.x - ly, d2 = lt.y - lz;
+      if (own0) q0 += yb0 * d1 + yc0 * d2;
+      if (own1) q1 += yb1 * d1 + yc1 * d2;
+      if (lane == 0) { art_[o + 9] = nl; art_[o + 10] = lt.x; art_[o + 11] = lt.y; }
+    }
+    if (touched) {
+      if (own0) aqd[ai.z + lane] = q0;
+      if (own1) aqd[ai.z + lane + 32] = q1;
     }
     return touched;
   }
@@ -1212,6 +1243,21 @@ struct Solver {
   }
 };
 
+// Dense rows of every articulation, one wave each in rounds, then the twists of the articulations they moved
+// (wgsl.js:3366-3377 dense_phase).
+template <int BS>
+DEV void dense_phase(const Solver& sv, uint32_t mode, uint32_t nd) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, NA = sv.w.NA();
+  for (uint32_t base = 0; base < NA; base += BS / 64) {
+    const uint32_t a = base + wave;
+    bool touched = false;
+    if (a < NA) touched = sv.art_dense(a, mode, nd, lane);
+    __syncthreads();
+    if (lane == 0 && a < NA && touched) sv.art_twists(a);
+    __syncthreads();
+  }
+}
+
 template <int BS>
 DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint32_t nd) {
   const W& w = sv.w;
@@ -1228,8 +1274,7 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
     }
   }
   __syncthreads();
-  for (uint32_t a = tid; a < NA; a += BS) if (sv.art_dense(a, 1, nd)) sv.art_twists(a);
-  __syncthreads();
+  dense_phase<BS>(sv, 1, nd);
   const uint4* slots = (const uint4*)(sched + S.SLOTT);
   for (uint32_t c = 0; c < NCOL && c < ncol; c++) {
     uint32_t s = sched[S.CSTART + c], e = sched[S.CSTART + c + 1];
@@ -1301,8 +1346,7 @@ __global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ world
   __syncthreads();
   for (uint32_t a = tid; a < NA; a += BS) sv.art_twists(a);
   __syncthreads();
-  for (uint32_t a = tid; a < NA; a += BS) if (sv.art_dense(a, 0, nd)) sv.art_twists(a);
-  __syncthreads();
+  dense_phase<BS>(sv, 0, nd);
   sweep<BS>(sv, 0, ncol, nu, nd);
   const uint32_t iters = w.h(W_ITERS);
   for (uint32_t it = 0; it < iters; it++) sweep<BS>(sv, 1, ncol, nu, nd);
