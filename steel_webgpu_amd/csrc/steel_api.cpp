@@ -46,6 +46,8 @@ int world_sizes(const uint32_t* h, Sizes& s) {
   if (h[W_NDA] > 64 || h[W_MAXLP] > 64) return fail(STEEL_BAD_ARGUMENT, "articulations of more than 64 dofs or links are not supported");
   if (h[W_MAXPATH] > 64 || h[W_MAXPATH] * h[W_JM] > 192) return fail(STEEL_BAD_ARGUMENT, "articulation root paths deeper than 64 links are not supported");
   if (h[W_UH] > MAXC || h[W_MAXU] > MAXC) return fail(STEEL_BAD_ARGUMENT, "unit capacities exceed the contact capacity");
+  if (steel_solve_lds_bytes(h[W_NL], ND) > STEEL_SOLVE_LDS_MAX)
+    return fail(STEEL_UNSUPPORTED, "a world of %u articulation links exceeds the solve kernel's LDS link mirror", h[W_NL]);
   SchedLayout L = sched_layout(MAXC, h[W_MAXU], h[W_UH], h[W_NCOLW] * 32, NB, h[W_NCOLW], h[W_NL]);
   s.b[K_HDR] = STEEL_WORLD_HEADER_WORDS * 4;
   s.b[K_STATF] = (size_t)h[W_NSTATF] * 16;
@@ -84,6 +86,7 @@ struct steel_batch {
   uint32_t n;
   uint32_t substeps;
   uint32_t block = 256;
+  size_t solve_lds = 0;
   bool use_graph = true;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -203,6 +206,7 @@ int steel_batch_create(steel_device* dev, uint32_t nworlds, const steel_world_de
     if (rc) { delete b; return rc; }
     for (int k = 0; k < K_COUNT; k++) { b->off[k][i] = b->total[k]; b->size[k][i] = s.b[k]; b->total[k] += align256(s.b[k]); }
     std::memcpy(&b->headers[(size_t)i * STEEL_WORLD_HEADER_WORDS], w.header, STEEL_WORLD_HEADER_WORDS * 4);
+    b->solve_lds = std::max(b->solve_lds, steel_solve_lds_bytes(w.header[W_NL], w.header[W_ND]));
   }
   for (int k = 0; k < K_COUNT; k++) {
     if (hipMalloc(&b->arena[k], std::max<size_t>(b->total[k], 256)) != hipSuccess) {
@@ -292,7 +296,7 @@ int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph) {
 }
 
 static int launch_frame(steel_batch* b) {
-  for (uint32_t s = 0; s < b->substeps; s++) HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->stream, STAGE_ALL));
+  for (uint32_t s = 0; s < b->substeps; s++) HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->stream, STAGE_ALL));
   return STEEL_OK;
 }
 
@@ -323,7 +327,7 @@ int steel_batch_step(steel_batch* b, uint32_t frames) {
 int steel_batch_substep_stages(steel_batch* b, uint32_t stages) {
   if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
   HIPCHK(hipSetDevice(b->dev->ordinal));
-  HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->stream, stages));
+  HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->stream, stages));
   return STEEL_OK;
 }
 
@@ -353,7 +357,7 @@ int steel_batch_profile_kernels(steel_batch* b, uint32_t substeps, double kernel
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   int rc = STEEL_OK;
   for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)
-    if (steel_launch_substep(b->d_worlds, b->n, b->block, b->stream, STAGE_ALL, &ev[4 * s]) != hipSuccess) rc = fail(STEEL_ERROR, "launch failed");
+    if (steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->stream, STAGE_ALL, &ev[4 * s]) != hipSuccess) rc = fail(STEEL_ERROR, "launch failed");
   if (rc == STEEL_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = fail(STEEL_ERROR, "sync failed");
   for (int k = 0; k < 3; k++) kernel_ms[k] = 0.0;
   for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)

@@ -165,8 +165,20 @@ DEV bool pair_kept(const W& w, uint32_t i, uint32_t j) {
   return true;
 }
 
+// Waves per SIMD each kernel is compiled for (a VGPR budget: 512 / waves).  Every phase of a world is a
+// latency-bound chain, so resident waves, not registers per lane, set the batch's throughput.
+#ifndef STEEL_WPE_COLLIDE
+#define STEEL_WPE_COLLIDE 4
+#endif
+#ifndef STEEL_WPE_PREPARE
+#define STEEL_WPE_PREPARE 4
+#endif
+#ifndef STEEL_WPE_SOLVE
+#define STEEL_WPE_SOLVE 4
+#endif
+
 template <int BS>
-__global__ void __launch_bounds__(BS) k_collide(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t stage_mask) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_collide(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t stage_mask) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ uint32_t s_hist[256];
   __shared__ uint32_t s_cnt[(BS / 64) * 256];
@@ -615,7 +627,7 @@ struct Prep {
 };
 
 template <int BS>
-__global__ void __launch_bounds__(BS) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ uint32_t s_w[BS / 64];
   uint32_t wid = blockIdx.x;
@@ -875,9 +887,51 @@ __global__ void __launch_bounds__(BS) k_prepare(const WorldDev* __restrict__ wor
 // ------------------------------------------------------------------------------------------------
 // k_solve (wgsl.js:2501-3891)
 // ------------------------------------------------------------------------------------------------
+// Per-link mirror of the fields the solve sweeps touch, in LDS for the whole kernel (SOLVE_LB floats per link):
+// S, U, D^-1 (36 each, the link block's layout), p, dv, v, u, tau (6 each), COM (3).  Loaded from the link
+// blocks (art[A_LINK + l*LB]) at kernel start; p, dv, v, u, tau are written back after the sweeps.
+static constexpr uint32_t SOLVE_LB = 144, M_S = 0, M_U = 36, M_DI = 72, M_P = 108, M_DV = 114, M_V = 120, M_UU = 126,
+                          M_TAU = 132, M_COM = 138;
+__host__ __device__ inline uint32_t mirror_src(uint32_t f) {  // link-block offset of mirror field f (LB = none)
+  if (f < 36) return L_S + f;
+  if (f < 72) return L_U + f - 36;
+  if (f < 108) return L_DI + f - 72;
+  if (f < 114) return L_P + f - 108;
+  if (f < 120) return L_DV + f - 114;
+  if (f < 126) return L_V + f - 120;
+  if (f < 132) return L_UU + f - 126;
+  if (f < 138) return L_TAU + f - 132;
+  if (f < 141) return L_COM + f - 138;
+  return LB;
+}
+// Dynamic LDS of k_solve: NL * SOLVE_LB floats, NL uint4 link records (parent, ndof, first dof, body), ND floats
+// of solver joint velocities.
+__host__ __device__ inline size_t solve_lds_bytes(uint32_t NL, uint32_t ND) {
+  return (size_t)NL * SOLVE_LB * 4 + (size_t)NL * 16 + (size_t)ND * 4;
+}
+
 struct Solver {
   W w;
   uint32_t parity;
+  float* lm;      // LDS link mirror
+  uint4* lmeta;   // LDS link records
+  float* laqd;    // LDS joint velocities (the reference's aqd)
+  DEV float* ml(uint32_t l) const { return lm + l * SOLVE_LB; }
+  DEV int mpar(uint32_t l) const { return (int)lmeta[l].x; }
+  DEV uint32_t mnd(uint32_t l) const { return lmeta[l].y; }
+  DEV uint32_t mdof(uint32_t l) const { return lmeta[l].z; }
+  DEV uint32_t mbody(uint32_t l) const { return lmeta[l].w; }
+  static DEV SVd msv(const float* p) { return mksv(mk3(p[0], p[1], p[2]), mk3(p[3], p[4], p[5])); }
+  static DEV void mst(float* p, SVd v) { p[0] = v.a.x; p[1] = v.a.y; p[2] = v.a.z; p[3] = v.l.x; p[4] = v.l.y; p[5] = v.l.z; }
+  DEV SVd mS(uint32_t l, uint32_t k) const { return msv(ml(l) + M_S + k * 6); }
+  DEV SVd mU(uint32_t l, uint32_t k) const { return msv(ml(l) + M_U + k * 6); }
+  DEV float mDI(uint32_t l, uint32_t r, uint32_t c) const { return ml(l)[M_DI + r * w.JM() + c]; }
+  DEV f3 mcom(uint32_t l) const { const float* p = ml(l) + M_COM; return mk3(p[0], p[1], p[2]); }
+  DEV f3 mr(uint32_t l) const {  // link_r from the mirror: COM(l) - COM(parent), zero at a root
+    int par = mpar(l);
+    if (par < 0) return mk3(0, 0, 0);
+    return mcom(l) - mcom((uint32_t)par);
+  }
   DEV float4 row(uint32_t r, uint32_t k) const { return w.d.rows[r * w.ROWS() + k]; }
   DEV void load_tw(int b, f3 out[2]) const {
     if (b < 0) { out[0] = mk3(0, 0, 0); out[1] = mk3(0, 0, 0); return; }
@@ -1026,60 +1080,61 @@ struct Solver {
     if (b >= 0) { w.d.tw[(uint32_t)b * 2] = F4(tb[0], 0.0f); w.d.tw[(uint32_t)b * 2 + 1] = F4(tb[1], 0.0f); }
   }
   DEV void set_link_twist(uint32_t l, SVd v) const {
-    uint32_t b = (uint32_t)w.link_body(l);
+    uint32_t b = mbody(l);
     w.d.tw[b * 2] = F4(v.l, 0.0f); w.d.tw[b * 2 + 1] = F4(v.a, 0.0f);
   }
-  DEVN void art_twists(uint32_t a) const {
+  DEVN void art_twists(uint32_t a) const {  // wgsl.js:3240-3251
     uint4 ai = w.art_info(a);
     for (uint32_t k = 0; k < ai.y; k++) {
       uint32_t l = ai.x + k;
-      int par = w.link_parent(l);
+      int par = mpar(l);
       SVd v = sv0();
-      if (par >= 0) v = mshift(w.ldsv(w.lo((uint32_t)par) + L_V), w.link_r(l));
-      for (uint32_t r = 0; r < w.link_ndof(l); r++) v = sadd(v, sscale(w.ldS(l, r), w.d.aqd[w.link_dof(l) + r]));
-      w.stsv(w.lo(l) + L_V, v);
+      if (par >= 0) v = mshift(msv(ml((uint32_t)par) + M_V), mr(l));
+      for (uint32_t r = 0; r < mnd(l); r++) v = sadd(v, sscale(mS(l, r), laqd[mdof(l) + r]));
+      mst(ml(l) + M_V, v);
       set_link_twist(l, v);
     }
   }
   DEVN void art_apply(uint32_t a) const {  // wgsl.js:3254-3299
     uint4 ai = w.art_info(a);
-    float* art_ = w.d.art;
     for (uint32_t kk = 0; kk < ai.y; kk++) {
-      uint32_t l = ai.x + ai.y - 1 - kk, L = w.lo(l);
-      uint32_t n = w.link_ndof(l);
-      SVd p = w.ldsv(L + L_P);
+      uint32_t l = ai.x + ai.y - 1 - kk;
+      float* L = ml(l);
+      uint32_t n = mnd(l);
+      SVd p = msv(L + M_P);
       float u[6];
-      for (uint32_t r = 0; r < n; r++) { u[r] = art_[L + L_TAU + r] - sdot(w.ldS(l, r), p); art_[L + L_UU + r] = u[r]; }
-      int par = w.link_parent(l);
+      for (uint32_t r = 0; r < n; r++) { u[r] = L[M_TAU + r] - sdot(mS(l, r), p); L[M_UU + r] = u[r]; }
+      int par = mpar(l);
       if (par >= 0) {
         SVd pp = p;
         for (uint32_t r = 0; r < n; r++) {
           float du = 0.0f;
-          for (uint32_t c = 0; c < n; c++) du += w.ldDI(l, r, c) * u[c];
-          pp = sadd(pp, sscale(w.ldU(l, r), du));
+          for (uint32_t c = 0; c < n; c++) du += mDI(l, r, c) * u[c];
+          pp = sadd(pp, sscale(mU(l, r), du));
         }
-        uint32_t P0 = w.lo((uint32_t)par) + L_P;
-        w.stsv(P0, sadd(w.ldsv(P0), fshift(pp, w.link_r(l))));
+        float* P0 = ml((uint32_t)par) + M_P;
+        mst(P0, sadd(msv(P0), fshift(pp, mr(l))));
       }
     }
     for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k, L = w.lo(l);
-      int par = w.link_parent(l);
+      uint32_t l = ai.x + k;
+      float* L = ml(l);
+      int par = mpar(l);
       SVd dv = sv0();
-      if (par >= 0) dv = mshift(w.ldsv(w.lo((uint32_t)par) + L_DV), w.link_r(l));
-      uint32_t n = w.link_ndof(l);
+      if (par >= 0) dv = mshift(msv(ml((uint32_t)par) + M_DV), mr(l));
+      uint32_t n = mnd(l);
       float dq[6];
       for (uint32_t r = 0; r < n; r++) {
         dq[r] = 0.0f;
-        for (uint32_t c = 0; c < n; c++) dq[r] += w.ldDI(l, r, c) * (art_[L + L_UU + c] - sdot(w.ldU(l, c), dv));
+        for (uint32_t c = 0; c < n; c++) dq[r] += mDI(l, r, c) * (L[M_UU + c] - sdot(mU(l, c), dv));
       }
-      for (uint32_t r = 0; r < n; r++) { dv = sadd(dv, sscale(w.ldS(l, r), dq[r])); w.d.aqd[w.link_dof(l) + r] += dq[r]; }
-      w.stsv(L + L_DV, dv);
-      SVd v = sadd(w.ldsv(L + L_V), dv);
-      w.stsv(L + L_V, v);
+      for (uint32_t r = 0; r < n; r++) { dv = sadd(dv, sscale(mS(l, r), dq[r])); laqd[mdof(l) + r] += dq[r]; }
+      mst(L + M_DV, dv);
+      SVd v = sadd(msv(L + M_V), dv);
+      mst(L + M_V, v);
       set_link_twist(l, v);
-      w.stsv(L + L_P, sv0());
-      for (uint32_t r = 0; r < w.JM(); r++) art_[L + L_TAU + r] = 0.0f;
+      mst(L + M_P, sv0());
+      for (uint32_t r = 0; r < w.JM(); r++) L[M_TAU + r] = 0.0f;
     }
   }
   // subgroupAdd of the reference's 32-lane path (wgsl.js:3305), order fixed as an xor butterfly over strides
@@ -1098,7 +1153,7 @@ struct Solver {
   DEVN bool art_dense(uint32_t a, uint32_t mode, uint32_t nd, uint32_t lane) const {
     uint4 ai = w.art_info(a);
     float* art_ = w.d.art;
-    float* aqd = w.d.aqd;
+    float* aqd = laqd;
     const uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR), DPS = w.h(W_DPS), AD = w.h(W_A_DENSE);
     const float omega = w.hf(W_F_OMEGA);
     const uint32_t n = ai.w;
@@ -1155,11 +1210,11 @@ struct Solver {
     uint4 ai = w.art_info(a);
     bool hit = false;
     for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k, b = (uint32_t)w.link_body(l);
+      uint32_t l = ai.x + k, b = mbody(l);
       f3 f = xyz(w.d.imp[b * 2]), t = xyz(w.d.imp[b * 2 + 1]);
       if (vne0(f) || vne0(t)) {
         hit = true;
-        w.stsv(w.lo(l) + L_P, mksv(-t, -f));
+        mst(ml(l) + M_P, mksv(-t, -f));
         w.d.imp[b * 2] = make_float4(0, 0, 0, 0); w.d.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
       }
     }
@@ -1183,10 +1238,10 @@ struct Solver {
           float lam = art_[w.h(W_A_LIM) + 2 * dd + side];
           float rhs = gap / dt;
           if (gap < 0.0f) rhs = mx(beta * gap / dt, -maxd);
-          float nl = mx(lam - omega * (sg * w.d.aqd[dd] + rhs) / hinv, 0.0f);
+          float nl = mx(lam - omega * (sg * laqd[dd] + rhs) / hinv, 0.0f);
           if (nl == lam) continue;
           art_[w.h(W_A_LIM) + 2 * dd + side] = nl;
-          art_[w.lo(l) + L_TAU + r] = sg * (nl - lam);
+          ml(l)[M_TAU + r] = sg * (nl - lam);
           art_apply(a);
         }
       }
@@ -1294,7 +1349,7 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
 }
 
 template <int BS>
-__global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_SOLVE))) k_solve(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
@@ -1313,7 +1368,18 @@ __global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ world
   const f3 g = mk3(w.hf(W_GX), w.hf(W_GY), w.hf(W_GZ));
   const uint32_t nu = sched[S.META], ncol = min(sched[S.META + 1], NCOL);
   const uint32_t nd = min(sched[S.META + 3], w.h(W_MAXDENSE));
-  for (uint32_t i = tid; i < ND; i += BS) wd.aqd[i] = art_[w.h(W_A_QDH) + i];
+  // LDS: link mirror, link records, joint velocities (sized by the launch: solve_lds_bytes of the batch's
+  // largest world).
+  extern __shared__ float s_dyn[];
+  sv.lm = s_dyn;
+  sv.lmeta = (uint4*)(s_dyn + (size_t)NL * SOLVE_LB);
+  sv.laqd = (float*)(sv.lmeta + NL);
+  for (uint32_t i = tid; i < NL * SOLVE_LB; i += BS) {
+    uint32_t l = i / SOLVE_LB, src = mirror_src(i - l * SOLVE_LB);
+    sv.lm[i] = src < LB ? art_[w.lo(l) + src] : 0.0f;
+  }
+  for (uint32_t l = tid; l < NL; l += BS) sv.lmeta[l] = make_uint4((uint32_t)w.link_parent(l), w.link_ndof(l), w.link_dof(l), (uint32_t)w.link_body(l));
+  for (uint32_t i = tid; i < ND; i += BS) sv.laqd[i] = art_[w.h(W_A_QDH) + i];
   for (uint32_t b = tid; b < NB; b += BS) {
     uint32_t kind = w.body_kind((int)b);
     wd.imp[b * 2] = make_float4(0, 0, 0, 0); wd.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
@@ -1350,6 +1416,12 @@ __global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ world
   sweep<BS>(sv, 0, ncol, nu, nd);
   const uint32_t iters = w.h(W_ITERS);
   for (uint32_t it = 0; it < iters; it++) sweep<BS>(sv, 1, ncol, nu, nd);
+  // Mutable mirror fields back to the link blocks (p, dv, v, u, tau; S, U, D^-1, COM are unchanged).
+  for (uint32_t i = tid; i < NL * SOLVE_LB; i += BS) {
+    uint32_t l = i / SOLVE_LB, f = i - l * SOLVE_LB;
+    if (f >= M_P && f < M_COM) art_[w.lo(l) + mirror_src(f)] = sv.lm[i];
+  }
+  __syncthreads();
   const uint32_t nrows = sched[S.USTART + nu];
   const bool PATCH = w.h(W_PATCH) != 0;
   const uint32_t PB = w.h(W_P_BODY), PP = w.h(W_P_PREV);
@@ -1368,7 +1440,7 @@ __global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ world
     wd.poses[PB + b * 2] = F4(c, 0.0f);
     wd.poses[PB + b * 2 + 1] = F4(q);
   }
-  for (uint32_t i = tid; i < ND; i += BS) art_[w.h(W_A_QD) + i] = wd.aqd[i];
+  for (uint32_t i = tid; i < ND; i += BS) art_[w.h(W_A_QD) + i] = sv.laqd[i];
   __syncthreads();
   for (uint32_t l = tid; l < NL; l += BS) {
     if (w.link_type(l) != JT_FREE || w.link_parent(l) >= 0) continue;
@@ -1382,16 +1454,16 @@ __global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ world
     for (uint32_t k = 0; k < ai.y; k++) {
       uint32_t l = ai.x + k, qs = w.link_qs(l), dd = w.link_dof(l), t = w.link_type(l);
       if (t == JT_REVOLUTE || t == JT_PRISMATIC || t == JT_D6)
-        for (uint32_t r = 0; r < w.link_ndof(l); r++) art_[AQ + qs + r] += wd.aqd[dd + r] * dt;
+        for (uint32_t r = 0; r < w.link_ndof(l); r++) art_[AQ + qs + r] += sv.laqd[dd + r] * dt;
       if (t == JT_BALL) {
-        f3 wv = mk3(wd.aqd[dd], wd.aqd[dd + 1], wd.aqd[dd + 2]);
+        f3 wv = mk3(sv.laqd[dd], sv.laqd[dd + 1], sv.laqd[dd + 2]);
         q4 q = mk4(art_[AQ + qs], art_[AQ + qs + 1], art_[AQ + qs + 2], art_[AQ + qs + 3]);
         q = nrm4(q + (0.5f * dt) * qmul(mk4(wv, 0.0f), q));
         art_[AQ + qs] = q.x; art_[AQ + qs + 1] = q.y; art_[AQ + qs + 2] = q.z; art_[AQ + qs + 3] = q.w;
       }
       if (t == JT_FREE) {
-        for (uint32_t c = 0; c < 3; c++) art_[AQ + qs + c] += wd.aqd[dd + c] * dt;
-        f3 wv = mk3(wd.aqd[dd + 3], wd.aqd[dd + 4], wd.aqd[dd + 5]);
+        for (uint32_t c = 0; c < 3; c++) art_[AQ + qs + c] += sv.laqd[dd + c] * dt;
+        f3 wv = mk3(sv.laqd[dd + 3], sv.laqd[dd + 4], sv.laqd[dd + 5]);
         q4 q = mk4(art_[AQ + qs + 3], art_[AQ + qs + 4], art_[AQ + qs + 5], art_[AQ + qs + 6]);
         q = nrm4(q + (0.5f * dt) * qmul(mk4(wv, 0.0f), q));
         art_[AQ + qs + 3] = q.x; art_[AQ + qs + 4] = q.y; art_[AQ + qs + 5] = q.z; art_[AQ + qs + 6] = q.w;
@@ -1471,8 +1543,10 @@ __global__ void __launch_bounds__(BS) k_solve(const WorldDev* __restrict__ world
 // ------------------------------------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------------------------------------
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, hipStream_t stream, uint32_t stages,
-                                hipEvent_t* marks) {
+size_t steel_solve_lds_bytes(uint32_t NL, uint32_t ND) { return solve_lds_bytes(NL, ND); }
+
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, size_t solve_lds, hipStream_t stream,
+                                uint32_t stages, hipEvent_t* marks) {
   if (nworlds == 0) return hipSuccess;
   dim3 grid(nworlds);
   if (marks) (void)hipEventRecord(marks[0], stream);
@@ -1487,9 +1561,9 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
   }
   if (marks) (void)hipEventRecord(marks[2], stream);
   if (stages & STAGE_SOLVE) {
-    if (block == 64) hipLaunchKernelGGL((k_solve<64>), grid, dim3(64), 0, stream, worlds, nworlds);
-    else if (block == 128) hipLaunchKernelGGL((k_solve<128>), grid, dim3(128), 0, stream, worlds, nworlds);
-    else hipLaunchKernelGGL((k_solve<256>), grid, dim3(256), 0, stream, worlds, nworlds);
+    if (block == 64) hipLaunchKernelGGL((k_solve<64>), grid, dim3(64), solve_lds, stream, worlds, nworlds);
+    else if (block == 128) hipLaunchKernelGGL((k_solve<128>), grid, dim3(128), solve_lds, stream, worlds, nworlds);
+    else hipLaunchKernelGGL((k_solve<256>), grid, dim3(256), solve_lds, stream, worlds, nworlds);
   }
   if (marks) (void)hipEventRecord(marks[3], stream);
   return hipGetLastError();

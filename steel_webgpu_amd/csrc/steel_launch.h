@@ -11,6 +11,12 @@ enum : uint32_t {
   STAGE_ALL = 15u,
 };
 
+// Dynamic LDS bytes k_solve needs for a world of NL links and ND dofs (its link mirror).
+size_t steel_solve_lds_bytes(uint32_t NL, uint32_t ND);
+// Upper bound of k_solve's dynamic LDS (worlds needing more are refused at batch creation).
+static constexpr size_t STEEL_SOLVE_LDS_MAX = 64 * 1024;
+
+// solve_lds: dynamic LDS of k_solve, the maximum of steel_solve_lds_bytes over the batch's worlds.
 // marks (optional): four events recorded before collide, prepare, solve and after solve.
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, hipStream_t stream, uint32_t stages,
-                                hipEvent_t* marks = nullptr);
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, size_t solve_lds, hipStream_t stream,
+                                uint32_t stages, hipEvent_t* marks = nullptr);

@@ -16,7 +16,8 @@ import numpy as np
 from . import pack as _pack
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsteel_amd.so")
+# STEEL_AMD_LIB names another build of the same library (e.g. one compiled with other waves-per-SIMD settings).
+LIB_PATH = os.environ.get("STEEL_AMD_LIB") or os.path.join(_HERE, "libsteel_amd.so")
 
 STEEL_OK, STEEL_ERROR, STEEL_UNSUPPORTED, STEEL_BAD_ARGUMENT, STEEL_OUT_OF_MEMORY = 0, -1, -2, -3, -4
 BUF = dict(poses=0, vel=1, art=2, anchor_keys=3, anchor_vals=4, pairs=5, contacts=6, rows=7, sched=8, counters=9, stats=10)
