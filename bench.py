@@ -3,15 +3,17 @@
 
 Workload (the batching configuration): 4096 worlds of uneven size, each a floating-base 12-joint quadruped plus
 K ~ Geometric(0.1) <= 64 boxes on a ground plane, FP32, dt = 1/200 s, 4 substeps, 12 iterations, all worlds in
-one launch per kernel with the frame's substep loop captured in a hipGraph.
+one launch per kernel with the frame's substep loop captured in a hipGraph; 1000 frames from rest.
 
-value       batched world-steps/s (worlds x frames / wall time of the timed frames, inputs resident in HBM)
+value       batched world-steps/s: worlds x frames / wall time of frames 0..K-1 after a reset, state resident
+            in HBM (warmup frames run before the reset)
 plain       world-steps/s of the plain per-world step (one world per launch, per-substep launches, as the
-            reference dispatches), measured on a sample of the same worlds
+            reference dispatches), frames 0..K-1 of a few of the same worlds
 latency     p50/p99 of the batched frame time (HIP events per frame)
-roofline    dominant kernel: algorithmic bytes per launch / average launch time (HIP events on the library's
-            stream) against 8 TB/s
-cpu_baseline  the FP32 CPU oracle (port of the reference step), one world per process on the host cores
+roofline    dominant kernel: algorithmic bytes per launch / average launch time, both over a replay of the
+            same K frames (each kernel bracketed by HIP events on the library's stream), against 8 TB/s
+cpu_baseline  the FP32 CPU oracle (port of the reference step) over the same frames of a sample of the
+            worlds, one world at a time per process on the host cores
 """
 import argparse
 import json
@@ -83,22 +85,26 @@ def hull_bytes(pw):
 
 
 def algorithmic_bytes(batch, worlds):
-    """Per-substep compulsory bytes of each kernel, summed over worlds, from each world's last-substep counts.
+    """Per-launch compulsory bytes of each kernel, summed over worlds and averaged over the substeps since the
+    last reset (stats words 12-15: sums of pairs, kept contacts, rows; substep count).
 
     collide  288 NS + 296 NP + Hp + 304 NC      (shape pose/AABB, pair record + two shape records, hull data of
                                                  hull pairs, contact write + sort read/key/write)
     prepare  96 NC + 432 NR + 1200 NL           (sorted contact read, row build, articulation factor/response)
     solve    (It+1)(384 NR + 576 NL) + 200 NB + 8 ND   (row read + impulse write per sweep, link propagation,
                                                  predictor/integrator, joint state)
+    Hp: hull bytes per pair of the world's last substep pair list times the window's mean pair count.
     """
     tot = np.zeros(3)
     for i, pw in enumerate(worlds):
         st, _ = batch.stats_raw(i)
-        NP, NC, NR = int(st[9]), int(st[5]), int(st[10])
+        nsub = max(int(st[15]), 1)
+        NP, NC, NR = st[12] / nsub, st[13] / nsub, st[14] / nsub
         NS, NB, NL, ND = pw.NS, pw.NB, pw.NL, pw.ND
         It = pw.h("W_ITERS")
         hb = hull_bytes(pw)
-        pairs = batch.read(i, "pairs", 8 * min(NP, pw.h("W_MAXP")), dtype=np.uint32).reshape(-1, 2) if NP else np.zeros((0, 2), np.int64)
+        lastp = int(st[9])
+        pairs = batch.read(i, "pairs", 8 * min(lastp, pw.h("W_MAXP")), dtype=np.uint32).reshape(-1, 2) if lastp else np.zeros((0, 2), np.int64)
         Hp = 0
         SF = pw.statF.reshape(-1, 4)
         for a, b in pairs:
@@ -111,6 +117,7 @@ def algorithmic_bytes(batch, worlds):
             for s, k in ((a, ka), (b, kb)):
                 if k == 0:
                     Hp += hb[int(SF[pw.h("W_SF_SHAPE") + s * 7][1])]
+        Hp = Hp / max(len(pairs), 1) * NP
         tot += [288 * NS + 296 * NP + Hp + 304 * NC, 96 * NC + 432 * NR + 1200 * NL,
                 (It + 1) * (384 * NR + 576 * NL) + 200 * NB + 8 * ND]
     return tot
@@ -162,13 +169,11 @@ def traffic_from_profile(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000, help="timed frames, from rest (the configuration's 1000 steps)")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed frames before the reset that starts the timed run")
     ap.add_argument("--worlds", type=int, default=4096)
-    ap.add_argument("--plain-worlds", type=int, default=8)
-    ap.add_argument("--plain-steps", type=int, default=40)
-    ap.add_argument("--cpu-worlds", type=int, default=64)
-    ap.add_argument("--cpu-frames", type=int, default=200)
+    ap.add_argument("--plain-worlds", type=int, default=3)
+    ap.add_argument("--cpu-worlds", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="prepare/solve workgroup size (0: autotune)")
     ap.add_argument("--worlds-cache", default=None, help="pickle file to load packed worlds from / save them to")
@@ -187,6 +192,7 @@ def main():
         sys.exit(2)
     procs = max(1, min(16, os.cpu_count() or 1))
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    K = args.steps
     # Host-side work that uses worker processes runs before anything touches the GPU.
     t0 = time.time()
     worlds = build_worlds(args.worlds, procs, args.worlds_cache)
@@ -196,10 +202,10 @@ def main():
         f"max {max(w.NB for w in worlds)}")
     cpu = None
     if not args.no_cpu:
-        rate, steps, cwall = cpu_baseline(args.cpu_worlds, args.cpu_frames, procs)
+        rate, steps, cwall = cpu_baseline(args.cpu_worlds, K, procs)
         cpu = {"value": round(rate, 1), "unit": "world-steps/s", "cores": procs, "kind": "port",
-               "sample": f"worlds 0..{args.cpu_worlds - 1} of the {args.worlds}-world quadruped batch x {args.cpu_frames} frames from "
-                         f"rest, FP32 oracle, one world at a time per process ({steps} world-steps, {cwall:.1f}s wall)"}
+               "sample": f"worlds 0..{args.cpu_worlds - 1} of the {args.worlds}-world quadruped batch, frames 0..{K - 1} from rest "
+                         f"(the timed window), FP32 oracle, one world at a time per process ({steps} world-steps, {cwall:.1f}s wall)"}
         log(f"cpu baseline: {cpu['value']} world-steps/s on {procs} processes")
     from steel_webgpu_amd import sim
     dev = sim.Device(0)
@@ -207,34 +213,49 @@ def main():
     graph = not args.no_graph
     batch = sim.Batch(worlds, dev, block=256, graph=graph)
     log(f"batch resident: {batch.device_bytes() / 2**30:.2f} GiB")
-    # Autotuner: prepare/solve workgroup size by measured frame time.
+    # Autotuner: prepare/solve workgroup size by measured frame time over the first frames.
     block = args.block
     if block == 0:
         best = None
         for bs in (64, 128, 256):
             batch.configure(bs, graph)
             batch.reset()
-            batch.step(3)
-            ms = float(np.median(batch.step_timed(4)))
-            log(f"  autotune block {bs}: {ms:.2f} ms/frame")
+            ms = float(np.sum(batch.step_timed(10)))
+            log(f"  autotune block {bs}: {ms / 10:.2f} ms/frame over frames 0-9")
             if best is None or ms < best[1]:
                 best = (bs, ms)
         block = best[0]
     batch.configure(block, graph)
-    batch.reset()
-    batch.step(args.warmup)
-    batch.sync()
-    t0 = time.perf_counter()
-    frame_ms = batch.step_timed(args.steps)
-    batch.sync()
-    wall = time.perf_counter() - t0
     nw = len(worlds)
-    value = nw * args.steps / wall
-    ms_per_step = wall / args.steps * 1e3
-    log(f"batched: {value:.0f} world-steps/s, {ms_per_step:.3f} ms/frame (block {block})")
-    # Roofline of the dominant kernel: the timed region's frame count again, each kernel bracketed by HIP events.
     S = worlds[0].h("W_SUBSTEPS")
-    kms = batch.profile_kernels(args.steps * S)
+    CH = 100
+    value = ms_per_step = None
+    frame_ms = np.zeros(0)
+    if not args.profile:
+        batch.step(args.warmup)
+        batch.reset()
+        batch.sync()
+        parts = []
+        t0 = time.perf_counter()
+        for f0 in range(0, K, CH):
+            parts.append(batch.step_timed(min(CH, K - f0)))
+            log(f"  timed frames {f0}..{f0 + len(parts[-1]) - 1}: {np.mean(parts[-1]):.3f} ms/frame")
+        batch.sync()
+        wall = time.perf_counter() - t0
+        frame_ms = np.concatenate(parts)
+        value = nw * K / wall
+        ms_per_step = wall / K * 1e3
+        log(f"batched: {value:.0f} world-steps/s, {ms_per_step:.3f} ms/frame over frames 0..{K - 1} (block {block})")
+    # Per-kernel times on a replay of the same frames (the step is bitwise deterministic), each kernel bracketed
+    # by HIP events on the library's stream; the algorithmic bytes are averaged over the same substeps.
+    batch.reset()
+    acc = np.zeros(3)
+    for f0 in range(0, K, CH):
+        n = min(CH, K - f0)
+        km = batch.profile_kernels(n * S)
+        acc += np.asarray(km) * n
+        log(f"  kernel replay frames {f0}..{f0 + n - 1}: collide {km[0]:.3f} prepare {km[1]:.3f} solve {km[2]:.3f} ms/launch")
+    kms = list(acc / K)
     algo = algorithmic_bytes(batch, worlds)
     names = ["collide", "prepare", "solve"]
     k = int(np.argmax(kms))
@@ -243,31 +264,31 @@ def main():
     log(f"kernels ms/launch collide {kms[0]:.3f} prepare {kms[1]:.3f} solve {kms[2]:.3f}; algorithmic MB/launch "
         f"{algo[0] / 1e6:.1f} {algo[1] / 1e6:.1f} {algo[2] / 1e6:.1f}; worlds at contact capacity: {overflow}")
     batch.destroy()
-    # Plain per-world step: one world per batch, one launch per kernel per substep (no graph).
-    plain_rates, plain_ms = [], []
-    for i in range(min(args.plain_worlds, nw)):
-        b1 = sim.Batch([worlds[i * (nw // max(1, args.plain_worlds))]], dev, block=64, graph=False)
-        b1.step(5)
-        b1.sync()
-        t1 = time.perf_counter()
-        fm = b1.step_timed(args.plain_steps)
-        b1.sync()
-        dtp = time.perf_counter() - t1
-        plain_rates.append(args.plain_steps / dtp)
-        plain_ms.extend(fm.tolist())
-        b1.destroy()
     if args.profile:
         copy_calibration()
         log("profile run done")
         return
+    # Plain per-world step: one world per batch, one launch per kernel per substep (no graph), same frames.
+    plain_rates, plain_ms = [], []
+    for i in range(min(args.plain_worlds, nw)):
+        b1 = sim.Batch([worlds[i * (nw // max(1, args.plain_worlds))]], dev, block=64, graph=False)
+        b1.sync()
+        t1 = time.perf_counter()
+        fm = b1.step_timed(K)
+        b1.sync()
+        dtp = time.perf_counter() - t1
+        plain_rates.append(K / dtp)
+        plain_ms.extend(fm.tolist())
+        b1.destroy()
+        log(f"  plain world {i * (nw // max(1, args.plain_worlds))}: {K / dtp:.1f} world-steps/s")
     plain = float(np.mean(plain_rates))
     log(f"plain per-world: {plain:.1f} world-steps/s ({np.median(plain_ms):.3f} ms/frame GPU)")
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "world-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "metric": METRIC, "value": round(value, 1), "unit": "world-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded PCG32 scenes)",
         "config": {"workload": "quadruped_batch", "worlds": nw, "fps": 200, "substeps": S, "iterations": worlds[0].h("W_ITERS"),
-                   "block": block, "graph": graph},
+                   "frames": f"0..{K - 1} from rest", "block": block, "graph": graph},
         "plain_world_steps_per_s": round(plain, 2),
         "speedup_vs_plain": round(value / plain, 1),
         "latency_ms": {"p50": round(float(np.percentile(frame_ms, 50)), 4), "p99": round(float(np.percentile(frame_ms, 99)), 4),

@@ -39,7 +39,8 @@ enum steel_buffer {
   STEEL_BUF_SCHED = 8,     /* uint32 schedule tables */
   STEEL_BUF_COUNTERS = 9,  /* uint32[4]: pairs, contacts, substep count, flags */
   STEEL_BUF_STATS = 10,    /* uint32[16]: the reference's stats words 0-8 (src/wgsl.js:3879-3890); 9-11 the last
-                              substep's candidate pairs, rows and units */
+                              substep's candidate pairs, rows and units; 12-15 sums since reset of pairs, kept
+                              contacts, rows and the substep count */
   STEEL_BUF_COUNT = 11
 };
 

@@ -1535,6 +1535,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     st[9] = wd.ctr[0];   // last substep: candidate pairs, rows, units (per-step traffic accounting)
     st[10] = nrows;
     st[11] = nu;
+    st[12] += wd.ctr[0];  // running sums since reset: pairs, contacts, rows, substeps
+    st[13] += min(wd.ctr[1], MAXC);
+    st[14] += nrows;
+    st[15] += 1;
     wd.ctr[0] = 0; wd.ctr[1] = 0;
     wd.ctr[2] += 1;
   }
