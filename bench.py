@@ -58,6 +58,42 @@ def build_worlds(n, workers, cache=None):
     return worlds
 
 
+def _pack_request(args):
+    i, nmax, hmax = args
+    from steel_webgpu_amd import pack, workloads
+    sc, h = workloads.serving_request(i, nmax=nmax, hmax=hmax)
+    pw = pack.pack_scene(sc, capacity_policy="compact")
+    pw.scene = None
+    return pw, h
+
+
+def serving(args, procs, log):
+    """Serving: Poisson requests (rate --rate) of N ~ logU[10, --nmax] bodies (30% hulls, 70% boxes, 0-4
+    quadrupeds) and horizons ~ U[50, --hmax] frames through the serving pool with an HBM budget."""
+    from steel_webgpu_amd import serve, workloads
+    t0 = time.time()
+    ctx = mp.get_context("fork")
+    with ctx.Pool(procs) as pool:
+        packed = pool.map(_pack_request, [(i, args.nmax, args.hmax) for i in range(args.requests)], chunksize=1)
+    log(f"packed {len(packed)} requests in {time.time() - t0:.1f}s; bodies mean {np.mean([p.NB for p, _ in packed]):.0f}, "
+        f"max {max(p.NB for p, _ in packed)}; frames {sum(h for _, h in packed)}")
+    from steel_webgpu_amd import sim
+    dev = sim.Device(0)
+    results = []
+    for rate in args.rates:
+        rng = workloads.PCG32(workloads.SEED + 5, int(rate))
+        arr = serve.poisson_arrivals(len(packed), rate, rng)
+        reqs = [serve.Request(rid=i, arrival=arr[i], world=pw, horizon=h, lane=serve.lane_of(pw)) for i, (pw, h) in enumerate(packed)]
+        srv = serve.Server(budget_bytes=int(args.budget_gb * 2**30), device=dev)
+        m = serve.run(reqs, srv, readback=())
+        srv.destroy()
+        m["rate"] = rate
+        log(f"  rate {rate}/s: {m['world_steps_per_s']:.0f} world-steps/s, step latency p50 {m['step_latency_ms']['p50']:.2f} ms "
+            f"p99 {m['step_latency_ms']['p99']:.2f} ms, queue wait p99 {m['queue_wait_ms']['p99']:.1f} ms, {m['wall_s']:.1f}s")
+        results.append(m)
+    return results
+
+
 def copy_calibration(nbytes=1 << 30, reps=4):
     """A plain device copy of known size (k_calib_copy, libsteel_calib.so), for scaling the profiler's
     FETCH_SIZE / WRITE_SIZE counters: each launch reads and writes `nbytes`."""
@@ -181,6 +217,12 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: batched leg only (no autotune, plain or CPU legs), then a 1 GiB copy for counter scaling")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--workload", default="batch", choices=["batch", "serving"])
+    ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
+    ap.add_argument("--rates", type=float, nargs="+", default=[200.0, 1000.0, 5000.0], help="serving: Poisson rates (req/s)")
+    ap.add_argument("--nmax", type=int, default=2000, help="serving: largest request body count")
+    ap.add_argument("--hmax", type=int, default=1000, help="serving: longest horizon (frames)")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="serving: HBM budget of the pool")
     args = ap.parse_args()
     if args.profile:
         args.no_cpu = True
@@ -192,6 +234,18 @@ def main():
         sys.exit(2)
     procs = max(1, min(16, os.cpu_count() or 1))
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.workload == "serving":
+        res = serving(args, procs, log)
+        tot = sum(r["world_steps"] for r in res)
+        wall = sum(r["wall_s"] for r in res)
+        print(json.dumps({
+            "metric": "served world-steps/s + p50/p99 step latency", "value": round(tot / wall, 1), "unit": "world-steps/s",
+            "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded PCG32 scenes)",
+            "config": {"workload": "serving", "requests_per_rate": args.requests, "rates": args.rates, "nmax": args.nmax,
+                       "hmax": args.hmax, "budget_gb": args.budget_gb, "fps": 240, "substeps": 4, "iterations": 12,
+                       "lanes": "NB<100 / <600 / >=600 bodies, blocks 64/128/256"},
+            "per_rate": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in res]}))
+        return
     K = args.steps
     # Host-side work that uses worker processes runs before anything touches the GPU.
     t0 = time.time()

@@ -104,6 +104,32 @@ int steel_batch_device_bytes(steel_batch* b, size_t* bytes);
 /* Device bytes a batch of these worlds would hold, without creating it. */
 int steel_estimate_bytes(uint32_t nworlds, const steel_world_desc* worlds, size_t* bytes);
 
+/* ---- Serving (no counterpart in the reference, which steps one Sim per device; this is the request loop's
+ * device side).  A server owns one device pool of `budget_bytes` (the HBM budget).  Worlds are admitted one
+ * at a time into a lane (a size class with its own stream); a lane frame steps every world the lane holds at
+ * launch by one Sim.step(1) frame, so worlds join and leave between frames while other lanes keep running.
+ * Each admitted world is stepped exactly as it would be alone (same kernels, no cross-world reads). */
+typedef struct steel_server steel_server;
+int steel_server_create(steel_device* dev, size_t budget_bytes, uint32_t nlanes, steel_server** out);
+int steel_server_destroy(steel_server* s);
+/* Workgroup size (64, 128 or 256) of a lane's prepare/solve kernels. */
+int steel_server_configure_lane(steel_server* s, uint32_t lane, uint32_t block);
+/* Pool bytes one world takes. */
+int steel_server_world_bytes(const steel_world_desc* w, size_t* bytes);
+/* Copy the world into the pool and add it to `lane` from the lane's next frame on.  STEEL_OUT_OF_MEMORY, with
+ * nothing allocated, when the pool cannot hold it now (the caller keeps it queued and retries). */
+int steel_server_admit(steel_server* s, const steel_world_desc* w, uint32_t lane, uint32_t* id);
+/* Remove a world from its lane; its pool block is reused once the lane's in-flight frame has finished. */
+int steel_server_release(steel_server* s, uint32_t id);
+/* Launch the lane's next frame over its current worlds (asynchronous; waits for the lane's previous frame). */
+int steel_server_step_lane(steel_server* s, uint32_t lane);
+/* done = 1 once the lane's last launched frame has finished; nworlds = worlds it stepped; gpu_ms = its GPU time. */
+int steel_server_lane_poll(steel_server* s, uint32_t lane, int* done, uint32_t* nworlds, float* gpu_ms);
+/* Sim.read of an admitted world (waits for its lane's in-flight frame). */
+int steel_server_read(steel_server* s, uint32_t id, uint32_t buf, size_t byte_off, size_t bytes, void* dst);
+/* Pool bytes in use, pool capacity, admitted worlds. */
+int steel_server_info(steel_server* s, size_t* used, size_t* capacity, uint32_t* nworlds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,4 +1,4 @@
-// Host runtime and C ABI of the Steel step library (declared in include/steel_amd.h).
+// Host runtime and C ABI of the Steel step library (declared in include/steel_amd.h): device and batch.
 //
 // A batch keeps every per-world buffer in one device arena per buffer kind (one hipMalloc each, 256-byte
 // aligned slices), a device table of per-world pointers, a copy of the initial state for reset, and one
@@ -12,11 +12,8 @@
 #include <string>
 #include <vector>
 #include <algorithm>
-#include "steel_dev.h"
-#include "steel_launch.h"
-#include "../../include/steel_amd.h"
+#include "steel_runtime.h"
 
-namespace {
 thread_local std::string g_err;
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -27,59 +24,7 @@ int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
-#define HIPCHK(expr)                                                                          \
-  do {                                                                                        \
-    hipError_t e_ = (expr);                                                                   \
-    if (e_ != hipSuccess) return fail(STEEL_ERROR, "%s: %s", #expr, hipGetErrorString(e_));   \
-  } while (0)
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-enum Kind { K_HDR, K_STATF, K_STATU, K_POSES, K_VEL, K_ART, K_AKEY, K_AVAL, K_CTR, K_STATS, K_PAIRS, K_CRAW, K_CONTACTS, K_ROWS, K_SCHED,
-            K_SKEY, K_SVAL, K_TW, K_IMP, K_AQD, K_POSES0, K_VEL0, K_ART0, K_COUNT };
-
-struct Sizes { size_t b[K_COUNT]; };
-
-int world_sizes(const uint32_t* h, Sizes& s) {
-  uint32_t NB = h[W_NB], NS = h[W_NS], ND = h[W_ND], MAXC = h[W_MAXC], MAXP = h[W_MAXP], AH = h[W_AH], ROWS = h[W_ROWS];
-  if (MAXC < NS || MAXC == 0) return fail(STEEL_BAD_ARGUMENT, "world capacity MAXC=%u is below its shape count %u", MAXC, NS);
-  if (h[W_NDA] > 64 || h[W_MAXLP] > 64) return fail(STEEL_BAD_ARGUMENT, "articulations of more than 64 dofs or links are not supported");
-  if (h[W_MAXPATH] > 64 || h[W_MAXPATH] * h[W_JM] > 192) return fail(STEEL_BAD_ARGUMENT, "articulation root paths deeper than 64 links are not supported");
-  if (h[W_UH] > MAXC || h[W_MAXU] > MAXC) return fail(STEEL_BAD_ARGUMENT, "unit capacities exceed the contact capacity");
-  if (steel_solve_lds_bytes(h[W_NL], ND) > STEEL_SOLVE_LDS_MAX)
-    return fail(STEEL_UNSUPPORTED, "a world of %u articulation links exceeds the solve kernel's LDS link mirror", h[W_NL]);
-  SchedLayout L = sched_layout(MAXC, h[W_MAXU], h[W_UH], h[W_NCOLW] * 32, NB, h[W_NCOLW], h[W_NL]);
-  s.b[K_HDR] = STEEL_WORLD_HEADER_WORDS * 4;
-  s.b[K_STATF] = (size_t)h[W_NSTATF] * 16;
-  s.b[K_STATU] = (size_t)std::max(h[W_NSTATU], 1u) * 4;
-  s.b[K_POSES] = (size_t)h[W_NPOSES] * 16;
-  s.b[K_VEL] = (size_t)std::max(NB, 1u) * 32;
-  s.b[K_ART] = (size_t)h[W_NART] * 4;
-  s.b[K_AKEY] = (size_t)2 * AH * 4;
-  s.b[K_AVAL] = (size_t)2 * AH * STEEL_AVN * 4;
-  s.b[K_CTR] = 16;
-  s.b[K_STATS] = 64;
-  s.b[K_PAIRS] = (size_t)2 * MAXP * 4;
-  s.b[K_CRAW] = (size_t)6 * MAXC * 16;
-  s.b[K_CONTACTS] = (size_t)6 * MAXC * 16;
-  s.b[K_ROWS] = (size_t)MAXC * ROWS * 16;
-  s.b[K_SCHED] = (size_t)L.TOTAL * 4;
-  s.b[K_SKEY] = (size_t)2 * MAXC * 8;
-  s.b[K_SVAL] = (size_t)2 * MAXC * 4;
-  s.b[K_TW] = (size_t)std::max(NB, 1u) * 32;
-  s.b[K_IMP] = (size_t)std::max(NB, 1u) * 32;
-  s.b[K_AQD] = (size_t)std::max(ND, 1u) * 4;
-  s.b[K_POSES0] = s.b[K_POSES];
-  s.b[K_VEL0] = s.b[K_VEL];
-  s.b[K_ART0] = s.b[K_ART];
-  return STEEL_OK;
-}
-}  // namespace
-
-struct steel_device {
-  int ordinal;
-  hipDeviceProp_t prop;
-};
 
 struct steel_batch {
   steel_device* dev;
@@ -174,14 +119,13 @@ int steel_batch_reset(steel_batch* b, int32_t world) {
     HIPCHK(hipMemcpyAsync(b->arena[K_POSES], b->arena[K_POSES0], b->total[K_POSES], hipMemcpyDeviceToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->arena[K_VEL], b->arena[K_VEL0], b->total[K_VEL], hipMemcpyDeviceToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->arena[K_ART], b->arena[K_ART0], b->total[K_ART], hipMemcpyDeviceToDevice, b->stream));
-    for (int k : {K_AKEY, K_AVAL, K_CTR, K_STATS, K_TW, K_IMP, K_AQD, K_ROWS, K_SCHED}) HIPCHK(hipMemsetAsync(b->arena[k], 0, b->total[k], b->stream));
+    for (int k : K_ZEROED) HIPCHK(hipMemsetAsync(b->arena[k], 0, b->total[k], b->stream));
   } else {
     for (uint32_t i = lo; i < hi; i++) {
       HIPCHK(hipMemcpyAsync(b->arena[K_POSES] + b->off[K_POSES][i], b->arena[K_POSES0] + b->off[K_POSES0][i], b->size[K_POSES][i], hipMemcpyDeviceToDevice, b->stream));
       HIPCHK(hipMemcpyAsync(b->arena[K_VEL] + b->off[K_VEL][i], b->arena[K_VEL0] + b->off[K_VEL0][i], b->size[K_VEL][i], hipMemcpyDeviceToDevice, b->stream));
       HIPCHK(hipMemcpyAsync(b->arena[K_ART] + b->off[K_ART][i], b->arena[K_ART0] + b->off[K_ART0][i], b->size[K_ART][i], hipMemcpyDeviceToDevice, b->stream));
-      for (int k : {K_AKEY, K_AVAL, K_CTR, K_STATS, K_TW, K_IMP, K_AQD, K_ROWS, K_SCHED})
-        HIPCHK(hipMemsetAsync(b->arena[k] + b->off[k][i], 0, b->size[k][i], b->stream));
+      for (int k : K_ZEROED) HIPCHK(hipMemsetAsync(b->arena[k] + b->off[k][i], 0, b->size[k][i], b->stream));
     }
   }
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -239,28 +183,9 @@ int steel_batch_create(steel_device* dev, uint32_t nworlds, const steel_world_de
   if (rc) { batch_free(b); delete b; return STEEL_ERROR; }
   std::vector<WorldDev> wd(nworlds);
   for (uint32_t i = 0; i < nworlds; i++) {
-    auto P = [&](int k) { return b->arena[k] + b->off[k][i]; };
-    WorldDev& x = wd[i];
-    x.hdr = (const uint32_t*)P(K_HDR);
-    x.statF = (const float4*)P(K_STATF);
-    x.statU = (const uint32_t*)P(K_STATU);
-    x.poses = (float4*)P(K_POSES);
-    x.vel = (float4*)P(K_VEL);
-    x.art = (float*)P(K_ART);
-    x.akey = (uint32_t*)P(K_AKEY);
-    x.aval = (uint32_t*)P(K_AVAL);
-    x.ctr = (uint32_t*)P(K_CTR);
-    x.stats = (uint32_t*)P(K_STATS);
-    x.pairs = (uint32_t*)P(K_PAIRS);
-    x.craw = (float4*)P(K_CRAW);
-    x.contacts = (float4*)P(K_CONTACTS);
-    x.rows = (float4*)P(K_ROWS);
-    x.sched = (uint32_t*)P(K_SCHED);
-    x.skey = (uint64_t*)P(K_SKEY);
-    x.sval = (uint32_t*)P(K_SVAL);
-    x.tw = (float4*)P(K_TW);
-    x.imp = (float4*)P(K_IMP);
-    x.aqd = (float*)P(K_AQD);
+    char* base[K_COUNT];
+    for (int k = 0; k < K_COUNT; k++) base[k] = b->arena[k] + b->off[k][i];
+    wd[i] = world_dev_at(base);
   }
   if (hipMalloc(&b->d_worlds, sizeof(WorldDev) * nworlds) != hipSuccess ||
       hipMemcpy(b->d_worlds, wd.data(), sizeof(WorldDev) * nworlds, hipMemcpyHostToDevice) != hipSuccess) {
@@ -377,10 +302,6 @@ int steel_batch_sync(steel_batch* b) {
   return STEEL_OK;
 }
 
-static int buf_kind(uint32_t buf) {
-  static const int map[STEEL_BUF_COUNT] = {K_POSES, K_VEL, K_ART, K_AKEY, K_AVAL, K_PAIRS, K_CONTACTS, K_ROWS, K_SCHED, K_CTR, K_STATS};
-  return buf < STEEL_BUF_COUNT ? map[buf] : -1;
-}
 
 int steel_batch_buffer_size(steel_batch* b, uint32_t world, uint32_t buf, size_t* bytes) {
   if (!b || !bytes) return fail(STEEL_BAD_ARGUMENT, "null argument");

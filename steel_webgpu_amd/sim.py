@@ -26,7 +26,10 @@ STAGE_COLLIDE, STAGE_NARROW, STAGE_PREPARE, STAGE_SOLVE, STAGE_ALL = 1, 2, 4, 8,
 EXPORTS = ["steel_last_error", "steel_device_open", "steel_device_close", "steel_device_info", "steel_batch_create",
            "steel_batch_destroy", "steel_batch_reset", "steel_batch_step", "steel_batch_substep_stages", "steel_batch_sync",
            "steel_batch_read", "steel_batch_write", "steel_batch_buffer_size", "steel_batch_stats", "steel_batch_configure",
-           "steel_batch_device_bytes", "steel_estimate_bytes", "steel_batch_step_timed", "steel_batch_profile_kernels"]
+           "steel_batch_device_bytes", "steel_estimate_bytes", "steel_batch_step_timed", "steel_batch_profile_kernels",
+           "steel_server_create", "steel_server_destroy", "steel_server_configure_lane", "steel_server_world_bytes",
+           "steel_server_admit", "steel_server_release", "steel_server_step_lane", "steel_server_lane_poll", "steel_server_read",
+           "steel_server_info"]
 
 
 class UnsupportedError(RuntimeError):
@@ -74,6 +77,16 @@ def load_library(path=LIB_PATH):
         "steel_estimate_bytes": ([U32, ctypes.POINTER(WorldDesc), ctypes.POINTER(SZ)], ctypes.c_int),
         "steel_batch_step_timed": ([V, U32, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "steel_batch_profile_kernels": ([V, U32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "steel_server_create": ([V, SZ, U32, ctypes.POINTER(V)], ctypes.c_int),
+        "steel_server_destroy": ([V], ctypes.c_int),
+        "steel_server_configure_lane": ([V, U32, U32], ctypes.c_int),
+        "steel_server_world_bytes": ([ctypes.POINTER(WorldDesc), ctypes.POINTER(SZ)], ctypes.c_int),
+        "steel_server_admit": ([V, ctypes.POINTER(WorldDesc), U32, ctypes.POINTER(U32)], ctypes.c_int),
+        "steel_server_release": ([V, U32], ctypes.c_int),
+        "steel_server_step_lane": ([V, U32], ctypes.c_int),
+        "steel_server_lane_poll": ([V, U32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "steel_server_read": ([V, U32, U32, SZ, SZ, V], ctypes.c_int),
+        "steel_server_info": ([V, ctypes.POINTER(SZ), ctypes.POINTER(SZ), ctypes.POINTER(U32)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -190,3 +190,68 @@ def quadruped_world(index, seed=SEED + 2, fps=200, substeps=4, iterations=12, km
     for _ in range(K):
         _add_box(b, rng, placed, (-1.5, -1.5, 0.2), (1.5, 1.5, 2.2), 0.6)
     return b.finalize(solver=dict(substeps=substeps, iterations=iterations))
+
+
+# --- serving requests ----------------------------------------------------------------------------------------
+_HULL_TEMPLATES = None
+
+
+def hull_templates(n=64, seed=SEED + 3):
+    """A fixed pool of convex hulls of 8-16 points on spheres of radius U[0.04, 0.12] (request hulls pick one)."""
+    global _HULL_TEMPLATES
+    if _HULL_TEMPLATES is None:
+        rng = PCG32(seed, 0)
+        out = []
+        for _ in range(n):
+            r = rng.uniform(0.04, 0.12)
+            npts = min(8 + int(rng.uniform(0, 9)), 16)
+            out.append([[r * c for c in rng.unit_vec()] for _ in range(npts)])
+        _HULL_TEMPLATES = out
+    return _HULL_TEMPLATES
+
+
+def _add_quadruped(b, x0, y0, rng, base_z=0.55):
+    base = b.addBody(xform=[x0, y0, base_z, 0, 0, 0, 1])
+    b.addShapeBox(base, hx=0.2, hy=0.05, hz=0.1, mu=0.6, density=1000.0)
+    axes = ([1, 0, 0], [0, 1, 0], [0, 1, 0])
+    for sx in (1, -1):
+        for sy in (1, -1):
+            parent = base
+            pxf = [sx * 0.2, sy * 0.05, -0.1, 0, 0, 0, 1]
+            for k in range(3):
+                body = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1])
+                b.addShapeCapsule(body, radius=0.02, halfHeight=0.075, mu=0.6, xform=[0, 0, -0.075, 0, 0, 0, 1])
+                b.addJointRevolute(parent, body, axis=axes[k], parentXform=pxf, q=rng.uniform(-0.3, 0.3))
+                parent, pxf = body, [0, 0, -0.15, 0, 0, 0, 1]
+
+
+def serving_request(index, seed=SEED + 3, fps=240, substeps=4, iterations=12, nmin=10, nmax=2000, hmin=50, hmax=1000):
+    """One serving request: (scene, horizon).  N ~ log-uniform [nmin, nmax] bodies, 30% hulls / 70% boxes,
+    plus 0-4 quadrupeds; horizon ~ U[hmin, hmax] frames.  Bodies start on a jittered 3D lattice (0.75 m cells,
+    4 layers from z = 0.4 m) so no two overlap; quadrupeds stand in a row beside the lattice."""
+    rng = PCG32(seed, 1000 + index)
+    n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+    nq = min(int(rng.uniform(0, 5)), 4)
+    horizon = int(rng.uniform(hmin, hmax + 1))
+    b = SceneBuilder(fps=fps)
+    b.addGroundPlane(mu=0.6)
+    layers = 4 if n >= 16 else 1
+    side = max(1, int(math.ceil(math.sqrt(n / layers))))
+    cell = 0.75
+    temps = hull_templates()
+    for i in range(n):
+        lz, rem = divmod(i, side * side)
+        ly, lx = divmod(rem, side)
+        p = [(lx - 0.5 * (side - 1)) * cell + rng.uniform(-0.02, 0.02), (ly - 0.5 * (side - 1)) * cell + rng.uniform(-0.02, 0.02),
+             0.4 + lz * cell + rng.uniform(0.0, 0.02)]
+        q = rng.unit_quat()
+        body = b.addBody(xform=p + q)
+        if rng.uniform() < 0.3:
+            b.addShapeConvexHull(body, vertices=temps[int(rng.uniform(0, len(temps))) % len(temps)], mu=0.6, density=1000.0)
+        else:
+            h = [rng.uniform(0.05, 0.2) for _ in range(3)]
+            b.addShapeBox(body, hx=h[0], hy=h[1], hz=h[2], mu=0.6, density=1000.0)
+    x_row = -0.5 * side * cell - 1.5
+    for k in range(nq):
+        _add_quadruped(b, x_row, (k - 1.5) * 1.2, rng)
+    return b.finalize(solver=dict(substeps=substeps, iterations=iterations)), horizon
