@@ -318,9 +318,8 @@ struct Prep {
     float ang = (float)k * 1.0471975511965976f;
     return make_uint2(fflip_u(fbits(dot(g.pos, mk2(cos_c(ang), sin_c(ang))))), g.pk);
   }
-  DEV uint32_t first_free_color(int a, int b) const {  // wgsl.js:1233-1246
-    uint32_t NCOLW = w.h(W_NCOLW);
-    uint32_t* M = w.sch(w.S.WMASK);
+  // wgsl.js:1233-1246; M: per-body color masks (NCOLW words per body)
+  static DEV uint32_t first_free_color(uint32_t* M, uint32_t NCOLW, uint32_t NCOL, int a, int b) {
     for (uint32_t k = 0; k < NCOLW; k++) {
       uint32_t used = 0;
       if (a >= 0) used |= M[(uint32_t)a * NCOLW + k];
@@ -332,7 +331,7 @@ struct Prep {
         return k * 32 + bit;
       }
     }
-    return w.NCOL();
+    return NCOL;
   }
   DEV uint32_t anchor_hash(uint32_t pair, uint32_t k) const { return max(hash_u32(pair * 2u + k), 1u); }
   DEV float4 row(uint32_t r, uint32_t k) const { return w.d.rows[r * w.ROWS() + k]; }
@@ -626,10 +625,16 @@ struct Prep {
   }
 };
 
+// LDS of prepare's coloring phase (words).
+static constexpr uint32_t PREP_CH = 256, PREP_MASK = 1024, PREP_WC = 258;
+
 template <int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ uint32_t s_w[BS / 64];
+  __shared__ uint32_t s_wr[PREP_CH], s_wk[PREP_CH];  // coloring: row and body-pair key of a chunk of walk entries
+  __shared__ uint32_t s_mask[PREP_MASK];              // coloring: per-body color masks when NB * NCOLW fits
+  __shared__ uint32_t s_wc[PREP_WC];                  // coloring: per-color counts and offsets when NCOL + 2 fits
   uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
   const WorldDev wd = worlds[wid];
@@ -826,25 +831,51 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     if (c < nc) sched[S.ROWOF + c] = r;
   }
   __syncthreads();
-  if (tid == 0) {
-    uint32_t ncu = min(nwalk, MAXC);
-    uint32_t* WC = sched + S.WCOUNT;
-    for (uint32_t i = 0; i < ncu; i++) {
-      uint32_t wc = sched[S.WALK + i];
+  // First-fit in walk order is serial by definition; lanes first gather each walk entry's row and body pair
+  // into LDS so lane 0 colors from LDS (body color masks in LDS when they fit).  Per-color counts, unit tags
+  // (sums and maxima) and the color-ordered row list (a stable scatter) are then built in parallel.
+  const bool mask_lds = max(NB, 1u) * NCOLW <= PREP_MASK;
+  uint32_t* CM = mask_lds ? s_mask : sched + S.WMASK;
+  if (mask_lds) for (uint32_t i = tid; i < max(NB, 1u) * NCOLW; i += BS) s_mask[i] = 0;
+  const bool wc_lds = NCOL + 2 <= PREP_WC;
+  uint32_t* WC = wc_lds ? s_wc : sched + S.WCOUNT;
+  if (wc_lds) for (uint32_t i = tid; i < NCOL + 2; i += BS) s_wc[i] = 0;
+  const uint32_t ncu = min(nwalk, MAXC);
+  for (uint32_t c0 = 0; c0 < ncu; c0 += PREP_CH) {
+    const uint32_t n = min(PREP_CH, ncu - c0);
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += BS) {
+      uint32_t wc = sched[S.WALK + c0 + i];
       uint32_t r = wc < MAXC ? sched[S.ROWOF + min(wc, MAXC - 1)] : 0xffffffffu;
-      if (r >= totp) continue;
-      uint32_t key = sched[S.UKEY + sched[S.RUNIT + min(r, MAXC - 1)]];
-      uint32_t col = P.first_free_color((int)(key & 0xffffu) - 1, (int)(key >> 16) - 1);
-      sched[S.PCOL + r] = col;
-      WC[col]++;
-      uint32_t& utp = sched[S.UTP + sched[S.RUNIT + r]];
-      utp = max(utp, ((col + 1) << 16) | wc);
+      s_wr[i] = r;
+      s_wk[i] = r < totp ? sched[S.UKEY + sched[S.RUNIT + min(r, MAXC - 1)]] : 0u;
     }
-    for (uint32_t sp = 0; sp < nu; sp++) {
-      uint32_t packed = sched[S.UTP + sp];
-      sched[S.UTC + sp] = packed >> 16;
-      sched[S.UTP + sp] = packed != 0 ? sched[S.ROWOF + (packed & 0xffffu)] : 0xffffffffu;
+    __syncthreads();
+    if (tid == 0) {
+      for (uint32_t i = 0; i < n; i++) {
+        uint32_t r = s_wr[i];
+        if (r >= totp) continue;
+        uint32_t key = s_wk[i];
+        sched[S.PCOL + r] = Prep::first_free_color(CM, NCOLW, NCOL, (int)(key & 0xffffu) - 1, (int)(key >> 16) - 1);
+      }
     }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < ncu; i += BS) {
+    uint32_t wc = sched[S.WALK + i];
+    uint32_t r = wc < MAXC ? sched[S.ROWOF + min(wc, MAXC - 1)] : 0xffffffffu;
+    if (r >= totp) continue;
+    uint32_t col = sched[S.PCOL + r];
+    atomicAdd(&WC[col], 1u);
+    atomicMax(&sched[S.UTP + sched[S.RUNIT + r]], ((col + 1) << 16) | wc);
+  }
+  __syncthreads();
+  for (uint32_t sp = tid; sp < nu; sp += BS) {
+    uint32_t packed = sched[S.UTP + sp];
+    sched[S.UTC + sp] = packed >> 16;
+    sched[S.UTP + sp] = packed != 0 ? sched[S.ROWOF + (packed & 0xffffu)] : 0xffffffffu;
+  }
+  if (tid == 0) {
     uint32_t acc = 0, ncol = 0;
     for (uint32_t c = 0; c <= NCOL; c++) {
       uint32_t n = WC[c];
@@ -853,10 +884,34 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     }
     sched[S.CSTART + NCOL + 1] = acc;
     sched[S.META + 1] = ncol;
-    for (uint32_t r = 0; r < totp; r++) {
-      uint32_t slot = WC[sched[S.PCOL + r]]++;
-      if (slot < MAXC) sched[S.PORD + slot] = r;
+  }
+  __syncthreads();
+  // Rows in increasing order within each color: wave 0 takes 64 rows at a time; a lane's slot is its color's
+  // running offset plus the number of lower lanes of the same color.
+  if (tid < 64) {
+    const uint32_t lane = tid;
+    for (uint32_t r0 = 0; r0 < totp; r0 += 64) {
+      const uint32_t r = r0 + lane;
+      const bool valid = r < totp;
+      const uint32_t col = valid ? sched[S.PCOL + r] : 0xffffffffu;
+      uint32_t rank = 0, cnt = 0;
+      bool last = true;
+      for (uint32_t j = 0; j < 64; j++) {
+        uint32_t cj = __shfl(col, j);
+        if (cj == col) { cnt++; if (j < lane) rank++; if (j > lane) last = false; }
+      }
+      const uint32_t base = valid ? WC[col] : 0u;
+      if (!wc_lds) __threadfence_block();
+      if (valid) {
+        uint32_t slot = base + rank;
+        if (slot < MAXC) sched[S.PORD + slot] = r;
+        if (last) WC[col] = base + cnt;
+      }
+      if (!wc_lds) __threadfence_block();
     }
+  }
+  __syncthreads();
+  if (tid == 0) {
     // overflow color by contact index (wgsl.js:1740-1780)
     uint32_t o0 = min(sched[S.CSTART + NCOL], MAXC), m = min(sched[S.CSTART + NCOL + 1], MAXC) - o0;
     for (uint32_t i = 1; i < m; i++) {
