@@ -32,6 +32,8 @@ struct steel_batch {
   uint32_t substeps;
   uint32_t block = 256;
   size_t solve_lds = 0;
+  uint32_t* pair_offs = nullptr;   // k_narrow's per-world pair offsets (n + 1 words)
+  uint32_t narrow_groups = 0;
   bool use_graph = true;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -102,6 +104,7 @@ static void batch_free(steel_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (int k = 0; k < K_COUNT; k++) if (b->arena[k]) (void)hipFree(b->arena[k]);
   if (b->d_worlds) (void)hipFree(b->d_worlds);
+  if (b->pair_offs) (void)hipFree(b->pair_offs);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -187,7 +190,9 @@ int steel_batch_create(steel_device* dev, uint32_t nworlds, const steel_world_de
     for (int k = 0; k < K_COUNT; k++) base[k] = b->arena[k] + b->off[k][i];
     wd[i] = world_dev_at(base);
   }
-  if (hipMalloc(&b->d_worlds, sizeof(WorldDev) * nworlds) != hipSuccess ||
+  b->narrow_groups = steel_narrow_groups((uint32_t)dev->prop.multiProcessorCount);
+  if (hipMalloc(&b->pair_offs, sizeof(uint32_t) * (nworlds + 1)) != hipSuccess ||
+      hipMalloc(&b->d_worlds, sizeof(WorldDev) * nworlds) != hipSuccess ||
       hipMemcpy(b->d_worlds, wd.data(), sizeof(WorldDev) * nworlds, hipMemcpyHostToDevice) != hipSuccess) {
     batch_free(b);
     delete b;
@@ -221,7 +226,7 @@ int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph) {
 }
 
 static int launch_frame(steel_batch* b) {
-  for (uint32_t s = 0; s < b->substeps; s++) HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->stream, STAGE_ALL));
+  for (uint32_t s = 0; s < b->substeps; s++) HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->pair_offs, b->narrow_groups, b->stream, STAGE_ALL));
   return STEEL_OK;
 }
 
@@ -252,7 +257,7 @@ int steel_batch_step(steel_batch* b, uint32_t frames) {
 int steel_batch_substep_stages(steel_batch* b, uint32_t stages) {
   if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
   HIPCHK(hipSetDevice(b->dev->ordinal));
-  HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->stream, stages));
+  HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->pair_offs, b->narrow_groups, b->stream, stages));
   return STEEL_OK;
 }
 
@@ -282,7 +287,7 @@ int steel_batch_profile_kernels(steel_batch* b, uint32_t substeps, double kernel
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   int rc = STEEL_OK;
   for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)
-    if (steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->stream, STAGE_ALL, &ev[4 * s]) != hipSuccess) rc = fail(STEEL_ERROR, "launch failed");
+    if (steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->pair_offs, b->narrow_groups, b->stream, STAGE_ALL, &ev[4 * s]) != hipSuccess) rc = fail(STEEL_ERROR, "launch failed");
   if (rc == STEEL_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = fail(STEEL_ERROR, "sync failed");
   for (int k = 0; k < 3; k++) kernel_ms[k] = 0.0;
   for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)

@@ -177,35 +177,55 @@ DEV bool pair_kept(const W& w, uint32_t i, uint32_t j) {
 #define STEEL_WPE_SOLVE 4
 #endif
 
+// Sorted shape bounds staged in LDS for the sweep when NS fits (BROAD_LDS shapes).
+static constexpr uint32_t BROAD_LDS = 1024;
+
+// Shapes and broadphase, one workgroup per world: sort shapes by the order-preserving integer of AABB min x,
+// then sweep the sorted list; pair (i, j) survives iff it passes the reference's all-pairs predicate
+// (wgsl.js:154-177), tested in full for every candidate the sweep reaches.
 template <int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_collide(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t stage_mask) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_broad(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ uint32_t s_hist[256];
   __shared__ uint32_t s_cnt[(BS / 64) * 256];
   __shared__ uint64_t s_red[2 * (BS / 64)];
-  __shared__ uint32_t s_np, s_nc, s_tie;
+  __shared__ uint32_t s_np;
+  __shared__ float4 s_lo[BROAD_LDS], s_hi[BROAD_LDS];  // sorted AABBs; .w holds the shape id
   uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
   const WorldDev wd = worlds[wid];
   load_header(Hs, wd);
   W w = make_world(Hs, wd);
   const uint32_t tid = threadIdx.x, NS = w.NS();
-  if (tid == 0) { s_np = 0; s_nc = 0; s_tie = 0; }
+  if (tid == 0) s_np = 0;
   for (uint32_t s = tid; s < NS; s += BS) shape_pose(w, s);
   __syncthreads();
-  // Broadphase: sort shapes by the order-preserving integer of AABB min x, then sweep the sorted list;
-  // pair (i, j) survives iff it passes the reference's all-pairs predicate.
-  uint32_t PA = w.h(W_P_AABB);
+  const uint32_t PA = w.h(W_P_AABB);
   for (uint32_t s = tid; s < NS; s += BS) { wd.skey[s] = fflip_u(fbits(wd.poses[PA + s * 2].x)); wd.sval[s] = s; }
   __syncthreads();
   block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), NS, s_hist, s_cnt, s_red);
-  uint32_t MAXP = w.MAXP();
+  const bool lds = NS <= BROAD_LDS;
+  if (lds) {
+    for (uint32_t i = tid; i < NS; i += BS) {
+      uint32_t si = wd.sval[i];
+      float4 lo = wd.poses[PA + si * 2], hi = wd.poses[PA + si * 2 + 1];
+      lo.w = ubits(si);
+      s_lo[i] = lo; s_hi[i] = hi;
+    }
+    __syncthreads();
+  }
+  const uint32_t MAXP = w.MAXP();
   for (uint32_t i = tid; i < NS; i += BS) {
-    uint32_t si = wd.sval[i];
-    float hi = wd.poses[PA + si * 2 + 1].x;
+    float4 lo_i = lds ? s_lo[i] : wd.poses[PA + wd.sval[i] * 2];
+    float4 hi_i = lds ? s_hi[i] : wd.poses[PA + wd.sval[i] * 2 + 1];
+    const uint32_t si = lds ? fbits(lo_i.w) : wd.sval[i];
     for (uint32_t j = i + 1; j < NS; j++) {
-      uint32_t sj = wd.sval[j];
-      if (wd.poses[PA + sj * 2].x > hi) break;
+      float4 lo_j = lds ? s_lo[j] : wd.poses[PA + wd.sval[j] * 2];
+      if (lo_j.x > hi_i.x) break;
+      float4 hi_j = lds ? s_hi[j] : wd.poses[PA + wd.sval[j] * 2 + 1];
+      // cheap bounds test first; pair_kept repeats it with the rest of the predicate
+      if (lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.y > hi_i.y || lo_j.z > hi_i.z) continue;
+      const uint32_t sj = lds ? fbits(lo_j.w) : wd.sval[j];
       uint32_t a = min(si, sj), b = max(si, sj);
       if (!pair_kept(w, a, b)) continue;
       uint32_t slot = atomicAdd(&s_np, 1u);
@@ -213,52 +233,105 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     }
   }
   __syncthreads();
-  uint32_t np = min(s_np, MAXP);
-  if (stage_mask & 1u) {
-    // Narrowphase: one wave64 per candidate pair.
-    Collider col;
-    col.w = w;
-    col.s_nc = &s_nc;
-    uint32_t lane = tid & 63, wave = tid >> 6;
-    for (uint32_t p = wave; p < np; p += BS / 64) col.narrow_pair(p, lane);
-    __syncthreads();
-    // Canonical order: (body pair, feature key) by the radix sort; equal pairs of those (a feature-hash
-    // collision) are ordered by (point bits, emission index) afterwards, as the reference's key does.
-    uint32_t nc = min(s_nc, w.MAXC());
-    for (uint32_t c = tid; c < nc; c += BS) {
-      float4 m = wd.craw[c * 6 + 2];
-      uint32_t pair = (uint32_t)(ti(m.x) + 1) | ((uint32_t)(ti(m.y) + 1) << 16);
-      uint32_t feat = tu(m.z) | (tu(m.w) << 16);
-      wd.skey[c] = ((uint64_t)pair << 32) | feat;
-      wd.sval[c] = c;
+  if (tid == 0) { wd.ctr[0] = s_np; wd.ctr[1] = 0; }
+}
+
+// Exclusive prefix of the worlds' pair counts (min(ctr[0], MAXP)) into offs[0..n], one block.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_pair_offsets(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t* __restrict__ offs) {
+  __shared__ uint32_t s_w[BS / 64];
+  const uint32_t per = (nworlds + BS - 1) / BS;
+  const uint32_t lo_ = min(threadIdx.x * per, nworlds), hi = min(lo_ + per, nworlds);
+  uint32_t sum = 0;
+  for (uint32_t i = lo_; i < hi; i++) sum += min(worlds[i].ctr[0], worlds[i].hdr[W_MAXP]);
+  uint32_t total;
+  uint32_t off = block_excl_scan<BS>(sum, s_w, total);
+  for (uint32_t i = lo_; i < hi; i++) { offs[i] = off; off += min(worlds[i].ctr[0], worlds[i].hdr[W_MAXP]); }
+  if (threadIdx.x == 0) offs[nworlds] = total;
+}
+
+// Narrowphase over every candidate pair of every world: one wave64 per pair (wgsl.js:1063-1195).  The
+// grid is fixed; wave g takes a contiguous chunk of the pairs laid end to end in world order (offs), so a
+// world with many pairs is spread over many waves and CUs.  Contacts take slots from the world's counter
+// ctr[1]; their order is made canonical by k_csort.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_narrow(const WorldDev* __restrict__ worlds, uint32_t nworlds, const uint32_t* __restrict__ offs) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t G = gridDim.x * 4, g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t total = offs[nworlds];
+  const uint32_t chunk = (total + G - 1) / G;
+  const uint32_t i0 = g * chunk, i1 = min(i0 + chunk, total);
+  if (i0 >= i1) return;
+  uint32_t lo = 0, hi = nworlds;  // largest w with offs[w] <= i0 (then offs[w + 1] > i0, so w < nworlds)
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= i0) lo = mid; else hi = mid - 1;
+  }
+  uint32_t w = lo, wend = offs[w + 1];
+  Collider col;
+  WorldDev wd = worlds[w];
+  col.w = make_world(wd.hdr, wd);
+  col.s_nc = wd.ctr + 1;
+  for (uint32_t i = i0; i < i1; i++) {
+    if (i >= wend) {
+      do { w++; wend = offs[w + 1]; } while (i >= wend);
+      wd = worlds[w];
+      col.w = make_world(wd.hdr, wd);
+      col.s_nc = wd.ctr + 1;
     }
-    __syncthreads();
-    block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), nc, s_hist, s_cnt, s_red);
-    for (uint32_t i = tid; i + 1 < nc; i += BS) if (wd.skey[i] == wd.skey[i + 1]) s_tie = 1;
-    __syncthreads();
-    if (s_tie && tid == 0) {
-      auto bits_of = [&](uint32_t c) {
-        float4 p = wd.craw[c * 6];
-        return hash_u32(hash_u32(hash_u32(fbits(p.x)) ^ fbits(p.y)) ^ fbits(p.z)) ^ fbits(p.w);
-      };
-      for (uint32_t i = 1; i < nc; i++) {
-        uint64_t k = wd.skey[i];
-        uint32_t v = wd.sval[i];
-        uint32_t bv = bits_of(v);
-        int j = (int)i - 1;
-        while (j >= 0 && wd.skey[j] == k && (bits_of(wd.sval[j]) > bv || (bits_of(wd.sval[j]) == bv && wd.sval[j] > v))) {
-          wd.skey[j + 1] = wd.skey[j]; wd.sval[j + 1] = wd.sval[j]; j--;
-        }
-        wd.skey[j + 1] = k; wd.sval[j + 1] = v;
+    col.narrow_pair(i - offs[w], lane);
+  }
+}
+
+// Canonical contact order, one workgroup per world: (body pair, feature key) by the LDS radix sort; equal
+// pairs of those (a feature-hash collision) are ordered by (point bits, emission index) afterwards, as the
+// reference's key does (wgsl.js:3906-3980).
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_csort(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ uint32_t s_hist[256];
+  __shared__ uint32_t s_cnt[(BS / 64) * 256];
+  __shared__ uint64_t s_red[2 * (BS / 64)];
+  __shared__ uint32_t s_tie;
+  uint32_t wid = blockIdx.x;
+  if (wid >= nworlds) return;
+  const WorldDev wd = worlds[wid];
+  load_header(Hs, wd);
+  W w = make_world(Hs, wd);
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_tie = 0;
+  const uint32_t nc = min(wd.ctr[1], w.MAXC());
+  for (uint32_t c = tid; c < nc; c += BS) {
+    float4 m = wd.craw[c * 6 + 2];
+    uint32_t pair = (uint32_t)(ti(m.x) + 1) | ((uint32_t)(ti(m.y) + 1) << 16);
+    uint32_t feat = tu(m.z) | (tu(m.w) << 16);
+    wd.skey[c] = ((uint64_t)pair << 32) | feat;
+    wd.sval[c] = c;
+  }
+  __syncthreads();
+  block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), nc, s_hist, s_cnt, s_red);
+  for (uint32_t i = tid; i + 1 < nc; i += BS) if (wd.skey[i] == wd.skey[i + 1]) s_tie = 1;
+  __syncthreads();
+  if (s_tie && tid == 0) {
+    auto bits_of = [&](uint32_t c) {
+      float4 p = wd.craw[c * 6];
+      return hash_u32(hash_u32(hash_u32(fbits(p.x)) ^ fbits(p.y)) ^ fbits(p.z)) ^ fbits(p.w);
+    };
+    for (uint32_t i = 1; i < nc; i++) {
+      uint64_t k = wd.skey[i];
+      uint32_t v = wd.sval[i];
+      uint32_t bv = bits_of(v);
+      int j = (int)i - 1;
+      while (j >= 0 && wd.skey[j] == k && (bits_of(wd.sval[j]) > bv || (bits_of(wd.sval[j]) == bv && wd.sval[j] > v))) {
+        wd.skey[j + 1] = wd.skey[j]; wd.sval[j + 1] = wd.sval[j]; j--;
       }
-    }
-    __syncthreads();
-    for (uint32_t i = tid; i < nc * 6; i += BS) {
-      uint32_t r = i / 6, q = i % 6;
-      wd.contacts[r * 6 + q] = wd.craw[wd.sval[r] * 6 + q];
+      wd.skey[j + 1] = k; wd.sval[j + 1] = v;
     }
   }
-  if (tid == 0) { wd.ctr[0] = s_np; wd.ctr[1] = s_nc; }
+  __syncthreads();
+  for (uint32_t i = tid; i < nc * 6; i += BS) {
+    uint32_t r = i / 6, q = i % 6;
+    wd.contacts[r * 6 + q] = wd.craw[wd.sval[r] * 6 + q];
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1604,13 +1677,18 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
 // ------------------------------------------------------------------------------------------------
 size_t steel_solve_lds_bytes(uint32_t NL, uint32_t ND) { return solve_lds_bytes(NL, ND); }
 
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, size_t solve_lds, hipStream_t stream,
-                                uint32_t stages, hipEvent_t* marks) {
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, size_t solve_lds, uint32_t* pair_offs,
+                                uint32_t narrow_groups, hipStream_t stream, uint32_t stages, hipEvent_t* marks) {
   if (nworlds == 0) return hipSuccess;
   dim3 grid(nworlds);
   if (marks) (void)hipEventRecord(marks[0], stream);
   if (stages & STAGE_COLLIDE) {
-    hipLaunchKernelGGL((k_collide<256>), grid, dim3(256), 0, stream, worlds, nworlds, (stages & STAGE_NARROW) ? 1u : 0u);
+    hipLaunchKernelGGL((k_broad<256>), grid, dim3(256), 0, stream, worlds, nworlds);
+    if (stages & STAGE_NARROW) {
+      hipLaunchKernelGGL((k_pair_offsets<1024>), dim3(1), dim3(1024), 0, stream, worlds, nworlds, pair_offs);
+      hipLaunchKernelGGL(k_narrow, dim3(narrow_groups), dim3(256), 0, stream, worlds, nworlds, (const uint32_t*)pair_offs);
+      hipLaunchKernelGGL((k_csort<256>), grid, dim3(256), 0, stream, worlds, nworlds);
+    }
   }
   if (marks) (void)hipEventRecord(marks[1], stream);
   if (stages & STAGE_PREPARE) {

@@ -16,7 +16,12 @@ size_t steel_solve_lds_bytes(uint32_t NL, uint32_t ND);
 // Upper bound of k_solve's dynamic LDS (worlds needing more are refused at batch creation).
 static constexpr size_t STEEL_SOLVE_LDS_MAX = 64 * 1024;
 
+// Workgroups of k_narrow per launch for a device of `cus` compute units (four 4-wave groups per CU).
+inline uint32_t steel_narrow_groups(uint32_t cus) { return cus * 4; }
+
 // solve_lds: dynamic LDS of k_solve, the maximum of steel_solve_lds_bytes over the batch's worlds.
+// pair_offs: device scratch of nworlds + 1 words (per-world pair offsets for k_narrow).
+// narrow_groups: k_narrow's grid (steel_narrow_groups).
 // marks (optional): four events recorded before collide, prepare, solve and after solve.
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, size_t solve_lds, hipStream_t stream,
-                                uint32_t stages, hipEvent_t* marks = nullptr);
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, size_t solve_lds, uint32_t* pair_offs,
+                                uint32_t narrow_groups, hipStream_t stream, uint32_t stages, hipEvent_t* marks = nullptr);

@@ -65,6 +65,7 @@ struct Lane {
   uint32_t nframe = 0;                // worlds in the last launched frame
   WorldDev* d_table = nullptr;
   WorldDev* h_table = nullptr;        // pinned staging of the table
+  uint32_t* pair_offs = nullptr;      // k_narrow's per-world pair offsets (cap + 1 words)
   uint32_t cap = 0;
   uint32_t substeps = 0;              // set by the lane's first world
   uint32_t block = 64;
@@ -145,6 +146,7 @@ int steel_server_destroy(steel_server* s) {
     if (L.stream) (void)hipStreamSynchronize(L.stream);
     if (L.d_table) (void)hipFree(L.d_table);
     if (L.h_table) (void)hipHostFree(L.h_table);
+    if (L.pair_offs) (void)hipFree(L.pair_offs);
     if (L.ev_start) (void)hipEventDestroy(L.ev_start);
     if (L.ev_end) (void)hipEventDestroy(L.ev_end);
     if (L.stream) (void)hipStreamDestroy(L.stream);
@@ -243,9 +245,11 @@ int steel_server_step_lane(steel_server* s, uint32_t lane) {
       uint32_t cap = std::max<uint32_t>(n, std::max<uint32_t>(256, L.cap * 2));
       if (L.d_table) HIPCHK(hipFree(L.d_table));
       if (L.h_table) HIPCHK(hipHostFree(L.h_table));
-      L.d_table = nullptr; L.h_table = nullptr; L.cap = 0;
+      if (L.pair_offs) HIPCHK(hipFree(L.pair_offs));
+      L.d_table = nullptr; L.h_table = nullptr; L.pair_offs = nullptr; L.cap = 0;
       HIPCHK(hipMalloc(&L.d_table, sizeof(WorldDev) * cap));
       HIPCHK(hipHostMalloc(&L.h_table, sizeof(WorldDev) * cap));
+      HIPCHK(hipMalloc(&L.pair_offs, sizeof(uint32_t) * (cap + 1)));
       L.cap = cap;
     }
     L.solve_lds = 0;
@@ -260,7 +264,8 @@ int steel_server_step_lane(steel_server* s, uint32_t lane) {
   L.nframe = n;
   if (n == 0) return STEEL_OK;
   HIPCHK(hipEventRecord(L.ev_start, L.stream));
-  for (uint32_t k = 0; k < L.substeps; k++) HIPCHK(steel_launch_substep(L.d_table, n, L.block, L.solve_lds, L.stream, STAGE_ALL));
+  for (uint32_t k = 0; k < L.substeps; k++) HIPCHK(steel_launch_substep(L.d_table, n, L.block, L.solve_lds, L.pair_offs,
+                                                                       steel_narrow_groups((uint32_t)s->dev->prop.multiProcessorCount), L.stream, STAGE_ALL));
   HIPCHK(hipEventRecord(L.ev_end, L.stream));
   L.in_flight = true;
   return STEEL_OK;
