@@ -31,7 +31,7 @@ struct steel_batch {
   uint32_t n;
   uint32_t substeps;
   uint32_t block = 256;
-  size_t solve_lds = 0;
+  SolveLds sl;                     // k_solve LDS over the batch's worlds
   uint32_t* pair_offs = nullptr;   // k_narrow's per-world pair offsets (n + 1 words)
   uint32_t narrow_groups = 0;
   bool use_graph = true;
@@ -153,7 +153,8 @@ int steel_batch_create(steel_device* dev, uint32_t nworlds, const steel_world_de
     if (rc) { delete b; return rc; }
     for (int k = 0; k < K_COUNT; k++) { b->off[k][i] = b->total[k]; b->size[k][i] = s.b[k]; b->total[k] += align256(s.b[k]); }
     std::memcpy(&b->headers[(size_t)i * STEEL_WORLD_HEADER_WORDS], w.header, STEEL_WORLD_HEADER_WORDS * 4);
-    b->solve_lds = std::max(b->solve_lds, steel_solve_lds_bytes(w.header[W_NL], w.header[W_ND]));
+    rc = world_solve_lds(w.header, w.statF, b->sl);
+    if (rc) { delete b; return rc; }
   }
   for (int k = 0; k < K_COUNT; k++) {
     if (hipMalloc(&b->arena[k], std::max<size_t>(b->total[k], 256)) != hipSuccess) {
@@ -226,7 +227,7 @@ int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph) {
 }
 
 static int launch_frame(steel_batch* b) {
-  for (uint32_t s = 0; s < b->substeps; s++) HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->pair_offs, b->narrow_groups, b->stream, STAGE_ALL));
+  for (uint32_t s = 0; s < b->substeps; s++) HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->sl, b->pair_offs, b->narrow_groups, b->stream, STAGE_ALL));
   return STEEL_OK;
 }
 
@@ -257,7 +258,7 @@ int steel_batch_step(steel_batch* b, uint32_t frames) {
 int steel_batch_substep_stages(steel_batch* b, uint32_t stages) {
   if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
   HIPCHK(hipSetDevice(b->dev->ordinal));
-  HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->pair_offs, b->narrow_groups, b->stream, stages));
+  HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->sl, b->pair_offs, b->narrow_groups, b->stream, stages));
   return STEEL_OK;
 }
 
@@ -287,7 +288,7 @@ int steel_batch_profile_kernels(steel_batch* b, uint32_t substeps, double kernel
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   int rc = STEEL_OK;
   for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)
-    if (steel_launch_substep(b->d_worlds, b->n, b->block, b->solve_lds, b->pair_offs, b->narrow_groups, b->stream, STAGE_ALL, &ev[4 * s]) != hipSuccess) rc = fail(STEEL_ERROR, "launch failed");
+    if (steel_launch_substep(b->d_worlds, b->n, b->block, b->sl, b->pair_offs, b->narrow_groups, b->stream, STAGE_ALL, &ev[4 * s]) != hipSuccess) rc = fail(STEEL_ERROR, "launch failed");
   if (rc == STEEL_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = fail(STEEL_ERROR, "sync failed");
   for (int k = 0; k < 3; k++) kernel_ms[k] = 0.0;
   for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)

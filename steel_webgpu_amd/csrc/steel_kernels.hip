@@ -1015,46 +1015,57 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
 // ------------------------------------------------------------------------------------------------
 // k_solve (wgsl.js:2501-3891)
 // ------------------------------------------------------------------------------------------------
-// Per-link mirror of the fields the solve sweeps touch, in LDS for the whole kernel (SOLVE_LB floats per link):
-// S, U, D^-1 (36 each, the link block's layout), p, dv, v, u, tau (6 each), COM (3).  Loaded from the link
-// blocks (art[A_LINK + l*LB]) at kernel start; p, dv, v, u, tau are written back after the sweeps.
-static constexpr uint32_t SOLVE_LB = 144, M_S = 0, M_U = 36, M_DI = 72, M_P = 108, M_DV = 114, M_V = 120, M_UU = 126,
-                          M_TAU = 132, M_COM = 138;
-__host__ __device__ inline uint32_t mirror_src(uint32_t f) {  // link-block offset of mirror field f (LB = none)
-  if (f < 36) return L_S + f;
-  if (f < 72) return L_U + f - 36;
-  if (f < 108) return L_DI + f - 72;
-  if (f < 114) return L_P + f - 108;
-  if (f < 120) return L_DV + f - 114;
-  if (f < 126) return L_V + f - 120;
-  if (f < 132) return L_UU + f - 126;
-  if (f < 138) return L_TAU + f - 132;
-  if (f < 141) return L_COM + f - 138;
+// Per-link mirror of the fields the solve sweeps touch, in LDS for the whole kernel.  A link of n dofs takes
+// mirror_floats(n) = n^2 + 14n + 21 floats: S (6n), U (6n), D^-1 (n x n), p, dv, v (6 each), u, tau (n each),
+// COM (3).  Loaded from the link blocks (art[A_LINK + l*LB]) at kernel start; p, dv, v, u, tau are written
+// back after the sweeps.
+// Link-block offset of mirror field f of a link with n dofs (the world's link blocks use stride JM); LB: none.
+__host__ __device__ inline uint32_t mirror_src(uint32_t f, uint32_t n, uint32_t JM) {
+  if (f < 6 * n) return L_S + f;
+  f -= 6 * n;
+  if (f < 6 * n) return L_U + f;
+  f -= 6 * n;
+  if (f < n * n) return L_DI + (f / n) * JM + f % n;
+  f -= n * n;
+  if (f < 6) return L_P + f;
+  f -= 6;
+  if (f < 6) return L_DV + f;
+  f -= 6;
+  if (f < 6) return L_V + f;
+  f -= 6;
+  if (f < n) return L_UU + f;
+  f -= n;
+  if (f < n) return L_TAU + f;
+  f -= n;
+  if (f < 3) return L_COM + f;
   return LB;
 }
-// Dynamic LDS of k_solve: NL * SOLVE_LB floats, NL uint4 link records (parent, ndof, first dof, body), ND floats
-// of solver joint velocities.
-__host__ __device__ inline size_t solve_lds_bytes(uint32_t NL, uint32_t ND) {
-  return (size_t)NL * SOLVE_LB * 4 + (size_t)NL * 16 + (size_t)ND * 4;
-}
-
+// Mirror fields written back after the sweeps (p, dv, v, u, tau).
+__host__ __device__ inline bool mirror_mutable(uint32_t f, uint32_t n) { return f >= 12 * n + n * n && f < 12 * n + n * n + 18 + 2 * n; }
 struct Solver {
   W w;
   uint32_t parity;
   float* lm;      // LDS link mirror
-  uint4* lmeta;   // LDS link records
+  uint4* lmeta;   // LDS link records: (parent, ndof | mirror offset << 8, first dof, body)
   float* laqd;    // LDS joint velocities (the reference's aqd)
-  DEV float* ml(uint32_t l) const { return lm + l * SOLVE_LB; }
+  float4* tw;     // body twists (LDS, or the world's buffer for worlds too large for LDS)
+  float4* imp;    // accumulated impulses on articulated bodies (same placement as tw)
   DEV int mpar(uint32_t l) const { return (int)lmeta[l].x; }
-  DEV uint32_t mnd(uint32_t l) const { return lmeta[l].y; }
+  DEV uint32_t mnd(uint32_t l) const { return lmeta[l].y & 0xffu; }
   DEV uint32_t mdof(uint32_t l) const { return lmeta[l].z; }
   DEV uint32_t mbody(uint32_t l) const { return lmeta[l].w; }
+  DEV float* ml(uint32_t l) const { return lm + (lmeta[l].y >> 8); }
   static DEV SVd msv(const float* p) { return mksv(mk3(p[0], p[1], p[2]), mk3(p[3], p[4], p[5])); }
   static DEV void mst(float* p, SVd v) { p[0] = v.a.x; p[1] = v.a.y; p[2] = v.a.z; p[3] = v.l.x; p[4] = v.l.y; p[5] = v.l.z; }
-  DEV SVd mS(uint32_t l, uint32_t k) const { return msv(ml(l) + M_S + k * 6); }
-  DEV SVd mU(uint32_t l, uint32_t k) const { return msv(ml(l) + M_U + k * 6); }
-  DEV float mDI(uint32_t l, uint32_t r, uint32_t c) const { return ml(l)[M_DI + r * w.JM() + c]; }
-  DEV f3 mcom(uint32_t l) const { const float* p = ml(l) + M_COM; return mk3(p[0], p[1], p[2]); }
+  DEV SVd mS(uint32_t l, uint32_t k) const { return msv(ml(l) + k * 6); }
+  DEV SVd mU(uint32_t l, uint32_t k) const { return msv(ml(l) + 6 * mnd(l) + k * 6); }
+  DEV float mDI(uint32_t l, uint32_t r, uint32_t c) const { uint32_t n = mnd(l); return ml(l)[12 * n + r * n + c]; }
+  DEV float* mP(uint32_t l) const { uint32_t n = mnd(l); return ml(l) + 12 * n + n * n; }
+  DEV float* mDV(uint32_t l) const { return mP(l) + 6; }
+  DEV float* mV(uint32_t l) const { return mP(l) + 12; }
+  DEV float* mUU(uint32_t l) const { return mP(l) + 18; }
+  DEV float* mTAU(uint32_t l) const { return mP(l) + 18 + mnd(l); }
+  DEV f3 mcom(uint32_t l) const { const float* p = mP(l) + 18 + 2 * mnd(l); return mk3(p[0], p[1], p[2]); }
   DEV f3 mr(uint32_t l) const {  // link_r from the mirror: COM(l) - COM(parent), zero at a root
     int par = mpar(l);
     if (par < 0) return mk3(0, 0, 0);
@@ -1063,14 +1074,14 @@ struct Solver {
   DEV float4 row(uint32_t r, uint32_t k) const { return w.d.rows[r * w.ROWS() + k]; }
   DEV void load_tw(int b, f3 out[2]) const {
     if (b < 0) { out[0] = mk3(0, 0, 0); out[1] = mk3(0, 0, 0); return; }
-    out[0] = xyz(w.d.tw[(uint32_t)b * 2]); out[1] = xyz(w.d.tw[(uint32_t)b * 2 + 1]);
+    out[0] = xyz(tw[(uint32_t)b * 2]); out[1] = xyz(tw[(uint32_t)b * 2 + 1]);
   }
   DEV void add_imp(int b, f3 f, f3 t) const {
     if (b < 0) return;
     if (w.body_kind(b) != 2) return;
-    float4 i0 = w.d.imp[(uint32_t)b * 2], i1 = w.d.imp[(uint32_t)b * 2 + 1];
-    w.d.imp[(uint32_t)b * 2] = make_float4(i0.x + f.x, i0.y + f.y, i0.z + f.z, i0.w + 0.0f);
-    w.d.imp[(uint32_t)b * 2 + 1] = make_float4(i1.x + t.x, i1.y + t.y, i1.z + t.z, i1.w + 0.0f);
+    float4 i0 = imp[(uint32_t)b * 2], i1 = imp[(uint32_t)b * 2 + 1];
+    imp[(uint32_t)b * 2] = make_float4(i0.x + f.x, i0.y + f.y, i0.z + f.z, i0.w + 0.0f);
+    imp[(uint32_t)b * 2 + 1] = make_float4(i1.x + t.x, i1.y + t.y, i1.z + t.z, i1.w + 0.0f);
   }
   static DEV f2 friction_pair_solve(float a0, float c0, float d0, float r0, float r1, float old0, float old1, float radius, float omega) {
     if (radius <= 0.0f) return mk2(0, 0);
@@ -1204,12 +1215,12 @@ struct Solver {
       }
       R[6] = lam;
     }
-    if (a >= 0) { w.d.tw[(uint32_t)a * 2] = F4(ta[0], 0.0f); w.d.tw[(uint32_t)a * 2 + 1] = F4(ta[1], 0.0f); }
-    if (b >= 0) { w.d.tw[(uint32_t)b * 2] = F4(tb[0], 0.0f); w.d.tw[(uint32_t)b * 2 + 1] = F4(tb[1], 0.0f); }
+    if (a >= 0) { tw[(uint32_t)a * 2] = F4(ta[0], 0.0f); tw[(uint32_t)a * 2 + 1] = F4(ta[1], 0.0f); }
+    if (b >= 0) { tw[(uint32_t)b * 2] = F4(tb[0], 0.0f); tw[(uint32_t)b * 2 + 1] = F4(tb[1], 0.0f); }
   }
   DEV void set_link_twist(uint32_t l, SVd v) const {
     uint32_t b = mbody(l);
-    w.d.tw[b * 2] = F4(v.l, 0.0f); w.d.tw[b * 2 + 1] = F4(v.a, 0.0f);
+    tw[b * 2] = F4(v.l, 0.0f); tw[b * 2 + 1] = F4(v.a, 0.0f);
   }
   DEVN void art_twists(uint32_t a) const {  // wgsl.js:3240-3251
     uint4 ai = w.art_info(a);
@@ -1217,9 +1228,9 @@ struct Solver {
       uint32_t l = ai.x + k;
       int par = mpar(l);
       SVd v = sv0();
-      if (par >= 0) v = mshift(msv(ml((uint32_t)par) + M_V), mr(l));
+      if (par >= 0) v = mshift(msv(mV((uint32_t)par)), mr(l));
       for (uint32_t r = 0; r < mnd(l); r++) v = sadd(v, sscale(mS(l, r), laqd[mdof(l) + r]));
-      mst(ml(l) + M_V, v);
+      mst(mV(l), v);
       set_link_twist(l, v);
     }
   }
@@ -1227,11 +1238,11 @@ struct Solver {
     uint4 ai = w.art_info(a);
     for (uint32_t kk = 0; kk < ai.y; kk++) {
       uint32_t l = ai.x + ai.y - 1 - kk;
-      float* L = ml(l);
       uint32_t n = mnd(l);
-      SVd p = msv(L + M_P);
+      float *UU = mUU(l), *TAU = mTAU(l);
+      SVd p = msv(mP(l));
       float u[6];
-      for (uint32_t r = 0; r < n; r++) { u[r] = L[M_TAU + r] - sdot(mS(l, r), p); L[M_UU + r] = u[r]; }
+      for (uint32_t r = 0; r < n; r++) { u[r] = TAU[r] - sdot(mS(l, r), p); UU[r] = u[r]; }
       int par = mpar(l);
       if (par >= 0) {
         SVd pp = p;
@@ -1240,29 +1251,30 @@ struct Solver {
           for (uint32_t c = 0; c < n; c++) du += mDI(l, r, c) * u[c];
           pp = sadd(pp, sscale(mU(l, r), du));
         }
-        float* P0 = ml((uint32_t)par) + M_P;
+        float* P0 = mP((uint32_t)par);
         mst(P0, sadd(msv(P0), fshift(pp, mr(l))));
       }
     }
     for (uint32_t k = 0; k < ai.y; k++) {
       uint32_t l = ai.x + k;
-      float* L = ml(l);
       int par = mpar(l);
       SVd dv = sv0();
-      if (par >= 0) dv = mshift(msv(ml((uint32_t)par) + M_DV), mr(l));
+      if (par >= 0) dv = mshift(msv(mDV((uint32_t)par)), mr(l));
       uint32_t n = mnd(l);
+      const float* UU = mUU(l);
       float dq[6];
       for (uint32_t r = 0; r < n; r++) {
         dq[r] = 0.0f;
-        for (uint32_t c = 0; c < n; c++) dq[r] += mDI(l, r, c) * (L[M_UU + c] - sdot(mU(l, c), dv));
+        for (uint32_t c = 0; c < n; c++) dq[r] += mDI(l, r, c) * (UU[c] - sdot(mU(l, c), dv));
       }
       for (uint32_t r = 0; r < n; r++) { dv = sadd(dv, sscale(mS(l, r), dq[r])); laqd[mdof(l) + r] += dq[r]; }
-      mst(L + M_DV, dv);
-      SVd v = sadd(msv(L + M_V), dv);
-      mst(L + M_V, v);
+      mst(mDV(l), dv);
+      SVd v = sadd(msv(mV(l)), dv);
+      mst(mV(l), v);
       set_link_twist(l, v);
-      mst(L + M_P, sv0());
-      for (uint32_t r = 0; r < w.JM(); r++) L[M_TAU + r] = 0.0f;
+      mst(mP(l), sv0());
+      float* TAU = mTAU(l);
+      for (uint32_t r = 0; r < n; r++) TAU[r] = 0.0f;
     }
   }
   // subgroupAdd of the reference's 32-lane path (wgsl.js:3305), order fixed as an xor butterfly over strides
@@ -1339,11 +1351,11 @@ struct Solver {
     bool hit = false;
     for (uint32_t k = 0; k < ai.y; k++) {
       uint32_t l = ai.x + k, b = mbody(l);
-      f3 f = xyz(w.d.imp[b * 2]), t = xyz(w.d.imp[b * 2 + 1]);
+      f3 f = xyz(imp[b * 2]), t = xyz(imp[b * 2 + 1]);
       if (vne0(f) || vne0(t)) {
         hit = true;
-        mst(ml(l) + M_P, mksv(-t, -f));
-        w.d.imp[b * 2] = make_float4(0, 0, 0, 0); w.d.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
+        mst(mP(l), mksv(-t, -f));
+        imp[b * 2] = make_float4(0, 0, 0, 0); imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
       }
     }
     if (hit) art_apply(a);
@@ -1369,7 +1381,7 @@ struct Solver {
           float nl = mx(lam - omega * (sg * laqd[dd] + rhs) / hinv, 0.0f);
           if (nl == lam) continue;
           art_[w.h(W_A_LIM) + 2 * dd + side] = nl;
-          ml(l)[M_TAU + r] = sg * (nl - lam);
+          mTAU(l)[r] = sg * (nl - lam);
           art_apply(a);
         }
       }
@@ -1476,9 +1488,10 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
   __syncthreads();
 }
 
-template <int BS>
+template <int BS, bool TWL>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_SOLVE))) k_solve(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ uint32_t s_w[BS / 64];
   uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
   const WorldDev wd = worlds[wid];
@@ -1496,21 +1509,53 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const f3 g = mk3(w.hf(W_GX), w.hf(W_GY), w.hf(W_GZ));
   const uint32_t nu = sched[S.META], ncol = min(sched[S.META + 1], NCOL);
   const uint32_t nd = min(sched[S.META + 3], w.h(W_MAXDENSE));
-  // LDS: link mirror, link records, joint velocities (sized by the launch: solve_lds_bytes of the batch's
-  // largest world).
-  extern __shared__ float s_dyn[];
-  sv.lm = s_dyn;
-  sv.lmeta = (uint4*)(s_dyn + (size_t)NL * SOLVE_LB);
-  sv.laqd = (float*)(sv.lmeta + NL);
-  for (uint32_t i = tid; i < NL * SOLVE_LB; i += BS) {
-    uint32_t l = i / SOLVE_LB, src = mirror_src(i - l * SOLVE_LB);
+  // Dynamic LDS (steel_solve_lds_bytes of the launch's largest world): link records, joint velocities,
+  // with TWL the body twists and impulses, then the link mirror.
+  extern __shared__ float4 s_dyn4[];
+  char* lds = (char*)s_dyn4;
+  sv.lmeta = (uint4*)lds;
+  lds += (size_t)NL * 16;
+  sv.laqd = (float*)lds;
+  lds += ((size_t)ND * 4 + 15) & ~(size_t)15;
+  if (TWL) {
+    sv.tw = (float4*)lds;
+    sv.imp = sv.tw + 2 * (size_t)max(NB, 1u);
+    lds += (size_t)max(NB, 1u) * 64;
+  } else {
+    sv.tw = wd.tw;
+    sv.imp = wd.imp;
+  }
+  sv.lm = (float*)lds;
+  // Link records and mirror offsets (exclusive prefix of mirror_floats(ndof) over links).
+  uint32_t* msz = (uint32_t*)sv.lm;
+  for (uint32_t l = tid; l < NL; l += BS) {
+    uint32_t n = w.link_ndof(l);
+    sv.lmeta[l] = make_uint4((uint32_t)w.link_parent(l), n, w.link_dof(l), (uint32_t)w.link_body(l));
+    msz[l] = mirror_floats(n);
+  }
+  __syncthreads();
+  const uint32_t MF = block_scan_array<BS>(msz, NL, s_w);
+  for (uint32_t l = tid; l < NL; l += BS) sv.lmeta[l].y |= msz[l] << 8;
+  __syncthreads();
+  const uint32_t JMw = w.JM();
+  auto mirror_link = [&](uint32_t i) {  // link holding mirror float i (largest l with offset <= i)
+    uint32_t lo = 0, hi = NL - 1;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi + 1) >> 1;
+      if ((sv.lmeta[mid].y >> 8) <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  };
+  for (uint32_t i = tid; i < MF; i += BS) {
+    uint32_t l = mirror_link(i), n = sv.mnd(l), src = mirror_src(i - (sv.lmeta[l].y >> 8), n, JMw);
     sv.lm[i] = src < LB ? art_[w.lo(l) + src] : 0.0f;
   }
-  for (uint32_t l = tid; l < NL; l += BS) sv.lmeta[l] = make_uint4((uint32_t)w.link_parent(l), w.link_ndof(l), w.link_dof(l), (uint32_t)w.link_body(l));
   for (uint32_t i = tid; i < ND; i += BS) sv.laqd[i] = art_[w.h(W_A_QDH) + i];
   for (uint32_t b = tid; b < NB; b += BS) {
     uint32_t kind = w.body_kind((int)b);
-    wd.imp[b * 2] = make_float4(0, 0, 0, 0); wd.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
+    sv.imp[b * 2] = make_float4(0, 0, 0, 0); sv.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
+    // Bodies other than free ones: zero (static bodies keep it; link bodies get theirs from art_twists below).
+    sv.tw[b * 2] = make_float4(0, 0, 0, 0); sv.tw[b * 2 + 1] = make_float4(0, 0, 0, 0);
     if (kind != 1) continue;
     uint32_t fl = w.body_flags((int)b);
     f3 v = xyz(wd.vel[b * 2]) + (fl != 0 ? mk3(0, 0, 0) : g * dt);
@@ -1527,7 +1572,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       m3 R = qmat(w.pose_q((int)b));
       wv = mul(R, sv.gyroscopic_velocity(I, mul(tr(R), wv), dt));
     }
-    wd.tw[b * 2] = F4(v, 0.0f); wd.tw[b * 2 + 1] = F4(wv, 0.0f);
+    sv.tw[b * 2] = F4(v, 0.0f); sv.tw[b * 2 + 1] = F4(wv, 0.0f);
   }
   for (uint32_t i = tid; i < 2 * ND; i += BS) art_[w.h(W_A_LIM) + i] = 0.0f;
   for (uint32_t i = tid; i < ND; i += BS) { art_[w.h(W_A_FRIC) + i] = 0.0f; art_[w.h(W_A_DRV) + i] = 0.0f; }
@@ -1545,9 +1590,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const uint32_t iters = w.h(W_ITERS);
   for (uint32_t it = 0; it < iters; it++) sweep<BS>(sv, 1, ncol, nu, nd);
   // Mutable mirror fields back to the link blocks (p, dv, v, u, tau; S, U, D^-1, COM are unchanged).
-  for (uint32_t i = tid; i < NL * SOLVE_LB; i += BS) {
-    uint32_t l = i / SOLVE_LB, f = i - l * SOLVE_LB;
-    if (f >= M_P && f < M_COM) art_[w.lo(l) + mirror_src(f)] = sv.lm[i];
+  for (uint32_t i = tid; i < MF; i += BS) {
+    uint32_t l = mirror_link(i), n = sv.mnd(l), f = i - (sv.lmeta[l].y >> 8);
+    if (mirror_mutable(f, n)) art_[w.lo(l) + mirror_src(f, n, JMw)] = sv.lm[i];
   }
   __syncthreads();
   const uint32_t nrows = sched[S.USTART + nu];
@@ -1559,7 +1604,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   }
   for (uint32_t b = tid; b < NB; b += BS) {
     if (w.body_kind((int)b) != 1) continue;
-    f3 v = xyz(wd.tw[b * 2]), wv = xyz(wd.tw[b * 2 + 1]);
+    f3 v = xyz(sv.tw[b * 2]), wv = xyz(sv.tw[b * 2 + 1]);
     float damp = (w.body_flags((int)b) & 1) != 0 ? 1.0f : 1.0f - w.body_limits((int)b).x * dt;
     wd.vel[b * 2] = F4(v, 0.0f); wd.vel[b * 2 + 1] = F4(wv * damp, 0.0f);
     f3 c = xyz(wd.poses[PB + b * 2]) + v * dt;
@@ -1675,10 +1720,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
 // ------------------------------------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------------------------------------
-size_t steel_solve_lds_bytes(uint32_t NL, uint32_t ND) { return solve_lds_bytes(NL, ND); }
-
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, size_t solve_lds, uint32_t* pair_offs,
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, const SolveLds& sl, uint32_t* pair_offs,
                                 uint32_t narrow_groups, hipStream_t stream, uint32_t stages, hipEvent_t* marks) {
+  const bool twl = sl.tw_lds(block);
+  const size_t solve_lds = sl.bytes(block);
   if (nworlds == 0) return hipSuccess;
   dim3 grid(nworlds);
   if (marks) (void)hipEventRecord(marks[0], stream);
@@ -1698,9 +1743,16 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
   }
   if (marks) (void)hipEventRecord(marks[2], stream);
   if (stages & STAGE_SOLVE) {
-    if (block == 64) hipLaunchKernelGGL((k_solve<64>), grid, dim3(64), solve_lds, stream, worlds, nworlds);
-    else if (block == 128) hipLaunchKernelGGL((k_solve<128>), grid, dim3(128), solve_lds, stream, worlds, nworlds);
-    else hipLaunchKernelGGL((k_solve<256>), grid, dim3(256), solve_lds, stream, worlds, nworlds);
+    if (block == 64) {
+      if (twl) hipLaunchKernelGGL((k_solve<64, true>), grid, dim3(64), solve_lds, stream, worlds, nworlds);
+      else hipLaunchKernelGGL((k_solve<64, false>), grid, dim3(64), solve_lds, stream, worlds, nworlds);
+    } else if (block == 128) {
+      if (twl) hipLaunchKernelGGL((k_solve<128, true>), grid, dim3(128), solve_lds, stream, worlds, nworlds);
+      else hipLaunchKernelGGL((k_solve<128, false>), grid, dim3(128), solve_lds, stream, worlds, nworlds);
+    } else {
+      if (twl) hipLaunchKernelGGL((k_solve<256, true>), grid, dim3(256), solve_lds, stream, worlds, nworlds);
+      else hipLaunchKernelGGL((k_solve<256, false>), grid, dim3(256), solve_lds, stream, worlds, nworlds);
+    }
   }
   if (marks) (void)hipEventRecord(marks[3], stream);
   return hipGetLastError();

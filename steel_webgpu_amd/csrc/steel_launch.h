@@ -11,17 +11,38 @@ enum : uint32_t {
   STAGE_ALL = 15u,
 };
 
-// Dynamic LDS bytes k_solve needs for a world of NL links and ND dofs (its link mirror).
-size_t steel_solve_lds_bytes(uint32_t NL, uint32_t ND);
-// Upper bound of k_solve's dynamic LDS (worlds needing more are refused at batch creation).
+// k_solve's per-link LDS mirror: a link of n dofs takes n^2 + 14n + 21 floats.
+__host__ __device__ inline uint32_t mirror_floats(uint32_t n) { return n * n + 14 * n + 21; }
+// k_solve's dynamic LDS for a world: link records (NL uint4), joint velocities (ND floats), with tw_lds the
+// twists and impulses of its NB bodies (64 bytes each), and the link mirror (mfloats floats).
+__host__ __device__ inline size_t solve_lds_bytes(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, bool tw_lds) {
+  size_t b = (size_t)NL * 16 + (((size_t)ND * 4 + 15) & ~(size_t)15);
+  if (tw_lds) b += (size_t)(NB > 0 ? NB : 1) * 64;
+  return b + (((size_t)mfloats * 4 + 15) & ~(size_t)15);
+}
+// Upper bound of k_solve's dynamic LDS without the twists (worlds needing more are refused at creation).
 static constexpr size_t STEEL_SOLVE_LDS_MAX = 64 * 1024;
+
+// k_solve's LDS over the worlds of one launch: the largest world's bytes without and with the twists in LDS.
+// Twists go to LDS when that still lets four waves per SIMD stay resident (160 KB of LDS per CU shared by
+// 16 waves: 16 / (block / 64) workgroups).
+struct SolveLds {
+  size_t notw = 0, tw = 0;
+  void add(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB) {
+    notw = notw > solve_lds_bytes(NL, ND, mfloats, NB, false) ? notw : solve_lds_bytes(NL, ND, mfloats, NB, false);
+    tw = tw > solve_lds_bytes(NL, ND, mfloats, NB, true) ? tw : solve_lds_bytes(NL, ND, mfloats, NB, true);
+  }
+  static size_t budget(uint32_t block) { return (size_t)160 * 1024 / (16 / (block / 64)) - 1024; }
+  bool tw_lds(uint32_t block) const { return tw <= budget(block); }
+  size_t bytes(uint32_t block) const { return tw_lds(block) ? tw : notw; }
+};
 
 // Workgroups of k_narrow per launch for a device of `cus` compute units (four 4-wave groups per CU).
 inline uint32_t steel_narrow_groups(uint32_t cus) { return cus * 4; }
 
-// solve_lds: dynamic LDS of k_solve, the maximum of steel_solve_lds_bytes over the batch's worlds.
+// sl: k_solve's LDS over the launch's worlds.
 // pair_offs: device scratch of nworlds + 1 words (per-world pair offsets for k_narrow).
 // narrow_groups: k_narrow's grid (steel_narrow_groups).
 // marks (optional): four events recorded before collide, prepare, solve and after solve.
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, size_t solve_lds, uint32_t* pair_offs,
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, const SolveLds& sl, uint32_t* pair_offs,
                                 uint32_t narrow_groups, hipStream_t stream, uint32_t stages, hipEvent_t* marks = nullptr);

@@ -32,8 +32,6 @@ inline int world_sizes(const uint32_t* h, Sizes& s) {
   if (h[W_NDA] > 64 || h[W_MAXLP] > 64) return fail(STEEL_BAD_ARGUMENT, "articulations of more than 64 dofs or links are not supported");
   if (h[W_MAXPATH] > 64 || h[W_MAXPATH] * h[W_JM] > 192) return fail(STEEL_BAD_ARGUMENT, "articulation root paths deeper than 64 links are not supported");
   if (h[W_UH] > MAXC || h[W_MAXU] > MAXC) return fail(STEEL_BAD_ARGUMENT, "unit capacities exceed the contact capacity");
-  if (steel_solve_lds_bytes(h[W_NL], ND) > STEEL_SOLVE_LDS_MAX)
-    return fail(STEEL_UNSUPPORTED, "a world of %u articulation links exceeds the solve kernel's LDS link mirror", h[W_NL]);
   SchedLayout L = sched_layout(MAXC, h[W_MAXU], h[W_UH], h[W_NCOLW] * 32, NB, h[W_NCOLW], h[W_NL]);
   s.b[K_HDR] = STEEL_WORLD_HEADER_WORDS * 4;
   s.b[K_STATF] = (size_t)h[W_NSTATF] * 16;
@@ -91,6 +89,22 @@ inline WorldDev world_dev_at(char* const* base) {
   x.imp = (float4*)base[K_IMP];
   x.aqd = (float*)base[K_AQD];
   return x;
+}
+
+// Floats of k_solve's link mirror for a world: the sum over its links of mirror_floats(ndof) (statF link
+// records: ndof in the .w of the second float4).
+inline uint32_t world_mirror_floats(const uint32_t* h, const float* statF) {
+  uint32_t t = 0;
+  for (uint32_t l = 0; l < h[W_NL]; l++) t += mirror_floats((uint32_t)statF[((size_t)h[W_SF_LINK] + l * 6 + 1) * 4 + 3]);
+  return t;
+}
+// Adds the world to sl; refuses worlds whose mirror does not fit k_solve's LDS.
+inline int world_solve_lds(const uint32_t* h, const float* statF, SolveLds& sl) {
+  uint32_t mf = world_mirror_floats(h, statF);
+  if (solve_lds_bytes(h[W_NL], h[W_ND], mf, h[W_NB], false) > STEEL_SOLVE_LDS_MAX)
+    return fail(STEEL_UNSUPPORTED, "a world of %u articulation links exceeds the solve kernel's LDS link mirror", h[W_NL]);
+  sl.add(h[W_NL], h[W_ND], mf, h[W_NB]);
+  return STEEL_OK;
 }
 
 // Kinds cleared to zero when a world starts or restarts.

@@ -53,7 +53,7 @@ struct ServerWorld {
   size_t off = 0, bytes = 0;         // pool block
   size_t koff[K_COUNT], ksize[K_COUNT];
   WorldDev dev;
-  size_t solve_lds = 0;
+  uint32_t NL = 0, ND = 0, NB = 0, mfloats = 0;  // k_solve LDS inputs
 };
 
 struct Lane {
@@ -69,7 +69,7 @@ struct Lane {
   uint32_t cap = 0;
   uint32_t substeps = 0;              // set by the lane's first world
   uint32_t block = 64;
-  size_t solve_lds = 0;
+  SolveLds sl;                        // k_solve LDS over the lane's worlds
   std::vector<uint32_t> pending_free; // released while a frame may still use them
 };
 
@@ -184,6 +184,9 @@ int steel_server_admit(steel_server* s, const steel_world_desc* w, uint32_t lane
   Sizes sz;
   int rc = world_sizes(w->header, sz);
   if (rc) return rc;
+  SolveLds probe;
+  rc = world_solve_lds(w->header, w->statF, probe);
+  if (rc) return rc;
   ServerWorld W;
   W.lane = lane;
   size_t t = 0;
@@ -209,7 +212,8 @@ int steel_server_admit(steel_server* s, const steel_world_desc* w, uint32_t lane
     return fail(STEEL_ERROR, "world upload failed: %s", hipGetErrorString(e));
   }
   W.dev = world_dev_at(base);
-  W.solve_lds = steel_solve_lds_bytes(w->header[W_NL], w->header[W_ND]);
+  W.NL = w->header[W_NL]; W.ND = w->header[W_ND]; W.NB = w->header[W_NB];
+  W.mfloats = world_mirror_floats(w->header, w->statF);
   uint32_t wid = s->next_id++;
   s->worlds[wid] = W;
   L.substeps = S;
@@ -252,11 +256,11 @@ int steel_server_step_lane(steel_server* s, uint32_t lane) {
       HIPCHK(hipMalloc(&L.pair_offs, sizeof(uint32_t) * (cap + 1)));
       L.cap = cap;
     }
-    L.solve_lds = 0;
+    L.sl = SolveLds();
     for (uint32_t i = 0; i < n; i++) {
       const ServerWorld& W = s->worlds[L.members[i]];
       L.h_table[i] = W.dev;
-      L.solve_lds = std::max(L.solve_lds, W.solve_lds);
+      L.sl.add(W.NL, W.ND, W.mfloats, W.NB);
     }
     if (n) HIPCHK(hipMemcpyAsync(L.d_table, L.h_table, sizeof(WorldDev) * n, hipMemcpyHostToDevice, L.stream));
     L.dirty = false;
@@ -264,7 +268,7 @@ int steel_server_step_lane(steel_server* s, uint32_t lane) {
   L.nframe = n;
   if (n == 0) return STEEL_OK;
   HIPCHK(hipEventRecord(L.ev_start, L.stream));
-  for (uint32_t k = 0; k < L.substeps; k++) HIPCHK(steel_launch_substep(L.d_table, n, L.block, L.solve_lds, L.pair_offs,
+  for (uint32_t k = 0; k < L.substeps; k++) HIPCHK(steel_launch_substep(L.d_table, n, L.block, L.sl, L.pair_offs,
                                                                        steel_narrow_groups((uint32_t)s->dev->prop.multiProcessorCount), L.stream, STAGE_ALL));
   HIPCHK(hipEventRecord(L.ev_end, L.stream));
   L.in_flight = true;
