@@ -1047,6 +1047,8 @@ struct Solver {
   uint32_t parity;
   float* lm;      // LDS link mirror
   uint4* lmeta;   // LDS link records: (parent, ndof | mirror offset << 8, first dof, body)
+  uint4* ltree;   // LDS link tree: (depth, highest-index child, next lower sibling, -)
+  float* lcon;    // LDS per-link contribution to the parent's bias force (8 floats per link)
   float* laqd;    // LDS joint velocities (the reference's aqd)
   float4* tw;     // body twists (LDS, or the world's buffer for worlds too large for LDS)
   float4* imp;    // accumulated impulses on articulated bodies (same placement as tw)
@@ -1222,59 +1224,90 @@ struct Solver {
     uint32_t b = mbody(l);
     tw[b * 2] = F4(v.l, 0.0f); tw[b * 2 + 1] = F4(v.a, 0.0f);
   }
-  DEVN void art_twists(uint32_t a) const {  // wgsl.js:3240-3251
+  // Orders LDS and memory between the lanes of one wave (the tree passes below run a wave per articulation).
+  static DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  // Tree passes over an articulation, one wave, lane k holding link ai.x + k, depth levels in turn.  Each
+  // link's arithmetic is the serial pass's; where the serial backward pass adds children into a parent in
+  // decreasing link order, the parent here sums its children's contributions in that same order, so the
+  // results are bit for bit those of the serial passes (wgsl.js:3240-3299).
+  DEVN void art_twists(uint32_t a, uint32_t lane) const {  // wgsl.js:3240-3251
     uint4 ai = w.art_info(a);
-    for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k;
-      int par = mpar(l);
-      SVd v = sv0();
-      if (par >= 0) v = mshift(msv(mV((uint32_t)par)), mr(l));
-      for (uint32_t r = 0; r < mnd(l); r++) v = sadd(v, sscale(mS(l, r), laqd[mdof(l) + r]));
-      mst(mV(l), v);
-      set_link_twist(l, v);
+    const uint32_t MP = w.h(W_MAXPATH);
+    const bool has = lane < ai.y;
+    const uint32_t l = ai.x + lane, dl = has ? ltree[l].x : 0xffffffffu;
+    wave_sync();
+    for (uint32_t d = 0; d < MP; d++) {
+      if (dl == d) {
+        int par = mpar(l);
+        SVd v = sv0();
+        if (par >= 0) v = mshift(msv(mV((uint32_t)par)), mr(l));
+        for (uint32_t r = 0; r < mnd(l); r++) v = sadd(v, sscale(mS(l, r), laqd[mdof(l) + r]));
+        mst(mV(l), v);
+        set_link_twist(l, v);
+      }
+      wave_sync();
     }
   }
-  DEVN void art_apply(uint32_t a) const {  // wgsl.js:3254-3299
+  DEVN void art_apply(uint32_t a, uint32_t lane) const {  // wgsl.js:3254-3299
     uint4 ai = w.art_info(a);
-    for (uint32_t kk = 0; kk < ai.y; kk++) {
-      uint32_t l = ai.x + ai.y - 1 - kk;
-      uint32_t n = mnd(l);
-      float *UU = mUU(l), *TAU = mTAU(l);
-      SVd p = msv(mP(l));
-      float u[6];
-      for (uint32_t r = 0; r < n; r++) { u[r] = TAU[r] - sdot(mS(l, r), p); UU[r] = u[r]; }
-      int par = mpar(l);
-      if (par >= 0) {
-        SVd pp = p;
-        for (uint32_t r = 0; r < n; r++) {
-          float du = 0.0f;
-          for (uint32_t c = 0; c < n; c++) du += mDI(l, r, c) * u[c];
-          pp = sadd(pp, sscale(mU(l, r), du));
+    const uint32_t MP = w.h(W_MAXPATH);
+    const bool has = lane < ai.y;
+    const uint32_t l = ai.x + lane, dl = has ? ltree[l].x : 0xffffffffu;
+    wave_sync();
+    for (int d = (int)MP - 1; d >= 0; d--) {
+      if (dl == (uint32_t)d) {
+        uint32_t n = mnd(l);
+        float *UU = mUU(l), *TAU = mTAU(l);
+        SVd p = msv(mP(l));
+        float u[6];
+        for (uint32_t r = 0; r < n; r++) { u[r] = TAU[r] - sdot(mS(l, r), p); UU[r] = u[r]; }
+        if (mpar(l) >= 0) {
+          SVd pp = p;
+          for (uint32_t r = 0; r < n; r++) {
+            float du = 0.0f;
+            for (uint32_t c = 0; c < n; c++) du += mDI(l, r, c) * u[c];
+            pp = sadd(pp, sscale(mU(l, r), du));
+          }
+          mst(lcon + l * 8, fshift(pp, mr(l)));
         }
-        float* P0 = mP((uint32_t)par);
-        mst(P0, sadd(msv(P0), fshift(pp, mr(l))));
       }
+      wave_sync();
+      if (d > 0 && dl == (uint32_t)(d - 1)) {
+        uint32_t c = ltree[l].y;
+        if (c != 0xffffffffu) {
+          SVd P = msv(mP(l));
+          for (; c != 0xffffffffu; c = ltree[c].z) P = sadd(P, msv(lcon + c * 8));
+          mst(mP(l), P);
+        }
+      }
+      wave_sync();
     }
-    for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k;
-      int par = mpar(l);
-      SVd dv = sv0();
-      if (par >= 0) dv = mshift(msv(mDV((uint32_t)par)), mr(l));
-      uint32_t n = mnd(l);
-      const float* UU = mUU(l);
-      float dq[6];
-      for (uint32_t r = 0; r < n; r++) {
-        dq[r] = 0.0f;
-        for (uint32_t c = 0; c < n; c++) dq[r] += mDI(l, r, c) * (UU[c] - sdot(mU(l, c), dv));
+    for (uint32_t d = 0; d < MP; d++) {
+      if (dl == d) {
+        int par = mpar(l);
+        SVd dv = sv0();
+        if (par >= 0) dv = mshift(msv(mDV((uint32_t)par)), mr(l));
+        uint32_t n = mnd(l);
+        const float* UU = mUU(l);
+        float dq[6];
+        for (uint32_t r = 0; r < n; r++) {
+          dq[r] = 0.0f;
+          for (uint32_t c = 0; c < n; c++) dq[r] += mDI(l, r, c) * (UU[c] - sdot(mU(l, c), dv));
+        }
+        for (uint32_t r = 0; r < n; r++) { dv = sadd(dv, sscale(mS(l, r), dq[r])); laqd[mdof(l) + r] += dq[r]; }
+        mst(mDV(l), dv);
+        SVd v = sadd(msv(mV(l)), dv);
+        mst(mV(l), v);
+        set_link_twist(l, v);
+        mst(mP(l), sv0());
+        float* TAU = mTAU(l);
+        for (uint32_t r = 0; r < n; r++) TAU[r] = 0.0f;
       }
-      for (uint32_t r = 0; r < n; r++) { dv = sadd(dv, sscale(mS(l, r), dq[r])); laqd[mdof(l) + r] += dq[r]; }
-      mst(mDV(l), dv);
-      SVd v = sadd(msv(mV(l)), dv);
-      mst(mV(l), v);
-      set_link_twist(l, v);
-      mst(mP(l), sv0());
-      float* TAU = mTAU(l);
-      for (uint32_t r = 0; r < n; r++) TAU[r] = 0.0f;
+      wave_sync();
     }
   }
   // subgroupAdd of the reference's 32-lane path (wgsl.js:3305), order fixed as an xor butterfly over strides
@@ -1346,11 +1379,11 @@ struct Solver {
     }
     return touched;
   }
-  DEVN void art_contact_impulses(uint32_t a) const {
+  DEVN void art_contact_impulses(uint32_t a, uint32_t lane) const {  // wgsl.js:3380-3394
     uint4 ai = w.art_info(a);
     bool hit = false;
-    for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k, b = mbody(l);
+    if (lane < ai.y) {
+      uint32_t l = ai.x + lane, b = mbody(l);
       f3 f = xyz(imp[b * 2]), t = xyz(imp[b * 2 + 1]);
       if (vne0(f) || vne0(t)) {
         hit = true;
@@ -1358,9 +1391,11 @@ struct Solver {
         imp[b * 2] = make_float4(0, 0, 0, 0); imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
       }
     }
-    if (hit) art_apply(a);
+    if (__ballot(hit) != 0) art_apply(a, lane);
   }
-  DEVN void art_propagate_rows(uint32_t a) const {  // joint-limit rows (wgsl.js:3404-3461)
+  // Joint-limit rows (wgsl.js:3404-3461), one wave: every lane computes the same row update (all inputs are
+  // uniform) and writes the same values; lane 0 sets the link's tau before the wave's art_apply.
+  DEVN void art_propagate_rows(uint32_t a, uint32_t lane) const {
     uint4 ai = w.art_info(a);
     float* art_ = w.d.art;
     uint32_t n = tu(art_[w.h(W_A_RLN) + a]);
@@ -1381,8 +1416,8 @@ struct Solver {
           float nl = mx(lam - omega * (sg * laqd[dd] + rhs) / hinv, 0.0f);
           if (nl == lam) continue;
           art_[w.h(W_A_LIM) + 2 * dd + side] = nl;
-          mTAU(l)[r] = sg * (nl - lam);
-          art_apply(a);
+          if (lane == 0) mTAU(l)[r] = sg * (nl - lam);
+          art_apply(a, lane);
         }
       }
     }
@@ -1448,7 +1483,7 @@ DEV void dense_phase(const Solver& sv, uint32_t mode, uint32_t nd) {
     bool touched = false;
     if (a < NA) touched = sv.art_dense(a, mode, nd, lane);
     __syncthreads();
-    if (lane == 0 && a < NA && touched) sv.art_twists(a);
+    if (a < NA && touched) sv.art_twists(a, lane);
     __syncthreads();
   }
 }
@@ -1459,7 +1494,8 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
   const uint32_t tid = threadIdx.x, NA = w.NA(), NCOL = w.NCOL(), PO = w.PO();
   const uint32_t* sched = w.d.sched;
   const SchedLayout& S = w.S;
-  for (uint32_t a = tid; a < NA; a += BS) sv.art_propagate_rows(a);
+  const uint32_t lane = tid & 63, wave = tid >> 6;
+  for (uint32_t a = wave; a < NA; a += BS / 64) sv.art_propagate_rows(a, lane);
   if (w.h(W_PATCH) && mode == 1) {
     for (uint32_t us = tid; us < nu; us += BS) {
       uint32_t s0 = sched[S.USTART + us];
@@ -1484,7 +1520,7 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
     }
   }
   __syncthreads();
-  for (uint32_t a = tid; a < NA; a += BS) sv.art_contact_impulses(a);
+  for (uint32_t a = wave; a < NA; a += BS / 64) sv.art_contact_impulses(a, lane);
   __syncthreads();
 }
 
@@ -1515,6 +1551,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   char* lds = (char*)s_dyn4;
   sv.lmeta = (uint4*)lds;
   lds += (size_t)NL * 16;
+  sv.ltree = (uint4*)lds;
+  lds += (size_t)NL * 16;
+  sv.lcon = (float*)lds;
+  lds += (size_t)NL * 32;
   sv.laqd = (float*)lds;
   lds += ((size_t)ND * 4 + 15) & ~(size_t)15;
   if (TWL) {
@@ -1536,6 +1576,19 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   __syncthreads();
   const uint32_t MF = block_scan_array<BS>(msz, NL, s_w);
   for (uint32_t l = tid; l < NL; l += BS) sv.lmeta[l].y |= msz[l] << 8;
+  __syncthreads();
+  // Link tree: depth by walking to the root; children lists in decreasing link order (built in link order).
+  for (uint32_t l = tid; l < NL; l += BS) {
+    uint32_t d = 0;
+    for (int p = sv.mpar(l); p >= 0; p = sv.mpar((uint32_t)p)) d++;
+    sv.ltree[l] = make_uint4(d, 0xffffffffu, 0xffffffffu, 0u);
+  }
+  __syncthreads();
+  if (tid == 0)
+    for (uint32_t l = 0; l < NL; l++) {
+      int p = sv.mpar(l);
+      if (p >= 0) { sv.ltree[l].z = sv.ltree[p].y; sv.ltree[p].y = l; }
+    }
   __syncthreads();
   const uint32_t JMw = w.JM();
   auto mirror_link = [&](uint32_t i) {  // link holding mirror float i (largest l with offset <= i)
@@ -1583,7 +1636,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     slots[u] = make_uint4(r, us, (uint32_t)((int)(key & 0xffffu) - 1), (uint32_t)((int)(key >> 16) - 1));
   }
   __syncthreads();
-  for (uint32_t a = tid; a < NA; a += BS) sv.art_twists(a);
+  for (uint32_t a = tid >> 6; a < NA; a += BS / 64) sv.art_twists(a, tid & 63);
   __syncthreads();
   dense_phase<BS>(sv, 0, nd);
   sweep<BS>(sv, 0, ncol, nu, nd);

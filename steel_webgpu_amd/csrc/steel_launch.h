@@ -13,10 +13,11 @@ enum : uint32_t {
 
 // k_solve's per-link LDS mirror: a link of n dofs takes n^2 + 14n + 21 floats.
 __host__ __device__ inline uint32_t mirror_floats(uint32_t n) { return n * n + 14 * n + 21; }
-// k_solve's dynamic LDS for a world: link records (NL uint4), joint velocities (ND floats), with tw_lds the
+// k_solve's dynamic LDS for a world: link records and tree (2 NL uint4), per-link contributions (8 NL floats),
+// joint velocities (ND floats), with tw_lds the
 // twists and impulses of its NB bodies (64 bytes each), and the link mirror (mfloats floats).
 __host__ __device__ inline size_t solve_lds_bytes(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, bool tw_lds) {
-  size_t b = (size_t)NL * 16 + (((size_t)ND * 4 + 15) & ~(size_t)15);
+  size_t b = (size_t)NL * 64 + (((size_t)ND * 4 + 15) & ~(size_t)15);
   if (tw_lds) b += (size_t)(NB > 0 ? NB : 1) * 64;
   return b + (((size_t)mfloats * 4 + 15) & ~(size_t)15);
 }
