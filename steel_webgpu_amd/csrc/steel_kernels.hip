@@ -708,6 +708,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   __shared__ uint32_t s_wr[PREP_CH], s_wk[PREP_CH];  // coloring: row and body-pair key of a chunk of walk entries
   __shared__ uint32_t s_mask[PREP_MASK];              // coloring: per-body color masks when NB * NCOLW fits
   __shared__ uint32_t s_wc[PREP_WC];                  // coloring: per-color counts and offsets when NCOL + 2 fits
+  __shared__ uint32_t s_tree[(BS / 64) * 64];        // factor: link tree of each wave's articulation
+  __shared__ float s_pab[(BS / 64) * 512];           // factor: children's bias forces
   uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
   const WorldDev wd = worlds[wid];
@@ -749,8 +751,16 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       if (rowf && w.scalar_dof(dd)) { art_[w.h(W_A_RL) + ai.z + n] = (float)dd; n++; }
     }
     art_[w.h(W_A_RLN) + a] = (float)n;
-    w.art_fk(a);
-    w.art_factor(a);
+  }
+  // FK and articulated-body factor, one wave per articulation, tree levels in parallel lanes.
+  for (uint32_t base = 0; base < NA; base += BS / 64) {
+    const uint32_t a = base + (tid >> 6), k = tid & 63;
+    if (a < NA) {
+      uint32_t* tree = s_tree + (tid >> 6) * 64;
+      w.art_tree(a, tree);
+      w.art_fk_lv(a, k < w.art_info(a).y ? (tree[k] & 0xffu) : 0xffu);
+      w.art_factor_lv(a, tree, s_pab + (tid >> 6) * 512);
+    }
   }
   __syncthreads();
   // ---- Phase 1: classes, run heads and dense slots
@@ -1674,11 +1684,14 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     uint32_t dd = w.h(W_A_QD) + w.link_dof(l);
     art_[dd + 3] *= k; art_[dd + 4] *= k; art_[dd + 5] *= k;
   }
-  for (uint32_t a = tid; a < NA; a += BS) {
-    uint4 ai = w.art_info(a);
-    uint32_t AQ = w.h(W_A_Q);
-    for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k, qs = w.link_qs(l), dd = w.link_dof(l), t = w.link_type(l);
+  // Joint positions (per link), then FK and body poses, one wave per articulation.
+  for (uint32_t base = 0; base < NA; base += BS / 64) {
+    const uint32_t a = base + (tid >> 6), k = tid & 63;
+    if (a >= NA) continue;
+    const uint4 ai = w.art_info(a);
+    const uint32_t AQ = w.h(W_A_Q), l = ai.x + k;
+    if (k < ai.y) {
+      uint32_t qs = w.link_qs(l), dd = w.link_dof(l), t = w.link_type(l);
       if (t == JT_REVOLUTE || t == JT_PRISMATIC || t == JT_D6)
         for (uint32_t r = 0; r < w.link_ndof(l); r++) art_[AQ + qs + r] += sv.laqd[dd + r] * dt;
       if (t == JT_BALL) {
@@ -1695,9 +1708,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
         art_[AQ + qs + 3] = q.x; art_[AQ + qs + 4] = q.y; art_[AQ + qs + 5] = q.z; art_[AQ + qs + 6] = q.w;
       }
     }
-    w.art_fk(a);
-    for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k, b = (uint32_t)w.link_body(l);
+    w.art_fk_lv(a, k < ai.y ? sv.ltree[l].x : 0xffffffffu);
+    if (k < ai.y) {
+      uint32_t b = (uint32_t)w.link_body(l);
       wd.poses[PB + b * 2] = F4(w.ld3(w.lo(l) + L_COM), 0.0f);
       wd.poses[PB + b * 2 + 1] = F4(w.link_rot(l));
     }

@@ -20,6 +20,12 @@ static constexpr uint32_t MAXJM = 6;
 static constexpr uint32_t MAX_PATHDOF = 192;  // MAXPATH * JM bound of the link-response scratch
 
 struct SVd { f3 a, l; };
+// Orders LDS and memory between the lanes of one wave (tree passes run one wave per articulation).
+DEV void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 DEV SVd sv0() { SVd s; s.a = mk3(0, 0, 0); s.l = mk3(0, 0, 0); return s; }
 DEV SVd mksv(f3 a, f3 l) { SVd s; s.a = a; s.l = l; return s; }
 DEV SVd sadd(SVd x, SVd y) { return mksv(x.a + y.a, x.l + y.l); }
@@ -132,11 +138,11 @@ struct W {
   DEV f3 pose_c(int b) const { return xyz(d.poses[H[W_P_BODY] + (uint32_t)b * 2]); }
 
   // ---- Featherstone (artwgsl.js), one thread per articulation
-  DEVN void art_fk(uint32_t a) const {
-    uint4 ai = art_info(a);
+  // Forward kinematics of one link from its parent's pose (artwgsl.js:98-187).
+  DEV void fk_link(uint32_t l) const {
     uint32_t JMv = JM();
-    for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k, o = lk(l);
+    {
+      uint32_t o = lk(l);
       f3 pp = mk3(0, 0, 0);
       q4 pq = mk4(0, 0, 0, 1);
       int par = link_parent(l);
@@ -187,6 +193,15 @@ struct W {
         }
     }
   }
+  // FK of articulation a, one wave (lane k: link ai.x + k at depth dl), root level first.
+  DEVN void art_fk_lv(uint32_t a, uint32_t dl) const {
+    const uint32_t MP = H[W_MAXPATH], l = art_info(a).x + (threadIdx.x & 63);
+    wave_sync();
+    for (uint32_t d = 0; d < MP; d++) {
+      if (dl == d) fk_link(l);
+      wave_sync();
+    }
+  }
   DEV void inv_spd(float* D, uint32_t n, float* Di) const {
     uint32_t JMv = JM();
     for (uint32_t i = 0; i < JMv * JMv; i++) Di[i] = 0.0f;
@@ -208,93 +223,105 @@ struct W {
     stm(o + 9, madd(madd(ldm(o + 9), B), mul(R, C)));
     stm(o + 18, madd(ldm(o + 18), C));
   }
-  DEVN void art_factor(uint32_t a) const {
-    uint4 ai = art_info(a);
+  // Articulated-body factor (artwgsl.js:210-369), split into per-link steps run level by level by
+  // art_factor_lv.  Pass 1 (root first): velocity, bias, rigid inertia and bias force of the link.
+  DEV void factor_fwd_link(uint32_t l) const {
     float* art_ = d.art;
-    uint32_t JMv = JM(), AQD = H[W_A_QD], AJF = H[W_A_JF], ABF = H[W_A_BF], AQDH = H[W_A_QDH];
+    uint32_t AQD = H[W_A_QD], ABF = H[W_A_BF];
+    f3 g = mk3(hf(W_GX), hf(W_GY), hf(W_GZ));
+    uint32_t L = lo(l);
+    int par = link_parent(l);
+    SVd vp = sv0();
+    if (par >= 0) vp = mshift(ldsv(lo((uint32_t)par) + L_V), link_r(l));
+    SVd sq = sv0();
+    for (uint32_t c = 0; c < link_ndof(l); c++) sq = sadd(sq, sscale(ldS(l, c), art_[AQD + link_dof(l) + c]));
+    SVd v = sadd(vp, sq);
+    stsv(L + L_V, v);
+    SVd cb = crm(vp, sq);
+    if (link_type(l) == JT_FREE) {
+      f3 vo = mk3(0, 0, 0), wj = mk3(0, 0, 0);
+      for (uint32_t c = 0; c < link_ndof(l); c++) {
+        SVd sc = ldS(l, c);
+        float q = art_[AQD + link_dof(l) + c];
+        if (c < 3) vo += sc.l * q; else wj += sc.a * q;
+      }
+      cb = sadd(cb, mksv(mk3(0, 0, 0), -cross(wj, vo)));
+    }
+    stsv(L + L_C, cb);
+    int b = link_body(l);
+    float m = body_mass(b);
+    m3 R = qmat(link_rot(l));
+    m3 Ic = mul(mul(R, body_inertia(b)), tr(R));
+    stm(L + L_IA, Ic);
+    stm(L + L_IA + 9, m3zero());
+    m3 Mm; Mm.c[0] = mk3(m, 0, 0); Mm.c[1] = mk3(0, m, 0); Mm.c[2] = mk3(0, 0, m);
+    stm(L + L_IA + 18, Mm);
+    SVd pa0 = crf(v, imul(L + L_IA, v));
+    if ((tu(sf(bo(b) + 7).w) & 2u) != 0) pa0 = sadd(pa0, mksv(mk3(0, 0, 0), m * g));
+    pa0 = sadd(pa0, mksv(-ld3(ABF + (uint32_t)b * 8 + 4), -ld3(ABF + (uint32_t)b * 8)));
+    stsv(L + L_PA, pa0);
+  }
+  // Pass 2 (leaves first), after the link's children were folded in: U, D^-1, u and the reduced inertia;
+  // returns the bias force pa the parent folds in (meaningless for a root).
+  DEV SVd factor_bwd_link(uint32_t l) const {
+    float* art_ = d.art;
+    uint32_t JMv = JM(), AQD = H[W_A_QD], AJF = H[W_A_JF];
+    float dt = hf(W_DT);
+    uint32_t L = lo(l);
+    uint32_t n = link_ndof(l);
+    SVd pA = ldsv(L + L_PA);
+    float D[36], u[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t i = 0; i < JMv * JMv; i++) D[i] = 0.0f;
+    for (uint32_t c = 0; c < n; c++) stsv(L + L_U + c * 6, imul(L + L_IA, ldS(l, c)));
+    for (uint32_t r = 0; r < n; r++) {
+      uint32_t dd = link_dof(l) + r;
+      for (uint32_t c = 0; c < n; c++) D[r * JMv + c] = sdot(ldS(l, r), ldU(l, c));
+      float4 dr = dof_drive(dd);
+      D[r * JMv + r] += dr.x;
+      float damp = dof_passive(dd).x;
+      D[r * JMv + r] += dt * damp;
+      float tau = -damp * art_[AQD + dd] + art_[AJF + dd];
+      u[r] = tau - sdot(ldS(l, r), pA);
+    }
+    float Di[36];
+    inv_spd(D, n, Di);
+    for (uint32_t i = 0; i < JMv * JMv; i++) art_[L + L_DI + i] = Di[i];
+    for (uint32_t r = 0; r < JMv; r++) art_[L + L_UU + r] = u[r];
+    if (link_parent(l) < 0) return sv0();
+    m3 A = ldm(L + L_IA), B = ldm(L + L_IA + 9), C = ldm(L + L_IA + 18);
+    SVd Duu = sv0();
+    for (uint32_t r = 0; r < n; r++) {
+      SVd Ur = ldU(l, r);
+      float du = 0.0f;
+      for (uint32_t c = 0; c < n; c++) {
+        SVd Uc = ldU(l, c);
+        float w = ldDI(l, r, c);
+        A = msub(A, mscale(w, outer3(Ur.a, Uc.a)));
+        B = msub(B, mscale(w, outer3(Ur.a, Uc.l)));
+        C = msub(C, mscale(w, outer3(Ur.l, Uc.l)));
+        du += w * u[c];
+      }
+      Duu = sadd(Duu, sscale(Ur, du));
+    }
+    stm(L + L_IA, A); stm(L + L_IA + 9, B); stm(L + L_IA + 18, C);
+    SVd cbias = ldsv(L + L_C);
+    return sadd(sadd(pA, imul(L + L_IA, cbias)), Duu);
+  }
+  // Fold child c (reduced inertia in its link block, bias force pa) into parent p.
+  DEV void factor_fold_child(uint32_t p, uint32_t c, SVd pa) const {
+    uint32_t P0 = lo(p), C0 = lo(c);
+    f3 r = link_r(c);
+    add_shifted_inertia(P0 + L_IA, ldm(C0 + L_IA), ldm(C0 + L_IA + 9), ldm(C0 + L_IA + 18), r);
+    stsv(P0 + L_PA, sadd(ldsv(P0 + L_PA), fshift(pa, r)));
+  }
+  // Pass 3 (root first): accelerations and the free joint velocities qd_hat.
+  DEV void factor_acc_link(uint32_t l) const {
+    float* art_ = d.art;
+    uint32_t AQD = H[W_A_QD], AQDH = H[W_A_QDH];
     float dt = hf(W_DT);
     f3 g = mk3(hf(W_GX), hf(W_GY), hf(W_GZ));
-    for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k, L = lo(l);
-      int par = link_parent(l);
-      SVd vp = sv0();
-      if (par >= 0) vp = mshift(ldsv(lo((uint32_t)par) + L_V), link_r(l));
-      SVd sq = sv0();
-      for (uint32_t c = 0; c < link_ndof(l); c++) sq = sadd(sq, sscale(ldS(l, c), art_[AQD + link_dof(l) + c]));
-      SVd v = sadd(vp, sq);
-      stsv(L + L_V, v);
-      SVd cb = crm(vp, sq);
-      if (link_type(l) == JT_FREE) {
-        f3 vo = mk3(0, 0, 0), wj = mk3(0, 0, 0);
-        for (uint32_t c = 0; c < link_ndof(l); c++) {
-          SVd sc = ldS(l, c);
-          float q = art_[AQD + link_dof(l) + c];
-          if (c < 3) vo += sc.l * q; else wj += sc.a * q;
-        }
-        cb = sadd(cb, mksv(mk3(0, 0, 0), -cross(wj, vo)));
-      }
-      stsv(L + L_C, cb);
-      int b = link_body(l);
-      float m = body_mass(b);
-      m3 R = qmat(link_rot(l));
-      m3 Ic = mul(mul(R, body_inertia(b)), tr(R));
-      stm(L + L_IA, Ic);
-      stm(L + L_IA + 9, m3zero());
-      m3 Mm; Mm.c[0] = mk3(m, 0, 0); Mm.c[1] = mk3(0, m, 0); Mm.c[2] = mk3(0, 0, m);
-      stm(L + L_IA + 18, Mm);
-      SVd pa0 = crf(v, imul(L + L_IA, v));
-      if ((tu(sf(bo(b) + 7).w) & 2u) != 0) pa0 = sadd(pa0, mksv(mk3(0, 0, 0), m * g));
-      pa0 = sadd(pa0, mksv(-ld3(ABF + (uint32_t)b * 8 + 4), -ld3(ABF + (uint32_t)b * 8)));
-      stsv(L + L_PA, pa0);
-    }
-    for (uint32_t kk = 0; kk < ai.y; kk++) {
-      uint32_t l = ai.x + ai.y - 1 - kk, L = lo(l);
-      uint32_t n = link_ndof(l);
-      SVd pA = ldsv(L + L_PA);
-      float D[36], u[6] = {0, 0, 0, 0, 0, 0};
-      for (uint32_t i = 0; i < JMv * JMv; i++) D[i] = 0.0f;
-      for (uint32_t c = 0; c < n; c++) stsv(L + L_U + c * 6, imul(L + L_IA, ldS(l, c)));
-      for (uint32_t r = 0; r < n; r++) {
-        uint32_t dd = link_dof(l) + r;
-        for (uint32_t c = 0; c < n; c++) D[r * JMv + c] = sdot(ldS(l, r), ldU(l, c));
-        float4 dr = dof_drive(dd);
-        D[r * JMv + r] += dr.x;
-        float damp = dof_passive(dd).x;
-        D[r * JMv + r] += dt * damp;
-        float tau = -damp * art_[AQD + dd] + art_[AJF + dd];
-        u[r] = tau - sdot(ldS(l, r), pA);
-      }
-      float Di[36];
-      inv_spd(D, n, Di);
-      for (uint32_t i = 0; i < JMv * JMv; i++) art_[L + L_DI + i] = Di[i];
-      for (uint32_t r = 0; r < JMv; r++) art_[L + L_UU + r] = u[r];
-      int par = link_parent(l);
-      if (par < 0) continue;
-      m3 A = ldm(L + L_IA), B = ldm(L + L_IA + 9), C = ldm(L + L_IA + 18);
-      SVd Duu = sv0();
-      for (uint32_t r = 0; r < n; r++) {
-        SVd Ur = ldU(l, r);
-        float du = 0.0f;
-        for (uint32_t c = 0; c < n; c++) {
-          SVd Uc = ldU(l, c);
-          float w = ldDI(l, r, c);
-          A = msub(A, mscale(w, outer3(Ur.a, Uc.a)));
-          B = msub(B, mscale(w, outer3(Ur.a, Uc.l)));
-          C = msub(C, mscale(w, outer3(Ur.l, Uc.l)));
-          du += w * u[c];
-        }
-        Duu = sadd(Duu, sscale(Ur, du));
-      }
-      stm(L + L_IA, A); stm(L + L_IA + 9, B); stm(L + L_IA + 18, C);
-      SVd cbias = ldsv(L + L_C);
-      SVd pa = sadd(sadd(pA, imul(L + L_IA, cbias)), Duu);
-      uint32_t P0 = lo((uint32_t)par);
-      f3 r = link_r(l);
-      add_shifted_inertia(P0 + L_IA, A, B, C, r);
-      stsv(P0 + L_PA, sadd(ldsv(P0 + L_PA), fshift(pa, r)));
-    }
-    for (uint32_t k = 0; k < ai.y; k++) {
-      uint32_t l = ai.x + k, L = lo(l);
+    uint32_t L = lo(l);
+    {
       int par = link_parent(l);
       SVd ap = mksv(mk3(0, 0, 0), -g);
       if (par >= 0) ap = mshift(ldsv(lo((uint32_t)par) + L_A), link_r(l));
@@ -310,6 +337,60 @@ struct W {
       }
       stsv(L + L_A, acc);
     }
+  }
+  // Factor of articulation a, one wave (lane k: link ai.x + k at depth dl).  tree[k]: bits 0-7 depth, 8-15 the
+  // highest-index child (0xff: none), 16-23 the next lower sibling; pabuf: 8 floats per link.  Children are
+  // folded into a parent in decreasing link order, the order of the serial pass, so results match it bit
+  // for bit.
+  DEVN void art_factor_lv(uint32_t a, const uint32_t* tree, float* pabuf) const {
+    const uint32_t lane = threadIdx.x & 63, MP = H[W_MAXPATH];
+    const uint4 ai = art_info(a);
+    const bool has = lane < ai.y;
+    const uint32_t l = ai.x + lane, dl = has ? (tree[lane] & 0xffu) : 0xffu;
+    wave_sync();
+    for (uint32_t dd = 0; dd < MP; dd++) {
+      if (dl == dd) factor_fwd_link(l);
+      wave_sync();
+    }
+    for (int dd = (int)MP - 1; dd >= 0; dd--) {
+      if (dl == (uint32_t)dd) {
+        SVd pa = factor_bwd_link(l);
+        float* pb = pabuf + lane * 8;
+        pb[0] = pa.a.x; pb[1] = pa.a.y; pb[2] = pa.a.z; pb[3] = pa.l.x; pb[4] = pa.l.y; pb[5] = pa.l.z;
+      }
+      wave_sync();
+      if (dd > 0 && dl == (uint32_t)(dd - 1)) {
+        for (uint32_t c = (tree[lane] >> 8) & 0xffu; c != 0xffu; c = (tree[c] >> 16) & 0xffu) {
+          const float* pb = pabuf + c * 8;
+          factor_fold_child(l, ai.x + c, mksv(mk3(pb[0], pb[1], pb[2]), mk3(pb[3], pb[4], pb[5])));
+        }
+      }
+      wave_sync();
+    }
+    for (uint32_t dd = 0; dd < MP; dd++) {
+      if (dl == dd) factor_acc_link(l);
+      wave_sync();
+    }
+  }
+  // tree of articulation a for art_factor_lv (one wave; lane k fills entry k).
+  DEVN void art_tree(uint32_t a, uint32_t* tree) const {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint4 ai = art_info(a);
+    if (lane < ai.y) {
+      uint32_t dep = 0;
+      for (int p = link_parent(ai.x + lane); p >= 0; p = link_parent((uint32_t)p)) dep++;
+      tree[lane] = dep | 0xffff00u;
+    }
+    wave_sync();
+    if (lane == 0)
+      for (uint32_t k = 0; k < ai.y; k++) {
+        int p = link_parent(ai.x + k);
+        if (p < 0) continue;
+        uint32_t pk = (uint32_t)p - ai.x;
+        tree[k] = (tree[k] & ~0xff0000u) | (((tree[pk] >> 8) & 0xffu) << 16);
+        tree[pk] = (tree[pk] & ~0xff00u) | (k << 8);
+      }
+    wave_sync();
   }
   DEV uint32_t link_path(uint32_t l0, uint32_t* path) const {
     uint32_t n = 0, MP = H[W_MAXPATH];
