@@ -811,9 +811,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     while (j >= 0) { atomicAdd(&sched[S.JSPLIT + (uint32_t)j], 1u); j = w.link_parent((uint32_t)j); }
   }
   __syncthreads();
-  for (uint32_t l = tid; l < NL; l += BS) {
-    if (sched[S.LINKHIT + (uint32_t)w.link_body(l)] == 0) continue;
-    w.link_response(l, sched + S.JSPLIT);
+  // Responses of contact-bearing links: six lanes per link (one per unit impulse), ten links per wave.
+  for (uint32_t l0 = (tid >> 6) * 10; l0 < NL; l0 += (BS / 64) * 10) {
+    const uint32_t lane = tid & 63, l = l0 + lane / 6, e = lane % 6;
+    if (lane < 60 && l < NL && sched[S.LINKHIT + (uint32_t)w.link_body(l)] != 0) w.link_response_e(l, e, lane - e, sched + S.JSPLIT);
   }
   for (uint32_t dd = tid; dd < ND; dd += BS) {
     float4 lim = w.dof_limit(dd);

@@ -507,6 +507,65 @@ struct W {
     }
     for (uint32_t k = 0; k < 36; k++) art_[lo(l) + L_W + k] = Wm[k];
   }
+  // link_response with the six unit impulses on six lanes: lane base + e computes direction e (column e of
+  // W2 and W).  The split terms need every direction's u along the path; a lane gets them from its link's
+  // other five lanes by shuffles.  Each element keeps the serial pass's summation order.  All six lanes of
+  // a link must call this together (same l, base).
+  DEVN void link_response_e(uint32_t l, uint32_t e, uint32_t base, const uint32_t* jsplit) const {  // artwgsl.js:431-510
+    uint32_t path[MAXPATH_DEV];
+    uint32_t np = link_path(l, path);
+    uint32_t JMv = JM();
+    float ub[MAX_PATHDOF];
+    f3 f = mk3(0, 0, 0), t = mk3(0, 0, 0);
+    if (e < 3) setc(f, e, 1.0f); else setc(t, e - 3, 1.0f);
+    SVd p = mksv(-t, -f);
+    for (uint32_t i = 0; i < np; i++) {
+      uint32_t pl = path[i], n = link_ndof(pl);
+      f3 r = link_r(pl);
+      float u[6];
+      for (uint32_t c = 0; c < n; c++) { u[c] = 0.0f - sdot(ldS(pl, c), p); ub[i * JMv + c] = u[c]; }
+      SVd pe = p;
+      for (uint32_t rr = 0; rr < n; rr++) {
+        float du = 0.0f;
+        for (uint32_t c = 0; c < n; c++) du += ldDI(pl, rr, c) * u[c];
+        pe = sadd(pe, sscale(ldU(pl, rr), du));
+      }
+      p = fshift(pe, r);
+    }
+    SVd dv = sv0();
+    for (uint32_t ii = 0; ii < np; ii++) {
+      uint32_t i = np - 1 - ii, pl = path[i], n = link_ndof(pl);
+      f3 r = link_r(pl);
+      SVd dd = mshift(dv, r);
+      float dq[6];
+      for (uint32_t rr = 0; rr < n; rr++) {
+        dq[rr] = 0.0f;
+        for (uint32_t c = 0; c < n; c++) dq[rr] += ldDI(pl, rr, c) * (ub[i * JMv + c] - sdot(ldU(pl, c), dd));
+      }
+      for (uint32_t rr = 0; rr < n; rr++) dd = sadd(dd, sscale(ldS(pl, rr), dq[rr]));
+      dv = dd;
+    }
+    float col[6] = {dv.l.x, dv.l.y, dv.l.z, dv.a.x, dv.a.y, dv.a.z};  // column e of W2
+    float* art_ = d.art;
+    for (uint32_t k = 0; k < 6; k++) art_[lo(l) + L_W2 + k * 6 + e] = col[k];
+    for (uint32_t i = 0; i < np; i++) {
+      float extra = (float)jsplit[path[i]] - 1.0f;
+      if (extra <= 0.0f) continue;
+      uint32_t pl = path[i], n = link_ndof(pl);
+      float tc[6];
+      for (uint32_t x = 0; x < n; x++) {
+        float v = 0.0f;
+        for (uint32_t y = 0; y < n; y++) v += ldDI(pl, x, y) * ub[i * JMv + y];
+        tc[x] = v;
+      }
+      for (uint32_t rr = 0; rr < 6; rr++) {
+        float sv = 0.0f;
+        for (uint32_t x = 0; x < n; x++) sv += __shfl(ub[i * JMv + x], (int)(base + rr), 64) * tc[x];
+        col[rr] += extra * sv;
+      }
+    }
+    for (uint32_t k = 0; k < 6; k++) art_[lo(l) + L_W + k * 6 + e] = col[k];
+  }
   DEV void add_gen_jac(uint32_t x, f3 p, float sgn, f3 d0, f3 d1, f3 d2, uint32_t ds, float* J) const {
     int l = (int)x;
     uint32_t NDAv = NDA();
