@@ -640,7 +640,9 @@ struct Prep {
     if (w.body_kind(b) == 1) { out[0] = xyz(w.d.vel[(uint32_t)b * 2]); out[1] = xyz(w.d.vel[(uint32_t)b * 2 + 1]); return; }
     if (w.body_kind(b) == 2) { uint32_t o = w.lo((uint32_t)w.body_link(b)) + L_V; out[0] = w.ld3(o + 3); out[1] = w.ld3(o); }
   }
-  DEVN void dense_row(uint32_t c, uint32_t slot) const {  // wgsl.js:2370-2444
+  // Dense row of contact c, three lanes (base + r, r = 0..2): lane r solves row r's Y = H^-1 J_r^T, the
+  // 3x3 K = J Y^T is gathered by shuffles, lane 0 writes the row header (wgsl.js:2370-2444).
+  DEVN void dense_row(uint32_t c, uint32_t slot, uint32_t r, uint32_t base) const {
     float4 c0 = con(c, 0), c1 = con(c, 1), cmeta = con(c, 2);
     int a = ti(cmeta.x), b = ti(cmeta.y);
     uint32_t key = max(tu(cmeta.z) | (tu(cmeta.w) << 16), 1u);
@@ -661,16 +663,20 @@ struct Prep {
     w.add_gen_jac((uint32_t)w.body_link(a), p, -1.0f, n, t1, t2, ai.z, J);
     float* art_ = w.d.art;
     uint32_t o = w.h(W_A_DENSE) + slot * w.h(W_DPS);
-    float K[9];
     SVd ps[64], dvs[64];
     float uu[6 * 64];
-    for (uint32_t r = 0; r < 3; r++) {
+    float kc[3];
+    {
       float tau[64], y[64];
       for (uint32_t k = 0; k < NDA; k++) { tau[k] = k < ai.w ? J[r * NDA + k] : 0.0f; y[k] = 0.0f; }
       w.art_solve_gen(art_id, tau, y, ps, dvs, uu);
       for (uint32_t k = 0; k < ai.w; k++) { art_[o + DHDR + r * NDA + k] = J[r * NDA + k]; art_[o + DHDR + 3 * NDA + r * NDA + k] = y[k]; }
-      for (uint32_t s2 = 0; s2 < 3; s2++) { float acc = 0.0f; for (uint32_t k = 0; k < ai.w; k++) acc += J[s2 * NDA + k] * y[k]; K[s2 * 3 + r] = acc; }
+      for (uint32_t s2 = 0; s2 < 3; s2++) { float acc = 0.0f; for (uint32_t k = 0; k < ai.w; k++) acc += J[s2 * NDA + k] * y[k]; kc[s2] = acc; }
     }
+    float K[9];  // K[s2 * 3 + rr] from lane base + rr
+    for (uint32_t s2 = 0; s2 < 3; s2++)
+      for (uint32_t rr = 0; rr < 3; rr++) K[s2 * 3 + rr] = __shfl(kc[s2], (int)(base + rr), 64);
+    if (r != 0) return;
     float cfm = w.hf(W_F_CFM), dt = w.hf(W_DT);
     float kscale = mx(K[0] + K[4] + K[8], 1e-30f);
     float effn = 0.0f;
@@ -1013,13 +1019,19 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     }
     sched[S.META + 3] = ndc;
   }
-  // ---- Phase 7: dense rows of same-articulation contacts, in contact order
+  // ---- Phase 7: dense rows of same-articulation contacts (slot = rank in contact order), three lanes per
+  // contact, 21 contacts per wave; RK0 (free after phase 5) maps slot -> contact.
   const uint32_t MAXDENSE = w.h(W_MAXDENSE);
   for (uint32_t c = tid; c < nc; c += BS) {
     if (sched[S.CSLOT + c] != 0xffffffffu || sched[S.CLOCAL + c] == 0) continue;
     uint32_t slot = sched[S.DSLOT + c];
-    if (slot >= MAXDENSE) continue;
-    P.dense_row(c, slot);
+    if (slot < MAXDENSE) sched[S.RK0 + slot] = c;
+  }
+  __syncthreads();
+  const uint32_t ndd = min(ndc, MAXDENSE);
+  for (uint32_t s0 = (tid >> 6) * 21; s0 < ndd; s0 += (BS / 64) * 21) {
+    const uint32_t lane = tid & 63, sl = s0 + lane / 3, r = lane % 3;
+    if (lane < 63 && sl < ndd) P.dense_row(sched[S.RK0 + sl], sl, r, lane - r);
   }
 }
 
