@@ -251,34 +251,44 @@ __global__ void __launch_bounds__(BS) k_pair_offsets(const WorldDev* __restrict_
 }
 
 // Narrowphase over every candidate pair of every world: one wave64 per pair (wgsl.js:1063-1195).  The
-// grid is fixed; wave g takes a contiguous chunk of the pairs laid end to end in world order (offs), so a
-// world with many pairs is spread over many waves and CUs.  Contacts take slots from the world's counter
-// ctr[1]; their order is made canonical by k_csort.
+// pairs are laid end to end in world order (offs) and cut into chunks of CH = clamp(total / waves, 1, 4)
+// consecutive pairs; wave g takes chunks g, g + G, g + 2G, ... so every wave draws a similar mix of cheap and
+// expensive pairs from all worlds, and a world with many pairs is spread over many CUs.  Contacts take slots
+// from the world's counter ctr[1]; their order is made canonical by k_csort.
+static constexpr uint32_t NARROW_LDS_OFFS = 4097;
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_narrow(const WorldDev* __restrict__ worlds, uint32_t nworlds, const uint32_t* __restrict__ offs) {
+  __shared__ uint32_t s_offs[NARROW_LDS_OFFS];
+  const bool lds_offs = nworlds + 1 <= NARROW_LDS_OFFS;
+  if (lds_offs)
+    for (uint32_t i = threadIdx.x; i <= nworlds; i += 256) s_offs[i] = offs[i];
+  __syncthreads();
+  const uint32_t* O = lds_offs ? s_offs : offs;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t G = gridDim.x * 4, g = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint32_t total = offs[nworlds];
-  const uint32_t chunk = (total + G - 1) / G;
-  const uint32_t i0 = g * chunk, i1 = min(i0 + chunk, total);
-  if (i0 >= i1) return;
-  uint32_t lo = 0, hi = nworlds;  // largest w with offs[w] <= i0 (then offs[w + 1] > i0, so w < nworlds)
-  while (lo < hi) {
-    uint32_t mid = (lo + hi + 1) >> 1;
-    if (offs[mid] <= i0) lo = mid; else hi = mid - 1;
-  }
-  uint32_t w = lo, wend = offs[w + 1];
-  Collider col;
-  WorldDev wd = worlds[w];
-  col.w = make_world(wd.hdr, wd);
-  col.s_nc = wd.ctr + 1;
-  for (uint32_t i = i0; i < i1; i++) {
-    if (i >= wend) {
-      do { w++; wend = offs[w + 1]; } while (i >= wend);
-      wd = worlds[w];
-      col.w = make_world(wd.hdr, wd);
-      col.s_nc = wd.ctr + 1;
+  const uint32_t total = O[nworlds];
+  const uint32_t CH = max(1u, min(4u, total / G));
+  const uint32_t nch = (total + CH - 1) / CH;
+  for (uint32_t c = g; c < nch; c += G) {
+    const uint32_t i0 = c * CH, i1 = min(i0 + CH, total);
+    uint32_t lo = 0, hi = nworlds;  // largest w with O[w] <= i0 (then O[w + 1] > i0, so w < nworlds)
+    while (lo < hi) {
+      uint32_t mid = (lo + hi + 1) >> 1;
+      if (O[mid] <= i0) lo = mid; else hi = mid - 1;
     }
-    col.narrow_pair(i - offs[w], lane);
+    uint32_t w = lo, wend = O[w + 1];
+    WorldDev wd = worlds[w];
+    Collider col;
+    col.w = make_world(wd.hdr, wd);
+    col.s_nc = wd.ctr + 1;
+    for (uint32_t i = i0; i < i1; i++) {
+      if (i >= wend) {
+        do { w++; wend = O[w + 1]; } while (i >= wend);
+        wd = worlds[w];
+        col.w = make_world(wd.hdr, wd);
+        col.s_nc = wd.ctr + 1;
+      }
+      col.narrow_pair(i - O[w], lane);
+    }
   }
 }
 
