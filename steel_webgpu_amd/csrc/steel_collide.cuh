@@ -31,9 +31,14 @@ DEV int wave_argmax(float sep, uint32_t key) {
 }
 DEV f3 shfl3(f3 v, int src) { return mk3(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64)); }
 
+// Per-wave LDS work area of the serial (lane 0) manifold code: the two clip polygons, caliper indices and
+// the manifold; private arrays indexed at run time would otherwise live in scratch memory.
+struct ManifoldLds { f2 A[10], B[10]; int idx[4]; Mand m; };
+
 struct Collider {
   W w;
-  uint32_t* s_nc;  // LDS contact counter
+  uint32_t* s_nc;      // the world's contact counter
+  ManifoldLds* mlds;   // this wave's LDS work area
 
   DEV float4 shape_rec(uint32_t s, uint32_t k) const { return w.sf(w.so(s) + k); }
   DEV Geod geo(uint32_t s) const {
@@ -533,18 +538,19 @@ struct Collider {
     float off = dot(dir, gsup(g, dir)) + dd;
     return off * off <= MEPS * MEPS * dot(dir, dir);
   }
-  DEVN Mand build_manifold(const Geod& ga, const Geod& gb, q4 rq, f3 rt, f3 wa, f3 wb, f3 n) const {
+  DEV void build_manifold(const Geod& ga, const Geod& gb, q4 rq, f3 rt, f3 wa, f3 wb, f3 n) const {
     const float COS_TILT = 0.9993908270190958f, SIN_TILT = 0.03489949670250097f, COS_DEEP = 0.9999984769132877f;
     const float PENTX[5] = {1.0f, 0.30901699437494745f, -0.8090169943749473f, -0.8090169943749476f, 0.30901699437494723f};
     const float PENTY[5] = {0.0f, 0.9510565162951535f, 0.5877852522924732f, -0.587785252292473f, -0.9510565162951536f};
-    Mand out;
+    Mand& out = mlds->m;
     out.n = 0;
     f3 fna = n;
     f3 tb[2];
     onb(n, tb);
     bool pa = planar_geo(ga), pb = planar_geo(gb);
     f3 center = 0.5f * (wa + wb);
-    f2 A[10], B[10];
+    f2* A = mlds->A;
+    f2* B = mlds->B;
     int ac = 0, bc = 0;
     Trkd ta;
     ta.r = ta.p = ta.n = mk3(0, 0, 0); ta.a2 = 0.0f; ta.np = false;
@@ -580,7 +586,8 @@ struct Collider {
         ndot = fabsf(dot(pr[0].n, pr[1].n));
         int lc = remove_zero_edges(B, trim_all(A, ac, B, bc));
         if (lc > 1) {
-          int idx[4] = {0, 1, 2, 3};
+          int* idx = mlds->idx;
+          idx[0] = 0; idx[1] = 1; idx[2] = 2; idx[3] = 3;
           if (lc > 4) { calipers(B, lc, idx); lc = 4; }
           for (int i = 0; i < lc; i++) {
             f2 q = B[idx[i]];
@@ -593,10 +600,10 @@ struct Collider {
     }
     if (ndot < COS_DEEP || cnt == 0) { out.p[cnt] = 0.5f * (wa + wb); out.d[cnt] = dot(wb - wa, n); cnt++; }
     out.n = cnt;
-    return out;
   }
-  DEVN void gen_run(uint32_t sa, uint32_t sb, const Geod& ga, const Geod& gb, f3 pa, q4 qa, q4 rq, f3 rt, f3 wa, f3 wb, f3 n, float thresh) const {
-    Mand m = build_manifold(ga, gb, rq, rt, wa, wb, n);
+  DEV void gen_run(uint32_t sa, uint32_t sb, const Geod& ga, const Geod& gb, f3 pa, q4 qa, q4 rq, f3 rt, f3 wa, f3 wb, f3 n, float thresh) const {
+    build_manifold(ga, gb, rq, rt, wa, wb, n);
+    const Mand& m = mlds->m;
     f3 nw = qrot(qa, n);
     float ra = (ga.t == GT_SPHERE || ga.t == GT_CAPSULE) ? shape_rec(sa, 6).y : 0.0f;
     float rb = (gb.t == GT_SPHERE || gb.t == GT_CAPSULE) ? shape_rec(sb, 6).y : 0.0f;
@@ -632,7 +639,7 @@ struct Collider {
     if ((refA && !(oa < ob - 1e-6f)) || (!refA && ob < oa - 1e-6f)) { wo[0] = wb0; wo[1] = wb1; return; }
     wo[0] = wa0; wo[1] = wa1;
   }
-  DEVN void sat_generic(uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb, f3 pa, q4 qa, q4 rBA, f3 tBA, float thresh, FaceQd fa, FaceQd fb,
+  DEV void sat_generic(uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb, f3 pa, q4 qa, q4 rBA, f3 tBA, float thresh, FaceQd fa, FaceQd fb,
                         const EdgeQd& eq) const {
     const float SAT_TIE = 1e-5f;
     Geod ga = geo(sa), gb = geo(sb);
@@ -648,7 +655,7 @@ struct Collider {
     sup_witness(ga, gb, rBA, tBA, n, refA, wv);
     gen_run(sa, sb, ga, gb, pa, qa, rBA, tBA, wv[0], wv[1], n, thresh);
   }
-  DEVN void plane_convex(uint32_t sa, uint32_t sb, f3 pa, q4 qa, f3 pb, q4 qb, float thresh) const {
+  DEV void plane_convex(uint32_t sa, uint32_t sb, f3 pa, q4 qa, f3 pb, q4 qb, float thresh) const {
     Geod gb = geo(sb);
     f3 nw = qrot(qa, mk3(0, 0, 1));
     float L = 10.0f * (len(xyz(w.sf(w.h(W_SF_HAABB) + gb.h * 2))) + len(xyz(w.sf(w.h(W_SF_HAABB) + gb.h * 2 + 1))));
@@ -662,7 +669,7 @@ struct Collider {
     Geod box; box.t = GT_BOX; box.s = mk3(L, L, L); box.h = 0;
     gen_run(sa, sb, box, gb, c, qa, rq, rt, wb - n * sep, wb, n, thresh);
   }
-  DEVN void capsule_generic(uint32_t c, uint32_t h, f3 pc, q4 qc, f3 wp, q4 wq, f3 wc, f3 wh, f3 nh, float thresh) const {
+  DEV void capsule_generic(uint32_t c, uint32_t h, f3 pc, q4 qc, f3 wp, q4 wq, f3 wc, f3 wh, f3 nh, float thresh) const {
     if (dot(wh - wc, nh) - shape_rec(c, 6).y > thresh) return;
     q4 rq = qmul(qconj(qc), wq);
     f3 rt = qinvrot(qc, wp - pc);

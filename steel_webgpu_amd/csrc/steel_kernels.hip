@@ -258,6 +258,7 @@ __global__ void __launch_bounds__(BS) k_pair_offsets(const WorldDev* __restrict_
 static constexpr uint32_t NARROW_LDS_OFFS = 4097;
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_narrow(const WorldDev* __restrict__ worlds, uint32_t nworlds, const uint32_t* __restrict__ offs) {
   __shared__ uint32_t s_offs[NARROW_LDS_OFFS];
+  __shared__ ManifoldLds s_man[4];
   const bool lds_offs = nworlds + 1 <= NARROW_LDS_OFFS;
   if (lds_offs)
     for (uint32_t i = threadIdx.x; i <= nworlds; i += 256) s_offs[i] = offs[i];
@@ -280,6 +281,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEEL_
     Collider col;
     col.w = make_world(wd.hdr, wd);
     col.s_nc = wd.ctr + 1;
+    col.mlds = s_man + (threadIdx.x >> 6);
     for (uint32_t i = i0; i < i1; i++) {
       if (i >= wend) {
         do { w++; wend = O[w + 1]; } while (i >= wend);
