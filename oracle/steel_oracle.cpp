@@ -1087,6 +1087,24 @@ template <class T> struct World {
         T damp = dof_passive(d).x;
         D[r * JM + r] += dt * damp;
         T tau = -damp * art[A_QD + d] + art[A_JF + d];
+        // Passive spring, explicit (artwgsl.js:298-300).
+        V4<T> sp = statF[SF_DOF + d * 5 + 4];
+        if (sp.x != T(0)) tau += sp.x * (sp.y - art[A_Q + link_qs(l) + r]);
+        // Implicit PD drive toward (A_TGT, A_TGTV), saturated on the implicit torque (artwgsl.js:301-313).
+        if (dr.y > T(0) || dr.z > T(0)) {
+          T q = art[A_Q + link_qs(l) + r], qd = art[A_QD + d];
+          T err = art[A_TGT + d] - q;
+          T tv = art[A_TGTV + d];
+          T m = mx(D[r * JM + r], T(1e-9));
+          T qdn = (m * qd + dt * (dr.y * err + dr.z * tv)) / (m + dt * dr.z + dt * dt * dr.y);
+          T ti = dr.y * (err - dt * qdn) + dr.z * (tv - qdn);
+          if (std::fabs(ti) <= dr.w || dr.w <= T(0)) {
+            D[r * JM + r] += dt * dt * dr.y + dt * dr.z;
+            tau += dr.y * err - dt * dr.y * qd + dr.z * (tv - qd);
+          } else {
+            tau += std::min(std::max(ti, -dr.w), dr.w);
+          }
+        }
         u[r] = tau - sdot(ldS(l, r), pA);
       }
       T Di[36];

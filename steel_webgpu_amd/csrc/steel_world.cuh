@@ -281,6 +281,24 @@ struct W {
       float damp = dof_passive(dd).x;
       D[r * JMv + r] += dt * damp;
       float tau = -damp * art_[AQD + dd] + art_[AJF + dd];
+      // Passive spring, explicit (artwgsl.js:298-300).
+      float4 sp = sf(H[W_SF_DOF] + dd * 5 + 4);
+      if (sp.x != 0.0f) tau += sp.x * (sp.y - art_[H[W_A_Q] + link_qs(l) + r]);
+      // Implicit PD drive toward (A_TGT, A_TGTV), saturated on the implicit torque (artwgsl.js:301-313).
+      if (dr.y > 0.0f || dr.z > 0.0f) {
+        float q = art_[H[W_A_Q] + link_qs(l) + r], qd = art_[AQD + dd];
+        float err = art_[H[W_A_TGT] + dd] - q;
+        float tv = art_[H[W_A_TGTV] + dd];
+        float m = mx(D[r * JMv + r], 1e-9f);
+        float qdn = (m * qd + dt * (dr.y * err + dr.z * tv)) / (m + dt * dr.z + dt * dt * dr.y);
+        float ti = dr.y * (err - dt * qdn) + dr.z * (tv - qdn);
+        if (fabsf(ti) <= dr.w || dr.w <= 0.0f) {
+          D[r * JMv + r] += dt * dt * dr.y + dt * dr.z;
+          tau += dr.y * err - dt * dr.y * qd + dr.z * (tv - qd);
+        } else {
+          tau += clampf_(ti, -dr.w, dr.w);
+        }
+      }
       u[r] = tau - sdot(ldS(l, r), pA);
     }
     float Di[36];

@@ -222,10 +222,6 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         raise UnsupportedError("scene controllers are not supported yet")
     if S.get("driven", 0) and S.get("target_frames", 0):
         raise UnsupportedError("recorded target paths are not supported yet")
-    if any(d["ke"] > 0 or d["kd"] > 0 for d in dofs):
-        raise UnsupportedError("joint drives are not supported yet")
-    if any(d["damping"] != 0 or d["springKe"] != 0 for d in dofs):
-        raise UnsupportedError("passive joint damping/springs are not supported yet")
     if any(kin):
         raise UnsupportedError("kinematic bodies are not supported yet")
 

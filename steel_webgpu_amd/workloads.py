@@ -166,24 +166,15 @@ def arm_hulls(seed=SEED + 1, nlinks=32, nhulls=256, fps=240, substeps=8, iterati
     return b.finalize(solver=dict(substeps=substeps, iterations=iterations))
 
 
-def quadruped_world(index, seed=SEED + 2, fps=200, substeps=4, iterations=12, kmax=64):
-    """One world of the batch: floating-base quadruped (box base, 4 legs x 3 revolute capsule links) + K boxes."""
+def quadruped_world(index, seed=SEED + 2, fps=200, substeps=4, iterations=12, kmax=64, drive=None):
+    """One world of the batch: floating-base quadruped (box base, 4 legs x 3 revolute capsule links) + K boxes.
+    `drive` (dict of targetKe, targetKd, effortLimit, ...) gives every leg joint a PD drive holding its initial
+    angle; the batch workload has none."""
     rng = PCG32(seed, index)
     b = SceneBuilder(fps=fps)
     b.addGroundPlane(mu=0.6)
     base_z = 0.55
-    base = b.addBody(xform=[0, 0, base_z, 0, 0, 0, 1])
-    b.addShapeBox(base, hx=0.2, hy=0.05, hz=0.1, mu=0.6, density=1000.0)
-    axes = ([1, 0, 0], [0, 1, 0], [0, 1, 0])
-    for sx in (1, -1):
-        for sy in (1, -1):
-            parent = base
-            pxf = [sx * 0.2, sy * 0.05, -0.1, 0, 0, 0, 1]
-            for k in range(3):
-                body = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1])
-                b.addShapeCapsule(body, radius=0.02, halfHeight=0.075, mu=0.6, xform=[0, 0, -0.075, 0, 0, 0, 1])
-                b.addJointRevolute(parent, body, axis=axes[k], parentXform=pxf, q=rng.uniform(-0.3, 0.3))
-                parent, pxf = body, [0, 0, -0.15, 0, 0, 0, 1]
+    _add_quadruped(b, 0.0, 0.0, rng, base_z, drive)
     K = min(rng.geometric(0.1), kmax)
     # keep-out box around the quadruped
     placed = [(np.array([0.0, 0.0, base_z - 0.2]), np.eye(3), np.array([0.45, 0.3, 0.45]), 0.71)]
@@ -210,7 +201,7 @@ def hull_templates(n=64, seed=SEED + 3):
     return _HULL_TEMPLATES
 
 
-def _add_quadruped(b, x0, y0, rng, base_z=0.55):
+def _add_quadruped(b, x0, y0, rng, base_z=0.55, drive=None):
     base = b.addBody(xform=[x0, y0, base_z, 0, 0, 0, 1])
     b.addShapeBox(base, hx=0.2, hy=0.05, hz=0.1, mu=0.6, density=1000.0)
     axes = ([1, 0, 0], [0, 1, 0], [0, 1, 0])
@@ -221,7 +212,8 @@ def _add_quadruped(b, x0, y0, rng, base_z=0.55):
             for k in range(3):
                 body = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1])
                 b.addShapeCapsule(body, radius=0.02, halfHeight=0.075, mu=0.6, xform=[0, 0, -0.075, 0, 0, 0, 1])
-                b.addJointRevolute(parent, body, axis=axes[k], parentXform=pxf, q=rng.uniform(-0.3, 0.3))
+                q = rng.uniform(-0.3, 0.3)
+                b.addJointRevolute(parent, body, axis=axes[k], parentXform=pxf, q=q, **(dict(drive, targetPos=q) if drive else {}))
                 parent, pxf = body, [0, 0, -0.15, 0, 0, 0, 1]
 
 

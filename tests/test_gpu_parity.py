@@ -19,11 +19,24 @@ def sim():
     return s
 
 
+def driven_quadruped(index=3):
+    """Quadruped whose 12 joints hold their initial angles with PD drives; efforts mix unlimited (0), loose and
+    tight enough to saturate under the legs' load, and some joints also carry passive springs and damping."""
+    scene = workloads.quadruped_world(index, drive=dict(targetKe=60.0, targetKd=2.0, armature=0.01, damping=0.05))
+    J = scene.json["joints"]
+    n = len(J["ke"])
+    J["effort"] = [(0.0, 0.3, 5.0)[k % 3] for k in range(n)]
+    J["spring_stiffness"] = [4.0 if k % 4 == 1 else 0.0 for k in range(n)]
+    J["spring_ref"] = [0.2 if k % 4 == 1 else 0.0 for k in range(n)]
+    return scene
+
+
 SCENES = {
     "boxes64": lambda: workloads.boxes64(),
     "quad0": lambda: workloads.quadruped_world(0),
     "quad5": lambda: workloads.quadruped_world(5),
     "arm": lambda: workloads.arm_hulls(),
+    "quad_driven": driven_quadruped,
 }
 
 
@@ -53,7 +66,7 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
     b.destroy()
 
 
-@pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12)])
+@pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])
@@ -130,8 +143,35 @@ def test_reset_restores_initial_state(sim):
     b.destroy()
 
 
+def test_joint_targets_set_mid_run_bitexact(sim):
+    """Sim.setJointTargets (a policy's action vector, sim.js:716-728) between frames: the GPU world and the oracle,
+    given the same targets, stay bitwise equal; the drives then pull the joints toward the new targets."""
+    from steel_webgpu_amd.sim import Sim
+    scene = driven_quadruped(7)
+    s = Sim(None, scene)
+    pw = s.packed
+    o = OracleWorld(pw)
+    A, ND = pw.A, pw.ND
+    rng = np.random.default_rng(3)
+    J = scene.json["joints"]
+    first = J["qd_start"][1]        # Newton dof of the first leg joint (joint 0 is the floating base)
+    for chunk in range(3):
+        tgt = rng.uniform(-0.4, 0.4, 12).astype(np.float32)
+        tv = rng.uniform(-0.5, 0.5, 12).astype(np.float32)
+        s.setJointTargets(tgt, tv, first)
+        for k in range(12):
+            d = pw.dofMap[first + k]
+            o.art[A["A_TGT"] + d] = tgt[k]
+            o.art[A["A_TGTV"] + d] = tv[k]
+        s.step(40)
+        o.frames(40)
+        assert np.array_equal(_bits(s.batch.read(0, "art")), _bits(o.art)), chunk
+        assert np.array_equal(_bits(s.batch.read(0, "poses")), _bits(o.poses)), chunk
+    assert s.userTargets
+
+
 def test_one_step_error_against_fp64_within_1e5(sim):
-    for name in ("boxes64", "quad0", "quad5"):
+    for name in ("boxes64", "quad0", "quad5", "quad_driven"):
         pw = pack.pack_scene(SCENES[name]())
         b = sim.Batch([pw])
         o64 = OracleWorld(pw, f64=True)

@@ -290,25 +290,28 @@ def _rot(q):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
-def test_fixed_base_chain_free_acceleration_matches_crba():
-    """qd_hat of a fixed-base revolute chain at rest = q_dot + dt * H^-1 tau_gravity, with H and tau from the
-    composite rigid-body formula (sum over links of COM Jacobians) and forward kinematics computed here from the
-    scene's joint arrays."""
-    from scipy.spatial.transform import Rotation as Rot
-    rng = np.random.default_rng(5)
+def _revolute_chain(rng, n, dof=None):
+    """Fixed-base chain of n capsule links on revolute joints with random unit axes and angles."""
     b = SceneBuilder(fps=240)
     parent, prev = -1, 0.0
-    for i in range(7):
+    for i in range(n):
         l = float(rng.uniform(0.1, 0.3))
         bd = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1])
         b.addShapeCapsule(bd, radius=0.03, halfHeight=l / 2, xform=[0, 0, l / 2, 0, 0, 0, 1])
         ax = rng.normal(size=3)
-        b.addJointRevolute(parent, bd, axis=list(ax / np.linalg.norm(ax)), parentXform=[0, 0, prev, 0, 0, 0, 1], q=float(rng.uniform(-1, 1)))
+        b.addJointRevolute(parent, bd, axis=list(ax / np.linalg.norm(ax)), parentXform=[0, 0, prev, 0, 0, 0, 1], q=float(rng.uniform(-1, 1)),
+                           **(dof(i) if dof else {}))
         parent, prev = bd, l
-    scene = b.finalize(solver=dict(substeps=1))
+    return b.finalize(solver=dict(substeps=1))
+
+
+def _crba(scene):
+    """Joint-space mass matrix H and gravity torque of a fixed-base revolute chain, from the composite rigid-body
+    formula (sum over links of COM Jacobians) with forward kinematics computed here from the scene's joint arrays
+    (body frame = joint child frame; X_c identity, X_p a translation)."""
+    from scipy.spatial.transform import Rotation as Rot
     S, J = scene.json, scene.json["joints"]
     n = J["count"]
-    # forward kinematics (body frame = joint child frame; X_c identity, X_p translation only here)
     R_prev, p_prev = np.eye(3), np.zeros(3)
     axes_w, orig_w, coms, Rs = [], [], [], []
     for j in range(n):
@@ -333,13 +336,124 @@ def test_fixed_base_chain_free_acceleration_matches_crba():
             Jv[:, k] = np.cross(axes_w[k], coms[l] - orig_w[k])
         H += m * Jv.T @ Jv + Jw.T @ Iw @ Jw
         tau += Jv.T @ (m * g)
+    return H, tau
+
+
+def test_fixed_base_chain_free_acceleration_matches_crba():
+    """qd_hat of a fixed-base revolute chain at rest = q_dot + dt * H^-1 tau_gravity."""
+    scene = _revolute_chain(np.random.default_rng(5), 7)
+    H, tau = _crba(scene)
     qdd = np.linalg.solve(H, tau)
     pw = pack_scene(scene)
     o = OracleWorld(pw, f64=True)
     o.substeps(1)
     dt = pw.hf("W_DT")
+    qdh = o.art[pw.A["A_QDH"]:pw.A["A_QDH"] + len(qdd)]
+    np.testing.assert_allclose(qdh / dt, qdd, rtol=2e-6)
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def test_implicit_drives_springs_and_damping_match_backward_euler_joint_dynamics():
+    """A chain at rest with a PD drive, armature, passive damping and a passive spring on every joint.  Backward
+    Euler on the drive and damping (forces evaluated at q + dt qd', qd'), forward on the spring, gives
+    (H + A + dt C + dt^2 K) qdd = tau_g + K (q_tgt - q) + C_d qd_tgt + K_s (q_ref - q), A armature, K drive
+    stiffness, C = drive + passive damping; qd_hat = dt qdd at rest.  Unsaturated: efforts are unlimited."""
+    rng = np.random.default_rng(11)
+    n = 6
+    cfg = [dict(targetKe=_f32(rng.uniform(20, 400)), targetKd=_f32(rng.uniform(0.5, 20)), targetPos=_f32(rng.uniform(-1, 1)),
+                targetVel=_f32(rng.uniform(-2, 2)), armature=_f32(rng.uniform(0, 0.02)), damping=_f32(rng.uniform(0, 0.5)), effortLimit=0.0)
+           for _ in range(n)]
+    scene = _revolute_chain(rng, n, dof=lambda i: cfg[i])
+    ks = [_f32(rng.uniform(0, 30)) for _ in range(n)]
+    qref = [_f32(rng.uniform(-1, 1)) for _ in range(n)]
+    scene.json["joints"]["spring_stiffness"] = ks
+    scene.json["joints"]["spring_ref"] = qref
+    H, tau = _crba(scene)
+    pw = pack_scene(scene)
+    dt = pw.hf("W_DT")
+    q = np.array(scene.json["joints"]["q"], np.float64)
+    K = np.array([c["targetKe"] for c in cfg])
+    Cd = np.array([c["targetKd"] for c in cfg])
+    C = Cd + np.array([c["damping"] for c in cfg])
+    A = np.array([c["armature"] for c in cfg])
+    tq = np.array([c["targetPos"] for c in cfg], np.float64)
+    tv = np.array([c["targetVel"] for c in cfg], np.float64)
+    rhs = tau + K * (tq - q) + Cd * tv + np.array(ks) * (np.array(qref) - q)
+    qdd = np.linalg.solve(H + np.diag(A + dt * C + dt * dt * K), rhs)
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
     qdh = o.art[pw.A["A_QDH"]:pw.A["A_QDH"] + n]
     np.testing.assert_allclose(qdh / dt, qdd, rtol=2e-6)
+
+
+def _vertical_hinge(**dof):
+    """One capsule on a vertical revolute axis: gravity exerts no torque about it and the joint-space equation
+    is I qdd = tau exactly, I the link's inertia about the axis."""
+    b = SceneBuilder(fps=240)
+    bd = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1])
+    b.addShapeCapsule(bd, radius=0.03, halfHeight=0.1, xform=[0.1, 0, 0, 0, 0.7071068, 0, 0.7071068])
+    b.addJointRevolute(-1, bd, axis=[0, 0, 1], parentXform=[0, 0, 1, 0, 0, 0, 1], q=0.2, **dof)
+    scene = b.finalize(solver=dict(substeps=1))
+    scene.json["bodies"]["angular_damping"] = [0.0]
+    return scene
+
+
+def test_moving_joint_drive_velocity_update_is_backward_euler():
+    """Moving hinge (qd != 0): qd' solves I (qd' - qd) / dt = K (q_tgt - q - dt qd') + C_d (qd_tgt - qd') - c qd'
+    + K_s (q_ref - q)."""
+    ke, kd, damp, arm, ks, qref, tq, tv, qd = 150.0, 4.0, 0.3, 0.01, 12.0, -0.4, 0.9, -1.5, 2.5
+    scene = _vertical_hinge(targetKe=ke, targetKd=kd, targetPos=tq, targetVel=tv, damping=damp, armature=arm, effortLimit=0.0)
+    scene.json["joints"]["spring_stiffness"] = [ks]
+    scene.json["joints"]["spring_ref"] = [qref]
+    H, _ = _crba(scene)
+    pw = pack_scene(scene)
+    pw.art0[pw.A["A_QD"]] = qd
+    dt = pw.hf("W_DT")
+    q = _f32(0.2)
+    qdv = _f32(qd)
+    I = H[0, 0] + _f32(arm)
+    qd_new = (I * qdv + dt * (_f32(ke) * (_f32(tq) - q) + _f32(kd) * _f32(tv) + _f32(ks) * (_f32(qref) - q))) / (
+        I + dt * (_f32(kd) + _f32(damp)) + dt * dt * _f32(ke))
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    np.testing.assert_allclose(o.art[pw.A["A_QDH"]], qd_new, rtol=2e-6)
+
+
+def test_saturated_drive_applies_exactly_the_effort_limit():
+    """A stiff drive far from its target on a light hinge saturates: the joint then receives +-effort as an
+    explicit torque (no implicit stiffness), qd_hat = dt effort / (I + armature) from rest."""
+    effort = 0.05
+    scene = _vertical_hinge(targetKe=1e4, targetKd=50.0, targetPos=1.5, effortLimit=effort, armature=0.002)
+    H, _ = _crba(scene)
+    pw = pack_scene(scene)
+    dt = pw.hf("W_DT")
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    np.testing.assert_allclose(o.art[pw.A["A_QDH"]], dt * _f32(effort) / (H[0, 0] + _f32(0.002)), rtol=2e-6)
+    # toward a target below the angle the torque is -effort
+    scene = _vertical_hinge(targetKe=1e4, targetKd=50.0, targetPos=-1.5, effortLimit=effort, armature=0.002)
+    pw = pack_scene(scene)
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    np.testing.assert_allclose(o.art[pw.A["A_QDH"]], -dt * _f32(effort) / (H[0, 0] + _f32(0.002)), rtol=2e-6)
+
+
+def test_drive_and_spring_settle_at_their_stiffness_weighted_equilibrium():
+    """Whole steps (solve + integration): a damped hinge under a drive and a spring comes to rest where their
+    torques cancel, q* = (K q_tgt + K_s q_ref) / (K + K_s)."""
+    ke, ks, tq, qref = 40.0, 10.0, 0.8, -0.6
+    scene = _vertical_hinge(targetKe=ke, targetKd=2.0, targetPos=tq, effortLimit=0.0)
+    scene.json["joints"]["spring_stiffness"] = [ks]
+    scene.json["joints"]["spring_ref"] = [qref]
+    pw = pack_scene(scene)
+    o = OracleWorld(pw)
+    o.substeps(2400)
+    qs = (ke * tq + ks * qref) / (ke + ks)
+    assert abs(o.art[pw.A["A_Q"]] - qs) < 1e-4
+    assert abs(o.art[pw.A["A_QD"]]) < 1e-3
 
 
 def test_fp32_oracle_tracks_fp64_for_one_step():
