@@ -31,7 +31,7 @@ struct steel_batch {
   uint32_t n;
   uint32_t substeps;
   uint32_t block = 256;
-  SolveLds sl;                     // k_solve LDS over the batch's worlds
+  LaunchLds sl;                    // dynamic LDS over the batch's worlds
   uint32_t* pair_offs = nullptr;   // k_narrow's per-world pair offsets (n + 1 words)
   uint32_t narrow_groups = 0;
   bool use_graph = true;

@@ -147,22 +147,22 @@ DEV void shape_pose(const W& w, uint32_t s) {  // wgsl.js:117-150
   w.d.poses[PA + s * 2 + 1] = F4(c + e, 0.0f);
 }
 
-// The reference's pair predicate (wgsl.js:158-172) minus the x-overlap the sweep already holds.
-DEV bool pair_kept(const W& w, uint32_t i, uint32_t j) {
-  int bi = ti(w.sf(w.so(i)).x), bj = ti(w.sf(w.so(j)).x);
-  if (bi == bj) return false;
-  if (bi < 0 && bj < 0) return false;
-  bool fi = bi < 0 || w.body_kind(bi) == 3 || (w.body_flags(bi) & 1) != 0;
-  bool fj = bj < 0 || w.body_kind(bj) == 3 || (w.body_flags(bj) & 1) != 0;
-  if (fi && fj) return false;
-  uint32_t NS = w.NS();
-  uint32_t k = i * NS + j;
-  if ((w.su(w.h(W_SU_FILTER) + (k >> 5)) & (1u << (k & 31))) != 0) return false;
-  uint32_t PA = w.h(W_P_AABB);
-  float4 lo_i = w.d.poses[PA + i * 2], hi_i = w.d.poses[PA + i * 2 + 1];
-  float4 lo_j = w.d.poses[PA + j * 2], hi_j = w.d.poses[PA + j * 2 + 1];
-  if (lo_i.x > hi_j.x || lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.x > hi_i.x || lo_j.y > hi_i.y || lo_j.z > hi_i.z) return false;
-  return true;
+// Shape tag for the pair predicate: body index + 1 in bits 0-30, bit 31 set when the shape cannot move
+// (static shape, or static or kinematic body).
+DEV uint32_t shape_tag(const W& w, uint32_t s) {
+  int b = ti(w.sf(w.so(s)).x);
+  bool fixed = b < 0 || w.body_kind(b) == 3 || (w.body_flags(b) & 1) != 0;
+  return (uint32_t)(b + 1) | (fixed ? 0x80000000u : 0u);
+}
+
+// The reference's pair predicate (wgsl.js:158-172) for shapes i < j with overlapping AABBs: different
+// bodies, not both unmovable, not filtered (explicit list, joint parent/child, world and group rules, all in
+// the filter bitset).
+DEV bool pair_allowed(const W& w, uint32_t i, uint32_t j, uint32_t tag_i, uint32_t tag_j) {
+  if ((tag_i & 0x7fffffffu) == (tag_j & 0x7fffffffu)) return false;
+  if ((tag_i & tag_j) >> 31) return false;
+  uint32_t k = i * w.NS() + j;
+  return (w.su(w.h(W_SU_FILTER) + (k >> 5)) & (1u << (k & 31))) == 0;
 }
 
 // Waves per SIMD each kernel is compiled for (a VGPR budget: 512 / waves).  Every phase of a world is a
@@ -177,20 +177,24 @@ DEV bool pair_kept(const W& w, uint32_t i, uint32_t j) {
 #define STEEL_WPE_SOLVE 4
 #endif
 
-// Sorted shape bounds staged in LDS for the sweep when NS fits (BROAD_LDS shapes).
-static constexpr uint32_t BROAD_LDS = 1024;
-
 // Shapes and broadphase, one workgroup per world: sort shapes by the order-preserving integer of AABB min x,
 // then sweep the sorted list; pair (i, j) survives iff it passes the reference's all-pairs predicate
-// (wgsl.js:154-177), tested in full for every candidate the sweep reaches.
+// (wgsl.js:154-177), tested in full for every candidate the sweep reaches.  Worlds of up to `cap` shapes
+// sweep from LDS.
 template <int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_broad(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_broad(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t cap) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ uint32_t s_hist[256];
   __shared__ uint32_t s_cnt[(BS / 64) * 256];
   __shared__ uint64_t s_red[2 * (BS / 64)];
   __shared__ uint32_t s_np;
-  __shared__ float4 s_lo[BROAD_LDS], s_hi[BROAD_LDS];  // sorted AABBs; .w holds the shape id
+  __shared__ uint32_t s_w[BS / 64];
+  // Dynamic LDS for worlds of up to `cap` shapes: the sorted AABBs (lo.w holds the shape id) and each sorted
+  // shape's first candidate in the flattened candidate list (cap + 1 words).
+  extern __shared__ float4 s_box[];
+  float4* s_lo = s_box;
+  float4* s_hi = s_box + cap;
+  uint32_t* s_run = (uint32_t*)(s_box + 2 * cap);
   uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
   const WorldDev wd = worlds[wid];
@@ -204,32 +208,76 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   for (uint32_t s = tid; s < NS; s += BS) { wd.skey[s] = fflip_u(fbits(wd.poses[PA + s * 2].x)); wd.sval[s] = s; }
   __syncthreads();
   block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), NS, s_hist, s_cnt, s_red);
-  const bool lds = NS <= BROAD_LDS;
+  const bool lds = NS <= cap;
+  const uint32_t MAXP = w.MAXP();
+  // Candidate (i, j), j after i in x order with min x_j <= max x_i.  The x intervals overlap (min x_i <= min
+  // x_j <= max x_j), so the AABB test is y and z, then the rest of the predicate.
+  auto test = [&](const float4& lo_i, const float4& hi_i, uint32_t si, uint32_t tag_i, const float4& lo_j, const float4& hi_j,
+                  uint32_t sj, uint32_t tag_j) {
+    if (lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.y > hi_i.y || lo_j.z > hi_i.z) return;
+    uint32_t a = min(si, sj), b = max(si, sj);
+    if (!pair_allowed(w, a, b, tag_i, tag_j)) return;
+    uint32_t slot = atomicAdd(&s_np, 1u);
+    if (slot < MAXP) { wd.pairs[slot * 2] = a; wd.pairs[slot * 2 + 1] = b; }
+  };
   if (lds) {
     for (uint32_t i = tid; i < NS; i += BS) {
       uint32_t si = wd.sval[i];
       float4 lo = wd.poses[PA + si * 2], hi = wd.poses[PA + si * 2 + 1];
       lo.w = ubits(si);
+      hi.w = ubits(shape_tag(w, si));
       s_lo[i] = lo; s_hi[i] = hi;
     }
     __syncthreads();
-  }
-  const uint32_t MAXP = w.MAXP();
-  for (uint32_t i = tid; i < NS; i += BS) {
-    float4 lo_i = lds ? s_lo[i] : wd.poses[PA + wd.sval[i] * 2];
-    float4 hi_i = lds ? s_hi[i] : wd.poses[PA + wd.sval[i] * 2 + 1];
-    const uint32_t si = lds ? fbits(lo_i.w) : wd.sval[i];
-    for (uint32_t j = i + 1; j < NS; j++) {
-      float4 lo_j = lds ? s_lo[j] : wd.poses[PA + wd.sval[j] * 2];
-      if (lo_j.x > hi_i.x) break;
-      float4 hi_j = lds ? s_hi[j] : wd.poses[PA + wd.sval[j] * 2 + 1];
-      // cheap bounds test first; pair_kept repeats it with the rest of the predicate
-      if (lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.y > hi_i.y || lo_j.z > hi_i.z) continue;
-      const uint32_t sj = lds ? fbits(lo_j.w) : wd.sval[j];
-      uint32_t a = min(si, sj), b = max(si, sj);
-      if (!pair_kept(w, a, b)) continue;
-      uint32_t slot = atomicAdd(&s_np, 1u);
-      if (slot < MAXP) { wd.pairs[slot * 2] = a; wd.pairs[slot * 2 + 1] = b; }
+    // Run length of each sorted shape (binary search for the first later min x above its max x), then the
+    // candidates are flattened by an exclusive scan and dealt out to the lanes in chunks of 8, so one shape
+    // with a long run (a ground plane spans every x) is shared by the whole workgroup.
+    for (uint32_t i = tid; i < NS; i += BS) {
+      const float hx = s_hi[i].x;
+      uint32_t lo_ = i + 1, hi_ = NS;
+      while (lo_ < hi_) {
+        uint32_t m = (lo_ + hi_) >> 1;
+        if (s_lo[m].x > hx) hi_ = m; else lo_ = m + 1;
+      }
+      s_run[i] = lo_ - i - 1;
+    }
+    __syncthreads();
+    const uint32_t T = block_scan_array<BS>(s_run, NS, s_w);
+    if (tid == 0) s_run[NS] = T;
+    __syncthreads();
+    constexpr uint32_t CH = 8;
+    for (uint32_t c = tid; c < (T + CH - 1) / CH; c += BS) {
+      const uint32_t k0 = c * CH, k1 = min(k0 + CH, T);
+      uint32_t lo_ = 0, hi_ = NS - 1;  // largest i with s_run[i] <= k0 (s_run[0] = 0, s_run[NS] = T > k0)
+      while (lo_ < hi_) {
+        uint32_t m = (lo_ + hi_ + 1) >> 1;
+        if (s_run[m] <= k0) lo_ = m; else hi_ = m - 1;
+      }
+      uint32_t i = lo_;
+      float4 lo_i = s_lo[i], hi_i = s_hi[i];
+      for (uint32_t k = k0; k < k1; k++) {
+        if (s_run[i + 1] <= k) {
+          do { i++; } while (s_run[i + 1] <= k);
+          lo_i = s_lo[i]; hi_i = s_hi[i];
+        }
+        const uint32_t j = i + 1 + (k - s_run[i]);
+        const float4 lo_j = s_lo[j], hi_j = s_hi[j];
+        test(lo_i, hi_i, fbits(lo_i.w), fbits(hi_i.w), lo_j, hi_j, fbits(lo_j.w), fbits(hi_j.w));
+      }
+    }
+  } else {
+    for (uint32_t i = tid; i < NS; i += BS) {
+      const uint32_t si = wd.sval[i];
+      const float4 lo_i = wd.poses[PA + si * 2], hi_i = wd.poses[PA + si * 2 + 1];
+      const uint32_t tag_i = shape_tag(w, si);
+      for (uint32_t j = i + 1; j < NS; j++) {
+        const uint32_t sj = wd.sval[j];
+        const float4 lo_j = wd.poses[PA + sj * 2];
+        if (lo_j.x > hi_i.x) break;
+        const float4 hi_j = wd.poses[PA + sj * 2 + 1];
+        if (lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.y > hi_i.y || lo_j.z > hi_i.z) continue;
+        test(lo_i, hi_i, si, tag_i, lo_j, hi_j, sj, shape_tag(w, sj));
+      }
     }
   }
   __syncthreads();
@@ -717,15 +765,17 @@ struct Prep {
 };
 
 // LDS of prepare's coloring phase (words).
-static constexpr uint32_t PREP_CH = 256, PREP_MASK = 1024, PREP_WC = 258;
+static constexpr uint32_t PREP_CH = 256, PREP_WC = 258;
 
+// cap: words of dynamic LDS, which hold a world's per-body color masks and first-pending slots when
+// NB * (NCOLW + 1) fits (steel_prep_lds_words).
 template <int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t cap) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ uint32_t s_w[BS / 64];
   __shared__ uint32_t s_wr[PREP_CH], s_wk[PREP_CH];  // coloring: row and body-pair key of a chunk of walk entries
-  __shared__ uint32_t s_mask[PREP_MASK];              // coloring: per-body color masks when NB * NCOLW fits
   __shared__ uint32_t s_wc[PREP_WC];                  // coloring: per-color counts and offsets when NCOL + 2 fits
+  extern __shared__ uint32_t s_pdyn[];                 // coloring: color masks, then first-pending slots
   __shared__ uint32_t s_tree[(BS / 64) * 64];        // factor: link tree of each wave's articulation
   __shared__ float s_pab[(BS / 64) * 512];           // factor: children's bias forces
   uint32_t wid = blockIdx.x;
@@ -933,12 +983,17 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     if (c < nc) sched[S.ROWOF + c] = r;
   }
   __syncthreads();
-  // First-fit in walk order is serial by definition; lanes first gather each walk entry's row and body pair
-  // into LDS so lane 0 colors from LDS (body color masks in LDS when they fit).  Per-color counts, unit tags
-  // (sums and maxima) and the color-ordered row list (a stable scatter) are then built in parallel.
-  const bool mask_lds = max(NB, 1u) * NCOLW <= PREP_MASK;
-  uint32_t* CM = mask_lds ? s_mask : sched + S.WMASK;
-  if (mask_lds) for (uint32_t i = tid; i < max(NB, 1u) * NCOLW; i += BS) s_mask[i] = 0;
+  // First-fit in walk order.  Lanes first gather each walk entry's row and body pair into LDS.  With the
+  // masks in LDS, wave 0 colors 64 entries at a time in rounds: an entry is colored once no earlier entry
+  // still pending shares one of its bodies (the lowest pending lane of each body wins an atomicMin), so
+  // every entry sees exactly the masks the serial walk gives it and entries colored together touch disjoint
+  // bodies.  Otherwise lane 0 colors serially from global masks.  Per-color counts, unit tags (sums and
+  // maxima) and the color-ordered row list (a stable scatter) are then built in parallel.
+  const uint32_t nbm = max(NB, 1u);
+  const bool mask_lds = nbm * (NCOLW + 1) <= cap;
+  uint32_t* LM = s_pdyn;                // nbm * NCOLW color masks
+  uint32_t* OWN = s_pdyn + nbm * NCOLW; // nbm first-pending-lane slots
+  if (mask_lds) for (uint32_t i = tid; i < nbm * NCOLW; i += BS) LM[i] = 0;
   const bool wc_lds = NCOL + 2 <= PREP_WC;
   uint32_t* WC = wc_lds ? s_wc : sched + S.WCOUNT;
   if (wc_lds) for (uint32_t i = tid; i < NCOL + 2; i += BS) s_wc[i] = 0;
@@ -953,12 +1008,34 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       s_wk[i] = r < totp ? sched[S.UKEY + sched[S.RUNIT + min(r, MAXC - 1)]] : 0u;
     }
     __syncthreads();
-    if (tid == 0) {
+    if (mask_lds) {
+      if (tid < 64) {
+        const uint32_t lane = tid;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+          const uint32_t i = i0 + lane;
+          const uint32_t r = i < n ? s_wr[i] : 0xffffffffu;
+          bool pending = r < totp;
+          const uint32_t key = pending ? s_wk[i] : 0u;
+          const int a = (int)(key & 0xffffu) - 1, b = (int)(key >> 16) - 1;
+          while (__ballot(pending) != 0) {
+            if (pending) { if (a >= 0) OWN[a] = 64u; if (b >= 0) OWN[b] = 64u; }
+            wave_sync();
+            if (pending) { if (a >= 0) atomicMin(&OWN[a], lane); if (b >= 0) atomicMin(&OWN[b], lane); }
+            wave_sync();
+            if (pending && (a < 0 || OWN[a] == lane) && (b < 0 || OWN[b] == lane)) {
+              sched[S.PCOL + r] = Prep::first_free_color(LM, NCOLW, NCOL, a, b);
+              pending = false;
+            }
+            wave_sync();
+          }
+        }
+      }
+    } else if (tid == 0) {
       for (uint32_t i = 0; i < n; i++) {
         uint32_t r = s_wr[i];
         if (r >= totp) continue;
         uint32_t key = s_wk[i];
-        sched[S.PCOL + r] = Prep::first_free_color(CM, NCOLW, NCOL, (int)(key & 0xffffu) - 1, (int)(key >> 16) - 1);
+        sched[S.PCOL + r] = Prep::first_free_color(sched + S.WMASK, NCOLW, NCOL, (int)(key & 0xffffu) - 1, (int)(key >> 16) - 1);
       }
     }
   }
@@ -1811,7 +1888,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
 // ------------------------------------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------------------------------------
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, const SolveLds& sl, uint32_t* pair_offs,
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, const LaunchLds& sl, uint32_t* pair_offs,
                                 uint32_t narrow_groups, hipStream_t stream, uint32_t stages, hipEvent_t* marks) {
   const bool twl = sl.tw_lds(block);
   const size_t solve_lds = sl.bytes(block);
@@ -1819,7 +1896,7 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
   dim3 grid(nworlds);
   if (marks) (void)hipEventRecord(marks[0], stream);
   if (stages & STAGE_COLLIDE) {
-    hipLaunchKernelGGL((k_broad<256>), grid, dim3(256), 0, stream, worlds, nworlds);
+    hipLaunchKernelGGL((k_broad<256>), grid, dim3(256), sl.broad_bytes(), stream, worlds, nworlds, sl.broad_cap());
     if (stages & STAGE_NARROW) {
       hipLaunchKernelGGL((k_pair_offsets<1024>), dim3(1), dim3(1024), 0, stream, worlds, nworlds, pair_offs);
       hipLaunchKernelGGL(k_narrow, dim3(narrow_groups), dim3(256), 0, stream, worlds, nworlds, (const uint32_t*)pair_offs);
@@ -1828,9 +1905,11 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
   }
   if (marks) (void)hipEventRecord(marks[1], stream);
   if (stages & STAGE_PREPARE) {
-    if (block == 64) hipLaunchKernelGGL((k_prepare<64>), grid, dim3(64), 0, stream, worlds, nworlds);
-    else if (block == 128) hipLaunchKernelGGL((k_prepare<128>), grid, dim3(128), 0, stream, worlds, nworlds);
-    else hipLaunchKernelGGL((k_prepare<256>), grid, dim3(256), 0, stream, worlds, nworlds);
+    const uint32_t pw = sl.prep_words();
+    const size_t pb = (size_t)pw * 4;
+    if (block == 64) hipLaunchKernelGGL((k_prepare<64>), grid, dim3(64), pb, stream, worlds, nworlds, pw);
+    else if (block == 128) hipLaunchKernelGGL((k_prepare<128>), grid, dim3(128), pb, stream, worlds, nworlds, pw);
+    else hipLaunchKernelGGL((k_prepare<256>), grid, dim3(256), pb, stream, worlds, nworlds, pw);
   }
   if (marks) (void)hipEventRecord(marks[2], stream);
   if (stages & STAGE_SOLVE) {

@@ -24,26 +24,41 @@ __host__ __device__ inline size_t solve_lds_bytes(uint32_t NL, uint32_t ND, uint
 // Upper bound of k_solve's dynamic LDS without the twists (worlds needing more are refused at creation).
 static constexpr size_t STEEL_SOLVE_LDS_MAX = 64 * 1024;
 
-// k_solve's LDS over the worlds of one launch: the largest world's bytes without and with the twists in LDS.
-// Twists go to LDS when that still lets four waves per SIMD stay resident (160 KB of LDS per CU shared by
-// 16 waves: 16 / (block / 64) workgroups).
-struct SolveLds {
+// Shapes whose sorted AABBs and candidate runs k_broad keeps in LDS (36 bytes each, 144 KB); a world with
+// more shapes sweeps from global memory.
+static constexpr uint32_t STEEL_BROAD_LDS_MAX = 4096;
+// Words of k_prepare's dynamic LDS: per-body color masks and first-pending slots, NB * (NCOLW + 1) words
+// (64 KB); a world needing more colors serially from global masks.
+static constexpr uint32_t STEEL_PREP_LDS_MAX_WORDS = 16384;
+
+// Dynamic LDS over the worlds of one launch.  k_solve: the largest world's bytes without and with the twists
+// in LDS; twists go to LDS when that still lets four waves per SIMD stay resident (160 KB of LDS per CU
+// shared by 16 waves: 16 / (block / 64) workgroups).  k_broad: the largest world's shape count, capped.
+// k_prepare: the largest world's coloring words, capped.
+struct LaunchLds {
   size_t notw = 0, tw = 0;
-  void add(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB) {
+  uint32_t ns = 0, cw = 0;
+  void add(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, uint32_t NS, uint32_t NCOLW) {
     notw = notw > solve_lds_bytes(NL, ND, mfloats, NB, false) ? notw : solve_lds_bytes(NL, ND, mfloats, NB, false);
     tw = tw > solve_lds_bytes(NL, ND, mfloats, NB, true) ? tw : solve_lds_bytes(NL, ND, mfloats, NB, true);
+    ns = ns > NS ? ns : NS;
+    uint32_t c = (NB > 0 ? NB : 1) * (NCOLW + 1);
+    cw = cw > c ? cw : c;
   }
   static size_t budget(uint32_t block) { return (size_t)160 * 1024 / (16 / (block / 64)) - 1024; }
   bool tw_lds(uint32_t block) const { return tw <= budget(block); }
   size_t bytes(uint32_t block) const { return tw_lds(block) ? tw : notw; }
+  uint32_t broad_cap() const { return ns < STEEL_BROAD_LDS_MAX ? ns : STEEL_BROAD_LDS_MAX; }
+  size_t broad_bytes() const { return (size_t)(broad_cap() > 0 ? broad_cap() : 1) * 36 + 16; }
+  uint32_t prep_words() const { return cw < STEEL_PREP_LDS_MAX_WORDS ? cw : STEEL_PREP_LDS_MAX_WORDS; }
 };
 
 // Workgroups of k_narrow per launch for a device of `cus` compute units (four 4-wave groups per CU).
 inline uint32_t steel_narrow_groups(uint32_t cus) { return cus * 4; }
 
-// sl: k_solve's LDS over the launch's worlds.
+// sl: dynamic LDS over the launch's worlds.
 // pair_offs: device scratch of nworlds + 1 words (per-world pair offsets for k_narrow).
 // narrow_groups: k_narrow's grid (steel_narrow_groups).
 // marks (optional): four events recorded before collide, prepare, solve and after solve.
-hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, const SolveLds& sl, uint32_t* pair_offs,
+hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, const LaunchLds& sl, uint32_t* pair_offs,
                                 uint32_t narrow_groups, hipStream_t stream, uint32_t stages, hipEvent_t* marks = nullptr);

@@ -53,7 +53,7 @@ struct ServerWorld {
   size_t off = 0, bytes = 0;         // pool block
   size_t koff[K_COUNT], ksize[K_COUNT];
   WorldDev dev;
-  uint32_t NL = 0, ND = 0, NB = 0, mfloats = 0;  // k_solve LDS inputs
+  uint32_t NL = 0, ND = 0, NB = 0, NS = 0, NCOLW = 0, mfloats = 0;  // dynamic LDS inputs
 };
 
 struct Lane {
@@ -69,7 +69,7 @@ struct Lane {
   uint32_t cap = 0;
   uint32_t substeps = 0;              // set by the lane's first world
   uint32_t block = 64;
-  SolveLds sl;                        // k_solve LDS over the lane's worlds
+  LaunchLds sl;                       // dynamic LDS over the lane's worlds
   std::vector<uint32_t> pending_free; // released while a frame may still use them
 };
 
@@ -184,7 +184,7 @@ int steel_server_admit(steel_server* s, const steel_world_desc* w, uint32_t lane
   Sizes sz;
   int rc = world_sizes(w->header, sz);
   if (rc) return rc;
-  SolveLds probe;
+  LaunchLds probe;
   rc = world_solve_lds(w->header, w->statF, probe);
   if (rc) return rc;
   ServerWorld W;
@@ -212,7 +212,7 @@ int steel_server_admit(steel_server* s, const steel_world_desc* w, uint32_t lane
     return fail(STEEL_ERROR, "world upload failed: %s", hipGetErrorString(e));
   }
   W.dev = world_dev_at(base);
-  W.NL = w->header[W_NL]; W.ND = w->header[W_ND]; W.NB = w->header[W_NB];
+  W.NL = w->header[W_NL]; W.ND = w->header[W_ND]; W.NB = w->header[W_NB]; W.NS = w->header[W_NS]; W.NCOLW = w->header[W_NCOLW];
   W.mfloats = world_mirror_floats(w->header, w->statF);
   uint32_t wid = s->next_id++;
   s->worlds[wid] = W;
@@ -256,11 +256,11 @@ int steel_server_step_lane(steel_server* s, uint32_t lane) {
       HIPCHK(hipMalloc(&L.pair_offs, sizeof(uint32_t) * (cap + 1)));
       L.cap = cap;
     }
-    L.sl = SolveLds();
+    L.sl = LaunchLds();
     for (uint32_t i = 0; i < n; i++) {
       const ServerWorld& W = s->worlds[L.members[i]];
       L.h_table[i] = W.dev;
-      L.sl.add(W.NL, W.ND, W.mfloats, W.NB);
+      L.sl.add(W.NL, W.ND, W.mfloats, W.NB, W.NS, W.NCOLW);
     }
     if (n) HIPCHK(hipMemcpyAsync(L.d_table, L.h_table, sizeof(WorldDev) * n, hipMemcpyHostToDevice, L.stream));
     L.dirty = false;

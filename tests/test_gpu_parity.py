@@ -37,6 +37,7 @@ SCENES = {
     "quad5": lambda: workloads.quadruped_world(5),
     "arm": lambda: workloads.arm_hulls(),
     "quad_driven": driven_quadruped,
+    "serve1500": lambda: workloads.serving_request(11, nmin=1500, nmax=1500)[0],
 }
 
 
@@ -44,7 +45,7 @@ def _bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-@pytest.mark.parametrize("name", ["boxes64", "quad5", "arm"])
+@pytest.mark.parametrize("name", ["boxes64", "quad5", "arm", "serve1500"])
 def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw], graph=False)
@@ -66,7 +67,7 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
     b.destroy()
 
 
-@pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200)])
+@pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])
@@ -98,6 +99,25 @@ def test_coloring_bitexact(sim, name):
     assert np.array_equal(sched[L["PCOL"]:L["PCOL"] + nr], info[:, 1])
     assert np.array_equal(sched[L["RUNIT"]:L["RUNIT"] + nr], info[:, 2])
     assert np.array_equal(sched[L["META"]:L["META"] + 4], o.dbg_bufs["meta"][:4])
+    b.destroy()
+
+
+def test_worlds_beyond_the_lds_capacities_take_the_global_paths_bitexact(sim):
+    """5640 shapes exceed k_broad's LDS sweep (4096 shapes) and 5639 bodies k_prepare's LDS coloring masks
+    (16384 words at 3 per body): both run from global memory, with the same results, in a batch beside a small
+    world that keeps the LDS paths."""
+    big = pack.pack_scene(workloads.serving_request(11, nmin=5600, nmax=5600)[0], capacity_policy="compact")
+    small = pack.pack_scene(workloads.quadruped_world(3))
+    assert big.NS > 4096 and big.NB * 3 > 16384
+    b = sim.Batch([small, big])
+    ob, osm = OracleWorld(big), OracleWorld(small)
+    b.step(16)
+    ob.frames(16)
+    osm.frames(16)
+    for i, o in ((0, osm), (1, ob)):
+        assert np.array_equal(_bits(b.read(i, "poses")), _bits(o.poses)), i
+        assert np.array_equal(_bits(b.read(i, "art")), _bits(o.art)), i
+    assert b.stats_raw(1)[0][5] > 0
     b.destroy()
 
 
