@@ -42,6 +42,9 @@ enum steel_world_field {
   /* per-substep parameters (float bits unless noted) */
   W_DT, W_BETA, W_MAX_DEPEN, W_GX, W_GY, W_GZ, W_WARM_SCALE, W_REG, W_ITERS /* u32 */, W_SUBSTEPS /* u32 */,
   W_FRAME_DT,
+  /* joint velocity limits: switch (u32), activation fraction (float bits), art offsets of the per-dof and
+     per-articulation activity flags */
+  W_VLIM_ON, W_F_VLIM_FRAC, W_A_VLG, W_A_AVL,
   W_FIELD_COUNT
 };
 

@@ -65,7 +65,9 @@ template <class T> struct World {
   uint32_t SF_BODY, SF_SHAPE, SF_HAABB, SF_VERTS, SF_FPLANE, SF_LINK, SF_DOF;
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
-  uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE;
+  uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
+  uint32_t VLIM_ON;
+  T F_VLIM_FRAC;
   T F_CFM, F_OMEGA, F_SPEC, F_GAP_GATE, F_FGAP, F_FSCALE, F_ANCHOR_BETA, F_BP_CELL, F_REST_SPEED, F_JL_GAP, F_JL_SPEED;
   T dt, beta, max_depen, warm_scale, reg;
   V3<T> g;
@@ -106,7 +108,8 @@ template <class T> struct World {
     P_BODY = h[W_P_BODY]; P_SHAPE = h[W_P_SHAPE]; P_AABB = h[W_P_AABB]; P_PREV = h[W_P_PREV];
     A_Q = h[W_A_Q]; A_QD = h[W_A_QD]; A_QDH = h[W_A_QDH]; A_TGT = h[W_A_TGT]; A_TGTV = h[W_A_TGTV]; A_LIM = h[W_A_LIM]; A_DRV = h[W_A_DRV];
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
-    A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE];
+    A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
+    VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC);
     F_CFM = hf(W_F_CFM); F_OMEGA = hf(W_F_OMEGA); F_SPEC = hf(W_F_SPEC); F_GAP_GATE = hf(W_F_GAP_GATE); F_FGAP = hf(W_F_FGAP);
     F_FSCALE = hf(W_F_FSCALE); F_ANCHOR_BETA = hf(W_F_ANCHOR_BETA); F_BP_CELL = hf(W_F_BP_CELL); F_REST_SPEED = hf(W_F_REST_SPEED);
     F_JL_GAP = hf(W_F_JL_GAP); F_JL_SPEED = hf(W_F_JL_SPEED);
@@ -1603,6 +1606,29 @@ template <class T> struct World {
     for (uint32_t a = 0; a < NA; a++) {
       uint32_t ai[4];
       art_info(a, ai);
+      if (VLIM_ON) {
+        // Velocity-limit prescale: the articulation's scalar dof velocities scaled so the most exceeded limit
+        // is met; then the dofs near their limit (activation fraction) and whether any is (wgsl.js:1828-1855).
+        T ratio = T(1);
+        for (uint32_t k = 0; k < ai[3]; k++) {
+          uint32_t d = ai[2] + k;
+          T vl = dof_passive(d).z;
+          if (vl < T(0)) continue;
+          T s = std::fabs(art[A_QD + d]);
+          if (s > vl) ratio = std::min(ratio, vl / s);
+        }
+        if (ratio < T(1))
+          for (uint32_t k = 0; k < ai[3]; k++)
+            if (scalar_dof(ai[2] + k)) art[A_QD + ai[2] + k] *= ratio;
+        bool any = false;
+        for (uint32_t k = 0; k < ai[3]; k++) {
+          uint32_t d = ai[2] + k;
+          bool on = std::fabs(art[A_QD + d]) >= F_VLIM_FRAC * dof_passive(d).z;
+          art[A_VLG + d] = on ? T(1) : T(0);
+          any = any || (on && dof_passive(d).z >= T(0) && scalar_dof(d));
+        }
+        art[A_AVL + a] = any ? T(1) : T(0);
+      }
       uint32_t n = 0;
       for (uint32_t k = 0; k < ai[3]; k++) {
         uint32_t d = ai[2] + k;
@@ -2251,6 +2277,29 @@ template <class T> struct World {
       art_dof_rows(a, l, d - link_dof(l), d);
     }
   }
+  // Velocity limits, last in a sweep: a stateless push of each active scalar dof back onto |qd| <= limit
+  // through the articulation (wgsl.js:3413-3419, 3462-3467).
+  void art_velocity_limits(uint32_t a) {
+    if (!(VLIM_ON && art[A_AVL + a] > T(0.5))) return;
+    uint32_t ai[4];
+    art_info(a, ai);
+    for (uint32_t k = 0; k < ai[1]; k++) {
+      uint32_t l = ai[0] + k, t = link_type(l);
+      if (t != JT_REVOLUTE && t != JT_PRISMATIC && t != JT_D6) continue;
+      for (uint32_t r = 0; r < link_ndof(l); r++) {
+        uint32_t d = link_dof(l) + r;
+        T hinv = mx(art[A_HINV + d], T(1e-12));
+        V4<T> pv = dof_passive(d);
+        if (pv.z >= T(0) && art[A_VLG + d] > T(0.5)) {
+          T over = aqd[d] - mn(mx(aqd[d], -pv.z), pv.z);
+          if (over != T(0)) {
+            art[lo(l) + L_TAU + r] = -over / hinv;
+            art_apply(a);
+          }
+        }
+      }
+    }
+  }
   void sweep(uint32_t mode, uint32_t ncol, uint32_t it, const std::vector<uint32_t>& wcs, const std::vector<uint32_t>& slot, uint32_t nu) {
     for (uint32_t a = 0; a < NA; a++) art_propagate_rows(a);
     if (PATCH && mode == 1) {
@@ -2268,7 +2317,10 @@ template <class T> struct World {
       uint32_t uu = std::min(u, MAXC - 1);
       solve_unit(slot[uu * 4 + 1], slot[uu * 4], (int)slot[uu * 4 + 2], (int)slot[uu * 4 + 3], mode, it);
     }
-    for (uint32_t a = 0; a < NA; a++) art_contact_impulses(a);
+    for (uint32_t a = 0; a < NA; a++) {
+      art_contact_impulses(a);
+      art_velocity_limits(a);
+    }
   }
   V3<T> gyroscopic_velocity(const M3<T>& inertia, V3<T> u0, T h0) const {  // wgsl.js:2543-2562
     auto inverse3 = [](const M3<T>& m) {

@@ -803,6 +803,29 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const float dt = w.hf(W_DT);
   for (uint32_t a = tid; a < NA; a += BS) {
     uint4 ai = w.art_info(a);
+    if (w.h(W_VLIM_ON)) {
+      // Velocity-limit prescale: the articulation's scalar dof velocities scaled so the most exceeded limit is
+      // met; then the dofs near their limit (activation fraction) and whether any is (wgsl.js:1828-1855).
+      float ratio = 1.0f;
+      for (uint32_t k = 0; k < ai.w; k++) {
+        float vl = w.dof_passive(ai.z + k).z;
+        if (vl < 0.0f) continue;
+        float s = fabsf(art_[w.h(W_A_QD) + ai.z + k]);
+        if (s > vl) ratio = mn(ratio, vl / s);
+      }
+      if (ratio < 1.0f)
+        for (uint32_t k = 0; k < ai.w; k++)
+          if (w.scalar_dof(ai.z + k)) art_[w.h(W_A_QD) + ai.z + k] *= ratio;
+      const float frac = w.hf(W_F_VLIM_FRAC);
+      bool any = false;
+      for (uint32_t k = 0; k < ai.w; k++) {
+        uint32_t dd = ai.z + k;
+        bool on = fabsf(art_[w.h(W_A_QD) + dd]) >= frac * w.dof_passive(dd).z;
+        art_[w.h(W_A_VLG) + dd] = on ? 1.0f : 0.0f;
+        any = any || (on && w.dof_passive(dd).z >= 0.0f && w.scalar_dof(dd));
+      }
+      art_[w.h(W_A_AVL) + a] = any ? 1.0f : 0.0f;
+    }
     uint32_t n = 0;
     for (uint32_t k = 0; k < ai.w; k++) {
       uint32_t dd = ai.z + k;
@@ -1534,6 +1557,29 @@ struct Solver {
       }
     }
   }
+  // Velocity limits, last in a sweep: a stateless push of each active scalar dof back onto |qd| <= limit
+  // through the articulation (wgsl.js:3413-3419, 3462-3467); a wave per articulation, as the limit rows.
+  DEVN void art_velocity_limits(uint32_t a, uint32_t lane) const {
+    float* art_ = w.d.art;
+    if (!(w.h(W_VLIM_ON) && art_[w.h(W_A_AVL) + a] > 0.5f)) return;
+    uint4 ai = w.art_info(a);
+    for (uint32_t k = 0; k < ai.y; k++) {
+      uint32_t l = ai.x + k, t = w.link_type(l);
+      if (t != JT_REVOLUTE && t != JT_PRISMATIC && t != JT_D6) continue;
+      for (uint32_t r = 0; r < w.link_ndof(l); r++) {
+        uint32_t dd = w.link_dof(l) + r;
+        float hinv = mx(art_[w.h(W_A_HINV) + dd], 1e-12f);
+        float4 pv = w.dof_passive(dd);
+        if (pv.z >= 0.0f && art_[w.h(W_A_VLG) + dd] > 0.5f) {
+          float over = laqd[dd] - clampf_(laqd[dd], -pv.z, pv.z);
+          if (over != 0.0f) {
+            if (lane == 0) mTAU(l)[r] = -over / hinv;
+            art_apply(a, lane);
+          }
+        }
+      }
+    }
+  }
   DEVN f3 gyroscopic_velocity(const m3& inertia, f3 u0, float h0) const {  // wgsl.js:2543-2562
     auto inverse3 = [](const m3& m) {
       f3 a = m.c[0], b = m.c[1], c = m.c[2];
@@ -1632,7 +1678,10 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
     }
   }
   __syncthreads();
-  for (uint32_t a = wave; a < NA; a += BS / 64) sv.art_contact_impulses(a, lane);
+  for (uint32_t a = wave; a < NA; a += BS / 64) {
+    sv.art_contact_impulses(a, lane);
+    sv.art_velocity_limits(a, lane);
+  }
   __syncthreads();
 }
 

@@ -158,6 +158,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     SO = dict(S.get("solver") or {})
     SO.update(o.get("solver") or {})
     limitsOn = bool(SO.get("enable_joint_limits", False))
+    vlimOn = bool(SO.get("enable_joint_velocity_limits", False))
     q0, tq0, tqd0 = [], [], []
     qMap, dofMap = {}, {}
     loops = []
@@ -193,7 +194,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
                                  friction=friction, axis=J["axis"][qd], scalar=scalar,
                                  springKe=(J["spring_stiffness"][qd] if J.get("spring_stiffness") else 0) if scalar else 0,
                                  springRef=(J["spring_ref"][qd] if J.get("spring_ref") else 0),
-                                 loadCoef=0, velLimit=-1))
+                                 loadCoef=0,
+                                 velLimit=J["vel_limit"][qd] if scalar and vlimOn and J["vel_limit"][qd] < 1e5 else -1))
                 tqd0.append(J["target_qd"][qd] if J.get("target_qd") else 0)
             for k in range(nq):
                 qMap[J["q_start"][j] + k] = len(q0)
@@ -480,7 +482,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         if SV.get(k) is not None and SV.get(k) != default:
             raise ValueError(f"{k}={SV[k]} is not supported")
     for k in ("enable_sleeping", "nesterov_momentum", "enable_torsional_rolling_friction", "contact_compliance", "contact_shared_anchor",
-              "contact_friction_shared_anchor", "enable_joint_velocity_limits", "enable_joint_friction"):
+              "contact_friction_shared_anchor", "enable_joint_friction"):
         if SV.get(k) or SO.get(k):
             raise UnsupportedError(f"{k} is not supported yet")
     if SV.get("velocity_iterations", 0):
@@ -569,8 +571,9 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for k, v in P.items():
         H("W_" + k, v)
     for k in ("A_Q", "A_QD", "A_QDH", "A_TGT", "A_TGTV", "A_LIM", "A_DRV", "A_JF", "A_BF", "A_FRIC", "A_HINV", "A_LINK", "A_LG", "A_RL",
-              "A_RLN", "A_ULOAD", "A_DENSE"):
+              "A_RLN", "A_ULOAD", "A_DENSE", "A_VLG", "A_AVL"):
         H("W_" + k, A[k])
+    H("W_VLIM_ON", 1 if vlimOn else 0)
     gate = SV.get("contact_gap_gate")
     fgap = SV.get("contact_friction_gap_max")
     jlg = SV.get("joint_limit_activation_gap")
@@ -581,7 +584,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
                  ("W_F_JL_SPEED", 2 if jlg is None else 0),
                  ("W_DT", dt), ("W_BETA", SV["baumgarte"]), ("W_MAX_DEPEN", 2.0), ("W_GX", grav[0]), ("W_GY", grav[1]), ("W_GZ", grav[2]),
                  ("W_WARM_SCALE", warm * SV["contact_warmstart_decay"]), ("W_REG", 0 if SV.get("contact_compliance") else o["regularization"]),
-                 ("W_FRAME_DT", 1 / fps)):
+                 ("W_FRAME_DT", 1 / fps), ("W_F_VLIM_FRAC", SV["velocity_limit_activation_fraction"])):
         HF(k, v)
 
     pw.statF = statF

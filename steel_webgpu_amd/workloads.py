@@ -166,10 +166,10 @@ def arm_hulls(seed=SEED + 1, nlinks=32, nhulls=256, fps=240, substeps=8, iterati
     return b.finalize(solver=dict(substeps=substeps, iterations=iterations))
 
 
-def quadruped_world(index, seed=SEED + 2, fps=200, substeps=4, iterations=12, kmax=64, drive=None):
+def quadruped_world(index, seed=SEED + 2, fps=200, substeps=4, iterations=12, kmax=64, drive=None, solver=None):
     """One world of the batch: floating-base quadruped (box base, 4 legs x 3 revolute capsule links) + K boxes.
     `drive` (dict of targetKe, targetKd, effortLimit, ...) gives every leg joint a PD drive holding its initial
-    angle; the batch workload has none."""
+    angle, `solver` adds solver options; the batch workload has neither."""
     rng = PCG32(seed, index)
     b = SceneBuilder(fps=fps)
     b.addGroundPlane(mu=0.6)
@@ -180,7 +180,7 @@ def quadruped_world(index, seed=SEED + 2, fps=200, substeps=4, iterations=12, km
     placed = [(np.array([0.0, 0.0, base_z - 0.2]), np.eye(3), np.array([0.45, 0.3, 0.45]), 0.71)]
     for _ in range(K):
         _add_box(b, rng, placed, (-1.5, -1.5, 0.2), (1.5, 1.5, 2.2), 0.6)
-    return b.finalize(solver=dict(substeps=substeps, iterations=iterations))
+    return b.finalize(solver=dict(substeps=substeps, iterations=iterations, **(solver or {})))
 
 
 # --- serving requests ----------------------------------------------------------------------------------------

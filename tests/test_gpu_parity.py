@@ -38,6 +38,9 @@ SCENES = {
     "arm": lambda: workloads.arm_hulls(),
     "quad_driven": driven_quadruped,
     "serve1500": lambda: workloads.serving_request(11, nmin=1500, nmax=1500)[0],
+    # joint velocity limits of 0.5 rad/s on driven legs (rows active at any speed: activation fraction 0)
+    "quad_vlim": lambda: workloads.quadruped_world(4, drive=dict(targetKe=60.0, targetKd=2.0, velocityLimit=0.5),
+                                                   solver=dict(enable_joint_velocity_limits=True)),
 }
 
 
@@ -67,7 +70,7 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
     b.destroy()
 
 
-@pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40)])
+@pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40), ("quad_vlim", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])
@@ -191,7 +194,7 @@ def test_joint_targets_set_mid_run_bitexact(sim):
 
 
 def test_one_step_error_against_fp64_within_1e5(sim):
-    for name in ("boxes64", "quad0", "quad5", "quad_driven"):
+    for name in ("boxes64", "quad0", "quad5", "quad_driven", "quad_vlim"):
         pw = pack.pack_scene(SCENES[name]())
         b = sim.Batch([pw])
         o64 = OracleWorld(pw, f64=True)

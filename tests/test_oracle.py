@@ -441,6 +441,76 @@ def test_saturated_drive_applies_exactly_the_effort_limit():
     np.testing.assert_allclose(o.art[pw.A["A_QDH"]], -dt * _f32(effort) / (H[0, 0] + _f32(0.002)), rtol=2e-6)
 
 
+def _spin_hinge(qd, vlim, frac=None, **dof):
+    scene = _vertical_hinge(velocityLimit=vlim, **dof)
+    scene.json["solver"]["enable_joint_velocity_limits"] = True
+    if frac is not None:
+        scene.json["solver"]["velocity_limit_activation_fraction"] = frac
+    pw = pack_scene(scene)
+    pw.art0[pw.A["A_QD"]] = qd
+    return pw
+
+
+def test_velocity_limit_caps_a_spinning_hinge_and_leaves_slower_ones_alone():
+    """A hinge spinning at 5 rad/s under a 2 rad/s limit leaves its first step at the limit (nothing else acts
+    on it: vertical axis, no damping) and moves by dt times the limit; at 1 rad/s with the rows active only
+    beyond 0.9 of the limit nothing changes."""
+    pw = _spin_hinge(5.0, 2.0)
+    dt = pw.hf("W_DT")
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    A = pw.A
+    assert abs(o.art[A["A_QD"]] - 2.0) <= 2e-7
+    assert abs(o.art[A["A_Q"]] - (_f32(0.2) + dt * 2.0)) <= 1e-7
+    pw = _spin_hinge(1.0, 2.0, frac=0.9)
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    assert o.art[A["A_VLG"]] == 0.0 and o.art[A["A_AVL"]] == 0.0
+    assert o.art[A["A_QD"]] == 1.0
+    # the same hinge without the switch keeps spinning at 5 rad/s
+    scene = _vertical_hinge(velocityLimit=2.0)
+    pw = pack_scene(scene)
+    pw.art0[pw.A["A_QD"]] = 5.0
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    assert o.art[A["A_QD"]] == 5.0
+
+
+def test_velocity_limit_holds_against_a_drive():
+    """A stiff drive toward a far target accelerates the hinge; the velocity limit holds |qd| at the limit
+    while the drive would go faster, and the joint still arrives."""
+    pw = _spin_hinge(0.0, 1.5, targetKe=400.0, targetKd=1.0, targetPos=2.0, effortLimit=0.0)
+    o = OracleWorld(pw)
+    A = pw.A
+    peak = 0.0
+    for _ in range(480):
+        o.substeps(1)
+        peak = max(peak, abs(float(o.art[A["A_QD"]])))
+    assert 1.5 * (1 - 1e-3) <= peak <= 1.5 * (1 + 1e-6)
+    assert abs(o.art[A["A_Q"]] - 2.0) < 0.05
+
+
+def test_velocity_limit_prescale_scales_every_scalar_dof_of_the_articulation():
+    """The pre-solve prescale multiplies all scalar dof velocities of an articulation by the ratio that meets
+    its most exceeded limit (wgsl.js:1828-1845).  A 3-link chain with one dof at 4x its 0.5 rad/s limit
+    (ratio 1/4, exact in binary) has the same free velocity qd_hat as the same chain started at qd / 4 without
+    velocity limits, and a different one from the chain started at qd."""
+    qd = np.array([0.3, 2.0, -0.7], np.float32)
+
+    def qd_hat(limits, start):
+        scene = _revolute_chain(np.random.default_rng(2), 3, dof=lambda i: dict(velocityLimit=0.5 if i == 1 else 1e6))
+        scene.json["solver"]["enable_joint_velocity_limits"] = limits
+        pw = pack_scene(scene)
+        pw.art0[pw.A["A_QD"]:pw.A["A_QD"] + 3] = start
+        o = OracleWorld(pw, f64=True)
+        o.substeps(1)
+        return o.art[pw.A["A_QDH"]:pw.A["A_QDH"] + 3]
+
+    limited = qd_hat(True, qd)
+    assert np.array_equal(limited, qd_hat(False, qd * np.float32(0.25)))
+    assert np.abs(limited - qd_hat(False, qd)).max() > 0.1
+
+
 def test_drive_and_spring_settle_at_their_stiffness_weighted_equilibrium():
     """Whole steps (solve + integration): a damped hinge under a drive and a spring comes to rest where their
     torques cancel, q* = (K q_tgt + K_s q_ref) / (K + K_s)."""
