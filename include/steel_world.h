@@ -45,6 +45,8 @@ enum steel_world_field {
   /* joint velocity limits: switch (u32), activation fraction (float bits), art offsets of the per-dof and
      per-articulation activity flags */
   W_VLIM_ON, W_F_VLIM_FRAC, W_A_VLG, W_A_AVL,
+  /* free-body velocity limits (u32 switch); load-dependent joint friction (u32 switch) and its art offset */
+  W_VLIM_FREE, W_JFL, W_A_JFL,
   W_FIELD_COUNT
 };
 

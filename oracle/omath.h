@@ -9,7 +9,11 @@
 
 namespace oracle {
 
-template <class T> struct V3 { T x, y, z; };
+template <class T> struct V3 {
+  T x, y, z;
+  T& operator[](uint32_t k) { return k == 0 ? x : (k == 1 ? y : z); }
+  T operator[](uint32_t k) const { return k == 0 ? x : (k == 1 ? y : z); }
+};
 template <class T> struct V4 { T x, y, z, w; };
 template <class T> struct V2 { T x, y; };
 

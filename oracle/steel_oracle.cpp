@@ -66,7 +66,7 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t VLIM_ON;
+  uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL;
   T F_VLIM_FRAC;
   T F_CFM, F_OMEGA, F_SPEC, F_GAP_GATE, F_FGAP, F_FSCALE, F_ANCHOR_BETA, F_BP_CELL, F_REST_SPEED, F_JL_GAP, F_JL_SPEED;
   T dt, beta, max_depen, warm_scale, reg;
@@ -109,7 +109,7 @@ template <class T> struct World {
     A_Q = h[W_A_Q]; A_QD = h[W_A_QD]; A_QDH = h[W_A_QDH]; A_TGT = h[W_A_TGT]; A_TGTV = h[W_A_TGTV]; A_LIM = h[W_A_LIM]; A_DRV = h[W_A_DRV];
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
-    VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC);
+    VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     F_CFM = hf(W_F_CFM); F_OMEGA = hf(W_F_OMEGA); F_SPEC = hf(W_F_SPEC); F_GAP_GATE = hf(W_F_GAP_GATE); F_FGAP = hf(W_F_FGAP);
     F_FSCALE = hf(W_F_FSCALE); F_ANCHOR_BETA = hf(W_F_ANCHOR_BETA); F_BP_CELL = hf(W_F_BP_CELL); F_REST_SPEED = hf(W_F_REST_SPEED);
     F_JL_GAP = hf(W_F_JL_GAP); F_JL_SPEED = hf(W_F_JL_SPEED);
@@ -1155,6 +1155,13 @@ template <class T> struct World {
         acc = sadd(acc, sscale(ldS(l, r), qdd));
       }
       stsv(L + L_A, acc);
+      if (JFL) {
+        // Load-dependent joint friction: the joint's transmitted force and torque under the free dynamics
+        // (articulated force at the child) raise its friction bound (artwgsl.js:361-367).
+        SV<T> f = sadd(imul(L + L_IA, acc), ldsv(L + L_PA));
+        T fm = length(f.l) + length(f.a);
+        for (uint32_t r = 0; r < n; r++) { uint32_t d = link_dof(l) + r; art[A_JFL + d] = statF[SF_DOF + d * 5 + 3].w * fm; }
+      }
     }
   }
   uint32_t link_path(uint32_t l0, uint32_t* path) const {
@@ -1641,12 +1648,24 @@ template <class T> struct World {
           on = q <= lim.x + gg || q >= lim.y - gg;
         }
         art[A_LG + d] = on ? T(1) : T(0);
-        bool row = on || dof_passive(d).y > T(0);
+        bool row = on || dof_passive(d).y > T(0) || JFL;
         if (row && scalar_dof(d)) { art[A_RL + ai[2] + n] = (T)d; n++; }
       }
       art[A_RLN + a] = (T)n;
       art_fk(a);
       art_factor(a);
+    }
+    if (VLIM_FREE) {
+      // Free-body velocity limits: per axis, whether the body's speed is near its limit (wgsl.js:1878-1887).
+      for (uint32_t b = 0; b < NB; b++) {
+        if (body_kind((int)b) != 1) continue;
+        V4<T> lm = body_limits((int)b);
+        V3<T> v = xyz(vel[b * 2]), w = xyz(vel[b * 2 + 1]);
+        for (uint32_t k = 0; k < 3; k++) {
+          art[A_VLG + ND + b * 6 + k] = std::fabs(v[k]) >= F_VLIM_FRAC * lm.y ? T(1) : T(0);
+          art[A_VLG + ND + b * 6 + 3 + k] = std::fabs(w[k]) >= F_VLIM_FRAC * lm.z ? T(1) : T(0);
+        }
+      }
     }
     // Phase 1
     uint32_t nc = std::min(counters[1], MAXC);
@@ -2248,8 +2267,20 @@ template <class T> struct World {
     }
     if (hit) art_apply(a);
   }
-  void art_dof_rows(uint32_t a, uint32_t l, uint32_t r, uint32_t d) {  // wgsl.js:3421-3468, limit rows
+  void art_dof_rows(uint32_t a, uint32_t l, uint32_t r, uint32_t d) {  // wgsl.js:3421-3468, friction and limit rows
     T hinv = mx(art[A_HINV + d], T(1e-12));
+    // Joint friction: a velocity row with |impulse| <= friction * dt (plus the load term).
+    T fr = dof_passive(d).y;
+    if (JFL) fr += art[A_JFL + d];
+    if (fr > T(0)) {
+      T lam = art[A_FRIC + d];
+      T nl = mn(mx(lam - F_OMEGA * aqd[d] / hinv, -fr * dt), fr * dt);
+      if (nl != lam) {
+        art[A_FRIC + d] = nl;
+        art[lo(l) + L_TAU + r] = nl - lam;
+        art_apply(a);
+      }
+    }
     V4<T> lim = dof_limit(d);
     if (art[A_LG + d] > T(0.5)) {
       T q = art[A_Q + link_qs(l) + r];
@@ -2320,6 +2351,32 @@ template <class T> struct World {
     for (uint32_t a = 0; a < NA; a++) {
       art_contact_impulses(a);
       art_velocity_limits(a);
+    }
+    if (VLIM_FREE) for (uint32_t b = 0; b < NB; b++) body_velocity_limits(b);
+  }
+  // Free-body velocity limits: a per-axis box on v (clamped) and w (pushed back through the body's inverse
+  // inertia), stateless (wgsl.js:3586-3610).
+  void body_velocity_limits(uint32_t b) {
+    if (body_kind((int)b) != 1 || (body_flags((int)b) & 1) != 0) return;
+    V4<T> lm = body_limits((int)b);
+    uint32_t g = A_VLG + ND + b * 6;
+    if (lm.y > T(0)) {
+      V3<T> v = xyz(tw[b * 2]);
+      for (uint32_t k = 0; k < 3; k++) if (art[g + k] > T(0.5)) v[k] = mn(mx(v[k], -lm.y), lm.y);
+      tw[b * 2] = v4(v, T(0));
+    }
+    if (lm.z > T(0)) {
+      V3<T> w = xyz(tw[b * 2 + 1]);
+      for (uint32_t k = 0; k < 3; k++) {
+        T over = w[k] - mn(mx(w[k], -lm.z), lm.z);
+        if (over == T(0) || art[g + 3 + k] < T(0.5)) continue;
+        V3<T> e{0, 0, 0};
+        e[k] = T(1);
+        M3<T> R = qmat(poses[P_BODY + b * 2 + 1]);
+        V3<T> iw = mul(R, mul(body_inv_inertia((int)b), mul(transpose(R), e)));
+        w = w - iw * (over / iw[k]);
+      }
+      tw[b * 2 + 1] = v4(w, T(0));
     }
   }
   V3<T> gyroscopic_velocity(const M3<T>& inertia, V3<T> u0, T h0) const {  // wgsl.js:2543-2562

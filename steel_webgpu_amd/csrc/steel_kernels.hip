@@ -838,7 +838,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
         on = q <= lim.x + gg || q >= lim.y - gg;
       }
       art_[w.h(W_A_LG) + dd] = on ? 1.0f : 0.0f;
-      bool rowf = on || w.dof_passive(dd).y > 0.0f;
+      bool rowf = on || w.dof_passive(dd).y > 0.0f || w.h(W_JFL) != 0;
       if (rowf && w.scalar_dof(dd)) { art_[w.h(W_A_RL) + ai.z + n] = (float)dd; n++; }
     }
     art_[w.h(W_A_RLN) + a] = (float)n;
@@ -851,6 +851,20 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       w.art_tree(a, tree);
       w.art_fk_lv(a, k < w.art_info(a).y ? (tree[k] & 0xffu) : 0xffu);
       w.art_factor_lv(a, tree, s_pab + (tid >> 6) * 512);
+    }
+  }
+  if (w.h(W_VLIM_FREE)) {
+    // Free-body velocity limits: per axis, whether the body's speed is near its limit (wgsl.js:1878-1887).
+    const float frac = w.hf(W_F_VLIM_FRAC);
+    for (uint32_t b = tid; b < NB; b += BS) {
+      if (w.body_kind((int)b) != 1) continue;
+      q4 lm = w.body_limits((int)b);
+      f3 v = xyz(wd.vel[b * 2]), wv = xyz(wd.vel[b * 2 + 1]);
+      const uint32_t g = w.h(W_A_VLG) + ND + b * 6;
+      for (int k = 0; k < 3; k++) {
+        art_[g + k] = fabsf(comp(v, k)) >= frac * lm.y ? 1.0f : 0.0f;
+        art_[g + 3 + k] = fabsf(comp(wv, k)) >= frac * lm.z ? 1.0f : 0.0f;
+      }
     }
   }
   __syncthreads();
@@ -1540,6 +1554,18 @@ struct Solver {
       float4 lim = w.dof_limit(dd);
       uint32_t l = tu(lim.w), r = dd - w.link_dof(l);
       float hinv = mx(art_[w.h(W_A_HINV) + dd], 1e-12f);
+      // Joint friction: a velocity row with |impulse| <= friction * dt (plus the load term).
+      float fr = w.dof_passive(dd).y;
+      if (w.h(W_JFL)) fr += art_[w.h(W_A_JFL) + dd];
+      if (fr > 0.0f) {
+        float lam = art_[w.h(W_A_FRIC) + dd];
+        float nl = clampf_(lam - omega * laqd[dd] / hinv, -fr * dt, fr * dt);
+        if (nl != lam) {
+          art_[w.h(W_A_FRIC) + dd] = nl;
+          if (lane == 0) mTAU(l)[r] = nl - lam;
+          art_apply(a, lane);
+        }
+      }
       if (art_[w.h(W_A_LG) + dd] > 0.5f) {
         float q = art_[w.h(W_A_Q) + w.link_qs(l) + r];
         for (uint32_t side = 0; side < 2; side++) {
@@ -1578,6 +1604,31 @@ struct Solver {
           }
         }
       }
+    }
+  }
+  // Free-body velocity limits: a per-axis box on v (clamped) and w (pushed back through the body's inverse
+  // inertia), stateless (wgsl.js:3586-3610).
+  DEV void body_velocity_limits(uint32_t b) const {
+    if (w.body_kind((int)b) != 1 || (w.body_flags((int)b) & 1) != 0) return;
+    const float* art_ = w.d.art;
+    q4 lm = w.body_limits((int)b);
+    const uint32_t g = w.h(W_A_VLG) + w.ND() + b * 6;
+    if (lm.y > 0.0f) {
+      f3 v = xyz(tw[b * 2]);
+      for (int k = 0; k < 3; k++) if (art_[g + k] > 0.5f) v = with_comp(v, k, clampf_(comp(v, k), -lm.y, lm.y));
+      tw[b * 2] = F4(v, 0.0f);
+    }
+    if (lm.z > 0.0f) {
+      f3 wv = xyz(tw[b * 2 + 1]);
+      for (int k = 0; k < 3; k++) {
+        float over = comp(wv, k) - clampf_(comp(wv, k), -lm.z, lm.z);
+        if (over == 0.0f || art_[g + 3 + k] < 0.5f) continue;
+        f3 e = with_comp(mk3(0, 0, 0), k, 1.0f);
+        m3 R = qmat(w.pose_q((int)b));
+        f3 iw = mul(R, mul(w.body_inv_inertia((int)b), mul(tr(R), e)));
+        wv = wv - iw * (over / comp(iw, k));
+      }
+      tw[b * 2 + 1] = F4(wv, 0.0f);
     }
   }
   DEVN f3 gyroscopic_velocity(const m3& inertia, f3 u0, float h0) const {  // wgsl.js:2543-2562
@@ -1681,6 +1732,10 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
   for (uint32_t a = wave; a < NA; a += BS / 64) {
     sv.art_contact_impulses(a, lane);
     sv.art_velocity_limits(a, lane);
+  }
+  if (w.h(W_VLIM_FREE)) {
+    __syncthreads();
+    for (uint32_t b = tid; b < w.NB(); b += BS) sv.body_velocity_limits(b);
   }
   __syncthreads();
 }

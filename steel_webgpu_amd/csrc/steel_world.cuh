@@ -354,6 +354,16 @@ struct W {
         acc = sadd(acc, sscale(ldS(l, r), qdd));
       }
       stsv(L + L_A, acc);
+      if (H[W_JFL]) {
+        // Load-dependent joint friction: the joint's transmitted force and torque under the free dynamics
+        // (articulated force at the child) raise its friction bound (artwgsl.js:361-367).
+        SVd f = sadd(imul(L + L_IA, acc), ldsv(L + L_PA));
+        float fm = len(f.l) + len(f.a);
+        for (uint32_t r = 0; r < n; r++) {
+          uint32_t dd = link_dof(l) + r;
+          art_[H[W_A_JFL] + dd] = sf(H[W_SF_DOF] + dd * 5 + 3).w * fm;
+        }
+      }
     }
   }
   // Factor of articulation a, one wave (lane k: link ai.x + k at depth dl).  tree[k]: bits 0-7 depth, 8-15 the

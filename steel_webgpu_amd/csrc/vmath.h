@@ -57,6 +57,7 @@ HD float mx(float a, float b) { return a < b ? b : a; }
 HD float clampf_(float x, float lo, float hi) { return mn(mx(x, lo), hi); }
 HD bool vne0(f3 a) { return a.x != 0.0f || a.y != 0.0f || a.z != 0.0f; }
 HD float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+HD f3 with_comp(f3 v, int k, float x) { if (k == 0) v.x = x; else if (k == 1) v.y = x; else v.z = x; return v; }
 HD void setc(f3& v, int k, float s) { if (k == 0) v.x = s; else if (k == 1) v.y = s; else v.z = s; }
 
 HD f3 qrot(q4 q, f3 v) {

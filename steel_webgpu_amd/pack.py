@@ -194,7 +194,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
                                  friction=friction, axis=J["axis"][qd], scalar=scalar,
                                  springKe=(J["spring_stiffness"][qd] if J.get("spring_stiffness") else 0) if scalar else 0,
                                  springRef=(J["spring_ref"][qd] if J.get("spring_ref") else 0),
-                                 loadCoef=0,
+                                 loadCoef=(J["friction_load_coefficient"][qd] or 0) if SO.get("enable_joint_friction") and scalar
+                                 and J.get("friction_load_coefficient") else 0,
                                  velLimit=J["vel_limit"][qd] if scalar and vlimOn and J["vel_limit"][qd] < 1e5 else -1))
                 tqd0.append(J["target_qd"][qd] if J.get("target_qd") else 0)
             for k in range(nq):
@@ -373,13 +374,13 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         statF[o4 + 11] = radius[b]
         ng = S["bodies"].get("no_gravity") or [False] * NB
         statF[o4 + 31] = (1 if kin[b] else 0) | (2 if ng[b] else 0)
-        lim = lambda k: (S["bodies"].get(k) or [-1] * NB)[b]
+        lim = lambda k: -1 if (S["bodies"].get(k) or [None] * NB)[b] is None else S["bodies"][k][b]
         statF[o4 + 15] = (S["bodies"].get("angular_damping") or [0.05] * NB)[b]
         statF[o4 + 19] = lim("max_linear_velocity")
         statF[o4 + 23] = lim("max_angular_velocity")
         statF[o4 + 27] = lim("max_depenetration_velocity")
-    if S["bodies"].get("max_linear_velocity") or S["bodies"].get("max_angular_velocity"):
-        raise UnsupportedError("free-body velocity limits are not supported yet")
+    # the reference switches the free-body limit pass on when either array is present (sim.js:471)
+    vlimFree = S["bodies"].get("max_linear_velocity") is not None or S["bodies"].get("max_angular_velocity") is not None
     # Reduction groups (sim.js:308-313).
     rkeys = {}
     rgroup = []
@@ -482,7 +483,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         if SV.get(k) is not None and SV.get(k) != default:
             raise ValueError(f"{k}={SV[k]} is not supported")
     for k in ("enable_sleeping", "nesterov_momentum", "enable_torsional_rolling_friction", "contact_compliance", "contact_shared_anchor",
-              "contact_friction_shared_anchor", "enable_joint_friction"):
+              "contact_friction_shared_anchor"):
         if SV.get(k) or SO.get(k):
             raise UnsupportedError(f"{k} is not supported yet")
     if SV.get("velocity_iterations", 0):
@@ -571,9 +572,11 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for k, v in P.items():
         H("W_" + k, v)
     for k in ("A_Q", "A_QD", "A_QDH", "A_TGT", "A_TGTV", "A_LIM", "A_DRV", "A_JF", "A_BF", "A_FRIC", "A_HINV", "A_LINK", "A_LG", "A_RL",
-              "A_RLN", "A_ULOAD", "A_DENSE", "A_VLG", "A_AVL"):
+              "A_RLN", "A_ULOAD", "A_DENSE", "A_VLG", "A_AVL", "A_JFL"):
         H("W_" + k, A[k])
     H("W_VLIM_ON", 1 if vlimOn else 0)
+    H("W_VLIM_FREE", 1 if vlimFree else 0)
+    H("W_JFL", 1 if any(d["loadCoef"] > 0 for d in dofs) else 0)
     gate = SV.get("contact_gap_gate")
     fgap = SV.get("contact_friction_gap_max")
     jlg = SV.get("joint_limit_activation_gap")

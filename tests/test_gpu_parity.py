@@ -31,6 +31,22 @@ def driven_quadruped(index=3):
     return scene
 
 
+def _with_body_limits(scene):
+    """Free-body velocity limits on every other body (1 m/s, 2 rad/s); the rest unbounded (-1)."""
+    B = scene.json["bodies"]
+    n = len(B["mass"])
+    B["max_linear_velocity"] = [1.0 if i % 2 else -1.0 for i in range(n)]
+    B["max_angular_velocity"] = [2.0 if i % 2 else -1.0 for i in range(n)]
+    return scene
+
+
+def _with_joint_friction(scene):
+    """Load-dependent joint friction (coefficient 0.02) on top of the plain friction bound."""
+    J = scene.json["joints"]
+    J["friction_load_coefficient"] = [0.02] * len(J["friction"])
+    return scene
+
+
 SCENES = {
     "boxes64": lambda: workloads.boxes64(),
     "quad0": lambda: workloads.quadruped_world(0),
@@ -38,6 +54,9 @@ SCENES = {
     "arm": lambda: workloads.arm_hulls(),
     "quad_driven": driven_quadruped,
     "serve1500": lambda: workloads.serving_request(11, nmin=1500, nmax=1500)[0],
+    "boxes_limits": lambda: _with_body_limits(workloads.boxes64(n=40)),
+    "quad_friction": lambda: _with_joint_friction(workloads.quadruped_world(6, drive=dict(targetKe=30.0, targetKd=1.0, friction=0.05),
+                                                                          solver=dict(enable_joint_friction=True))),
     # joint velocity limits of 0.5 rad/s on driven legs (rows active at any speed: activation fraction 0)
     "quad_vlim": lambda: workloads.quadruped_world(4, drive=dict(targetKe=60.0, targetKd=2.0, velocityLimit=0.5),
                                                    solver=dict(enable_joint_velocity_limits=True)),
@@ -70,7 +89,8 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
     b.destroy()
 
 
-@pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40), ("quad_vlim", 200)])
+@pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40), ("quad_vlim", 200),
+                                         ("boxes_limits", 150), ("quad_friction", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])
@@ -194,7 +214,7 @@ def test_joint_targets_set_mid_run_bitexact(sim):
 
 
 def test_one_step_error_against_fp64_within_1e5(sim):
-    for name in ("boxes64", "quad0", "quad5", "quad_driven", "quad_vlim"):
+    for name in ("boxes64", "quad0", "quad5", "quad_driven", "quad_vlim", "boxes_limits", "quad_friction"):
         pw = pack.pack_scene(SCENES[name]())
         b = sim.Batch([pw])
         o64 = OracleWorld(pw, f64=True)

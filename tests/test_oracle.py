@@ -511,6 +511,90 @@ def test_velocity_limit_prescale_scales_every_scalar_dof_of_the_articulation():
     assert np.abs(limited - qd_hat(False, qd)).max() > 0.1
 
 
+def _friction_hinge(qd, **dof):
+    scene = _vertical_hinge(**dof)
+    scene.json["solver"]["enable_joint_friction"] = True
+    return scene
+
+
+def test_joint_friction_removes_at_most_its_bound_per_step_and_stops_slow_joints():
+    """Coulomb joint friction is an impulse bounded by friction * dt: a fast hinge loses exactly
+    friction * dt / I of speed in a substep (nothing else acts about its vertical axis), a slow one stops."""
+    fr = 0.02
+    scene = _friction_hinge(0.0, friction=fr)
+    H, _ = _crba(scene)
+    pw = pack_scene(scene)
+    dt = pw.hf("W_DT")
+    pw.art0[pw.A["A_QD"]] = 3.0
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    np.testing.assert_allclose(o.art[pw.A["A_QD"]], 3.0 - dt * _f32(fr) / H[0, 0], rtol=1e-6)
+    pw.art0[pw.A["A_QD"]] = 0.3 * dt * _f32(fr) / H[0, 0]
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    assert abs(o.art[pw.A["A_QD"]]) < 1e-6 * dt * fr / H[0, 0]
+
+
+def test_load_dependent_joint_friction_bound_is_coefficient_times_transmitted_weight():
+    """With a load coefficient c the bound grows by c times the force the joint transmits (force plus torque
+    magnitude of the articulated force at the child COM).  A vertical hinge spinning at w carries the link's
+    weight and its centripetal force, m sqrt(g^2 + (w^2 r)^2) for a COM at r from the axis, and no torque
+    about the COM (w is along a principal axis), so it loses c m sqrt(g^2 + (w^2 r)^2) dt / I of speed."""
+    c = 0.01
+    scene = _friction_hinge(0.0)
+    scene.json["joints"]["friction_load_coefficient"] = [c]
+    H, _ = _crba(scene)
+    pw = pack_scene(scene)
+    assert pw.h("W_JFL") == 1
+    dt = pw.hf("W_DT")
+    m = scene.json["bodies"]["mass"][0]
+    pw.art0[pw.A["A_QD"]] = 3.0
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    r = float(np.hypot(*scene.json["bodies"]["com"][0][:2]))
+    fm = m * np.hypot(float(np.float32(9.81)), 9.0 * r)
+    np.testing.assert_allclose(o.art[pw.A["A_QD"]], 3.0 - dt * _f32(c) * fm / H[0, 0], rtol=1e-6)
+
+
+def _flying_box(twist, quat=(0, 0, 0, 1), vmax=None, wmax=None):
+    b = SceneBuilder(fps=240)
+    bd = b.addBody(xform=[0, 0, 5, *quat], twist=twist)
+    b.addShapeBox(bd, hx=0.2, hy=0.1, hz=0.05)
+    scene = b.finalize(solver=dict(substeps=1))
+    B = scene.json["bodies"]
+    B["angular_damping"] = [0.0]
+    B["max_linear_velocity"] = [vmax]
+    B["max_angular_velocity"] = [wmax]
+    return scene
+
+
+def test_free_body_velocity_limits_clamp_v_and_push_w_about_the_exceeded_axis():
+    """Linear speed is boxed per axis; an angular component over its limit is brought back by an angular
+    impulse about that world axis, so I * dw is parallel to the axis (a rotated box makes I non-diagonal)."""
+    from scipy.spatial.transform import Rotation as Rot
+    quat = Rot.from_euler("xyz", [0.4, -0.7, 1.1]).as_quat()
+    scene = _flying_box([3.0, -0.2, 0.5, 0.3, -0.1, 3.0], quat, vmax=1.0, wmax=2.5)
+    pw = pack_scene(scene)
+    assert pw.h("W_VLIM_FREE") == 1
+    o = OracleWorld(pw, f64=True)
+    o.substeps(1)
+    v, w = o.vel[0:3], o.vel[4:7]
+    assert v[0] == 1.0 and abs(v[1] + 0.2) < 1e-6
+    assert abs(w[2] - 2.5) < 1e-9 and np.abs(w).max() <= 2.5 + 1e-9
+    # the same box without limits: nothing is clamped, and the difference is the push alone
+    o2 = OracleWorld(pack_scene(_flying_box([3.0, -0.2, 0.5, 0.3, -0.1, 3.0], quat)), f64=True)
+    o2.substeps(1)
+    assert o2.vel[0] == 3.0
+    R = Rot.from_quat(quat).as_matrix()
+    I = R @ np.array(scene.json["bodies"]["inertia"][0]).reshape(3, 3) @ R.T
+    L = I @ (w - o2.vel[4:7])
+    assert abs(L[0]) < 1e-6 * abs(L[2]) and abs(L[1]) < 1e-6 * abs(L[2]) and L[2] < 0  # float32 packed inertia
+    # a component below -limit is pushed up to it
+    o3 = OracleWorld(pack_scene(_flying_box([0, 0, 0, 0.2, -3.0, 0.4], quat, wmax=2.5)), f64=True)
+    o3.substeps(1)
+    assert abs(o3.vel[5] + 2.5) < 1e-9
+
+
 def test_drive_and_spring_settle_at_their_stiffness_weighted_equilibrium():
     """Whole steps (solve + integration): a damped hinge under a drive and a spring comes to rest where their
     torques cancel, q* = (K q_tgt + K_s q_ref) / (K + K_s)."""
