@@ -442,6 +442,50 @@ template <class T> struct World {
           for (uint32_t k = 1; k < 4; k++) if (d[k] > d[worst]) worst = k;
         }
       }
+    } else if (gt == GT_CYL) {
+      // Deepest rim point, then three cap points (within 22.5 degrees of upright) or the opposite rim and the
+      // cap's side point (rolling).
+      V3<T> ax = qrot(oq, V3<T>{0, 0, 1});
+      T dna = dot(n, ax);
+      if (dna > T(0)) { ax = -ax; dna = -dna; }
+      V3<T> cap = op + ax * g.z;
+      V3<T> pa = -n + ax * dna;
+      T l2 = dot(pa, pa);
+      bool has = l2 > T(1e-10);
+      if (has) pa = pa * (T(1) / std::sqrt(l2));
+      bool flat = -dna >= T(0.9238795325112867);
+      V3<T> pf{0, 0, 0};
+      if (flat || !has) {
+        V3<T> rf{1, 0, 0};
+        if (std::fabs(dot(ax, rf)) > T(0.9)) rf = {0, 1, 0};
+        pf = normalize(rf - ax * dot(ax, rf));
+      }
+      V3<T> dp = has ? pa : pf;
+      V3<T> dpt = cap + dp * g.y;
+      d[0] = dot(dpt - pp, n);
+      x[0] = dpt - n * (d[0] * T(0.5));
+      nc = 1;
+      T mt = T(0.01) * mx(g.y, g.z);
+      V3<T> cand[3];
+      uint32_t ncand = 0;
+      if (flat) {
+        V3<T> u = pf * g.y, v = cross(ax, pf) * g.y;
+        cand[0] = cap + u;
+        cand[1] = cap - T(0.5) * u + T(0.8660254) * v;
+        cand[2] = cap - T(0.5) * u - T(0.8660254) * v;
+        ncand = 3;
+      } else {
+        V3<T> u = dp * g.y, v = cross(ax, dp) * g.y;
+        cand[0] = op - ax * g.z + u;
+        T dpv = dot(cap + v - pp, n), dnv = dot(cap - v - pp, n);
+        cand[1] = dpv <= dnv ? cap + v : cap - v;
+        ncand = 2;
+      }
+      for (uint32_t k = 0; k < ncand; k++) {
+        T dk = dot(cand[k] - pp, n);
+        V3<T> pk = cand[k] - n * (dk * T(0.5));
+        if (nc < 4 && dot(pk - x[0], pk - x[0]) > mt * mt) { d[nc] = dk; x[nc] = pk; nc++; }
+      }
     }
     for (uint32_t i = 0; i < nc; i++)
       if (d[i] <= thresh) emit(sa, sb, x[i], nab, d[i], hash_u32(base ^ (i + 7u)));
@@ -528,6 +572,13 @@ template <class T> struct World {
       T l2 = dot(d, d);
       V3<T> p = (l2 > T(1e-12) ? d * inverse_sqrt(l2) : V3<T>{1, 0, 0}) * g.s.x;
       if (g.t == GT_CAPSULE) p.z += d.z >= T(0) ? g.s.y : -g.s.y;
+      return p;
+    }
+    if (g.t == GT_CYL) {  // rim point toward d's xy, on the cap d.z points to
+      V3<T> xy{d.x, d.y, T(0)};
+      T l2 = dot(xy, xy);
+      V3<T> p = l2 > T(1e-12) ? xy * inverse_sqrt(l2) * g.s.x : V3<T>{g.s.x, 0, 0};
+      if (d.z > T(0)) p.z = g.s.y; else if (d.z < T(0)) p.z = -g.s.y;
       return p;
     }
     T best = T(-1e10);
@@ -783,10 +834,27 @@ template <class T> struct World {
     V3<T> nw = qrot(qa, n);
     T ra = (ga.t == GT_SPHERE || ga.t == GT_CAPSULE) ? statF[SF_SHAPE + sa * 7 + 6].y : T(0);
     T rb = (gb.t == GT_SPHERE || gb.t == GT_CAPSULE) ? statF[SF_SHAPE + sb * 7 + 6].y : T(0);
+    // A cylinder against a discrete shape rolls when its axis is within 2 degrees of the tangent plane: its
+    // points are moved onto the line through the cylinder's center across the axis (wgsl.js:955-967).
+    auto discrete = [](uint32_t t) { return t == GT_BOX || t == GT_CONVEX || t == GT_PLANE; };
+    bool axA = ga.t == GT_CYL || ga.t == GT_CONE, axB = gb.t == GT_CYL || gb.t == GT_CONE;
+    bool roll = false;
+    V3<T> rpos{0, 0, 0}, rn{0, 0, 0};
+    if ((discrete(ga.t) && axB) || (discrete(gb.t) && axA)) {
+      bool onB = discrete(ga.t) && axB;
+      V3<T> ax = onB ? qrot(qmul(qa, rq), V3<T>{0, 0, 1}) : qrot(qa, V3<T>{0, 0, 1});
+      V3<T> an = onB ? nw : -nw;
+      if (std::fabs(dot(ax, an)) <= T(0.03489949670250097)) {
+        roll = true;
+        rpos = onB ? pa + qrot(qa, rt) : pa;
+        rn = normalize(cross(ax, an));
+      }
+    }
     uint32_t base = hash_u32(sa * 7919u + sb * 104729u);
     for (uint32_t i = 0; i < m.n; i++) {
       V3<T> p = pa + qrot(qa, m.p[i]) + nw * (T(0.5) * ra) - nw * (T(0.5) * rb);
       T d = m.d[i] - ra - rb;
+      if (roll) p = p - rn * dot(p - rpos, rn);
       if (d > thresh) continue;
       emit(sa, sb, p, nw, d, hash_u32(base ^ (i * 0x9e3779b1u + 0x51u)));
     }
@@ -814,6 +882,88 @@ template <class T> struct World {
     T oa = mx(hull_out(ga.h, wa[0]), hull_out(gb.h, qinvrot(rq, wa[1] - rt)));
     if ((refA && !(oa < ob - T(1e-6))) || (!refA && ob < oa - T(1e-6))) { w[0] = wb[0]; w[1] = wb[1]; return; }
     w[0] = wa[0]; w[1] = wa[1];
+  }
+  // ---- MPR for hulls of more than 16 vertices (wgsl.js:310-389, 1206-1214)
+  // Support point of hull h along d in the reference's subgroup order: lane l scans vertices l, l + 32, ...
+  // keeping its first maximum, and the lowest lane holding the overall maximum wins; serially, lanes in turn
+  // with one running strict maximum gives the same point.
+  V3<T> sup_lane(uint32_t h, V3<T> d) const {
+    T best = T(-1e30);
+    V3<T> bp{0, 0, 0};
+    for (uint32_t l = 0; l < NLANE; l++)
+      for (uint32_t v = l; v < hnv(h); v += NLANE) {
+        V3<T> x = hv(h, v);
+        T s = dot(d, x);
+        if (s > best) { best = s; bp = x; }
+      }
+    return bp;
+  }
+  // Support of (B + sphere r) - (A + sphere r) along d, in A's frame.
+  V3<T> sup_md(uint32_t ha, uint32_t hb, V4<T> rq, V3<T> t, T r, V3<T> d) const {
+    V3<T> sb = qrot(rq, sup_lane(hb, qinvrot(rq, d))) + t;
+    V3<T> sa = sup_lane(ha, -d);
+    return sb - sa + (T(2) * r) * normalize(d);
+  }
+  // Minkowski portal refinement (XenoCollide) on the shapes inflated by r: (outward Minkowski normal, inflated
+  // depth), w < 0 when the inflated shapes miss.
+  V4<T> mpr(uint32_t ha, uint32_t hb, V4<T> rq, V3<T> t, T r) const {
+    const V4<T> none{0, 0, 1, -1};
+    V3<T> v0 = qrot(rq, xyz(statF[SF_HAABB + hb * 2])) + t - xyz(statF[SF_HAABB + ha * 2]);
+    if (dot(v0, v0) < T(1e-14)) v0 = {T(1e-5), 0, 0};
+    V3<T> n = -v0;
+    V3<T> v1 = sup_md(ha, hb, rq, t, r, n);
+    if (dot(v1, n) <= T(0)) return none;
+    n = cross(v1, v0);
+    if (dot(n, n) < T(1e-14)) {
+      V3<T> nn = normalize(v1 - v0);
+      return v4(nn, dot(v1, nn));
+    }
+    V3<T> v2 = sup_md(ha, hb, rq, t, r, n);
+    if (dot(v2, n) <= T(0)) return none;
+    n = cross(v1 - v0, v2 - v0);
+    if (dot(n, v0) > T(0)) { V3<T> tmp = v1; v1 = v2; v2 = tmp; n = -n; }
+    V3<T> v3{0, 0, 0};
+    bool found = false;
+    for (uint32_t i = 0; i < 32; i++) {
+      v3 = sup_md(ha, hb, rq, t, r, n);
+      if (dot(v3, n) <= T(0)) return none;
+      if (dot(cross(v1, v3), v0) < T(0)) { v2 = v3; n = cross(v1 - v0, v3 - v0); continue; }
+      if (dot(cross(v3, v2), v0) < T(0)) { v1 = v3; n = cross(v3 - v0, v2 - v0); continue; }
+      found = true;
+      break;
+    }
+    if (!found) return none;
+    for (uint32_t i = 0; i < 32; i++) {
+      n = cross(v2 - v1, v3 - v1);
+      T ln = length(n);
+      if (ln < T(1e-12)) break;
+      n = n / ln;
+      if (dot(n, v1 - v0) < T(0)) n = -n;
+      V3<T> v4_ = sup_md(ha, hb, rq, t, r, n);
+      T d1 = dot(n, v1);
+      if (dot(v4_, n) - d1 < T(1e-6) || i == 31) {
+        if (d1 < T(0)) return none;
+        return v4(n, d1);
+      }
+      if (dot(v4_, n) < T(0)) return none;
+      V3<T> c = cross(v4_, v0);
+      if (dot(v1, c) > T(0)) { if (dot(v2, c) > T(0)) v1 = v4_; else v3 = v4_; }
+      else { if (dot(v3, c) > T(0)) v2 = v4_; else v1 = v4_; }
+    }
+    T d1 = dot(n, v1);
+    if (d1 < T(0)) return none;
+    return v4(n, d1);
+  }
+  // Large hulls: MPR's normal, the reference side from face alignment.
+  void mpr_generic(uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb, V3<T> pa, V4<T> qa, V4<T> rBA, V3<T> tBA, T thresh, V3<T> n) {
+    T alA = T(-2);
+    for (uint32_t f = 0; f < hnf(ha); f++) alA = mx(alA, dot(xyz(fplane(hfo(ha) + f)), n));
+    T alB = T(-2);
+    for (uint32_t f = 0; f < hnf(hb); f++) alB = mx(alB, -dot(qrot(rBA, xyz(fplane(hfo(hb) + f))), n));
+    Geo<T> ga = geo(sa), gb = geo(sb);
+    V3<T> w[2];
+    sup_witness(ga, gb, rBA, tBA, n, alA >= alB, w);
+    gen_run(sa, sb, ga, gb, pa, qa, rBA, tBA, w[0], w[1], n, thresh);
   }
   void sat_generic(uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb, V3<T> pa, V4<T> qa, V4<T> rBA, V3<T> tBA, T thresh, FaceQ<T> fa,
                    FaceQ<T> fb, const EdgeQ<T>& eq) {
@@ -926,7 +1076,22 @@ template <class T> struct World {
     if (fa.sep > thresh) return;
     FaceQ<T> fb = reduce_lanes_max([&](uint32_t l) { return face_query(hb, ha, rAB, tAB, l); });
     if (fb.sep > thresh) return;
-    // hnv <= 16 on both sides (larger hulls, the MPR path, are rejected at packing)
+    if (!(hnv(ha) <= 16 && hnv(hb) <= 16)) {
+      // Face-only SAT misjudges edge contacts on large hulls: MPR's normal instead, unless a face axis
+      // separates the hulls better (wgsl.js:1164-1176).
+      V4<T> m = mpr(ha, hb, rBA, tBA, T(0.5) * thresh);
+      if (m.w >= T(0)) {
+        V3<T> n = -xyz(m);
+        T sep = dot(n, tBA) + support_min(hb, qinvrot(rBA, n)) - support_max(ha, n);
+        if (mx(fa.sep, fb.sep) > sep + T(1e-4)) {
+          EdgeQ<T> none{T(-1e30), {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, 0};
+          sat_generic(sa, sb, ha, hb, pa, qa, rBA, tBA, thresh, fa, fb, none);
+        } else {
+          mpr_generic(sa, sb, ha, hb, pa, qa, rBA, tBA, thresh, n);
+        }
+      }
+      return;
+    }
     EdgeQ<T> r[NLANE];
     for (uint32_t l = 0; l < NLANE; l++) r[l] = edge_query(ha, hb, rBA, tBA, l);
     uint32_t b = 0;
@@ -2692,6 +2857,30 @@ void oracle_anchor_export_f32(uint32_t AH, const uint32_t* akey, const void* ava
   }
 }
 // Known-answer hooks for the oracle's own pinning tests.
+// World of static data only (geometry queries): header, statF and statU of a packed world.
+static World<double> geometry_world(const uint32_t* header, const float* statF, const uint32_t* statU) {
+  World<double> w;
+  w.load_header(header);
+  w.statF.resize(header[W_NSTATF]);
+  for (uint32_t i = 0; i < header[W_NSTATF]; i++)
+    w.statF[i] = {(double)statF[i * 4], (double)statF[i * 4 + 1], (double)statF[i * 4 + 2], (double)statF[i * 4 + 3]};
+  w.statU = statU;
+  return w;
+}
+// MPR of hulls ha (frame of reference) and hb (rotation rq = x, y, z, w and offset t in ha's frame), shapes
+// inflated by r: out = (normal, depth), depth < 0 when they miss.
+void oracle_mpr_f64(const uint32_t* header, const float* statF, const uint32_t* statU, uint32_t ha, uint32_t hb, const double* rq,
+                    const double* t, double r, double* out) {
+  World<double> w = geometry_world(header, statF, statU);
+  V4<double> m = w.mpr(ha, hb, V4<double>{rq[0], rq[1], rq[2], rq[3]}, V3<double>{t[0], t[1], t[2]}, r);
+  out[0] = m.x; out[1] = m.y; out[2] = m.z; out[3] = m.w;
+}
+// Support point of shape s's collision geometry along d (shape frame).
+void oracle_gsup_f64(const uint32_t* header, const float* statF, const uint32_t* statU, uint32_t s, const double* d, double* out) {
+  World<double> w = geometry_world(header, statF, statU);
+  V3<double> p = w.gsup(w.geo(s), V3<double>{d[0], d[1], d[2]});
+  out[0] = p.x; out[1] = p.y; out[2] = p.z;
+}
 uint32_t oracle_hash_u32(uint32_t x) { return hash_u32(x); }
 float oracle_sinf(float x) { return sinf_c(x); }
 float oracle_cosf(float x) { return cosf_c(x); }

@@ -245,6 +245,50 @@ struct Collider {
           for (uint32_t k = 1; k < 4; k++) if (d[k] > d[worst]) worst = k;
         }
       }
+    } else if (gt == GT_CYL) {
+      // Deepest rim point, then three cap points (within 22.5 degrees of upright) or the opposite rim and the
+      // cap's side point (rolling).
+      f3 ax = qrot(oq, mk3(0, 0, 1));
+      float dna = dot(n, ax);
+      if (dna > 0.0f) { ax = -ax; dna = -dna; }
+      f3 cap = op + ax * g.z;
+      f3 pa = -n + ax * dna;
+      float l2 = dot(pa, pa);
+      bool has = l2 > 1e-10f;
+      if (has) pa = pa * (1.0f / sqrtf(l2));
+      bool flat = -dna >= 0.9238795325112867f;
+      f3 pf = mk3(0, 0, 0);
+      if (flat || !has) {
+        f3 rf = mk3(1, 0, 0);
+        if (fabsf(dot(ax, rf)) > 0.9f) rf = mk3(0, 1, 0);
+        pf = nrm(rf - ax * dot(ax, rf));
+      }
+      f3 dp = has ? pa : pf;
+      f3 dpt = cap + dp * g.y;
+      d[0] = dot(dpt - pp, n);
+      x[0] = dpt - n * (d[0] * 0.5f);
+      nc = 1;
+      float mt = 0.01f * mx(g.y, g.z);
+      f3 cand[3];
+      uint32_t ncand = 0;
+      if (flat) {
+        f3 u = pf * g.y, v = cross(ax, pf) * g.y;
+        cand[0] = cap + u;
+        cand[1] = cap - 0.5f * u + 0.8660254f * v;
+        cand[2] = cap - 0.5f * u - 0.8660254f * v;
+        ncand = 3;
+      } else {
+        f3 u = dp * g.y, v = cross(ax, dp) * g.y;
+        cand[0] = op - ax * g.z + u;
+        float dpv = dot(cap + v - pp, n), dnv = dot(cap - v - pp, n);
+        cand[1] = dpv <= dnv ? cap + v : cap - v;
+        ncand = 2;
+      }
+      for (uint32_t k = 0; k < ncand; k++) {
+        float dk = dot(cand[k] - pp, n);
+        f3 pk = cand[k] - n * (dk * 0.5f);
+        if (nc < 4 && dot(pk - x[0], pk - x[0]) > mt * mt) { d[nc] = dk; x[nc] = pk; nc++; }
+      }
     }
     for (uint32_t i = 0; i < nc; i++)
       if (d[i] <= thresh) emit(sa, sb, x[i], n, d[i], hash_u32(base ^ (i + 7u)));
@@ -344,6 +388,13 @@ struct Collider {
       float l2 = dot(dv, dv);
       f3 p = (l2 > 1e-12f ? dv * rsq(l2) : mk3(1, 0, 0)) * g.s.x;
       if (g.t == GT_CAPSULE) p.z += dv.z >= 0.0f ? g.s.y : -g.s.y;
+      return p;
+    }
+    if (g.t == GT_CYL) {  // rim point toward dv's xy, on the cap dv.z points to
+      f3 xy = mk3(dv.x, dv.y, 0.0f);
+      float l2 = dot(xy, xy);
+      f3 p = l2 > 1e-12f ? xy * rsq(l2) * g.s.x : mk3(g.s.x, 0, 0);
+      if (dv.z > 0.0f) p.z = g.s.y; else if (dv.z < 0.0f) p.z = -g.s.y;
       return p;
     }
     float best = -1e10f;
@@ -607,10 +658,27 @@ struct Collider {
     f3 nw = qrot(qa, n);
     float ra = (ga.t == GT_SPHERE || ga.t == GT_CAPSULE) ? shape_rec(sa, 6).y : 0.0f;
     float rb = (gb.t == GT_SPHERE || gb.t == GT_CAPSULE) ? shape_rec(sb, 6).y : 0.0f;
+    // A cylinder against a discrete shape rolls when its axis is within 2 degrees of the tangent plane: its
+    // points are moved onto the line through the cylinder's center across the axis (wgsl.js:955-967).
+    auto discrete = [](uint32_t t) { return t == GT_BOX || t == GT_CONVEX || t == GT_PLANE; };
+    const bool axA = ga.t == GT_CYL || ga.t == GT_CONE, axB = gb.t == GT_CYL || gb.t == GT_CONE;
+    bool roll = false;
+    f3 rpos = mk3(0, 0, 0), rn = mk3(0, 0, 0);
+    if ((discrete(ga.t) && axB) || (discrete(gb.t) && axA)) {
+      const bool onB = discrete(ga.t) && axB;
+      f3 ax = onB ? qrot(qmul(qa, rq), mk3(0, 0, 1)) : qrot(qa, mk3(0, 0, 1));
+      f3 an = onB ? nw : -nw;
+      if (fabsf(dot(ax, an)) <= 0.03489949670250097f) {
+        roll = true;
+        rpos = onB ? pa + qrot(qa, rt) : pa;
+        rn = nrm(cross(ax, an));
+      }
+    }
     uint32_t base = hash_u32(sa * 7919u + sb * 104729u);
     for (uint32_t i = 0; i < m.n; i++) {
       f3 p = pa + qrot(qa, m.p[i]) + nw * (0.5f * ra) - nw * (0.5f * rb);
       float dd = m.d[i] - ra - rb;
+      if (roll) p = p - rn * dot(p - rpos, rn);
       if (dd > thresh) continue;
       emit(sa, sb, p, nw, dd, hash_u32(base ^ (i * 0x9e3779b1u + 0x51u)));
     }
@@ -638,6 +706,92 @@ struct Collider {
     float oa = mx(hull_out(ga.h, wa0), hull_out(gb.h, qinvrot(rq, wa1 - rt)));
     if ((refA && !(oa < ob - 1e-6f)) || (!refA && ob < oa - 1e-6f)) { wo[0] = wb0; wo[1] = wb1; return; }
     wo[0] = wa0; wo[1] = wa1;
+  }
+  // ---- MPR for hulls of more than 16 vertices (wgsl.js:310-389, 1206-1214); whole-wave, uniform control flow
+  // Support point of hull h along dv in the reference's 32-lane subgroup form: lane l < 32 scans vertices l,
+  // l + 32, ... keeping its first maximum, and the lowest lane holding the overall maximum wins (lanes 32-63
+  // hold no candidate).  Every lane returns the winner.
+  DEV f3 sup_lane(uint32_t h, f3 dv, uint32_t lane) const {
+    float best = -1e30f;
+    f3 bp = mk3(0, 0, 0);
+    if (lane < 32)
+      for (uint32_t v = lane; v < w.hnv(h); v += 32) {
+        f3 x = w.hv(h, v);
+        float s = dot(dv, x);
+        if (s > best) { best = s; bp = x; }
+      }
+    float m = best;
+    for (int off = 32; off >= 1; off >>= 1) m = mx(m, __shfl_xor(m, off, 64));
+    int win = (lane < 32 && best == m) ? (int)lane : 64;
+    for (int off = 32; off >= 1; off >>= 1) win = min(win, __shfl_xor(win, off, 64));
+    return mk3(__shfl(bp.x, win, 64), __shfl(bp.y, win, 64), __shfl(bp.z, win, 64));
+  }
+  // Support of (B + sphere r) - (A + sphere r) along dv, in A's frame.
+  DEV f3 sup_md(uint32_t ha, uint32_t hb, q4 rq, f3 t, float r, f3 dv, uint32_t lane) const {
+    f3 sb = qrot(rq, sup_lane(hb, qinvrot(rq, dv), lane)) + t;
+    f3 sa = sup_lane(ha, -dv, lane);
+    return sb - sa + (2.0f * r) * nrm(dv);
+  }
+  // Minkowski portal refinement (XenoCollide) on the shapes inflated by r: (outward Minkowski normal,
+  // inflated depth), w < 0 when the inflated shapes miss.
+  DEV q4 mpr(uint32_t ha, uint32_t hb, q4 rq, f3 t, float r, uint32_t lane) const {
+    const q4 none = mk4(0, 0, 1, -1);
+    f3 v0 = qrot(rq, xyz(w.sf(w.h(W_SF_HAABB) + hb * 2))) + t - xyz(w.sf(w.h(W_SF_HAABB) + ha * 2));
+    if (dot(v0, v0) < 1e-14f) v0 = mk3(1e-5f, 0, 0);
+    f3 n = -v0;
+    f3 v1 = sup_md(ha, hb, rq, t, r, n, lane);
+    if (dot(v1, n) <= 0.0f) return none;
+    n = cross(v1, v0);
+    if (dot(n, n) < 1e-14f) {
+      f3 nn = nrm(v1 - v0);
+      return mk4(nn, dot(v1, nn));
+    }
+    f3 v2 = sup_md(ha, hb, rq, t, r, n, lane);
+    if (dot(v2, n) <= 0.0f) return none;
+    n = cross(v1 - v0, v2 - v0);
+    if (dot(n, v0) > 0.0f) { f3 tmp = v1; v1 = v2; v2 = tmp; n = -n; }
+    f3 v3 = mk3(0, 0, 0);
+    bool found = false;
+    for (uint32_t i = 0; i < 32; i++) {
+      v3 = sup_md(ha, hb, rq, t, r, n, lane);
+      if (dot(v3, n) <= 0.0f) return none;
+      if (dot(cross(v1, v3), v0) < 0.0f) { v2 = v3; n = cross(v1 - v0, v3 - v0); continue; }
+      if (dot(cross(v3, v2), v0) < 0.0f) { v1 = v3; n = cross(v3 - v0, v2 - v0); continue; }
+      found = true;
+      break;
+    }
+    if (!found) return none;
+    for (uint32_t i = 0; i < 32; i++) {
+      n = cross(v2 - v1, v3 - v1);
+      float ln = len(n);
+      if (ln < 1e-12f) break;
+      n = n / ln;
+      if (dot(n, v1 - v0) < 0.0f) n = -n;
+      f3 v4 = sup_md(ha, hb, rq, t, r, n, lane);
+      float d1 = dot(n, v1);
+      if (dot(v4, n) - d1 < 1e-6f || i == 31) {
+        if (d1 < 0.0f) return none;
+        return mk4(n, d1);
+      }
+      if (dot(v4, n) < 0.0f) return none;
+      f3 c = cross(v4, v0);
+      if (dot(v1, c) > 0.0f) { if (dot(v2, c) > 0.0f) v1 = v4; else v3 = v4; }
+      else { if (dot(v3, c) > 0.0f) v2 = v4; else v1 = v4; }
+    }
+    float d1 = dot(n, v1);
+    if (d1 < 0.0f) return none;
+    return mk4(n, d1);
+  }
+  // Large hulls: MPR's normal, the reference side from face alignment.
+  DEV void mpr_generic(uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb, f3 pa, q4 qa, q4 rBA, f3 tBA, float thresh, f3 n) const {
+    float alA = -2.0f;
+    for (uint32_t f = 0; f < w.hnf(ha); f++) alA = mx(alA, dot(xyz(w.fplane(w.hfo(ha) + f)), n));
+    float alB = -2.0f;
+    for (uint32_t f = 0; f < w.hnf(hb); f++) alB = mx(alB, -dot(qrot(rBA, xyz(w.fplane(w.hfo(hb) + f))), n));
+    Geod ga = geo(sa), gb = geo(sb);
+    f3 wv[2];
+    sup_witness(ga, gb, rBA, tBA, n, alA >= alB, wv);
+    gen_run(sa, sb, ga, gb, pa, qa, rBA, tBA, wv[0], wv[1], n, thresh);
   }
   DEV void sat_generic(uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb, f3 pa, q4 qa, q4 rBA, f3 tBA, float thresh, FaceQd fa, FaceQd fb,
                         const EdgeQd& eq) const {
@@ -751,6 +905,23 @@ struct Collider {
     if (fa.sep > thresh) return;
     FaceQd fb = face_query(hb, ha, rAB, tAB, lane);
     if (fb.sep > thresh) return;
+    if (!(w.hnv(ha) <= 16 && w.hnv(hb) <= 16)) {
+      // Face-only SAT misjudges edge contacts on large hulls: MPR's normal instead, unless a face axis
+      // separates the hulls better (wgsl.js:1164-1176).
+      q4 m = mpr(ha, hb, rBA, tBA, 0.5f * thresh, lane);
+      if (lane == 0 && m.w >= 0.0f) {
+        f3 n = -xyz(m);
+        float sep = dot(n, tBA) + support_min(hb, qinvrot(rBA, n)) - support_max(ha, n);
+        if (mx(fa.sep, fb.sep) > sep + 1e-4f) {
+          EdgeQd none;
+          none.sep = -1e30f; none.axis = none.a0 = none.a1 = none.b0 = none.b1 = mk3(0, 0, 0); none.id = 0;
+          sat_generic(sa, sb, ha, hb, pa, qa, rBA, tBA, thresh, fa, fb, none);
+        } else {
+          mpr_generic(sa, sb, ha, hb, pa, qa, rBA, tBA, thresh, n);
+        }
+      }
+      return;
+    }
     EdgeQd eq = edge_query(ha, hb, rBA, tBA, lane);
     if (eq.sep > thresh) return;
     if (lane == 0) sat_generic(sa, sb, ha, hb, pa, qa, rBA, tBA, thresh, fa, fb, eq);

@@ -262,10 +262,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
             continue
         pk = (s.get("prim") or {}).get("kind", 0)
         t = s["type"]
-        if t in (GT["CYLINDER"], GT["CONE"]):
-            raise UnsupportedError("cylinder and cone shapes are not supported yet")
-        if pk == 0 and s["hull"]["nv"] > 16:
-            raise UnsupportedError("hulls of more than 16 vertices (MPR path) are not supported yet")
+        if t == GT["CONE"]:
+            raise UnsupportedError("cone shapes are not supported yet")
         if (s.get("condim", 3) or 3) >= 4 or (s.get("torsion_radius") or 0) > 0:
             raise UnsupportedError("torsional/rolling friction rows are not supported yet")
 

@@ -32,6 +32,8 @@ def lib():
         L.oracle_atan2f.restype = ctypes.c_float
         L.oracle_atan2f.argtypes = [ctypes.c_float, ctypes.c_float]
         L.oracle_friction_pair_solve.argtypes = [ctypes.c_float] * 9 + [ctypes.c_void_p]
+        L.oracle_mpr_f64.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint32] * 2 + [ctypes.c_void_p] * 2 + [ctypes.c_double, ctypes.c_void_p]
+        L.oracle_gsup_f64.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint32] + [ctypes.c_void_p] * 2
         _lib = L
     return _lib
 
@@ -80,3 +82,19 @@ class OracleWorld:
         out = np.zeros(2 * AH + 2 * AH * 14, np.uint32)
         lib().oracle_anchor_export_f32(ctypes.c_uint32(AH), P(self.akey), P(self.aval), P(out))
         return out
+
+
+def mpr(pw, ha, hb, rq, t, r):
+    """The oracle's MPR on hulls ha, hb of a packed world (hb rotated by rq, offset t in ha's frame)."""
+    out = np.zeros(4)
+    statU = pw.statU if pw.statU.size else np.zeros(1, np.uint32)
+    lib().oracle_mpr_f64(P(pw.header), P(pw.statF), P(statU), ha, hb, P(np.asarray(rq, np.float64)), P(np.asarray(t, np.float64)), r, P(out))
+    return out
+
+
+def gsup(pw, shape, d):
+    """The oracle's support point of a shape's collision geometry along d (shape frame)."""
+    out = np.zeros(3)
+    statU = pw.statU if pw.statU.size else np.zeros(1, np.uint32)
+    lib().oracle_gsup_f64(P(pw.header), P(pw.statF), P(statU), shape, P(np.asarray(d, np.float64)), P(out))
+    return out

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle_bind import OracleWorld
-from steel_webgpu_amd import pack, workloads
+from steel_webgpu_amd import SceneBuilder, pack, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +47,40 @@ def _with_joint_friction(scene):
     return scene
 
 
+def big_hull_pile(seed=21, n=27):
+    """Hulls of 40 vertices, boxes with proud edge midpoints (20 vertices) and cylinders (48-vertex hulls,
+    analytic support), dropped in random orientations onto a static slab and the ground: the MPR path, plane and
+    box contacts of cylinders and their rolling."""
+    rng = np.random.default_rng(seed)
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane(mu=0.6)
+    b.addShapeBox(-1, hx=0.4, hy=0.4, hz=0.05, xform=[0.3, 0.0, 0.05, 0, 0, 0, 1], mu=0.6)
+    for i in range(n):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        x, y, z = (i % 3) * 0.3 - 0.2, ((i // 3) % 3) * 0.3 - 0.3, 0.35 + (i // 9) * 0.3
+        bd = b.addBody(xform=[x, y, z, *q])
+        k = i % 3
+        if k == 0:
+            v = rng.normal(size=(40, 3))
+            v *= rng.uniform(0.05, 0.1) / np.linalg.norm(v, axis=1, keepdims=True)
+            b.addShapeConvexHull(bd, v.tolist(), mu=0.6)
+        elif k == 1:
+            h = rng.uniform(0.04, 0.08)
+            pts = [[sx * h, sy * h, sz * h] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]
+            for ax in range(3):
+                o = [j for j in range(3) if j != ax]
+                for s1 in (-1, 1):
+                    for s2 in (-1, 1):
+                        p = [0.0, 0.0, 0.0]
+                        p[o[0]], p[o[1]] = s1 * h * 1.01, s2 * h * 1.01
+                        pts.append(p)
+            b.addShapeConvexHull(bd, pts, mu=0.6)
+        else:
+            b.addShapeCylinder(bd, radius=rng.uniform(0.04, 0.07), halfHeight=rng.uniform(0.05, 0.12), mu=0.6)
+    return b.finalize(solver=dict(substeps=4, iterations=12))
+
+
 SCENES = {
     "boxes64": lambda: workloads.boxes64(),
     "quad0": lambda: workloads.quadruped_world(0),
@@ -54,6 +88,7 @@ SCENES = {
     "arm": lambda: workloads.arm_hulls(),
     "quad_driven": driven_quadruped,
     "serve1500": lambda: workloads.serving_request(11, nmin=1500, nmax=1500)[0],
+    "bighulls": big_hull_pile,
     "boxes_limits": lambda: _with_body_limits(workloads.boxes64(n=40)),
     "quad_friction": lambda: _with_joint_friction(workloads.quadruped_world(6, drive=dict(targetKe=30.0, targetKd=1.0, friction=0.05),
                                                                           solver=dict(enable_joint_friction=True))),
@@ -67,7 +102,7 @@ def _bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-@pytest.mark.parametrize("name", ["boxes64", "quad5", "arm", "serve1500"])
+@pytest.mark.parametrize("name", ["boxes64", "quad5", "arm", "serve1500", "bighulls"])
 def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw], graph=False)
@@ -90,7 +125,7 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
 
 
 @pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40), ("quad_vlim", 200),
-                                         ("boxes_limits", 150), ("quad_friction", 200)])
+                                         ("boxes_limits", 150), ("quad_friction", 200), ("bighulls", 120)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])
@@ -214,7 +249,7 @@ def test_joint_targets_set_mid_run_bitexact(sim):
 
 
 def test_one_step_error_against_fp64_within_1e5(sim):
-    for name in ("boxes64", "quad0", "quad5", "quad_driven", "quad_vlim", "boxes_limits", "quad_friction"):
+    for name in ("boxes64", "quad0", "quad5", "quad_driven", "quad_vlim", "boxes_limits", "quad_friction", "bighulls"):
         pw = pack.pack_scene(SCENES[name]())
         b = sim.Batch([pw])
         o64 = OracleWorld(pw, f64=True)

@@ -96,8 +96,8 @@ def test_unsupported_options_raise():
     with pytest.raises(ValueError):
         pack.pack_scene(workloads.boxes64(n=4), opts={"substeps": 64})
     b = SceneBuilder()
-    bd = b.addBody()
-    b.addShapeCylinder(bd)
+    bd = b.addBody(kinematic=True)
+    b.addShapeBox(bd)
     with pytest.raises(pack.UnsupportedError):
         pack.pack_scene(b.finalize())
 

@@ -163,6 +163,195 @@ def test_stacked_boxes_face_manifold_on_overlap():
     np.testing.assert_allclose(np.array(xs), np.array(want), atol=3e-5)
 
 
+def _sphere_points(n, r, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(n, 3))
+    return (r * v / np.linalg.norm(v, axis=1, keepdims=True)).tolist()
+
+
+def test_large_hulls_take_mpr_and_report_the_penetration_along_the_normal():
+    """Hulls of 40 vertices take the MPR path.  Touching (the spheres they are sampled from overlap), the
+    deepest contact's depth equals the brute-force separation of the two vertex sets along the contact normal
+    (min over B minus max over A), and the normal roughly follows the line between the centers; 3 cm apart
+    with 5 mm gaps they give no contact."""
+    for seed, u, overlap in ((1, (0.1, -0.2, 1.0), 0.03), (2, (0.7, 0.3, 0.4), 0.012)):
+        u = np.array(u) / np.linalg.norm(u)
+        A, B = np.array(_sphere_points(40, 0.12, seed)), np.array(_sphere_points(40, 0.09, seed + 10))
+        cA = np.array([0.0, 0.0, 1.0])
+        for dist, expect in ((0.12 + 0.09 - overlap, True), (0.12 + 0.09 + 0.03, False)):
+            cB = cA + u * dist
+            b = SceneBuilder(fps=240)
+            for c, P in ((cA, A), (cB, B)):
+                bd = b.addBody(xform=list(c) + [0, 0, 0, 1], gravity=False)
+                b.addShapeConvexHull(bd, P.tolist(), gap=0.005)
+            scene = b.finalize(solver=dict(substeps=1))
+            assert all(sh["hull"]["nv"] == 40 for sh in scene.json["shapes"] if sh.get("hull"))
+            pw, o = _one_substep(scene, f64=True)
+            C = _contacts(o)
+            if not expect:
+                assert len(C) == 0
+                continue
+            assert len(C) >= 1
+            n = C[0, 4:7].astype(np.float64)
+            assert np.all(np.abs(C[:, 4:7] - n) < 1e-6)
+            assert n @ u > np.cos(np.radians(35))  # a face normal of the coarse hulls may win
+            sep = ((cB + B) @ n).min() - ((cA + A) @ n).max()
+            assert abs(C[:, 3].min() - sep) < 1e-5
+
+
+def _rounded_box(h):
+    """A box of half size h whose 12 edge midpoints stand 1 % proud (20 hull vertices: the MPR path)."""
+    pts = [[sx * h, sy * h, sz * h] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]
+    for ax in range(3):
+        o = [k for k in range(3) if k != ax]
+        for s1 in (-1, 1):
+            for s2 in (-1, 1):
+                p = [0.0, 0.0, 0.0]
+                p[o[0]], p[o[1]] = s1 * h * 1.01, s2 * h * 1.01
+                pts.append(p)
+    return np.array(pts)
+
+
+def test_large_hulls_crossed_edges_contact_along_the_edge_normal():
+    """Two such boxes, one turned 45 degrees about x (an edge along x on top), the other 45 degrees about y (an
+    edge along y below), stacked so the edges cross: one contact whose normal is the cross product of the edge
+    directions (+z) and whose depth is minus the overlap of the edges (2 mm in, then 3 mm apart).  Face axes
+    alone would report a tilted normal and a deep penetration."""
+    from scipy.spatial.transform import Rotation as Rot
+    P_ = _rounded_box(0.1)
+    qa, qb = Rot.from_euler("x", 45, degrees=True), Rot.from_euler("y", 45, degrees=True)
+    zA, zB = qa.apply(P_)[:, 2].max(), qb.apply(P_)[:, 2].min()
+    for overlap in (0.002, -0.003):
+        b = SceneBuilder(fps=240)
+        a = b.addBody(xform=[0, 0, 1, *qa.as_quat()], gravity=False)
+        b.addShapeConvexHull(a, P_.tolist(), gap=0.005)
+        c = b.addBody(xform=[0, 0, 1 + zA - zB - overlap, *qb.as_quat()], gravity=False)
+        b.addShapeConvexHull(c, P_.tolist(), gap=0.005)
+        pw, o = _one_substep(b.finalize(solver=dict(substeps=1)), f64=True)
+        C = _contacts(o)
+        assert len(C) == 1
+        np.testing.assert_allclose(C[0, 4:7], [0, 0, 1], atol=1e-6)
+        assert abs(C[0, 3] + overlap) < 2e-7
+
+
+def test_mpr_depth_is_the_minkowski_support_along_its_normal():
+    """MPR (XenoCollide) on 40-vertex hulls under random relative poses: when the inflated shapes overlap it
+    returns a unit normal n of the Minkowski difference D = B - A whose depth is D's support along n plus the
+    inflation (brute force over the vertex sets), and no less than the true penetration depth, the minimum of
+    that support over 20000 directions; far apart it reports a miss."""
+    from scipy.spatial.transform import Rotation as Rot
+    from oracle_bind import mpr
+    rng = np.random.default_rng(7)
+    b = SceneBuilder(fps=240)
+    A, B = np.array(_sphere_points(40, 0.12, 3)), np.array(_sphere_points(40, 0.08, 4))
+    for P_ in (A, B):
+        bd = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1], gravity=False)
+        b.addShapeConvexHull(bd, P_.tolist())
+    pw = pack_scene(b.finalize(solver=dict(substeps=1)))
+    so = pw.h("W_SF_SHAPE")
+    ha, hb = (int(pw.statF[(so + s * 7) * 4 + 1]) for s in (0, 1))
+    dirs = rng.normal(size=(20000, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    hits = 0
+    for trial in range(12):
+        rot = Rot.random(random_state=trial)
+        q = rot.as_quat()
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        t = u * rng.uniform(0.1, 0.22)
+        r = 0.005
+        n_d = mpr(pw, ha, hb, q, t, r)
+        Bw = rot.apply(B) + t
+        hD = lambda n: (Bw @ n).max() - (A @ n).min()
+        pen = min((Bw @ dirs.T).max(axis=0) - (A @ dirs.T).min(axis=0)) + 2 * r
+        if n_d[3] < 0:
+            assert pen > 0, (trial, pen)  # reported misses are real misses of the inflated shapes
+            continue
+        hits += 1
+        n = n_d[:3]
+        assert abs(np.linalg.norm(n) - 1) < 1e-9
+        assert abs(n_d[3] - (hD(n) + 2 * r)) < 2e-6
+        assert n_d[3] >= -pen - 1e-9
+    assert hits >= 4
+    assert mpr(pw, ha, hb, (0, 0, 0, 1), (1.0, 0.2, 0), 0.005)[3] < 0
+
+
+def test_cylinder_support_is_the_extreme_point_of_the_solid_cylinder():
+    """The analytic cylinder support against a dense sampling of the cap rims (3600 points each): no sample
+    reaches further along d, and the point lies on the cylinder."""
+    from oracle_bind import gsup
+    r, hh = 0.05, 0.1
+    b = SceneBuilder(fps=240)
+    bd = b.addBody(xform=[0, 0, 1, 0, 0, 0, 1], gravity=False)
+    b.addShapeCylinder(bd, radius=r, halfHeight=hh)
+    pw = pack_scene(b.finalize(solver=dict(substeps=1)))
+    r, hh = _f32(r), _f32(hh)  # the packed dimensions
+    a = np.linspace(0, 2 * np.pi, 3600, endpoint=False)
+    rim = np.concatenate([np.stack([r * np.cos(a), r * np.sin(a), np.full_like(a, z)], 1) for z in (-hh, hh)])
+    rng = np.random.default_rng(1)
+    for d in list(rng.normal(size=(50, 3))) + [np.array([0.3, -0.4, 0.0]), np.array([0, 0, -1.0])]:
+        p = gsup(pw, 0, d)
+        assert p @ d >= (rim @ d).max() - 1e-9
+        assert np.hypot(p[0], p[1]) <= r * (1 + 1e-9) and abs(p[2]) <= hh * (1 + 1e-9)
+        if d[2] == 0:
+            assert p[2] == 0
+
+
+def _cylinder_on(ground, quat, z, r=0.05, hh=0.1):
+    b = SceneBuilder(fps=240)
+    if ground == "plane":
+        b.addGroundPlane(gap=0.001)
+    else:
+        bd = b.addBody(xform=[0, 0, -0.2, 0, 0, 0, 1], gravity=False)
+        b.addShapeBox(bd, hx=0.5, hy=0.5, hz=0.2, gap=0.001)
+    bd = b.addBody(xform=[0.1, -0.05, z, *quat], gravity=False)
+    b.addShapeCylinder(bd, radius=r, halfHeight=hh, gap=0.001)
+    return b.finalize(solver=dict(substeps=1))
+
+
+def test_upright_cylinder_on_plane_touches_at_three_rim_points():
+    """Upright, sunk by 2 mm: the deepest rim point and two more at 120 degrees on the bottom rim, each at depth
+    -2 mm, midway between the surfaces, normal +z."""
+    r, hh, dz = 0.05, 0.1, 0.002
+    pw, o = _one_substep(_cylinder_on("plane", (0, 0, 0, 1), hh - dz), f64=True)
+    C = _contacts(o)
+    assert len(C) == 3
+    np.testing.assert_allclose(C[:, 3], -dz, atol=1e-6)
+    np.testing.assert_allclose(C[:, 4:7], np.tile([0, 0, 1], (3, 1)), atol=1e-7)
+    rel = C[:, :2] - [0.1, -0.05]
+    np.testing.assert_allclose(np.hypot(rel[:, 0], rel[:, 1]), r, atol=1e-6)
+    ang = np.sort(np.degrees(np.arctan2(rel[:, 1], rel[:, 0])) % 360)
+    np.testing.assert_allclose(np.diff(ang), 120.0, atol=1e-3)
+    np.testing.assert_allclose(C[:, 2], -dz / 2, atol=1e-7)
+
+
+def test_cylinder_lying_on_plane_touches_at_both_ends_of_its_bottom_line():
+    """Lying along y, sunk by 2 mm: its two bottom end points (y = +-hh) at depth -2 mm."""
+    r, hh, dz = 0.05, 0.1, 0.002
+    s = np.sqrt(0.5)
+    pw, o = _one_substep(_cylinder_on("plane", (s, 0, 0, s), r - dz), f64=True)
+    C = _contacts(o)
+    assert len(C) == 2
+    np.testing.assert_allclose(C[:, 3], -dz, atol=1e-6)
+    np.testing.assert_allclose(np.sort(C[:, 1]), [-0.05 - hh, -0.05 + hh], atol=1e-6)
+    np.testing.assert_allclose(C[:, 0], 0.1, atol=1e-6)
+
+
+def test_cylinder_rolling_on_a_box_keeps_its_contacts_under_the_axis():
+    """A cylinder lying on a box (a discrete shape; 48-vertex hull, MPR path) within 2 degrees of flat rolls: its
+    contact points are moved onto the plane through its axis and the normal, so they sit straight under the axis,
+    at depth -2 mm along +z."""
+    r, hh, dz = 0.05, 0.1, 0.002
+    s = np.sqrt(0.5)
+    pw, o = _one_substep(_cylinder_on("box", (s, 0, 0, s), r - dz), f64=True)
+    C = _contacts(o)
+    assert len(C) >= 2
+    np.testing.assert_allclose(C[:, 0], 0.1, atol=1e-6)
+    np.testing.assert_allclose(np.abs(C[:, 6]), 1.0, atol=1e-6)
+    assert abs(C[:, 3].min() + dz) < 2e-5
+    assert C[:, 1].min() < -0.05 - 0.5 * hh and C[:, 1].max() > -0.05 + 0.5 * hh
+
+
 def test_contacts_sorted_by_canonical_key():
     pw, o = _one_substep(workloads.boxes64())
     C = _contacts(o)
