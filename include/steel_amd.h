@@ -37,7 +37,9 @@ enum steel_buffer {
   STEEL_BUF_CONTACTS = 6,  /* float4[6*MAXC], the last substep's contacts in canonical order */
   STEEL_BUF_ROWS = 7,      /* float4[MAXC*ROWS] */
   STEEL_BUF_SCHED = 8,     /* uint32 schedule tables */
-  STEEL_BUF_COUNTERS = 9,  /* uint32[4]: pairs, contacts, substep count, flags */
+  STEEL_BUF_COUNTERS = 9,  /* uint32[8]: pairs, contacts, substep count, control flags (bit 0: the user set joint
+                              targets, so a recorded target path no longer drives them), then the frame clock in
+                              seconds as a double (words 4-5, the reference's Sim.phase), 2 reserved */
   STEEL_BUF_STATS = 10,    /* uint32[16]: the reference's stats words 0-8 (src/wgsl.js:3879-3890); 9-11 the last
                               substep's candidate pairs, rows and units; 12-15 sums since reset of pairs, kept
                               contacts, rows and the substep count */

@@ -47,6 +47,10 @@ enum steel_world_field {
   W_VLIM_ON, W_F_VLIM_FRAC, W_A_VLG, W_A_AVL,
   /* free-body velocity limits (u32 switch); load-dependent joint friction (u32 switch) and its art offset */
   W_VLIM_FREE, W_JFL, W_A_JFL,
+  /* recorded target paths: statF section (float4 index), driven dofs, path frames (60 Hz), dofs with the
+     integral term, hold-per-frame switch (u32 each), frames per second (float bits), art offset of the
+     integral terms */
+  W_SF_PATH, W_DRIVEN, W_PATH_FRAMES, W_INTEGRAL_COUNT, W_TGT_HELD, W_FPS, W_A_INT,
   W_FIELD_COUNT
 };
 

@@ -66,12 +66,17 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL;
+  uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT;
+  T FPS, FRAME_DT;
   T F_VLIM_FRAC;
   T F_CFM, F_OMEGA, F_SPEC, F_GAP_GATE, F_FGAP, F_FSCALE, F_ANCHOR_BETA, F_BP_CELL, F_REST_SPEED, F_JL_GAP, F_JL_SPEED;
   T dt, beta, max_depen, warm_scale, reg;
   V3<T> g;
   uint32_t parity;
+  // Frame clock: substep index within the frame, control flags (bit 0: user-set targets), seconds since reset
+  // (the reference's Sim.phase, accumulated in double as the host does).
+  uint32_t substep_in_frame = 0, control = 0;
+  double phase = 0;
   // buffers
   std::vector<V4<T>> statF;
   const uint32_t* statU;
@@ -110,6 +115,8 @@ template <class T> struct World {
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
+    SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
+    TGT_HELD = h[W_TGT_HELD]; A_INT = h[W_A_INT]; FPS = hf(W_FPS); FRAME_DT = hf(W_FRAME_DT);
     F_CFM = hf(W_F_CFM); F_OMEGA = hf(W_F_OMEGA); F_SPEC = hf(W_F_SPEC); F_GAP_GATE = hf(W_F_GAP_GATE); F_FGAP = hf(W_F_FGAP);
     F_FSCALE = hf(W_F_FSCALE); F_ANCHOR_BETA = hf(W_F_ANCHOR_BETA); F_BP_CELL = hf(W_F_BP_CELL); F_REST_SPEED = hf(W_F_REST_SPEED);
     F_JL_GAP = hf(W_F_JL_GAP); F_JL_SPEED = hf(W_F_JL_SPEED);
@@ -1775,6 +1782,30 @@ template <class T> struct World {
     std::fill(wcount.begin(), wcount.end(), 0u);
     S_META[3] = 0;
     std::fill(wjsplit.begin(), wjsplit.end(), 0u);
+    // Recorded target paths (60 Hz samples), interpolated at every substep at its midpoint half a frame back
+    // (or held per frame), plus the integral term of the first dofs (wgsl.js:1800-1820); off once the user
+    // set targets.
+    uint32_t driven = (control & 1u) ? 0u : DRIVEN;
+    if (driven && PATH_FRAMES) {
+      T ph = T((float)phase);  // the per-substep parameter is a float
+      for (uint32_t i = 0; i < driven; i++) {
+        T ramp = (ph + (T(substep_in_frame) + T(0.5)) * dt) * T(60) - T(0.5);
+        T frame = mn(mx(TGT_HELD ? ph * T(60) : ramp, T(0)), T(PATH_FRAMES - 1u));
+        uint32_t fa = (uint32_t)frame, fb = std::min(fa + 1u, PATH_FRAMES - 1u);
+        uint32_t ia = SF_PATH * 4 + fa * driven + i, ib = SF_PATH * 4 + fb * driven + i;
+        T pa = (&statF[ia / 4].x)[ia % 4], pb = (&statF[ib / 4].x)[ib % 4];
+        T t = frame - T(fa);
+        T value = pa * (T(1) - t) + pb * t;
+        if (i < INTEGRAL_COUNT) {
+          if (substep_in_frame == 0) {
+            T err = mn(mx(value - art[A_Q + i], T(-0.08)), T(0.08));
+            art[A_INT + i] = mn(mx(art[A_INT + i] + T(3) * FRAME_DT * err, T(-0.12)), T(0.12));
+          }
+          value += art[A_INT + i];
+        }
+        art[A_TGT + i] = value;
+      }
+    }
     for (uint32_t a = 0; a < NA; a++) {
       uint32_t ai[4];
       art_info(a, ai);
@@ -2771,7 +2802,7 @@ struct OracleOut {
 
 template <class T>
 static int run_substeps(const uint32_t* header, const float* statF, const uint32_t* statU, T* poses, T* vel, T* art, uint32_t* akey,
-                        AnchorVal<T>* aval, uint32_t* substep_count, uint32_t nsub, uint32_t* stats_out, OracleOut* dbg) {
+                        AnchorVal<T>* aval, uint32_t* clock, uint32_t nsub, uint32_t* stats_out, OracleOut* dbg) {
   World<T> w;
   w.load_header(header);
   w.statF.resize(header[W_NSTATF]);
@@ -2785,8 +2816,13 @@ static int run_substeps(const uint32_t* header, const float* statF, const uint32
   w.aval = aval;
   w.alloc_scratch();
   if (stats_out) for (int i = 0; i < 16; i++) w.stats[i] = stats_out[i];
+  // clock: substep count, control flags, frame clock (double, words 2-3)
+  uint32_t* substep_count = clock;
+  std::memcpy(&w.phase, clock + 2, 8);
+  w.control = clock[1];
   for (uint32_t s = 0; s < nsub; s++) {
     w.parity = *substep_count % 2;
+    w.substep_in_frame = *substep_count % header[W_SUBSTEPS];
     w.shapes();
     w.broad();
     if (dbg && dbg->pairs) {
@@ -2821,7 +2857,9 @@ static int run_substeps(const uint32_t* header, const float* statF, const uint32
     }
     w.solve();
     *substep_count += 1;
+    if (*substep_count % header[W_SUBSTEPS] == 0) w.phase += 1.0 / (double)u2f(header[W_FPS]);
   }
+  std::memcpy(clock + 2, &w.phase, 8);
   if (stats_out) for (int i = 0; i < 16; i++) stats_out[i] = w.stats[i];
   return 0;
 }
@@ -2833,14 +2871,15 @@ size_t oracle_anchor_val_size_f32() { return sizeof(AnchorVal<float>); }
 size_t oracle_anchor_val_size_f64() { return sizeof(AnchorVal<double>); }
 
 // Runs nsub substeps on one world, in place.  poses/vel/art are float; akey is uint32[2*AH]; aval is
-// opaque storage of 2*AH * oracle_anchor_val_size_f32() bytes (zeroed at reset).
+// opaque storage of 2*AH * oracle_anchor_val_size_f32() bytes (zeroed at reset); clock is uint32[4]: substep
+// count, control flags (bit 0: user-set targets), frame clock in seconds (double), all zero at reset.
 int oracle_substeps_f32(const uint32_t* header, const float* statF, const uint32_t* statU, float* poses, float* vel, float* art,
-                        uint32_t* akey, void* aval, uint32_t* substep_count, uint32_t nsub, uint32_t* stats, OracleOut* dbg) {
-  return run_substeps<float>(header, statF, statU, poses, vel, art, akey, (AnchorVal<float>*)aval, substep_count, nsub, stats, dbg);
+                        uint32_t* akey, void* aval, uint32_t* clock, uint32_t nsub, uint32_t* stats, OracleOut* dbg) {
+  return run_substeps<float>(header, statF, statU, poses, vel, art, akey, (AnchorVal<float>*)aval, clock, nsub, stats, dbg);
 }
 int oracle_substeps_f64(const uint32_t* header, const float* statF, const uint32_t* statU, double* poses, double* vel, double* art,
-                        uint32_t* akey, void* aval, uint32_t* substep_count, uint32_t nsub, uint32_t* stats, OracleOut* dbg) {
-  return run_substeps<double>(header, statF, statU, poses, vel, art, akey, (AnchorVal<double>*)aval, substep_count, nsub, stats, dbg);
+                        uint32_t* akey, void* aval, uint32_t* clock, uint32_t nsub, uint32_t* stats, OracleOut* dbg) {
+  return run_substeps<double>(header, statF, statU, poses, vel, art, akey, (AnchorVal<double>*)aval, clock, nsub, stats, dbg);
 }
 // Anchor table of the float oracle in the kernels' uint32 layout (keys [2*AH], then AVN words per slot):
 // lets a test compare the carried anchors entry by entry.

@@ -801,6 +801,35 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   for (uint32_t i = tid; i < max(NL, 1u); i += BS) sched[S.JSPLIT + i] = 0;
   if (tid == 0) sched[S.META + 3] = 0;
   const float dt = w.hf(W_DT);
+  // Recorded target paths (60 Hz samples), interpolated at every substep at its midpoint half a frame back
+  // (or held per frame), plus the integral term of the first dofs (wgsl.js:1800-1820).  Off once the user
+  // set targets.
+  {
+    const uint32_t driven = (wd.ctr[3] & 1u) ? 0u : w.h(W_DRIVEN), frames = w.h(W_PATH_FRAMES);
+    if (driven && frames) {
+      const uint32_t sub = wd.ctr[2] % w.h(W_SUBSTEPS);
+      const float phase = (float)__hiloint2double((int)wd.ctr[5], (int)wd.ctr[4]);
+      const uint32_t PF = w.h(W_SF_PATH) * 4, NI = w.h(W_INTEGRAL_COUNT);
+      const float* path = (const float*)wd.statF;
+      for (uint32_t i = tid; i < driven; i += BS) {
+        float ramp = (phase + ((float)sub + 0.5f) * dt) * 60.0f - 0.5f;
+        float frame = clampf_(w.h(W_TGT_HELD) ? phase * 60.0f : ramp, 0.0f, (float)(frames - 1u));
+        uint32_t fa = (uint32_t)frame, fb = min(fa + 1u, frames - 1u);
+        float pa = path[PF + fa * driven + i], pb = path[PF + fb * driven + i];
+        float t = frame - (float)fa;
+        float value = pa * (1.0f - t) + pb * t;
+        if (i < NI) {
+          if (sub == 0) {
+            float err = clampf_(value - art_[w.h(W_A_Q) + i], -0.08f, 0.08f);
+            art_[w.h(W_A_INT) + i] = clampf_(art_[w.h(W_A_INT) + i] + 3.0f * w.hf(W_FRAME_DT) * err, -0.12f, 0.12f);
+          }
+          value += art_[w.h(W_A_INT) + i];
+        }
+        art_[w.h(W_A_TGT) + i] = value;
+      }
+      __syncthreads();
+    }
+  }
   for (uint32_t a = tid; a < NA; a += BS) {
     uint4 ai = w.art_info(a);
     if (w.h(W_VLIM_ON)) {
@@ -1986,6 +2015,11 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     st[15] += 1;
     wd.ctr[0] = 0; wd.ctr[1] = 0;
     wd.ctr[2] += 1;
+    if (wd.ctr[2] % w.h(W_SUBSTEPS) == 0) {  // end of a frame: the frame clock advances by 1 / fps, in double
+      double ph = __hiloint2double((int)wd.ctr[5], (int)wd.ctr[4]) + 1.0 / (double)w.hf(W_FPS);
+      wd.ctr[4] = (uint32_t)__double2loint(ph);
+      wd.ctr[5] = (uint32_t)__double2hiint(ph);
+    }
   }
 }
 

@@ -41,7 +41,7 @@ inline int world_sizes(const uint32_t* h, Sizes& s) {
   s.b[K_ART] = (size_t)h[W_NART] * 4;
   s.b[K_AKEY] = (size_t)2 * AH * 4;
   s.b[K_AVAL] = (size_t)2 * AH * STEEL_AVN * 4;
-  s.b[K_CTR] = 16;
+  s.b[K_CTR] = 32;
   s.b[K_STATS] = 64;
   s.b[K_PAIRS] = (size_t)2 * MAXP * 4;
   s.b[K_CRAW] = (size_t)6 * MAXC * 16;

@@ -223,8 +223,6 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         raise UnsupportedError("mimic joints are not supported yet")
     if S.get("control"):
         raise UnsupportedError("scene controllers are not supported yet")
-    if S.get("driven", 0) and S.get("target_frames", 0):
-        raise UnsupportedError("recorded target paths are not supported yet")
     if any(kin):
         raise UnsupportedError("kinematic bodies are not supported yet")
 
@@ -276,6 +274,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     SF["SF_FPLANE"] = off; off += nFaces
     SF["SF_LINK"] = off; off += max(NL, 1) * 6
     SF["SF_DOF"] = off; off += max(ND, 1) * 5
+    pathFloats = (S.get("target_frames") or 0) * (S.get("driven") or 0)
+    SF["SF_PATH"] = off; off += (pathFloats + 3) // 4
     nstatf = off
     statF = np.zeros(nstatf * 4, np.float32)
     SU = {}
@@ -329,10 +329,11 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     bodyPose = np.zeros(NB * 8, np.float32)
     pw = PackedWorld()
     pw.bodyKind = list(bodyKind)
-    # Bodies the mouse spring may pull (sim.js:176-178); drives are refused above, so no articulation is driven.
+    # Bodies the mouse spring may pull (sim.js:175-178): free bodies and movable links of undriven articulations.
     kinem = (S["bodies"].get("kinematic") or [0] * NB)
+    artDriven = [any(d["ke"] > 0 or d["kd"] > 0 for d in dofs[a["dofStart"]:a["dofStart"] + a["dofCount"]]) for a in arts]
     pw.draggable = [S.get("mouse_drag") is not False and not kinem[b] and
-                    (k == 1 or (k == 2 and links[bodyLink[b]]["movable"])) for b, k in enumerate(bodyKind)]
+                    (k == 1 or (k == 2 and links[bodyLink[b]]["movable"] and not artDriven[bodyArt[b]])) for b, k in enumerate(bodyKind)]
     for b in range(NB):
         o4 = (SF["SF_BODY"] + b * 8) * 4
         m = S["bodies"]["mass"][b]
@@ -570,11 +571,17 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for k, v in P.items():
         H("W_" + k, v)
     for k in ("A_Q", "A_QD", "A_QDH", "A_TGT", "A_TGTV", "A_LIM", "A_DRV", "A_JF", "A_BF", "A_FRIC", "A_HINV", "A_LINK", "A_LG", "A_RL",
-              "A_RLN", "A_ULOAD", "A_DENSE", "A_VLG", "A_AVL", "A_JFL"):
+              "A_RLN", "A_ULOAD", "A_DENSE", "A_VLG", "A_AVL", "A_JFL", "A_INT"):
         H("W_" + k, A[k])
     H("W_VLIM_ON", 1 if vlimOn else 0)
     H("W_VLIM_FREE", 1 if vlimFree else 0)
     H("W_JFL", 1 if any(d["loadCoef"] > 0 for d in dofs) else 0)
+    if pathFloats:
+        statF[SF["SF_PATH"] * 4:SF["SF_PATH"] * 4 + pathFloats] = f32(S["targets"])[:pathFloats]
+    H("W_DRIVEN", S.get("driven") or 0)
+    H("W_PATH_FRAMES", S.get("target_frames") or 0)
+    H("W_INTEGRAL_COUNT", S.get("integral_count") or 0)
+    H("W_TGT_HELD", 1 if S.get("targets_held") else 0)
     gate = SV.get("contact_gap_gate")
     fgap = SV.get("contact_friction_gap_max")
     jlg = SV.get("joint_limit_activation_gap")
@@ -585,7 +592,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
                  ("W_F_JL_SPEED", 2 if jlg is None else 0),
                  ("W_DT", dt), ("W_BETA", SV["baumgarte"]), ("W_MAX_DEPEN", 2.0), ("W_GX", grav[0]), ("W_GY", grav[1]), ("W_GZ", grav[2]),
                  ("W_WARM_SCALE", warm * SV["contact_warmstart_decay"]), ("W_REG", 0 if SV.get("contact_compliance") else o["regularization"]),
-                 ("W_FRAME_DT", 1 / fps), ("W_F_VLIM_FRAC", SV["velocity_limit_activation_fraction"])):
+                 ("W_FRAME_DT", 1 / fps), ("W_F_VLIM_FRAC", SV["velocity_limit_activation_fraction"]), ("W_FPS", fps)):
         HF(k, v)
 
     pw.statF = statF

@@ -112,7 +112,7 @@ def buffer_bytes(pw, name):
     if name == "stats":
         return 64
     if name == "counters":
-        return 16
+        return 32
     raise ValueError(f"buffer {name}: use an explicit byte count")
 
 

@@ -262,9 +262,17 @@ class Sim:
         self.batch.destroy()
 
     def reset(self):
+        """Sim.reset (sim.js:613-623): the initial state; user-set targets stay in force, as there."""
         self.batch.reset(0)
         self.time = 0.0
         self.frame = 0
+        if self.userTargets:
+            self._user_targets_flag()
+
+    def _user_targets_flag(self):
+        # counters word 3 bit 0: the user set joint targets, so a recorded target path stops driving them
+        # (the reference passes driven = 0 once userTargets is set, sim.js:633)
+        self.batch.write(0, "counters", np.asarray([1], np.uint32), 12)
 
     def step(self, frames=1, timestamps=False):
         self.batch.step(frames)
@@ -322,7 +330,9 @@ class Sim:
         if d is None:
             raise ValueError(f"dof {newtonDof} is not simulated as an articulation dof")
         A = self.packed.A
-        self.userTargets = True
+        if not self.userTargets:
+            self.userTargets = True
+            self._user_targets_flag()
         self.batch.write(0, "art", np.asarray([pos], np.float32), (A["A_TGT"] + d) * 4)
         self.batch.write(0, "art", np.asarray([vel], np.float32), (A["A_TGTV"] + d) * 4)
 

@@ -247,3 +247,28 @@ def serving_request(index, seed=SEED + 3, fps=240, substeps=4, iterations=12, nm
     for k in range(nq):
         _add_quadruped(b, x_row, (k - 1.5) * 1.2, rng)
     return b.finalize(solver=dict(substeps=substeps, iterations=iterations)), horizon
+
+
+def with_target_path(scene, path, integral=0, held=False):
+    """Attach a recorded joint-target path to a scene: `path` is frames x driven (60 Hz samples of the targets of
+    the first `driven` dofs, the scene-folder fields targets / target_frames / driven, sim.js:210, 372, 633),
+    `integral` dofs get the path's integral term, `held` holds each frame's sample."""
+    path = np.ascontiguousarray(path, np.float32)
+    off = len(scene.bin)
+    scene.bin = bytes(scene.bin) + path.tobytes()
+    J = scene.json
+    J["targets"] = {"off": off, "n": int(path.size)}
+    J["target_frames"], J["driven"] = path.shape
+    J["integral_count"], J["targets_held"] = integral, held
+    return scene
+
+
+def gait_path(frames=120, dofs=18, free=6, amp=0.3, hz=1.5, seed=0):
+    """A 60 Hz sinusoidal gait for a floating-base quadruped: the first `free` dofs (the base) stay 0, each leg
+    dof swings with its own phase."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(frames) / 60.0
+    ph = rng.uniform(0, 2 * np.pi, dofs)
+    path = amp * np.sin(2 * np.pi * hz * t[:, None] + ph[None, :])
+    path[:, :free] = 0.0
+    return path

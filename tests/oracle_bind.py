@@ -57,7 +57,7 @@ class OracleWorld:
         self.akey = np.zeros(2 * AH, np.uint32)
         vs = L.oracle_anchor_val_size_f64() if f64 else L.oracle_anchor_val_size_f32()
         self.aval = np.zeros(2 * AH * vs, np.uint8)
-        self.count = np.zeros(1, np.uint32)
+        self.count = np.zeros(4, np.uint32)  # substep count, control flags, frame clock (double)
         self.stats = np.zeros(16, np.uint32)
         self.statU = pw.statU if pw.statU.size else np.zeros(1, np.uint32)
         MAXC, MAXP = pw.h("W_MAXC"), pw.h("W_MAXP")

@@ -89,6 +89,9 @@ SCENES = {
     "quad_driven": driven_quadruped,
     "serve1500": lambda: workloads.serving_request(11, nmin=1500, nmax=1500)[0],
     "bighulls": big_hull_pile,
+    # a recorded 60 Hz gait driving the legs (base dofs of the path are 0; their drives are off)
+    "quad_path": lambda: workloads.with_target_path(workloads.quadruped_world(8, drive=dict(targetKe=40.0, targetKd=1.0)),
+                                                    workloads.gait_path(), integral=8),
     "boxes_limits": lambda: _with_body_limits(workloads.boxes64(n=40)),
     "quad_friction": lambda: _with_joint_friction(workloads.quadruped_world(6, drive=dict(targetKe=30.0, targetKd=1.0, friction=0.05),
                                                                           solver=dict(enable_joint_friction=True))),
@@ -125,7 +128,8 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
 
 
 @pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40), ("quad_vlim", 200),
-                                         ("boxes_limits", 150), ("quad_friction", 200), ("bighulls", 120)])
+                                         ("boxes_limits", 150), ("quad_friction", 200), ("bighulls", 120),
+                                         ("quad_path", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])
@@ -249,7 +253,7 @@ def test_joint_targets_set_mid_run_bitexact(sim):
 
 
 def test_one_step_error_against_fp64_within_1e5(sim):
-    for name in ("boxes64", "quad0", "quad5", "quad_driven", "quad_vlim", "boxes_limits", "quad_friction", "bighulls"):
+    for name in ("boxes64", "quad0", "quad5", "quad_driven", "quad_vlim", "boxes_limits", "quad_friction", "bighulls", "quad_path"):
         pw = pack.pack_scene(SCENES[name]())
         b = sim.Batch([pw])
         o64 = OracleWorld(pw, f64=True)

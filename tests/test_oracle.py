@@ -784,6 +784,57 @@ def test_free_body_velocity_limits_clamp_v_and_push_w_about_the_exceeded_axis():
     assert abs(o3.vel[5] + 2.5) < 1e-9
 
 
+def _path_hinge(path, fps=60, substeps=4, **kw):
+    scene = _vertical_hinge()
+    scene.json["fps"] = fps
+    scene.json["substeps"] = scene.json["solver"]["substeps"] = substeps
+    return workloads.with_target_path(scene, np.asarray(path).reshape(-1, 1), **kw)
+
+
+def test_recorded_path_substeps_average_to_each_frames_sample():
+    """The path is sampled at each substep's midpoint half a frame back, so over the substeps of frame k the
+    interpolated targets average to sample k (a linear path), and frame 0 starts clamped at sample 0; held
+    paths give sample k at every substep of frame k."""
+    path = 0.05 * np.arange(20)
+    pw = pack_scene(_path_hinge(path))
+    A = pw.A["A_TGT"]
+    o = OracleWorld(pw, f64=True)
+    tg = []
+    for k in range(12 * 4):
+        o.substeps(1)
+        tg.append(o.art[A])
+    tg = np.array(tg).reshape(12, 4)
+    assert tg[0, 0] == np.float32(path[0])
+    np.testing.assert_allclose(tg[1:].mean(axis=1), np.float32(path[1:12]), atol=2e-6)
+    pw = pack_scene(_path_hinge(path, held=True))
+    o = OracleWorld(pw, f64=True)
+    for k in range(6):
+        got = []
+        for s_ in range(4):
+            o.substeps(1)
+            got.append(o.art[A])
+        np.testing.assert_allclose(got, np.float32(path[k]), atol=1e-6)
+
+
+def test_recorded_path_integral_term_and_user_override():
+    """The integral term of the first dofs grows by 3 * frame_dt * the clamped error once per frame, saturating at
+    0.12 (a hinge that does not move, target 1 rad away); once the user sets targets the path stops driving."""
+    pw = pack_scene(_path_hinge(np.full(200, 1.0), integral=1))
+    A = pw.A
+    o = OracleWorld(pw, f64=True)
+    o.frames(10)
+    assert abs(o.art[A["A_INT"]] - 10 * 3 * _f32(1 / 60) * 0.08) < 1e-6
+    np.testing.assert_allclose(o.art[A["A_TGT"]], 1.0 + o.art[A["A_INT"]], atol=1e-6)
+    o.frames(30)
+    assert abs(o.art[A["A_INT"]] - 0.12) < 1e-7
+    pw = pack_scene(_path_hinge(np.full(200, 1.0)))
+    o = OracleWorld(pw, f64=True)
+    o.count[1] = 1
+    before = o.art[A["A_TGT"]]
+    o.frames(5)
+    assert o.art[A["A_TGT"]] == before
+
+
 def test_drive_and_spring_settle_at_their_stiffness_weighted_equilibrium():
     """Whole steps (solve + integration): a damped hinge under a drive and a spring comes to rest where their
     torques cancel, q* = (K q_tgt + K_s q_ref) / (K + K_s)."""
