@@ -51,6 +51,9 @@ enum steel_world_field {
      integral term, hold-per-frame switch (u32 each), frames per second (float bits), art offset of the
      integral terms */
   W_SF_PATH, W_DRIVEN, W_PATH_FRAMES, W_INTEGRAL_COUNT, W_TGT_HELD, W_FPS, W_A_INT,
+  /* sleeping: switch (u32), linear and angular speed thresholds, quiet time (float bits), art offset of the
+     per-node sleep records (age, asleep, -, -; free body b is node b, articulation a node NB + a) */
+  W_SLEEP, W_F_SLEEP_LIN, W_F_SLEEP_ANG, W_F_SLEEP_TIME, W_A_SLP,
   W_FIELD_COUNT
 };
 

@@ -66,8 +66,8 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT;
-  T FPS, FRAME_DT;
+  uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP;
+  T FPS, FRAME_DT, F_SLEEP_LIN, F_SLEEP_ANG, F_SLEEP_TIME;
   T F_VLIM_FRAC;
   T F_CFM, F_OMEGA, F_SPEC, F_GAP_GATE, F_FGAP, F_FSCALE, F_ANCHOR_BETA, F_BP_CELL, F_REST_SPEED, F_JL_GAP, F_JL_SPEED;
   T dt, beta, max_depen, warm_scale, reg;
@@ -93,7 +93,7 @@ template <class T> struct World {
   // sched sections (sim.js:442-444), one vector each
   std::vector<uint32_t> S_HKEY, S_HCNT, S_SLOT2U, S_CSLOT, S_CLOCAL, S_USIZE_TMP, S_UKEY, S_USIZE, S_USTART, S_CSTART, S_META, S_CORDER,
       S_RK0, S_RK1, S_RK2, S_KEPT, S_OSTART, S_LEAD, S_SLOT, S_ROWOF, S_RUNIT, S_PCOL, S_PORD, S_UTC, S_UTP, S_RCON, S_UWOFF, S_WALK;
-  std::vector<uint32_t> wkey, wcol, wmask, wcount, wlinkhit, wjsplit;
+  std::vector<uint32_t> wkey, wcol, wmask, wcount, wlinkhit, wjsplit, uf, uff;
   uint32_t wnu, wnwalk, ndc;
   std::vector<V4<T>> tw, imp;
   std::vector<T> aqd;
@@ -117,6 +117,7 @@ template <class T> struct World {
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
     TGT_HELD = h[W_TGT_HELD]; A_INT = h[W_A_INT]; FPS = hf(W_FPS); FRAME_DT = hf(W_FRAME_DT);
+    SLEEP = h[W_SLEEP]; A_SLP = h[W_A_SLP]; F_SLEEP_LIN = hf(W_F_SLEEP_LIN); F_SLEEP_ANG = hf(W_F_SLEEP_ANG); F_SLEEP_TIME = hf(W_F_SLEEP_TIME);
     F_CFM = hf(W_F_CFM); F_OMEGA = hf(W_F_OMEGA); F_SPEC = hf(W_F_SPEC); F_GAP_GATE = hf(W_F_GAP_GATE); F_FGAP = hf(W_F_FGAP);
     F_FSCALE = hf(W_F_FSCALE); F_ANCHOR_BETA = hf(W_F_ANCHOR_BETA); F_BP_CELL = hf(W_F_BP_CELL); F_REST_SPEED = hf(W_F_REST_SPEED);
     F_JL_GAP = hf(W_F_JL_GAP); F_JL_SPEED = hf(W_F_JL_SPEED);
@@ -1771,6 +1772,76 @@ template <class T> struct World {
       rows[o + PO + 3] = v4(qinvrot(qa, fra), put_u32((T*)nullptr, pair));
     }
   }
+  // ---- sleeping (wgsl.js:1370-1457; artwgsl.js:41-49): nodes are free bodies (b) and articulations (NB + a)
+  int body_node(int b) const {
+    if (b < 0) return -1;
+    uint32_t k = body_kind(b);
+    if (k == 1) return b;
+    if (k == 2 || k == 3) return (int)NB + (int)body_art(b);
+    return -1;
+  }
+  bool node_asleep(int n) const { return SLEEP && n >= 0 && art[A_SLP + (uint32_t)n * 4 + 1] > T(0.5); }
+  bool pair_asleep(int a, int b) const {
+    if (!SLEEP) return false;
+    int na = body_node(a), nb = body_node(b);
+    return (na < 0 || node_asleep(na)) && (nb < 0 || node_asleep(nb));
+  }
+  // Islands of nodes joined by contacts, each rooted at its lowest node; island flags on the root: 1 supported
+  // (a fixed-base articulation, or a contact with static geometry at depth <= 1e-5), 2 vetoed (moving, or an
+  // articulation with drives), 4 a member still too young to sleep.  Ages grow while an island is supported and
+  // quiet; an island sleeps once all its members reached the quiet time.
+  void sleep_islands() {
+    const uint32_t NN = NB + NA;
+    for (uint32_t n = 0; n < NN; n++) { uf[n] = n; uff[n] = 0; }
+    S_META[5] = 0;
+    uint32_t nc = std::min(counters[1], MAXC);
+    auto find = [&](uint32_t n) { while (uf[n] != n) n = uf[n]; return n; };
+    for (uint32_t c = 0; c < nc; c++) {
+      const V4<T>& m = contacts[c].q[2];
+      int na = body_node(ti(m.x)), nb = body_node(ti(m.y));
+      if ((na < 0 || nb < 0) && std::max(na, nb) >= 0 && contacts[c].q[0].w <= T(1e-5)) uff[(uint32_t)std::max(na, nb)] |= 1u;
+    }
+    for (uint32_t c = 0; c < nc; c++) {
+      const V4<T>& m = contacts[c].q[2];
+      int na = body_node(ti(m.x)), nb = body_node(ti(m.y));
+      if (na < 0 || nb < 0) continue;
+      uint32_t ra = find((uint32_t)na), rb = find((uint32_t)nb);
+      if (ra != rb) uf[std::max(ra, rb)] = std::min(ra, rb);
+    }
+    for (uint32_t n = 0; n < NN; n++) uf[n] = find(n);
+    for (uint32_t n = 0; n < NN; n++) {
+      bool moving = false, fixed = false, veto = false;
+      if (n < NB) {
+        if (body_kind((int)n) != 1) continue;
+        if (art[A_SLP + n * 4 + 1] < T(0.5)) moving = length(xyz(vel[n * 2])) > F_SLEEP_LIN || length(xyz(vel[n * 2 + 1])) > F_SLEEP_ANG;
+      } else {
+        uint32_t ai[4];
+        art_info(n - NB, ai);
+        fixed = link_type(ai[0]) != JT_FREE;
+        for (uint32_t k = 0; k < ai[1]; k++) {
+          SV<T> v = ldsv(lo(ai[0] + k) + L_V);
+          if (art[A_SLP + n * 4 + 1] < T(0.5) && (length(v.l) > F_SLEEP_LIN || length(v.a) > F_SLEEP_ANG)) moving = true;
+        }
+        for (uint32_t d = ai[2]; d < ai[2] + ai[3]; d++) { V4<T> dr = dof_drive(d); if (dr.y > T(0) || dr.z > T(0)) veto = true; }
+      }
+      uint32_t f = (fixed ? 1u : 0u) | ((moving || veto) ? 2u : 0u) | (uff[n] & 1u);
+      uff[uf[n]] |= f;
+    }
+    for (uint32_t n = 0; n < NN; n++) {
+      if (n < NB && body_kind((int)n) != 1) continue;
+      uint32_t r = uf[n], f = uff[r];
+      T age = art[A_SLP + n * 4];
+      if ((f & 2u) != 0 || (f & 1u) == 0) age = T(0); else age = mn(age + dt, F_SLEEP_TIME);
+      art[A_SLP + n * 4] = age;
+      if (age < F_SLEEP_TIME) uff[r] |= 4u;
+    }
+    for (uint32_t n = 0; n < NN; n++) {
+      if (n < NB && body_kind((int)n) != 1) continue;
+      bool s = (uff[uf[n]] & 4u) == 0;
+      art[A_SLP + n * 4 + 1] = s ? T(1) : T(0);
+      if (s) S_META[5]++;
+    }
+  }
   void prepare() {
     uint32_t ROWSN = ROWS;
     // Phase 0
@@ -1782,6 +1853,7 @@ template <class T> struct World {
     std::fill(wcount.begin(), wcount.end(), 0u);
     S_META[3] = 0;
     std::fill(wjsplit.begin(), wjsplit.end(), 0u);
+    if (SLEEP) sleep_islands();
     // Recorded target paths (60 Hz samples), interpolated at every substep at its midpoint half a frame back
     // (or held per frame), plus the integral term of the first dofs (wgsl.js:1800-1820); off once the user
     // set targets.
@@ -1870,9 +1942,17 @@ template <class T> struct World {
       const V4<T>& m = contacts[c].q[2];
       int a = ti(m.x), b = ti(m.y);
       uint32_t pk = contact_pair(c);
-      uint32_t cls;
-      if (same_art(a, b)) { cls = 0xffffffffu; dn++; S_CLOCAL[c] = 1; }
-      else cls = (c == 0 || contact_pair(c - 1) != pk) ? 1u : 0u;
+      uint32_t cls = 0xfffffffeu;  // no rows: both nodes asleep (or static)
+      if (!pair_asleep(a, b)) {
+        if (same_art(a, b)) {
+          cls = 0xffffffffu;
+          bool rws = !node_asleep(body_node(a));
+          if (rws) dn++;
+          S_CLOCAL[c] = rws ? 1u : 0u;
+        } else {
+          cls = (c == 0 || contact_pair(c - 1) != pk) ? 1u : 0u;
+        }
+      }
       S_CSLOT[c] = cls;
     }
     ndc = dn;
@@ -2628,12 +2708,19 @@ template <class T> struct World {
     std::vector<uint32_t> wcs(NCOL + 2);
     for (uint32_t i = 0; i < NCOL + 2; i++) wcs[i] = S_CSTART[i];
     for (uint32_t i = 0; i < ND; i++) aqd[i] = art[A_QDH + i];
+    if (SLEEP)
+      for (uint32_t a = 0; a < NA; a++) {
+        if (!node_asleep((int)(NB + a))) continue;
+        uint32_t ai[4];
+        art_info(a, ai);
+        for (uint32_t d = ai[2]; d < ai[2] + ai[3]; d++) aqd[d] = T(0);
+      }
     for (uint32_t b = 0; b < NB; b++) {
       uint32_t kind = body_kind((int)b);
       imp[b * 2] = {0, 0, 0, 0}; imp[b * 2 + 1] = {0, 0, 0, 0};
       if (kind == 1) {
         uint32_t fl = body_flags((int)b);
-        V3<T> v = xyz(vel[b * 2]) + (fl != 0 ? V3<T>{0, 0, 0} : g * dt);
+        V3<T> v = xyz(vel[b * 2]) + ((node_asleep((int)b) || fl != 0) ? V3<T>{0, 0, 0} : g * dt);
         V3<T> w = xyz(vel[b * 2 + 1]);
         uint32_t bfo = A_BF + b * 8;
         if ((fl & 1) == 0 && (vne0(ld3(bfo)) || vne0(ld3(bfo + 4)))) {
@@ -2666,6 +2753,7 @@ template <class T> struct World {
     if (PATCH) for (uint32_t b = 0; b < NB * 2; b++) poses[P_PREV + b] = poses[P_BODY + b];
     for (uint32_t b = 0; b < NB; b++) {
       if (body_kind((int)b) != 1) continue;
+      if (node_asleep((int)b)) { vel[b * 2] = {0, 0, 0, 0}; vel[b * 2 + 1] = {0, 0, 0, 0}; continue; }
       V3<T> v = xyz(tw[b * 2]), w = xyz(tw[b * 2 + 1]);
       T damp = (body_flags((int)b) & 1) != 0 ? T(1) : T(1) - body_limits((int)b).x * dt;
       vel[b * 2] = v4(v, T(0)); vel[b * 2 + 1] = v4(w * damp, T(0));
@@ -2758,7 +2846,7 @@ template <class T> struct World {
     stats[4] = std::max(stats[4], counters[0]);
     stats[5] = counters[1];
     stats[6] = ncol;
-    stats[8] = 0;
+    stats[8] = S_META[5];
     stats[7] = std::max(stats[7], S_META[3]);
     counters[0] = 0; counters[1] = 0;
   }
@@ -2774,6 +2862,7 @@ template <class T> struct World {
     z(S_PCOL, MAXC); z(S_PORD, MAXC); z(S_UTC, MAXU); z(S_UTP, MAXU); z(S_RCON, MAXC); z(S_UWOFF, UH); z(S_WALK, MAXC);
     z(wkey, MAXU); z(wcol, MAXU); z(wmask, (size_t)std::max(NB, 1u) * NCOLW); z(wcount, NCOL + 2); z(wlinkhit, std::max(NB, 1u));
     z(wjsplit, std::max(NL, 1u));
+    z(uf, NB + NA); z(uff, NB + NA);
     tw.assign(std::max(NB, 1u) * 2, V4<T>{0, 0, 0, 0});
     imp.assign(std::max(NB, 1u) * 2, V4<T>{0, 0, 0, 0});
     aqd.assign(std::max(ND, 1u), T(0));

@@ -31,7 +31,7 @@ struct WorldDev {
 // Offsets of the schedule tables inside WorldDev::sched (uint32 words), from the world's capacities.
 struct SchedLayout {
   uint32_t HKEY, HCNT, CSLOT, CLOCAL, UKEY, USIZE, USTART, CSTART, META, CORDER, RK0, RK1, RK2, KEPT, OSTART, LEAD, SLOT, ROWOF,
-      RUNIT, PCOL, PORD, UTC, UTP, RCON, UWOFF, WALK, HEADS, DSLOT, WCOL, WMASK, WCOUNT, LINKHIT, JSPLIT, SLOTT, NPAIRL, TOTAL;
+      RUNIT, PCOL, PORD, UTC, UTP, RCON, UWOFF, WALK, HEADS, DSLOT, WCOL, WMASK, WCOUNT, LINKHIT, JSPLIT, SLOTT, NPAIRL, UF, UFF, TOTAL;
 };
 
 __host__ __device__ inline SchedLayout sched_layout(uint32_t MAXC, uint32_t MAXU, uint32_t UH, uint32_t NCOL, uint32_t NB, uint32_t NCOLW,
@@ -73,6 +73,8 @@ __host__ __device__ inline SchedLayout sched_layout(uint32_t MAXC, uint32_t MAXU
   s.JSPLIT = o; o += NL > 0 ? NL : 1;
   s.SLOTT = o; o += MAXC * 4;
   s.NPAIRL = o; o += 4;
+  s.UF = o; o += (NB > 0 ? NB : 1) + (NL > 0 ? NL : 1);   // sleep islands: union-find parent per node (NB + NA <= NB + NL)
+  s.UFF = o; o += (NB > 0 ? NB : 1) + (NL > 0 ? NL : 1);  // island flags
   s.TOTAL = o;
   return s;
 }

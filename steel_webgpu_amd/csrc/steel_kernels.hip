@@ -775,6 +775,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   __shared__ uint32_t s_w[BS / 64];
   __shared__ uint32_t s_wr[PREP_CH], s_wk[PREP_CH];  // coloring: row and body-pair key of a chunk of walk entries
   __shared__ uint32_t s_wc[PREP_WC];                  // coloring: per-color counts and offsets when NCOL + 2 fits
+  __shared__ uint32_t s_chg;                          // sleep islands: a contact joined two islands this pass
   extern __shared__ uint32_t s_pdyn[];                 // coloring: color masks, then first-pending slots
   __shared__ uint32_t s_tree[(BS / 64) * 64];        // factor: link tree of each wave's articulation
   __shared__ float s_pab[(BS / 64) * 512];           // factor: children's bias forces
@@ -801,6 +802,83 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   for (uint32_t i = tid; i < max(NL, 1u); i += BS) sched[S.JSPLIT + i] = 0;
   if (tid == 0) sched[S.META + 3] = 0;
   const float dt = w.hf(W_DT);
+  if (w.h(W_SLEEP)) {
+    // Sleep islands (wgsl.js:1370-1457): nodes joined by contacts, each island rooted at its lowest node (roots
+    // hook under the smaller root with atomicMin and paths are compressed until no contact joins two islands,
+    // the serial union-find's result); island flags on the root: 1 supported, 2 vetoed (moving or driven), 4 a
+    // member still too young.  An island sleeps once all its members have been quiet for the quiet time.
+    const uint32_t NN = NB + NA, nc0 = min(wd.ctr[1], MAXC);
+    uint32_t* UF = sched + S.UF;
+    uint32_t* UFF = sched + S.UFF;
+    const float LIN = w.hf(W_F_SLEEP_LIN), ANG = w.hf(W_F_SLEEP_ANG), QUIET = w.hf(W_F_SLEEP_TIME);
+    const uint32_t ASLP = w.h(W_A_SLP);
+    auto find = [&](uint32_t n) {
+      for (uint32_t k = 0; k < NN; k++) {
+        uint32_t p = __atomic_load_n(&UF[n], __ATOMIC_RELAXED);
+        if (p == n) break;
+        n = p;
+      }
+      return n;
+    };
+    for (uint32_t n = tid; n < NN; n += BS) { UF[n] = n; UFF[n] = 0; }
+    if (tid == 0) sched[S.META + 5] = 0;
+    __syncthreads();
+    for (uint32_t c = tid; c < nc0; c += BS) {
+      float4 m = P.con(c, 2);
+      int na = w.body_node(ti(m.x)), nb = w.body_node(ti(m.y));
+      if ((na < 0 || nb < 0) && max(na, nb) >= 0 && P.con(c, 0).w <= 1e-5f) atomicOr(&UFF[(uint32_t)max(na, nb)], 1u);
+    }
+    for (;;) {
+      if (tid == 0) s_chg = 0;
+      __syncthreads();
+      for (uint32_t c = tid; c < nc0; c += BS) {
+        float4 m = P.con(c, 2);
+        int na = w.body_node(ti(m.x)), nb = w.body_node(ti(m.y));
+        if (na < 0 || nb < 0) continue;
+        uint32_t ra = find((uint32_t)na), rb = find((uint32_t)nb);
+        if (ra != rb) { atomicMin(&UF[max(ra, rb)], min(ra, rb)); s_chg = 1; }
+      }
+      __syncthreads();
+      for (uint32_t n = tid; n < NN; n += BS) atomicMin(&UF[n], find(n));
+      __syncthreads();
+      if (s_chg == 0) break;
+      __syncthreads();
+    }
+    for (uint32_t n = tid; n < NN; n += BS) {
+      bool moving = false, fixed = false, veto = false;
+      if (n < NB) {
+        if (w.body_kind((int)n) != 1) continue;
+        if (art_[ASLP + n * 4 + 1] < 0.5f) moving = len(xyz(wd.vel[n * 2])) > LIN || len(xyz(wd.vel[n * 2 + 1])) > ANG;
+      } else {
+        uint4 ai = w.art_info(n - NB);
+        fixed = w.link_type(ai.x) != JT_FREE;
+        for (uint32_t k = 0; k < ai.y; k++) {
+          SVd v = w.ldsv(w.lo(ai.x + k) + L_V);
+          if (art_[ASLP + n * 4 + 1] < 0.5f && (len(v.l) > LIN || len(v.a) > ANG)) moving = true;
+        }
+        for (uint32_t dd = ai.z; dd < ai.z + ai.w; dd++) { float4 dr = w.dof_drive(dd); if (dr.y > 0.0f || dr.z > 0.0f) veto = true; }
+      }
+      uint32_t f = (fixed ? 1u : 0u) | ((moving || veto) ? 2u : 0u) | (UFF[n] & 1u);
+      atomicOr(&UFF[UF[n]], f);
+    }
+    __syncthreads();
+    for (uint32_t n = tid; n < NN; n += BS) {
+      if (n < NB && w.body_kind((int)n) != 1) continue;
+      uint32_t r = UF[n], f = __atomic_load_n(&UFF[r], __ATOMIC_RELAXED);
+      float age = art_[ASLP + n * 4];
+      if ((f & 2u) != 0 || (f & 1u) == 0) age = 0.0f; else age = mn(age + dt, QUIET);
+      art_[ASLP + n * 4] = age;
+      if (age < QUIET) atomicOr(&UFF[r], 4u);
+    }
+    __syncthreads();
+    for (uint32_t n = tid; n < NN; n += BS) {
+      if (n < NB && w.body_kind((int)n) != 1) continue;
+      bool s = (UFF[UF[n]] & 4u) == 0;
+      art_[ASLP + n * 4 + 1] = s ? 1.0f : 0.0f;
+      if (s) atomicAdd(&sched[S.META + 5], 1u);
+    }
+    __syncthreads();
+  }
   // Recorded target paths (60 Hz samples), interpolated at every substep at its midpoint half a frame back
   // (or held per frame), plus the integral term of the first dofs (wgsl.js:1800-1820).  Off once the user
   // set targets.
@@ -903,9 +981,18 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     float4 m = P.con(c, 2);
     int a = ti(m.x), b = ti(m.y);
     uint32_t pk = P.contact_pair(c);
-    uint32_t cls, head = 0, dense = 0;
-    if (w.same_art(a, b)) { cls = 0xffffffffu; dense = 1; sched[S.CLOCAL + c] = 1; }
-    else { cls = (c == 0 || P.contact_pair(c - 1) != pk) ? 1u : 0u; head = cls; }
+    uint32_t cls = 0xfffffffeu, head = 0, dense = 0;  // no rows: both nodes asleep (or static)
+    if (!w.pair_asleep(a, b)) {
+      if (w.same_art(a, b)) {
+        cls = 0xffffffffu;
+        const bool rws = !w.node_asleep(w.body_node(a));
+        dense = rws ? 1u : 0u;
+        sched[S.CLOCAL + c] = dense;
+      } else {
+        cls = (c == 0 || P.contact_pair(c - 1) != pk) ? 1u : 0u;
+        head = cls;
+      }
+    }
     sched[S.CSLOT + c] = cls;
     sched[S.HEADS + c] = head;
     sched[S.DSLOT + c] = dense;
@@ -1849,6 +1936,14 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     sv.lm[i] = src < LB ? art_[w.lo(l) + src] : 0.0f;
   }
   for (uint32_t i = tid; i < ND; i += BS) sv.laqd[i] = art_[w.h(W_A_QDH) + i];
+  if (w.h(W_SLEEP)) {  // asleep articulations start at rest (wgsl.js:3640-3646)
+    __syncthreads();
+    for (uint32_t a = tid; a < NA; a += BS) {
+      if (!w.node_asleep((int)(NB + a))) continue;
+      uint4 ai = w.art_info(a);
+      for (uint32_t dd = ai.z; dd < ai.z + ai.w; dd++) sv.laqd[dd] = 0.0f;
+    }
+  }
   for (uint32_t b = tid; b < NB; b += BS) {
     uint32_t kind = w.body_kind((int)b);
     sv.imp[b * 2] = make_float4(0, 0, 0, 0); sv.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
@@ -1856,7 +1951,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     sv.tw[b * 2] = make_float4(0, 0, 0, 0); sv.tw[b * 2 + 1] = make_float4(0, 0, 0, 0);
     if (kind != 1) continue;
     uint32_t fl = w.body_flags((int)b);
-    f3 v = xyz(wd.vel[b * 2]) + (fl != 0 ? mk3(0, 0, 0) : g * dt);
+    f3 v = xyz(wd.vel[b * 2]) + ((w.node_asleep((int)b) || fl != 0) ? mk3(0, 0, 0) : g * dt);
     f3 wv = xyz(wd.vel[b * 2 + 1]);
     uint32_t bfo = w.h(W_A_BF) + b * 8;
     if ((fl & 1) == 0 && (vne0(w.ld3(bfo)) || vne0(w.ld3(bfo + 4)))) {
@@ -1902,6 +1997,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   }
   for (uint32_t b = tid; b < NB; b += BS) {
     if (w.body_kind((int)b) != 1) continue;
+    if (w.node_asleep((int)b)) { wd.vel[b * 2] = make_float4(0, 0, 0, 0); wd.vel[b * 2 + 1] = make_float4(0, 0, 0, 0); continue; }
     f3 v = xyz(sv.tw[b * 2]), wv = xyz(sv.tw[b * 2 + 1]);
     float damp = (w.body_flags((int)b) & 1) != 0 ? 1.0f : 1.0f - w.body_limits((int)b).x * dt;
     wd.vel[b * 2] = F4(v, 0.0f); wd.vel[b * 2 + 1] = F4(wv * damp, 0.0f);
@@ -2004,7 +2100,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     st[4] = max(st[4], wd.ctr[0]);
     st[5] = wd.ctr[1];
     st[6] = ncol;
-    st[8] = 0;
+    st[8] = sched[S.META + 5];
     st[7] = max(st[7], sched[S.META + 3]);
     st[9] = wd.ctr[0];   // last substep: candidate pairs, rows, units (per-step traffic accounting)
     st[10] = nrows;

@@ -70,6 +70,20 @@ struct W {
   DEV int body_link(int b) const { return ti(sf(bo(b)).w); }
   DEV f3 body_com_local(int b) const { return xyz(sf(bo(b) + 1)); }
   DEV bool same_art(int a, int b) const { return a >= 0 && b >= 0 && body_kind(a) == 2 && body_kind(b) == 2 && body_art(a) == body_art(b); }
+  // Sleep nodes: free body b is node b, the bodies of articulation a node NB + a (artwgsl.js:41-49).
+  DEV int body_node(int b) const {
+    if (b < 0) return -1;
+    uint32_t k = body_kind(b);
+    if (k == 1) return b;
+    if (k == 2 || k == 3) return (int)H[W_NB] + body_art(b);
+    return -1;
+  }
+  DEV bool node_asleep(int n) const { return H[W_SLEEP] && n >= 0 && d.art[H[W_A_SLP] + (uint32_t)n * 4 + 1] > 0.5f; }
+  DEV bool pair_asleep(int a, int b) const {
+    if (!H[W_SLEEP]) return false;
+    int na = body_node(a), nb = body_node(b);
+    return (na < 0 || node_asleep(na)) && (nb < 0 || node_asleep(nb));
+  }
   DEV float body_mass(int b) const { return sf(bo(b) + 1).w; }
   DEV q4 body_limits(int b) const { uint32_t o = bo(b); return mk4(sf(o + 3).w, sf(o + 4).w, sf(o + 5).w, sf(o + 6).w); }
   DEV float depen_cap(int a, int b) const {

@@ -117,7 +117,7 @@ def sched_layout(MAXC, MAXU, UH, NCOL, NB, NCOLW, NL):
              ("OSTART", MAXU), ("LEAD", MAXC), ("SLOT", MAXC * 26), ("ROWOF", MAXC), ("RUNIT", MAXC), ("PCOL", MAXC), ("PORD", MAXC),
              ("UTC", MAXU), ("UTP", MAXU), ("RCON", MAXC), ("UWOFF", UH), ("WALK", MAXC), ("HEADS", MAXC + 1), ("DSLOT", MAXC + 1),
              ("WCOL", MAXC), ("WMASK", max(NB, 1) * NCOLW), ("WCOUNT", NCOL + 2), ("LINKHIT", max(NB, 1)), ("JSPLIT", max(NL, 1)),
-             ("SLOTT", MAXC * 4), ("NPAIRL", 4)]
+             ("SLOTT", MAXC * 4), ("NPAIRL", 4), ("UF", max(NB, 1) + max(NL, 1)), ("UFF", max(NB, 1) + max(NL, 1))]
     out, o = {}, 0
     for k, n in names:
         out[k] = o
@@ -481,7 +481,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for k, default in (("enable_bilateral_preelimination", False), ("propagation_same_articulation_rows", False), ("update_mass_matrix_interval", 1)):
         if SV.get(k) is not None and SV.get(k) != default:
             raise ValueError(f"{k}={SV[k]} is not supported")
-    for k in ("enable_sleeping", "nesterov_momentum", "enable_torsional_rolling_friction", "contact_compliance", "contact_shared_anchor",
+    for k in ("nesterov_momentum", "enable_torsional_rolling_friction", "contact_compliance", "contact_shared_anchor",
               "contact_friction_shared_anchor"):
         if SV.get(k) or SO.get(k):
             raise UnsupportedError(f"{k} is not supported yet")
@@ -571,7 +571,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for k, v in P.items():
         H("W_" + k, v)
     for k in ("A_Q", "A_QD", "A_QDH", "A_TGT", "A_TGTV", "A_LIM", "A_DRV", "A_JF", "A_BF", "A_FRIC", "A_HINV", "A_LINK", "A_LG", "A_RL",
-              "A_RLN", "A_ULOAD", "A_DENSE", "A_VLG", "A_AVL", "A_JFL", "A_INT"):
+              "A_RLN", "A_ULOAD", "A_DENSE", "A_VLG", "A_AVL", "A_JFL", "A_INT", "A_SLP"):
         H("W_" + k, A[k])
     H("W_VLIM_ON", 1 if vlimOn else 0)
     H("W_VLIM_FREE", 1 if vlimFree else 0)
@@ -582,6 +582,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_PATH_FRAMES", S.get("target_frames") or 0)
     H("W_INTEGRAL_COUNT", S.get("integral_count") or 0)
     H("W_TGT_HELD", 1 if S.get("targets_held") else 0)
+    H("W_SLEEP", 1 if (SV.get("enable_sleeping") or SO.get("enable_sleeping")) else 0)
     gate = SV.get("contact_gap_gate")
     fgap = SV.get("contact_friction_gap_max")
     jlg = SV.get("joint_limit_activation_gap")
@@ -592,7 +593,10 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
                  ("W_F_JL_SPEED", 2 if jlg is None else 0),
                  ("W_DT", dt), ("W_BETA", SV["baumgarte"]), ("W_MAX_DEPEN", 2.0), ("W_GX", grav[0]), ("W_GY", grav[1]), ("W_GZ", grav[2]),
                  ("W_WARM_SCALE", warm * SV["contact_warmstart_decay"]), ("W_REG", 0 if SV.get("contact_compliance") else o["regularization"]),
-                 ("W_FRAME_DT", 1 / fps), ("W_F_VLIM_FRAC", SV["velocity_limit_activation_fraction"]), ("W_FPS", fps)):
+                 ("W_FRAME_DT", 1 / fps), ("W_F_VLIM_FRAC", SV["velocity_limit_activation_fraction"]), ("W_FPS", fps),
+                 ("W_F_SLEEP_LIN", SV.get("sleep_linear_threshold") if SV.get("sleep_linear_threshold") is not None else 0.05),
+                 ("W_F_SLEEP_ANG", SV.get("sleep_angular_threshold") if SV.get("sleep_angular_threshold") is not None else 0.15),
+                 ("W_F_SLEEP_TIME", SV.get("sleep_quiet_time") if SV.get("sleep_quiet_time") is not None else 0.5)):
         HF(k, v)
 
     pw.statF = statF

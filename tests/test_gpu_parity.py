@@ -40,6 +40,11 @@ def _with_body_limits(scene):
     return scene
 
 
+def _with_sleeping(scene, quiet=0.1):
+    scene.json["solver"].update(enable_sleeping=True, sleep_quiet_time=quiet)
+    return scene
+
+
 def _with_joint_friction(scene):
     """Load-dependent joint friction (coefficient 0.02) on top of the plain friction bound."""
     J = scene.json["joints"]
@@ -89,6 +94,8 @@ SCENES = {
     "quad_driven": driven_quadruped,
     "serve1500": lambda: workloads.serving_request(11, nmin=1500, nmax=1500)[0],
     "bighulls": big_hull_pile,
+    "boxes_sleep": lambda: _with_sleeping(workloads.boxes64()),
+    "quad_sleep": lambda: _with_sleeping(workloads.quadruped_world(9)),
     # a recorded 60 Hz gait driving the legs (base dofs of the path are 0; their drives are off)
     "quad_path": lambda: workloads.with_target_path(workloads.quadruped_world(8, drive=dict(targetKe=40.0, targetKd=1.0)),
                                                     workloads.gait_path(), integral=8),
@@ -129,7 +136,7 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
 
 @pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40), ("quad_vlim", 200),
                                          ("boxes_limits", 150), ("quad_friction", 200), ("bighulls", 120),
-                                         ("quad_path", 200)])
+                                         ("quad_path", 200), ("boxes_sleep", 400), ("quad_sleep", 300)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])
@@ -141,6 +148,8 @@ def test_trajectory_bitexact(sim, name, frames):
     assert np.array_equal(_bits(b.read(0, "art")), _bits(o.art))
     gs = b.stats_raw(0)[0]
     assert np.array_equal(gs[:9], o.stats[:9])
+    if name.endswith("sleep"):
+        assert o.stats[8] > 0  # some islands are asleep
     b.destroy()
 
 

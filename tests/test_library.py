@@ -92,7 +92,7 @@ def test_capacities_are_exact_bounds():
 
 def test_unsupported_options_raise():
     with pytest.raises(pack.UnsupportedError):
-        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"enable_sleeping": True}})
+        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"nesterov_momentum": True}})
     with pytest.raises(ValueError):
         pack.pack_scene(workloads.boxes64(n=4), opts={"substeps": 64})
     b = SceneBuilder()

@@ -835,6 +835,79 @@ def test_recorded_path_integral_term_and_user_override():
     assert o.art[A["A_TGT"]] == before
 
 
+def _sleepy(scene, quiet=0.25):
+    scene.json["solver"].update(enable_sleeping=True, sleep_quiet_time=quiet)
+    return pack_scene(scene)
+
+
+def test_resting_stack_falls_asleep_together_and_freezes():
+    """Two boxes stacked on the ground: supported and quiet, their ages grow by dt per substep; the island
+    (joined by their contact) sleeps in the same substep once both reached the quiet time, after which they
+    stay exactly still with zero velocity.  A box in free fall never sleeps (not supported)."""
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane(gap=0.002)
+    lo = b.addBody(xform=[0, 0, 0.1, 0, 0, 0, 1])
+    b.addShapeBox(lo, hx=0.2, hy=0.2, hz=0.1, gap=0.002)
+    hi = b.addBody(xform=[0.02, 0, 0.25, 0, 0, 0, 1])
+    b.addShapeBox(hi, hx=0.1, hy=0.1, hz=0.05, gap=0.002)
+    fl = b.addBody(xform=[2, 0, 50, 0, 0, 0, 1])
+    b.addShapeBox(fl, hx=0.1, hy=0.1, hz=0.1, gap=0.002)
+    pw = _sleepy(b.finalize(solver=dict(substeps=4)))
+    S, A = pw.A["A_SLP"], pw.A
+    dt = pw.hf("W_DT")
+    o = OracleWorld(pw)
+    asleep_at = None
+    for k in range(4 * 240):
+        o.substeps(1)
+        sl = o.art[S:S + 12].reshape(3, 4)
+        assert sl[0, 1] == sl[1, 1]          # one island
+        assert sl[2, 1] == 0 and sl[2, 0] == 0  # free fall: never supported
+        if sl[0, 1] == 1 and asleep_at is None:
+            asleep_at = k
+            assert o.stats[8] == 2
+            break
+    assert asleep_at is not None
+    # ages reached the quiet time, having grown by dt per substep while quiet
+    assert np.all(sl[:2, 0] == np.float32(0.25))
+    p0, v0 = o.poses[:16].copy(), o.vel[:16].copy()
+    o.substeps(40)
+    assert np.array_equal(o.poses[:16], p0) and np.all(o.vel[:16] == 0)
+
+
+def test_age_grows_by_dt_per_quiet_substep():
+    """A box at rest on the ground: its age is k * dt (float32 sums) after k quiet substeps, until the quiet time."""
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane(gap=0.002)
+    bd = b.addBody(xform=[0, 0, 0.1, 0, 0, 0, 1])
+    b.addShapeBox(bd, hx=0.2, hy=0.2, hz=0.1, gap=0.002)
+    pw = _sleepy(b.finalize(solver=dict(substeps=4)), quiet=1.0)
+    S = pw.A["A_SLP"]
+    dt = np.float32(pw.hf("W_DT"))
+    o = OracleWorld(pw)
+    ages = []
+    for k in range(60):
+        o.substeps(1)
+        ages.append(o.art[S])
+    # the first substeps may still see the settling motion; from the first nonzero age on it grows by dt
+    ages = np.array(ages, np.float32)
+    k0 = int(np.argmax(ages > 0))
+    acc = np.float32(0)
+    for a in ages[k0:]:
+        acc = np.float32(acc + dt)
+        assert a == acc
+
+
+def test_driven_articulation_never_sleeps():
+    """Drives veto sleep (robots follow their targets), even at rest on a fixed base."""
+    scene = _vertical_hinge(targetKe=10.0, targetKd=1.0, targetPos=0.2, effortLimit=0.0)
+    pw = _sleepy(scene, quiet=0.05)
+    o = OracleWorld(pw)
+    o.frames(60)
+    S = pw.A["A_SLP"]
+    node = pw.NB  # the articulation's node
+    assert o.art[S + node * 4 + 1] == 0 and o.art[S + node * 4] == 0
+
+
 def test_drive_and_spring_settle_at_their_stiffness_weighted_equilibrium():
     """Whole steps (solve + integration): a damped hinge under a drive and a spring comes to rest where their
     torques cancel, q* = (K q_tgt + K_s q_ref) / (K + K_s)."""
