@@ -43,7 +43,9 @@ enum steel_buffer {
   STEEL_BUF_STATS = 10,    /* uint32[16]: the reference's stats words 0-8 (src/wgsl.js:3879-3890); 9-11 the last
                               substep's candidate pairs, rows and units; 12-15 sums since reset of pairs, kept
                               contacts, rows and the substep count */
-  STEEL_BUF_COUNT = 11
+  STEEL_BUF_WARM = 11,     /* uint32[10*W_WH] contact warm start (two parities): keys [2*W_WH], then per slot the
+                              normal impulse and the tangent impulse as a world vector (float bits) [2*W_WH*4] */
+  STEEL_BUF_COUNT = 12
 };
 
 /* Stage mask for steel_batch_substep_stages (stage-isolated parity runs). */

@@ -54,6 +54,11 @@ enum steel_world_field {
   /* sleeping: switch (u32), linear and angular speed thresholds, quiet time (float bits), art offset of the
      per-node sleep records (age, asleep, -, -; free body b is node b, articulation a node NB + a) */
   W_SLEEP, W_F_SLEEP_LIN, W_F_SLEEP_ANG, W_F_SLEEP_TIME, W_A_SLP,
+  /* mimic joints: count (u32), statF section of their records (2 float4 each: articulation, follower dof,
+     leader dof, c1; follower q, leader q, c0, -), statU section of the per-articulation "owns a constraint" flags */
+  W_NMIMIC, W_SF_MIMIC, W_SU_ACON,
+  /* contact warm start: slots of the warm table per parity (u32; 0 when warm start is off) */
+  W_WH,
   W_FIELD_COUNT
 };
 

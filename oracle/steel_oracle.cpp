@@ -49,6 +49,8 @@ inline uint32_t fflip_u(uint32_t i) { return i ^ (((i >> 31) != 0 ? 0xffffffffu 
 
 template <class T> struct Contact { V4<T> q[5]; T smu[2]; };
 template <class T> struct AnchorVal { uint32_t pair, k; V3<T> ll, el, il, nl; };
+// Contact warm-start table slot: row key, then the normal impulse and the tangent impulse as a world vector.
+template <class T> struct WarmSlot { uint32_t key; V4<T> v; };
 template <class T> struct FaceQ { T sep; uint32_t face; };
 template <class T> struct EdgeQ { T sep; V3<T> axis, a0, a1, b0, b1; uint32_t id; };
 template <class T> struct SegQ { T d2; V3<T> ps, ph; uint32_t face, kind; };
@@ -66,7 +68,8 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP;
+  uint32_t WH;
+  uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
   T FPS, FRAME_DT, F_SLEEP_LIN, F_SLEEP_ANG, F_SLEEP_TIME;
   T F_VLIM_FRAC;
   T F_CFM, F_OMEGA, F_SPEC, F_GAP_GATE, F_FGAP, F_FSCALE, F_ANCHOR_BETA, F_BP_CELL, F_REST_SPEED, F_JL_GAP, F_JL_SPEED;
@@ -85,6 +88,7 @@ template <class T> struct World {
   T* art;
   uint32_t* akey;           // [2*AH]
   AnchorVal<T>* aval;       // [2*AH]
+  WarmSlot<T>* wtab;        // [2*WH] contact warm start (two parities)
   // scratch
   uint32_t counters[2];
   std::vector<uint32_t> pairs;  // 2 per pair
@@ -114,10 +118,11 @@ template <class T> struct World {
     A_Q = h[W_A_Q]; A_QD = h[W_A_QD]; A_QDH = h[W_A_QDH]; A_TGT = h[W_A_TGT]; A_TGTV = h[W_A_TGTV]; A_LIM = h[W_A_LIM]; A_DRV = h[W_A_DRV];
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
+    WH = h[W_WH];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
     TGT_HELD = h[W_TGT_HELD]; A_INT = h[W_A_INT]; FPS = hf(W_FPS); FRAME_DT = hf(W_FRAME_DT);
-    SLEEP = h[W_SLEEP]; A_SLP = h[W_A_SLP]; F_SLEEP_LIN = hf(W_F_SLEEP_LIN); F_SLEEP_ANG = hf(W_F_SLEEP_ANG); F_SLEEP_TIME = hf(W_F_SLEEP_TIME);
+    SLEEP = h[W_SLEEP]; A_SLP = h[W_A_SLP]; NMIMIC = h[W_NMIMIC]; SF_MIMIC = h[W_SF_MIMIC]; SU_ACON = h[W_SU_ACON]; F_SLEEP_LIN = hf(W_F_SLEEP_LIN); F_SLEEP_ANG = hf(W_F_SLEEP_ANG); F_SLEEP_TIME = hf(W_F_SLEEP_TIME);
     F_CFM = hf(W_F_CFM); F_OMEGA = hf(W_F_OMEGA); F_SPEC = hf(W_F_SPEC); F_GAP_GATE = hf(W_F_GAP_GATE); F_FGAP = hf(W_F_FGAP);
     F_FSCALE = hf(W_F_FSCALE); F_ANCHOR_BETA = hf(W_F_ANCHOR_BETA); F_BP_CELL = hf(W_F_BP_CELL); F_REST_SPEED = hf(W_F_REST_SPEED);
     F_JL_GAP = hf(W_F_JL_GAP); F_JL_SPEED = hf(W_F_JL_SPEED);
@@ -1765,7 +1770,15 @@ template <class T> struct World {
           h = (h + 1) % AH;
         }
       }
-      (void)impv; (void)carried;  // warm start of anchors is off on this path (warm_scale 0)
+      // Warm start: the carried anchor impulse, else the contact's, within the anchor's share mu / na of the
+      // members' warm normal load (wgsl.js:1711-1719).
+      if (warm_scale > T(0)) {
+        V2<T> lw{rows[o + 6].y, rows[o + 6].z};
+        if (carried) lw = V2<T>{dot(impv, t1), dot(impv, t2)} * warm_scale;
+        T rad = mx(rows[o + 2].w * load, T(0));
+        if (length(lw) > rad) lw = lw * (rad / length(lw));
+        rows[o + 6] = {rows[o + 6].x, lw.x, lw.y, rows[o + 6].w};
+      }
       rows[o + PO] = v4(fra, T(2));
       rows[o + PO + 1] = v4(frb, (T)k);
       rows[o + PO + 2] = v4(e, T(0));
@@ -1823,6 +1836,7 @@ template <class T> struct World {
           if (art[A_SLP + n * 4 + 1] < T(0.5) && (length(v.l) > F_SLEEP_LIN || length(v.a) > F_SLEEP_ANG)) moving = true;
         }
         for (uint32_t d = ai[2]; d < ai[2] + ai[3]; d++) { V4<T> dr = dof_drive(d); if (dr.y > T(0) || dr.z > T(0)) veto = true; }
+        if (statU[SU_ACON + n - NB] != 0) veto = true;  // owns a mimic
       }
       uint32_t f = (fixed ? 1u : 0u) | ((moving || veto) ? 2u : 0u) | (uff[n] & 1u);
       uff[uf[n]] |= f;
@@ -1842,12 +1856,71 @@ template <class T> struct World {
       if (s) S_META[5]++;
     }
   }
+  // ---- contact warm start (wgsl.js:2175-2190, 2421-2435; 3614-3631, 3737-3767)
+  // The previous substep's impulses stored under `key`, scaled by the warm scale; the tangent part is projected on
+  // (t1, t2) and kept within the friction cone mu * ln.
+  void warm_lookup(uint32_t key, T mu, V3<T> t1, V3<T> t2, T& ln, V2<T>& lt) const {
+    if (!WH) return;
+    V3<T> lf{0, 0, 0};
+    uint32_t rbase = parity * WH, h = key % WH;
+    for (uint32_t probe = 0; probe < 32; probe++) {
+      const WarmSlot<T>& s = wtab[rbase + h];
+      if (s.key == 0) break;
+      if (s.key == key) { ln = s.v.x * warm_scale; lf = V3<T>{s.v.y, s.v.z, s.v.w} * warm_scale; break; }
+      h = (h + 1) % WH;
+    }
+    lt = {dot(lf, t1), dot(lf, t2)};
+    T lim = mu * ln, lm = length(lt);
+    if (lm > lim) lt = lt * (lim / mx(lm, T(1e-12)));
+  }
+  // Row i's key (contact rows, then dense rows; 0xffffffff for a bilateral row, which stores nothing).
+  uint32_t warm_key(uint32_t i, uint32_t nc) const {
+    if (i < nc) return (uint32_t)rows[i * ROWS + 3].w | ((uint32_t)rows[i * ROWS + 4].w << 16);
+    uint32_t o = A_DENSE + (i - nc) * DPS;
+    if (art[o + 21] > T(0.5)) return 0xffffffffu;
+    return (uint32_t)art[o + 1] | ((uint32_t)art[o + 2] << 16);
+  }
+  V4<T> warm_value(uint32_t i, uint32_t nc) const {
+    if (i < nc) {
+      uint32_t o = i * ROWS;
+      V4<T> lam = rows[o + 6];
+      V3<T> f = xyz(rows[o + 1]) * lam.y + xyz(rows[o + 2]) * lam.z;
+      return {lam.x, f.x, f.y, f.z};
+    }
+    uint32_t o = A_DENSE + (i - nc) * DPS;
+    V3<T> f = ld3(o + 12) * art[o + 10] + ld3(o + 15) * art[o + 11];
+    return {art[o + 9], f.x, f.y, f.z};
+  }
+  // Table of the next substep: every key inserted by linear probing (at most 32 probes); its value is that of
+  // the row with the largest normal impulse (negative and -0 count as 0), ties going to the lowest row.
+  void warm_store(uint32_t nc) {
+    uint32_t nw = nc + std::min(S_META[3], MAXDENSE), wb = (1 - parity) * WH;
+    std::vector<int64_t> best(WH, -1);
+    for (uint32_t i = 0; i < nw; i++) {
+      uint32_t key = warm_key(i, nc);
+      if (key == 0xffffffffu) continue;
+      uint32_t h = key % WH;
+      for (uint32_t probe = 0; probe < 32; probe++) {
+        WarmSlot<T>& s = wtab[wb + h];
+        if (s.key == 0) s.key = key;
+        if (s.key == key) {
+          T x = warm_value(i, nc).x, y = best[h] < 0 ? T(0) : warm_value((uint32_t)best[h], nc).x;
+          if (best[h] < 0 || (x > T(0) ? x : T(0)) > (y > T(0) ? y : T(0))) best[h] = i;
+          break;
+        }
+        h = (h + 1) % WH;
+      }
+    }
+    for (uint32_t h = 0; h < WH; h++)
+      if (best[h] >= 0) wtab[wb + h].v = warm_value((uint32_t)best[h], nc);
+  }
   void prepare() {
     uint32_t ROWSN = ROWS;
     // Phase 0
     std::fill(S_HKEY.begin(), S_HKEY.end(), 0u);
     std::fill(S_HCNT.begin(), S_HCNT.end(), 0u);
     if (PATCH) for (uint32_t i = 0; i < AH; i++) akey[(1 - parity) * AH + i] = 0;
+    for (uint32_t i = 0; i < WH; i++) wtab[(1 - parity) * WH + i].key = 0;
     std::fill(wlinkhit.begin(), wlinkhit.end(), 0u);
     std::fill(wmask.begin(), wmask.end(), 0u);
     std::fill(wcount.begin(), wcount.end(), 0u);
@@ -2129,6 +2202,7 @@ template <class T> struct World {
       }
       T ln = 0;
       V2<T> lt{0, 0};
+      warm_lookup(key, mu, t1, t2, ln, lt);
       rows[o + 0] = v4(n, effn);
       rows[o + 1] = v4(t1, rhs);
       rows[o + 2] = v4(t2, mu);
@@ -2208,8 +2282,8 @@ template <class T> struct World {
         std::copy(rr.begin(), rr.end(), S_PORD.begin() + o0);
       }
     }
-    // Phase 7: dense rows for same-articulation contacts, in contact order
-    S_META[3] = ndc;
+    // Phase 7: dense rows for same-articulation contacts, in contact order, then the mimics in fixed slots
+    S_META[3] = ndc + NMIMIC;
     uint32_t dslot = 0;
     for (uint32_t c = 0; c < nc; c++) {
       if (S_CSLOT[c] != 0xffffffffu || S_CLOCAL[c] == 0) continue;
@@ -2257,15 +2331,50 @@ template <class T> struct World {
         bounce = restitution_fires(e, phi, vn0);
         if (bounce) rhs = e * vn0;
       }
+      T ln = 0;
+      V2<T> lt{0, 0};
+      warm_lookup(key, mu, t1, t2, ln, lt);
       art[o] = (T)art_id; art[o + 1] = (T)(key & 0xffffu); art[o + 2] = (T)(key >> 16); art[o + 3] = mu;
       art[o + 4] = rhs; art[o + 5] = effn;
       art[o + 6] = k11 / det; art[o + 7] = -k01 / det; art[o + 8] = k00 / det;
-      art[o + 9] = 0; art[o + 10] = 0; art[o + 11] = 0;
+      art[o + 9] = ln; art[o + 10] = lt.x; art[o + 11] = lt.y;
       st3(o + 12, t1); st3(o + 15, t2); st3(o + 18, n);
       art[o + 21] = 0;
       art[o + 24] = bounce ? T(1) : row_weight(phi);
       art[o + 22] = bounce ? T(1e30) : phi;
     }
+    // Mimic rows: qd_follower - c1 qd_leader, Baumgarte on q_follower - c1 q_leader - c0 (wgsl.js:2482-2492).
+    for (uint32_t i = 0; i < NMIMIC; i++) {
+      V4<T> M0 = statF[SF_MIMIC + i * 2], M1 = statF[SF_MIMIC + i * 2 + 1];
+      uint32_t slot = ndc + i;
+      if (slot >= MAXDENSE) continue;
+      std::vector<T> J(3 * NDA, T(0));
+      J[tu(M0.y)] = T(1);
+      J[tu(M0.z)] -= M0.w;
+      T phi = art[A_Q + tu(M1.x)] - M0.w * art[A_Q + tu(M1.y)] - M1.z;
+      put_bilateral(A_DENSE + slot * DPS, tu(M0.x), J, 1, V3<T>{beta * phi / dt, T(0), T(0)});
+    }
+  }
+  // Bilateral dense block: nb unclamped rows of J, their Y = H^-1 J^T, 1 / (J Y) and the rhs (wgsl.js:1351-1369).
+  void put_bilateral(uint32_t o, uint32_t art_id, const std::vector<T>& J, uint32_t nb, V3<T> rhs) {
+    uint32_t ai[4];
+    art_info(art_id, ai);
+    for (uint32_t r = 0; r < nb; r++) {
+      std::vector<T> tau(NDA), y(NDA, T(0));
+      for (uint32_t k = 0; k < NDA; k++) tau[k] = k < ai[3] ? J[r * NDA + k] : T(0);
+      art_solve_gen(art_id, tau.data(), y.data());
+      T kk = 0;
+      for (uint32_t k = 0; k < ai[3]; k++) {
+        art[o + DHDR + r * NDA + k] = J[r * NDA + k];
+        art[o + DHDR + 3 * NDA + r * NDA + k] = y[k];
+        kk += J[r * NDA + k] * y[k];
+      }
+      art[o + 5 + r] = kk > T(1e-12) ? T(1) / kk : T(0);
+    }
+    art[o] = (T)art_id; art[o + 1] = 0; art[o + 2] = 0;
+    art[o + 4] = rhs.x; art[o + 22] = rhs.y; art[o + 23] = rhs.z;
+    art[o + 9] = 0; art[o + 10] = 0; art[o + 11] = 0;
+    art[o + 21] = (T)nb;
   }
 
   // ============================ solve (wgsl.js:2501-3891) ============================
@@ -2493,6 +2602,25 @@ template <class T> struct World {
       touched = true;
       uint32_t jo = o + DHDR, yo = o + DHDR + 3 * NDA;
       V3<T> lam{art[o + 9], art[o + 10], art[o + 11]};
+      if (art[o + 21] > T(0.5)) {
+        // Bilateral rows (mimics): unclamped, one at a time; no warm impulse (wgsl.js:3319-3332).
+        if (mode == 0) continue;
+        T nlam[3] = {lam.x, lam.y, lam.z};
+        for (uint32_t r = 0; r < tu(art[o + 21]); r++) {
+          T part[32];
+          for (uint32_t l = 0; l < 32; l++) {
+            part[l] = T(0);
+            for (uint32_t k = l; k < n; k += 32) part[l] += art[jo + r * NDA + k] * aqd[ai[2] + k];
+          }
+          T jv = lane_sum32(part);
+          T rhs = r == 0 ? art[o + 4] : (r == 1 ? art[o + 22] : art[o + 23]);
+          T dl = -F_OMEGA * (jv + rhs) * art[o + 5 + r];
+          nlam[r] += dl;
+          for (uint32_t k = 0; k < n; k++) aqd[ai[2] + k] += art[yo + r * NDA + k] * dl;
+        }
+        art[o + 9] = nlam[0]; art[o + 10] = nlam[1]; art[o + 11] = nlam[2];
+        continue;
+      }
       if (mode == 0) {
         for (uint32_t k = 0; k < n; k++) aqd[ai[2] + k] += art[yo + k] * lam.x + art[yo + NDA + k] * lam.y + art[yo + 2 * NDA + k] * lam.z;
         continue;
@@ -2750,6 +2878,7 @@ template <class T> struct World {
     sweep(0, ncol, 0, wcs, slot, nu);
     for (uint32_t it = 0; it < ITERS; it++) sweep(1, ncol, it, wcs, slot, nu);
     uint32_t nc = S_USTART[nu];
+    if (WH) warm_store(std::min(nc, MAXC));
     if (PATCH) for (uint32_t b = 0; b < NB * 2; b++) poses[P_PREV + b] = poses[P_BODY + b];
     for (uint32_t b = 0; b < NB; b++) {
       if (body_kind((int)b) != 1) continue;
@@ -2887,11 +3016,12 @@ struct OracleOut {
   uint32_t* rowinfo;    // [MAXC * 4]: (contact key lo|hi<<16, color, unit, body pair)
   uint32_t* nrows;
   uint32_t* meta;       // [8]: nu, ncol, nc, ndense, ...
+  float* rows;          // [MAXC * ROWS * 4]: contact rows after prepare (initial impulses included)
 };
 
 template <class T>
 static int run_substeps(const uint32_t* header, const float* statF, const uint32_t* statU, T* poses, T* vel, T* art, uint32_t* akey,
-                        AnchorVal<T>* aval, uint32_t* clock, uint32_t nsub, uint32_t* stats_out, OracleOut* dbg) {
+                        AnchorVal<T>* aval, WarmSlot<T>* wtab, uint32_t* clock, uint32_t nsub, uint32_t* stats_out, OracleOut* dbg) {
   World<T> w;
   w.load_header(header);
   w.statF.resize(header[W_NSTATF]);
@@ -2903,6 +3033,7 @@ static int run_substeps(const uint32_t* header, const float* statF, const uint32
   w.art = art;
   w.akey = akey;
   w.aval = aval;
+  w.wtab = wtab;
   w.alloc_scratch();
   if (stats_out) for (int i = 0; i < 16; i++) w.stats[i] = stats_out[i];
   // clock: substep count, control flags, frame clock (double, words 2-3)
@@ -2944,6 +3075,11 @@ static int run_substeps(const uint32_t* header, const float* statF, const uint32
       }
       for (int i = 0; i < 8; i++) dbg->meta[i] = w.S_META[i];
     }
+    if (dbg && dbg->rows)
+      for (size_t i = 0; i < (size_t)w.MAXC * w.ROWS; i++) {
+        dbg->rows[i * 4] = (float)w.rows[i].x; dbg->rows[i * 4 + 1] = (float)w.rows[i].y;
+        dbg->rows[i * 4 + 2] = (float)w.rows[i].z; dbg->rows[i * 4 + 3] = (float)w.rows[i].w;
+      }
     w.solve();
     *substep_count += 1;
     if (*substep_count % header[W_SUBSTEPS] == 0) w.phase += 1.0 / (double)u2f(header[W_FPS]);
@@ -2955,20 +3091,25 @@ static int run_substeps(const uint32_t* header, const float* statF, const uint32
 
 extern "C" {
 
-// Size in bytes of one anchor-table value record for the float and double oracles.
+// Size in bytes of one anchor-table value record and of one warm-start table slot for the float and double oracles.
 size_t oracle_anchor_val_size_f32() { return sizeof(AnchorVal<float>); }
 size_t oracle_anchor_val_size_f64() { return sizeof(AnchorVal<double>); }
+size_t oracle_warm_slot_size_f32() { return sizeof(WarmSlot<float>); }
+size_t oracle_warm_slot_size_f64() { return sizeof(WarmSlot<double>); }
 
 // Runs nsub substeps on one world, in place.  poses/vel/art are float; akey is uint32[2*AH]; aval is
-// opaque storage of 2*AH * oracle_anchor_val_size_f32() bytes (zeroed at reset); clock is uint32[4]: substep
+// opaque storage of 2*AH * oracle_anchor_val_size_f32() bytes (zeroed at reset); warm is opaque storage of
+// 2*W_WH * oracle_warm_slot_size_f32() bytes (zeroed at reset; unused when W_WH is 0); clock is uint32[4]: substep
 // count, control flags (bit 0: user-set targets), frame clock in seconds (double), all zero at reset.
 int oracle_substeps_f32(const uint32_t* header, const float* statF, const uint32_t* statU, float* poses, float* vel, float* art,
-                        uint32_t* akey, void* aval, uint32_t* clock, uint32_t nsub, uint32_t* stats, OracleOut* dbg) {
-  return run_substeps<float>(header, statF, statU, poses, vel, art, akey, (AnchorVal<float>*)aval, clock, nsub, stats, dbg);
+                        uint32_t* akey, void* aval, void* warm, uint32_t* clock, uint32_t nsub, uint32_t* stats, OracleOut* dbg) {
+  return run_substeps<float>(header, statF, statU, poses, vel, art, akey, (AnchorVal<float>*)aval, (WarmSlot<float>*)warm, clock, nsub,
+                             stats, dbg);
 }
 int oracle_substeps_f64(const uint32_t* header, const float* statF, const uint32_t* statU, double* poses, double* vel, double* art,
-                        uint32_t* akey, void* aval, uint32_t* clock, uint32_t nsub, uint32_t* stats, OracleOut* dbg) {
-  return run_substeps<double>(header, statF, statU, poses, vel, art, akey, (AnchorVal<double>*)aval, clock, nsub, stats, dbg);
+                        uint32_t* akey, void* aval, void* warm, uint32_t* clock, uint32_t nsub, uint32_t* stats, OracleOut* dbg) {
+  return run_substeps<double>(header, statF, statU, poses, vel, art, akey, (AnchorVal<double>*)aval, (WarmSlot<double>*)warm, clock,
+                              nsub, stats, dbg);
 }
 // Anchor table of the float oracle in the kernels' uint32 layout (keys [2*AH], then AVN words per slot):
 // lets a test compare the carried anchors entry by entry.
@@ -2982,6 +3123,16 @@ void oracle_anchor_export_f32(uint32_t AH, const uint32_t* akey, const void* ava
     o[5] = f2u(v[i].el.x); o[6] = f2u(v[i].el.y); o[7] = f2u(v[i].el.z);
     o[8] = f2u(v[i].il.x); o[9] = f2u(v[i].il.y); o[10] = f2u(v[i].il.z);
     o[11] = f2u(v[i].nl.x); o[12] = f2u(v[i].nl.y); o[13] = f2u(v[i].nl.z);
+  }
+}
+// Warm-start table of the float oracle in the kernels' uint32 layout (keys [2*WH], then 4 words per slot: normal
+// impulse, tangent impulse as a world vector).
+void oracle_warm_export_f32(uint32_t WH, const void* warm, uint32_t* out) {
+  const WarmSlot<float>* s = (const WarmSlot<float>*)warm;
+  for (uint32_t i = 0; i < 2 * WH; i++) {
+    out[i] = s[i].key;
+    uint32_t* o = out + 2 * WH + i * 4;
+    o[0] = f2u(s[i].v.x); o[1] = f2u(s[i].v.y); o[2] = f2u(s[i].v.z); o[3] = f2u(s[i].v.w);
   }
 }
 // Known-answer hooks for the oracle's own pinning tests.

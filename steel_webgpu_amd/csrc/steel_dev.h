@@ -14,7 +14,7 @@ struct WorldDev {
   float* art;
   uint32_t* akey;          // friction-anchor keys [2*AH]
   uint32_t* aval;          // anchor values [2*AH*AVN]
-  uint32_t* ctr;           // [0] pairs, [1] contacts, [2] substep count (parity), [3] overflow flags
+  uint32_t* ctr;           // [0] pairs, [1] contacts, [2] substep count (parity), [3] control flags, [4-5] frame clock
   uint32_t* stats;         // [16], the reference's stats words
   uint32_t* pairs;         // [2*MAXP]
   float4* craw;            // [6*MAXC] narrowphase contacts, then
@@ -26,16 +26,18 @@ struct WorldDev {
   float4* tw;              // [2*NB] solver twists
   float4* imp;             // [2*NB] accumulated link impulses
   float* aqd;              // [ND] solver joint velocities
+  uint32_t* warm;          // contact warm start: keys [2*WH] (two parities), then values [2*WH*4] (float bits)
 };
 
 // Offsets of the schedule tables inside WorldDev::sched (uint32 words), from the world's capacities.
 struct SchedLayout {
   uint32_t HKEY, HCNT, CSLOT, CLOCAL, UKEY, USIZE, USTART, CSTART, META, CORDER, RK0, RK1, RK2, KEPT, OSTART, LEAD, SLOT, ROWOF,
-      RUNIT, PCOL, PORD, UTC, UTP, RCON, UWOFF, WALK, HEADS, DSLOT, WCOL, WMASK, WCOUNT, LINKHIT, JSPLIT, SLOTT, NPAIRL, UF, UFF, TOTAL;
+      RUNIT, PCOL, PORD, UTC, UTP, RCON, UWOFF, WALK, HEADS, DSLOT, WCOL, WMASK, WCOUNT, LINKHIT, JSPLIT, SLOTT, NPAIRL, UF, UFF, WWIN,
+      WSLOT, TOTAL;
 };
 
 __host__ __device__ inline SchedLayout sched_layout(uint32_t MAXC, uint32_t MAXU, uint32_t UH, uint32_t NCOL, uint32_t NB, uint32_t NCOLW,
-                                                    uint32_t NL) {
+                                                    uint32_t NL, uint32_t WH, uint32_t MAXDENSE) {
   SchedLayout s;
   uint32_t o = 0;
   s.HKEY = o; o += UH;
@@ -75,6 +77,9 @@ __host__ __device__ inline SchedLayout sched_layout(uint32_t MAXC, uint32_t MAXU
   s.NPAIRL = o; o += 4;
   s.UF = o; o += (NB > 0 ? NB : 1) + (NL > 0 ? NL : 1);   // sleep islands: union-find parent per node (NB + NA <= NB + NL)
   s.UFF = o; o += (NB > 0 ? NB : 1) + (NL > 0 ? NL : 1);  // island flags
+  o = (o + 1) & ~1u;
+  s.WWIN = o; o += 2 * WH;                                 // warm table: winning row order per slot (uint64)
+  s.WSLOT = o; o += WH ? MAXC + MAXDENSE : 0;              // warm table: slot of each row
   s.TOTAL = o;
   return s;
 }

@@ -469,7 +469,31 @@ struct Prep {
   DEV uint32_t anchor_hash(uint32_t pair, uint32_t k) const { return max(hash_u32(pair * 2u + k), 1u); }
   DEV float4 row(uint32_t r, uint32_t k) const { return w.d.rows[r * w.ROWS() + k]; }
   DEV void set_row(uint32_t r, uint32_t k, float4 v) const { w.d.rows[r * w.ROWS() + k] = v; }
-  DEVN void build_patch(uint32_t sp) const {  // wgsl.js:1562-1725 (warm start of anchors off: warm scale 0)
+  // Contact warm start (wgsl.js:2175-2190, 2421-2431): the previous substep's impulses stored under `key`, scaled by
+  // the warm scale; the tangent part projected on (t1, t2) within the friction cone mu * ln.
+  DEV void warm_lookup(uint32_t key, float mu, f3 t1, f3 t2, float& ln, f2& lt) const {
+    ln = 0.0f; lt = mk2(0.0f, 0.0f);
+    const uint32_t WH = w.h(W_WH);
+    if (WH == 0) return;
+    const float ws = w.hf(W_WARM_SCALE);
+    f3 lf = mk3(0, 0, 0);
+    uint32_t rbase = parity * WH, h = key % WH;
+    for (uint32_t probe = 0; probe < 32; probe++) {
+      uint32_t k = w.d.warm[rbase + h];
+      if (k == 0) break;
+      if (k == key) {
+        const uint4 v = ((const uint4*)(w.d.warm + 2 * WH))[rbase + h];
+        ln = ubits(v.x) * ws;
+        lf = mk3(ubits(v.y), ubits(v.z), ubits(v.w)) * ws;
+        break;
+      }
+      h = (h + 1) % WH;
+    }
+    lt = mk2(dot(lf, t1), dot(lf, t2));
+    float lim = mu * ln, lm = len(lt);
+    if (lm > lim) lt = lt * (lim / mx(lm, 1e-12f));
+  }
+  DEVN void build_patch(uint32_t sp) const {  // wgsl.js:1562-1725
     const uint32_t* US = w.sch(w.S.USTART);
     uint32_t MAXC = w.MAXC(), PO = w.PO(), AH = w.AH();
     uint32_t st = US[sp];
@@ -555,10 +579,13 @@ struct Prep {
         loc1 += dot(e1 - centroid, across) * across + dot(e1 - centroid, ns) * ns;
       }
     }
+    float load = 0.0f;
     for (uint32_t i = st; i < st + n; i++) {
       if (row(i, PO).w != 0.0f) continue;
+      load += row(i, 6).x;
       if (i != first && !(na == 2 && i == second)) { float4 l6 = row(i, 6); set_row(i, 6, make_float4(l6.x, 0.0f, 0.0f, l6.w)); }
     }
+    const float ws = w.hf(W_WARM_SCALE);
     uint32_t rbase = parity * AH;
     for (uint32_t k = 0; k < na; k++) {
       uint32_t i = k == 1 ? second : first;
@@ -584,7 +611,8 @@ struct Prep {
       float det = k00 * k11 - k01 * k01;
       set_row(i, 5, make_float4(k11 / det, -k01 / det, k00 / det, row(i, 5).w));
       set_row(i, 2, F4(t2, row(i, 2).w / (float)na));
-      f3 e = mk3(0, 0, 0);
+      f3 e = mk3(0, 0, 0), imp = mk3(0, 0, 0);
+      bool carried = false;
       float best = 4.0f * r * r;
       for (uint32_t kk = 0; kk < 2; kk++) {
         uint32_t hk = anchor_hash(pair, kk);
@@ -604,12 +632,20 @@ struct Prep {
               float cs = dot(on, nn);
               if (cs > 0.0f) { f3 ax = cross(on, nn); f3 ce = cross(ax, ew); ew += ce + cross(ax, ce) / (1.0f + cs); }
               ew -= nn * dot(nn, ew);
-              if (dot(ew, ew) <= 0.01f * r * r) { e = ew; best = dot(dl, dl); }
+              if (dot(ew, ew) <= 0.01f * r * r) { e = ew; best = dot(dl, dl); carried = true; imp = qrot(qa, mk3(ubits(av[8]), ubits(av[9]), ubits(av[10]))); }
             }
             break;
           }
           h = (h + 1) % AH;
         }
+      }
+      if (ws > 0.0f) {  // the carried anchor impulse, else the contact's, within mu / na of the members' load
+        float4 l6 = row(i, 6);
+        f2 lw = mk2(l6.y, l6.z);
+        if (carried) lw = mk2(dot(imp, t1), dot(imp, t2)) * ws;
+        float rad = mx(row(i, 2).w * load, 0.0f);
+        if (len(lw) > rad) lw = lw * (rad / len(lw));
+        set_row(i, 6, make_float4(l6.x, lw.x, lw.y, l6.w));
       }
       set_row(i, PO, F4(fra, 2.0f));
       set_row(i, PO + 1, F4(frb, (float)k));
@@ -687,7 +723,10 @@ struct Prep {
       if (du > 0.0f && K[0] != du) wrow = wrow * K[0] / (wrow * K[0] + (1.0f - wrow) * du);
     }
     set_row(idx, 5, make_float4(k11 / det, -k01 / det, k00 / det, bounce ? 1.0f : wrow));
-    set_row(idx, 6, make_float4(0.0f, 0.0f, 0.0f, bounce ? 1e30f : phi));
+    float ln;
+    f2 lt;
+    warm_lookup(key, mu, t1, t2, ln, lt);
+    set_row(idx, 6, make_float4(ln, lt.x, lt.y, bounce ? 1e30f : phi));
   }
   DEV float row_weight(float phi) const { float reg = w.hf(W_REG); if (phi > 0.0f || reg <= 0.0f) return 1.0f; return 1.0f / (1.0f + reg); }
   DEV bool restitution_fires(float e, float phi, float vn) const {
@@ -702,6 +741,35 @@ struct Prep {
   }
   // Dense row of contact c, three lanes (base + r, r = 0..2): lane r solves row r's Y = H^-1 J_r^T, the
   // 3x3 K = J Y^T is gathered by shuffles, lane 0 writes the row header (wgsl.js:2370-2444).
+  // Mimic row i in dense slot `slot`: qd_follower - c1 qd_leader, Baumgarte on q_follower - c1 q_leader - c0;
+  // a bilateral block (J, Y = H^-1 J^T, 1 / (J Y), rhs) as in wgsl.js:1351-1369.
+  DEVN void mimic_row(uint32_t i, uint32_t slot) const {
+    const float4 M0 = w.sf(w.h(W_SF_MIMIC) + i * 2), M1 = w.sf(w.h(W_SF_MIMIC) + i * 2 + 1);
+    const uint32_t art_id = tu(M0.x);
+    const uint4 ai = w.art_info(art_id);
+    const uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR);
+    float* art_ = w.d.art;
+    const uint32_t o = w.h(W_A_DENSE) + slot * w.h(W_DPS);
+    float tau[64], y[64];
+    SVd ps[64], dvs[64];
+    float uu[6 * 64];
+    for (uint32_t k = 0; k < NDA; k++) { tau[k] = 0.0f; y[k] = 0.0f; }
+    if (tu(M0.y) < ai.w) tau[tu(M0.y)] = 1.0f;
+    if (tu(M0.z) < ai.w) tau[tu(M0.z)] -= M0.w;
+    w.art_solve_gen(art_id, tau, y, ps, dvs, uu);
+    float kk = 0.0f;
+    for (uint32_t k = 0; k < ai.w; k++) {
+      art_[o + DHDR + k] = tau[k];
+      art_[o + DHDR + 3 * NDA + k] = y[k];
+      kk += tau[k] * y[k];
+    }
+    art_[o + 5] = kk > 1e-12f ? 1.0f / kk : 0.0f;
+    const float phi = art_[w.h(W_A_Q) + tu(M1.x)] - M0.w * art_[w.h(W_A_Q) + tu(M1.y)] - M1.z;
+    art_[o] = (float)art_id; art_[o + 1] = 0.0f; art_[o + 2] = 0.0f;
+    art_[o + 4] = w.hf(W_BETA) * phi / w.hf(W_DT); art_[o + 22] = 0.0f; art_[o + 23] = 0.0f;
+    art_[o + 9] = 0.0f; art_[o + 10] = 0.0f; art_[o + 11] = 0.0f;
+    art_[o + 21] = 1.0f;
+  }
   DEVN void dense_row(uint32_t c, uint32_t slot, uint32_t r, uint32_t base) const {
     float4 c0 = con(c, 0), c1 = con(c, 1), cmeta = con(c, 2);
     int a = ti(cmeta.x), b = ti(cmeta.y);
@@ -753,10 +821,13 @@ struct Prep {
       bounce = restitution_fires(e, phi, vn0);
       if (bounce) rhs = e * vn0;
     }
+    float ln;
+    f2 lt;
+    warm_lookup(key, mu, t1, t2, ln, lt);
     art_[o] = (float)art_id; art_[o + 1] = (float)(key & 0xffffu); art_[o + 2] = (float)(key >> 16); art_[o + 3] = mu;
     art_[o + 4] = rhs; art_[o + 5] = effn;
     art_[o + 6] = k11 / det; art_[o + 7] = -k01 / det; art_[o + 8] = k00 / det;
-    art_[o + 9] = 0.0f; art_[o + 10] = 0.0f; art_[o + 11] = 0.0f;
+    art_[o + 9] = ln; art_[o + 10] = lt.x; art_[o + 11] = lt.y;
     w.st3(o + 12, t1); w.st3(o + 15, t2); w.st3(o + 18, n);
     art_[o + 21] = 0.0f;
     art_[o + 24] = bounce ? 1.0f : row_weight(phi);
@@ -796,6 +867,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   // ---- Phase 0
   for (uint32_t i = tid; i < UH; i += BS) { sched[S.HKEY + i] = 0; sched[S.HCNT + i] = 0; }
   if (PATCH) for (uint32_t i = tid; i < AH; i += BS) wd.akey[(1 - P.parity) * AH + i] = 0;
+  if (const uint32_t WH = w.h(W_WH)) {  // the table this substep's solve writes, and its winners
+    for (uint32_t i = tid; i < WH; i += BS) wd.warm[(1 - P.parity) * WH + i] = 0;
+    for (uint32_t i = tid; i < 2 * WH; i += BS) sched[S.WWIN + i] = 0;
+  }
   for (uint32_t i = tid; i < max(NB, 1u); i += BS) sched[S.LINKHIT + i] = 0;
   for (uint32_t i = tid; i < max(NB, 1u) * NCOLW; i += BS) sched[S.WMASK + i] = 0;
   for (uint32_t i = tid; i < NCOL + 2; i += BS) sched[S.WCOUNT + i] = 0;
@@ -857,6 +932,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
           if (art_[ASLP + n * 4 + 1] < 0.5f && (len(v.l) > LIN || len(v.a) > ANG)) moving = true;
         }
         for (uint32_t dd = ai.z; dd < ai.z + ai.w; dd++) { float4 dr = w.dof_drive(dd); if (dr.y > 0.0f || dr.z > 0.0f) veto = true; }
+        if (w.su(w.h(W_SU_ACON) + n - NB) != 0) veto = true;  // owns a mimic
       }
       uint32_t f = (fixed ? 1u : 0u) | ((moving || veto) ? 2u : 0u) | (UFF[n] & 1u);
       atomicOr(&UFF[UF[n]], f);
@@ -1259,7 +1335,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       }
       sched[S.PORD + o0 + j + 1] = r;
     }
-    sched[S.META + 3] = ndc;
+    sched[S.META + 3] = ndc + w.h(W_NMIMIC);
   }
   // ---- Phase 7: dense rows of same-articulation contacts (slot = rank in contact order), three lanes per
   // contact, 21 contacts per wave; RK0 (free after phase 5) maps slot -> contact.
@@ -1275,6 +1351,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     const uint32_t lane = tid & 63, sl = s0 + lane / 3, r = lane % 3;
     if (lane < 63 && sl < ndd) P.dense_row(sched[S.RK0 + sl], sl, r, lane - r);
   }
+  // Mimic rows in the slots after the contacts' (wgsl.js:2482-2492), a lane each.
+  for (uint32_t i = tid; i < w.h(W_NMIMIC); i += BS)
+    if (ndc + i < MAXDENSE) P.mimic_row(i, ndc + i);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1339,6 +1418,28 @@ struct Solver {
     return mcom(l) - mcom((uint32_t)par);
   }
   DEV float4 row(uint32_t r, uint32_t k) const { return w.d.rows[r * w.ROWS() + k]; }
+  // Contact warm start (wgsl.js:3614-3631): row i's key (contact rows, then dense rows; 0xffffffff for a bilateral
+  // row, which stores nothing), its value (normal impulse, then the tangent impulse as a world vector) and its order
+  // among the rows sharing a key: the larger normal impulse (negative and -0 count as 0, and non-negative floats
+  // order as their bits), then the lower row.
+  DEV uint32_t warm_key(uint32_t i, uint32_t nc) const {
+    if (i < nc) return (uint32_t)row(i, 3).w | ((uint32_t)row(i, 4).w << 16);
+    const uint32_t o = w.h(W_A_DENSE) + (i - nc) * w.h(W_DPS);
+    if (w.d.art[o + 21] > 0.5f) return 0xffffffffu;
+    return (uint32_t)w.d.art[o + 1] | ((uint32_t)w.d.art[o + 2] << 16);
+  }
+  DEV float4 warm_value(uint32_t i, uint32_t nc) const {
+    if (i < nc) {
+      const float4 lam = row(i, 6);
+      return F4(xyz(row(i, 1)) * lam.y + xyz(row(i, 2)) * lam.z, lam.x);  // (tangent, normal): reordered on store
+    }
+    const uint32_t o = w.h(W_A_DENSE) + (i - nc) * w.h(W_DPS);
+    const float* art_ = w.d.art;
+    return F4(w.ld3(o + 12) * art_[o + 10] + w.ld3(o + 15) * art_[o + 11], art_[o + 9]);
+  }
+  DEV unsigned long long warm_order(float x, uint32_t i) const {
+    return ((unsigned long long)(x > 0.0f ? fbits(x) + 1u : 1u) << 32) | (0xffffffffu - i);
+  }
   DEV void load_tw(int b, f3 out[2]) const {
     if (b < 0) { out[0] = mk3(0, 0, 0); out[1] = mk3(0, 0, 0); return; }
     out[0] = xyz(tw[(uint32_t)b * 2]); out[1] = xyz(tw[(uint32_t)b * 2 + 1]);
@@ -1604,6 +1705,27 @@ struct Solver {
       touched = true;
       const uint32_t jo = o + DHDR, yo = o + DHDR + 3 * NDA;
       const float lx = art_[o + 9], ly = art_[o + 10], lz = art_[o + 11];
+      if (art_[o + 21] > 0.5f) {
+        // Bilateral rows (mimics): unclamped, one at a time; no warm impulse (wgsl.js:3319-3332).
+        if (mode == 0) continue;
+        float nlam[3] = {lx, ly, lz};
+        const uint32_t nb = tu(art_[o + 21]);
+        for (uint32_t r = 0; r < nb && r < 3; r++) {
+          const float jr0 = own0 ? art_[jo + r * NDA + lane] : 0.0f, jr1 = own1 ? art_[jo + r * NDA + lane + 32] : 0.0f;
+          const float yr0 = own0 ? art_[yo + r * NDA + lane] : 0.0f, yr1 = own1 ? art_[yo + r * NDA + lane + 32] : 0.0f;
+          float part = 0.0f;
+          if (own0) part += jr0 * q0;
+          if (own1) part += jr1 * q1;
+          const float jv = lane_sum32(part);
+          const float rhs = r == 0 ? art_[o + 4] : (r == 1 ? art_[o + 22] : art_[o + 23]);
+          const float dl = -omega * (jv + rhs) * art_[o + 5 + r];
+          nlam[r] += dl;
+          if (own0) q0 += yr0 * dl;
+          if (own1) q1 += yr1 * dl;
+        }
+        if (lane == 0) { art_[o + 9] = nlam[0]; art_[o + 10] = nlam[1]; art_[o + 11] = nlam[2]; }
+        continue;
+      }
       float ja0 = 0, jb0 = 0, jc0 = 0, ya0 = 0, yb0 = 0, yc0 = 0, ja1 = 0, jb1 = 0, jc1 = 0, ya1 = 0, yb1 = 0, yc1 = 0;
       if (own0) {
         ja0 = art_[jo + lane]; jb0 = art_[jo + NDA + lane]; jc0 = art_[jo + 2 * NDA + lane];
@@ -1989,6 +2111,33 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   }
   __syncthreads();
   const uint32_t nrows = sched[S.USTART + nu];
+  if (const uint32_t WH = w.h(W_WH)) {
+    // Impulses for the next substep's warm start (wgsl.js:3737-3767): every row's key is inserted by linear probing
+    // (compare-and-swap, at most 32 probes) and each key keeps the value of its largest row order, so the table
+    // repeats run to run.  The winners were cleared with this table in prepare.
+    const uint32_t nc = min(nrows, MAXC), nw = nc + min(sched[S.META + 3], w.h(W_MAXDENSE)), wb = (1 - sv.parity) * WH;
+    unsigned long long* win = (unsigned long long*)(sched + S.WWIN);
+    for (uint32_t i = tid; i < nw; i += BS) {
+      sched[S.WSLOT + i] = 0xffffffffu;
+      const uint32_t key = sv.warm_key(i, nc);
+      if (key == 0xffffffffu) continue;
+      uint32_t h = key % WH;
+      for (uint32_t probe = 0; probe < 32; probe++) {
+        const uint32_t old = atomicCAS(&wd.warm[wb + h], 0u, key);
+        if (old == 0u || old == key) { atomicMax(&win[h], sv.warm_order(sv.warm_value(i, nc).w, i)); sched[S.WSLOT + i] = h; break; }
+        h = (h + 1) % WH;
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nw; i += BS) {
+      const uint32_t h = sched[S.WSLOT + i];  // written by this thread above
+      if (h >= WH) continue;
+      const float4 v = sv.warm_value(i, nc);
+      if (__hip_atomic_load(&win[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != sv.warm_order(v.w, i)) continue;
+      ((uint4*)(wd.warm + 2 * WH))[wb + h] = make_uint4(fbits(v.w), fbits(v.x), fbits(v.y), fbits(v.z));
+    }
+    __syncthreads();
+  }
   const bool PATCH = w.h(W_PATCH) != 0;
   const uint32_t PB = w.h(W_P_BODY), PP = w.h(W_P_PREV);
   if (PATCH) {

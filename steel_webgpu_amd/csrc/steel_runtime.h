@@ -22,7 +22,7 @@ int fail(int code, const char* fmt, ...);
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 enum Kind { K_HDR, K_STATF, K_STATU, K_POSES, K_VEL, K_ART, K_AKEY, K_AVAL, K_CTR, K_STATS, K_PAIRS, K_CRAW, K_CONTACTS, K_ROWS, K_SCHED,
-            K_SKEY, K_SVAL, K_TW, K_IMP, K_AQD, K_POSES0, K_VEL0, K_ART0, K_COUNT };
+            K_SKEY, K_SVAL, K_TW, K_IMP, K_AQD, K_POSES0, K_VEL0, K_ART0, K_WARM, K_COUNT };
 
 struct Sizes { size_t b[K_COUNT]; };
 
@@ -32,7 +32,7 @@ inline int world_sizes(const uint32_t* h, Sizes& s) {
   if (h[W_NDA] > 64 || h[W_MAXLP] > 64) return fail(STEEL_BAD_ARGUMENT, "articulations of more than 64 dofs or links are not supported");
   if (h[W_MAXPATH] > 64 || h[W_MAXPATH] * h[W_JM] > 192) return fail(STEEL_BAD_ARGUMENT, "articulation root paths deeper than 64 links are not supported");
   if (h[W_UH] > MAXC || h[W_MAXU] > MAXC) return fail(STEEL_BAD_ARGUMENT, "unit capacities exceed the contact capacity");
-  SchedLayout L = sched_layout(MAXC, h[W_MAXU], h[W_UH], h[W_NCOLW] * 32, NB, h[W_NCOLW], h[W_NL]);
+  SchedLayout L = sched_layout(MAXC, h[W_MAXU], h[W_UH], h[W_NCOLW] * 32, NB, h[W_NCOLW], h[W_NL], h[W_WH], h[W_MAXDENSE]);
   s.b[K_HDR] = STEEL_WORLD_HEADER_WORDS * 4;
   s.b[K_STATF] = (size_t)h[W_NSTATF] * 16;
   s.b[K_STATU] = (size_t)std::max(h[W_NSTATU], 1u) * 4;
@@ -56,12 +56,14 @@ inline int world_sizes(const uint32_t* h, Sizes& s) {
   s.b[K_POSES0] = s.b[K_POSES];
   s.b[K_VEL0] = s.b[K_VEL];
   s.b[K_ART0] = s.b[K_ART];
+  s.b[K_WARM] = std::max((size_t)h[W_WH] * 10 * 4, (size_t)4);
   return STEEL_OK;
 }
 
 
 inline int buf_kind(uint32_t buf) {
-  static const int map[STEEL_BUF_COUNT] = {K_POSES, K_VEL, K_ART, K_AKEY, K_AVAL, K_PAIRS, K_CONTACTS, K_ROWS, K_SCHED, K_CTR, K_STATS};
+  static const int map[STEEL_BUF_COUNT] = {K_POSES, K_VEL, K_ART, K_AKEY, K_AVAL, K_PAIRS, K_CONTACTS, K_ROWS, K_SCHED, K_CTR, K_STATS,
+                                           K_WARM};
   return buf < STEEL_BUF_COUNT ? map[buf] : -1;
 }
 
@@ -88,6 +90,7 @@ inline WorldDev world_dev_at(char* const* base) {
   x.tw = (float4*)base[K_TW];
   x.imp = (float4*)base[K_IMP];
   x.aqd = (float*)base[K_AQD];
+  x.warm = (uint32_t*)base[K_WARM];
   return x;
 }
 
@@ -108,7 +111,7 @@ inline int world_solve_lds(const uint32_t* h, const float* statF, LaunchLds& sl)
 }
 
 // Kinds cleared to zero when a world starts or restarts.
-static constexpr int K_ZEROED[] = {K_AKEY, K_AVAL, K_CTR, K_STATS, K_TW, K_IMP, K_AQD, K_ROWS, K_SCHED};
+static constexpr int K_ZEROED[] = {K_AKEY, K_AVAL, K_CTR, K_STATS, K_TW, K_IMP, K_AQD, K_ROWS, K_SCHED, K_WARM};
 
 struct steel_device {
   int ordinal;

@@ -685,6 +685,6 @@ DEV W make_world(const uint32_t* Hs, const WorldDev& wd) {
   W w;
   w.H = Hs;
   w.d = wd;
-  w.S = sched_layout(Hs[W_MAXC], Hs[W_MAXU], Hs[W_UH], Hs[W_NCOLW] * 32, Hs[W_NB], Hs[W_NCOLW], Hs[W_NL]);
+  w.S = sched_layout(Hs[W_MAXC], Hs[W_MAXU], Hs[W_UH], Hs[W_NCOLW] * 32, Hs[W_NB], Hs[W_NCOLW], Hs[W_NL], Hs[W_WH], Hs[W_MAXDENSE]);
   return w;
 }

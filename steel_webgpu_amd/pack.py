@@ -110,16 +110,23 @@ def capacities(NS, NB, NA, policy="reference"):
     return maxp, maxc, maxu, uh, ah, maxdense
 
 
-def sched_layout(MAXC, MAXU, UH, NCOL, NB, NCOLW, NL):
+# Contact warm-start table slots per parity (sim.js:396 WH).
+WARM_SLOTS = 16384
+
+
+def sched_layout(MAXC, MAXU, UH, NCOL, NB, NCOLW, NL, WH=0, MAXDENSE=0):
     """Word offsets of the kernels' schedule tables (mirror of sched_layout in csrc/steel_dev.h)."""
     names = [("HKEY", UH), ("HCNT", UH), ("CSLOT", MAXC), ("CLOCAL", MAXC), ("UKEY", MAXU), ("USIZE", MAXU), ("USTART", MAXU + 1),
              ("CSTART", NCOL + 2), ("META", 8), ("CORDER", MAXC), ("RK0", MAXC), ("RK1", MAXC), ("RK2", MAXC), ("KEPT", MAXC),
              ("OSTART", MAXU), ("LEAD", MAXC), ("SLOT", MAXC * 26), ("ROWOF", MAXC), ("RUNIT", MAXC), ("PCOL", MAXC), ("PORD", MAXC),
              ("UTC", MAXU), ("UTP", MAXU), ("RCON", MAXC), ("UWOFF", UH), ("WALK", MAXC), ("HEADS", MAXC + 1), ("DSLOT", MAXC + 1),
              ("WCOL", MAXC), ("WMASK", max(NB, 1) * NCOLW), ("WCOUNT", NCOL + 2), ("LINKHIT", max(NB, 1)), ("JSPLIT", max(NL, 1)),
-             ("SLOTT", MAXC * 4), ("NPAIRL", 4), ("UF", max(NB, 1) + max(NL, 1)), ("UFF", max(NB, 1) + max(NL, 1))]
+             ("SLOTT", MAXC * 4), ("NPAIRL", 4), ("UF", max(NB, 1) + max(NL, 1)), ("UFF", max(NB, 1) + max(NL, 1)),
+             ("WWIN", 2 * WH), ("WSLOT", (MAXC + MAXDENSE) if WH else 0)]
     out, o = {}, 0
     for k, n in names:
+        if k == "WWIN":
+            o = (o + 1) & ~1  # uint64 entries
         out[k] = o
         o += n
     out["TOTAL"] = o
@@ -127,7 +134,8 @@ def sched_layout(MAXC, MAXU, UH, NCOL, NB, NCOLW, NL):
 
 
 def world_sched_layout(pw):
-    return sched_layout(pw.h("W_MAXC"), pw.h("W_MAXU"), pw.h("W_UH"), pw.h("W_NCOLW") * 32, pw.NB, pw.h("W_NCOLW"), pw.NL)
+    return sched_layout(pw.h("W_MAXC"), pw.h("W_MAXU"), pw.h("W_UH"), pw.h("W_NCOLW") * 32, pw.NB, pw.h("W_NCOLW"), pw.NL,
+                        pw.h("W_WH"), pw.h("W_MAXDENSE"))
 
 
 def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=None):
@@ -219,8 +227,23 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         raise UnsupportedError(f"{NA} articulations; one solve workgroup handles at most 256")
     if loops:
         raise UnsupportedError("loop closures are not supported yet")
-    if any(m >= 0 for m in (J.get("mimic_joint") or [])) or any(m.get("enabled") for m in (S.get("mimics") or [])):
-        raise UnsupportedError("mimic joints are not supported yet")
+    # Mimic joints (sim.js:160-172): follower q = c0 + c1 * leader q, per dof, as dense bilateral rows.
+    mimics = []
+
+    def add_mimic(f, r, c0, c1):
+        for k in range(J["qd_start"][f + 1] - J["qd_start"][f]):
+            fd, ld = dofMap.get(J["qd_start"][f] + k), dofMap.get(J["qd_start"][r] + k)
+            a = bodyArt[J["child"][f]]
+            if fd is None or ld is None or a != bodyArt[J["child"][r]]:
+                raise ValueError("mimic joints must share an articulation")
+            d0 = arts[a]["dofStart"]
+            mimics.append(dict(art=a, f=fd - d0, l=ld - d0, c0=c0, c1=c1, fq=qMap[J["q_start"][f] + k], lq=qMap[J["q_start"][r] + k]))
+    for f, r in enumerate(J.get("mimic_joint") or []):
+        if r >= 0:
+            add_mimic(f, r, J["mimic_coeffs"][f][0], J["mimic_coeffs"][f][1])
+    for m in S.get("mimics") or []:
+        if m.get("enabled"):
+            add_mimic(m["follower"], m["leader"], m["c0"], m["c1"])
     if S.get("control"):
         raise UnsupportedError("scene controllers are not supported yet")
     if any(kin):
@@ -276,6 +299,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     SF["SF_DOF"] = off; off += max(ND, 1) * 5
     pathFloats = (S.get("target_frames") or 0) * (S.get("driven") or 0)
     SF["SF_PATH"] = off; off += (pathFloats + 3) // 4
+    SF["SF_MIMIC"] = off; off += max(len(mimics), 1) * 2
     nstatf = off
     statF = np.zeros(nstatf * 4, np.float32)
     SU = {}
@@ -286,6 +310,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     SU["SU_HEDGE"] = uoff; uoff += NH * 2
     SU["SU_EDGE"] = uoff; uoff += nEdges * 4
     SU["SU_ART"] = uoff; uoff += max(NA, 1) * 4
+    SU["SU_ACON"] = uoff; uoff += max(NA, 1)
     SU["SU_FILTER"] = uoff; uoff += (NS * NS + 31) // 32
     nstatu = uoff
     statU = np.zeros(max(nstatu, 1), np.uint32)
@@ -470,8 +495,6 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     warm = o.get("warmStart")
     if warm is None:
         warm = 1 if SV.get("contact_warmstart") else 0
-    if warm:
-        raise UnsupportedError("contact warm start is not supported yet")
     fmodes = ["fixed_normal", "bisection", "bisection_desaxce", "coulomb_newton"]
     fm = SV.get("friction_mode") or "fixed_normal"
     if fm not in fmodes:
@@ -583,6 +606,11 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_INTEGRAL_COUNT", S.get("integral_count") or 0)
     H("W_TGT_HELD", 1 if S.get("targets_held") else 0)
     H("W_SLEEP", 1 if (SV.get("enable_sleeping") or SO.get("enable_sleeping")) else 0)
+    H("W_NMIMIC", len(mimics))
+    H("W_WH", WARM_SLOTS if warm else 0)
+    for i, m in enumerate(mimics):
+        statF[(SF["SF_MIMIC"] + i * 2) * 4:(SF["SF_MIMIC"] + i * 2) * 4 + 8] = [m["art"], m["f"], m["l"], m["c1"], m["fq"], m["lq"], m["c0"], 0]
+        statU[SU["SU_ACON"] + m["art"]] = 1  # articulations owning a mimic never sleep (sim.js:369-370)
     gate = SV.get("contact_gap_gate")
     fgap = SV.get("contact_friction_gap_max")
     jlg = SV.get("joint_limit_activation_gap")

@@ -113,6 +113,8 @@ def buffer_bytes(pw, name):
         return 64
     if name == "counters":
         return 32
+    if name == "warm":
+        return pw.h("W_WH") * 40
     raise ValueError(f"buffer {name}: use an explicit byte count")
 
 

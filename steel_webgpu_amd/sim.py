@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("STEEL_AMD_LIB") or os.path.join(_HERE, "libsteel_amd.so")
 
 STEEL_OK, STEEL_ERROR, STEEL_UNSUPPORTED, STEEL_BAD_ARGUMENT, STEEL_OUT_OF_MEMORY = 0, -1, -2, -3, -4
-BUF = dict(poses=0, vel=1, art=2, anchor_keys=3, anchor_vals=4, pairs=5, contacts=6, rows=7, sched=8, counters=9, stats=10)
+BUF = dict(poses=0, vel=1, art=2, anchor_keys=3, anchor_vals=4, pairs=5, contacts=6, rows=7, sched=8, counters=9, stats=10, warm=11)
 STAGE_COLLIDE, STAGE_NARROW, STAGE_PREPARE, STAGE_SOLVE, STAGE_ALL = 1, 2, 4, 8, 15
 # Symbols include/steel_amd.h declares.
 EXPORTS = ["steel_last_error", "steel_device_open", "steel_device_close", "steel_device_info", "steel_batch_create",

@@ -9,7 +9,7 @@ ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
 
 
 class Dbg(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in ("pairs", "npairs", "contacts", "ncontacts", "rowinfo", "nrows", "meta")]
+    _fields_ = [(n, ctypes.c_void_p) for n in ("pairs", "npairs", "contacts", "ncontacts", "rowinfo", "nrows", "meta", "rows")]
 
 
 _lib = None
@@ -24,6 +24,8 @@ def lib():
         L = ctypes.CDLL(ORACLE)
         L.oracle_anchor_val_size_f32.restype = ctypes.c_size_t
         L.oracle_anchor_val_size_f64.restype = ctypes.c_size_t
+        L.oracle_warm_slot_size_f32.restype = ctypes.c_size_t
+        L.oracle_warm_slot_size_f64.restype = ctypes.c_size_t
         L.oracle_hash_u32.restype = ctypes.c_uint32
         L.oracle_hash_u32.argtypes = [ctypes.c_uint32]
         for f in ("oracle_sinf", "oracle_cosf"):
@@ -57,20 +59,22 @@ class OracleWorld:
         self.akey = np.zeros(2 * AH, np.uint32)
         vs = L.oracle_anchor_val_size_f64() if f64 else L.oracle_anchor_val_size_f32()
         self.aval = np.zeros(2 * AH * vs, np.uint8)
+        ws = L.oracle_warm_slot_size_f64() if f64 else L.oracle_warm_slot_size_f32()
+        self.warm = np.zeros(max(2 * pw.h("W_WH"), 1) * ws, np.uint8)
         self.count = np.zeros(4, np.uint32)  # substep count, control flags, frame clock (double)
         self.stats = np.zeros(16, np.uint32)
         self.statU = pw.statU if pw.statU.size else np.zeros(1, np.uint32)
         MAXC, MAXP = pw.h("W_MAXC"), pw.h("W_MAXP")
         self.dbg_bufs = dict(pairs=np.zeros(2 * MAXP, np.uint32), npairs=np.zeros(1, np.uint32), contacts=np.zeros(MAXC * 24, np.float32),
                              ncontacts=np.zeros(1, np.uint32), rowinfo=np.zeros(MAXC * 4, np.uint32), nrows=np.zeros(1, np.uint32),
-                             meta=np.zeros(8, np.uint32))
+                             meta=np.zeros(8, np.uint32), rows=np.zeros(MAXC * pw.h("W_ROWS") * 4, np.float32))
         self.dbg = Dbg(**{k: P(v).value for k, v in self.dbg_bufs.items()})
 
     def substeps(self, n, debug=False):
         L = lib()
         fn = L.oracle_substeps_f64 if self.f64 else L.oracle_substeps_f32
         rc = fn(P(self.pw.header), P(self.pw.statF), P(self.statU), P(self.poses), P(self.vel), P(self.art), P(self.akey), P(self.aval),
-                P(self.count), ctypes.c_uint32(n), P(self.stats), ctypes.byref(self.dbg) if debug else None)
+                P(self.warm), P(self.count), ctypes.c_uint32(n), P(self.stats), ctypes.byref(self.dbg) if debug else None)
         assert rc == 0
 
     def frames(self, n):
@@ -82,6 +86,30 @@ class OracleWorld:
         out = np.zeros(2 * AH + 2 * AH * 14, np.uint32)
         lib().oracle_anchor_export_f32(ctypes.c_uint32(AH), P(self.akey), P(self.aval), P(out))
         return out
+
+    def warm_u32(self):
+        """The float oracle's contact warm-start table in the kernels' layout (keys, then 4 words per slot)."""
+        assert not self.f64
+        WH = self.pw.h("W_WH")
+        out = np.zeros(10 * WH, np.uint32)
+        lib().oracle_warm_export_f32(ctypes.c_uint32(WH), P(self.warm), P(out))
+        return out
+
+    def warm_map(self, parity=None):
+        """{key: (normal, tangent world vector)} of one parity's table (default: the one the next substep reads)."""
+        WH = self.pw.h("W_WH")
+        if parity is None:
+            parity = int(self.count[0]) % 2
+        if self.f64:
+            raise NotImplementedError
+        return warm_table_map(self.warm_u32(), WH, parity)
+
+
+def warm_table_map(buf, WH, parity):
+    """{key: float32[4]} of one parity of a warm-start table in the kernels' uint32 layout."""
+    keys = buf[parity * WH:(parity + 1) * WH]
+    vals = buf[2 * WH:].reshape(2 * WH, 4)[parity * WH:(parity + 1) * WH].view(np.float32)
+    return {int(k): vals[i].copy() for i, k in enumerate(keys) if k}
 
 
 def mpr(pw, ha, hb, rq, t, r):

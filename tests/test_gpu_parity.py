@@ -6,7 +6,7 @@ which is stronger than the 1e-5 relative one-step bound (also checked, against t
 import numpy as np
 import pytest
 
-from oracle_bind import OracleWorld
+from oracle_bind import OracleWorld, warm_table_map
 from steel_webgpu_amd import SceneBuilder, pack, workloads
 
 pytestmark = pytest.mark.gpu
@@ -42,6 +42,22 @@ def _with_body_limits(scene):
 
 def _with_sleeping(scene, quiet=0.1):
     scene.json["solver"].update(enable_sleeping=True, sleep_quiet_time=quiet)
+    return scene
+
+
+def _with_warm(scene):
+    """Contact warm start (the reference's contact_warmstart solver option, scale 1)."""
+    scene.json["solver"].update(contact_warmstart=True)
+    return scene
+
+
+def _with_mimics(scene):
+    """Each leg's third joint mimics its second (q = 0.1 - 1.2 q_leader)."""
+    J = scene.json["joints"]
+    for leg in range(4):
+        f, r = 1 + leg * 3 + 2, 1 + leg * 3 + 1
+        J["mimic_joint"][f] = r
+        J["mimic_coeffs"][f] = [0.1, -1.2]
     return scene
 
 
@@ -96,6 +112,9 @@ SCENES = {
     "bighulls": big_hull_pile,
     "boxes_sleep": lambda: _with_sleeping(workloads.boxes64()),
     "quad_sleep": lambda: _with_sleeping(workloads.quadruped_world(9)),
+    "quad_mimic": lambda: _with_mimics(workloads.quadruped_world(10)),
+    "boxes_warm": lambda: _with_warm(workloads.boxes64()),
+    "quad_warm": lambda: _with_warm(workloads.quadruped_world(12)),
     # a recorded 60 Hz gait driving the legs (base dofs of the path are 0; their drives are off)
     "quad_path": lambda: workloads.with_target_path(workloads.quadruped_world(8, drive=dict(targetKe=40.0, targetKd=1.0)),
                                                     workloads.gait_path(), integral=8),
@@ -136,7 +155,8 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
 
 @pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40), ("quad_vlim", 200),
                                          ("boxes_limits", 150), ("quad_friction", 200), ("bighulls", 120),
-                                         ("quad_path", 200), ("boxes_sleep", 400), ("quad_sleep", 300)])
+                                         ("quad_path", 200), ("boxes_sleep", 400), ("quad_sleep", 300),
+                                         ("quad_mimic", 200), ("boxes_warm", 150), ("quad_warm", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])
@@ -150,6 +170,15 @@ def test_trajectory_bitexact(sim, name, frames):
     assert np.array_equal(gs[:9], o.stats[:9])
     if name.endswith("sleep"):
         assert o.stats[8] > 0  # some islands are asleep
+    if name.endswith("warm"):
+        # both parities of the warm-start table hold the same key -> impulse entries (slots may differ)
+        WH = pw.h("W_WH")
+        gw, ow = b.read(0, "warm", dtype=np.uint32), o.warm_u32()
+        for par in (0, 1):
+            gm, om = warm_table_map(gw, WH, par), warm_table_map(ow, WH, par)
+            assert gm.keys() == om.keys() and len(om) > 0
+            assert all(np.array_equal(_bits(gm[k]), _bits(om[k])) for k in om)
+        assert o.stats[7] > 0 or name.startswith("boxes")  # dense rows stored and read back
     b.destroy()
 
 
@@ -262,7 +291,8 @@ def test_joint_targets_set_mid_run_bitexact(sim):
 
 
 def test_one_step_error_against_fp64_within_1e5(sim):
-    for name in ("boxes64", "quad0", "quad5", "quad_driven", "quad_vlim", "boxes_limits", "quad_friction", "bighulls", "quad_path"):
+    for name in ("boxes64", "quad0", "quad5", "quad_driven", "quad_vlim", "boxes_limits", "quad_friction", "bighulls", "quad_path",
+                 "quad_mimic"):
         pw = pack.pack_scene(SCENES[name]())
         b = sim.Batch([pw])
         o64 = OracleWorld(pw, f64=True)

@@ -47,15 +47,21 @@ def test_header_fields_match_c_enum():
 
 def test_sched_layout_mirror_matches_device_header():
     src = open(os.path.join(ROOT, "steel_webgpu_amd", "csrc", "steel_dev.h")).read()
-    steps = re.findall(r"s\.(\w+) = o; o \+= ([^;]+);", src)
-    L = pack.sched_layout(1000, 300, 256, 64, 40, 2, 12)
-    env = dict(MAXC=1000, MAXU=300, UH=256, NCOL=64, NB=40, NCOLW=2, NL=12)
-    o = 0
-    for name, expr in steps:
-        assert L[name] == o, name
-        o += eval(expr.replace("(NB > 0 ? NB : 1)", "max(NB,1)").replace("NB > 0 ? NB : 1", "max(NB,1)")
-                  .replace("NL > 0 ? NL : 1", "max(NL,1)"), {"max": max}, env)
-    assert L["TOTAL"] == o
+    steps = re.findall(r"s\.(\w+) = o; o \+= ([^;]+);|(o = \(o \+ 1\) & ~1u;)", src)
+    assert any(al for _, _, al in steps)
+    for WH, MAXDENSE in ((0, 7), (16384, 7)):
+        L = pack.sched_layout(1001, 300, 256, 64, 40, 2, 12, WH, MAXDENSE)
+        env = dict(MAXC=1001, MAXU=300, UH=256, NCOL=64, NB=40, NCOLW=2, NL=12, WH=WH, MAXDENSE=MAXDENSE)
+        o = 0
+        for name, expr, align in steps:
+            if align:
+                o = (o + 1) & ~1
+                continue
+            assert L[name] == o, name
+            o += eval(expr.replace("(NB > 0 ? NB : 1)", "max(NB,1)").replace("NB > 0 ? NB : 1", "max(NB,1)")
+                      .replace("NL > 0 ? NL : 1", "max(NL,1)").replace("WH ? MAXC + MAXDENSE : 0", "(MAXC + MAXDENSE if WH else 0)"),
+                      {"max": max}, env)
+        assert L["TOTAL"] == o and L["WWIN"] % 2 == 0
 
 
 def test_filter_bitset_holds_joint_parent_child_and_explicit_filters():

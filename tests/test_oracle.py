@@ -908,6 +908,41 @@ def test_driven_articulation_never_sleeps():
     assert o.art[S + node * 4 + 1] == 0 and o.art[S + node * 4] == 0
 
 
+def test_mimic_joint_holds_its_linear_relation_while_the_leader_moves():
+    """A follower hinge mimics its leader (q_f = c0 + c1 q_l, here c0 = 0.1, c1 = -0.5) through a dense bilateral
+    row with Baumgarte feedback: starting off the relation it converges onto it, and it stays on it (and the
+    velocities with it, qd_f = c1 qd_l) while a joint torque swings the leader through a third of a radian."""
+    b = SceneBuilder(fps=240)
+    l1 = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1])
+    b.addShapeCapsule(l1, radius=0.03, halfHeight=0.1, xform=[0.1, 0, 0, 0, 0.7071068, 0, 0.7071068])
+    j1 = b.addJointRevolute(-1, l1, axis=[0, 0, 1], parentXform=[0, 0, 1, 0, 0, 0, 1], q=0.0)
+    l2 = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1])
+    b.addShapeCapsule(l2, radius=0.03, halfHeight=0.1, xform=[0.1, 0, 0, 0, 0.7071068, 0, 0.7071068])
+    j2 = b.addJointRevolute(l1, l2, axis=[0, 0, 1], parentXform=[0.2, 0, 0, 0, 0, 0, 1], q=0.3)
+    scene = b.finalize(solver=dict(substeps=4))
+    J = scene.json["joints"]
+    J["mimic_joint"][j2] = j1
+    J["mimic_coeffs"][j2] = [0.1, -0.5]
+    B = scene.json["bodies"]
+    B["angular_damping"] = [0.0, 0.0]
+    pw = pack_scene(scene)
+    assert pw.h("W_NMIMIC") == 1
+    A = pw.A
+    o = OracleWorld(pw)
+    err = lambda: o.art[A["A_Q"] + 1] - (0.1 - 0.5 * o.art[A["A_Q"]])
+    assert abs(err() - 0.2) < 1e-6
+    o.frames(120)
+    assert abs(err()) < 1e-4
+    o.art[A["A_JF"]] = 0.05   # a torque on the leader
+    worst, qd_worst = 0.0, 0.0
+    for f in range(240):
+        o.frames(1)
+        worst = max(worst, abs(err()))
+        qd_worst = max(qd_worst, abs(o.art[A["A_QD"] + 1] + 0.5 * o.art[A["A_QD"]]))
+    assert abs(o.art[A["A_Q"]]) > 0.3
+    assert worst < 2e-3 and qd_worst < 2e-2
+
+
 def test_drive_and_spring_settle_at_their_stiffness_weighted_equilibrium():
     """Whole steps (solve + integration): a damped hinge under a drive and a spring comes to rest where their
     torques cancel, q* = (K q_tgt + K_s q_ref) / (K + K_s)."""
@@ -933,3 +968,155 @@ def test_fp32_oracle_tracks_fp64_for_one_step():
         NB = pw.NB
         p32, p64 = o32.poses[:NB * 8].astype(np.float64), o64.poses[:NB * 8]
         assert np.abs(p32 - p64).max() <= 1e-5 * max(1.0, np.abs(p64).max())
+
+
+def _resting_sphere(warm, relaxation=0.5, r=0.1, z0=0.099):
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane(gap=0.01)
+    s = b.addBody(xform=[0, 0, z0, 0, 0, 0, 1])
+    b.addShapeSphere(s, radius=r, gap=0.01)
+    scene = b.finalize(solver=dict(substeps=4, iterations=1, relaxation=relaxation))
+    return pack_scene(scene, opts=dict(warmStart=warm)), float(np.float32(r))
+
+
+def test_warm_start_sphere_follows_the_relaxed_single_row_recursion():
+    """A sphere resting on the ground, one contact row, one relaxed iteration (omega 0.5): each substep starts the
+    row at s times the previous substep's final normal impulse (s = 0, 0.5, 1), applies it in the warm pass, then
+    relaxes once: v2 = v + g dt + s l / m; l' = max(s l + omega (-(v2 + rhs) m_eff w - (1 - w) s l), 0);
+    v' = v2 + (l' - s l) / m; z' = z + v' dt.  The FP64 oracle must follow this recursion substep by substep,
+    and the three scales must give different trajectories (so the warm table is read, scaled and applied)."""
+    trajs = {}
+    for s in (0.0, 0.5, 1.0):
+        pw, r = _resting_sphere(s)
+        o = OracleWorld(pw, f64=True)
+        SF = pw.h("W_SF_BODY")
+        statF = pw.statF.reshape(-1, 4).astype(np.float64)
+        im = statF[SF][0]
+        cap = statF[SF + 6][3]
+        cap = cap if cap > 0 else 1e20
+        dt, gz, om = pw.hf("W_DT"), pw.hf("W_GZ"), pw.hf("W_F_OMEGA")
+        beta, spec, reg, cfm = pw.hf("W_BETA"), pw.hf("W_F_SPEC"), pw.hf("W_REG"), pw.hf("W_F_CFM")
+        assert om == 0.5 and (pw.h("W_WH") > 0) == (s > 0)
+        eff = 1.0 / (im + cfm)
+        z, v, lam = float(o.poses[pw.h("W_P_BODY") * 4 + 2]), 0.0, 0.0
+        zs = []
+        for k in range(240):
+            o.substeps(1, debug=True)
+            assert o.dbg_bufs["ncontacts"][0] == 1
+            phi = z - r
+            rhs = max(beta * phi / dt, -cap) if phi < 0 else spec * phi / dt
+            w = 1.0 if (phi > 0 or reg <= 0) else 1.0 / (1.0 + reg)
+            ln = s * lam
+            v2 = v + gz * dt + ln * im
+            nl = max(ln + om * (-(v2 + rhs) * eff * w - (1.0 - w) * ln), 0.0)
+            v = v2 + (nl - ln) * im
+            lam = nl
+            z = z + v * dt
+            P = pw.h("W_P_BODY") * 4
+            assert abs(o.poses[P + 2] - z) <= 1e-12, (s, k)
+            assert abs(o.vel[2] - v) <= 1e-10, (s, k)
+            zs.append(z)
+        trajs[s] = np.array(zs)
+    assert np.abs(trajs[0.5] - trajs[0.0]).max() > 1e-6 and np.abs(trajs[1.0] - trajs[0.5]).max() > 1e-6
+
+
+def _incline(theta, warm=1, mu=1.0, anchors=True):
+    """A box resting on a static slab tilted by theta about x (static friction holds it)."""
+    b = SceneBuilder(fps=240)
+    c, s_ = math.cos(theta / 2), math.sin(theta / 2)
+    q = [s_, 0, 0, c]
+    b.addShapeBox(-1, hx=2, hy=2, hz=0.1, gap=0.01, mu=mu, xform=[0, 0, 0, *q])
+    n = np.array([0, -math.sin(theta), math.cos(theta)])
+    p = n * (0.1 + 0.1 - 0.0005)
+    box = b.addBody(xform=[*p, *q])
+    b.addShapeBox(box, hx=0.1, hy=0.1, hz=0.1, gap=0.01, mu=mu)
+    scene = b.finalize(solver=dict(substeps=4, iterations=8, **({} if anchors else dict(friction_anchor_beta=None))))
+    return pack_scene(scene, opts=dict(warmStart=warm)), n
+
+
+def test_warm_table_holds_the_equilibrium_impulses_of_a_box_on_an_incline():
+    """Box held by static friction on a 20 degree slope: once at rest, the warm table's entries (one per kept
+    contact row) sum, as impulses n * normal + tangent vector on the box, to the weight impulse m g dt straight
+    up; the normal parts to m g cos(theta) dt and the tangent vectors to m g sin(theta) dt up the slope
+    (a swapped or mis-signed tangent basis fails this)."""
+    theta = math.radians(20)
+    pw, n = _incline(theta)
+    o = OracleWorld(pw)
+    o.substeps(4 * 240)
+    SF = pw.h("W_SF_BODY")
+    im = float(pw.statF.reshape(-1, 4)[SF][0])
+    m, dt, g = 1.0 / im, pw.hf("W_DT"), -pw.hf("W_GZ")
+    assert np.abs(o.vel[:8]).max() < 1e-3
+    tab = o.warm_map()
+    assert 1 <= len(tab) <= 8
+    vals = np.array(list(tab.values()), np.float64)
+    # the slab is static geometry (a = -1) and the normal points from it to the box
+    J = vals[:, 0].sum() * n + vals[:, 1:].sum(0)
+    W = m * g * dt
+    assert np.allclose(J, [0, 0, W], atol=0.02 * W), (J, W)
+    assert abs(vals[:, 0].sum() - W * math.cos(theta)) < 0.02 * W
+    up = np.array([0, math.cos(theta), math.sin(theta)])
+    assert abs(vals[:, 1:].sum(0) @ up - W * math.sin(theta)) < 0.02 * W
+    assert np.abs(vals[:, 1:] @ n).max() < 1e-6 * W + 1e-9
+
+
+def _rows_after_prepare(o):
+    pw = o.pw
+    ROWS = pw.h("W_ROWS")
+    nr = int(o.dbg_bufs["nrows"][0])
+    return o.dbg_bufs["rows"][:nr * ROWS * 4].reshape(nr, ROWS, 4).astype(np.float64)
+
+
+def test_warm_start_rows_begin_at_the_scaled_table_values_within_the_cone():
+    """After prepare, every contact row starts at the table entry of its key from the previous substep: normal
+    s * l and tangent = the scaled stored vector projected on (t1, t2), clamped to mu * normal (checked without
+    friction anchors).  With anchors, non-anchor members start at no friction and anchor rows at the carried
+    anchor impulse (the anchor table's, rotated by the static side's identity) projected and scaled, clamped to
+    mu / na of the members' load.  Incline box, s = 0.5."""
+    theta = math.radians(20)
+    for anchors in (False, True):
+        pw, n = _incline(theta, warm=0.5, anchors=anchors)
+        assert pw.h("W_PATCH") == int(anchors)
+        o = OracleWorld(pw)
+        o.substeps(4 * 60)
+        s = pw.hf("W_WARM_SCALE")
+        assert s == 0.5
+        tab = o.warm_map()
+        AH, par = pw.h("W_AH"), int(o.count[0]) % 2
+        at = o.anchors_u32()
+        akeys, avals = at[par * AH:(par + 1) * AH], at[2 * AH:].reshape(2 * AH, 14)[par * AH:(par + 1) * AH]
+        o.substeps(1, debug=True)
+        R = _rows_after_prepare(o)
+        nr, PO = len(R), pw.h("W_PO")
+        assert nr >= 2
+        load = sum(R[i, 6, 0] for i in range(nr) if R[i, PO, 3] != 1.0) if anchors else 0.0
+        carried = 0
+        for i in range(nr):
+            key = int(R[i, 3, 3]) | (int(R[i, 4, 3]) << 16)
+            v = tab[key].astype(np.float64)
+            t1, t2, mu = R[i, 1, :3], R[i, 2, :3], R[i, 2, 3]
+            assert abs(R[i, 6, 0] - s * v[0]) <= 1e-6 * abs(v[0])
+            if not anchors:
+                lt = s * np.array([v[1:] @ t1, v[1:] @ t2])
+                lim = mu * s * v[0]
+                if np.hypot(*lt) > lim:
+                    lt *= lim / np.hypot(*lt)
+                assert np.allclose(R[i, 6, 1:3], lt, rtol=1e-5, atol=1e-7 * abs(v[0])), (i, R[i, 6], lt)
+                continue
+            role = R[i, PO, 3]
+            if role == 0.0:
+                assert R[i, 6, 1] == 0 and R[i, 6, 2] == 0
+            elif role == 2.0:
+                pair = int(np.float32(R[i, PO + 3, 3]).view(np.uint32))
+                k = int(R[i, PO + 1, 3])
+                hit = [j for j in range(AH) if akeys[j] and avals[j, 0] == pair and avals[j, 1] == k]
+                if not hit:
+                    continue
+                il = avals[hit[0], 8:11].view(np.float32).astype(np.float64)
+                lw = s * np.array([il @ t1, il @ t2])
+                rad = max(mu * load, 0.0)
+                if np.hypot(*lw) > rad:
+                    lw *= rad / np.hypot(*lw)
+                assert np.allclose(R[i, 6, 1:3], lw, rtol=1e-5, atol=1e-7 * load), (i, R[i, 6], lw)
+                carried += 1
+        assert carried >= 1 or not anchors
