@@ -59,6 +59,8 @@ enum steel_world_field {
   W_NMIMIC, W_SF_MIMIC, W_SU_ACON,
   /* contact warm start: slots of the warm table per parity (u32; 0 when warm start is off) */
   W_WH,
+  /* Nesterov momentum on the contact rows (u32 flag) and its row-record offset (u32) */
+  W_NEST, W_MO,
   W_FIELD_COUNT
 };
 

@@ -68,7 +68,8 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t WH;
+  uint32_t WH, NEST, MO;
+  T mom_k = 0, mom_beta = 0;  // Nesterov restart counter and extrapolation factor of the current substep
   uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
   T FPS, FRAME_DT, F_SLEEP_LIN, F_SLEEP_ANG, F_SLEEP_TIME;
   T F_VLIM_FRAC;
@@ -118,7 +119,7 @@ template <class T> struct World {
     A_Q = h[W_A_Q]; A_QD = h[W_A_QD]; A_QDH = h[W_A_QDH]; A_TGT = h[W_A_TGT]; A_TGTV = h[W_A_TGTV]; A_LIM = h[W_A_LIM]; A_DRV = h[W_A_DRV];
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
-    WH = h[W_WH];
+    WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
     TGT_HELD = h[W_TGT_HELD]; A_INT = h[W_A_INT]; FPS = hf(W_FPS); FRAME_DT = hf(W_FRAME_DT);
@@ -2378,6 +2379,66 @@ template <class T> struct World {
   }
 
   // ============================ solve (wgsl.js:2501-3891) ============================
+  // ---- Nesterov momentum on the colored contact rows (wgsl.js:3492-3557): at the start of each position sweep
+  // the impulses move by beta along the previous sweep's change; the represented change e = y - lam reaches the
+  // velocities through the row's own responses.  rows[o + MO] keeps lam at the last extrapolation, rows[o + MO + 1]
+  // keeps e.
+  void momentum_unit(uint32_t row, int a, int b, T beta, bool init) {
+    uint32_t o = row * ROWS;
+    V4<T> lam = rows[o + 6];
+    V3<T> l3 = xyz(lam);
+    if (init) { rows[o + MO] = v4(l3, T(0)); rows[o + MO + 1] = {0, 0, 0, 0}; return; }
+    V3<T> y = l3 + beta * (l3 - xyz(rows[o + MO]));
+    V3<T> e = y - l3;
+    rows[o + MO] = v4(l3, T(0)); rows[o + MO + 1] = v4(e, T(0));
+    if (e.x == T(0) && e.y == T(0) && e.z == T(0)) return;
+    rows[o + 6] = v4(y, lam.w);
+    V3<T> ta[2], tb[2];
+    load_tw(a, ta); load_tw(b, tb);
+    ta[0] += xyz(rows[o + 7]) * e.x + xyz(rows[o + 11]) * e.y + xyz(rows[o + 15]) * e.z;
+    ta[1] += xyz(rows[o + 8]) * e.x + xyz(rows[o + 12]) * e.y + xyz(rows[o + 16]) * e.z;
+    tb[0] += xyz(rows[o + 9]) * e.x + xyz(rows[o + 13]) * e.y + xyz(rows[o + 17]) * e.z;
+    tb[1] += xyz(rows[o + 10]) * e.x + xyz(rows[o + 14]) * e.y + xyz(rows[o + 18]) * e.z;
+    V3<T> ra = xyz(rows[o + 3]), rb = xyz(rows[o + 4]), fa = ra, fb = rb;
+    if (EXTRA) { fa = xyz(rows[o + PO]); fb = xyz(rows[o + PO + 1]); }
+    V3<T> fn = xyz(rows[o]) * e.x, ft = xyz(rows[o + 1]) * e.y + xyz(rows[o + 2]) * e.z;
+    add_imp(a, -fn - ft, -cross(ra, fn) - cross(fa, ft));
+    add_imp(b, fn + ft, cross(rb, fn) + cross(fb, ft));
+    if (a >= 0) { tw[(uint32_t)a * 2] = v4(ta[0], T(0)); tw[(uint32_t)a * 2 + 1] = v4(ta[1], T(0)); }
+    if (b >= 0) { tw[(uint32_t)b * 2] = v4(tb[0], T(0)); tw[(uint32_t)b * 2 + 1] = v4(tb[1], T(0)); }
+  }
+  // One momentum step before position sweep `it`: the O'Donoghue-Candes restart (k = 0) when the last sweep's
+  // correction opposed its extrapolation (sum over rows of (lam - lam_last - e) . e below zero), else k + 1 and
+  // beta = (k - 1) / (k + 2) for k > 1.  The sum runs as 256 strided partials (partial j over rows j, j + 256, ...)
+  // added in order, the reference's workgroup reduction.
+  void momentum_step(uint32_t ncol, uint32_t it, const std::vector<uint32_t>& wcs, const std::vector<uint32_t>& slot) {
+    uint32_t nrows = std::min(wcs[NCOL + 1], MAXC);
+    if (it > 0) {
+      T part[256];
+      for (uint32_t j = 0; j < 256; j++) part[j] = 0;
+      for (uint32_t u = 0; u < nrows; u++) {
+        uint32_t o = S_PORD[u] * ROWS;
+        V3<T> e = xyz(rows[o + MO + 1]);
+        part[u % 256] += dot(xyz(rows[o + 6]) - xyz(rows[o + MO]) - e, e);
+      }
+      T total = 0;
+      for (uint32_t j = 0; j < 256; j++) total += part[j];
+      mom_k = (total < T(0)) ? T(0) : mom_k + T(1);
+      mom_beta = mom_k > T(1) ? (mom_k - T(1)) / (mom_k + T(2)) : T(0);
+    } else {
+      mom_k = 0; mom_beta = 0;
+    }
+    for (uint32_t c = 0; c < NCOL && c < ncol; c++)
+      for (uint32_t u = wcs[c]; u < wcs[c + 1]; u++) momentum_unit(slot[u * 4], (int)slot[u * 4 + 2], (int)slot[u * 4 + 3], mom_beta, it == 0);
+    for (uint32_t u = wcs[NCOL]; u < wcs[NCOL + 1]; u++) {
+      uint32_t uu = std::min(u, MAXC - 1);
+      momentum_unit(slot[uu * 4], (int)slot[uu * 4 + 2], (int)slot[uu * 4 + 3], mom_beta, it == 0);
+    }
+    for (uint32_t a = 0; a < NA; a++) {
+      art_contact_impulses(a);
+      art_velocity_limits(a);
+    }
+  }
   void load_tw(int b, V3<T> out[2]) const {
     if (b < 0) { out[0] = {0, 0, 0}; out[1] = {0, 0, 0}; return; }
     out[0] = xyz(tw[(uint32_t)b * 2]); out[1] = xyz(tw[(uint32_t)b * 2 + 1]);
@@ -2876,7 +2937,10 @@ template <class T> struct World {
     for (uint32_t a = 0; a < NA; a++) art_twists(a);
     dense_phase(0, 0);
     sweep(0, ncol, 0, wcs, slot, nu);
-    for (uint32_t it = 0; it < ITERS; it++) sweep(1, ncol, it, wcs, slot, nu);
+    for (uint32_t it = 0; it < ITERS; it++) {
+      if (NEST) momentum_step(ncol, it, wcs, slot);
+      sweep(1, ncol, it, wcs, slot, nu);
+    }
     uint32_t nc = S_USTART[nu];
     if (WH) warm_store(std::min(nc, MAXC));
     if (PATCH) for (uint32_t b = 0; b < NB * 2; b++) poses[P_PREV + b] = poses[P_BODY + b];

@@ -1396,6 +1396,7 @@ struct Solver {
   float* laqd;    // LDS joint velocities (the reference's aqd)
   float4* tw;     // body twists (LDS, or the world's buffer for worlds too large for LDS)
   float4* imp;    // accumulated impulses on articulated bodies (same placement as tw)
+  float* red;     // LDS partial sums of the momentum restart test (256 floats; Nesterov worlds only)
   DEV int mpar(uint32_t l) const { return (int)lmeta[l].x; }
   DEV uint32_t mnd(uint32_t l) const { return lmeta[l].y & 0xffu; }
   DEV uint32_t mdof(uint32_t l) const { return lmeta[l].z; }
@@ -1443,6 +1444,34 @@ struct Solver {
   DEV void load_tw(int b, f3 out[2]) const {
     if (b < 0) { out[0] = mk3(0, 0, 0); out[1] = mk3(0, 0, 0); return; }
     out[0] = xyz(tw[(uint32_t)b * 2]); out[1] = xyz(tw[(uint32_t)b * 2 + 1]);
+  }
+  // Nesterov momentum on contact row r (wgsl.js:3499-3523): y = lam + beta (lam - lam_last), the change e = y - lam
+  // applied through the row's responses; rows[MO] keeps lam, rows[MO + 1] keeps e; init records lam only.
+  DEVN void momentum_unit(uint32_t r, int a, int b, float beta, bool init) const {
+    float4* R = w.d.rows + (size_t)r * w.ROWS();
+    const uint32_t MO = w.h(W_MO), PO = w.PO();
+    const float4 lam = R[6];
+    const f3 l3 = xyz(lam);
+    if (init) { R[MO] = F4(l3, 0.0f); R[MO + 1] = make_float4(0, 0, 0, 0); return; }
+    const f3 y = l3 + beta * (l3 - xyz(R[MO]));
+    const f3 e = y - l3;
+    R[MO] = F4(l3, 0.0f); R[MO + 1] = F4(e, 0.0f);
+    if (e.x == 0.0f && e.y == 0.0f && e.z == 0.0f) return;
+    R[6] = F4(y, lam.w);
+    f3 ta[2], tb[2];
+    load_tw(a, ta); load_tw(b, tb);
+    ta[0] += xyz(R[7]) * e.x + xyz(R[11]) * e.y + xyz(R[15]) * e.z;
+    ta[1] += xyz(R[8]) * e.x + xyz(R[12]) * e.y + xyz(R[16]) * e.z;
+    tb[0] += xyz(R[9]) * e.x + xyz(R[13]) * e.y + xyz(R[17]) * e.z;
+    tb[1] += xyz(R[10]) * e.x + xyz(R[14]) * e.y + xyz(R[18]) * e.z;
+    const f3 ra = xyz(R[3]), rb = xyz(R[4]);
+    f3 fa = ra, fb = rb;
+    if (w.h(W_EXTRA)) { fa = xyz(R[PO]); fb = xyz(R[PO + 1]); }
+    const f3 fn = xyz(R[0]) * e.x, ft = xyz(R[1]) * e.y + xyz(R[2]) * e.z;
+    add_imp(a, -fn - ft, -cross(ra, fn) - cross(fa, ft));
+    add_imp(b, fn + ft, cross(rb, fn) + cross(fb, ft));
+    if (a >= 0) { tw[(uint32_t)a * 2] = F4(ta[0], 0.0f); tw[(uint32_t)a * 2 + 1] = F4(ta[1], 0.0f); }
+    if (b >= 0) { tw[(uint32_t)b * 2] = F4(tb[0], 0.0f); tw[(uint32_t)b * 2 + 1] = F4(tb[1], 0.0f); }
   }
   DEV void add_imp(int b, f3 f, f3 t) const {
     if (b < 0) return;
@@ -1978,6 +2007,57 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
   __syncthreads();
 }
 
+// One Nesterov momentum step before position sweep `it` (wgsl.js:3527-3557).  Restart test: the sum over the
+// colored rows of (lam - lam_last - e) . e, as 256 strided partials (partial j over rows j, j + 256, ...) added in
+// order, whatever the block size; below zero restarts (k = 0), else k + 1 and beta = (k - 1) / (k + 2) for k > 1.
+// Then the rows extrapolate color by color (the overflow color serially) and the articulations take their impulses.
+template <int BS>
+DEV void momentum_step(const Solver& sv, uint32_t ncol, uint32_t it, float& mk, float& mbeta) {
+  const W& w = sv.w;
+  const uint32_t tid = threadIdx.x, NA = w.NA(), NCOL = w.NCOL(), ROWS = w.ROWS(), MO = w.h(W_MO), MAXC = w.MAXC();
+  const uint32_t lane = tid & 63, wave = tid >> 6;
+  const uint32_t* sched = w.d.sched;
+  const SchedLayout& S = w.S;
+  const uint4* slots = (const uint4*)(sched + S.SLOTT);
+  if (it > 0) {
+    const uint32_t nrows = min(sched[S.CSTART + NCOL + 1], MAXC);
+    for (uint32_t j = tid; j < 256; j += BS) {
+      float p = 0.0f;
+      for (uint32_t u = j; u < nrows; u += 256) {
+        const float4* R = w.d.rows + (size_t)slots[u].x * ROWS;
+        const f3 e = xyz(R[MO + 1]);
+        p += dot(xyz(R[6]) - xyz(R[MO]) - e, e);
+      }
+      sv.red[j] = p;
+    }
+    __syncthreads();
+    float total = 0.0f;
+    for (uint32_t j = 0; j < 256; j++) total += sv.red[j];
+    mk = total < 0.0f ? 0.0f : mk + 1.0f;
+    mbeta = mk > 1.0f ? (mk - 1.0f) / (mk + 2.0f) : 0.0f;
+    __syncthreads();
+  } else {
+    mk = 0.0f; mbeta = 0.0f;
+  }
+  for (uint32_t c = 0; c < NCOL && c < ncol; c++) {
+    const uint32_t s = sched[S.CSTART + c], e = sched[S.CSTART + c + 1];
+    for (uint32_t u = s + tid; u < e; u += BS) { const uint4 sl = slots[u]; sv.momentum_unit(sl.x, (int)sl.z, (int)sl.w, mbeta, it == 0); }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    for (uint32_t u = sched[S.CSTART + NCOL]; u < sched[S.CSTART + NCOL + 1]; u++) {
+      const uint4 sl = slots[min(u, MAXC - 1)];
+      sv.momentum_unit(sl.x, (int)sl.z, (int)sl.w, mbeta, it == 0);
+    }
+  }
+  __syncthreads();
+  for (uint32_t a = wave; a < NA; a += BS / 64) {
+    sv.art_contact_impulses(a, lane);
+    sv.art_velocity_limits(a, lane);
+  }
+  __syncthreads();
+}
+
 template <int BS, bool TWL>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_SOLVE))) k_solve(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
@@ -2019,6 +2099,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     sv.tw = wd.tw;
     sv.imp = wd.imp;
   }
+  const bool NEST = w.h(W_NEST) != 0;
+  sv.red = (float*)lds;
+  if (NEST) lds += 1024;
   sv.lm = (float*)lds;
   // Link records and mirror offsets (exclusive prefix of mirror_floats(ndof) over links).
   uint32_t* msz = (uint32_t*)sv.lm;
@@ -2103,7 +2186,11 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   dense_phase<BS>(sv, 0, nd);
   sweep<BS>(sv, 0, ncol, nu, nd);
   const uint32_t iters = w.h(W_ITERS);
-  for (uint32_t it = 0; it < iters; it++) sweep<BS>(sv, 1, ncol, nu, nd);
+  float mk = 0.0f, mbeta = 0.0f;
+  for (uint32_t it = 0; it < iters; it++) {
+    if (NEST) momentum_step<BS>(sv, ncol, it, mk, mbeta);
+    sweep<BS>(sv, 1, ncol, nu, nd);
+  }
   // Mutable mirror fields back to the link blocks (p, dv, v, u, tau; S, U, D^-1, COM are unchanged).
   for (uint32_t i = tid; i < MF; i += BS) {
     uint32_t l = mirror_link(i), n = sv.mnd(l), f = i - (sv.lmeta[l].y >> 8);

@@ -14,11 +14,12 @@ enum : uint32_t {
 // k_solve's per-link LDS mirror: a link of n dofs takes n^2 + 14n + 21 floats.
 __host__ __device__ inline uint32_t mirror_floats(uint32_t n) { return n * n + 14 * n + 21; }
 // k_solve's dynamic LDS for a world: link records and tree (2 NL uint4), per-link contributions (8 NL floats),
-// joint velocities (ND floats), with tw_lds the
-// twists and impulses of its NB bodies (64 bytes each), and the link mirror (mfloats floats).
-__host__ __device__ inline size_t solve_lds_bytes(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, bool tw_lds) {
+// joint velocities (ND floats), with tw_lds the twists and impulses of its NB bodies (64 bytes each), with nest
+// the 256 partial sums of the momentum restart test (1 KB), and the link mirror (mfloats floats).
+__host__ __device__ inline size_t solve_lds_bytes(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, bool tw_lds, bool nest = false) {
   size_t b = (size_t)NL * 64 + (((size_t)ND * 4 + 15) & ~(size_t)15);
   if (tw_lds) b += (size_t)(NB > 0 ? NB : 1) * 64;
+  if (nest) b += 1024;
   return b + (((size_t)mfloats * 4 + 15) & ~(size_t)15);
 }
 // Upper bound of k_solve's dynamic LDS without the twists (worlds needing more are refused at creation).
@@ -38,9 +39,9 @@ static constexpr uint32_t STEEL_PREP_LDS_MAX_WORDS = 16384;
 struct LaunchLds {
   size_t notw = 0, tw = 0;
   uint32_t ns = 0, cw = 0;
-  void add(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, uint32_t NS, uint32_t NCOLW) {
-    notw = notw > solve_lds_bytes(NL, ND, mfloats, NB, false) ? notw : solve_lds_bytes(NL, ND, mfloats, NB, false);
-    tw = tw > solve_lds_bytes(NL, ND, mfloats, NB, true) ? tw : solve_lds_bytes(NL, ND, mfloats, NB, true);
+  void add(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, uint32_t NS, uint32_t NCOLW, bool nest) {
+    notw = notw > solve_lds_bytes(NL, ND, mfloats, NB, false, nest) ? notw : solve_lds_bytes(NL, ND, mfloats, NB, false, nest);
+    tw = tw > solve_lds_bytes(NL, ND, mfloats, NB, true, nest) ? tw : solve_lds_bytes(NL, ND, mfloats, NB, true, nest);
     ns = ns > NS ? ns : NS;
     uint32_t c = (NB > 0 ? NB : 1) * (NCOLW + 1);
     cw = cw > c ? cw : c;

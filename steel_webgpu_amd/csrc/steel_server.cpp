@@ -53,7 +53,7 @@ struct ServerWorld {
   size_t off = 0, bytes = 0;         // pool block
   size_t koff[K_COUNT], ksize[K_COUNT];
   WorldDev dev;
-  uint32_t NL = 0, ND = 0, NB = 0, NS = 0, NCOLW = 0, mfloats = 0;  // dynamic LDS inputs
+  uint32_t NL = 0, ND = 0, NB = 0, NS = 0, NCOLW = 0, mfloats = 0, NEST = 0;  // dynamic LDS inputs
 };
 
 struct Lane {
@@ -213,6 +213,7 @@ int steel_server_admit(steel_server* s, const steel_world_desc* w, uint32_t lane
   }
   W.dev = world_dev_at(base);
   W.NL = w->header[W_NL]; W.ND = w->header[W_ND]; W.NB = w->header[W_NB]; W.NS = w->header[W_NS]; W.NCOLW = w->header[W_NCOLW];
+  W.NEST = w->header[W_NEST];
   W.mfloats = world_mirror_floats(w->header, w->statF);
   uint32_t wid = s->next_id++;
   s->worlds[wid] = W;
@@ -260,7 +261,7 @@ int steel_server_step_lane(steel_server* s, uint32_t lane) {
     for (uint32_t i = 0; i < n; i++) {
       const ServerWorld& W = s->worlds[L.members[i]];
       L.h_table[i] = W.dev;
-      L.sl.add(W.NL, W.ND, W.mfloats, W.NB, W.NS, W.NCOLW);
+      L.sl.add(W.NL, W.ND, W.mfloats, W.NB, W.NS, W.NCOLW, W.NEST != 0);
     }
     if (n) HIPCHK(hipMemcpyAsync(L.d_table, L.h_table, sizeof(WorldDev) * n, hipMemcpyHostToDevice, L.stream));
     L.dirty = false;

@@ -504,7 +504,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for k, default in (("enable_bilateral_preelimination", False), ("propagation_same_articulation_rows", False), ("update_mass_matrix_interval", 1)):
         if SV.get(k) is not None and SV.get(k) != default:
             raise ValueError(f"{k}={SV[k]} is not supported")
-    for k in ("nesterov_momentum", "enable_torsional_rolling_friction", "contact_compliance", "contact_shared_anchor",
+    for k in ("enable_torsional_rolling_friction", "contact_compliance", "contact_shared_anchor",
               "contact_friction_shared_anchor"):
         if SV.get(k) or SO.get(k):
             raise UnsupportedError(f"{k} is not supported yet")
@@ -523,7 +523,9 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         anchorBeta = 0.0
     PO = 19
     FMO = PO + (4 if anchorBeta > 0 else 0)
-    ROWS = FMO
+    nest = bool(SV.get("nesterov_momentum"))
+    MO = FMO
+    ROWS = MO + (2 if nest else 0)
     P = dict(P_BODY=0, P_SHAPE=NB * 2, P_AABB=NB * 2 + NS * 2, P_PREV=NB * 2 + NS * 4)
     maxp, maxc, maxu, uh, ah, maxdense = capacities(NS, NB, NA, capacity_policy)
     if capacity_override:
@@ -608,6 +610,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_SLEEP", 1 if (SV.get("enable_sleeping") or SO.get("enable_sleeping")) else 0)
     H("W_NMIMIC", len(mimics))
     H("W_WH", WARM_SLOTS if warm else 0)
+    H("W_NEST", 1 if nest else 0)
+    H("W_MO", MO)
     for i, m in enumerate(mimics):
         statF[(SF["SF_MIMIC"] + i * 2) * 4:(SF["SF_MIMIC"] + i * 2) * 4 + 8] = [m["art"], m["f"], m["l"], m["c1"], m["fq"], m["lq"], m["c0"], 0]
         statU[SU["SU_ACON"] + m["art"]] = 1  # articulations owning a mimic never sleep (sim.js:369-370)

@@ -51,6 +51,12 @@ def _with_warm(scene):
     return scene
 
 
+def _with_nesterov(scene, warm=False):
+    """Nesterov momentum on the contact rows (with contact warm start on top when warm)."""
+    scene.json["solver"].update(nesterov_momentum=True, contact_warmstart=warm)
+    return scene
+
+
 def _with_mimics(scene):
     """Each leg's third joint mimics its second (q = 0.1 - 1.2 q_leader)."""
     J = scene.json["joints"]
@@ -115,6 +121,8 @@ SCENES = {
     "quad_mimic": lambda: _with_mimics(workloads.quadruped_world(10)),
     "boxes_warm": lambda: _with_warm(workloads.boxes64()),
     "quad_warm": lambda: _with_warm(workloads.quadruped_world(12)),
+    "boxes_nest": lambda: _with_nesterov(workloads.boxes64()),
+    "quad_nest": lambda: _with_nesterov(workloads.quadruped_world(13), warm=True),
     # a recorded 60 Hz gait driving the legs (base dofs of the path are 0; their drives are off)
     "quad_path": lambda: workloads.with_target_path(workloads.quadruped_world(8, drive=dict(targetKe=40.0, targetKd=1.0)),
                                                     workloads.gait_path(), integral=8),
@@ -156,7 +164,8 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
 @pytest.mark.parametrize("name,frames", [("boxes64", 120), ("quad0", 200), ("quad5", 80), ("arm", 12), ("quad_driven", 200), ("serve1500", 40), ("quad_vlim", 200),
                                          ("boxes_limits", 150), ("quad_friction", 200), ("bighulls", 120),
                                          ("quad_path", 200), ("boxes_sleep", 400), ("quad_sleep", 300),
-                                         ("quad_mimic", 200), ("boxes_warm", 150), ("quad_warm", 200)])
+                                         ("quad_mimic", 200), ("boxes_warm", 150), ("quad_warm", 200),
+                                         ("boxes_nest", 150), ("quad_nest", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])
@@ -236,8 +245,9 @@ def test_batched_equals_alone_and_mixed_sizes(sim):
     mixed.destroy()
 
 
-def test_deterministic_across_runs_block_sizes_and_graphs(sim):
-    pw = pack.pack_scene(workloads.quadruped_world(5))
+@pytest.mark.parametrize("name", ["quad5", "boxes_nest"])
+def test_deterministic_across_runs_block_sizes_and_graphs(sim, name):
+    pw = pack.pack_scene(SCENES[name]())
     ref = None
     for block, graph in ((256, True), (256, True), (64, True), (128, False), (64, False)):
         b = sim.Batch([pw], block=block, graph=graph)

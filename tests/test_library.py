@@ -98,7 +98,7 @@ def test_capacities_are_exact_bounds():
 
 def test_unsupported_options_raise():
     with pytest.raises(pack.UnsupportedError):
-        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"nesterov_momentum": True}})
+        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"contact_compliance": True}})
     with pytest.raises(ValueError):
         pack.pack_scene(workloads.boxes64(n=4), opts={"substeps": 64})
     b = SceneBuilder()

@@ -1120,3 +1120,64 @@ def test_warm_start_rows_begin_at_the_scaled_table_values_within_the_cone():
                 assert np.allclose(R[i, 6, 1:3], lw, rtol=1e-5, atol=1e-7 * load), (i, R[i, 6], lw)
                 carried += 1
         assert carried >= 1 or not anchors
+
+
+def test_nesterov_momentum_sphere_follows_the_accelerated_single_row_recursion():
+    """Nesterov momentum (one frictionless-in-effect contact row, omega 0.5, 6 iterations): before sweep it > 0 the
+    restart test sums (lam - lam_last - e) * e (below zero: k = 0, else k + 1), beta = (k - 1) / (k + 2) for
+    k > 1, the row jumps to y = lam + beta (lam - lam_last) and the change reaches the velocity through the row's
+    response, then the sweep relaxes from y.  The FP64 oracle follows this recursion substep by substep, and the
+    trajectory differs from the plain one."""
+    trajs = {}
+    for nest in (False, True):
+        b = SceneBuilder(fps=240)
+        b.addGroundPlane(gap=0.01)
+        s = b.addBody(xform=[0, 0, 0.0985, 0, 0, 0, 1])
+        b.addShapeSphere(s, radius=0.1, gap=0.01)
+        pw = pack_scene(b.finalize(solver=dict(substeps=4, iterations=6, relaxation=0.5, nesterov_momentum=nest)))
+        assert pw.h("W_NEST") == int(nest)
+        r = float(np.float32(0.1))
+        o = OracleWorld(pw, f64=True)
+        SF = pw.h("W_SF_BODY")
+        statF = pw.statF.reshape(-1, 4).astype(np.float64)
+        im = statF[SF][0]
+        dt, gz, om = pw.hf("W_DT"), pw.hf("W_GZ"), pw.hf("W_F_OMEGA")
+        beta_b, spec, reg, cfm = pw.hf("W_BETA"), pw.hf("W_F_SPEC"), pw.hf("W_REG"), pw.hf("W_F_CFM")
+        eff = 1.0 / (im + cfm)
+        P = pw.h("W_P_BODY") * 4
+        z, v = float(o.poses[P + 2]), 0.0
+        zs, restarts = [], 0
+        for k in range(120):
+            o.substeps(1, debug=True)
+            assert o.dbg_bufs["ncontacts"][0] == 1
+            phi = z - r
+            rhs = beta_b * phi / dt if phi < 0 else spec * phi / dt
+            w = 1.0 if (phi > 0 or reg <= 0) else 1.0 / (1.0 + reg)
+            v = v + gz * dt
+            lam, last, e, kk = 0.0, 0.0, 0.0, 0.0
+            for it in range(6):
+                if nest:
+                    if it > 0:
+                        total = (lam - last - e) * e
+                        kk = 0.0 if total < 0 else kk + 1.0
+                        restarts += total < 0
+                        bt = (kk - 1.0) / (kk + 2.0) if kk > 1 else 0.0
+                        y = lam + bt * (lam - last)
+                        e = y - lam
+                        last = lam
+                        if e != 0:
+                            lam = y
+                            v += e * im
+                    else:
+                        last, e = lam, 0.0
+                nl = max(lam + om * (-(v + rhs) * eff * w - (1.0 - w) * lam), 0.0)
+                v += (nl - lam) * im
+                lam = nl
+            z = z + v * dt
+            assert abs(o.poses[P + 2] - z) <= 1e-12, (nest, k)
+            assert abs(o.vel[2] - v) <= 1e-10, (nest, k)
+            zs.append(z)
+        trajs[nest] = np.array(zs)
+        if nest:
+            assert restarts > 0  # the restart branch is exercised
+    assert np.abs(trajs[True] - trajs[False]).max() > 1e-6
