@@ -61,6 +61,8 @@ enum steel_world_field {
   W_WH,
   /* Nesterov momentum on the contact rows (u32 flag) and its row-record offset (u32) */
   W_NEST, W_MO,
+  /* contact torsion of articulated units (u32 flag) and its row-record offset (u32) */
+  W_TORS, W_TO,
   W_FIELD_COUNT
 };
 

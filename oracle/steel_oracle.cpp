@@ -68,7 +68,7 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t WH, NEST, MO;
+  uint32_t WH, NEST, MO, TORS, TO;
   T mom_k = 0, mom_beta = 0;  // Nesterov restart counter and extrapolation factor of the current substep
   uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
   T FPS, FRAME_DT, F_SLEEP_LIN, F_SLEEP_ANG, F_SLEEP_TIME;
@@ -119,7 +119,7 @@ template <class T> struct World {
     A_Q = h[W_A_Q]; A_QD = h[W_A_QD]; A_QDH = h[W_A_QDH]; A_TGT = h[W_A_TGT]; A_TGTV = h[W_A_TGTV]; A_LIM = h[W_A_LIM]; A_DRV = h[W_A_DRV];
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
-    WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO];
+    WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
     TGT_HELD = h[W_TGT_HELD]; A_INT = h[W_A_INT]; FPS = hf(W_FPS); FRAME_DT = hf(W_FRAME_DT);
@@ -1628,6 +1628,42 @@ template <class T> struct World {
     return NCOL;
   }
   uint32_t anchor_hash(uint32_t pair, uint32_t k) const { return std::max(hash_u32(pair * 2u + k), 1u); }
+  // Contact torsion (wgsl.js:1526-1557): one spin row per articulated unit about the normal of its deepest point,
+  // over the members touching in that plane (normal within 0.995, offset along it within 0.002, penetrating),
+  // with the largest member torsion radius r.  Header in the unit's first row: (Ma lin, mu), (Ma ang, 1 / K),
+  // (Mb lin, spin impulse), (Mb ang, r), (n, first row's member flag); every row's TO + 4 .w is its member flag.
+  void build_torsion(uint32_t sp) {
+    uint32_t st = S_USTART[sp];
+    uint32_t n = std::min(S_USIZE[sp], MAXC - std::min(st, MAXC));
+    if (n == 0) return;
+    uint32_t pair = S_UKEY[sp];
+    int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+    uint32_t L = st * ROWS;
+    bool art_u = (a >= 0 && body_kind(a) >= 2) || (b >= 0 && body_kind(b) >= 2);
+    uint32_t seed = st;
+    for (uint32_t i = st; i < st + n; i++) if (rows[i * ROWS + 1].w < rows[seed * ROWS + 1].w) seed = i;
+    V3<T> ns = xyz(rows[seed * ROWS]), ps = xyz(rows[seed * ROWS + TO + 4]);
+    T r = 0;
+    for (uint32_t i = st; i < st + n; i++) {
+      uint32_t o = i * ROWS;
+      bool member = art_u && rows[o + 6].w <= T(0) && dot(xyz(rows[o]), ns) >= T(0.995) &&
+                    std::fabs(dot(xyz(rows[o + TO + 4]) - ps, ns)) <= T(0.002);
+      if (member) r = mx(r, rows[o + TO + 3].w);
+      rows[o + TO + 4].w = member ? T(1) : T(0);
+    }
+    T mu = rows[seed * ROWS + 2].w;
+    if (r <= T(0) || mu <= T(0)) { rows[L + TO] = {0, 0, 0, 0}; return; }
+    V3<T> Ma[2], Mb[2];
+    body_response(a, V3<T>{0, 0, 0}, -ns, Ma);
+    body_response(b, V3<T>{0, 0, 0}, ns, Mb);
+    T k = dot(-ns, Ma[1]) + dot(ns, Mb[1]);
+    T lead = rows[L + TO + 4].w;
+    rows[L + TO] = v4(Ma[0], mu);
+    rows[L + TO + 1] = v4(Ma[1], k > T(1e-12) ? T(1) / (k + F_CFM) : T(0));
+    rows[L + TO + 2] = v4(Mb[0], T(0));
+    rows[L + TO + 3] = v4(Mb[1], r);
+    rows[L + TO + 4] = v4(ns, lead);
+  }
   void build_patch(uint32_t sp) {  // wgsl.js:1562-1725
     uint32_t st = S_USTART[sp];
     uint32_t n = std::min(S_USIZE[sp], MAXC - std::min(st, MAXC));
@@ -2185,6 +2221,7 @@ template <class T> struct World {
           K[r * 3 + s] = dot(-d, Ma[s][0]) + dot(-cross(aa, d), Ma[s][1]) + dot(d, Mb[s][0]) + dot(cross(ab, d), Mb[s][1]);
         }
       if (EXTRA) { rows[o + PO] = v4(fa, T(1)); rows[o + PO + 1] = v4(fb, T(0)); rows[o + PO + 2] = v4(p, T(0)); }
+      if (TORS) { rows[o + TO + 3] = {0, 0, 0, C.q[3].w}; rows[o + TO + 4] = v4(p, T(0)); }
       T kscale = mx(K[0] + K[4] + K[8], T(1e-30));
       T effn = 0;
       if (K[0] > T(1e-6) * kscale && K[0] > T(1e-12)) effn = T(1) / (K[0] + F_CFM);
@@ -2220,6 +2257,7 @@ template <class T> struct World {
       rows[o + 5] = {k11 / det, -k01 / det, k00 / det, bounce ? T(1) : wrow};
       rows[o + 6] = {ln, lt.x, lt.y, bounce ? T(1e30) : phi};
     }
+    if (TORS) for (uint32_t sp = 0; sp < nu; sp++) build_torsion(sp);
     if (PATCH) for (uint32_t sp = 0; sp < nu; sp++) build_patch(sp);
     // Phase 6c: coloring
     uint32_t totp = std::min(S_USTART[nu], MAXC);
@@ -2576,9 +2614,56 @@ template <class T> struct World {
         add_imp(b, f, cross(frb, f));
       }
       rows[o + 6] = lam;
+      if (TORS && row == S_UTP[us] && rows[S_USTART[us] * ROWS + TO].w > T(0)) unit_torsion(us, a, b, ta, tb);
     }
     if (a >= 0) { tw[(uint32_t)a * 2] = v4(ta[0], T(0)); tw[(uint32_t)a * 2 + 1] = v4(ta[1], T(0)); }
     if (b >= 0) { tw[(uint32_t)b * 2] = v4(tb[0], T(0)); tw[(uint32_t)b * 2 + 1] = v4(tb[1], T(0)); }
+  }
+  // Contact torsion after the unit's last colored row (wgsl.js:3185-3228): sliding and spin share the members'
+  // Coulomb budget mu * N; when sliding exceeds it (later normals lowered the load) the members' tangent pairs are
+  // scaled onto it, then the spin row is relaxed within r * (budget - sliding).
+  void unit_torsion(uint32_t us, int a, int b, V3<T> ta[2], V3<T> tb[2]) {
+    uint32_t s0 = S_USTART[us], s1 = std::min(s0 + S_USIZE[us], MAXC), L = s0 * ROWS;
+    T mu = rows[L + TO].w;
+    T load = 0, slide = 0;
+    for (uint32_t m = s0; m < s1; m++) {
+      uint32_t om = m * ROWS;
+      if (rows[om + TO + 4].w < T(0.5)) continue;
+      V4<T> lm = rows[om + 6];
+      load += mx(lm.x, T(0));
+      if (!EXTRA || rows[om + PO].w != T(0)) slide += length(V2<T>{lm.y, lm.z});
+    }
+    T budget = mu * load;
+    if (slide > budget && slide > T(0)) {
+      T scale = budget / slide;
+      for (uint32_t m = s0; m < s1; m++) {
+        uint32_t om = m * ROWS;
+        if (rows[om + TO + 4].w < T(0.5) || (EXTRA && rows[om + PO].w == T(0))) continue;
+        V4<T> lm = rows[om + 6];
+        T d1 = lm.y * (scale - T(1)), d2 = lm.z * (scale - T(1));
+        ta[0] += xyz(rows[om + 11]) * d1 + xyz(rows[om + 15]) * d2; ta[1] += xyz(rows[om + 12]) * d1 + xyz(rows[om + 16]) * d2;
+        tb[0] += xyz(rows[om + 13]) * d1 + xyz(rows[om + 17]) * d2; tb[1] += xyz(rows[om + 14]) * d1 + xyz(rows[om + 18]) * d2;
+        V3<T> fa = xyz(rows[om + 3]), fb = xyz(rows[om + 4]);
+        if (EXTRA) { fa = xyz(rows[om + PO]); fb = xyz(rows[om + PO + 1]); }
+        V3<T> f = xyz(rows[om + 1]) * d1 + xyz(rows[om + 2]) * d2;
+        add_imp(a, -f, -cross(fa, f));
+        add_imp(b, f, cross(fb, f));
+        rows[om + 6] = {lm.x, lm.y * scale, lm.z * scale, lm.w};
+      }
+      slide = budget;
+    }
+    T bound = rows[L + TO + 3].w * mx(budget - slide, T(0));
+    V3<T> ng = xyz(rows[L + TO + 4]);
+    T ls = rows[L + TO + 2].w;
+    T nls = clampv(ls - F_OMEGA * dot(ng, tb[1] - ta[1]) * rows[L + TO + 1].w, -bound, bound);
+    T ds = nls - ls;
+    if (ds != T(0)) {
+      ta[0] += xyz(rows[L + TO]) * ds; ta[1] += xyz(rows[L + TO + 1]) * ds;
+      tb[0] += xyz(rows[L + TO + 2]) * ds; tb[1] += xyz(rows[L + TO + 3]) * ds;
+      rows[L + TO + 2].w = nls;
+      add_imp(a, V3<T>{0, 0, 0}, -ng * ds);
+      add_imp(b, V3<T>{0, 0, 0}, ng * ds);
+    }
   }
   void set_link_twist(uint32_t l, SV<T> v) {
     uint32_t b = (uint32_t)link_body(l);

@@ -493,6 +493,42 @@ struct Prep {
     float lim = mu * ln, lm = len(lt);
     if (lm > lim) lt = lt * (lim / mx(lm, 1e-12f));
   }
+  // Contact torsion (wgsl.js:1526-1557): one spin row per articulated unit about the normal of its deepest point,
+  // over the members touching in that plane (normal within 0.995, offset along it within 0.002, penetrating),
+  // with the largest member torsion radius r.  Header in the unit's first row: (Ma lin, mu), (Ma ang, 1 / K),
+  // (Mb lin, spin impulse), (Mb ang, r), (n, first row's member flag); every row's TO + 4 .w is its member flag.
+  DEVN void build_torsion(uint32_t sp) const {
+    const uint32_t MAXC = w.MAXC(), TO = w.h(W_TO);
+    const uint32_t st = w.sch(w.S.USTART)[sp];
+    const uint32_t n = min(w.sch(w.S.USIZE)[sp], MAXC - min(st, MAXC));
+    if (n == 0) return;
+    const uint32_t pair = w.sch(w.S.UKEY)[sp];
+    const int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+    const bool art_u = (a >= 0 && w.body_kind(a) >= 2) || (b >= 0 && w.body_kind(b) >= 2);
+    uint32_t seed = st;
+    for (uint32_t i = st; i < st + n; i++) if (row(i, 1).w < row(seed, 1).w) seed = i;
+    const f3 ns = xyz(row(seed, 0)), ps = xyz(row(seed, TO + 4));
+    float r = 0.0f;
+    for (uint32_t i = st; i < st + n; i++) {
+      const bool member = art_u && row(i, 6).w <= 0.0f && dot(xyz(row(i, 0)), ns) >= 0.995f && fabsf(dot(xyz(row(i, TO + 4)) - ps, ns)) <= 0.002f;
+      if (member) r = mx(r, row(i, TO + 3).w);
+      float4 t4 = row(i, TO + 4);
+      t4.w = member ? 1.0f : 0.0f;
+      set_row(i, TO + 4, t4);
+    }
+    const float mu = row(seed, 2).w;
+    if (r <= 0.0f || mu <= 0.0f) { set_row(st, TO, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
+    f3 Ma[2], Mb[2];
+    w.body_response(a, mk3(0, 0, 0), -ns, Ma);
+    w.body_response(b, mk3(0, 0, 0), ns, Mb);
+    const float k = dot(-ns, Ma[1]) + dot(ns, Mb[1]);
+    const float lead = row(st, TO + 4).w;
+    set_row(st, TO, F4(Ma[0], mu));
+    set_row(st, TO + 1, F4(Ma[1], k > 1e-12f ? 1.0f / (k + w.hf(W_F_CFM)) : 0.0f));
+    set_row(st, TO + 2, F4(Mb[0], 0.0f));
+    set_row(st, TO + 3, F4(Mb[1], r));
+    set_row(st, TO + 4, F4(ns, lead));
+  }
   DEVN void build_patch(uint32_t sp) const {  // wgsl.js:1562-1725
     const uint32_t* US = w.sch(w.S.USTART);
     uint32_t MAXC = w.MAXC(), PO = w.PO(), AH = w.AH();
@@ -692,6 +728,7 @@ struct Prep {
       }
     uint32_t PO = w.PO();
     if (w.h(W_EXTRA)) { set_row(idx, PO, F4(fa, 1.0f)); set_row(idx, PO + 1, F4(fb, 0.0f)); set_row(idx, PO + 2, F4(p, 0.0f)); }
+    if (w.h(W_TORS)) { set_row(idx, w.h(W_TO) + 3, make_float4(0.0f, 0.0f, 0.0f, con(c, 3).w)); set_row(idx, w.h(W_TO) + 4, F4(p, 0.0f)); }
     float cfm = w.hf(W_F_CFM), dt = w.hf(W_DT);
     float kscale = mx(K[0] + K[4] + K[8], 1e-30f);
     float effn = 0.0f;
@@ -1196,6 +1233,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const uint32_t total = min(tot, MAXC);
   for (uint32_t idx = tid; idx < total; idx += BS) P.build_row(idx, nu);
   __syncthreads();
+  if (w.h(W_TORS)) {
+    for (uint32_t sp = tid; sp < nu; sp += BS) P.build_torsion(sp);
+    __syncthreads();
+  }
   if (PATCH) {
     for (uint32_t sp = tid; sp < nu; sp += BS) P.build_patch(sp);
     __syncthreads();
@@ -1611,9 +1652,64 @@ struct Solver {
         add_imp(b, f, cross(frb, f));
       }
       R[6] = lam;
+      if (w.h(W_TORS) && r == w.d.sched[w.S.UTP + us]) unit_torsion(us, a, b, ta, tb);
     }
     if (a >= 0) { tw[(uint32_t)a * 2] = F4(ta[0], 0.0f); tw[(uint32_t)a * 2 + 1] = F4(ta[1], 0.0f); }
     if (b >= 0) { tw[(uint32_t)b * 2] = F4(tb[0], 0.0f); tw[(uint32_t)b * 2 + 1] = F4(tb[1], 0.0f); }
+  }
+  // Contact torsion after the unit's last colored row (wgsl.js:3185-3228): sliding and spin share the members'
+  // Coulomb budget mu * N; when sliding exceeds it (later normals lowered the load) the members' tangent pairs are
+  // scaled onto it, then the spin row is relaxed within r * (budget - sliding).  The unit's rows were all solved
+  // in earlier colors, by this thread or before the last barrier.
+  DEVN void unit_torsion(uint32_t us, int a, int b, f3 ta[2], f3 tb[2]) const {
+    const uint32_t* sched = w.d.sched;
+    const uint32_t ROWS = w.ROWS(), TO = w.h(W_TO), PO = w.PO(), MAXC = w.MAXC();
+    const bool EXTRA = w.h(W_EXTRA) != 0;
+    const uint32_t s0 = sched[w.S.USTART + us], s1 = min(s0 + sched[w.S.USIZE + us], MAXC);
+    float4* L = w.d.rows + (size_t)s0 * ROWS;
+    if (!(L[TO].w > 0.0f)) return;
+    const float mu = L[TO].w, omega = w.hf(W_F_OMEGA);
+    float load = 0.0f, slide = 0.0f;
+    for (uint32_t m = s0; m < s1; m++) {
+      const float4* M = w.d.rows + (size_t)m * ROWS;
+      if (M[TO + 4].w < 0.5f) continue;
+      const float4 lm = M[6];
+      load += mx(lm.x, 0.0f);
+      if (!EXTRA || M[PO].w != 0.0f) slide += len(mk2(lm.y, lm.z));
+    }
+    const float budget = mu * load;
+    if (slide > budget && slide > 0.0f) {
+      const float scale = budget / slide;
+      for (uint32_t m = s0; m < s1; m++) {
+        float4* M = w.d.rows + (size_t)m * ROWS;
+        if (M[TO + 4].w < 0.5f || (EXTRA && M[PO].w == 0.0f)) continue;
+        const float4 lm = M[6];
+        const float d1 = lm.y * (scale - 1.0f), d2 = lm.z * (scale - 1.0f);
+        ta[0] += xyz(M[11]) * d1 + xyz(M[15]) * d2; ta[1] += xyz(M[12]) * d1 + xyz(M[16]) * d2;
+        tb[0] += xyz(M[13]) * d1 + xyz(M[17]) * d2; tb[1] += xyz(M[14]) * d1 + xyz(M[18]) * d2;
+        f3 fa = xyz(M[3]), fb = xyz(M[4]);
+        if (EXTRA) { fa = xyz(M[PO]); fb = xyz(M[PO + 1]); }
+        const f3 f = xyz(M[1]) * d1 + xyz(M[2]) * d2;
+        add_imp(a, -f, -cross(fa, f));
+        add_imp(b, f, cross(fb, f));
+        M[6] = make_float4(lm.x, lm.y * scale, lm.z * scale, lm.w);
+      }
+      slide = budget;
+    }
+    const float bound = L[TO + 3].w * mx(budget - slide, 0.0f);
+    const f3 ng = xyz(L[TO + 4]);
+    const float ls = L[TO + 2].w;
+    const float nls = clampf_(ls - omega * dot(ng, tb[1] - ta[1]) * L[TO + 1].w, -bound, bound);
+    const float ds = nls - ls;
+    if (ds != 0.0f) {
+      ta[0] += xyz(L[TO]) * ds; ta[1] += xyz(L[TO + 1]) * ds;
+      tb[0] += xyz(L[TO + 2]) * ds; tb[1] += xyz(L[TO + 3]) * ds;
+      float4 l2 = L[TO + 2];
+      l2.w = nls;
+      L[TO + 2] = l2;
+      add_imp(a, mk3(0, 0, 0), -ng * ds);
+      add_imp(b, mk3(0, 0, 0), ng * ds);
+    }
   }
   DEV void set_link_twist(uint32_t l, SVd v) const {
     uint32_t b = mbody(l);

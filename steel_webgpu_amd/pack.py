@@ -285,7 +285,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         t = s["type"]
         if t == GT["CONE"]:
             raise UnsupportedError("cone shapes are not supported yet")
-        if (s.get("condim", 3) or 3) >= 4 or (s.get("torsion_radius") or 0) > 0:
+        if (s.get("condim", 3) or 3) >= 4:
             raise UnsupportedError("torsional/rolling friction rows are not supported yet")
 
     SF = {}
@@ -524,7 +524,9 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     PO = 19
     FMO = PO + (4 if anchorBeta > 0 else 0)
     nest = bool(SV.get("nesterov_momentum"))
-    MO = FMO
+    tors = any((s.get("torsion_radius") or 0) > 0 for s in S["shapes"])
+    TO = FMO
+    MO = TO + (5 if tors else 0)
     ROWS = MO + (2 if nest else 0)
     P = dict(P_BODY=0, P_SHAPE=NB * 2, P_AABB=NB * 2 + NS * 2, P_PREV=NB * 2 + NS * 4)
     maxp, maxc, maxu, uh, ah, maxdense = capacities(NS, NB, NA, capacity_policy)
@@ -612,6 +614,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_WH", WARM_SLOTS if warm else 0)
     H("W_NEST", 1 if nest else 0)
     H("W_MO", MO)
+    H("W_TORS", 1 if tors else 0)
+    H("W_TO", TO)
     for i, m in enumerate(mimics):
         statF[(SF["SF_MIMIC"] + i * 2) * 4:(SF["SF_MIMIC"] + i * 2) * 4 + 8] = [m["art"], m["f"], m["l"], m["c1"], m["fq"], m["lq"], m["c0"], 0]
         statU[SU["SU_ACON"] + m["art"]] = 1  # articulations owning a mimic never sleep (sim.js:369-370)

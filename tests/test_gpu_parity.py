@@ -57,6 +57,13 @@ def _with_nesterov(scene, warm=False):
     return scene
 
 
+def _with_torsion(scene, radius=0.02):
+    """Contact torsion radius on every shape (the reference's torsion_radius; articulated contacts use it)."""
+    for s in scene.json["shapes"]:
+        s["torsion_radius"] = radius
+    return scene
+
+
 def _with_mimics(scene):
     """Each leg's third joint mimics its second (q = 0.1 - 1.2 q_leader)."""
     J = scene.json["joints"]
@@ -123,6 +130,7 @@ SCENES = {
     "quad_warm": lambda: _with_warm(workloads.quadruped_world(12)),
     "boxes_nest": lambda: _with_nesterov(workloads.boxes64()),
     "quad_nest": lambda: _with_nesterov(workloads.quadruped_world(13), warm=True),
+    "quad_tors": lambda: _with_torsion(workloads.quadruped_world(14)),
     # a recorded 60 Hz gait driving the legs (base dofs of the path are 0; their drives are off)
     "quad_path": lambda: workloads.with_target_path(workloads.quadruped_world(8, drive=dict(targetKe=40.0, targetKd=1.0)),
                                                     workloads.gait_path(), integral=8),
@@ -165,7 +173,7 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
                                          ("boxes_limits", 150), ("quad_friction", 200), ("bighulls", 120),
                                          ("quad_path", 200), ("boxes_sleep", 400), ("quad_sleep", 300),
                                          ("quad_mimic", 200), ("boxes_warm", 150), ("quad_warm", 200),
-                                         ("boxes_nest", 150), ("quad_nest", 200)])
+                                         ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])

@@ -1181,3 +1181,60 @@ def test_nesterov_momentum_sphere_follows_the_accelerated_single_row_recursion()
         if nest:
             assert restarts > 0  # the restart branch is exercised
     assert np.abs(trajs[True] - trajs[False]).max() > 1e-6
+
+
+def _spinning_ball(tr, spin, R=0.1, mu=0.8):
+    """Floating-base ball resting slightly sunk on the ground, spinning about z; its shape has contact torsion
+    radius tr.  A small inner ball on a revolute joint about z makes it an articulation without adding to the
+    base's spin inertia (the joint decouples the child's z rotation)."""
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane(gap=0.01, mu=mu)
+    s = b.addBody(xform=[0, 0, R - 0.0003, 0, 0, 0, 1])
+    b.addShapeSphere(s, radius=R, gap=0.01, mu=mu)
+    b.addJointFree(s)
+    c = b.addBody(xform=[0, 0, R - 0.0003, 0, 0, 0, 1])
+    b.addShapeSphere(c, radius=0.01, gap=0.001)
+    b.addJointRevolute(s, c, axis=(0, 0, 1))
+    scene = b.finalize(solver=dict(substeps=4, iterations=8, friction_anchor_beta=None))
+    scene.json["shapes"][1]["torsion_radius"] = tr
+    pw = pack_scene(scene)
+    o = OracleWorld(pw, f64=True)
+    o.art[pw.A["A_QD"] + 5] = spin
+    return pw, o
+
+
+def test_contact_torsion_spins_down_at_radius_times_coulomb_budget():
+    """An articulated ball spinning on its contact point: the point does not slide, so sliding friction is zero and
+    the spin row alone brakes the spin, at the bound r * mu * N per substep while the ball sinks into the ground
+    (a torsion member needs phi <= 0): I_zz dw_z = -r mu N, with N the substep's normal impulse read as
+    m (dv_z + g dt), and dw_z measured before the root's angular damping (w' = (w + dw) (1 - c dt)).  With the ball
+    above the ground (phi > 0) the spin only damps.  Without a torsion radius the spin only damps."""
+    r, mu, w0 = 0.02, 0.8, 6.0
+    R = float(np.float32(0.1))
+    for tr in (r, 0.0):
+        pw, o = _spinning_ball(tr, w0)
+        assert pw.h("W_TORS") == int(tr > 0) and pw.NA == 1 and pw.NL == 2
+        SF, A = pw.h("W_SF_BODY"), pw.A
+        F = pw.statF.reshape(-1, 4).astype(np.float64)
+        m, Izz, damp = 1.0 / F[SF][0] + 1.0 / F[SF + 8][0], F[SF + 4][2], F[SF + 3][3]
+        dt, gz = pw.hf("W_DT"), pw.hf("W_GZ")
+        QD, Q = A["A_QD"], A["A_Q"]
+        braked = free = 0
+        for k in range(4 * 120):
+            v0, w_0, phi = o.art[QD + 2], o.art[QD + 5], o.art[Q + 2] - R
+            o.substeps(1)
+            v1, w_1 = o.art[QD + 2], o.art[QD + 5]
+            N = m * (v1 - v0 - gz * dt)
+            dw = w_1 / (1.0 - damp * dt) - w_0
+            if tr == 0 or N == 0:  # no torsion radius, or no load this substep: the spin only damps
+                assert abs(dw) <= 1e-12 * w0, k
+                free += 1
+            elif w_1 > 1e-9:
+                assert phi <= 0 and N > 0
+                # 1e-7: the body record's inverse mass and the link's mass are separately rounded to float32
+                assert abs(Izz * dw + r * mu * N) <= 1e-7 * r * mu * N, k
+                braked += 1
+            else:
+                assert abs(w_1) <= 1e-12, k  # stopped within the bound, and held
+                free += 1
+        assert free > 20 and (braked > 100 or tr == 0)
