@@ -187,6 +187,8 @@ template <class T> struct World {
   uint32_t link_ndof(uint32_t l) const { return tu(statF[SF_LINK + l * 6 + 1].w); }
   uint32_t link_qs(uint32_t l) const { return tu(statF[SF_LINK + l * 6 + 3].w); }
   V4<T> dof_drive(uint32_t d) const { return statF[SF_DOF + d * 5]; }
+  V3<T> dof_axis(uint32_t d) const { return xyz(statF[SF_DOF + d * 5 + 3]); }
+  uint32_t link_nlin(uint32_t l) const { return tu(statF[SF_LINK + l * 6 + 5].w); }  // D6: leading translation dofs
   V4<T> dof_limit(uint32_t d) const { return statF[SF_DOF + d * 5 + 1]; }
   V4<T> dof_passive(uint32_t d) const { return statF[SF_DOF + d * 5 + 2]; }
   bool scalar_dof(uint32_t d) const { return statF[SF_DOF + d * 5 + 2].w > T(0.5); }
@@ -1161,6 +1163,34 @@ template <class T> struct World {
       if (t == JT_REVOLUTE) mq = qaxis(axis, art[A_Q + qs]);
       if (t == JT_PRISMATIC) mp = axis * art[A_Q + qs];
       if (t == JT_BALL) mq = normalize4(V4<T>{art[A_Q + qs], art[A_Q + qs + 1], art[A_Q + qs + 2], art[A_Q + qs + 3]});
+      // D6: translations along the leading axes, then rotations about axes carried through the rotations before
+      // them (artwgsl.js:117-141)
+      uint32_t d0 = link_dof(l), nlin = link_nlin(l), nang = link_ndof(l) - nlin;
+      V3<T> ang[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      if (t == JT_D6) {
+        for (uint32_t c = 0; c < nlin; c++) mp += dof_axis(d0 + c) * art[A_Q + qs + c];
+        uint32_t iq = qs + nlin;
+        if (nang == 1) {
+          ang[0] = dof_axis(d0 + nlin);
+          mq = qaxis(ang[0], art[A_Q + iq]);
+        }
+        if (nang == 2) {
+          V3<T> x0 = normalize(dof_axis(d0 + nlin));
+          V3<T> x2 = normalize(cross(x0, dof_axis(d0 + nlin + 1)));
+          ang[0] = x0;
+          V4<T> q0 = qaxis(x0, art[A_Q + iq]);
+          ang[1] = qrot(q0, cross(x2, x0));
+          mq = qmul(qaxis(ang[1], art[A_Q + iq + 1]), q0);
+        }
+        if (nang == 3) {
+          ang[0] = dof_axis(d0 + nlin);
+          V4<T> q0 = qaxis(ang[0], art[A_Q + iq]);
+          ang[1] = qrot(q0, dof_axis(d0 + nlin + 1));
+          V4<T> q1 = qaxis(ang[1], art[A_Q + iq + 1]);
+          ang[2] = qrot(qmul(q1, q0), dof_axis(d0 + nlin + 2));
+          mq = qmul(qaxis(ang[2], art[A_Q + iq + 2]), qmul(q1, q0));
+        }
+      }
       if (t == JT_FREE) {
         mp = {art[A_Q + qs], art[A_Q + qs + 1], art[A_Q + qs + 2]};
         mq = normalize4(V4<T>{art[A_Q + qs + 3], art[A_Q + qs + 4], art[A_Q + qs + 5], art[A_Q + qs + 6]});
@@ -1187,6 +1217,13 @@ template <class T> struct World {
         }
       V3<T> rf = com - cp;
       st3(L + L_JR, rf);
+      if (t == JT_D6) {
+        for (uint32_t c = 0; c < nlin; c++) stsv(L + L_S + c * 6, SV<T>{{0, 0, 0}, qrot(jq, dof_axis(d0 + c))});
+        for (uint32_t c = 0; c < nang; c++) {
+          V3<T> ew = qrot(jq, ang[c]);
+          stsv(L + L_S + (nlin + c) * 6, SV<T>{ew, cross(ew, rf)});
+        }
+      }
       if (t == JT_FREE)
         for (uint32_t c = 0; c < 3; c++) {
           V3<T> e{0, 0, 0}; comp(e, c) = 1;
@@ -1230,8 +1267,17 @@ template <class T> struct World {
       stsv(L + L_V, v);
       SV<T> cb = crm(vp, sq);
       uint32_t jt = link_type(l);
-      if (jt == JT_FREE) {
-        uint32_t nlin = 3;
+      if (jt == JT_D6) {
+        // rotation axes transported by earlier D6 angles: d(a_k)/dt = sum_{j<k} a_j x a_k qd_j (artwgsl.js:223-233)
+        uint32_t nlin = link_nlin(l);
+        V3<T> ca{0, 0, 0};
+        for (uint32_t j = nlin; j < link_ndof(l); j++)
+          for (uint32_t k2 = j + 1; k2 < link_ndof(l); k2++)
+            ca += cross(ldS(l, j).a, ldS(l, k2).a) * art[A_QD + link_dof(l) + j] * art[A_QD + link_dof(l) + k2];
+        cb = sadd(cb, SV<T>{ca, cross(ca, ld3(L + L_JR))});
+      }
+      if (jt == JT_FREE || jt == JT_D6) {
+        uint32_t nlin = jt == JT_FREE ? 3 : link_nlin(l);
         V3<T> vo{0, 0, 0}, wj{0, 0, 0};
         for (uint32_t c = 0; c < link_ndof(l); c++) {
           SV<T> sc = ldS(l, c);

@@ -123,6 +123,8 @@ struct W {
   DEV uint32_t link_ndof(uint32_t l) const { return tu(sf(lk(l) + 1).w); }
   DEV uint32_t link_qs(uint32_t l) const { return tu(sf(lk(l) + 3).w); }
   DEV float4 dof_drive(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5); }
+  DEV f3 dof_axis(uint32_t dd) const { return xyz(sf(H[W_SF_DOF] + dd * 5 + 3)); }
+  DEV uint32_t link_nlin(uint32_t l) const { return tu(sf(lk(l) + 5).w); }  // D6: leading translation dofs
   DEV float4 dof_limit(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 1); }
   DEV float4 dof_passive(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 2); }
   DEV bool scalar_dof(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 2).w > 0.5f; }
@@ -172,6 +174,34 @@ struct W {
       if (t == JT_REVOLUTE) mq = qaxis(axis, art_[AQ + qs]);
       if (t == JT_PRISMATIC) mp = axis * art_[AQ + qs];
       if (t == JT_BALL) mq = nrm4(mk4(art_[AQ + qs], art_[AQ + qs + 1], art_[AQ + qs + 2], art_[AQ + qs + 3]));
+      // D6: translations along the leading axes, then rotations about axes carried through the rotations before
+      // them (artwgsl.js:117-141)
+      const uint32_t d0 = link_dof(l), nlin = link_nlin(l), nang = link_ndof(l) - nlin;
+      f3 ang[3] = {mk3(0, 0, 0), mk3(0, 0, 0), mk3(0, 0, 0)};
+      if (t == JT_D6) {
+        for (uint32_t c = 0; c < nlin; c++) mp += dof_axis(d0 + c) * art_[AQ + qs + c];
+        const uint32_t iq = qs + nlin;
+        if (nang == 1) {
+          ang[0] = dof_axis(d0 + nlin);
+          mq = qaxis(ang[0], art_[AQ + iq]);
+        }
+        if (nang == 2) {
+          const f3 x0 = nrm(dof_axis(d0 + nlin));
+          const f3 x2 = nrm(cross(x0, dof_axis(d0 + nlin + 1)));
+          ang[0] = x0;
+          const q4 q0 = qaxis(x0, art_[AQ + iq]);
+          ang[1] = qrot(q0, cross(x2, x0));
+          mq = qmul(qaxis(ang[1], art_[AQ + iq + 1]), q0);
+        }
+        if (nang == 3) {
+          ang[0] = dof_axis(d0 + nlin);
+          const q4 q0 = qaxis(ang[0], art_[AQ + iq]);
+          ang[1] = qrot(q0, dof_axis(d0 + nlin + 1));
+          const q4 q1 = qaxis(ang[1], art_[AQ + iq + 1]);
+          ang[2] = qrot(qmul(q1, q0), dof_axis(d0 + nlin + 2));
+          mq = qmul(qaxis(ang[2], art_[AQ + iq + 2]), qmul(q1, q0));
+        }
+      }
       if (t == JT_FREE) {
         mp = mk3(art_[AQ + qs], art_[AQ + qs + 1], art_[AQ + qs + 2]);
         mq = nrm4(mk4(art_[AQ + qs + 3], art_[AQ + qs + 4], art_[AQ + qs + 5], art_[AQ + qs + 6]));
@@ -198,6 +228,13 @@ struct W {
         }
       f3 rf = com - cp;
       st3(L + L_JR, rf);
+      if (t == JT_D6) {
+        for (uint32_t c = 0; c < nlin; c++) stsv(L + L_S + c * 6, mksv(mk3(0, 0, 0), qrot(jq, dof_axis(d0 + c))));
+        for (uint32_t c = 0; c < nang; c++) {
+          const f3 ew = qrot(jq, ang[c]);
+          stsv(L + L_S + (nlin + c) * 6, mksv(ew, cross(ew, rf)));
+        }
+      }
       if (t == JT_FREE)
         for (uint32_t c = 0; c < 3; c++) {
           f3 e = mk3(0, 0, 0); setc(e, c, 1.0f);
@@ -252,12 +289,22 @@ struct W {
     SVd v = sadd(vp, sq);
     stsv(L + L_V, v);
     SVd cb = crm(vp, sq);
-    if (link_type(l) == JT_FREE) {
+    const uint32_t jt = link_type(l);
+    if (jt == JT_D6) {
+      // rotation axes transported by earlier D6 angles: d(a_k)/dt = sum_{j<k} a_j x a_k qd_j (artwgsl.js:223-233)
+      const uint32_t nlin = link_nlin(l), n = link_ndof(l), dd = link_dof(l);
+      f3 ca = mk3(0, 0, 0);
+      for (uint32_t j = nlin; j < n; j++)
+        for (uint32_t k = j + 1; k < n; k++) ca += cross(ldS(l, j).a, ldS(l, k).a) * art_[AQD + dd + j] * art_[AQD + dd + k];
+      cb = sadd(cb, mksv(ca, cross(ca, ld3(L + L_JR))));
+    }
+    if (jt == JT_FREE || jt == JT_D6) {
+      const uint32_t nlin = jt == JT_FREE ? 3u : link_nlin(l);
       f3 vo = mk3(0, 0, 0), wj = mk3(0, 0, 0);
       for (uint32_t c = 0; c < link_ndof(l); c++) {
         SVd sc = ldS(l, c);
         float q = art_[AQD + link_dof(l) + c];
-        if (c < 3) vo += sc.l * q; else wj += sc.a * q;
+        if (c < nlin) vo += sc.l * q; else wj += sc.a * q;
       }
       cb = sadd(cb, mksv(mk3(0, 0, 0), -cross(wj, vo)));
     }

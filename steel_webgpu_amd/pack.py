@@ -186,7 +186,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
             l = len(links)
             localLink[b] = l
             parentLink = localLink[J["parent"][j]] if J["parent"][j] >= 0 else -1
-            if t not in (JT["FIXED"], JT["REVOLUTE"], JT["PRISMATIC"], JT["BALL"], JT["FREE"]):
+            if t not in (JT["FIXED"], JT["REVOLUTE"], JT["PRISMATIC"], JT["BALL"], JT["FREE"], JT["D6"]):
                 raise UnsupportedError(f"joint type {t} is not supported yet")
             ndof = J["qd_start"][j + 1] - J["qd_start"][j]
             nq = J["q_start"][j + 1] - J["q_start"][j]
@@ -475,7 +475,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         statF[o4:o4 + 4] = [l["type"], l["parent"], l["dof"], l["body"]]
         statF[o4 + 4:o4 + 12] = [xp[0], xp[1], xp[2], l["ndof"], xp[3], xp[4], xp[5], xp[6]]
         statF[o4 + 12:o4 + 20] = [xc[0], xc[1], xc[2], l["qs"], xc[3], xc[4], xc[5], xc[6]]
-        statF[o4 + 20:o4 + 24] = [ax[0], ax[1], ax[2], 0]
+        statF[o4 + 20:o4 + 24] = [ax[0], ax[1], ax[2], l["nlin"] if l["type"] == JT["D6"] else 0]
     for i, d in enumerate(dofs):
         o4 = (SF["SF_DOF"] + i * 5) * 4
         statF[o4:o4 + 20] = [d["armature"], d["ke"], d["kd"], d["effort"], d["lower"], d["upper"], d["limited"], d["link"],

@@ -64,6 +64,24 @@ def _with_torsion(scene, radius=0.02):
     return scene
 
 
+def d6_rig():
+    """A floating box with three D6-jointed boxes (three rotations; two rotations; two translations and a rotation)
+    dropping onto the ground."""
+    from steel_webgpu_amd.scene import JT
+    X, Y, Z = [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane()
+    base = b.addBody(xform=[0, 0, 0.5, 0, 0, 0, 1])
+    b.addShapeBox(base, hx=0.3, hy=0.1, hz=0.05)
+    b.addJointFree(base)
+    for k, (axes, lin, q) in enumerate([([X, Y, Z], 0, [0.3, 0.2, 0.1]), ([X, Y], 0, [0.4, -0.3]), ([X, Y, Z], 2, [0.05, 0.02, 0.5])]):
+        x = -0.25 + 0.25 * k
+        c = b.addBody(xform=[x, 0, 0.5, 0, 0, 0, 1])
+        b.addShapeBox(c, hx=0.04, hy=0.04, hz=0.1, xform=[0, 0, -0.15, 0, 0, 0, 1])
+        b.addJoint(JT["D6"], base, c, parentXform=[x, 0, 0, 0, 0, 0, 1], axes=axes, linear=lin, q=q)
+    return b.finalize(solver=dict(substeps=4))
+
+
 def _with_mimics(scene):
     """Each leg's third joint mimics its second (q = 0.1 - 1.2 q_leader)."""
     J = scene.json["joints"]
@@ -131,6 +149,7 @@ SCENES = {
     "boxes_nest": lambda: _with_nesterov(workloads.boxes64()),
     "quad_nest": lambda: _with_nesterov(workloads.quadruped_world(13), warm=True),
     "quad_tors": lambda: _with_torsion(workloads.quadruped_world(14)),
+    "d6": d6_rig,
     # a recorded 60 Hz gait driving the legs (base dofs of the path are 0; their drives are off)
     "quad_path": lambda: workloads.with_target_path(workloads.quadruped_world(8, drive=dict(targetKe=40.0, targetKd=1.0)),
                                                     workloads.gait_path(), integral=8),
@@ -173,7 +192,7 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
                                          ("boxes_limits", 150), ("quad_friction", 200), ("bighulls", 120),
                                          ("quad_path", 200), ("boxes_sleep", 400), ("quad_sleep", 300),
                                          ("quad_mimic", 200), ("boxes_warm", 150), ("quad_warm", 200),
-                                         ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200)])
+                                         ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200), ("d6", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])

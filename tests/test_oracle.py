@@ -1238,3 +1238,69 @@ def test_contact_torsion_spins_down_at_radius_times_coulomb_budget():
                 assert abs(w_1) <= 1e-12, k  # stopped within the bound, and held
                 free += 1
         assert free > 20 and (braked > 100 or tr == 0)
+
+
+def _qax(ax, a):
+    ax = np.asarray(ax, float) / np.linalg.norm(ax)
+    return [*(ax * math.sin(a / 2)), math.cos(a / 2)]
+
+
+def _qmul(p, q):
+    px, py, pz, pw_ = p
+    qx, qy, qz, qw = q
+    return [pw_ * qx + px * qw + py * qz - pz * qy, pw_ * qy - px * qz + py * qw + pz * qx,
+            pw_ * qz + px * qy - py * qx + pz * qw, pw_ * qw - px * qx - py * qy - pz * qz]
+
+
+def _jointed_box(joints, frame=(0, 0, 0, 0, 0, 0, 1)):
+    """A box (COM 0.3 below its frame) hanging from the world origin through `joints`: a list of (type, axes, linear,
+    q), chained through tiny-mass intermediate bodies; returns the packed world and the box's body index."""
+    from steel_webgpu_amd.scene import JT
+    b = SceneBuilder(fps=240)
+    parent = -1
+    for k, (t, axes, lin, q) in enumerate(joints):
+        last = k == len(joints) - 1
+        body = b.addBody(xform=[0, 0, 0, 0, 0, 0, 1], **({} if last else dict(mass=1e-6, com=[0, 0, 0], inertia=[1e-9, 0, 0, 0, 1e-9, 0, 0, 0, 1e-9])))
+        if last:
+            b.addShapeBox(body, hx=0.05, hy=0.08, hz=0.12, xform=[0, 0, -0.3, 0, 0, 0, 1])
+        b.addJoint(JT[t], parent, body, parentXform=list(frame) if k == 0 else [0, 0, 0, 0, 0, 0, 1], axes=axes, linear=lin, q=q)
+        parent = body
+    return pack_scene(b.finalize(solver=dict(substeps=4))), parent
+
+
+def _track(pw, body, frames, qd=None):
+    o = OracleWorld(pw, f64=True)
+    if qd is not None:
+        o.art[pw.A["A_QD"]:pw.A["A_QD"] + len(qd)] = qd
+    P = pw.h("W_P_BODY") * 4
+    out = []
+    for f in range(frames):
+        o.frames(1)
+        out.append(np.concatenate([o.poses[P + body * 8:P + body * 8 + 3], o.poses[P + body * 8 + 4:P + body * 8 + 8] * np.sign(o.poses[P + body * 8 + 7])]))
+    return np.array(out)
+
+
+def test_d6_joints_move_like_the_equivalent_ball_universal_and_planar_chains():
+    """A D6 joint is Newton's stack of translations then rotations about carried axes.  Three rotations swing a
+    hanging box exactly like a ball joint (same initial orientation, intrinsic x then y); two rotations like a
+    revolute-revolute chain; two translations and a rotation like a prismatic-prismatic-revolute chain (tiny-mass
+    intermediate links; started sliding and turning, qd = (1, 0, 3), the same coordinates in both).  FK, the motion
+    subspace and its velocity-product terms all enter: the trajectories agree to the integrators' O(dt) difference
+    while the box moves by tenths of a meter."""
+    X, Y, Z = [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]
+    a0, a1 = 0.5, 0.35
+    cases = [
+        ([("D6", [X, Y, Z], 0, [a0, a1, 0.0])], [("BALL", [X, Y, Z], 0, _qmul(_qax(X, a0), _qax(Y, a1)))], (0, 0, 0, 0, 0, 0, 1)),
+        ([("D6", [X, Y], 0, [a0, a1])], [("REVOLUTE", [X], 0, [a0]), ("REVOLUTE", [Y], 0, [a1])], (0, 0, 0, 0, 0, 0, 1)),
+        ([("D6", [X, Y, Z], 2, [0.0, 0.0, a0])], [("PRISMATIC", [X], 1, [0.0]), ("PRISMATIC", [Y], 1, [0.0]), ("REVOLUTE", [Z], 0, [a0])],
+         (0, 0, 0, *_qax(X, math.pi / 2))),
+    ]
+    for i, (d6, chain, frame) in enumerate(cases):
+        qd = [1.0, 0.0, 3.0] if i == 2 else None
+        pa, ba = _jointed_box(d6, frame)
+        pb, bb = _jointed_box(chain, frame)
+        ta, tb = _track(pa, ba, 120, qd), _track(pb, bb, 120, qd)
+        swing = np.abs(ta[:, :3] - ta[0, :3]).max()
+        assert swing > 0.05, (d6, swing)
+        err = np.abs(ta - tb).max()
+        assert err < 0.01 * swing, (d6, err, swing)
