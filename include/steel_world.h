@@ -63,6 +63,8 @@ enum steel_world_field {
   W_NEST, W_MO,
   /* contact torsion of articulated units (u32 flag) and its row-record offset (u32) */
   W_TORS, W_TO,
+  /* velocity iterations after the position sweeps (u32) and the art offset of their velocity snapshot (u32) */
+  W_VELITERS, W_A_SNAP,
   W_FIELD_COUNT
 };
 

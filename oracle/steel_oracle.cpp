@@ -68,7 +68,8 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t WH, NEST, MO, TORS, TO;
+  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP;
+  bool vpass = false;  // velocity pass: rows without position bias or regularization
   T mom_k = 0, mom_beta = 0;  // Nesterov restart counter and extrapolation factor of the current substep
   uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
   T FPS, FRAME_DT, F_SLEEP_LIN, F_SLEEP_ANG, F_SLEEP_TIME;
@@ -119,7 +120,7 @@ template <class T> struct World {
     A_Q = h[W_A_Q]; A_QD = h[W_A_QD]; A_QDH = h[W_A_QDH]; A_TGT = h[W_A_TGT]; A_TGTV = h[W_A_TGTV]; A_LIM = h[W_A_LIM]; A_DRV = h[W_A_DRV];
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
-    WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO];
+    WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO]; VELITERS = h[W_VELITERS]; A_SNAP = h[W_A_SNAP];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
     TGT_HELD = h[W_TGT_HELD]; A_INT = h[W_A_INT]; FPS = hf(W_FPS); FRAME_DT = hf(W_FRAME_DT);
@@ -2523,6 +2524,47 @@ template <class T> struct World {
       art_velocity_limits(a);
     }
   }
+  // Twists of the velocity pass's snapshot (wgsl.js:2930-2934).
+  void snap_tw(int b, V3<T> out[2]) const {
+    if (b < 0) { out[0] = {0, 0, 0}; out[1] = {0, 0, 0}; return; }
+    out[0] = ld3(A_SNAP + (uint32_t)b * 8); out[1] = ld3(A_SNAP + (uint32_t)b * 8 + 4);
+  }
+  // Velocity pass (velocity_iterations, wgsl.js:3700-3736): positions integrate with the velocities reached by the
+  // position sweeps (kept as a snapshot); bias-free sweeps from the accumulated impulses then set the output
+  // velocities.  Contact right-hand sides become 0, or a speculative gap still open at the snapshot velocity.
+  void velocity_pass(uint32_t ncol, const std::vector<uint32_t>& wcs, const std::vector<uint32_t>& slot, uint32_t nu) {
+    for (uint32_t b = 0; b < NB; b++) { st3(A_SNAP + b * 8, xyz(tw[b * 2])); st3(A_SNAP + b * 8 + 4, xyz(tw[b * 2 + 1])); }
+    for (uint32_t i = 0; i < ND; i++) art[A_SNAP + NB * 8 + i] = aqd[i];
+    for (uint32_t us = 0; us < nu; us++) {
+      uint32_t key = S_UKEY[us];
+      int a = (int)(key & 0xffffu) - 1, b = (int)(key >> 16) - 1;
+      V3<T> ta[2], tb[2];
+      load_tw(a, ta); load_tw(b, tb);
+      uint32_t s0 = S_USTART[us];
+      for (uint32_t i = s0; i < std::min(s0 + S_USIZE[us], MAXC); i++) {
+        uint32_t o = i * ROWS;
+        T phi = rows[o + 6].w;
+        if (phi >= T(1e29)) continue;
+        V3<T> n = xyz(rows[o]);
+        T jv = dot(n, (tb[0] + cross(tb[1], xyz(rows[o + 4]))) - (ta[0] + cross(ta[1], xyz(rows[o + 3]))));
+        rows[o + 1].w = (phi > T(0) && phi + dt * jv > T(1e-6)) ? phi / dt : T(0);
+      }
+    }
+    uint32_t nd = std::min(S_META[3], MAXDENSE);
+    for (uint32_t i = 0; i < nd; i++) {
+      uint32_t o = A_DENSE + i * DPS;
+      if (art[o + 21] > T(0.5) || art[o + 22] >= T(1e29)) continue;
+      uint32_t ai[4];
+      art_info(tu(art[o]), ai);
+      T jv = 0;
+      for (uint32_t k = 0; k < ai[3]; k++) jv += art[o + DHDR + k] * aqd[ai[2] + k];
+      T phi = art[o + 22];
+      art[o + 4] = (phi > T(0) && phi + dt * jv > T(1e-6)) ? phi / dt : T(0);
+    }
+    vpass = true;
+    for (uint32_t it = 0; it < VELITERS; it++) sweep(1, ncol, ITERS + it, wcs, slot, nu);
+    vpass = false;
+  }
   void load_tw(int b, V3<T> out[2]) const {
     if (b < 0) { out[0] = {0, 0, 0}; out[1] = {0, 0, 0}; return; }
     out[0] = xyz(tw[(uint32_t)b * 2]); out[1] = xyz(tw[(uint32_t)b * 2 + 1]);
@@ -2618,7 +2660,7 @@ template <class T> struct World {
     } else {
       V3<T> vrel = (tb[0] + cross(tb[1], rb)) - (ta[0] + cross(ta[1], ra));
       T jv = dot(n, vrel);
-      T w = rows[o + 5].w;
+      T w = vpass ? T(1) : rows[o + 5].w;
       T res = jv + r1.w;
       T nl = mx(lam.x + F_OMEGA * (-res * r0.w * w - (T(1) - w) * lam.x), T(0));
       dn = nl - lam.x;
@@ -2639,7 +2681,7 @@ template <class T> struct World {
           T load = art[A_ULOAD + us];
           lim = r2.w * load;
           V3<T> e = xyz(rows[o + PO + 2]);
-          bias = {F_ANCHOR_BETA * dot(t1, e) / dt, F_ANCHOR_BETA * dot(t2, e) / dt};
+          if (!vpass) bias = {F_ANCHOR_BETA * dot(t1, e) / dt, F_ANCHOR_BETA * dot(t2, e) / dt};
           rows[o + PO + 2].w = lim;
         }
       }
@@ -2805,7 +2847,7 @@ template <class T> struct World {
             for (uint32_t k = l; k < n; k += 32) part[l] += art[jo + r * NDA + k] * aqd[ai[2] + k];
           }
           T jv = lane_sum32(part);
-          T rhs = r == 0 ? art[o + 4] : (r == 1 ? art[o + 22] : art[o + 23]);
+          T rhs = vpass ? T(0) : (r == 0 ? art[o + 4] : (r == 1 ? art[o + 22] : art[o + 23]));
           T dl = -F_OMEGA * (jv + rhs) * art[o + 5 + r];
           nlam[r] += dl;
           for (uint32_t k = 0; k < n; k++) aqd[ai[2] + k] += art[yo + r * NDA + k] * dl;
@@ -2823,7 +2865,7 @@ template <class T> struct World {
         for (uint32_t k = l; k < n; k += 32) part[l] += art[jo + k] * aqd[ai[2] + k];
       }
       T jv = lane_sum32(part);
-      T w = art[o + 24];
+      T w = vpass ? T(1) : art[o + 24];
       T nl = mx(lam.x + F_OMEGA * (-(jv + art[o + 4]) * art[o + 5] * w - (T(1) - w) * lam.x), T(0));
       T dn = nl - lam.x;
       T p1[32], p2[32];
@@ -2885,7 +2927,7 @@ template <class T> struct World {
         T gap = side == 0 ? q - lim.x : lim.y - q;
         T lam = art[A_LIM + 2 * d + side];
         T rhs = gap / dt;
-        if (gap < T(0)) rhs = mx(beta * gap / dt, -max_depen);
+        if (gap < T(0)) rhs = vpass ? T(0) : mx(beta * gap / dt, -max_depen);
         T nl = mx(lam - F_OMEGA * (sg * aqd[d] + rhs) / hinv, T(0));
         if (nl == lam) continue;
         art[A_LIM + 2 * d + side] = nl;
@@ -3072,6 +3114,7 @@ template <class T> struct World {
       if (NEST) momentum_step(ncol, it, wcs, slot);
       sweep(1, ncol, it, wcs, slot, nu);
     }
+    if (VELITERS) velocity_pass(ncol, wcs, slot, nu);
     uint32_t nc = S_USTART[nu];
     if (WH) warm_store(std::min(nc, MAXC));
     if (PATCH) for (uint32_t b = 0; b < NB * 2; b++) poses[P_PREV + b] = poses[P_BODY + b];
@@ -3081,13 +3124,17 @@ template <class T> struct World {
       V3<T> v = xyz(tw[b * 2]), w = xyz(tw[b * 2 + 1]);
       T damp = (body_flags((int)b) & 1) != 0 ? T(1) : T(1) - body_limits((int)b).x * dt;
       vel[b * 2] = v4(v, T(0)); vel[b * 2 + 1] = v4(w * damp, T(0));
+      if (VELITERS) { v = ld3(A_SNAP + b * 8); w = ld3(A_SNAP + b * 8 + 4); }
       V3<T> c = xyz(poses[P_BODY + b * 2]) + v * dt;
       V4<T> q = poses[P_BODY + b * 2 + 1];
       q = normalize4(q + (T(0.5) * dt) * qmul(v4(w, T(0)), q));
       poses[P_BODY + b * 2] = v4(c, T(0));
       poses[P_BODY + b * 2 + 1] = q;
     }
-    for (uint32_t i = 0; i < ND; i++) art[A_QD + i] = aqd[i];
+    for (uint32_t i = 0; i < ND; i++) {
+      art[A_QD + i] = aqd[i];
+      if (VELITERS) aqd[i] = art[A_SNAP + NB * 8 + i];
+    }
     for (uint32_t l = 0; l < NL; l++) {
       if (link_type(l) != JT_FREE || link_parent(l) >= 0) continue;
       T k = T(1) - body_limits(link_body(l)).x * dt;
@@ -3136,7 +3183,7 @@ template <class T> struct World {
         if (lim <= T(0)) continue;
         V3<T> n = xyz(rows[o]), t1 = xyz(rows[o + 1]), t2 = xyz(rows[o + 2]);
         V3<T> ta[2], tb[2];
-        load_tw(a, ta); load_tw(b, tb);
+        if (VELITERS) { snap_tw(a, ta); snap_tw(b, tb); } else { load_tw(a, ta); load_tw(b, tb); }
         V3<T> vrel = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
         V4<T> lam = rows[o + 6];
         V3<T> lt = t1 * lam.y + t2 * lam.z;

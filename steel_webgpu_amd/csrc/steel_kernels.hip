@@ -1438,6 +1438,7 @@ struct Solver {
   float4* tw;     // body twists (LDS, or the world's buffer for worlds too large for LDS)
   float4* imp;    // accumulated impulses on articulated bodies (same placement as tw)
   float* red;     // LDS partial sums of the momentum restart test (256 floats; Nesterov worlds only)
+  bool vpass = false;  // velocity pass: rows without position bias or regularization
   DEV int mpar(uint32_t l) const { return (int)lmeta[l].x; }
   DEV uint32_t mnd(uint32_t l) const { return lmeta[l].y & 0xffu; }
   DEV uint32_t mdof(uint32_t l) const { return lmeta[l].z; }
@@ -1481,6 +1482,12 @@ struct Solver {
   }
   DEV unsigned long long warm_order(float x, uint32_t i) const {
     return ((unsigned long long)(x > 0.0f ? fbits(x) + 1u : 1u) << 32) | (0xffffffffu - i);
+  }
+  // Twists of the velocity pass's snapshot (wgsl.js:2930-2934).
+  DEV void snap_tw(int b, f3 out[2]) const {
+    if (b < 0) { out[0] = mk3(0, 0, 0); out[1] = mk3(0, 0, 0); return; }
+    const uint32_t o = w.h(W_A_SNAP) + (uint32_t)b * 8;
+    out[0] = w.ld3(o); out[1] = w.ld3(o + 4);
   }
   DEV void load_tw(int b, f3 out[2]) const {
     if (b < 0) { out[0] = mk3(0, 0, 0); out[1] = mk3(0, 0, 0); return; }
@@ -1608,7 +1615,7 @@ struct Solver {
     } else {
       f3 vrel = (tb[0] + cross(tb[1], rb)) - (ta[0] + cross(ta[1], ra));
       float jv = dot(n, vrel);
-      float wgt = R[5].w;
+      float wgt = vpass ? 1.0f : R[5].w;
       float res = jv + r1.w;
       float omega = w.hf(W_F_OMEGA);
       float nl = mx(lam.x + omega * (-res * r0.w * wgt - (1.0f - wgt) * lam.x), 0.0f);
@@ -1631,7 +1638,7 @@ struct Solver {
           lim = r2.w * load;
           f3 e = xyz(R[PO + 2]);
           float beta = w.hf(W_F_ANCHOR_BETA), dt = w.hf(W_DT);
-          bias = mk2(beta * dot(t1, e) / dt, beta * dot(t2, e) / dt);
+          if (!vpass) bias = mk2(beta * dot(t1, e) / dt, beta * dot(t2, e) / dt);
           float4 ex = R[PO + 2]; ex.w = lim; R[PO + 2] = ex;
         }
       }
@@ -1842,7 +1849,7 @@ struct Solver {
           if (own0) part += jr0 * q0;
           if (own1) part += jr1 * q1;
           const float jv = lane_sum32(part);
-          const float rhs = r == 0 ? art_[o + 4] : (r == 1 ? art_[o + 22] : art_[o + 23]);
+          const float rhs = vpass ? 0.0f : (r == 0 ? art_[o + 4] : (r == 1 ? art_[o + 22] : art_[o + 23]));
           const float dl = -omega * (jv + rhs) * art_[o + 5 + r];
           nlam[r] += dl;
           if (own0) q0 += yr0 * dl;
@@ -1865,7 +1872,7 @@ struct Solver {
         if (own1) q1 += ya1 * lx + yb1 * ly + yc1 * lz;
         continue;
       }
-      const float rhs = art_[o + 4], meff = art_[o + 5], wgt = art_[o + 24], mu = art_[o + 3];
+      const float rhs = art_[o + 4], meff = art_[o + 5], wgt = vpass ? 1.0f : art_[o + 24], mu = art_[o + 3];
       const float k6 = art_[o + 6], k7 = art_[o + 7], k8 = art_[o + 8];
       float part = 0.0f;
       if (own0) part += ja0 * q0;
@@ -1936,7 +1943,7 @@ struct Solver {
           float gap = side == 0 ? q - lim.x : lim.y - q;
           float lam = art_[w.h(W_A_LIM) + 2 * dd + side];
           float rhs = gap / dt;
-          if (gap < 0.0f) rhs = mx(beta * gap / dt, -maxd);
+          if (gap < 0.0f) rhs = vpass ? 0.0f : mx(beta * gap / dt, -maxd);
           float nl = mx(lam - omega * (sg * laqd[dd] + rhs) / hinv, 0.0f);
           if (nl == lam) continue;
           art_[w.h(W_A_LIM) + 2 * dd + side] = nl;
@@ -2287,6 +2294,42 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     if (NEST) momentum_step<BS>(sv, ncol, it, mk, mbeta);
     sweep<BS>(sv, 1, ncol, nu, nd);
   }
+  if (const uint32_t VI = w.h(W_VELITERS)) {
+    // Velocity pass (velocity_iterations, wgsl.js:3700-3736): positions integrate with the velocities reached by
+    // the position sweeps (kept as a snapshot); bias-free sweeps from the accumulated impulses then set the output
+    // velocities.  Contact right-hand sides become 0, or a speculative gap still open at the snapshot velocity.
+    const uint32_t SN = w.h(W_A_SNAP), ROWS = w.ROWS();
+    const float dt = w.hf(W_DT);
+    for (uint32_t b = tid; b < NB; b += BS) { w.st3(SN + b * 8, xyz(sv.tw[b * 2])); w.st3(SN + b * 8 + 4, xyz(sv.tw[b * 2 + 1])); }
+    for (uint32_t i = tid; i < ND; i += BS) art_[SN + NB * 8 + i] = sv.laqd[i];
+    for (uint32_t us = tid; us < nu; us += BS) {
+      const uint32_t key = sched[S.UKEY + us];
+      const int a = (int)(key & 0xffffu) - 1, b = (int)(key >> 16) - 1;
+      f3 ta[2], tb[2];
+      sv.load_tw(a, ta); sv.load_tw(b, tb);
+      const uint32_t s0 = sched[S.USTART + us];
+      for (uint32_t i = s0; i < min(s0 + sched[S.USIZE + us], MAXC); i++) {
+        float4* R = wd.rows + (size_t)i * ROWS;
+        const float phi = R[6].w;
+        if (phi >= 1e29f) continue;
+        const float jv = dot(xyz(R[0]), (tb[0] + cross(tb[1], xyz(R[4]))) - (ta[0] + cross(ta[1], xyz(R[3]))));
+        R[1].w = (phi > 0.0f && phi + dt * jv > 1e-6f) ? phi / dt : 0.0f;
+      }
+    }
+    for (uint32_t i = tid; i < nd; i += BS) {
+      const uint32_t o = w.h(W_A_DENSE) + i * w.h(W_DPS);
+      if (art_[o + 21] > 0.5f || art_[o + 22] >= 1e29f) continue;
+      const uint4 ai = w.art_info(tu(art_[o]));
+      float jv = 0.0f;
+      for (uint32_t k = 0; k < ai.w; k++) jv += art_[o + w.h(W_DHDR) + k] * sv.laqd[ai.z + k];
+      const float phi = art_[o + 22];
+      art_[o + 4] = (phi > 0.0f && phi + dt * jv > 1e-6f) ? phi / dt : 0.0f;
+    }
+    __syncthreads();
+    sv.vpass = true;
+    for (uint32_t it = 0; it < VI; it++) sweep<BS>(sv, 1, ncol, nu, nd);
+    sv.vpass = false;
+  }
   // Mutable mirror fields back to the link blocks (p, dv, v, u, tau; S, U, D^-1, COM are unchanged).
   for (uint32_t i = tid; i < MF; i += BS) {
     uint32_t l = mirror_link(i), n = sv.mnd(l), f = i - (sv.lmeta[l].y >> 8);
@@ -2333,13 +2376,17 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     f3 v = xyz(sv.tw[b * 2]), wv = xyz(sv.tw[b * 2 + 1]);
     float damp = (w.body_flags((int)b) & 1) != 0 ? 1.0f : 1.0f - w.body_limits((int)b).x * dt;
     wd.vel[b * 2] = F4(v, 0.0f); wd.vel[b * 2 + 1] = F4(wv * damp, 0.0f);
+    if (w.h(W_VELITERS)) { v = w.ld3(w.h(W_A_SNAP) + b * 8); wv = w.ld3(w.h(W_A_SNAP) + b * 8 + 4); }
     f3 c = xyz(wd.poses[PB + b * 2]) + v * dt;
     q4 q = Q(wd.poses[PB + b * 2 + 1]);
     q = nrm4(q + (0.5f * dt) * qmul(mk4(wv, 0.0f), q));
     wd.poses[PB + b * 2] = F4(c, 0.0f);
     wd.poses[PB + b * 2 + 1] = F4(q);
   }
-  for (uint32_t i = tid; i < ND; i += BS) art_[w.h(W_A_QD) + i] = sv.laqd[i];
+  for (uint32_t i = tid; i < ND; i += BS) {
+    art_[w.h(W_A_QD) + i] = sv.laqd[i];
+    if (w.h(W_VELITERS)) sv.laqd[i] = art_[w.h(W_A_SNAP) + NB * 8 + i];
+  }
   __syncthreads();
   for (uint32_t l = tid; l < NL; l += BS) {
     if (w.link_type(l) != JT_FREE || w.link_parent(l) >= 0) continue;
@@ -2392,7 +2439,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       if (lim <= 0.0f) continue;
       f3 n = xyz(sv.row(i, 0)), t1 = xyz(sv.row(i, 1)), t2 = xyz(sv.row(i, 2));
       f3 ta[2], tb[2];
-      sv.load_tw(a, ta); sv.load_tw(b, tb);
+      if (w.h(W_VELITERS)) { sv.snap_tw(a, ta); sv.snap_tw(b, tb); } else { sv.load_tw(a, ta); sv.load_tw(b, tb); }
       f3 vrel = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
       float4 lam = sv.row(i, 6);
       f3 lt = t1 * lam.y + t2 * lam.z;

@@ -508,8 +508,6 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
               "contact_friction_shared_anchor"):
         if SV.get(k) or SO.get(k):
             raise UnsupportedError(f"{k} is not supported yet")
-    if SV.get("velocity_iterations", 0):
-        raise UnsupportedError("velocity_iterations is not supported yet")
     if SV.get("contact_friction_position_iterations", -1) >= 0:
         raise UnsupportedError("contact_friction_position_iterations is not supported yet")
     if SV.get("row_schedule") == "contact_then_internal" or SV.get("propagation_color_order") == "longest_first":
@@ -598,7 +596,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for k, v in P.items():
         H("W_" + k, v)
     for k in ("A_Q", "A_QD", "A_QDH", "A_TGT", "A_TGTV", "A_LIM", "A_DRV", "A_JF", "A_BF", "A_FRIC", "A_HINV", "A_LINK", "A_LG", "A_RL",
-              "A_RLN", "A_ULOAD", "A_DENSE", "A_VLG", "A_AVL", "A_JFL", "A_INT", "A_SLP"):
+              "A_RLN", "A_ULOAD", "A_DENSE", "A_VLG", "A_AVL", "A_JFL", "A_INT", "A_SLP", "A_SNAP"):
         H("W_" + k, A[k])
     H("W_VLIM_ON", 1 if vlimOn else 0)
     H("W_VLIM_FREE", 1 if vlimFree else 0)
@@ -615,6 +613,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_NEST", 1 if nest else 0)
     H("W_MO", MO)
     H("W_TORS", 1 if tors else 0)
+    H("W_VELITERS", int(SV.get("velocity_iterations") or 0))
     H("W_TO", TO)
     for i, m in enumerate(mimics):
         statF[(SF["SF_MIMIC"] + i * 2) * 4:(SF["SF_MIMIC"] + i * 2) * 4 + 8] = [m["art"], m["f"], m["l"], m["c1"], m["fq"], m["lq"], m["c0"], 0]

@@ -82,6 +82,11 @@ def d6_rig():
     return b.finalize(solver=dict(substeps=4))
 
 
+def _with_velocity_iterations(scene, n):
+    scene.json["solver"].update(velocity_iterations=n)
+    return scene
+
+
 def _with_mimics(scene):
     """Each leg's third joint mimics its second (q = 0.1 - 1.2 q_leader)."""
     J = scene.json["joints"]
@@ -150,6 +155,8 @@ SCENES = {
     "quad_nest": lambda: _with_nesterov(workloads.quadruped_world(13), warm=True),
     "quad_tors": lambda: _with_torsion(workloads.quadruped_world(14)),
     "d6": d6_rig,
+    "boxes_vel": lambda: _with_velocity_iterations(workloads.boxes64(), 4),
+    "quad_vel": lambda: _with_velocity_iterations(_with_mimics(workloads.quadruped_world(15)), 2),
     # a recorded 60 Hz gait driving the legs (base dofs of the path are 0; their drives are off)
     "quad_path": lambda: workloads.with_target_path(workloads.quadruped_world(8, drive=dict(targetKe=40.0, targetKd=1.0)),
                                                     workloads.gait_path(), integral=8),
@@ -192,7 +199,8 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
                                          ("boxes_limits", 150), ("quad_friction", 200), ("bighulls", 120),
                                          ("quad_path", 200), ("boxes_sleep", 400), ("quad_sleep", 300),
                                          ("quad_mimic", 200), ("boxes_warm", 150), ("quad_warm", 200),
-                                         ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200), ("d6", 200)])
+                                         ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200), ("d6", 200),
+                                         ("boxes_vel", 150), ("quad_vel", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])

@@ -1304,3 +1304,50 @@ def test_d6_joints_move_like_the_equivalent_ball_universal_and_planar_chains():
         assert swing > 0.05, (d6, swing)
         err = np.abs(ta - tb).max()
         assert err < 0.01 * swing, (d6, err, swing)
+
+
+def test_velocity_iterations_sphere_follows_the_two_phase_recursion():
+    """velocity_iterations = 2 on a resting sphere (one row, omega 0.5, one position iteration): the position
+    sweep's velocity v_s moves the sphere (z' = z + v_s dt); then, from the accumulated impulse, two bias-free
+    sweeps (row weight 1, rhs = phi / dt only for a speculative gap still open at v_s, else 0) set the output
+    velocity.  The FP64 oracle follows this substep by substep, and differs from the run without them."""
+    trajs = {}
+    for vi in (0, 2):
+        b = SceneBuilder(fps=240)
+        b.addGroundPlane(gap=0.01)
+        s = b.addBody(xform=[0, 0, 0.0985, 0, 0, 0, 1])
+        b.addShapeSphere(s, radius=0.1, gap=0.01)
+        pw = pack_scene(b.finalize(solver=dict(substeps=4, iterations=1, relaxation=0.5, velocity_iterations=vi)))
+        assert pw.h("W_VELITERS") == vi
+        r = float(np.float32(0.1))
+        o = OracleWorld(pw, f64=True)
+        SF = pw.h("W_SF_BODY")
+        im = pw.statF.reshape(-1, 4).astype(np.float64)[SF][0]
+        dt, gz, om = pw.hf("W_DT"), pw.hf("W_GZ"), pw.hf("W_F_OMEGA")
+        beta_b, spec, reg, cfm = pw.hf("W_BETA"), pw.hf("W_F_SPEC"), pw.hf("W_REG"), pw.hf("W_F_CFM")
+        eff = 1.0 / (im + cfm)
+        P = pw.h("W_P_BODY") * 4
+        z, v = float(o.poses[P + 2]), 0.0
+        zs = []
+        for k in range(160):
+            o.substeps(1, debug=True)
+            assert o.dbg_bufs["ncontacts"][0] == 1
+            phi = z - r
+            rhs = beta_b * phi / dt if phi < 0 else spec * phi / dt
+            w = 1.0 if (phi > 0 or reg <= 0) else 1.0 / (1.0 + reg)
+            v = v + gz * dt
+            lam = max(om * (-(v + rhs) * eff * w), 0.0)
+            v += lam * im
+            vs = v
+            if vi:
+                rv = phi / dt if (phi > 0 and phi + dt * v > 1e-6) else 0.0
+                for _ in range(vi):
+                    nl = max(lam + om * (-(v + rv) * eff), 0.0)
+                    v += (nl - lam) * im
+                    lam = nl
+            z = z + vs * dt
+            assert abs(o.poses[P + 2] - z) <= 1e-12, (vi, k)
+            assert abs(o.vel[2] - v) <= 1e-10, (vi, k)
+            zs.append(v)
+        trajs[vi] = np.array(zs)
+    assert np.abs(trajs[2] - trajs[0]).max() > 1e-4
