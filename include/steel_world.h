@@ -65,6 +65,8 @@ enum steel_world_field {
   W_TORS, W_TO,
   /* velocity iterations after the position sweeps (u32) and the art offset of their velocity snapshot (u32) */
   W_VELITERS, W_A_SNAP,
+  /* contact compliance (u32 flag: spring-damper normals, contacts kept whole) and its row-record offset (u32) */
+  W_COMPL, W_CO,
   W_FIELD_COUNT
 };
 

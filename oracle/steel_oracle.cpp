@@ -68,7 +68,7 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP;
+  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO;
   bool vpass = false;  // velocity pass: rows without position bias or regularization
   T mom_k = 0, mom_beta = 0;  // Nesterov restart counter and extrapolation factor of the current substep
   uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
@@ -120,7 +120,7 @@ template <class T> struct World {
     A_Q = h[W_A_Q]; A_QD = h[W_A_QD]; A_QDH = h[W_A_QDH]; A_TGT = h[W_A_TGT]; A_TGTV = h[W_A_TGTV]; A_LIM = h[W_A_LIM]; A_DRV = h[W_A_DRV];
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
-    WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO]; VELITERS = h[W_VELITERS]; A_SNAP = h[W_A_SNAP];
+    WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO]; VELITERS = h[W_VELITERS]; A_SNAP = h[W_A_SNAP]; COMPL = h[W_COMPL]; CO = h[W_CO];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
     TGT_HELD = h[W_TGT_HELD]; A_INT = h[W_A_INT]; FPS = hf(W_FPS); FRAME_DT = hf(W_FRAME_DT);
@@ -2213,7 +2213,7 @@ template <class T> struct World {
         uint32_t v[2]; bp_value(gg, q, v);
         if (v[1] != 0 && v[0] == S_SLOT[sl + q * 2] && v[1] == S_SLOT[sl + q * 2 + 1]) win = true;
       }
-      S_LEAD[p] = win ? 1 : 0;
+      S_LEAD[p] = (win || COMPL) ? 1 : 0;  // compliant contacts stay whole
     }
     for (uint32_t sp = 0; sp < nu; sp++) {
       uint32_t n = S_USIZE[sp], st = S_USTART[sp], kept = 0;
@@ -2288,8 +2288,22 @@ template <class T> struct World {
       T ln = 0;
       V2<T> lt{0, 0};
       warm_lookup(key, mu, t1, t2, ln, lt);
-      rows[o + 0] = v4(n, effn);
-      rows[o + 1] = v4(t1, rhs);
+      T effc = effn, rhsc = rhs;
+      if (COMPL) {
+        // contact_compliance: a spring-damper normal; the pair's ke and kd split over its kept points (wgsl.js:2194-2207)
+        T kx = C.q[4].x / (T)std::max(S_USIZE[sp], 1u), ky = C.q[4].y / (T)std::max(S_USIZE[sp], 1u);
+        T gam = 0;
+        if (kx > T(0)) {
+          T ce = phi <= T(0) ? ky : T(0);
+          T den = dt * kx + ce;
+          gam = T(1) / (dt * den);
+          rhsc = kx * phi / den;
+          if (effn > T(0)) effc = T(1) / (T(1) / effn + gam);
+        }
+        rows[o + CO] = {gam, 0, 0, 0};
+      }
+      rows[o + 0] = v4(n, effc);
+      rows[o + 1] = v4(t1, rhsc);
       rows[o + 2] = v4(t2, mu);
       rows[o + 3] = v4(ra, (T)(key & 0xffffu));
       rows[o + 4] = v4(rb, (T)(key >> 16));
@@ -2662,6 +2676,7 @@ template <class T> struct World {
       T jv = dot(n, vrel);
       T w = vpass ? T(1) : rows[o + 5].w;
       T res = jv + r1.w;
+      if (COMPL) res += rows[o + CO].x * lam.x;
       T nl = mx(lam.x + F_OMEGA * (-res * r0.w * w - (T(1) - w) * lam.x), T(0));
       dn = nl - lam.x;
       lam.x = nl;

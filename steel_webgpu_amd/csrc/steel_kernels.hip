@@ -746,8 +746,24 @@ struct Prep {
       bounce = restitution_fires(e, phi, vn0);
       if (bounce) rhs = e * vn0;
     }
-    set_row(idx, 0, F4(n, effn));
-    set_row(idx, 1, F4(t1, rhs));
+    float effc = effn, rhsc = rhs;
+    if (w.h(W_COMPL)) {
+      // contact_compliance: a spring-damper normal; the pair's ke and kd split over its kept points (wgsl.js:2194-2207)
+      const float4 kc4 = con(c, 4);
+      const float nk = (float)max(w.sch(w.S.USIZE)[sp], 1u);
+      const float kx = kc4.x / nk, ky = kc4.y / nk;
+      float gam = 0.0f;
+      if (kx > 0.0f) {
+        const float ce = phi <= 0.0f ? ky : 0.0f;
+        const float den = dt * kx + ce;
+        gam = 1.0f / (dt * den);
+        rhsc = kx * phi / den;
+        if (effn > 0.0f) effc = 1.0f / (1.0f / effn + gam);
+      }
+      set_row(idx, w.h(W_CO), make_float4(gam, 0.0f, 0.0f, 0.0f));
+    }
+    set_row(idx, 0, F4(n, effc));
+    set_row(idx, 1, F4(t1, rhsc));
     set_row(idx, 2, F4(t2, mu));
     set_row(idx, 3, F4(ra, (float)(key & 0xffffu)));
     set_row(idx, 4, F4(rb, (float)(key >> 16)));
@@ -1212,7 +1228,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       uint2 v = P.bp_value(g, q);
       if (v.y != 0 && v.x == sched[sl + q * 2] && v.y == sched[sl + q * 2 + 1]) win = true;
     }
-    sched[S.HEADS + p] = win ? 1u : 0u;
+    sched[S.HEADS + p] = (win || w.h(W_COMPL)) ? 1u : 0u;  // compliant contacts stay whole
   }
   __syncthreads();
   for (uint32_t sp = tid; sp < nu; sp += BS) {
@@ -1617,6 +1633,7 @@ struct Solver {
       float jv = dot(n, vrel);
       float wgt = vpass ? 1.0f : R[5].w;
       float res = jv + r1.w;
+      if (w.h(W_COMPL)) res += R[w.h(W_CO)].x * lam.x;
       float omega = w.hf(W_F_OMEGA);
       float nl = mx(lam.x + omega * (-res * r0.w * wgt - (1.0f - wgt) * lam.x), 0.0f);
       dn = nl - lam.x;

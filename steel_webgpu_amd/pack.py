@@ -246,8 +246,6 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
             add_mimic(m["follower"], m["leader"], m["c0"], m["c1"])
     if S.get("control"):
         raise UnsupportedError("scene controllers are not supported yet")
-    if any(kin):
-        raise UnsupportedError("kinematic bodies are not supported yet")
 
     # Hulls (dedup by their blob range), unique edges with their two faces (sim.js:180-209).
     shapes = [s for s in S["shapes"] if s.get("hull") or s.get("plane")]
@@ -365,7 +363,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         link = bodyLink[b]
         kind = 3 if (bodyKind[b] == 2 and not links[link]["movable"]) else bodyKind[b]
         pw.bodyKind[b] = kind
-        statF[o4] = 1 / m if m > 0 else 0
+        statF[o4] = 1 / m if m > 0 and not kin[b] else 0  # kinematic: infinite mass (sim.js:266-275)
         statF[o4 + 1] = kind
         statF[o4 + 2] = bodyArt[b]
         statF[o4 + 3] = link
@@ -375,7 +373,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         Ii = inv3(I)
         for r in range(3):
             statF[o4 + 8 + r * 4:o4 + 11 + r * 4] = I[r * 3:r * 3 + 3]
-            statF[o4 + 20 + r * 4:o4 + 23 + r * 4] = Ii[r * 3:r * 3 + 3]
+            statF[o4 + 20 + r * 4:o4 + 23 + r * 4] = [0, 0, 0] if kin[b] else Ii[r * 3:r * 3 + 3]
         bq = S["bodies"]["q"][b]
         q = [bq[3], bq[4], bq[5], bq[6]]
         c = quat_rotate(q, com)
@@ -504,8 +502,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for k, default in (("enable_bilateral_preelimination", False), ("propagation_same_articulation_rows", False), ("update_mass_matrix_interval", 1)):
         if SV.get(k) is not None and SV.get(k) != default:
             raise ValueError(f"{k}={SV[k]} is not supported")
-    for k in ("enable_torsional_rolling_friction", "contact_compliance", "contact_shared_anchor",
-              "contact_friction_shared_anchor"):
+    for k in ("enable_torsional_rolling_friction", "contact_shared_anchor", "contact_friction_shared_anchor"):
         if SV.get(k) or SO.get(k):
             raise UnsupportedError(f"{k} is not supported yet")
     if SV.get("contact_friction_position_iterations", -1) >= 0:
@@ -516,15 +513,17 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         raise UnsupportedError("drive_mode rows is not supported yet")
     if SV.get("free_rigid_torsional_friction", -1) > 0 or SV.get("free_rigid_rolling_friction", -1) > 0:
         raise UnsupportedError("free-body torsional/rolling friction is not supported yet")
+    compl = bool(SV.get("contact_compliance"))
     anchorBeta = SV.get("friction_anchor_beta", 0.2)
-    if anchorBeta is None:
+    if anchorBeta is None or compl:  # contact_compliance turns anchors off (sim.js:421-422)
         anchorBeta = 0.0
     PO = 19
     FMO = PO + (4 if anchorBeta > 0 else 0)
     nest = bool(SV.get("nesterov_momentum"))
     tors = any((s.get("torsion_radius") or 0) > 0 for s in S["shapes"])
     TO = FMO
-    MO = TO + (5 if tors else 0)
+    CO = TO + (5 if tors else 0)
+    MO = CO + (1 if compl else 0)
     ROWS = MO + (2 if nest else 0)
     P = dict(P_BODY=0, P_SHAPE=NB * 2, P_AABB=NB * 2 + NS * 2, P_PREV=NB * 2 + NS * 4)
     maxp, maxc, maxu, uh, ah, maxdense = capacities(NS, NB, NA, capacity_policy)
@@ -614,6 +613,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_MO", MO)
     H("W_TORS", 1 if tors else 0)
     H("W_VELITERS", int(SV.get("velocity_iterations") or 0))
+    H("W_COMPL", 1 if compl else 0)
+    H("W_CO", CO)
     H("W_TO", TO)
     for i, m in enumerate(mimics):
         statF[(SF["SF_MIMIC"] + i * 2) * 4:(SF["SF_MIMIC"] + i * 2) * 4 + 8] = [m["art"], m["f"], m["l"], m["c1"], m["fq"], m["lq"], m["c0"], 0]

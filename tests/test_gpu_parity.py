@@ -82,6 +82,23 @@ def d6_rig():
     return b.finalize(solver=dict(substeps=4))
 
 
+def kinematic_turntable(n=6):
+    """A kinematic slab spinning at 1.2 rad/s (prescribed twist, infinite mass, no gravity) carrying boxes."""
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane()
+    t = b.addBody(xform=[0, 0, 0.05, 0, 0, 0, 1], kinematic=True, twist=(0, 0, 0, 0, 0, 1.2))
+    b.addShapeBox(t, hx=0.6, hy=0.6, hz=0.05)
+    for k in range(n):
+        bx = b.addBody(xform=[0.12 * (k - 2.5), 0.03 * k, 0.15 + 0.001 * k, 0, 0, 0, 1])
+        b.addShapeBox(bx, hx=0.05, hy=0.05, hz=0.05)
+    return b.finalize(solver=dict(substeps=4))
+
+
+def _with_solver(scene, **opts):
+    scene.json["solver"].update(opts)
+    return scene
+
+
 def _with_velocity_iterations(scene, n):
     scene.json["solver"].update(velocity_iterations=n)
     return scene
@@ -156,6 +173,9 @@ SCENES = {
     "quad_tors": lambda: _with_torsion(workloads.quadruped_world(14)),
     "d6": d6_rig,
     "boxes_vel": lambda: _with_velocity_iterations(workloads.boxes64(), 4),
+    "kinematic": kinematic_turntable,
+    "boxes_compl": lambda: _with_solver(workloads.boxes64(n=40), contact_compliance=True),
+    "quad_compl": lambda: _with_solver(workloads.quadruped_world(16), contact_compliance=True),
     "quad_vel": lambda: _with_velocity_iterations(_with_mimics(workloads.quadruped_world(15)), 2),
     # a recorded 60 Hz gait driving the legs (base dofs of the path are 0; their drives are off)
     "quad_path": lambda: workloads.with_target_path(workloads.quadruped_world(8, drive=dict(targetKe=40.0, targetKd=1.0)),
@@ -200,7 +220,8 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
                                          ("quad_path", 200), ("boxes_sleep", 400), ("quad_sleep", 300),
                                          ("quad_mimic", 200), ("boxes_warm", 150), ("quad_warm", 200),
                                          ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200), ("d6", 200),
-                                         ("boxes_vel", 150), ("quad_vel", 200)])
+                                         ("boxes_vel", 150), ("quad_vel", 200), ("kinematic", 200),
+                                         ("boxes_compl", 150), ("quad_compl", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])

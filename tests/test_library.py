@@ -98,14 +98,11 @@ def test_capacities_are_exact_bounds():
 
 def test_unsupported_options_raise():
     with pytest.raises(pack.UnsupportedError):
-        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"contact_compliance": True}})
+        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"drive_mode": "rows"}})
     with pytest.raises(ValueError):
         pack.pack_scene(workloads.boxes64(n=4), opts={"substeps": 64})
-    b = SceneBuilder()
-    bd = b.addBody(kinematic=True)
-    b.addShapeBox(bd)
     with pytest.raises(pack.UnsupportedError):
-        pack.pack_scene(b.finalize())
+        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"friction_mode": "bisection_desaxce"}})
 
 
 def test_convex_hull_faces_are_outward_ccw_planes():

@@ -1351,3 +1351,65 @@ def test_velocity_iterations_sphere_follows_the_two_phase_recursion():
             zs.append(v)
         trajs[vi] = np.array(zs)
     assert np.abs(trajs[2] - trajs[0]).max() > 1e-4
+
+
+def kinematic_turntable(n=3):
+    """A kinematic slab spinning at 1.2 rad/s about z (prescribed twist, infinite mass, no gravity) carrying boxes."""
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane()
+    t = b.addBody(xform=[0, 0, 0.05, 0, 0, 0, 1], kinematic=True, twist=(0, 0, 0, 0, 0, 1.2))
+    b.addShapeBox(t, hx=0.6, hy=0.6, hz=0.05)
+    for k in range(n):
+        bx = b.addBody(xform=[0.15 + 0.15 * k, 0.02 * k, 0.15 + 0.001 * k, 0, 0, 0, 1])
+        b.addShapeBox(bx, hx=0.05, hy=0.05, hz=0.05)
+    return b.finalize(solver=dict(substeps=4))
+
+
+def test_kinematic_body_keeps_its_prescribed_twist_and_carries_what_rests_on_it():
+    """A kinematic body has infinite mass and no gravity: contacts and the boxes' weight leave its twist exactly as
+    prescribed, its COM stays put (zero linear velocity) and its yaw advances by w t (quaternion integration of a
+    constant rate, to O(dt^2) per step); boxes resting on it are carried round by friction until they turn with
+    it, at w r about the axis."""
+    pw = pack_scene(kinematic_turntable())
+    o = OracleWorld(pw, f64=True)
+    dt = pw.hf("W_DT")
+    P = pw.h("W_P_BODY") * 4
+    c0 = o.poses[P:P + 3].copy()
+    n = 4 * 240
+    o.substeps(n)
+    w = float(np.float32(1.2))
+    assert np.array_equal(o.vel[:3], [0, 0, 0]) and o.vel[6] == w and o.vel[4] == 0 and o.vel[5] == 0
+    assert np.abs(o.poses[P:P + 3] - c0).max() < 1e-12
+    q = o.poses[P + 4:P + 8]
+    yaw = 2 * math.atan2(q[2], q[3])
+    assert abs(yaw - w * n * dt) < 1e-3
+    for k in range(3):
+        v = o.vel[(1 + k) * 8:(1 + k) * 8 + 3]
+        c = o.poses[P + (1 + k) * 8:P + (1 + k) * 8 + 3]
+        expect = w * np.array([-c[1], c[0], 0.0])
+        assert np.abs(v - expect).max() < 0.02 * np.linalg.norm(expect) + 1e-3, (k, v, expect)
+
+
+def test_compliant_contact_settles_at_the_static_spring_deflection():
+    """contact_compliance: the normal row is an implicit spring-damper (ke of the two shapes in series, kd summed)
+    whose fixed point with the sphere at rest is the static deflection phi = -m g / ke; without compliance the
+    rigid contact settles at phi = 0 (Baumgarte)."""
+    for compl in (True, False):
+        b = SceneBuilder(fps=240)
+        b.addGroundPlane(gap=0.01, ke=2e4, kd=50.0)
+        s = b.addBody(xform=[0, 0, 0.05, 0, 0, 0, 1])
+        b.addShapeSphere(s, radius=0.05, gap=0.01, ke=2e4, kd=50.0)
+        pw = pack_scene(b.finalize(solver=dict(substeps=4, iterations=4, contact_compliance=compl)))
+        assert pw.h("W_COMPL") == int(compl) and pw.h("W_PATCH") == int(not compl)
+        o = OracleWorld(pw, f64=True)
+        o.substeps(4 * 480)
+        SF = pw.h("W_SF_BODY")
+        m = 1.0 / pw.statF.reshape(-1, 4).astype(np.float64)[SF][0]
+        g = -pw.hf("W_GZ")
+        k = float(np.float32(2e4)) * float(np.float32(2e4)) / (2 * float(np.float32(2e4)))
+        phi = o.poses[pw.h("W_P_BODY") * 4 + 2] - float(np.float32(0.05))
+        assert abs(o.vel[2]) < 1e-9
+        if compl:
+            assert abs(phi + m * g / k) <= 1e-6 * m * g / k, (phi, -m * g / k)
+        else:
+            assert abs(phi) < 1e-6 * m * g / k
