@@ -67,6 +67,9 @@ enum steel_world_field {
   W_VELITERS, W_A_SNAP,
   /* contact compliance (u32 flag: spring-damper normals, contacts kept whole) and its row-record offset (u32) */
   W_COMPL, W_CO,
+  /* drive_mode rows: drives as velocity rows in the sweeps instead of the implicit factor term (u32), velocity
+     limits of driven dofs left to the sweep clamp (u32), drive rows active in the velocity pass (u32) */
+  W_DRIVE_ROWS, W_FUSE_VLIM, W_VEL_DRIVE_ACTIVE,
   W_FIELD_COUNT
 };
 

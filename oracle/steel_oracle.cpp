@@ -68,7 +68,7 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO;
+  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE;
   bool vpass = false;  // velocity pass: rows without position bias or regularization
   T mom_k = 0, mom_beta = 0;  // Nesterov restart counter and extrapolation factor of the current substep
   uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
@@ -121,6 +121,7 @@ template <class T> struct World {
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
     WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO]; VELITERS = h[W_VELITERS]; A_SNAP = h[W_A_SNAP]; COMPL = h[W_COMPL]; CO = h[W_CO];
+    DRIVE_ROWS = h[W_DRIVE_ROWS]; FUSE_VLIM = h[W_FUSE_VLIM]; VEL_DRIVE_ACTIVE = h[W_VEL_DRIVE_ACTIVE];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
     TGT_HELD = h[W_TGT_HELD]; A_INT = h[W_A_INT]; FPS = hf(W_FPS); FRAME_DT = hf(W_FRAME_DT);
@@ -188,6 +189,9 @@ template <class T> struct World {
   uint32_t link_ndof(uint32_t l) const { return tu(statF[SF_LINK + l * 6 + 1].w); }
   uint32_t link_qs(uint32_t l) const { return tu(statF[SF_LINK + l * 6 + 3].w); }
   V4<T> dof_drive(uint32_t d) const { return statF[SF_DOF + d * 5]; }
+  bool driven_dof(uint32_t d) const { V4<T> dr = dof_drive(d); return dr.y > T(0) || dr.z > T(0); }
+  // drive_mode rows: a driven dof's velocity limit is left to the sweep clamp (artwgsl.js:54)
+  bool fused_dof(uint32_t d) const { return DRIVE_ROWS && FUSE_VLIM && driven_dof(d); }
   V3<T> dof_axis(uint32_t d) const { return xyz(statF[SF_DOF + d * 5 + 3]); }
   uint32_t link_nlin(uint32_t l) const { return tu(statF[SF_LINK + l * 6 + 5].w); }  // D6: leading translation dofs
   V4<T> dof_limit(uint32_t d) const { return statF[SF_DOF + d * 5 + 1]; }
@@ -1320,7 +1324,7 @@ template <class T> struct World {
         V4<T> sp = statF[SF_DOF + d * 5 + 4];
         if (sp.x != T(0)) tau += sp.x * (sp.y - art[A_Q + link_qs(l) + r]);
         // Implicit PD drive toward (A_TGT, A_TGTV), saturated on the implicit torque (artwgsl.js:301-313).
-        if (dr.y > T(0) || dr.z > T(0)) {
+        if (!DRIVE_ROWS && (dr.y > T(0) || dr.z > T(0))) {
           T q = art[A_Q + link_qs(l) + r], qd = art[A_QD + d];
           T err = art[A_TGT + d] - q;
           T tv = art[A_TGTV + d];
@@ -2045,13 +2049,13 @@ template <class T> struct World {
         for (uint32_t k = 0; k < ai[3]; k++) {
           uint32_t d = ai[2] + k;
           T vl = dof_passive(d).z;
-          if (vl < T(0)) continue;
+          if (vl < T(0) || fused_dof(d)) continue;
           T s = std::fabs(art[A_QD + d]);
           if (s > vl) ratio = std::min(ratio, vl / s);
         }
         if (ratio < T(1))
           for (uint32_t k = 0; k < ai[3]; k++)
-            if (scalar_dof(ai[2] + k)) art[A_QD + ai[2] + k] *= ratio;
+            if (scalar_dof(ai[2] + k) && !fused_dof(ai[2] + k)) art[A_QD + ai[2] + k] *= ratio;
         bool any = false;
         for (uint32_t k = 0; k < ai[3]; k++) {
           uint32_t d = ai[2] + k;
@@ -2073,7 +2077,7 @@ template <class T> struct World {
           on = q <= lim.x + gg || q >= lim.y - gg;
         }
         art[A_LG + d] = on ? T(1) : T(0);
-        bool row = on || dof_passive(d).y > T(0) || JFL;
+        bool row = on || dof_passive(d).y > T(0) || JFL || (DRIVE_ROWS && driven_dof(d));
         if (row && scalar_dof(d)) { art[A_RL + ai[2] + n] = (T)d; n++; }
       }
       art[A_RLN + a] = (T)n;
@@ -2156,7 +2160,8 @@ template <class T> struct World {
     }
     for (uint32_t d = 0; d < ND; d++) {
       V4<T> lim = dof_limit(d);
-      if (lim.z > T(0.5) || dof_passive(d).y > T(0) || dof_passive(d).z >= T(0) || statF[SF_DOF + d * 5 + 3].w > T(0)) {
+      if (lim.z > T(0.5) || dof_passive(d).y > T(0) || dof_passive(d).z >= T(0) || statF[SF_DOF + d * 5 + 3].w > T(0) ||
+          (DRIVE_ROWS && driven_dof(d))) {
         uint32_t l = tu(lim.w);
         art[A_HINV + d] = dof_hinv(l, d - link_dof(l));
       }
@@ -2920,8 +2925,25 @@ template <class T> struct World {
     }
     if (hit) art_apply(a);
   }
-  void art_dof_rows(uint32_t a, uint32_t l, uint32_t r, uint32_t d) {  // wgsl.js:3421-3468, friction and limit rows
+  void art_dof_rows(uint32_t a, uint32_t l, uint32_t r, uint32_t d) {  // wgsl.js:3421-3468, drive, friction, limit rows
     T hinv = mx(art[A_HINV + d], T(1e-12));
+    V4<T> dr = dof_drive(d);
+    if (DRIVE_ROWS && driven_dof(d) && !(vpass && !VEL_DRIVE_ACTIVE)) {
+      // drive_mode rows: a PhysX-style drive row, cold-started each step, not relaxed (wgsl.js:3425-3438).
+      T q = art[A_Q + link_qs(l) + r];
+      T pd = vpass ? dt * art[A_SNAP + NB * 8 + d] : T(0);
+      T ca = dt * (dt * dr.y + dr.z);
+      T x = T(1) / (T(1) + ca * hinv);
+      T bias = x * dt * dr.z * art[A_TGTV + d] + dr.y * x * dt * (art[A_TGT + d] - q - pd);
+      T cap = dr.w > T(0) ? dr.w * dt : T(1e20);
+      T lam = art[A_DRV + d];
+      T nl = mn(mx(lam * (T(1) - x) - x * ca * aqd[d] + bias, -cap), cap);
+      if (nl != lam) {
+        art[A_DRV + d] = nl;
+        art[lo(l) + L_TAU + r] = nl - lam;
+        art_apply(a);
+      }
+    }
     // Joint friction: a velocity row with |impulse| <= friction * dt (plus the load term).
     T fr = dof_passive(d).y;
     if (JFL) fr += art[A_JFL + d];

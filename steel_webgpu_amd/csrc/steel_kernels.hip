@@ -1045,13 +1045,13 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       float ratio = 1.0f;
       for (uint32_t k = 0; k < ai.w; k++) {
         float vl = w.dof_passive(ai.z + k).z;
-        if (vl < 0.0f) continue;
+        if (vl < 0.0f || w.fused_dof(ai.z + k)) continue;
         float s = fabsf(art_[w.h(W_A_QD) + ai.z + k]);
         if (s > vl) ratio = mn(ratio, vl / s);
       }
       if (ratio < 1.0f)
         for (uint32_t k = 0; k < ai.w; k++)
-          if (w.scalar_dof(ai.z + k)) art_[w.h(W_A_QD) + ai.z + k] *= ratio;
+          if (w.scalar_dof(ai.z + k) && !w.fused_dof(ai.z + k)) art_[w.h(W_A_QD) + ai.z + k] *= ratio;
       const float frac = w.hf(W_F_VLIM_FRAC);
       bool any = false;
       for (uint32_t k = 0; k < ai.w; k++) {
@@ -1074,7 +1074,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
         on = q <= lim.x + gg || q >= lim.y - gg;
       }
       art_[w.h(W_A_LG) + dd] = on ? 1.0f : 0.0f;
-      bool rowf = on || w.dof_passive(dd).y > 0.0f || w.h(W_JFL) != 0;
+      bool rowf = on || w.dof_passive(dd).y > 0.0f || w.h(W_JFL) != 0 || (w.h(W_DRIVE_ROWS) && w.driven_dof(dd));
       if (rowf && w.scalar_dof(dd)) { art_[w.h(W_A_RL) + ai.z + n] = (float)dd; n++; }
     }
     art_[w.h(W_A_RLN) + a] = (float)n;
@@ -1168,7 +1168,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   }
   for (uint32_t dd = tid; dd < ND; dd += BS) {
     float4 lim = w.dof_limit(dd);
-    if (lim.z > 0.5f || w.dof_passive(dd).y > 0.0f || w.dof_passive(dd).z >= 0.0f || w.sf(w.h(W_SF_DOF) + dd * 5 + 3).w > 0.0f) {
+    if (lim.z > 0.5f || w.dof_passive(dd).y > 0.0f || w.dof_passive(dd).z >= 0.0f || w.sf(w.h(W_SF_DOF) + dd * 5 + 3).w > 0.0f ||
+        (w.h(W_DRIVE_ROWS) && w.driven_dof(dd))) {
       uint32_t l = tu(lim.w);
       art_[w.h(W_A_HINV) + dd] = w.dof_hinv(l, dd - w.link_dof(l));
     }
@@ -1941,6 +1942,23 @@ struct Solver {
       float4 lim = w.dof_limit(dd);
       uint32_t l = tu(lim.w), r = dd - w.link_dof(l);
       float hinv = mx(art_[w.h(W_A_HINV) + dd], 1e-12f);
+      const float4 dr = w.dof_drive(dd);
+      if (w.h(W_DRIVE_ROWS) && (dr.y > 0.0f || dr.z > 0.0f) && !(vpass && !w.h(W_VEL_DRIVE_ACTIVE))) {
+        // drive_mode rows (wgsl.js:3425-3438): a PhysX-style drive row, cold-started each step, not relaxed.
+        const float q = art_[w.h(W_A_Q) + w.link_qs(l) + r];
+        const float pd = vpass ? dt * art_[w.h(W_A_SNAP) + w.NB() * 8 + dd] : 0.0f;
+        const float ca = dt * (dt * dr.y + dr.z);
+        const float x = 1.0f / (1.0f + ca * hinv);
+        const float bias = x * dt * dr.z * art_[w.h(W_A_TGTV) + dd] + dr.y * x * dt * (art_[w.h(W_A_TGT) + dd] - q - pd);
+        const float cap = dr.w > 0.0f ? dr.w * dt : 1e20f;
+        const float lam = art_[w.h(W_A_DRV) + dd];
+        const float nl = clampf_(lam * (1.0f - x) - x * ca * laqd[dd] + bias, -cap, cap);
+        if (nl != lam) {
+          art_[w.h(W_A_DRV) + dd] = nl;
+          if (lane == 0) mTAU(l)[r] = nl - lam;
+          art_apply(a, lane);
+        }
+      }
       // Joint friction: a velocity row with |impulse| <= friction * dt (plus the load term).
       float fr = w.dof_passive(dd).y;
       if (w.h(W_JFL)) fr += art_[w.h(W_A_JFL) + dd];

@@ -123,6 +123,9 @@ struct W {
   DEV uint32_t link_ndof(uint32_t l) const { return tu(sf(lk(l) + 1).w); }
   DEV uint32_t link_qs(uint32_t l) const { return tu(sf(lk(l) + 3).w); }
   DEV float4 dof_drive(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5); }
+  DEV bool driven_dof(uint32_t dd) const { const float4 dr = dof_drive(dd); return dr.y > 0.0f || dr.z > 0.0f; }
+  // drive_mode rows: a driven dof's velocity limit is left to the sweep clamp (artwgsl.js:54)
+  DEV bool fused_dof(uint32_t dd) const { return H[W_DRIVE_ROWS] && H[W_FUSE_VLIM] && driven_dof(dd); }
   DEV f3 dof_axis(uint32_t dd) const { return xyz(sf(H[W_SF_DOF] + dd * 5 + 3)); }
   DEV uint32_t link_nlin(uint32_t l) const { return tu(sf(lk(l) + 5).w); }  // D6: leading translation dofs
   DEV float4 dof_limit(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 1); }
@@ -346,7 +349,7 @@ struct W {
       float4 sp = sf(H[W_SF_DOF] + dd * 5 + 4);
       if (sp.x != 0.0f) tau += sp.x * (sp.y - art_[H[W_A_Q] + link_qs(l) + r]);
       // Implicit PD drive toward (A_TGT, A_TGTV), saturated on the implicit torque (artwgsl.js:301-313).
-      if (dr.y > 0.0f || dr.z > 0.0f) {
+      if (!H[W_DRIVE_ROWS] && (dr.y > 0.0f || dr.z > 0.0f)) {
         float q = art_[H[W_A_Q] + link_qs(l) + r], qd = art_[AQD + dd];
         float err = art_[H[W_A_TGT] + dd] - q;
         float tv = art_[H[W_A_TGTV] + dd];

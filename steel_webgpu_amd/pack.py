@@ -509,8 +509,6 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         raise UnsupportedError("contact_friction_position_iterations is not supported yet")
     if SV.get("row_schedule") == "contact_then_internal" or SV.get("propagation_color_order") == "longest_first":
         raise UnsupportedError("row_schedule/propagation_color_order variants are not supported yet")
-    if SV.get("drive_mode") == "rows":
-        raise UnsupportedError("drive_mode rows is not supported yet")
     if SV.get("free_rigid_torsional_friction", -1) > 0 or SV.get("free_rigid_rolling_friction", -1) > 0:
         raise UnsupportedError("free-body torsional/rolling friction is not supported yet")
     compl = bool(SV.get("contact_compliance"))
@@ -614,6 +612,9 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_TORS", 1 if tors else 0)
     H("W_VELITERS", int(SV.get("velocity_iterations") or 0))
     H("W_COMPL", 1 if compl else 0)
+    H("W_DRIVE_ROWS", 1 if SV.get("drive_mode") == "rows" else 0)
+    H("W_FUSE_VLIM", 0 if SV.get("fuse_joint_velocity_limits") is False else 1)
+    H("W_VEL_DRIVE_ACTIVE", 1 if SV.get("velocity_drive_mode") == "active" else 0)
     H("W_CO", CO)
     H("W_TO", TO)
     for i, m in enumerate(mimics):

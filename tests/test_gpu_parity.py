@@ -174,6 +174,9 @@ SCENES = {
     "d6": d6_rig,
     "boxes_vel": lambda: _with_velocity_iterations(workloads.boxes64(), 4),
     "kinematic": kinematic_turntable,
+    # drive rows with velocity limits (driven dofs fused into the sweep clamp) and effort limits
+    "quad_drows": lambda: _with_solver(workloads.quadruped_world(17, drive=dict(targetKe=60.0, targetKd=2.0, velocityLimit=0.8, effortLimit=4.0)),
+                                       drive_mode="rows", enable_joint_velocity_limits=True),
     "boxes_compl": lambda: _with_solver(workloads.boxes64(n=40), contact_compliance=True),
     "quad_compl": lambda: _with_solver(workloads.quadruped_world(16), contact_compliance=True),
     "quad_vel": lambda: _with_velocity_iterations(_with_mimics(workloads.quadruped_world(15)), 2),
@@ -221,7 +224,7 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
                                          ("quad_mimic", 200), ("boxes_warm", 150), ("quad_warm", 200),
                                          ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200), ("d6", 200),
                                          ("boxes_vel", 150), ("quad_vel", 200), ("kinematic", 200),
-                                         ("boxes_compl", 150), ("quad_compl", 200)])
+                                         ("boxes_compl", 150), ("quad_compl", 200), ("quad_drows", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])

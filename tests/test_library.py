@@ -98,7 +98,7 @@ def test_capacities_are_exact_bounds():
 
 def test_unsupported_options_raise():
     with pytest.raises(pack.UnsupportedError):
-        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"drive_mode": "rows"}})
+        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"row_schedule": "contact_then_internal"}})
     with pytest.raises(ValueError):
         pack.pack_scene(workloads.boxes64(n=4), opts={"substeps": 64})
     with pytest.raises(pack.UnsupportedError):

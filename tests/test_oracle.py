@@ -1413,3 +1413,39 @@ def test_compliant_contact_settles_at_the_static_spring_deflection():
             assert abs(phi + m * g / k) <= 1e-6 * m * g / k, (phi, -m * g / k)
         else:
             assert abs(phi) < 1e-6 * m * g / k
+
+
+def test_drive_rows_reproduce_the_implicit_drive_and_clamp_at_effort_times_dt():
+    """drive_mode rows: each substep the drive row starts at zero and, with x = 1 / (1 + c_a / I), c_a = dt (dt K + C_d),
+    settles at x (dt C_d qd_tgt + dt K (q_tgt - q)) - x c_a qd (the first visit is already its fixed point), i.e.
+    qd' = (I qd + dt (K (q_tgt - q) + C_d qd_tgt)) / (I + dt C_d + dt^2 K): the implicit drive's backward-Euler
+    update.  So an isolated hinge follows the same trajectory under both drive modes; with a small effort limit
+    the row's impulse saturates at effort * dt, so |qd' - qd| = effort dt / I."""
+    ke, kd, tq, tv = 150.0, 4.0, 0.9, -0.5
+    runs = {}
+    for mode in ("implicit", "rows"):
+        scene = _vertical_hinge(targetKe=ke, targetKd=kd, targetPos=tq, targetVel=tv, effortLimit=0.0)
+        scene.json["solver"]["drive_mode"] = mode
+        pw = pack_scene(scene)
+        assert pw.h("W_DRIVE_ROWS") == int(mode == "rows")
+        o = OracleWorld(pw, f64=True)
+        qs = []
+        for k in range(240):
+            o.substeps(1)
+            qs.append((o.art[pw.A["A_Q"]], o.art[pw.A["A_QD"]]))
+        runs[mode] = np.array(qs)
+    assert np.abs(runs["rows"] - runs["implicit"]).max() < 1e-9
+    assert abs(runs["rows"][-1, 0] - runs["rows"][0, 0]) > 0.1  # it moved
+    # saturated: effort 0.05 N m
+    eff = 0.05
+    scene = _vertical_hinge(targetKe=ke, targetKd=kd, targetPos=tq, targetVel=tv, effortLimit=eff)
+    scene.json["solver"]["drive_mode"] = "rows"
+    H, _ = _crba(scene)
+    pw = pack_scene(scene)
+    o = OracleWorld(pw, f64=True)
+    dt = pw.hf("W_DT")
+    I = H[0, 0]
+    for k in range(20):
+        qd0 = o.art[pw.A["A_QD"]]
+        o.substeps(1)
+        assert abs(abs(o.art[pw.A["A_QD"]] - qd0) - _f32(eff) * dt / I) <= 1e-6 * _f32(eff) * dt / I, k
