@@ -70,6 +70,12 @@ enum steel_world_field {
   /* drive_mode rows: drives as velocity rows in the sweeps instead of the implicit factor term (u32), velocity
      limits of driven dofs left to the sweep clamp (u32), drive rows active in the velocity pass (u32) */
   W_DRIVE_ROWS, W_FUSE_VLIM, W_VEL_DRIVE_ACTIVE,
+  /* loop closures (ball or revolute joints closing inside one articulation or to the world): count (u32) and
+     statF section (u32, 5 vec4 each: parent link, child link, articulation, revolute flag; parent and child
+     anchors; parent and child hinge axes) */
+  W_NLOOP, W_SF_LOOP,
+  /* spin and rolling rows of free-body contacts (shapes with condim >= 4; u32 flag; rows 19-33, anchors at 34) */
+  W_ANG,
   W_FIELD_COUNT
 };
 

@@ -68,7 +68,7 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE;
+  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE, NLOOP, SF_LOOP, ANG;
   bool vpass = false;  // velocity pass: rows without position bias or regularization
   T mom_k = 0, mom_beta = 0;  // Nesterov restart counter and extrapolation factor of the current substep
   uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
@@ -121,6 +121,7 @@ template <class T> struct World {
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
     WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO]; VELITERS = h[W_VELITERS]; A_SNAP = h[W_A_SNAP]; COMPL = h[W_COMPL]; CO = h[W_CO];
+    NLOOP = h[W_NLOOP]; SF_LOOP = h[W_SF_LOOP]; ANG = h[W_ANG];
     DRIVE_ROWS = h[W_DRIVE_ROWS]; FUSE_VLIM = h[W_FUSE_VLIM]; VEL_DRIVE_ACTIVE = h[W_VEL_DRIVE_ACTIVE];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
@@ -1519,6 +1520,25 @@ template <class T> struct World {
     }
     for (uint32_t k = 0; k < 36; k++) art[lo(l) + L_W + k] = W[k];
   }
+  // Dense rows of the loop closures: three per closure, five for a revolute one (wgsl.js:2367-2368).
+  uint32_t loop_rows() const {
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < NLOOP; k++) n += statF[SF_LOOP + k * 5].w > T(0.5) ? 2 : 1;
+    return n;
+  }
+  // Angular Jacobian rows d0, d1 of link x's world angular velocity over its root path (artwgsl.js:542-554).
+  void add_gen_ang(uint32_t x, T sgn, V3<T> d0, V3<T> d1, uint32_t ds, T* J) const {
+    int l = (int)x;
+    while (l >= 0) {
+      uint32_t k = (uint32_t)l;
+      for (uint32_t r = 0; r < link_ndof(k); r++) {
+        V3<T> wv = ldS(k, r).a;
+        uint32_t i = link_dof(k) + r - ds;
+        J[i] += sgn * dot(d0, wv); J[NDA + i] += sgn * dot(d1, wv);
+      }
+      l = link_parent(k);
+    }
+  }
   void add_gen_jac(uint32_t x, V3<T> p, T sgn, V3<T> d0, V3<T> d1, V3<T> d2, uint32_t ds, T* J) const {
     int l = (int)x;
     while (l >= 0) {
@@ -2322,6 +2342,27 @@ template <class T> struct World {
       }
       rows[o + 5] = {k11 / det, -k01 / det, k00 / det, bounce ? T(1) : wrow};
       rows[o + 6] = {ln, lt.x, lt.y, bounce ? T(1e30) : phi};
+      if (ANG) {
+        // Spin (about n) and rolling (about t1, t2) rows: angular-only Jacobians (wgsl.js:2223-2243).
+        V4<T> mat = C.q[3];
+        V3<T> MA[3][2], MB[3][2];
+        for (uint32_t r = 0; r < 3; r++) {
+          V3<T> d = dirs[r];
+          body_response(a, V3<T>{0, 0, 0}, -d, MA[r]);
+          body_response(b, V3<T>{0, 0, 0}, d, MB[r]);
+          rows[o + 22 + r * 4] = v4(MA[r][0], T(0)); rows[o + 23 + r * 4] = v4(MA[r][1], T(0));
+          rows[o + 24 + r * 4] = v4(MB[r][0], T(0)); rows[o + 25 + r * 4] = v4(MB[r][1], T(0));
+        }
+        T KA[9];
+        for (uint32_t r = 0; r < 3; r++)
+          for (uint32_t s3 = 0; s3 < 3; s3++) KA[r * 3 + s3] = dot(-dirs[r], MA[s3][1]) + dot(dirs[r], MB[s3][1]);
+        T ks = mx(KA[0], T(1e-12));
+        T r00 = KA[4] + T(1e-9), r01 = T(0.5) * (KA[5] + KA[7]), r11 = KA[8] + T(1e-9);
+        T rd = r00 * r11 - r01 * r01;
+        rows[o + 19] = {mat.y, mat.z, KA[0] > T(1e-12) ? T(1) / ks : T(0), T(0)};
+        rows[o + 20] = {r11 / rd, -r01 / rd, r00 / rd, T(0)};
+        rows[o + 21] = {0, 0, 0, 0};
+      }
     }
     if (TORS) for (uint32_t sp = 0; sp < nu; sp++) build_torsion(sp);
     if (PATCH) for (uint32_t sp = 0; sp < nu; sp++) build_patch(sp);
@@ -2388,7 +2429,7 @@ template <class T> struct World {
       }
     }
     // Phase 7: dense rows for same-articulation contacts, in contact order, then the mimics in fixed slots
-    S_META[3] = ndc + NMIMIC;
+    S_META[3] = ndc + loop_rows() + NMIMIC;
     uint32_t dslot = 0;
     for (uint32_t c = 0; c < nc; c++) {
       if (S_CSLOT[c] != 0xffffffffu || S_CLOCAL[c] == 0) continue;
@@ -2448,10 +2489,47 @@ template <class T> struct World {
       art[o + 24] = bounce ? T(1) : row_weight(phi);
       art[o + 22] = bounce ? T(1e30) : phi;
     }
+    // Loop closures (Steel's connect rows, wgsl.js:2445-2480): three point rows at the anchors' midpoint, plus two
+    // hinge-alignment rows for a revolute closure; a world parent contributes no dofs.
+    uint32_t slot = ndc;
+    for (uint32_t i = 0; i < NLOOP; i++) {
+      V4<T> L0 = statF[SF_LOOP + i * 5];
+      int pl = ti(L0.x);
+      uint32_t cl = tu(L0.y), art_id = tu(L0.z);
+      bool rev = L0.w > T(0.5);
+      uint32_t s = slot;
+      slot += rev ? 2 : 1;
+      if (s + (rev ? 1 : 0) >= MAXDENSE) continue;
+      V3<T> pa = xyz(statF[SF_LOOP + i * 5 + 1]);
+      V4<T> rp{0, 0, 0, 1};
+      if (pl >= 0) { rp = link_rot((uint32_t)pl); pa = ld3(lo((uint32_t)pl) + L_POS) + qrot(rp, pa); }
+      V4<T> rc = link_rot(cl);
+      V3<T> pc = ld3(lo(cl) + L_POS) + qrot(rc, xyz(statF[SF_LOOP + i * 5 + 2]));
+      V3<T> pm = T(0.5) * (pa + pc);
+      uint32_t ai[4];
+      art_info(art_id, ai);
+      V3<T> ex{1, 0, 0}, ey{0, 1, 0}, ez{0, 0, 1};
+      std::vector<T> J(3 * NDA, T(0));
+      add_gen_jac(cl, pm, T(1), ex, ey, ez, ai[2], J.data());
+      if (pl >= 0) add_gen_jac((uint32_t)pl, pm, T(-1), ex, ey, ez, ai[2], J.data());
+      put_bilateral(A_DENSE + s * DPS, art_id, J, 3, beta * (pc - pa) / dt);
+      if (rev) {
+        V3<T> ap = qrot(rp, xyz(statF[SF_LOOP + i * 5 + 3])), ac = qrot(rc, xyz(statF[SF_LOOP + i * 5 + 4]));
+        V3<T> e1 = cross(ap, ex);
+        if (dot(e1, e1) < T(1e-6)) e1 = cross(ap, ey);
+        e1 = normalize(e1);
+        V3<T> e2 = cross(ap, e1);
+        std::vector<T> Ja(3 * NDA, T(0));
+        add_gen_ang(cl, T(1), e1, e2, ai[2], Ja.data());
+        if (pl >= 0) add_gen_ang((uint32_t)pl, T(-1), e1, e2, ai[2], Ja.data());
+        V3<T> err = cross(ap, ac);
+        put_bilateral(A_DENSE + (s + 1) * DPS, art_id, Ja, 2, beta * V3<T>{dot(e1, err), dot(e2, err), T(0)} / dt);
+      }
+    }
     // Mimic rows: qd_follower - c1 qd_leader, Baumgarte on q_follower - c1 q_leader - c0 (wgsl.js:2482-2492).
     for (uint32_t i = 0; i < NMIMIC; i++) {
       V4<T> M0 = statF[SF_MIMIC + i * 2], M1 = statF[SF_MIMIC + i * 2 + 1];
-      uint32_t slot = ndc + i;
+      uint32_t slot = ndc + loop_rows() + i;
       if (slot >= MAXDENSE) continue;
       std::vector<T> J(3 * NDA, T(0));
       J[tu(M0.y)] = T(1);
@@ -2690,7 +2768,7 @@ template <class T> struct World {
       add_imp(a, -n * dn, -cross(ra, n * dn));
       add_imp(b, n * dn, cross(rb, n * dn));
       V3<T> fra = ra, frb = rb;
-      T lim = r2.w * nl;
+      T lim = r2.w * nl, pooled = nl;
       V2<T> bias{0, 0};
       T role = 1;
       if (EXTRA) {
@@ -2700,6 +2778,7 @@ template <class T> struct World {
         if (PATCH && role == T(2)) {
           T load = art[A_ULOAD + us];
           lim = r2.w * load;
+          pooled = load;
           V3<T> e = xyz(rows[o + PO + 2]);
           if (!vpass) bias = {F_ANCHOR_BETA * dot(t1, e) / dt, F_ANCHOR_BETA * dot(t2, e) / dt};
           rows[o + PO + 2].w = lim;
@@ -2722,6 +2801,33 @@ template <class T> struct World {
         add_imp(b, f, cross(frb, f));
       }
       rows[o + 6] = lam;
+      if (ANG) {
+        // Spin within mu_t N, rolling within a disk of mu_r N; under friction patches only the region's first anchor
+        // carries them, with the pooled load (wgsl.js:3144-3173).
+        V4<T> g = rows[o + 19];
+        bool aon = !PATCH || role == T(1) || (role == T(2) && rows[o + PO + 1].w == T(0));
+        if (aon && (g.x > T(0) || g.y > T(0))) {
+          V4<T> la = rows[o + 21];
+          V3<T> wrel = tb[1] - ta[1];
+          T ns = clampv(la.x - dot(n, wrel) * g.z, -g.x * pooled, g.x * pooled);
+          T ds = ns - la.x;
+          ta[0] += xyz(rows[o + 22]) * ds; ta[1] += xyz(rows[o + 23]) * ds;
+          tb[0] += xyz(rows[o + 24]) * ds; tb[1] += xyz(rows[o + 25]) * ds;
+          V3<T> w2 = tb[1] - ta[1];
+          V4<T> ri = rows[o + 20];
+          T u1 = dot(t1, w2), u2 = dot(t2, w2);
+          V2<T> lr = V2<T>{la.y, la.z} - V2<T>{ri.x * u1 + ri.y * u2, ri.y * u1 + ri.z * u2};
+          T rl = g.y * pooled;
+          if (length(lr) > rl) lr = lr * (rl / mx(length(lr), T(1e-12)));
+          T d1 = lr.x - la.y, d2 = lr.y - la.z;
+          ta[0] += xyz(rows[o + 26]) * d1 + xyz(rows[o + 30]) * d2; ta[1] += xyz(rows[o + 27]) * d1 + xyz(rows[o + 31]) * d2;
+          tb[0] += xyz(rows[o + 28]) * d1 + xyz(rows[o + 32]) * d2; tb[1] += xyz(rows[o + 29]) * d1 + xyz(rows[o + 33]) * d2;
+          rows[o + 21] = {ns, lr.x, lr.y, T(0)};
+          V3<T> tq = n * ds + t1 * d1 + t2 * d2;
+          add_imp(a, V3<T>{0, 0, 0}, -tq);
+          add_imp(b, V3<T>{0, 0, 0}, tq);
+        }
+      }
       if (TORS && row == S_UTP[us] && rows[S_USTART[us] * ROWS + TO].w > T(0)) unit_torsion(us, a, b, ta, tb);
     }
     if (a >= 0) { tw[(uint32_t)a * 2] = v4(ta[0], T(0)); tw[(uint32_t)a * 2 + 1] = v4(ta[1], T(0)); }

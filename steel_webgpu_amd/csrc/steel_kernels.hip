@@ -780,6 +780,27 @@ struct Prep {
     f2 lt;
     warm_lookup(key, mu, t1, t2, ln, lt);
     set_row(idx, 6, make_float4(ln, lt.x, lt.y, bounce ? 1e30f : phi));
+    if (w.h(W_ANG)) {
+      // Spin (about n) and rolling (about t1, t2) rows: angular-only Jacobians (wgsl.js:2223-2243).
+      const float4 mat = con(c, 3);
+      f3 MA[3][2], MB[3][2];
+      for (uint32_t r = 0; r < 3; r++) {
+        const f3 dd = dirs[r];
+        w.body_response(a, mk3(0, 0, 0), -dd, MA[r]);
+        w.body_response(b, mk3(0, 0, 0), dd, MB[r]);
+        set_row(idx, 22 + r * 4, F4(MA[r][0], 0.0f)); set_row(idx, 23 + r * 4, F4(MA[r][1], 0.0f));
+        set_row(idx, 24 + r * 4, F4(MB[r][0], 0.0f)); set_row(idx, 25 + r * 4, F4(MB[r][1], 0.0f));
+      }
+      float KA[9];
+      for (uint32_t r = 0; r < 3; r++)
+        for (uint32_t s3 = 0; s3 < 3; s3++) KA[r * 3 + s3] = dot(-dirs[r], MA[s3][1]) + dot(dirs[r], MB[s3][1]);
+      const float ks = mx(KA[0], 1e-12f);
+      const float r00 = KA[4] + 1e-9f, r01 = 0.5f * (KA[5] + KA[7]), r11 = KA[8] + 1e-9f;
+      const float rd = r00 * r11 - r01 * r01;
+      set_row(idx, 19, make_float4(mat.y, mat.z, KA[0] > 1e-12f ? 1.0f / ks : 0.0f, 0.0f));
+      set_row(idx, 20, make_float4(r11 / rd, -r01 / rd, r00 / rd, 0.0f));
+      set_row(idx, 21, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    }
   }
   DEV float row_weight(float phi) const { float reg = w.hf(W_REG); if (phi > 0.0f || reg <= 0.0f) return 1.0f; return 1.0f / (1.0f + reg); }
   DEV bool restitution_fires(float e, float phi, float vn) const {
@@ -796,6 +817,66 @@ struct Prep {
   // 3x3 K = J Y^T is gathered by shuffles, lane 0 writes the row header (wgsl.js:2370-2444).
   // Mimic row i in dense slot `slot`: qd_follower - c1 qd_leader, Baumgarte on q_follower - c1 q_leader - c0;
   // a bilateral block (J, Y = H^-1 J^T, 1 / (J Y), rhs) as in wgsl.js:1351-1369.
+  // Bilateral dense block at art offset o: nb unclamped rows of J (3 NDA floats), Y = H^-1 J^T, 1 / (J Y) and the
+  // rhs (wgsl.js:1351-1369).
+  DEVN void put_bilateral(uint32_t o, uint32_t art_id, const float* J, uint32_t nb, f3 rhs) const {
+    const uint4 ai = w.art_info(art_id);
+    const uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR);
+    float* art_ = w.d.art;
+    for (uint32_t r = 0; r < nb; r++) {
+      float tau[64], y[64];
+      SVd ps[64], dvs[64];
+      float uu[6 * 64];
+      for (uint32_t k = 0; k < NDA; k++) { tau[k] = k < ai.w ? J[r * NDA + k] : 0.0f; y[k] = 0.0f; }
+      w.art_solve_gen(art_id, tau, y, ps, dvs, uu);
+      float kk = 0.0f;
+      for (uint32_t k = 0; k < ai.w; k++) {
+        art_[o + DHDR + r * NDA + k] = J[r * NDA + k];
+        art_[o + DHDR + 3 * NDA + r * NDA + k] = y[k];
+        kk += J[r * NDA + k] * y[k];
+      }
+      art_[o + 5 + r] = kk > 1e-12f ? 1.0f / kk : 0.0f;
+    }
+    art_[o] = (float)art_id; art_[o + 1] = 0.0f; art_[o + 2] = 0.0f;
+    art_[o + 4] = rhs.x; art_[o + 22] = rhs.y; art_[o + 23] = rhs.z;
+    art_[o + 9] = 0.0f; art_[o + 10] = 0.0f; art_[o + 11] = 0.0f;
+    art_[o + 21] = (float)nb;
+  }
+  // Loop closure i in dense slot `slot` (and slot + 1 when revolute), wgsl.js:2445-2480: three point rows at the
+  // anchors' midpoint, plus two hinge-alignment rows for a revolute closure; a world parent contributes no dofs.
+  DEVN void loop_row(uint32_t i, uint32_t slot) const {
+    const uint32_t SL = w.h(W_SF_LOOP) + i * 5;
+    const float4 L0 = w.sf(SL);
+    const int pl = ti(L0.x);
+    const uint32_t cl = tu(L0.y), art_id = tu(L0.z);
+    const bool rev = L0.w > 0.5f;
+    f3 pa = xyz(w.sf(SL + 1));
+    q4 rp = mk4(0, 0, 0, 1);
+    if (pl >= 0) { rp = w.link_rot((uint32_t)pl); pa = w.ld3(w.lo((uint32_t)pl) + L_POS) + qrot(rp, pa); }
+    const q4 rc = w.link_rot(cl);
+    const f3 pc = w.ld3(w.lo(cl) + L_POS) + qrot(rc, xyz(w.sf(SL + 2)));
+    const f3 pm = 0.5f * (pa + pc);
+    const uint4 ai = w.art_info(art_id);
+    const uint32_t NDA = w.NDA(), A_DENSE = w.h(W_A_DENSE), DPS = w.h(W_DPS);
+    const float beta = w.hf(W_BETA), dt = w.hf(W_DT);
+    const f3 ex = mk3(1, 0, 0), ey = mk3(0, 1, 0), ez = mk3(0, 0, 1);
+    float J[3 * 64];
+    for (uint32_t k = 0; k < 3 * NDA; k++) J[k] = 0.0f;
+    w.add_gen_jac(cl, pm, 1.0f, ex, ey, ez, ai.z, J);
+    if (pl >= 0) w.add_gen_jac((uint32_t)pl, pm, -1.0f, ex, ey, ez, ai.z, J);
+    put_bilateral(A_DENSE + slot * DPS, art_id, J, 3, beta * (pc - pa) / dt);
+    if (!rev) return;
+    const f3 ap = qrot(rp, xyz(w.sf(SL + 3))), ac = qrot(rc, xyz(w.sf(SL + 4)));
+    f3 e1 = cross(ap, ex);
+    if (dot(e1, e1) < 1e-6f) e1 = cross(ap, ey);
+    e1 = nrm(e1);
+    const f3 e2 = cross(ap, e1);
+    for (uint32_t k = 0; k < 3 * NDA; k++) J[k] = 0.0f;
+    w.add_gen_ang(cl, 1.0f, e1, e2, ai.z, J);
+    if (pl >= 0) w.add_gen_ang((uint32_t)pl, -1.0f, e1, e2, ai.z, J);
+    const f3 err = cross(ap, ac);
+    put_bilateral(A_DENSE + (slot + 1) * DPS, art_id, J, 2, beta * mk3(dot(e1, err), dot(e2, err), 0.0f) / dt);
+  }
   DEVN void mimic_row(uint32_t i, uint32_t slot) const {
     const float4 M0 = w.sf(w.h(W_SF_MIMIC) + i * 2), M1 = w.sf(w.h(W_SF_MIMIC) + i * 2 + 1);
     const uint32_t art_id = tu(M0.x);
@@ -1393,7 +1474,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       }
       sched[S.PORD + o0 + j + 1] = r;
     }
-    sched[S.META + 3] = ndc + w.h(W_NMIMIC);
+    sched[S.META + 3] = ndc + w.loop_rows() + w.h(W_NMIMIC);
   }
   // ---- Phase 7: dense rows of same-articulation contacts (slot = rank in contact order), three lanes per
   // contact, 21 contacts per wave; RK0 (free after phase 5) maps slot -> contact.
@@ -1409,9 +1490,15 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     const uint32_t lane = tid & 63, sl = s0 + lane / 3, r = lane % 3;
     if (lane < 63 && sl < ndd) P.dense_row(sched[S.RK0 + sl], sl, r, lane - r);
   }
-  // Mimic rows in the slots after the contacts' (wgsl.js:2482-2492), a lane each.
+  // Loop closures, then mimic rows, in the slots after the contacts' (wgsl.js:2445-2492), a lane each.
+  const uint32_t nloop = w.h(W_NLOOP), nlr = w.loop_rows();
+  for (uint32_t i = tid; i < nloop; i += BS) {
+    uint32_t slot = ndc;
+    for (uint32_t k = 0; k < i; k++) slot += w.sf(w.h(W_SF_LOOP) + k * 5).w > 0.5f ? 2u : 1u;
+    if (slot + (w.sf(w.h(W_SF_LOOP) + i * 5).w > 0.5f ? 1u : 0u) < MAXDENSE) P.loop_row(i, slot);
+  }
   for (uint32_t i = tid; i < w.h(W_NMIMIC); i += BS)
-    if (ndc + i < MAXDENSE) P.mimic_row(i, ndc + i);
+    if (ndc + nlr + i < MAXDENSE) P.mimic_row(i, ndc + nlr + i);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1644,7 +1731,7 @@ struct Solver {
       add_imp(a, -n * dn, -cross(ra, n * dn));
       add_imp(b, n * dn, cross(rb, n * dn));
       f3 fra = ra, frb = rb;
-      float lim = r2.w * nl;
+      float lim = r2.w * nl, pooled = nl;
       f2 bias = mk2(0, 0);
       float role = 1.0f;
       if (EXTRA) {
@@ -1654,6 +1741,7 @@ struct Solver {
         if (PATCH && role == 2.0f) {
           float load = art_[w.h(W_A_ULOAD) + us];
           lim = r2.w * load;
+          pooled = load;
           f3 e = xyz(R[PO + 2]);
           float beta = w.hf(W_F_ANCHOR_BETA), dt = w.hf(W_DT);
           if (!vpass) bias = mk2(beta * dot(t1, e) / dt, beta * dot(t2, e) / dt);
@@ -1677,6 +1765,33 @@ struct Solver {
         add_imp(b, f, cross(frb, f));
       }
       R[6] = lam;
+      if (w.h(W_ANG)) {
+        // Spin within mu_t N, rolling within a disk of mu_r N; under friction patches only the region's first
+        // anchor carries them, with the pooled load (wgsl.js:3144-3173).
+        const float4 g = R[19];
+        const bool aon = !PATCH || role == 1.0f || (role == 2.0f && R[PO + 1].w == 0.0f);
+        if (aon && (g.x > 0.0f || g.y > 0.0f)) {
+          const float4 la = R[21];
+          const f3 wrel = tb[1] - ta[1];
+          const float ns = clampf_(la.x - dot(n, wrel) * g.z, -g.x * pooled, g.x * pooled);
+          const float ds = ns - la.x;
+          ta[0] += xyz(R[22]) * ds; ta[1] += xyz(R[23]) * ds;
+          tb[0] += xyz(R[24]) * ds; tb[1] += xyz(R[25]) * ds;
+          const f3 w2 = tb[1] - ta[1];
+          const float4 ri = R[20];
+          const float u1 = dot(t1, w2), u2 = dot(t2, w2);
+          f2 lr = mk2(la.y, la.z) - mk2(ri.x * u1 + ri.y * u2, ri.y * u1 + ri.z * u2);
+          const float rl = g.y * pooled;
+          if (len(lr) > rl) lr = lr * (rl / mx(len(lr), 1e-12f));
+          const float d1 = lr.x - la.y, d2 = lr.y - la.z;
+          ta[0] += xyz(R[26]) * d1 + xyz(R[30]) * d2; ta[1] += xyz(R[27]) * d1 + xyz(R[31]) * d2;
+          tb[0] += xyz(R[28]) * d1 + xyz(R[32]) * d2; tb[1] += xyz(R[29]) * d1 + xyz(R[33]) * d2;
+          R[21] = make_float4(ns, lr.x, lr.y, 0.0f);
+          const f3 tq = n * ds + t1 * d1 + t2 * d2;
+          add_imp(a, mk3(0, 0, 0), -tq);
+          add_imp(b, mk3(0, 0, 0), tq);
+        }
+      }
       if (w.h(W_TORS) && r == w.d.sched[w.S.UTP + us]) unit_torsion(us, a, b, ta, tb);
     }
     if (a >= 0) { tw[(uint32_t)a * 2] = F4(ta[0], 0.0f); tw[(uint32_t)a * 2 + 1] = F4(ta[1], 0.0f); }

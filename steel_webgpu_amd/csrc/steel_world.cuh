@@ -658,6 +658,26 @@ struct W {
     }
     for (uint32_t k = 0; k < 6; k++) art_[lo(l) + L_W + k * 6 + e] = col[k];
   }
+  // Angular Jacobian rows d0, d1 of link x's world angular velocity over its root path (artwgsl.js:542-554).
+  DEV void add_gen_ang(uint32_t x, float sgn, f3 d0, f3 d1, uint32_t ds, float* J) const {
+    int l = (int)x;
+    const uint32_t NDAv = NDA();
+    while (l >= 0) {
+      const uint32_t k = (uint32_t)l;
+      for (uint32_t r = 0; r < link_ndof(k); r++) {
+        const f3 wv = ldS(k, r).a;
+        const uint32_t i = link_dof(k) + r - ds;
+        J[i] += sgn * dot(d0, wv); J[NDAv + i] += sgn * dot(d1, wv);
+      }
+      l = link_parent(k);
+    }
+  }
+  // Dense rows of the loop closures: one slot per closure, two for a revolute one (wgsl.js:2367-2368).
+  DEV uint32_t loop_rows() const {
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < H[W_NLOOP]; k++) n += sf(H[W_SF_LOOP] + k * 5).w > 0.5f ? 2u : 1u;
+    return n;
+  }
   DEV void add_gen_jac(uint32_t x, f3 p, float sgn, f3 d0, f3 d1, f3 d2, uint32_t ds, float* J) const {
     int l = (int)x;
     uint32_t NDAv = NDA();

@@ -225,8 +225,6 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     NA, NL, ND = len(arts), len(links), len(dofs)
     if NA > 256:
         raise UnsupportedError(f"{NA} articulations; one solve workgroup handles at most 256")
-    if loops:
-        raise UnsupportedError("loop closures are not supported yet")
     # Mimic joints (sim.js:160-172): follower q = c0 + c1 * leader q, per dof, as dense bilateral rows.
     mimics = []
 
@@ -283,8 +281,6 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         t = s["type"]
         if t == GT["CONE"]:
             raise UnsupportedError("cone shapes are not supported yet")
-        if (s.get("condim", 3) or 3) >= 4:
-            raise UnsupportedError("torsional/rolling friction rows are not supported yet")
 
     SF = {}
     off = 0
@@ -298,6 +294,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     pathFloats = (S.get("target_frames") or 0) * (S.get("driven") or 0)
     SF["SF_PATH"] = off; off += (pathFloats + 3) // 4
     SF["SF_MIMIC"] = off; off += max(len(mimics), 1) * 2
+    SF["SF_LOOP"] = off; off += max(len(loops), 1) * 5
     nstatf = off
     statF = np.zeros(nstatf * 4, np.float32)
     SU = {}
@@ -515,7 +512,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     anchorBeta = SV.get("friction_anchor_beta", 0.2)
     if anchorBeta is None or compl:  # contact_compliance turns anchors off (sim.js:421-422)
         anchorBeta = 0.0
-    PO = 19
+    angOn = any((s.get("condim", 3) or 3) >= 4 for s in S["shapes"])  # sim.js:425-429 (articulated and free-rigid forms refused above)
+    PO = 34 if angOn else 19
     FMO = PO + (4 if anchorBeta > 0 else 0)
     nest = bool(SV.get("nesterov_momentum"))
     tors = any((s.get("torsion_radius") or 0) > 0 for s in S["shapes"])
@@ -620,6 +618,27 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for i, m in enumerate(mimics):
         statF[(SF["SF_MIMIC"] + i * 2) * 4:(SF["SF_MIMIC"] + i * 2) * 4 + 8] = [m["art"], m["f"], m["l"], m["c1"], m["fq"], m["lq"], m["c0"], 0]
         statU[SU["SU_ACON"] + m["art"]] = 1  # articulations owning a mimic never sleep (sim.js:369-370)
+    # Loop closures (ball or revolute) inside one articulation or to the world (sim.js:372-385).
+    H("W_NLOOP", len(loops))
+    H("W_ANG", 1 if angOn else 0)
+    for i, j in enumerate(loops):
+        t = J["type"][j]
+        if t not in (JT["BALL"], JT["REVOLUTE"]):
+            raise UnsupportedError(f"loop joint type {t} is not supported")
+        world = J["parent"][j] < 0
+        cb = J["child"][j]
+        pl = -1 if world else bodyLink[J["parent"][j]]
+        cl = bodyLink[cb]
+        if cl < 0 or (not world and (pl < 0 or bodyArt[J["parent"][j]] != bodyArt[cb])):
+            raise ValueError("loop joints must close inside one articulation or to the world")
+        xp, xc = J["X_p"][j], J["X_c"][j]
+        qd0 = J["qd_start"][j]
+        ax = J["axis"][qd0] if qd0 < len(J["axis"]) else [0, 0, 1]
+        ap, ac = quat_rotate([xp[3], xp[4], xp[5], xp[6]], ax), quat_rotate([xc[3], xc[4], xc[5], xc[6]], ax)
+        statF[(SF["SF_LOOP"] + i * 5) * 4:(SF["SF_LOOP"] + i * 5 + 5) * 4] = [pl, cl, bodyArt[cb], 1 if t == JT["REVOLUTE"] else 0,
+                                                                          xp[0], xp[1], xp[2], 0, xc[0], xc[1], xc[2], 0,
+                                                                          *ap, 0, *ac, 0]
+        statU[SU["SU_ACON"] + bodyArt[cb]] = 1  # articulations owning a loop closure never sleep
     gate = SV.get("contact_gap_gate")
     fgap = SV.get("contact_friction_gap_max")
     jlg = SV.get("joint_limit_activation_gap")

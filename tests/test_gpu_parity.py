@@ -99,6 +99,45 @@ def _with_solver(scene, **opts):
     return scene
 
 
+def loop_rig():
+    """Two closed chains: a hinged bar plus a bar on a ball joint, each closed to the world at its foot, one by a
+    revolute loop joint, one by a ball loop joint; the ball-jointed bars start spinning; a box drops on each."""
+    from steel_webgpu_amd.scene import JT
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane()
+    for k, lt in enumerate(("REVOLUTE", "BALL")):
+        y = 0.6 * k
+        A = b.addBody(xform=[0.25, y, 1, 0, 0, 0, 1])
+        b.addShapeBox(A, hx=0.25, hy=0.02, hz=0.02)
+        B = b.addBody(xform=[0.5, y, 0.75, 0, 0, 0, 1])
+        b.addShapeBox(B, hx=0.02, hy=0.02, hz=0.25)
+        b.addJointRevolute(-1, A, axis=[0, 1, 0], parentXform=[0, y, 1, 0, 0, 0, 1], childXform=[-0.25, 0, 0, 0, 0, 0, 1])
+        b.addJointBall(A, B, parentXform=[0.25, 0, 0, 0, 0, 0, 1], childXform=[0, 0, 0.25, 0, 0, 0, 1])
+        axes = [[0, 1, 0]] if lt == "REVOLUTE" else [[1, 0, 0], [0, 1, 0], [0, 0, 1]]
+        b.addJoint(JT[lt], -1, B, parentXform=[0.5, y, 0.5, 0, 0, 0, 1], childXform=[0, 0, -0.25, 0, 0, 0, 1], axes=axes,
+                   q=[0.0] if lt == "REVOLUTE" else [0, 0, 0, 1])
+        bx = b.addBody(xform=[0.3, y, 1.4, 0, 0, 0, 1])
+        b.addShapeBox(bx, hx=0.06, hy=0.06, hz=0.06)
+    return b.finalize(solver=dict(substeps=4))
+
+
+def spin_roll_rig():
+    """Balls rolling and spinning with spin and rolling rows (condim 6), and boxes with spin rows (condim 4) dropping
+    onto them and the ground."""
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane()
+    for k in range(4):
+        s = b.addBody(xform=[0.3 * k, 0, 0.08, 0, 0, 0, 1], twist=(0.2 * k, 0, 0, 0, 2.0 + k, 3.0 - k))
+        b.addShapeSphere(s, radius=0.08)
+    for k in range(4):
+        bx = b.addBody(xform=[0.3 * k + 0.05, 0.02, 0.4 + 0.1 * k, 0, 0, 0, 1], twist=(0, 0, 0, 0, 0, 4.0))
+        b.addShapeBox(bx, hx=0.05, hy=0.05, hz=0.05)
+    scene = b.finalize(solver=dict(substeps=4))
+    for i, sh in enumerate(scene.json["shapes"]):
+        sh.update(condim=6 if 1 <= i <= 4 else 4, mu_torsional=0.02, mu_rolling=0.01)
+    return scene
+
+
 def _with_velocity_iterations(scene, n):
     scene.json["solver"].update(velocity_iterations=n)
     return scene
@@ -174,6 +213,8 @@ SCENES = {
     "d6": d6_rig,
     "boxes_vel": lambda: _with_velocity_iterations(workloads.boxes64(), 4),
     "kinematic": kinematic_turntable,
+    "loops": loop_rig,
+    "spin_roll": spin_roll_rig,
     # drive rows with velocity limits (driven dofs fused into the sweep clamp) and effort limits
     "quad_drows": lambda: _with_solver(workloads.quadruped_world(17, drive=dict(targetKe=60.0, targetKd=2.0, velocityLimit=0.8, effortLimit=4.0)),
                                        drive_mode="rows", enable_joint_velocity_limits=True),
@@ -224,7 +265,8 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
                                          ("quad_mimic", 200), ("boxes_warm", 150), ("quad_warm", 200),
                                          ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200), ("d6", 200),
                                          ("boxes_vel", 150), ("quad_vel", 200), ("kinematic", 200),
-                                         ("boxes_compl", 150), ("quad_compl", 200), ("quad_drows", 200)])
+                                         ("boxes_compl", 150), ("quad_compl", 200), ("quad_drows", 200),
+                                         ("loops", 200), ("spin_roll", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])

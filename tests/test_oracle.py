@@ -1449,3 +1449,94 @@ def test_drive_rows_reproduce_the_implicit_drive_and_clamp_at_effort_times_dt():
         qd0 = o.art[pw.A["A_QD"]]
         o.substeps(1)
         assert abs(abs(o.art[pw.A["A_QD"]] - qd0) - _f32(eff) * dt / I) <= 1e-6 * _f32(eff) * dt / I, k
+
+
+def _three_bar(loop_type):
+    """A bar hinged (about y) at the world pivot (0, 0, 1) and a second bar on a ball joint at its end, closed to the
+    world point (0.5, 0, 0.5) by a loop joint at the second bar's foot: a triangle, rigid under gravity only
+    through the closure."""
+    from steel_webgpu_amd.scene import JT
+    b = SceneBuilder(fps=240)
+    A = b.addBody(xform=[0.25, 0, 1, 0, 0, 0, 1])
+    b.addShapeBox(A, hx=0.25, hy=0.02, hz=0.02)
+    B = b.addBody(xform=[0.5, 0, 0.75, 0, 0, 0, 1])
+    b.addShapeBox(B, hx=0.02, hy=0.02, hz=0.25)
+    b.addJointRevolute(-1, A, axis=[0, 1, 0], parentXform=[0, 0, 1, 0, 0, 0, 1], childXform=[-0.25, 0, 0, 0, 0, 0, 1])
+    b.addJointBall(A, B, parentXform=[0.25, 0, 0, 0, 0, 0, 1], childXform=[0, 0, 0.25, 0, 0, 0, 1])
+    if loop_type is not None:
+        axes = [[0, 1, 0]] if loop_type == "REVOLUTE" else [[1, 0, 0], [0, 1, 0], [0, 0, 1]]
+        b.addJoint(JT[loop_type], -1, B, parentXform=[0.5, 0, 0.5, 0, 0, 0, 1], childXform=[0, 0, -0.25, 0, 0, 0, 1], axes=axes,
+                   q=[0.0] if loop_type == "REVOLUTE" else [0, 0, 0, 1])
+    return pack_scene(b.finalize(solver=dict(substeps=4, iterations=12))), B
+
+
+def test_loop_closures_hold_the_closed_chain_and_its_hinge_axis():
+    """A ball or revolute loop joint closes a two-bar chain to the world (Baumgarte-stabilized bilateral rows at
+    the anchors' midpoint): under gravity the foot stays on its world point (gap under 1 mm after 2 s, from exactly
+    0).  The second bar starts spinning about its own axis (2 rad/s through its ball joint): a revolute closure's
+    two hinge rows stop that and keep the bar's y axis on the world's, a ball closure lets it turn.  Without the
+    closure the chain collapses."""
+    for lt in ("BALL", "REVOLUTE", None):
+        pw, B = _three_bar(lt)
+        assert pw.h("W_NLOOP") == (0 if lt is None else 1)
+        o = OracleWorld(pw, f64=True)
+        o.art[pw.A["A_QD"] + 3] = 2.0  # the ball joint's rate about z
+        o.substeps(4 * 480)
+        P = pw.h("W_P_BODY") * 4
+        c = o.poses[P + B * 8:P + B * 8 + 3]
+        q = o.poses[P + B * 8 + 4:P + B * 8 + 8]
+        foot = c + _rot(q) @ np.array([0, 0, -0.25])
+        gap = np.linalg.norm(foot - [0.5, 0, 0.5])
+        tilt = np.linalg.norm(np.cross(_rot(q) @ [0, 1, 0], [0, 1, 0]))
+        if lt is None:
+            assert gap > 0.1
+        else:
+            assert gap < 1e-3, (lt, gap)
+            assert (tilt < 1e-3) if lt == "REVOLUTE" else (tilt > 0.1), (lt, tilt)
+
+
+def _spinning_free_ball(condim, spin=6.0, roll=0.0, mut=0.02, mur=0.01, R=0.1):
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane(gap=0.01)
+    s = b.addBody(xform=[0, 0, R - 0.0003, 0, 0, 0, 1], twist=(roll * R, 0, 0, 0, roll, spin))
+    b.addShapeSphere(s, radius=R, gap=0.01)
+    scene = b.finalize(solver=dict(substeps=4, iterations=8))
+    for sh in scene.json["shapes"]:
+        sh.update(condim=condim, mu_torsional=mut, mu_rolling=mur)
+    scene.json["bodies"]["angular_damping"] = [0.0]
+    return pack_scene(scene)
+
+
+def test_free_body_spin_row_brakes_at_mu_t_times_load_and_rolling_rows_stop_a_rolling_ball():
+    """condim >= 4 on a free body: a ball spinning on its contact point is braked only by the spin row, within
+    mu_t N per substep: I_zz dw_z = -mu_t N (N read from the vertical momentum change), until it stops exactly.
+    condim 6 adds rolling rows (mu_r N): a rolling ball comes to rest, while with condim 3 it keeps rolling."""
+    pw = _spinning_free_ball(4)
+    assert pw.h("W_ANG") == 1 and pw.h("W_PO") == 34
+    o = OracleWorld(pw, f64=True)
+    SF = pw.h("W_SF_BODY")
+    F = pw.statF.reshape(-1, 4).astype(np.float64)
+    m, Izz = 1.0 / F[SF][0], F[SF + 4][2]
+    dt, gz = pw.hf("W_DT"), pw.hf("W_GZ")
+    mut = float(np.float32(0.02))
+    braked = stopped = 0
+    for k in range(4 * 240):
+        v0, w0 = o.vel[2], o.vel[6]
+        o.substeps(1)
+        N = m * (o.vel[2] - v0 - gz * dt)
+        if N == 0:
+            assert o.vel[6] == w0
+        elif o.vel[6] > 1e-9:
+            assert abs(Izz * (o.vel[6] - w0) + mut * N) <= 1e-7 * mut * N, k
+            braked += 1
+        else:
+            assert abs(o.vel[6]) <= 1e-12
+            stopped += 1
+    assert braked > 50 and stopped > 20
+    finals = {}
+    for condim in (6, 3):
+        pw = _spinning_free_ball(condim, spin=0.0, roll=3.0)
+        o = OracleWorld(pw, f64=True)
+        o.substeps(4 * 480)
+        finals[condim] = np.abs(o.vel[:8]).max()
+    assert finals[6] < 1e-3 and finals[3] > 0.1, finals
