@@ -76,10 +76,14 @@ enum steel_world_field {
   W_NLOOP, W_SF_LOOP,
   /* spin and rolling rows of free-body contacts (shapes with condim >= 4; u32 flag; rows 19-33, anchors at 34) */
   W_ANG,
+  /* demo controller (the scene's "control"): kind (u32; 1 fader: the wheel dof's target velocity follows the
+     fader coordinate, max_speed * clamp(q / travel, 0, 1)), fader coordinate (u32), wheel dof (u32), statF record
+     (u32: travel, max_speed) */
+  W_CTRL_KIND, W_CTRL_A, W_CTRL_B, W_SF_CTRL,
   W_FIELD_COUNT
 };
 
-#define STEEL_WORLD_HEADER_WORDS 128
+#define STEEL_WORLD_HEADER_WORDS 160
 #define STEEL_LB 268      /* floats per link block in art (artwgsl.js:8) */
 #define STEEL_AVN 14      /* u32 per friction-anchor value record (wgsl.js:96) */
 

@@ -68,7 +68,7 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE, NLOOP, SF_LOOP, ANG;
+  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE, NLOOP, SF_LOOP, ANG, CTRL_KIND, CTRL_A, CTRL_B, SF_CTRL;
   bool vpass = false;  // velocity pass: rows without position bias or regularization
   T mom_k = 0, mom_beta = 0;  // Nesterov restart counter and extrapolation factor of the current substep
   uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
@@ -122,6 +122,7 @@ template <class T> struct World {
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
     WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO]; VELITERS = h[W_VELITERS]; A_SNAP = h[W_A_SNAP]; COMPL = h[W_COMPL]; CO = h[W_CO];
     NLOOP = h[W_NLOOP]; SF_LOOP = h[W_SF_LOOP]; ANG = h[W_ANG];
+    CTRL_KIND = h[W_CTRL_KIND]; CTRL_A = h[W_CTRL_A]; CTRL_B = h[W_CTRL_B]; SF_CTRL = h[W_SF_CTRL];
     DRIVE_ROWS = h[W_DRIVE_ROWS]; FUSE_VLIM = h[W_FUSE_VLIM]; VEL_DRIVE_ACTIVE = h[W_VEL_DRIVE_ACTIVE];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
     SF_PATH = h[W_SF_PATH]; DRIVEN = h[W_DRIVEN]; PATH_FRAMES = h[W_PATH_FRAMES]; INTEGRAL_COUNT = h[W_INTEGRAL_COUNT];
@@ -2058,6 +2059,11 @@ template <class T> struct World {
         }
         art[A_TGT + i] = value;
       }
+    }
+    // Demo controller, every substep (update_control, wgsl.js:1821-1825): the fader sets the wheel's target speed.
+    if (CTRL_KIND == 1) {
+      V4<T> cc = statF[SF_CTRL];
+      art[A_TGTV + CTRL_B] = cc.y * mn(mx(art[A_Q + CTRL_A] / cc.x, T(0)), T(1));
     }
     for (uint32_t a = 0; a < NA; a++) {
       uint32_t ai[4];

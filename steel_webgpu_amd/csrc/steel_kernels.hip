@@ -1118,6 +1118,14 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       __syncthreads();
     }
   }
+  if (w.h(W_CTRL_KIND) == 1) {
+    // Demo controller, every substep (update_control, wgsl.js:1821-1825): the fader sets the wheel's target speed.
+    if (tid == 0) {
+      const float4 cc = w.sf(w.h(W_SF_CTRL));
+      art_[w.h(W_A_TGTV) + w.h(W_CTRL_B)] = cc.y * clampf_(art_[w.h(W_A_Q) + w.h(W_CTRL_A)] / cc.x, 0.0f, 1.0f);
+    }
+    __syncthreads();
+  }
   for (uint32_t a = tid; a < NA; a += BS) {
     uint4 ai = w.art_info(a);
     if (w.h(W_VLIM_ON)) {

@@ -37,7 +37,7 @@ def _parse_fields():
 
 
 FIELDS = _parse_fields()
-HEADER_WORDS = 128
+HEADER_WORDS = 160
 
 
 def _f2u(x):
@@ -242,8 +242,9 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for m in S.get("mimics") or []:
         if m.get("enabled"):
             add_mimic(m["follower"], m["leader"], m["c0"], m["c1"])
-    if S.get("control"):
-        raise UnsupportedError("scene controllers are not supported yet")
+    ctrl = S.get("control")
+    if ctrl and ctrl.get("type") != "fader":
+        raise UnsupportedError(f"scene controller {ctrl.get('type')} is not supported")
 
     # Hulls (dedup by their blob range), unique edges with their two faces (sim.js:180-209).
     shapes = [s for s in S["shapes"] if s.get("hull") or s.get("plane")]
@@ -295,6 +296,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     SF["SF_PATH"] = off; off += (pathFloats + 3) // 4
     SF["SF_MIMIC"] = off; off += max(len(mimics), 1) * 2
     SF["SF_LOOP"] = off; off += max(len(loops), 1) * 5
+    SF["SF_CTRL"] = off; off += 1
     nstatf = off
     statF = np.zeros(nstatf * 4, np.float32)
     SU = {}
@@ -621,6 +623,11 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     # Loop closures (ball or revolute) inside one articulation or to the world (sim.js:372-385).
     H("W_NLOOP", len(loops))
     H("W_ANG", 1 if angOn else 0)
+    if ctrl:  # the fader controller (sim.js:388-393)
+        H("W_CTRL_KIND", 1)
+        H("W_CTRL_A", qMap[ctrl["fader_q"]])
+        H("W_CTRL_B", dofMap[ctrl["wheel_v"]])
+        statF[SF["SF_CTRL"] * 4:SF["SF_CTRL"] * 4 + 4] = [ctrl["travel"], ctrl["max_speed"], 0, 0]
     for i, j in enumerate(loops):
         t = J["type"][j]
         if t not in (JT["BALL"], JT["REVOLUTE"]):

@@ -138,6 +138,23 @@ def spin_roll_rig():
     return scene
 
 
+def fader_rig():
+    """A slider on a tilted rail coupled to a velocity-driven wheel by the fader controller, plus a box on the wheel."""
+    b = SceneBuilder(fps=240)
+    b.addGroundPlane()
+    s = b.addBody(xform=[0, 0, 1, 0, 0, 0, 1])
+    b.addShapeBox(s, hx=0.03, hy=0.03, hz=0.03)
+    b.addJointPrismatic(-1, s, axis=[1, 0, 0], parentXform=[0, 0, 1, 0, 0.2588190, 0, 0.9659258])
+    w = b.addBody(xform=[1, 0, 0.3, 0, 0, 0, 1])
+    b.addShapeCylinder(w, radius=0.3, halfHeight=0.02)
+    b.addJointRevolute(-1, w, axis=[0, 0, 1], parentXform=[1, 0, 0.3, 0, 0, 0, 1], targetKd=5.0)
+    bx = b.addBody(xform=[1.15, 0, 0.4, 0, 0, 0, 1])
+    b.addShapeBox(bx, hx=0.04, hy=0.04, hz=0.04)
+    scene = b.finalize(solver=dict(substeps=4))
+    scene.json["control"] = dict(type="fader", fader_q=0, wheel_v=1, travel=0.24, max_speed=6.283185307179586)
+    return scene
+
+
 def _with_velocity_iterations(scene, n):
     scene.json["solver"].update(velocity_iterations=n)
     return scene
@@ -215,6 +232,7 @@ SCENES = {
     "kinematic": kinematic_turntable,
     "loops": loop_rig,
     "spin_roll": spin_roll_rig,
+    "fader": fader_rig,
     # drive rows with velocity limits (driven dofs fused into the sweep clamp) and effort limits
     "quad_drows": lambda: _with_solver(workloads.quadruped_world(17, drive=dict(targetKe=60.0, targetKd=2.0, velocityLimit=0.8, effortLimit=4.0)),
                                        drive_mode="rows", enable_joint_velocity_limits=True),
@@ -266,7 +284,7 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
                                          ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200), ("d6", 200),
                                          ("boxes_vel", 150), ("quad_vel", 200), ("kinematic", 200),
                                          ("boxes_compl", 150), ("quad_compl", 200), ("quad_drows", 200),
-                                         ("loops", 200), ("spin_roll", 200)])
+                                         ("loops", 200), ("spin_roll", 200), ("fader", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])

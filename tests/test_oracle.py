@@ -1540,3 +1540,38 @@ def test_free_body_spin_row_brakes_at_mu_t_times_load_and_rolling_rows_stop_a_ro
         o.substeps(4 * 480)
         finals[condim] = np.abs(o.vel[:8]).max()
     assert finals[6] < 1e-3 and finals[3] > 0.1, finals
+
+
+def fader_rig():
+    """A slider (prismatic, pushed along +x by gravity on a tilted rail) and a velocity-driven wheel, coupled by
+    the reference's fader controller: the wheel's target speed is max_speed * clamp(q_slider / travel, 0, 1)."""
+    b = SceneBuilder(fps=240)
+    s = b.addBody(xform=[0, 0, 1, 0, 0, 0, 1])
+    b.addShapeBox(s, hx=0.03, hy=0.03, hz=0.03)
+    b.addJointPrismatic(-1, s, axis=[1, 0, 0], parentXform=[0, 0, 1, 0, 0.2588190, 0, 0.9659258])
+    w = b.addBody(xform=[1, 0, 1, 0, 0, 0, 1])
+    b.addShapeCylinder(w, radius=0.1, halfHeight=0.02)
+    b.addJointRevolute(-1, w, axis=[0, 0, 1], parentXform=[1, 0, 1, 0, 0, 0, 1], targetKd=5.0)
+    scene = b.finalize(solver=dict(substeps=4))
+    scene.json["control"] = dict(type="fader", fader_q=0, wheel_v=1, travel=0.24, max_speed=6.283185307179586)
+    return scene
+
+
+def test_fader_controller_sets_the_wheel_target_speed_from_the_slider_each_substep():
+    """The fader controller (the reference's demo update_control): every substep, before the drives act, the
+    wheel's target velocity becomes max_speed * clamp(q_slider / travel, 0, 1) of the slider coordinate at the
+    substep's start; the velocity-driven wheel follows it up to max_speed once the slider passes the travel."""
+    pw = pack_scene(fader_rig())
+    assert pw.h("W_CTRL_KIND") == 1
+    o = OracleWorld(pw, f64=True)
+    A = pw.A
+    qa, vb = A["A_Q"] + pw.h("W_CTRL_A"), A["A_TGTV"] + pw.h("W_CTRL_B")
+    travel, vmax = float(np.float32(0.24)), float(np.float32(6.283185307179586))
+    seen = set()
+    for k in range(4 * 240):
+        q0 = o.art[qa]
+        o.substeps(1)
+        assert o.art[vb] == vmax * min(max(q0 / travel, 0.0), 1.0), k
+        seen.add(round(min(max(q0 / travel, 0.0), 1.0), 1))
+    assert 0.0 in seen and 1.0 in seen and len(seen) > 4
+    assert abs(o.art[A["A_QD"] + pw.h("W_CTRL_B")] - vmax) < 0.05 * vmax
