@@ -80,6 +80,9 @@ enum steel_world_field {
      fader coordinate, max_speed * clamp(q / travel, 0, 1)), fader coordinate (u32), wheel dof (u32), statF record
      (u32: travel, max_speed) */
   W_CTRL_KIND, W_CTRL_A, W_CTRL_B, W_SF_CTRL,
+  /* friction_mode (u32: 0 fixed_normal, 1 bisection, 2 bisection_desaxce, 3 coulomb_newton) and the row-record
+     offset of the coupled block's Delassus entries (u32) */
+  W_FMODE, W_FMO,
   W_FIELD_COUNT
 };
 

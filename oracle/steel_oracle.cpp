@@ -68,7 +68,7 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE, NLOOP, SF_LOOP, ANG, CTRL_KIND, CTRL_A, CTRL_B, SF_CTRL;
+  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE, NLOOP, SF_LOOP, ANG, CTRL_KIND, CTRL_A, CTRL_B, SF_CTRL, FMODE, FMO;
   bool vpass = false;  // velocity pass: rows without position bias or regularization
   T mom_k = 0, mom_beta = 0;  // Nesterov restart counter and extrapolation factor of the current substep
   uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
@@ -122,6 +122,7 @@ template <class T> struct World {
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
     WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO]; VELITERS = h[W_VELITERS]; A_SNAP = h[W_A_SNAP]; COMPL = h[W_COMPL]; CO = h[W_CO];
     NLOOP = h[W_NLOOP]; SF_LOOP = h[W_SF_LOOP]; ANG = h[W_ANG];
+    FMODE = h[W_FMODE]; FMO = h[W_FMO];
     CTRL_KIND = h[W_CTRL_KIND]; CTRL_A = h[W_CTRL_A]; CTRL_B = h[W_CTRL_B]; SF_CTRL = h[W_SF_CTRL];
     DRIVE_ROWS = h[W_DRIVE_ROWS]; FUSE_VLIM = h[W_FUSE_VLIM]; VEL_DRIVE_ACTIVE = h[W_VEL_DRIVE_ACTIVE];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
@@ -2300,6 +2301,10 @@ template <class T> struct World {
         }
       if (EXTRA) { rows[o + PO] = v4(fa, T(1)); rows[o + PO + 1] = v4(fb, T(0)); rows[o + PO + 2] = v4(p, T(0)); }
       if (TORS) { rows[o + TO + 3] = {0, 0, 0, C.q[3].w}; rows[o + TO + 4] = v4(p, T(0)); }
+      if (FMODE) {
+        rows[o + FMO] = {K[0], T(0.5) * (K[1] + K[3]), T(0.5) * (K[2] + K[6]), T(0)};
+        rows[o + FMO + 1] = {K[4], T(0.5) * (K[5] + K[7]), K[8], T(0)};
+      }
       T kscale = mx(K[0] + K[4] + K[8], T(1e-30));
       T effn = 0;
       if (K[0] > T(1e-6) * kscale && K[0] > T(1e-12)) effn = T(1) / (K[0] + F_CFM);
@@ -2678,6 +2683,92 @@ template <class T> struct World {
     imp[(uint32_t)b * 2] = imp[(uint32_t)b * 2] + v4(f, T(0));
     imp[(uint32_t)b * 2 + 1] = imp[(uint32_t)b * 2 + 1] + v4(t, T(0));
   }
+  // ---- Steel's coupled contact solves (friction_mode bisection, bisection_desaxce, coulomb_newton; wgsl.js:2564-2653):
+  // u = live (normal, t1, t2) velocities, G the 3x3 Delassus block (nn, nt1, nt2 | t1t1, t1t2, t2t2).
+  static V2<T> tangent_step(V2<T> ut, V4<T> g1) {
+    T det = g1.x * g1.z - g1.y * g1.y;
+    if (std::fabs(det) <= T(1e-20)) return {0, 0};
+    return {(-ut.x * g1.z + ut.y * g1.y) / det, (ut.x * g1.y - ut.y * g1.x) / det};
+  }
+  static V2<T> clip_cone(V2<T> t, T lim) {
+    T m = length(t);
+    if (m > lim && m > T(1e-20)) return t * (lim / m);
+    return t;
+  }
+  static V3<T> fixed_normal_pair(V2<T> ut0, V4<T> g1, T mu, V3<T> lam) {
+    V2<T> f = friction_pair_solve(g1.x, g1.y, g1.z, ut0.x, ut0.y, lam.y, lam.z, mx(mu * lam.x, T(0)), T(1));
+    return {lam.x, f.x, f.y};
+  }
+  static V3<T> contact_block(uint32_t mode, T un, V2<T> ut0, V4<T> g0, V4<T> g1, T vtarget0, T mu, V3<T> lam) {
+    T gnn = g0.x;
+    V2<T> gnt{g0.y, g0.z};
+    T trace = g1.x + g1.z;
+    if (g1.x * g1.z - g1.y * g1.y <= T(1e-6) * trace * trace) return fixed_normal_pair(ut0, g1, mu, lam);
+    if (gnn < T(1e-20)) return lam;
+    V2<T> lt{lam.y, lam.z};
+    T vt_n = vtarget0;
+    if (mode == 2) vt_n += mu * length(ut0);
+    if (mode == 3) {
+      T un_f = un - (gnn * lam.x + dot(gnt, lt));
+      V2<T> ut_f = ut0 - V2<T>{gnt.x * lam.x + g1.x * lt.x + g1.y * lt.y, gnt.y * lam.x + g1.y * lt.x + g1.z * lt.y};
+      if (un_f >= vt_n) return {0, 0, 0};
+      T bN = un_f - vt_n;
+      T A00 = g1.x - gnt.x * gnt.x / gnn, A01 = g1.y - gnt.x * gnt.y / gnn, A11 = g1.z - gnt.y * gnt.y / gnn;
+      if (A00 * A11 - A01 * A01 <= T(1e-6) * trace * trace) return fixed_normal_pair(ut0, g1, mu, lam);
+      V2<T> c = ut_f - (bN / gnn) * gnt;
+      T id0 = T(1) / (A00 * A11 - A01 * A01);
+      V2<T> s = V2<T>{A11 * c.x - A01 * c.y, A00 * c.y - A01 * c.x} * id0;
+      T phi0 = length(s) - mu * (dot(gnt, s) - bN) / gnn;
+      if (phi0 > T(0)) {
+        T hi = 1;
+        for (uint32_t e = 0; e < 30; e++) {
+          T a = A00 + hi, d = A11 + hi, idt = T(1) / (a * d - A01 * A01);
+          V2<T> sh = V2<T>{d * c.x - A01 * c.y, a * c.y - A01 * c.x} * idt;
+          if (length(sh) - mu * (dot(gnt, sh) - bN) / gnn < T(0)) break;
+          hi *= T(2);
+        }
+        T lo = 0, x = T(0.5) * hi;
+        T tol = T(1e-6) + T(1e-6) * phi0;
+        for (uint32_t k = 0; k < 50; k++) {
+          T a = A00 + x, d = A11 + x, idt = T(1) / (a * d - A01 * A01);
+          V2<T> sx = V2<T>{d * c.x - A01 * c.y, a * c.y - A01 * c.x} * idt;
+          V2<T> tx = V2<T>{d * sx.x - A01 * sx.y, a * sx.y - A01 * sx.x} * idt;
+          T ns = length(sx);
+          T fx = ns - mu * (dot(gnt, sx) - bN) / gnn;
+          T dfx = -dot(sx, tx) / ns + mu * dot(gnt, tx) / gnn;
+          s = sx;
+          if (std::fabs(fx) < tol || std::fabs(hi - lo) < T(1e-6) * (T(1) + hi)) break;
+          if (fx > T(0)) lo = x; else hi = x;
+          T xn = T(0.5) * (lo + hi);
+          if (dfx != T(0)) { T xt = x - fx / dfx; if (xt > lo && xt < hi) xn = xt; }
+          x = xn;
+        }
+      }
+      V2<T> rt = T(-1) * s;
+      T ln = mx(-(dot(gnt, rt) + bN) / gnn, T(0));
+      T lim = mx(mu * ln, T(0));
+      T m = length(rt);
+      V2<T> r2 = m > lim ? rt * (lim / m) : rt;
+      return {ln, r2.x, r2.y};
+    }
+    // Bisection on the normal impulse with the tangent pair re-solved and clipped at each trial.
+    T u0 = un - gnn * lam.x - dot(gnt, lt);
+    if (u0 >= vt_n) return {0, 0, 0};
+    T lo = 0;
+    T hi = mx(mx(T(2) * lam.x, (vt_n - un) / gnn + lam.x), T(1));
+    T flo = u0 - vt_n, fhi = -1;
+    for (uint32_t k = 0; k < 20; k++) {
+      T mid = T(0.5) * (lo + hi);
+      T dn = mid - lam.x;
+      V2<T> tr = clip_cone(lt + tangent_step(ut0 + gnt * dn, g1), mu * mid);
+      T u = un + gnn * dn + dot(gnt, tr - lt);
+      if (u < vt_n) { lo = mid; flo = u - vt_n; } else { hi = mid; fhi = u - vt_n; }
+    }
+    T ln = T(0.5) * (lo + hi);
+    if (fhi >= T(0)) ln = lo - flo * (hi - lo) / (fhi - flo);
+    V2<T> tf = clip_cone(lt + tangent_step(ut0 + gnt * (ln - lam.x), g1), mu * ln);
+    return {ln, tf.x, tf.y};
+  }
   static V2<T> friction_pair_solve(T a0, T c0, T d0, T r0, T r1, T old0, T old1, T radius, T omega) {  // wgsl.js:2952-3011
     if (radius <= T(0)) return {0, 0};
     T scale = mx(mx(a0, d0), T(1e-20));
@@ -2790,7 +2881,23 @@ template <class T> struct World {
           rows[o + PO + 2].w = lim;
         }
       }
-      if (role != T(0)) {
+      if (FMODE) {
+        // Coupled (normal, t1, t2) re-solve; overwrites all three impulses (wgsl.js:3081-3097).
+        V3<T> vn = (tb[0] + cross(tb[1], rb)) - (ta[0] + cross(ta[1], ra));
+        V3<T> vt = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
+        V3<T> nlam = contact_block(FMODE, dot(n, vn), V2<T>{dot(t1, vt), dot(t2, vt)}, rows[o + FMO], rows[o + FMO + 1], -r1.w, r2.w,
+                                   xyz(lam));
+        V3<T> d = nlam - xyz(lam);
+        if (PATCH && role != T(1)) art[A_ULOAD + us] += d.x;
+        ta[0] += xyz(rows[o + 7]) * d.x + xyz(rows[o + 11]) * d.y + xyz(rows[o + 15]) * d.z;
+        ta[1] += xyz(rows[o + 8]) * d.x + xyz(rows[o + 12]) * d.y + xyz(rows[o + 16]) * d.z;
+        tb[0] += xyz(rows[o + 9]) * d.x + xyz(rows[o + 13]) * d.y + xyz(rows[o + 17]) * d.z;
+        tb[1] += xyz(rows[o + 10]) * d.x + xyz(rows[o + 14]) * d.y + xyz(rows[o + 18]) * d.z;
+        V3<T> fn_ = n * d.x, ft = t1 * d.y + t2 * d.z;
+        add_imp(a, -fn_ - ft, -cross(ra, fn_) - cross(fra, ft));
+        add_imp(b, fn_ + ft, cross(rb, fn_) + cross(frb, ft));
+        lam = v4(nlam, lam.w);
+      } else if (role != T(0)) {
         V3<T> vr2 = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
         V4<T> ki = rows[o + 5];
         T v1 = dot(t1, vr2) + bias.x, v2 = dot(t2, vr2) + bias.y;
@@ -3558,5 +3665,12 @@ float oracle_atan2f(float y, float x) { return atan2f_c(y, x); }
 void oracle_friction_pair_solve(float a, float c, float d, float r0, float r1, float o0, float o1, float radius, float omega, float* out) {
   V2<float> r = World<float>::friction_pair_solve(a, c, d, r0, r1, o0, o1, radius, omega);
   out[0] = r.x; out[1] = r.y;
+}
+// The coupled contact block in FP64: in[0] un, in[1..2] ut0, in[3..5] (gnn, gnt), in[6..8] (g_t1t1, g_t1t2,
+// g_t2t2), in[9] vtarget, in[10] mu, in[11..13] lam; out (ln, lt1, lt2).
+void oracle_contact_block_f64(uint32_t mode, const double* in, double* out) {
+  V3<double> r = World<double>::contact_block(mode, in[0], V2<double>{in[1], in[2]}, V4<double>{in[3], in[4], in[5], 0},
+                                              V4<double>{in[6], in[7], in[8], 0}, in[9], in[10], V3<double>{in[11], in[12], in[13]});
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
 }

@@ -729,6 +729,10 @@ struct Prep {
     uint32_t PO = w.PO();
     if (w.h(W_EXTRA)) { set_row(idx, PO, F4(fa, 1.0f)); set_row(idx, PO + 1, F4(fb, 0.0f)); set_row(idx, PO + 2, F4(p, 0.0f)); }
     if (w.h(W_TORS)) { set_row(idx, w.h(W_TO) + 3, make_float4(0.0f, 0.0f, 0.0f, con(c, 3).w)); set_row(idx, w.h(W_TO) + 4, F4(p, 0.0f)); }
+    if (w.h(W_FMODE)) {
+      set_row(idx, w.h(W_FMO), make_float4(K[0], 0.5f * (K[1] + K[3]), 0.5f * (K[2] + K[6]), 0.0f));
+      set_row(idx, w.h(W_FMO) + 1, make_float4(K[4], 0.5f * (K[5] + K[7]), K[8], 0.0f));
+    }
     float cfm = w.hf(W_F_CFM), dt = w.hf(W_DT);
     float kscale = mx(K[0] + K[4] + K[8], 1e-30f);
     float effn = 0.0f;
@@ -1640,6 +1644,92 @@ struct Solver {
     imp[(uint32_t)b * 2] = make_float4(i0.x + f.x, i0.y + f.y, i0.z + f.z, i0.w + 0.0f);
     imp[(uint32_t)b * 2 + 1] = make_float4(i1.x + t.x, i1.y + t.y, i1.z + t.z, i1.w + 0.0f);
   }
+  // ---- Steel's coupled contact solves (friction_mode bisection, bisection_desaxce, coulomb_newton; wgsl.js:2564-2653):
+  // u = live (normal, t1, t2) velocities, G the 3x3 Delassus block (nn, nt1, nt2 | t1t1, t1t2, t2t2).
+  static DEV f2 tangent_step(f2 ut, float4 g1) {
+    const float det = g1.x * g1.z - g1.y * g1.y;
+    if (fabsf(det) <= 1e-20f) return mk2(0.0f, 0.0f);
+    return mk2((-ut.x * g1.z + ut.y * g1.y) / det, (ut.x * g1.y - ut.y * g1.x) / det);
+  }
+  static DEV f2 clip_cone(f2 t, float lim) {
+    const float m = len(t);
+    if (m > lim && m > 1e-20f) return t * (lim / m);
+    return t;
+  }
+  static DEVN f3 fixed_normal_pair(f2 ut0, float4 g1, float mu, f3 lam) {
+    const f2 f = friction_pair_solve(g1.x, g1.y, g1.z, ut0.x, ut0.y, lam.y, lam.z, mx(mu * lam.x, 0.0f), 1.0f);
+    return mk3(lam.x, f.x, f.y);
+  }
+  static DEVN f3 contact_block(uint32_t mode, float un, f2 ut0, float4 g0, float4 g1, float vtarget0, float mu, f3 lam) {
+    const float gnn = g0.x;
+    const f2 gnt = mk2(g0.y, g0.z);
+    const float trace = g1.x + g1.z;
+    if (g1.x * g1.z - g1.y * g1.y <= 1e-6f * trace * trace) return fixed_normal_pair(ut0, g1, mu, lam);
+    if (gnn < 1e-20f) return lam;
+    const f2 lt = mk2(lam.y, lam.z);
+    float vt_n = vtarget0;
+    if (mode == 2) vt_n += mu * len(ut0);
+    if (mode == 3) {
+      const float un_f = un - (gnn * lam.x + dot(gnt, lt));
+      const f2 ut_f = ut0 - mk2(gnt.x * lam.x + g1.x * lt.x + g1.y * lt.y, gnt.y * lam.x + g1.y * lt.x + g1.z * lt.y);
+      if (un_f >= vt_n) return mk3(0, 0, 0);
+      const float bN = un_f - vt_n;
+      const float A00 = g1.x - gnt.x * gnt.x / gnn, A01 = g1.y - gnt.x * gnt.y / gnn, A11 = g1.z - gnt.y * gnt.y / gnn;
+      if (A00 * A11 - A01 * A01 <= 1e-6f * trace * trace) return fixed_normal_pair(ut0, g1, mu, lam);
+      const f2 c = ut_f - (bN / gnn) * gnt;
+      const float id0 = 1.0f / (A00 * A11 - A01 * A01);
+      f2 s = mk2(A11 * c.x - A01 * c.y, A00 * c.y - A01 * c.x) * id0;
+      const float phi0 = len(s) - mu * (dot(gnt, s) - bN) / gnn;
+      if (phi0 > 0.0f) {
+        float hi = 1.0f;
+        for (uint32_t e = 0; e < 30; e++) {
+          const float a = A00 + hi, d = A11 + hi, idt = 1.0f / (a * d - A01 * A01);
+          const f2 sh = mk2(d * c.x - A01 * c.y, a * c.y - A01 * c.x) * idt;
+          if (len(sh) - mu * (dot(gnt, sh) - bN) / gnn < 0.0f) break;
+          hi *= 2.0f;
+        }
+        float lo = 0.0f, x = 0.5f * hi;
+        const float tol = 1e-6f + 1e-6f * phi0;
+        for (uint32_t k = 0; k < 50; k++) {
+          const float a = A00 + x, d = A11 + x, idt = 1.0f / (a * d - A01 * A01);
+          const f2 sx = mk2(d * c.x - A01 * c.y, a * c.y - A01 * c.x) * idt;
+          const f2 tx = mk2(d * sx.x - A01 * sx.y, a * sx.y - A01 * sx.x) * idt;
+          const float ns = len(sx);
+          const float fx = ns - mu * (dot(gnt, sx) - bN) / gnn;
+          const float dfx = -dot(sx, tx) / ns + mu * dot(gnt, tx) / gnn;
+          s = sx;
+          if (fabsf(fx) < tol || fabsf(hi - lo) < 1e-6f * (1.0f + hi)) break;
+          if (fx > 0.0f) lo = x; else hi = x;
+          float xn = 0.5f * (lo + hi);
+          if (dfx != 0.0f) { const float xt = x - fx / dfx; if (xt > lo && xt < hi) xn = xt; }
+          x = xn;
+        }
+      }
+      const f2 rt = -1.0f * s;
+      const float ln = mx(-(dot(gnt, rt) + bN) / gnn, 0.0f);
+      const float lim = mx(mu * ln, 0.0f);
+      const float m = len(rt);
+      const f2 r2 = m > lim ? rt * (lim / m) : rt;
+      return mk3(ln, r2.x, r2.y);
+    }
+    // Bisection on the normal impulse with the tangent pair re-solved and clipped at each trial.
+    const float u0 = un - gnn * lam.x - dot(gnt, lt);
+    if (u0 >= vt_n) return mk3(0, 0, 0);
+    float lo = 0.0f;
+    float hi = mx(mx(2.0f * lam.x, (vt_n - un) / gnn + lam.x), 1.0f);
+    float flo = u0 - vt_n, fhi = -1.0f;
+    for (uint32_t k = 0; k < 20; k++) {
+      const float mid = 0.5f * (lo + hi);
+      const float dn = mid - lam.x;
+      const f2 tr = clip_cone(lt + tangent_step(ut0 + gnt * dn, g1), mu * mid);
+      const float u = un + gnn * dn + dot(gnt, tr - lt);
+      if (u < vt_n) { lo = mid; flo = u - vt_n; } else { hi = mid; fhi = u - vt_n; }
+    }
+    float ln = 0.5f * (lo + hi);
+    if (fhi >= 0.0f) ln = lo - flo * (hi - lo) / (fhi - flo);
+    const f2 tf = clip_cone(lt + tangent_step(ut0 + gnt * (ln - lam.x), g1), mu * ln);
+    return mk3(ln, tf.x, tf.y);
+  }
   static DEV f2 friction_pair_solve(float a0, float c0, float d0, float r0, float r1, float old0, float old1, float radius, float omega) {
     if (radius <= 0.0f) return mk2(0, 0);
     float scale = mx(mx(a0, d0), 1e-20f);
@@ -1700,6 +1790,9 @@ struct Solver {
     float m = len(relaxed);
     return m > radius ? relaxed * (radius / m) : relaxed;
   }
+  // EXT: the off-default row kinds (coupled friction modes, spin/rolling rows, contact torsion, compliance) are
+  // compiled in; the default path instantiates without them, so it makes no calls and keeps its registers.
+  template <bool EXT>
   DEVN void solve_unit(uint32_t us, uint32_t r, int a, int b, uint32_t mode) const {  // wgsl.js:3033-3231
     const uint32_t PO = w.PO();
     const bool EXTRA = w.h(W_EXTRA) != 0, PATCH = w.h(W_PATCH) != 0;
@@ -1729,7 +1822,7 @@ struct Solver {
       float jv = dot(n, vrel);
       float wgt = vpass ? 1.0f : R[5].w;
       float res = jv + r1.w;
-      if (w.h(W_COMPL)) res += R[w.h(W_CO)].x * lam.x;
+      if (EXT && w.h(W_COMPL)) res += R[w.h(W_CO)].x * lam.x;
       float omega = w.hf(W_F_OMEGA);
       float nl = mx(lam.x + omega * (-res * r0.w * wgt - (1.0f - wgt) * lam.x), 0.0f);
       dn = nl - lam.x;
@@ -1756,7 +1849,23 @@ struct Solver {
           float4 ex = R[PO + 2]; ex.w = lim; R[PO + 2] = ex;
         }
       }
-      if (role != 0.0f) {
+      if (const uint32_t FM = EXT ? w.h(W_FMODE) : 0u) {
+        // Coupled (normal, t1, t2) re-solve; overwrites all three impulses (wgsl.js:3081-3097).
+        const f3 vn = (tb[0] + cross(tb[1], rb)) - (ta[0] + cross(ta[1], ra));
+        const f3 vt = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
+        const uint32_t FMO = w.h(W_FMO);
+        const f3 nlam = contact_block(FM, dot(n, vn), mk2(dot(t1, vt), dot(t2, vt)), R[FMO], R[FMO + 1], -r1.w, r2.w, xyz(lam));
+        const f3 d = nlam - xyz(lam);
+        if (PATCH && role != 1.0f) art_[w.h(W_A_ULOAD) + us] += d.x;
+        ta[0] += xyz(R[7]) * d.x + xyz(R[11]) * d.y + xyz(R[15]) * d.z;
+        ta[1] += xyz(R[8]) * d.x + xyz(R[12]) * d.y + xyz(R[16]) * d.z;
+        tb[0] += xyz(R[9]) * d.x + xyz(R[13]) * d.y + xyz(R[17]) * d.z;
+        tb[1] += xyz(R[10]) * d.x + xyz(R[14]) * d.y + xyz(R[18]) * d.z;
+        const f3 fn_ = n * d.x, ft = t1 * d.y + t2 * d.z;
+        add_imp(a, -fn_ - ft, -cross(ra, fn_) - cross(fra, ft));
+        add_imp(b, fn_ + ft, cross(rb, fn_) + cross(frb, ft));
+        lam = F4(nlam, lam.w);
+      } else if (role != 0.0f) {
         f3 vr2 = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
         float4 ki = R[5];
         float v1 = dot(t1, vr2) + bias.x, v2 = dot(t2, vr2) + bias.y;
@@ -1773,7 +1882,7 @@ struct Solver {
         add_imp(b, f, cross(frb, f));
       }
       R[6] = lam;
-      if (w.h(W_ANG)) {
+      if (EXT && w.h(W_ANG)) {
         // Spin within mu_t N, rolling within a disk of mu_r N; under friction patches only the region's first
         // anchor carries them, with the pooled load (wgsl.js:3144-3173).
         const float4 g = R[19];
@@ -1800,7 +1909,7 @@ struct Solver {
           add_imp(b, mk3(0, 0, 0), tq);
         }
       }
-      if (w.h(W_TORS) && r == w.d.sched[w.S.UTP + us]) unit_torsion(us, a, b, ta, tb);
+      if (EXT && w.h(W_TORS) && r == w.d.sched[w.S.UTP + us]) unit_torsion(us, a, b, ta, tb);
     }
     if (a >= 0) { tw[(uint32_t)a * 2] = F4(ta[0], 0.0f); tw[(uint32_t)a * 2 + 1] = F4(ta[1], 0.0f); }
     if (b >= 0) { tw[(uint32_t)b * 2] = F4(tb[0], 0.0f); tw[(uint32_t)b * 2 + 1] = F4(tb[1], 0.0f); }
@@ -2232,6 +2341,7 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
   const uint32_t* sched = w.d.sched;
   const SchedLayout& S = w.S;
   const uint32_t lane = tid & 63, wave = tid >> 6;
+  const bool ext = w.h(W_FMODE) || w.h(W_ANG) || w.h(W_TORS) || w.h(W_COMPL);
   for (uint32_t a = wave; a < NA; a += BS / 64) sv.art_propagate_rows(a, lane);
   if (w.h(W_PATCH) && mode == 1) {
     for (uint32_t us = tid; us < nu; us += BS) {
@@ -2246,14 +2356,18 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
   const uint4* slots = (const uint4*)(sched + S.SLOTT);
   for (uint32_t c = 0; c < NCOL && c < ncol; c++) {
     uint32_t s = sched[S.CSTART + c], e = sched[S.CSTART + c + 1];
-    for (uint32_t u = s + tid; u < e; u += BS) { uint4 sl = slots[u]; sv.solve_unit(sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
+    if (ext)
+      for (uint32_t u = s + tid; u < e; u += BS) { uint4 sl = slots[u]; sv.solve_unit<true>(sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
+    else
+      for (uint32_t u = s + tid; u < e; u += BS) { uint4 sl = slots[u]; sv.solve_unit<false>(sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
     __syncthreads();
   }
   if (tid == 0) {
     uint32_t MAXC = w.MAXC();
     for (uint32_t u = sched[S.CSTART + NCOL]; u < sched[S.CSTART + NCOL + 1]; u++) {
       uint4 sl = slots[min(u, MAXC - 1)];
-      sv.solve_unit(sl.y, sl.x, (int)sl.z, (int)sl.w, mode);
+      if (ext) sv.solve_unit<true>(sl.y, sl.x, (int)sl.z, (int)sl.w, mode);
+      else sv.solve_unit<false>(sl.y, sl.x, (int)sl.z, (int)sl.w, mode);
     }
   }
   __syncthreads();
@@ -2272,8 +2386,10 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
 // colored rows of (lam - lam_last - e) . e, as 256 strided partials (partial j over rows j, j + 256, ...) added in
 // order, whatever the block size; below zero restarts (k = 0), else k + 1 and beta = (k - 1) / (k + 2) for k > 1.
 // Then the rows extrapolate color by color (the overflow color serially) and the articulations take their impulses.
+// kb: (k, beta) of the previous step, returned updated.  Not inlined: inlined, it pushed k_solve into spilling.
 template <int BS>
-DEV void momentum_step(const Solver& sv, uint32_t ncol, uint32_t it, float& mk, float& mbeta) {
+__device__ __noinline__ f2 momentum_step(const Solver& sv, uint32_t ncol, uint32_t it, f2 kb) {
+  float mk = kb.x, mbeta = kb.y;
   const W& w = sv.w;
   const uint32_t tid = threadIdx.x, NA = w.NA(), NCOL = w.NCOL(), ROWS = w.ROWS(), MO = w.h(W_MO), MAXC = w.MAXC();
   const uint32_t lane = tid & 63, wave = tid >> 6;
@@ -2317,6 +2433,88 @@ DEV void momentum_step(const Solver& sv, uint32_t ncol, uint32_t it, float& mk, 
     sv.art_velocity_limits(a, lane);
   }
   __syncthreads();
+  return mk2(mk, mbeta);
+}
+
+// Impulses for the next substep's warm start (wgsl.js:3737-3767): every row's key is inserted by linear probing
+// (compare-and-swap, at most 32 probes) and each key keeps the value of its largest row order, so the table repeats
+// run to run.  The winners were cleared with this table in prepare.  Not inlined, to keep k_solve's registers for
+// the common path.
+template <int BS>
+__device__ __noinline__ void warm_store(const Solver& sv, uint32_t nrows) {
+  const W& w = sv.w;
+  const WorldDev& wd = w.d;
+  const SchedLayout& S = w.S;
+  const uint32_t tid = threadIdx.x, MAXC = w.MAXC(), WH = w.h(W_WH);
+  uint32_t* sched = wd.sched;
+  const uint32_t nc = min(nrows, MAXC), nw = nc + min(sched[S.META + 3], w.h(W_MAXDENSE)), wb = (1 - sv.parity) * WH;
+  unsigned long long* win = (unsigned long long*)(sched + S.WWIN);
+  for (uint32_t i = tid; i < nw; i += BS) {
+    sched[S.WSLOT + i] = 0xffffffffu;
+    const uint32_t key = sv.warm_key(i, nc);
+    if (key == 0xffffffffu) continue;
+    uint32_t h = key % WH;
+    for (uint32_t probe = 0; probe < 32; probe++) {
+      const uint32_t old = atomicCAS(&wd.warm[wb + h], 0u, key);
+      if (old == 0u || old == key) { atomicMax(&win[h], sv.warm_order(sv.warm_value(i, nc).w, i)); sched[S.WSLOT + i] = h; break; }
+      h = (h + 1) % WH;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < nw; i += BS) {
+    const uint32_t h = sched[S.WSLOT + i];  // written by this thread above
+    if (h >= WH) continue;
+    const float4 v = sv.warm_value(i, nc);
+    if (__hip_atomic_load(&win[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != sv.warm_order(v.w, i)) continue;
+    ((uint4*)(wd.warm + 2 * WH))[wb + h] = make_uint4(fbits(v.w), fbits(v.x), fbits(v.y), fbits(v.z));
+  }
+  __syncthreads();
+}
+
+// Velocity pass (velocity_iterations, wgsl.js:3700-3736): positions integrate with the velocities reached by the
+// position sweeps (kept as a snapshot); bias-free sweeps from the accumulated impulses then set the output
+// velocities.  Contact right-hand sides become 0, or a speculative gap still open at the snapshot velocity.
+// Not inlined, to keep k_solve's registers for the common path.
+template <int BS>
+__device__ __noinline__ void velocity_pass(const Solver& sv, uint32_t ncol, uint32_t nu, uint32_t nd) {
+  const W& w = sv.w;
+  const WorldDev& wd = w.d;
+  const SchedLayout& S = w.S;
+  const uint32_t tid = threadIdx.x, NB = w.NB(), ND = w.ND(), MAXC = w.MAXC();
+  const uint32_t* sched = wd.sched;
+  float* art_ = wd.art;
+  const uint32_t VI = w.h(W_VELITERS);
+  const uint32_t SN = w.h(W_A_SNAP), ROWS = w.ROWS();
+  const float dt = w.hf(W_DT);
+  for (uint32_t b = tid; b < NB; b += BS) { w.st3(SN + b * 8, xyz(sv.tw[b * 2])); w.st3(SN + b * 8 + 4, xyz(sv.tw[b * 2 + 1])); }
+  for (uint32_t i = tid; i < ND; i += BS) art_[SN + NB * 8 + i] = sv.laqd[i];
+  for (uint32_t us = tid; us < nu; us += BS) {
+    const uint32_t key = sched[S.UKEY + us];
+    const int a = (int)(key & 0xffffu) - 1, b = (int)(key >> 16) - 1;
+    f3 ta[2], tb[2];
+    sv.load_tw(a, ta); sv.load_tw(b, tb);
+    const uint32_t s0 = sched[S.USTART + us];
+    for (uint32_t i = s0; i < min(s0 + sched[S.USIZE + us], MAXC); i++) {
+      float4* R = wd.rows + (size_t)i * ROWS;
+      const float phi = R[6].w;
+      if (phi >= 1e29f) continue;
+      const float jv = dot(xyz(R[0]), (tb[0] + cross(tb[1], xyz(R[4]))) - (ta[0] + cross(ta[1], xyz(R[3]))));
+      R[1].w = (phi > 0.0f && phi + dt * jv > 1e-6f) ? phi / dt : 0.0f;
+    }
+  }
+  for (uint32_t i = tid; i < nd; i += BS) {
+    const uint32_t o = w.h(W_A_DENSE) + i * w.h(W_DPS);
+    if (art_[o + 21] > 0.5f || art_[o + 22] >= 1e29f) continue;
+    const uint4 ai = w.art_info(tu(art_[o]));
+    float jv = 0.0f;
+    for (uint32_t k = 0; k < ai.w; k++) jv += art_[o + w.h(W_DHDR) + k] * sv.laqd[ai.z + k];
+    const float phi = art_[o + 22];
+    art_[o + 4] = (phi > 0.0f && phi + dt * jv > 1e-6f) ? phi / dt : 0.0f;
+  }
+  __syncthreads();
+  Solver vs = sv;  // the bias-free copy; k_solve's solver is never written after set-up
+  vs.vpass = true;
+  for (uint32_t it = 0; it < VI; it++) sweep<BS>(vs, 1, ncol, nu, nd);
 }
 
 template <int BS, bool TWL>
@@ -2447,47 +2645,12 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   dense_phase<BS>(sv, 0, nd);
   sweep<BS>(sv, 0, ncol, nu, nd);
   const uint32_t iters = w.h(W_ITERS);
-  float mk = 0.0f, mbeta = 0.0f;
+  f2 kb = mk2(0.0f, 0.0f);
   for (uint32_t it = 0; it < iters; it++) {
-    if (NEST) momentum_step<BS>(sv, ncol, it, mk, mbeta);
+    if (NEST) kb = momentum_step<BS>(sv, ncol, it, kb);
     sweep<BS>(sv, 1, ncol, nu, nd);
   }
-  if (const uint32_t VI = w.h(W_VELITERS)) {
-    // Velocity pass (velocity_iterations, wgsl.js:3700-3736): positions integrate with the velocities reached by
-    // the position sweeps (kept as a snapshot); bias-free sweeps from the accumulated impulses then set the output
-    // velocities.  Contact right-hand sides become 0, or a speculative gap still open at the snapshot velocity.
-    const uint32_t SN = w.h(W_A_SNAP), ROWS = w.ROWS();
-    const float dt = w.hf(W_DT);
-    for (uint32_t b = tid; b < NB; b += BS) { w.st3(SN + b * 8, xyz(sv.tw[b * 2])); w.st3(SN + b * 8 + 4, xyz(sv.tw[b * 2 + 1])); }
-    for (uint32_t i = tid; i < ND; i += BS) art_[SN + NB * 8 + i] = sv.laqd[i];
-    for (uint32_t us = tid; us < nu; us += BS) {
-      const uint32_t key = sched[S.UKEY + us];
-      const int a = (int)(key & 0xffffu) - 1, b = (int)(key >> 16) - 1;
-      f3 ta[2], tb[2];
-      sv.load_tw(a, ta); sv.load_tw(b, tb);
-      const uint32_t s0 = sched[S.USTART + us];
-      for (uint32_t i = s0; i < min(s0 + sched[S.USIZE + us], MAXC); i++) {
-        float4* R = wd.rows + (size_t)i * ROWS;
-        const float phi = R[6].w;
-        if (phi >= 1e29f) continue;
-        const float jv = dot(xyz(R[0]), (tb[0] + cross(tb[1], xyz(R[4]))) - (ta[0] + cross(ta[1], xyz(R[3]))));
-        R[1].w = (phi > 0.0f && phi + dt * jv > 1e-6f) ? phi / dt : 0.0f;
-      }
-    }
-    for (uint32_t i = tid; i < nd; i += BS) {
-      const uint32_t o = w.h(W_A_DENSE) + i * w.h(W_DPS);
-      if (art_[o + 21] > 0.5f || art_[o + 22] >= 1e29f) continue;
-      const uint4 ai = w.art_info(tu(art_[o]));
-      float jv = 0.0f;
-      for (uint32_t k = 0; k < ai.w; k++) jv += art_[o + w.h(W_DHDR) + k] * sv.laqd[ai.z + k];
-      const float phi = art_[o + 22];
-      art_[o + 4] = (phi > 0.0f && phi + dt * jv > 1e-6f) ? phi / dt : 0.0f;
-    }
-    __syncthreads();
-    sv.vpass = true;
-    for (uint32_t it = 0; it < VI; it++) sweep<BS>(sv, 1, ncol, nu, nd);
-    sv.vpass = false;
-  }
+  if (w.h(W_VELITERS)) velocity_pass<BS>(sv, ncol, nu, nd);
   // Mutable mirror fields back to the link blocks (p, dv, v, u, tau; S, U, D^-1, COM are unchanged).
   for (uint32_t i = tid; i < MF; i += BS) {
     uint32_t l = mirror_link(i), n = sv.mnd(l), f = i - (sv.lmeta[l].y >> 8);
@@ -2495,33 +2658,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   }
   __syncthreads();
   const uint32_t nrows = sched[S.USTART + nu];
-  if (const uint32_t WH = w.h(W_WH)) {
-    // Impulses for the next substep's warm start (wgsl.js:3737-3767): every row's key is inserted by linear probing
-    // (compare-and-swap, at most 32 probes) and each key keeps the value of its largest row order, so the table
-    // repeats run to run.  The winners were cleared with this table in prepare.
-    const uint32_t nc = min(nrows, MAXC), nw = nc + min(sched[S.META + 3], w.h(W_MAXDENSE)), wb = (1 - sv.parity) * WH;
-    unsigned long long* win = (unsigned long long*)(sched + S.WWIN);
-    for (uint32_t i = tid; i < nw; i += BS) {
-      sched[S.WSLOT + i] = 0xffffffffu;
-      const uint32_t key = sv.warm_key(i, nc);
-      if (key == 0xffffffffu) continue;
-      uint32_t h = key % WH;
-      for (uint32_t probe = 0; probe < 32; probe++) {
-        const uint32_t old = atomicCAS(&wd.warm[wb + h], 0u, key);
-        if (old == 0u || old == key) { atomicMax(&win[h], sv.warm_order(sv.warm_value(i, nc).w, i)); sched[S.WSLOT + i] = h; break; }
-        h = (h + 1) % WH;
-      }
-    }
-    __syncthreads();
-    for (uint32_t i = tid; i < nw; i += BS) {
-      const uint32_t h = sched[S.WSLOT + i];  // written by this thread above
-      if (h >= WH) continue;
-      const float4 v = sv.warm_value(i, nc);
-      if (__hip_atomic_load(&win[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != sv.warm_order(v.w, i)) continue;
-      ((uint4*)(wd.warm + 2 * WH))[wb + h] = make_uint4(fbits(v.w), fbits(v.x), fbits(v.y), fbits(v.z));
-    }
-    __syncthreads();
-  }
+  if (w.h(W_WH)) warm_store<BS>(sv, nrows);
   const bool PATCH = w.h(W_PATCH) != 0;
   const uint32_t PB = w.h(W_P_BODY), PP = w.h(W_P_PREV);
   if (PATCH) {

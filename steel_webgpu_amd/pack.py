@@ -496,8 +496,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     fm = SV.get("friction_mode") or "fixed_normal"
     if fm not in fmodes:
         raise ValueError(f"friction_mode {fm} is not supported")
-    if fmodes.index(fm) != 0:
-        raise UnsupportedError(f"friction_mode {fm} is not supported yet")
+    fmode = fmodes.index(fm)
     for k, default in (("enable_bilateral_preelimination", False), ("propagation_same_articulation_rows", False), ("update_mass_matrix_interval", 1)):
         if SV.get(k) is not None and SV.get(k) != default:
             raise ValueError(f"{k}={SV[k]} is not supported")
@@ -512,14 +511,14 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         raise UnsupportedError("free-body torsional/rolling friction is not supported yet")
     compl = bool(SV.get("contact_compliance"))
     anchorBeta = SV.get("friction_anchor_beta", 0.2)
-    if anchorBeta is None or compl:  # contact_compliance turns anchors off (sim.js:421-422)
+    if anchorBeta is None or compl or fmode > 0:  # coupled friction modes and compliance turn anchors off (sim.js:421-422)
         anchorBeta = 0.0
     angOn = any((s.get("condim", 3) or 3) >= 4 for s in S["shapes"])  # sim.js:425-429 (articulated and free-rigid forms refused above)
     PO = 34 if angOn else 19
     FMO = PO + (4 if anchorBeta > 0 else 0)
     nest = bool(SV.get("nesterov_momentum"))
     tors = any((s.get("torsion_radius") or 0) > 0 for s in S["shapes"])
-    TO = FMO
+    TO = FMO + (2 if fmode > 0 else 0)
     CO = TO + (5 if tors else 0)
     MO = CO + (1 if compl else 0)
     ROWS = MO + (2 if nest else 0)
@@ -623,6 +622,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     # Loop closures (ball or revolute) inside one articulation or to the world (sim.js:372-385).
     H("W_NLOOP", len(loops))
     H("W_ANG", 1 if angOn else 0)
+    H("W_FMODE", fmode)
+    H("W_FMO", FMO)
     if ctrl:  # the fader controller (sim.js:388-393)
         H("W_CTRL_KIND", 1)
         H("W_CTRL_A", qMap[ctrl["fader_q"]])

@@ -233,6 +233,9 @@ SCENES = {
     "loops": loop_rig,
     "spin_roll": spin_roll_rig,
     "fader": fader_rig,
+    "boxes_desaxce": lambda: _with_solver(workloads.boxes64(n=40), friction_mode="bisection_desaxce"),
+    "quad_newton": lambda: _with_solver(workloads.quadruped_world(18), friction_mode="coulomb_newton"),
+    "quad_bisect": lambda: _with_solver(workloads.quadruped_world(19), friction_mode="bisection"),
     # drive rows with velocity limits (driven dofs fused into the sweep clamp) and effort limits
     "quad_drows": lambda: _with_solver(workloads.quadruped_world(17, drive=dict(targetKe=60.0, targetKd=2.0, velocityLimit=0.8, effortLimit=4.0)),
                                        drive_mode="rows", enable_joint_velocity_limits=True),
@@ -284,7 +287,8 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
                                          ("boxes_nest", 150), ("quad_nest", 200), ("quad_tors", 200), ("d6", 200),
                                          ("boxes_vel", 150), ("quad_vel", 200), ("kinematic", 200),
                                          ("boxes_compl", 150), ("quad_compl", 200), ("quad_drows", 200),
-                                         ("loops", 200), ("spin_roll", 200), ("fader", 200)])
+                                         ("loops", 200), ("spin_roll", 200), ("fader", 200),
+                                         ("boxes_desaxce", 150), ("quad_newton", 200), ("quad_bisect", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])

@@ -2,6 +2,7 @@
 published inverses, libm, closed forms, brute force, definitions (first-fit coloring, KKT conditions),
 invariants (momentum, energy).  The reference ships no physics fixtures or golden vectors, so these are
 what pins the oracle."""
+import ctypes
 import math
 
 import numpy as np
@@ -1575,3 +1576,53 @@ def test_fader_controller_sets_the_wheel_target_speed_from_the_slider_each_subst
         seen.add(round(min(max(q0 / travel, 0.0), 1.0), 1))
     assert 0.0 in seen and 1.0 in seen and len(seen) > 4
     assert abs(o.art[A["A_QD"] + pw.h("W_CTRL_B")] - vmax) < 0.05 * vmax
+
+
+def test_coulomb_newton_block_meets_the_coulomb_law_and_bisection_the_normal_target():
+    """friction_mode blocks (wgsl.js:2583-2653) on random well-conditioned 3x3 Delassus blocks G, start impulses
+    lam and live velocities u (post-velocity u' = u + G (lam' - lam)):
+    - coulomb_newton: lam' is a Coulomb point: either separating (lam' = 0, u'_n >= target) or touching with
+      u'_n = target and either sticking (u'_t = 0, |lam'_t| <= mu lam'_n) or sliding (|lam'_t| = mu lam'_n,
+      lam'_t opposite u'_t);
+    - bisection and bisection_desaxce: lam'_n >= 0, |lam'_t| <= mu lam'_n, and u'_n meets the (De Saxce-shifted
+      for mode 2) target when the root lies inside the search bracket [0, max(2 lam_n, (target - u_n) / G_nn +
+      lam_n, 1)] (a coupled block can put it beyond; the search then returns the bracket's end)."""
+    L = lib()
+    L.oracle_contact_block_f64.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(11)
+    kinds = {"stick": 0, "slide": 0, "sep": 0}
+    for trial in range(400):
+        M = rng.normal(size=(3, 3))
+        G = M @ M.T + 0.3 * np.eye(3)
+        lam = np.array([abs(rng.normal()), 0.1 * rng.normal(), 0.1 * rng.normal()])
+        u = rng.normal(size=3)
+        mu = rng.uniform(0.1, 1.2)
+        vt = rng.uniform(-0.2, 0.2)
+        for mode in (1, 2, 3):
+            inp = np.array([u[0], u[1], u[2], G[0, 0], G[0, 1], G[0, 2], G[1, 1], G[1, 2], G[2, 2], vt, mu, *lam])
+            out = np.zeros(3)
+            L.oracle_contact_block_f64(mode, inp.ctypes.data, out.ctypes.data)
+            up = u + G @ (out - lam)
+            lt = np.hypot(out[1], out[2])
+            assert out[0] >= 0 and lt <= mu * out[0] * (1 + 1e-9) + 1e-12, (trial, mode)
+            if mode == 3:
+                if out[0] == 0:
+                    assert np.all(out == 0) and up[0] >= vt - 1e-9, trial
+                    kinds["sep"] += 1
+                    continue
+                # the root search stops at |f| < 1e-6 (1 + phi0): residuals of that order remain
+                assert abs(up[0] - vt) <= 2e-5 * (1 + abs(vt) + out[0] * G[0, 0]), (trial, up[0], vt)
+                vtn = np.hypot(up[1], up[2])
+                if lt < mu * out[0] * (1 - 1e-6):
+                    assert vtn <= 2e-5 * (1 + out[0] * G[0, 0]), (trial, vtn)
+                    kinds["stick"] += 1
+                else:
+                    assert vtn <= 2e-5 * (1 + out[0] * G[0, 0]) or np.dot(out[1:], up[1:]) <= -0.999 * lt * vtn, trial
+                    kinds["slide"] += 1
+            else:
+                target = vt + (mu * np.hypot(u[1], u[2]) if mode == 2 else 0.0)
+                hi = max(2 * lam[0], (target - u[0]) / G[0, 0] + lam[0], 1.0)
+                if 0 < out[0] < 0.999 * hi:
+                    assert abs(up[0] - target) <= 1e-4 * (1 + abs(target) + out[0] * G[0, 0]), (trial, mode, up[0], target)
+                    kinds["bis"] = kinds.get("bis", 0) + 1
+    assert min(kinds.values()) > 10, kinds
