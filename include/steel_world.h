@@ -83,6 +83,9 @@ enum steel_world_field {
   /* friction_mode (u32: 0 fixed_normal, 1 bisection, 2 bisection_desaxce, 3 coulomb_newton) and the row-record
      offset of the coupled block's Delassus entries (u32) */
   W_FMODE, W_FMO,
+  /* enable_torsional_rolling_friction: articulated contacts solve (t1, t2, spin, roll1, roll2) in Steel's elliptic
+     cone block (u32 flag), the row offset of its 5x5 response (u32), the creep speed (f32) */
+  W_ANG_ART, W_EO, W_F_CREEP,
   W_FIELD_COUNT
 };
 

@@ -68,7 +68,8 @@ template <class T> struct World {
   uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
-  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE, NLOOP, SF_LOOP, ANG, CTRL_KIND, CTRL_A, CTRL_B, SF_CTRL, FMODE, FMO;
+  uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE, NLOOP, SF_LOOP, ANG, CTRL_KIND, CTRL_A, CTRL_B, SF_CTRL, FMODE, FMO, ANG_ART, EO;
+  T F_CREEP;
   bool vpass = false;  // velocity pass: rows without position bias or regularization
   T mom_k = 0, mom_beta = 0;  // Nesterov restart counter and extrapolation factor of the current substep
   uint32_t VLIM_ON, VLIM_FREE, JFL, A_JFL, SF_PATH, DRIVEN, PATH_FRAMES, INTEGRAL_COUNT, TGT_HELD, A_INT, SLEEP, A_SLP, NMIMIC, SF_MIMIC, SU_ACON;
@@ -122,7 +123,7 @@ template <class T> struct World {
     A_RLN = h[W_A_RLN]; A_ULOAD = h[W_A_ULOAD]; A_DENSE = h[W_A_DENSE]; A_VLG = h[W_A_VLG]; A_AVL = h[W_A_AVL];
     WH = h[W_WH]; NEST = h[W_NEST]; MO = h[W_MO]; TORS = h[W_TORS]; TO = h[W_TO]; VELITERS = h[W_VELITERS]; A_SNAP = h[W_A_SNAP]; COMPL = h[W_COMPL]; CO = h[W_CO];
     NLOOP = h[W_NLOOP]; SF_LOOP = h[W_SF_LOOP]; ANG = h[W_ANG];
-    FMODE = h[W_FMODE]; FMO = h[W_FMO];
+    FMODE = h[W_FMODE]; FMO = h[W_FMO]; ANG_ART = h[W_ANG_ART]; EO = h[W_EO]; F_CREEP = hf(W_F_CREEP);
     CTRL_KIND = h[W_CTRL_KIND]; CTRL_A = h[W_CTRL_A]; CTRL_B = h[W_CTRL_B]; SF_CTRL = h[W_SF_CTRL];
     DRIVE_ROWS = h[W_DRIVE_ROWS]; FUSE_VLIM = h[W_FUSE_VLIM]; VEL_DRIVE_ACTIVE = h[W_VEL_DRIVE_ACTIVE];
     VLIM_ON = h[W_VLIM_ON]; F_VLIM_FRAC = hf(W_F_VLIM_FRAC); VLIM_FREE = h[W_VLIM_FREE]; JFL = h[W_JFL]; A_JFL = h[W_A_JFL];
@@ -346,7 +347,9 @@ template <class T> struct World {
     T mu = combine(statF[SF_SHAPE + sa * 7].z, statF[SF_SHAPE + sb * 7].z, FCOMB);
     bool art_c = (ba >= 0 && body_kind(ba) >= 2) || (bb >= 0 && body_kind(bb) >= 2);
     T mt = 0, mr = 0;
-    if (!art_c) {
+    if (art_c) {
+      if (ANG_ART) { mt = combine(ma.y, mb.y, FCOMB); mr = combine(ma.z, mb.z, FCOMB); }
+    } else {
       mt = mx(ma.w >= T(4) ? ma.y : T(0), mb.w >= T(4) ? mb.y : T(0));
       mr = mx(ma.w >= T(6) ? ma.z : T(0), mb.w >= T(6) ? mb.z : T(0));
     }
@@ -2373,6 +2376,26 @@ template <class T> struct World {
         rows[o + 19] = {mat.y, mat.z, KA[0] > T(1e-12) ? T(1) / ks : T(0), T(0)};
         rows[o + 20] = {r11 / rd, -r01 / rd, r00 / rd, T(0)};
         rows[o + 21] = {0, 0, 0, 0};
+        if (ANG_ART) {
+          // Raw 5x5 response H (t1, t2, spin, roll1, roll2), upper triangle, for the elliptic block (wgsl.js:2244-2267).
+          V3<T> JA[5][2], JB[5][2], RA[5][2], RB[5][2];
+          for (uint32_t q = 0; q < 2; q++) {
+            V3<T> d = dirs[q + 1];
+            JA[q][0] = -d; JA[q][1] = -cross(fa, d); JB[q][0] = d; JB[q][1] = cross(fb, d);
+            RA[q][0] = Ma[q + 1][0]; RA[q][1] = Ma[q + 1][1]; RB[q][0] = Mb[q + 1][0]; RB[q][1] = Mb[q + 1][1];
+          }
+          for (uint32_t q = 0; q < 3; q++) {
+            JA[q + 2][0] = {0, 0, 0}; JA[q + 2][1] = -dirs[q]; JB[q + 2][0] = {0, 0, 0}; JB[q + 2][1] = dirs[q];
+            RA[q + 2][0] = MA[q][0]; RA[q + 2][1] = MA[q][1]; RB[q + 2][0] = MB[q][0]; RB[q + 2][1] = MB[q][1];
+          }
+          T Hu[16];
+          uint32_t idx = 0;
+          for (uint32_t x = 0; x < 5; x++)
+            for (uint32_t y = x; y < 5; y++)
+              Hu[idx++] = dot(JA[x][0], RA[y][0]) + dot(JA[x][1], RA[y][1]) + dot(JB[x][0], RB[y][0]) + dot(JB[x][1], RB[y][1]);
+          Hu[15] = 0;
+          for (uint32_t q = 0; q < 4; q++) rows[o + EO + q] = {Hu[q * 4], Hu[q * 4 + 1], Hu[q * 4 + 2], Hu[q * 4 + 3]};
+        }
       }
     }
     if (TORS) for (uint32_t sp = 0; sp < nu; sp++) build_torsion(sp);
@@ -2683,6 +2706,160 @@ template <class T> struct World {
     imp[(uint32_t)b * 2] = imp[(uint32_t)b * 2] + v4(f, T(0));
     imp[(uint32_t)b * 2 + 1] = imp[(uint32_t)b * 2 + 1] + v4(t, T(0));
   }
+  // ---- Steel's joint torsional/rolling cone for articulated contacts (PropagationAngularFrictionBlock, elliptic cone;
+  // wgsl.js:2655-2925): rows (t1, t2, spin, roll1, roll2) minimize their quadratic over the ball |lambda / mu| <= N,
+  // by a More-Sorensen trust region (proximal fallback), relaxed by omega and line-searched from the current
+  // impulses moved into the cone.
+  struct Ball { T y[5]; bool ok, interior; };
+  static T cone_norm(const T* y) { T s = 0; for (uint32_t k = 0; k < 5; k++) s += y[k] * y[k]; return std::sqrt(s); }
+  static void cone_project(T* y, T load) {
+    T n = cone_norm(y);
+    if (n > load) for (uint32_t k = 0; k < 5; k++) y[k] *= load / n;
+  }
+  static bool live5(const T* H, const bool* act, uint32_t k) { return act[k] && H[k * 5 + k] > T(0); }
+  // Cholesky of H + alpha I over the live rows; fails when a pivot keeps under 1e-6 of its diagonal.
+  static bool factor5(const T* H, const bool* act, T alpha, T* L) {
+    for (uint32_t k = 0; k < 5; k++)
+      for (uint32_t l = 0; l <= k; l++) {
+        T v = (live5(H, act, k) && live5(H, act, l)) ? H[k * 5 + l] : T(0);
+        if (k == l) v = live5(H, act, k) ? v + alpha : T(1);
+        for (uint32_t m = 0; m < l; m++) v -= L[k * 5 + m] * L[l * 5 + m];
+        if (k == l) {
+          if (!(v > T(1e-6) * (live5(H, act, k) ? H[k * 5 + k] + alpha : T(1)))) return false;
+          L[k * 5 + k] = std::sqrt(v);
+        } else {
+          L[k * 5 + l] = v / L[l * 5 + l];
+        }
+      }
+    return true;
+  }
+  static void lower5(const T* L, T* x) {
+    for (uint32_t k = 0; k < 5; k++) {
+      for (uint32_t m = 0; m < k; m++) x[k] -= L[k * 5 + m] * x[m];
+      x[k] /= L[k * 5 + k];
+    }
+  }
+  static void upper5(const T* L, T* x) {
+    for (uint32_t kk = 0; kk < 5; kk++) {
+      uint32_t k = 4 - kk;
+      for (uint32_t m = k + 1; m < 5; m++) x[k] -= L[m * 5 + k] * x[m];
+      x[k] /= L[k * 5 + k];
+    }
+  }
+  // y = -(H + alpha I)^-1 c with alpha by Newton on 1/|y(alpha)| inside a bracket; ok when it met the 1e-5 boundary
+  // residual or found an interior minimizer.
+  static Ball cholesky_ball(const T* H, const T* c, const bool* act, T load) {
+    Ball r;
+    T b[5], cn = 0;
+    for (uint32_t k = 0; k < 5; k++) { b[k] = live5(H, act, k) ? c[k] : T(0); cn += b[k] * b[k]; r.y[k] = 0; }
+    cn = std::sqrt(cn);
+    r.interior = !(cn > T(0));
+    r.ok = true;
+    if (!(load > T(0) && cn > T(0))) return r;
+    r.ok = false;
+    T lo = 0, hi = cn / load, alpha = 0;
+    for (uint32_t it = 0; it < 40; it++) {
+      T nx = -1;
+      T L[25] = {};
+      if (factor5(H, act, alpha, L)) {
+        T yy[5];
+        for (uint32_t k = 0; k < 5; k++) yy[k] = -b[k];
+        lower5(L, yy); upper5(L, yy);
+        for (uint32_t k = 0; k < 5; k++) r.y[k] = yy[k];
+        T n = cone_norm(r.y);
+        if (alpha == T(0) && n <= load) { r.interior = true; r.ok = true; return r; }
+        if (std::fabs(n - load) <= T(1e-5) * load) { r.ok = true; break; }
+        if (n < load) hi = alpha; else lo = alpha;
+        T wv[5];
+        for (uint32_t k = 0; k < 5; k++) wv[k] = r.y[k];
+        lower5(L, wv);
+        T wn = 0;
+        for (uint32_t k = 0; k < 5; k++) wn += wv[k] * wv[k];
+        nx = alpha + n * n / wn * (n - load) / load;
+      } else {
+        lo = alpha;
+      }
+      if (hi - lo <= T(1e-4) * hi) break;
+      alpha = (nx > lo && nx < hi) ? nx : (lo > T(0) ? std::sqrt(lo * hi) : T(1e-3) * hi);
+    }
+    T n = cone_norm(r.y);
+    if (n > load) for (uint32_t k = 0; k < 5; k++) r.y[k] *= load / n;
+    return r;
+  }
+  // Minimize 0.5 y'Hy + c'y over |y| <= load; when dependent rows defeat Cholesky, a proximal term
+  // 0.5 rho (y - center/scale)' diag(H) (y - center/scale), rho raised tenfold until the ball solve converges.
+  static Ball trust_region(const T* H, const T* c, const bool* act, T load, const T* center, const T* scale) {
+    Ball first = cholesky_ball(H, c, act, load);
+    if (first.ok) return first;
+    T ce[5];
+    for (uint32_t k = 0; k < 5; k++) ce[k] = (act[k] && scale[k] > T(0)) ? center[k] / scale[k] : T(0);
+    T rho = T(1e-5);
+    for (uint32_t it = 0; it < 7; it++) {
+      T K[25], g[5];
+      for (uint32_t i = 0; i < 25; i++) K[i] = H[i];
+      for (uint32_t k = 0; k < 5; k++) {
+        T wt = live5(H, act, k) ? rho * H[k * 5 + k] : T(0);
+        K[k * 5 + k] += wt;
+        g[k] = c[k] - wt * ce[k];
+      }
+      rho *= T(10);
+      Ball t = cholesky_ball(K, g, act, load);
+      if (!t.ok) continue;
+      bool fin = true;
+      for (uint32_t k = 0; k < 5; k++) fin = fin && std::fabs(t.y[k]) < T(3.0e38);
+      if (fin) return t;
+    }
+    return first;
+  }
+  // One block visit: H0, g0 the rows' response and current velocity residual, mu the row coefficients, lam the
+  // current impulses, load the normal impulse; returns the new impulses.
+  void ell_block(const T* H0, const T* g0, const T* mu_in, const T* lam, T load_in, T* out) const {
+    T load = mx(load_in, T(0));
+    T mu[5], H[25], gr[5], big = 0;
+    for (uint32_t i = 0; i < 25; i++) H[i] = H0[i];
+    for (uint32_t k = 0; k < 5; k++) { gr[k] = g0[k]; mu[k] = mx(mu_in[k], T(0)); big = mx(big, H[k * 5 + k]); }
+    bool act[5];
+    T comp[5];
+    for (uint32_t k = 0; k < 5; k++) {
+      act[k] = mu[k] > T(0) && H[k * 5 + k] > T(1e-12) * big;
+      comp[k] = (k >= 2 && act[k] && load > T(0)) ? F_CREEP / (mu[k] * mu[k] * load) : T(0);
+      H[k * 5 + k] += comp[k]; gr[k] += comp[k] * lam[k];
+    }
+    T y0[5], c[5], y[5];
+    for (uint32_t k = 0; k < 5; k++) {
+      y0[k] = act[k] ? lam[k] / mu[k] : T(0);
+      gr[k] = act[k] ? gr[k] * mu[k] : T(0);
+      for (uint32_t l = 0; l < 5; l++) H[k * 5 + l] *= mu[k] * mu[l];
+    }
+    for (uint32_t k = 0; k < 5; k++) {
+      T v = gr[k];
+      for (uint32_t l = 0; l < 5; l++) v -= H[k * 5 + l] * y0[l];
+      c[k] = act[k] ? v : T(0);
+    }
+    Ball tr = trust_region(H, c, act, load, lam, mu_in);
+    for (uint32_t k = 0; k < 5; k++) y[k] = tr.y[k];
+    bool stick = tr.interior && cone_norm(y) <= load;
+    if (!stick) cone_project(y, load);
+    T stp[5], shift[5];
+    for (uint32_t k = 0; k < 5; k++) stp[k] = lam[k] + F_OMEGA * (mu[k] * y[k] - lam[k]);
+    if (F_OMEGA != T(1)) {
+      for (uint32_t k = 0; k < 5; k++) y[k] = act[k] ? stp[k] / mu_in[k] : T(0);
+      cone_project(y, load);
+      for (uint32_t k = 0; k < 5; k++) stp[k] = mu[k] * y[k];
+    }
+    for (uint32_t k = 0; k < 5; k++) y[k] = act[k] ? lam[k] / mu_in[k] : T(0);
+    cone_project(y, load);
+    for (uint32_t k = 0; k < 5; k++) { T b = mu[k] * y[k]; shift[k] = b - lam[k]; stp[k] -= b; }
+    T slope = 0, curv = 0;
+    for (uint32_t k = 0; k < 5; k++) {
+      T hs = 0, hp = 0;
+      for (uint32_t l = 0; l < 5; l++) { hs += H0[k * 5 + l] * shift[l]; hp += H0[k * 5 + l] * stp[l]; }
+      slope += stp[k] * (g0[k] + hs + comp[k] * (lam[k] + shift[k]));
+      curv += stp[k] * (hp + comp[k] * stp[k]);
+    }
+    T t = slope < T(0) ? (curv > -slope ? -slope / curv : T(1)) : T(0);
+    for (uint32_t k = 0; k < 5; k++) out[k] = lam[k] + shift[k] + (t > T(0) ? t * stp[k] : T(0));
+  }
   // ---- Steel's coupled contact solves (friction_mode bisection, bisection_desaxce, coulomb_newton; wgsl.js:2564-2653):
   // u = live (normal, t1, t2) velocities, G the 3x3 Delassus block (nn, nt1, nt2 | t1t1, t1t2, t2t2).
   static V2<T> tangent_step(V2<T> ut, V4<T> g1) {
@@ -2913,12 +3090,44 @@ template <class T> struct World {
         add_imp(a, -f, -cross(fra, f));
         add_imp(b, f, cross(frb, f));
       }
+      const bool art_u = (a >= 0 && body_kind(a) >= 2) || (b >= 0 && body_kind(b) >= 2);
+      const bool ell = ANG_ART && ANG && art_u && (rows[o + 19].x > T(0) || rows[o + 19].y > T(0));
+      if (ell) {
+        // Articulated spin/rolling: after the sliding pair, Steel's block over sliding, spin and rolling (wgsl.js:3113-3141).
+        T H5[25];
+        uint32_t idx = 0;
+        for (uint32_t x = 0; x < 5; x++)
+          for (uint32_t y2 = x; y2 < 5; y2++) {
+            T hv = (&rows[o + EO + idx / 4].x)[idx % 4];
+            H5[x * 5 + y2] = hv; H5[y2 * 5 + x] = hv;
+            idx++;
+          }
+        V3<T> vt = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
+        V3<T> wr = tb[1] - ta[1];
+        V4<T> la0 = rows[o + 21];
+        T g5[5] = {dot(t1, vt), dot(t2, vt), dot(n, wr), dot(t1, wr), dot(t2, wr)};
+        T mu5[5] = {r2.w, r2.w, rows[o + 19].x, rows[o + 19].y, rows[o + 19].y};
+        T l5[5] = {lam.y, lam.z, la0.x, la0.y, la0.z};
+        T nl5[5], d[5];
+        ell_block(H5, g5, mu5, l5, pooled, nl5);
+        for (uint32_t k = 0; k < 5; k++) d[k] = nl5[k] - l5[k];
+        ta[0] += xyz(rows[o + 11]) * d[0] + xyz(rows[o + 15]) * d[1] + xyz(rows[o + 22]) * d[2] + xyz(rows[o + 26]) * d[3] + xyz(rows[o + 30]) * d[4];
+        ta[1] += xyz(rows[o + 12]) * d[0] + xyz(rows[o + 16]) * d[1] + xyz(rows[o + 23]) * d[2] + xyz(rows[o + 27]) * d[3] + xyz(rows[o + 31]) * d[4];
+        tb[0] += xyz(rows[o + 13]) * d[0] + xyz(rows[o + 17]) * d[1] + xyz(rows[o + 24]) * d[2] + xyz(rows[o + 28]) * d[3] + xyz(rows[o + 32]) * d[4];
+        tb[1] += xyz(rows[o + 14]) * d[0] + xyz(rows[o + 18]) * d[1] + xyz(rows[o + 25]) * d[2] + xyz(rows[o + 29]) * d[3] + xyz(rows[o + 33]) * d[4];
+        V3<T> f = t1 * d[0] + t2 * d[1];
+        V3<T> tq = n * d[2] + t1 * d[3] + t2 * d[4];
+        add_imp(a, -f, -cross(fra, f) - tq);
+        add_imp(b, f, cross(frb, f) + tq);
+        lam.y = nl5[0]; lam.z = nl5[1];
+        rows[o + 21] = {nl5[2], nl5[3], nl5[4], T(0)};
+      }
       rows[o + 6] = lam;
       if (ANG) {
         // Spin within mu_t N, rolling within a disk of mu_r N; under friction patches only the region's first anchor
         // carries them, with the pooled load (wgsl.js:3144-3173).
         V4<T> g = rows[o + 19];
-        bool aon = !PATCH || role == T(1) || (role == T(2) && rows[o + PO + 1].w == T(0));
+        bool aon = !ell && (!PATCH || role == T(1) || (role == T(2) && rows[o + PO + 1].w == T(0)));
         if (aon && (g.x > T(0) || g.y > T(0))) {
           V4<T> la = rows[o + 21];
           V3<T> wrel = tb[1] - ta[1];
@@ -3665,6 +3874,15 @@ float oracle_atan2f(float y, float x) { return atan2f_c(y, x); }
 void oracle_friction_pair_solve(float a, float c, float d, float r0, float r1, float o0, float o1, float radius, float omega, float* out) {
   V2<float> r = World<float>::friction_pair_solve(a, c, d, r0, r1, o0, o1, radius, omega);
   out[0] = r.x; out[1] = r.y;
+}
+// The articulated spin/rolling cone block in FP64 (relaxation omega, creep speed creep): H0[25], g0[5], mu[5], lam[5],
+// load; out[5].
+void oracle_ell_block_f64(const double* H0, const double* g0, const double* mu, const double* lam, double load, double omega,
+                          double creep, double* out) {
+  World<double> w;
+  w.F_OMEGA = omega;
+  w.F_CREEP = creep;
+  w.ell_block(H0, g0, mu, lam, load, out);
 }
 // The coupled contact block in FP64: in[0] un, in[1..2] ut0, in[3..5] (gnn, gnt), in[6..8] (g_t1t1, g_t1t2,
 // g_t2t2), in[9] vtarget, in[10] mu, in[11..13] lam; out (ln, lt1, lt2).

@@ -127,7 +127,9 @@ struct Collider {
     float mu = combine(A0.z, B0.z, w.h(W_FRICTION_COMBINE));
     bool art_c = (ba >= 0 && w.body_kind(ba) >= 2) || (bb >= 0 && w.body_kind(bb) >= 2);
     float mt = 0.0f, mr = 0.0f;
-    if (!art_c) {
+    if (art_c) {
+      if (w.h(W_ANG_ART)) { mt = combine(ma.y, mb.y, w.h(W_FRICTION_COMBINE)); mr = combine(ma.z, mb.z, w.h(W_FRICTION_COMBINE)); }
+    } else {
       mt = mx(ma.w >= 4.0f ? ma.y : 0.0f, mb.w >= 4.0f ? mb.y : 0.0f);
       mr = mx(ma.w >= 6.0f ? ma.z : 0.0f, mb.w >= 6.0f ? mb.z : 0.0f);
     }

@@ -804,6 +804,26 @@ struct Prep {
       set_row(idx, 19, make_float4(mat.y, mat.z, KA[0] > 1e-12f ? 1.0f / ks : 0.0f, 0.0f));
       set_row(idx, 20, make_float4(r11 / rd, -r01 / rd, r00 / rd, 0.0f));
       set_row(idx, 21, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+      if (w.h(W_ANG_ART)) {
+        // Raw 5x5 response H (t1, t2, spin, roll1, roll2), upper triangle, for the elliptic block (wgsl.js:2244-2267).
+        f3 JA[5][2], JB[5][2], RA[5][2], RB[5][2];
+        for (uint32_t q = 0; q < 2; q++) {
+          const f3 dd = dirs[q + 1];
+          JA[q][0] = -dd; JA[q][1] = -cross(fa, dd); JB[q][0] = dd; JB[q][1] = cross(fb, dd);
+          RA[q][0] = Ma[q + 1][0]; RA[q][1] = Ma[q + 1][1]; RB[q][0] = Mb[q + 1][0]; RB[q][1] = Mb[q + 1][1];
+        }
+        for (uint32_t q = 0; q < 3; q++) {
+          JA[q + 2][0] = mk3(0, 0, 0); JA[q + 2][1] = -dirs[q]; JB[q + 2][0] = mk3(0, 0, 0); JB[q + 2][1] = dirs[q];
+          RA[q + 2][0] = MA[q][0]; RA[q + 2][1] = MA[q][1]; RB[q + 2][0] = MB[q][0]; RB[q + 2][1] = MB[q][1];
+        }
+        float Hu[16];
+        uint32_t hi = 0;
+        for (uint32_t x = 0; x < 5; x++)
+          for (uint32_t y = x; y < 5; y++)
+            Hu[hi++] = dot(JA[x][0], RA[y][0]) + dot(JA[x][1], RA[y][1]) + dot(JB[x][0], RB[y][0]) + dot(JB[x][1], RB[y][1]);
+        Hu[15] = 0.0f;
+        for (uint32_t q = 0; q < 4; q++) set_row(idx, w.h(W_EO) + q, make_float4(Hu[q * 4], Hu[q * 4 + 1], Hu[q * 4 + 2], Hu[q * 4 + 3]));
+      }
     }
   }
   DEV float row_weight(float phi) const { float reg = w.hf(W_REG); if (phi > 0.0f || reg <= 0.0f) return 1.0f; return 1.0f / (1.0f + reg); }
@@ -1644,6 +1664,158 @@ struct Solver {
     imp[(uint32_t)b * 2] = make_float4(i0.x + f.x, i0.y + f.y, i0.z + f.z, i0.w + 0.0f);
     imp[(uint32_t)b * 2 + 1] = make_float4(i1.x + t.x, i1.y + t.y, i1.z + t.z, i1.w + 0.0f);
   }
+  // ---- Steel's joint torsional/rolling cone for articulated contacts (PropagationAngularFrictionBlock, elliptic cone;
+  // wgsl.js:2655-2925): rows (t1, t2, spin, roll1, roll2) minimize their quadratic over the ball |lambda / mu| <= N,
+  // by a More-Sorensen trust region (proximal fallback), relaxed by omega and line-searched from the current
+  // impulses moved into the cone.
+  struct Ball { float y[5]; bool ok, interior; };
+  static DEV float cone_norm(const float* y) { float s = 0; for (uint32_t k = 0; k < 5; k++) s += y[k] * y[k]; return sqrtf(s); }
+  static DEV void cone_project(float* y, float load) {
+    float n = cone_norm(y);
+    if (n > load) for (uint32_t k = 0; k < 5; k++) y[k] *= load / n;
+  }
+  static DEV bool live5(const float* H, const bool* act, uint32_t k) { return act[k] && H[k * 5 + k] > 0.0f; }
+  // Cholesky of H + alpha I over the live rows; fails when a pivot keeps under 1e-6 of its diagonal.
+  static DEV bool factor5(const float* H, const bool* act, float alpha, float* L) {
+    for (uint32_t k = 0; k < 5; k++)
+      for (uint32_t l = 0; l <= k; l++) {
+        float v = (live5(H, act, k) && live5(H, act, l)) ? H[k * 5 + l] : 0.0f;
+        if (k == l) v = live5(H, act, k) ? v + alpha : 1.0f;
+        for (uint32_t m = 0; m < l; m++) v -= L[k * 5 + m] * L[l * 5 + m];
+        if (k == l) {
+          if (!(v > 1e-6f * (live5(H, act, k) ? H[k * 5 + k] + alpha : 1.0f))) return false;
+          L[k * 5 + k] = sqrtf(v);
+        } else {
+          L[k * 5 + l] = v / L[l * 5 + l];
+        }
+      }
+    return true;
+  }
+  static DEV void lower5(const float* L, float* x) {
+    for (uint32_t k = 0; k < 5; k++) {
+      for (uint32_t m = 0; m < k; m++) x[k] -= L[k * 5 + m] * x[m];
+      x[k] /= L[k * 5 + k];
+    }
+  }
+  static DEV void upper5(const float* L, float* x) {
+    for (uint32_t kk = 0; kk < 5; kk++) {
+      uint32_t k = 4 - kk;
+      for (uint32_t m = k + 1; m < 5; m++) x[k] -= L[m * 5 + k] * x[m];
+      x[k] /= L[k * 5 + k];
+    }
+  }
+  // y = -(H + alpha I)^-1 c with alpha by Newton on 1/|y(alpha)| inside a bracket; ok when it met the 1e-5 boundary
+  // residual or found an interior minimizer.
+  static DEVN Ball cholesky_ball(const float* H, const float* c, const bool* act, float load) {
+    Ball r;
+    float b[5], cn = 0;
+    for (uint32_t k = 0; k < 5; k++) { b[k] = live5(H, act, k) ? c[k] : 0.0f; cn += b[k] * b[k]; r.y[k] = 0; }
+    cn = sqrtf(cn);
+    r.interior = !(cn > 0.0f);
+    r.ok = true;
+    if (!(load > 0.0f && cn > 0.0f)) return r;
+    r.ok = false;
+    float lo = 0, hi = cn / load, alpha = 0;
+    for (uint32_t it = 0; it < 40; it++) {
+      float nx = -1;
+      float L[25] = {};
+      if (factor5(H, act, alpha, L)) {
+        float yy[5];
+        for (uint32_t k = 0; k < 5; k++) yy[k] = -b[k];
+        lower5(L, yy); upper5(L, yy);
+        for (uint32_t k = 0; k < 5; k++) r.y[k] = yy[k];
+        float n = cone_norm(r.y);
+        if (alpha == 0.0f && n <= load) { r.interior = true; r.ok = true; return r; }
+        if (fabsf(n - load) <= 1e-5f * load) { r.ok = true; break; }
+        if (n < load) hi = alpha; else lo = alpha;
+        float wv[5];
+        for (uint32_t k = 0; k < 5; k++) wv[k] = r.y[k];
+        lower5(L, wv);
+        float wn = 0;
+        for (uint32_t k = 0; k < 5; k++) wn += wv[k] * wv[k];
+        nx = alpha + n * n / wn * (n - load) / load;
+      } else {
+        lo = alpha;
+      }
+      if (hi - lo <= 1e-4f * hi) break;
+      alpha = (nx > lo && nx < hi) ? nx : (lo > 0.0f ? sqrtf(lo * hi) : 1e-3f * hi);
+    }
+    float n = cone_norm(r.y);
+    if (n > load) for (uint32_t k = 0; k < 5; k++) r.y[k] *= load / n;
+    return r;
+  }
+  // Minimize 0.5 y'Hy + c'y over |y| <= load; when dependent rows defeat Cholesky, a proximal term
+  // 0.5 rho (y - center/scale)' diag(H) (y - center/scale), rho raised tenfold until the ball solve converges.
+  static DEVN Ball trust_region(const float* H, const float* c, const bool* act, float load, const float* center, const float* scale) {
+    Ball first = cholesky_ball(H, c, act, load);
+    if (first.ok) return first;
+    float ce[5];
+    for (uint32_t k = 0; k < 5; k++) ce[k] = (act[k] && scale[k] > 0.0f) ? center[k] / scale[k] : 0.0f;
+    float rho = 1e-5f;
+    for (uint32_t it = 0; it < 7; it++) {
+      float K[25], g[5];
+      for (uint32_t i = 0; i < 25; i++) K[i] = H[i];
+      for (uint32_t k = 0; k < 5; k++) {
+        float wt = live5(H, act, k) ? rho * H[k * 5 + k] : 0.0f;
+        K[k * 5 + k] += wt;
+        g[k] = c[k] - wt * ce[k];
+      }
+      rho *= 10.0f;
+      Ball t = cholesky_ball(K, g, act, load);
+      if (!t.ok) continue;
+      bool fin = true;
+      for (uint32_t k = 0; k < 5; k++) fin = fin && fabsf(t.y[k]) < 3.0e38f;
+      if (fin) return t;
+    }
+    return first;
+  }
+  DEVN void ell_block(const float* H0, const float* g0, const float* mu_in, const float* lam, float load_in, float* out) const {
+    float load = mx(load_in, 0.0f);
+    float mu[5], H[25], gr[5], big = 0;
+    for (uint32_t i = 0; i < 25; i++) H[i] = H0[i];
+    for (uint32_t k = 0; k < 5; k++) { gr[k] = g0[k]; mu[k] = mx(mu_in[k], 0.0f); big = mx(big, H[k * 5 + k]); }
+    bool act[5];
+    float comp[5];
+    for (uint32_t k = 0; k < 5; k++) {
+      act[k] = mu[k] > 0.0f && H[k * 5 + k] > 1e-12f * big;
+      comp[k] = (k >= 2 && act[k] && load > 0.0f) ? w.hf(W_F_CREEP) / (mu[k] * mu[k] * load) : 0.0f;
+      H[k * 5 + k] += comp[k]; gr[k] += comp[k] * lam[k];
+    }
+    float y0[5], c[5], y[5];
+    for (uint32_t k = 0; k < 5; k++) {
+      y0[k] = act[k] ? lam[k] / mu[k] : 0.0f;
+      gr[k] = act[k] ? gr[k] * mu[k] : 0.0f;
+      for (uint32_t l = 0; l < 5; l++) H[k * 5 + l] *= mu[k] * mu[l];
+    }
+    for (uint32_t k = 0; k < 5; k++) {
+      float v = gr[k];
+      for (uint32_t l = 0; l < 5; l++) v -= H[k * 5 + l] * y0[l];
+      c[k] = act[k] ? v : 0.0f;
+    }
+    Ball tr = trust_region(H, c, act, load, lam, mu_in);
+    for (uint32_t k = 0; k < 5; k++) y[k] = tr.y[k];
+    bool stick = tr.interior && cone_norm(y) <= load;
+    if (!stick) cone_project(y, load);
+    float stp[5], shift[5];
+    for (uint32_t k = 0; k < 5; k++) stp[k] = lam[k] + w.hf(W_F_OMEGA) * (mu[k] * y[k] - lam[k]);
+    if (w.hf(W_F_OMEGA) != 1.0f) {
+      for (uint32_t k = 0; k < 5; k++) y[k] = act[k] ? stp[k] / mu_in[k] : 0.0f;
+      cone_project(y, load);
+      for (uint32_t k = 0; k < 5; k++) stp[k] = mu[k] * y[k];
+    }
+    for (uint32_t k = 0; k < 5; k++) y[k] = act[k] ? lam[k] / mu_in[k] : 0.0f;
+    cone_project(y, load);
+    for (uint32_t k = 0; k < 5; k++) { float b = mu[k] * y[k]; shift[k] = b - lam[k]; stp[k] -= b; }
+    float slope = 0, curv = 0;
+    for (uint32_t k = 0; k < 5; k++) {
+      float hs = 0, hp = 0;
+      for (uint32_t l = 0; l < 5; l++) { hs += H0[k * 5 + l] * shift[l]; hp += H0[k * 5 + l] * stp[l]; }
+      slope += stp[k] * (g0[k] + hs + comp[k] * (lam[k] + shift[k]));
+      curv += stp[k] * (hp + comp[k] * stp[k]);
+    }
+    float t = slope < 0.0f ? (curv > -slope ? -slope / curv : 1.0f) : 0.0f;
+    for (uint32_t k = 0; k < 5; k++) out[k] = lam[k] + shift[k] + (t > 0.0f ? t * stp[k] : 0.0f);
+  }
   // ---- Steel's coupled contact solves (friction_mode bisection, bisection_desaxce, coulomb_newton; wgsl.js:2564-2653):
   // u = live (normal, t1, t2) velocities, G the 3x3 Delassus block (nn, nt1, nt2 | t1t1, t1t2, t2t2).
   static DEV f2 tangent_step(f2 ut, float4 g1) {
@@ -1881,12 +2053,50 @@ struct Solver {
         add_imp(a, -f, -cross(fra, f));
         add_imp(b, f, cross(frb, f));
       }
+      bool ell = false;
+      if (EXT && w.h(W_ANG_ART)) {
+        const bool art_u = (a >= 0 && w.body_kind(a) >= 2) || (b >= 0 && w.body_kind(b) >= 2);
+        ell = w.h(W_ANG) && art_u && (R[19].x > 0.0f || R[19].y > 0.0f);
+      }
+      if (ell) {
+        // Articulated spin/rolling: after the sliding pair, Steel's block over sliding, spin and rolling (wgsl.js:3113-3141).
+        float H5[25];
+        uint32_t hidx = 0;
+        const uint32_t EO = w.h(W_EO);
+        for (uint32_t x = 0; x < 5; x++)
+          for (uint32_t y2 = x; y2 < 5; y2++) {
+            const float4 hv4 = R[EO + hidx / 4];
+            const uint32_t c4 = hidx % 4;
+            const float hv = c4 == 0 ? hv4.x : (c4 == 1 ? hv4.y : (c4 == 2 ? hv4.z : hv4.w));
+            H5[x * 5 + y2] = hv; H5[y2 * 5 + x] = hv;
+            hidx++;
+          }
+        const f3 vt = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
+        const f3 wr = tb[1] - ta[1];
+        const float4 la0 = R[21];
+        const float g5[5] = {dot(t1, vt), dot(t2, vt), dot(n, wr), dot(t1, wr), dot(t2, wr)};
+        const float mu5[5] = {r2.w, r2.w, R[19].x, R[19].y, R[19].y};
+        const float l5[5] = {lam.y, lam.z, la0.x, la0.y, la0.z};
+        float nl5[5], d[5];
+        ell_block(H5, g5, mu5, l5, pooled, nl5);
+        for (uint32_t k = 0; k < 5; k++) d[k] = nl5[k] - l5[k];
+        ta[0] += xyz(R[11]) * d[0] + xyz(R[15]) * d[1] + xyz(R[22]) * d[2] + xyz(R[26]) * d[3] + xyz(R[30]) * d[4];
+        ta[1] += xyz(R[12]) * d[0] + xyz(R[16]) * d[1] + xyz(R[23]) * d[2] + xyz(R[27]) * d[3] + xyz(R[31]) * d[4];
+        tb[0] += xyz(R[13]) * d[0] + xyz(R[17]) * d[1] + xyz(R[24]) * d[2] + xyz(R[28]) * d[3] + xyz(R[32]) * d[4];
+        tb[1] += xyz(R[14]) * d[0] + xyz(R[18]) * d[1] + xyz(R[25]) * d[2] + xyz(R[29]) * d[3] + xyz(R[33]) * d[4];
+        const f3 f = t1 * d[0] + t2 * d[1];
+        const f3 tq = n * d[2] + t1 * d[3] + t2 * d[4];
+        add_imp(a, -f, -cross(fra, f) - tq);
+        add_imp(b, f, cross(frb, f) + tq);
+        lam.y = nl5[0]; lam.z = nl5[1];
+        R[21] = make_float4(nl5[2], nl5[3], nl5[4], 0.0f);
+      }
       R[6] = lam;
       if (EXT && w.h(W_ANG)) {
         // Spin within mu_t N, rolling within a disk of mu_r N; under friction patches only the region's first
         // anchor carries them, with the pooled load (wgsl.js:3144-3173).
         const float4 g = R[19];
-        const bool aon = !PATCH || role == 1.0f || (role == 2.0f && R[PO + 1].w == 0.0f);
+        const bool aon = !ell && (!PATCH || role == 1.0f || (role == 2.0f && R[PO + 1].w == 0.0f));
         if (aon && (g.x > 0.0f || g.y > 0.0f)) {
           const float4 la = R[21];
           const f3 wrel = tb[1] - ta[1];

@@ -500,7 +500,9 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     for k, default in (("enable_bilateral_preelimination", False), ("propagation_same_articulation_rows", False), ("update_mass_matrix_interval", 1)):
         if SV.get(k) is not None and SV.get(k) != default:
             raise ValueError(f"{k}={SV[k]} is not supported")
-    for k in ("enable_torsional_rolling_friction", "contact_shared_anchor", "contact_friction_shared_anchor"):
+    if SV.get("torsional_rolling_friction_cone") == "pyramidal":
+        raise UnsupportedError("the pyramidal torsional/rolling cone is not supported yet")
+    for k in ("contact_shared_anchor", "contact_friction_shared_anchor"):
         if SV.get(k) or SO.get(k):
             raise UnsupportedError(f"{k} is not supported yet")
     if SV.get("contact_friction_position_iterations", -1) >= 0:
@@ -511,15 +513,17 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
         raise UnsupportedError("free-body torsional/rolling friction is not supported yet")
     compl = bool(SV.get("contact_compliance"))
     anchorBeta = SV.get("friction_anchor_beta", 0.2)
-    if anchorBeta is None or compl or fmode > 0:  # coupled friction modes and compliance turn anchors off (sim.js:421-422)
+    angArt = bool(SV.get("enable_torsional_rolling_friction"))
+    if anchorBeta is None or compl or fmode > 0 or angArt:  # these turn anchors off (sim.js:421-422)
         anchorBeta = 0.0
-    angOn = any((s.get("condim", 3) or 3) >= 4 for s in S["shapes"])  # sim.js:425-429 (articulated and free-rigid forms refused above)
+    angOn = angArt or any((s.get("condim", 3) or 3) >= 4 for s in S["shapes"])  # sim.js:425-429 (free-rigid overrides refused above)
     PO = 34 if angOn else 19
     FMO = PO + (4 if anchorBeta > 0 else 0)
     nest = bool(SV.get("nesterov_momentum"))
     tors = any((s.get("torsion_radius") or 0) > 0 for s in S["shapes"])
     TO = FMO + (2 if fmode > 0 else 0)
-    CO = TO + (5 if tors else 0)
+    EO = TO + (5 if tors else 0)
+    CO = EO + (4 if angArt else 0)
     MO = CO + (1 if compl else 0)
     ROWS = MO + (2 if nest else 0)
     P = dict(P_BODY=0, P_SHAPE=NB * 2, P_AABB=NB * 2 + NS * 2, P_PREV=NB * 2 + NS * 4)
@@ -623,6 +627,8 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_NLOOP", len(loops))
     H("W_ANG", 1 if angOn else 0)
     H("W_FMODE", fmode)
+    H("W_ANG_ART", 1 if angArt else 0)
+    H("W_EO", EO)
     H("W_FMO", FMO)
     if ctrl:  # the fader controller (sim.js:388-393)
         H("W_CTRL_KIND", 1)
@@ -652,6 +658,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     jlg = SV.get("joint_limit_activation_gap")
     for k, v in (("W_F_CFM", SV["cfm"]), ("W_F_OMEGA", SV["relaxation"]), ("W_F_SPEC", SV["contact_speculative_scale"]),
                  ("W_F_GAP_GATE", 1e30 if gate is None else gate), ("W_F_FGAP", 1e30 if fgap is None else fgap),
+                 ("W_F_CREEP", SV.get("torsional_rolling_friction_creep_speed") or 0),
                  ("W_F_FSCALE", SV["contact_friction_scale"]), ("W_F_ANCHOR_BETA", anchorBeta), ("W_F_BP_CELL", SV["body_pair_cell_size"]),
                  ("W_F_REST_SPEED", SV["restitution_velocity_threshold"]), ("W_F_JL_GAP", min(0.05 if jlg is None else jlg, 1e30)),
                  ("W_F_JL_SPEED", 2 if jlg is None else 0),

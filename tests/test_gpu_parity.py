@@ -236,6 +236,8 @@ SCENES = {
     "boxes_desaxce": lambda: _with_solver(workloads.boxes64(n=40), friction_mode="bisection_desaxce"),
     "quad_newton": lambda: _with_solver(workloads.quadruped_world(18), friction_mode="coulomb_newton"),
     "quad_bisect": lambda: _with_solver(workloads.quadruped_world(19), friction_mode="bisection"),
+    # articulated spin/rolling through the elliptic cone block (anchors off), with torsion-radius rows too
+    "quad_cone": lambda: _with_torsion(_with_solver(workloads.quadruped_world(20), enable_torsional_rolling_friction=True)),
     # drive rows with velocity limits (driven dofs fused into the sweep clamp) and effort limits
     "quad_drows": lambda: _with_solver(workloads.quadruped_world(17, drive=dict(targetKe=60.0, targetKd=2.0, velocityLimit=0.8, effortLimit=4.0)),
                                        drive_mode="rows", enable_joint_velocity_limits=True),
@@ -288,7 +290,8 @@ def test_broadphase_pair_set_and_contacts_bitexact(sim, name):
                                          ("boxes_vel", 150), ("quad_vel", 200), ("kinematic", 200),
                                          ("boxes_compl", 150), ("quad_compl", 200), ("quad_drows", 200),
                                          ("loops", 200), ("spin_roll", 200), ("fader", 200),
-                                         ("boxes_desaxce", 150), ("quad_newton", 200), ("quad_bisect", 200)])
+                                         ("boxes_desaxce", 150), ("quad_newton", 200), ("quad_bisect", 200),
+                                         ("quad_cone", 200)])
 def test_trajectory_bitexact(sim, name, frames):
     pw = pack.pack_scene(SCENES[name]())
     b = sim.Batch([pw])

@@ -102,7 +102,8 @@ def test_unsupported_options_raise():
     with pytest.raises(ValueError):
         pack.pack_scene(workloads.boxes64(n=4), opts={"substeps": 64})
     with pytest.raises(pack.UnsupportedError):
-        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"enable_torsional_rolling_friction": True}})
+        pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"enable_torsional_rolling_friction": True,
+                                                                  "torsional_rolling_friction_cone": "pyramidal"}})
 
 
 def test_convex_hull_faces_are_outward_ccw_planes():

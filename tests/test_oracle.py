@@ -1626,3 +1626,37 @@ def test_coulomb_newton_block_meets_the_coulomb_law_and_bisection_the_normal_tar
                     assert abs(up[0] - target) <= 1e-4 * (1 + abs(target) + out[0] * G[0, 0]), (trial, mode, up[0], target)
                     kinds["bis"] = kinds.get("bis", 0) + 1
     assert min(kinds.values()) > 10, kinds
+
+
+def test_articulated_cone_block_solves_the_ball_constrained_quadratic():
+    """The elliptic spin/rolling block (wgsl.js:2655-2925) with omega 1 and no creep: on random SPD 5x5 responses H,
+    residuals g, coefficients mu, start impulses lam and loads N, the new impulses lam' minimize
+    Q(lam') = 0.5 (lam' - lam)' H (lam' - lam) + g' (lam' - lam) over |lam' / mu| <= N: either interior with zero
+    gradient, or on the ellipsoid with the gradient g + H (lam' - lam) = -nu lam' / mu^2, nu >= 0 (KKT)."""
+    L = lib()
+    L.oracle_ell_block_f64.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p]
+    rng = np.random.default_rng(5)
+    seen = {"interior": 0, "boundary": 0}
+    for trial in range(300):
+        M = rng.normal(size=(5, 5))
+        H = M @ M.T + 0.05 * np.eye(5)
+        g = rng.normal(size=5) * rng.uniform(0.1, 3)
+        mu = rng.uniform(0.05, 1.0, size=5)
+        lam = rng.normal(size=5) * 0.1
+        N = rng.uniform(0.1, 2.0)
+        out = np.zeros(5)
+        L.oracle_ell_block_f64(H.ctypes.data, g.ctypes.data, mu.ctypes.data, lam.ctypes.data, N, 1.0, 0.0, out.ctypes.data)
+        y = out / mu
+        ny = np.linalg.norm(y)
+        assert ny <= N * (1 + 1e-4), (trial, ny, N)
+        grad = g + H @ (out - lam)
+        if ny < N * (1 - 1e-4):
+            assert np.linalg.norm(grad) <= 1e-6 * (1 + np.linalg.norm(g)), trial
+            seen["interior"] += 1
+        else:
+            nrm = out / mu ** 2
+            nu = -grad @ nrm / (nrm @ nrm)
+            assert nu >= -1e-8, trial
+            assert np.linalg.norm(grad + nu * nrm) <= 2e-3 * (np.linalg.norm(grad) + 1e-9), (trial, np.linalg.norm(grad + nu * nrm))
+            seen["boundary"] += 1
+    assert min(seen.values()) >= 5, seen
