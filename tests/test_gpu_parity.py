@@ -3,6 +3,9 @@
 Integer work (broadphase pair sets, contact records and their canonical order, per-row colors) must match bit
 for bit; the kernels keep the oracle's float operation order, so whole trajectories match bit for bit too,
 which is stronger than the 1e-5 relative one-step bound (also checked, against the FP64 oracle)."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
@@ -254,6 +257,26 @@ SCENES = {
     "quad_vlim": lambda: workloads.quadruped_world(4, drive=dict(targetKe=60.0, targetKd=2.0, velocityLimit=0.5),
                                                    solver=dict(enable_joint_velocity_limits=True)),
 }
+
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+import scene_worlds  # noqa: E402
+
+
+@pytest.mark.parametrize("name", scene_worlds.names())
+def test_reference_scene_folders_bitexact(sim, name):
+    """Each of the reference's 41 shipped scene folders (packed fixtures): 60 frames from their initial state,
+    GPU equal to the FP32 oracle bit for bit (poses, velocities, articulation state, stats words)."""
+    pw = scene_worlds.load(name)
+    b = sim.Batch([pw])
+    o = OracleWorld(pw)
+    b.step(60)
+    o.frames(60)
+    assert np.array_equal(_bits(b.read(0, "poses")), _bits(o.poses))
+    assert np.array_equal(_bits(b.read(0, "vel")), _bits(o.vel[:2 * max(pw.NB, 1) * 4]))
+    assert np.array_equal(_bits(b.read(0, "art")), _bits(o.art))
+    assert np.array_equal(b.stats_raw(0)[0][:9], o.stats[:9])
+    b.destroy()
 
 
 def _bits(a):

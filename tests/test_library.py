@@ -140,3 +140,22 @@ def test_box_mass_properties_closed_form():
     assert mh["m"] == pytest.approx(m)
     np.testing.assert_allclose(mh["I"], mp["I"], atol=1e-12)
     assert len(h["faces"]) == 6 and len(h["verts"]) == 8
+
+
+def test_scene_fixtures_match_a_fresh_pack():
+    """The packed reference scene fixtures (tests/golden/scenes) equal what the packer produces now from the
+    shipped scene folders (skipped where the reference tree is absent)."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import scene_worlds
+    from steel_webgpu_amd.scene import loadScene
+    root = "/root/reference/assets/scenes"
+    names = scene_worlds.names()
+    assert len(names) == 41
+    if not os.path.isdir(root):
+        pytest.skip("reference scene folders not present")
+    for n in names:
+        fx = scene_worlds.load(n)
+        pw = pack.pack_scene(loadScene(os.path.join(root, n)))
+        for k in scene_worlds.FIELDS:
+            assert np.array_equal(getattr(fx, k), getattr(pw, k)), (n, k)
