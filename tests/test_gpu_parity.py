@@ -265,13 +265,13 @@ import scene_worlds  # noqa: E402
 
 @pytest.mark.parametrize("name", scene_worlds.names())
 def test_reference_scene_folders_bitexact(sim, name):
-    """Each of the reference's 41 shipped scene folders (packed fixtures): 60 frames from their initial state,
+    """Each of the reference's 41 shipped scene folders (packed fixtures): 150 frames from their initial state,
     GPU equal to the FP32 oracle bit for bit (poses, velocities, articulation state, stats words)."""
     pw = scene_worlds.load(name)
     b = sim.Batch([pw])
     o = OracleWorld(pw)
-    b.step(60)
-    o.frames(60)
+    b.step(150)
+    o.frames(150)
     assert np.array_equal(_bits(b.read(0, "poses")), _bits(o.poses))
     assert np.array_equal(_bits(b.read(0, "vel")), _bits(o.vel[:2 * max(pw.NB, 1) * 4]))
     assert np.array_equal(_bits(b.read(0, "art")), _bits(o.art))
