@@ -4,10 +4,14 @@ The output is one `PackedWorld`: statF/statU with the reference's record layouts
 poses/vel/art state, and the header of layout constants and solver parameters declared in
 include/steel_world.h.  The step kernels and the CPU oracle read exactly these arrays.
 
-Supported here: the solver's default option path (friction anchors on, warm start off, no sleeping,
-fixed-normal friction, no torsional/rolling rows, no compliance) and FREE / REVOLUTE / PRISMATIC / BALL /
-FIXED joints in tree articulations.  Anything else raises, as the reference raises for options it does not
-take (sim.js:73, 128, 159, 166, 376, 379, 411-420).
+Supported: the default option path and the off-default features every shipped scene uses (warm start,
+sleeping, Nesterov momentum, velocity iterations, the four friction modes, contact compliance and torsion,
+spin/rolling rows with the elliptic articulated cone, drive rows, implicit drives and springs, joint and body
+velocity limits, joint friction, recorded target paths, the fader controller, kinematic bodies), FREE /
+REVOLUTE / PRISMATIC / BALL / FIXED / D6 joints in tree articulations, and ball or revolute loop closures and
+mimic joints.  Anything else (the pyramidal cone, free-rigid torsional/rolling overrides, cone shapes, shared
+anchors, friction position iterations, row-schedule variants, other controllers) raises UnsupportedError, as the
+reference raises for options it does not take (sim.js:73, 128, 159, 166, 376, 379, 411-420).
 """
 from __future__ import annotations
 
