@@ -131,6 +131,9 @@ int steel_server_step_lane(steel_server* s, uint32_t lane);
 int steel_server_lane_poll(steel_server* s, uint32_t lane, int* done, uint32_t* nworlds, float* gpu_ms);
 /* Sim.read of an admitted world (waits for its lane's in-flight frame). */
 int steel_server_read(steel_server* s, uint32_t id, uint32_t buf, size_t byte_off, size_t bytes, void* dst);
+/* Sim's queue.writeBuffer for an admitted world (setJointTargets, setBodyWrench, setBodyPose/Twist, the control
+   flags word): waits for its lane's in-flight frame, then writes, so the next lane frame sees the input. */
+int steel_server_write(steel_server* s, uint32_t id, uint32_t buf, size_t byte_off, size_t bytes, const void* src);
 /* Pool bytes in use, pool capacity, admitted worlds. */
 int steel_server_info(steel_server* s, size_t* used, size_t* capacity, uint32_t* nworlds);
 

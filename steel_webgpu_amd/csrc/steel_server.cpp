@@ -303,6 +303,19 @@ int steel_server_read(steel_server* s, uint32_t id, uint32_t buf, size_t byte_of
   return STEEL_OK;
 }
 
+int steel_server_write(steel_server* s, uint32_t id, uint32_t buf, size_t byte_off, size_t bytes, const void* src) {
+  if (!s || (!src && bytes)) return fail(STEEL_BAD_ARGUMENT, "null argument");
+  auto it = s->worlds.find(id);
+  int k = buf_kind(buf);
+  if (it == s->worlds.end() || k < 0) return fail(STEEL_BAD_ARGUMENT, "unknown world %u or buffer %u", id, buf);
+  const ServerWorld& W = it->second;
+  if (byte_off + bytes > W.ksize[k]) return fail(STEEL_BAD_ARGUMENT, "write of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, W.ksize[k]);
+  HIPCHK(hipSetDevice(s->dev->ordinal));
+  HIPCHK(hipStreamSynchronize(s->lanes[W.lane].stream));
+  HIPCHK(hipMemcpy(s->pool + W.off + W.koff[k] + byte_off, src, bytes, hipMemcpyHostToDevice));
+  return STEEL_OK;
+}
+
 int steel_server_info(steel_server* s, size_t* used, size_t* capacity, uint32_t* nworlds) {
   if (!s) return fail(STEEL_BAD_ARGUMENT, "null server");
   reclaim(s);

@@ -84,6 +84,12 @@ class Server:
         _sim._check(self.lib.steel_server_read(self.handle, wid, _sim.BUF[name], offset, out.nbytes, out.ctypes.data))
         return out
 
+    def write(self, wid, name, data, offset=0):
+        """Control input for an admitted world between lane frames (Sim's writeBuffer: joint targets, wrenches,
+        poses, twists, the control flags word)."""
+        data = np.ascontiguousarray(data)
+        _sim._check(self.lib.steel_server_write(self.handle, wid, _sim.BUF[name], offset, data.nbytes, data.ctypes.data))
+
     def info(self):
         used, cap, n = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_uint32()
         _sim._check(self.lib.steel_server_info(self.handle, ctypes.byref(used), ctypes.byref(cap), ctypes.byref(n)))

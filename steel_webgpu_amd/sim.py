@@ -28,7 +28,7 @@ EXPORTS = ["steel_last_error", "steel_device_open", "steel_device_close", "steel
            "steel_batch_read", "steel_batch_write", "steel_batch_buffer_size", "steel_batch_stats", "steel_batch_configure",
            "steel_batch_device_bytes", "steel_estimate_bytes", "steel_batch_step_timed", "steel_batch_profile_kernels",
            "steel_server_create", "steel_server_destroy", "steel_server_configure_lane", "steel_server_world_bytes",
-           "steel_server_admit", "steel_server_release", "steel_server_step_lane", "steel_server_lane_poll", "steel_server_read",
+           "steel_server_admit", "steel_server_release", "steel_server_step_lane", "steel_server_lane_poll", "steel_server_read", "steel_server_write",
            "steel_server_info"]
 
 
@@ -86,6 +86,7 @@ def load_library(path=LIB_PATH):
         "steel_server_step_lane": ([V, U32], ctypes.c_int),
         "steel_server_lane_poll": ([V, U32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "steel_server_read": ([V, U32, U32, SZ, SZ, V], ctypes.c_int),
+        "steel_server_write": ([V, U32, U32, SZ, SZ, V], ctypes.c_int),
         "steel_server_info": ([V, ctypes.POINTER(SZ), ctypes.POINTER(SZ), ctypes.POINTER(U32)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():

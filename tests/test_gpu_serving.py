@@ -67,3 +67,31 @@ def test_world_larger_than_pool_is_refused():
     m = serve.run(reqs, srv)
     srv.destroy()
     assert m["rejected"] == 1 and m["world_steps"] == 0
+
+
+def test_policy_writes_between_lane_frames_match_a_batch_world():
+    """Control in the serving loop: a policy writes new joint targets into an admitted world before each lane
+    frame (steel_server_write); the served rollout equals the same writes into a batch world between its frames,
+    bit for bit."""
+    from steel_webgpu_amd import sim
+    pw = pack.pack_scene(workloads.quadruped_world(3, drive=dict(targetKe=40.0, targetKd=1.0)))
+    off, nd = pw.A["A_TGT"] * 4, pw.ND
+    srv = serve.Server(budget_bytes=1 << 30, lanes=1)
+    wid = srv.admit(pw, 0)
+    b = sim.Batch([pw], graph=False)
+    rng = np.random.default_rng(4)
+    for f in range(30):
+        tgt = rng.uniform(-0.4, 0.4, nd).astype(np.float32)
+        srv.write(wid, "art", tgt, offset=off)
+        b.write(0, "art", tgt, offset=off)
+        srv.step(0)
+        while not srv.poll(0)[0]:
+            pass
+        b.step(1)
+    for name in ("poses", "art"):
+        got, ref = srv.read(wid, name), b.read(0, name)
+        assert np.array_equal(_bits(got), _bits(ref[:got.size])), name
+    assert not np.array_equal(srv.read(wid, "art")[pw.A["A_TGT"]:pw.A["A_TGT"] + nd], pw.art0[pw.A["A_TGT"]:pw.A["A_TGT"] + nd])
+    srv.release(wid)
+    srv.destroy()
+    b.destroy()
