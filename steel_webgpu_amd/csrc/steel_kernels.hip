@@ -953,9 +953,10 @@ struct Prep {
     float uu[6 * 64];
     float kc[3];
     {
-      float tau[64], y[64];
-      for (uint32_t k = 0; k < NDA; k++) { tau[k] = k < ai.w ? J[r * NDA + k] : 0.0f; y[k] = 0.0f; }
-      w.art_solve_gen(art_id, tau, y, ps, dvs, uu);
+      // Row r of J is the impulse (its entries past the articulation's dofs are zero and never read);
+      // the solve writes every dof of y that is read.
+      float y[64];
+      w.art_solve_gen(art_id, J + r * NDA, y, ps, dvs, uu);
       for (uint32_t k = 0; k < ai.w; k++) { art_[o + DHDR + r * NDA + k] = J[r * NDA + k]; art_[o + DHDR + 3 * NDA + r * NDA + k] = y[k]; }
       for (uint32_t s2 = 0; s2 < 3; s2++) { float acc = 0.0f; for (uint32_t k = 0; k < ai.w; k++) acc += J[s2 * NDA + k] * y[k]; kc[s2] = acc; }
     }

@@ -701,7 +701,7 @@ struct W {
       uint32_t k = ai.y - 1 - kk, l = ai.x + k, n = link_ndof(l);
       float u[6] = {0, 0, 0, 0, 0, 0};
       for (uint32_t r = 0; r < n; r++) u[r] = tau[link_dof(l) + r - ai.z] - sdot(ldS(l, r), p[k]);
-      for (uint32_t r = 0; r < 6; r++) uu[k * 6 + r] = u[r];
+      for (uint32_t r = 0; r < n; r++) uu[k * 6 + r] = u[r];
       int par = link_parent(l);
       if (par >= 0) {
         SVd pp = p[k];
