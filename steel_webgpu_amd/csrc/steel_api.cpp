@@ -321,7 +321,7 @@ int steel_batch_read(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_o
   if (!b || (!dst && bytes)) return fail(STEEL_BAD_ARGUMENT, "null argument");
   int k = buf_kind(buf);
   if (k < 0 || world >= b->n) return fail(STEEL_BAD_ARGUMENT, "bad buffer %u or world %u", buf, world);
-  if (byte_off + bytes > b->size[k][world]) return fail(STEEL_BAD_ARGUMENT, "read of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, b->size[k][world]);
+  if (byte_off > b->size[k][world] || bytes > b->size[k][world] - byte_off) return fail(STEEL_BAD_ARGUMENT, "read of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, b->size[k][world]);
   HIPCHK(hipSetDevice(b->dev->ordinal));
   HIPCHK(hipStreamSynchronize(b->stream));
   HIPCHK(hipMemcpy(dst, b->arena[k] + b->off[k][world] + byte_off, bytes, hipMemcpyDeviceToHost));
@@ -332,7 +332,7 @@ int steel_batch_write(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_
   if (!b || (!src && bytes)) return fail(STEEL_BAD_ARGUMENT, "null argument");
   int k = buf_kind(buf);
   if (k < 0 || world >= b->n) return fail(STEEL_BAD_ARGUMENT, "bad buffer %u or world %u", buf, world);
-  if (byte_off + bytes > b->size[k][world]) return fail(STEEL_BAD_ARGUMENT, "write of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, b->size[k][world]);
+  if (byte_off > b->size[k][world] || bytes > b->size[k][world] - byte_off) return fail(STEEL_BAD_ARGUMENT, "write of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, b->size[k][world]);
   HIPCHK(hipSetDevice(b->dev->ordinal));
   HIPCHK(hipMemcpyAsync(b->arena[k] + b->off[k][world] + byte_off, src, bytes, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));

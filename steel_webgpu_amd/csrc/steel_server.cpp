@@ -67,7 +67,7 @@ struct Lane {
   WorldDev* h_table = nullptr;        // pinned staging of the table
   uint32_t* pair_offs = nullptr;      // k_narrow's per-world pair offsets (cap + 1 words)
   uint32_t cap = 0;
-  uint32_t substeps = 0;              // set by the lane's first world
+  uint32_t substeps = 0;              // substeps per frame of the lane's worlds; set while the lane is empty
   uint32_t block = 64;
   LaunchLds sl;                       // dynamic LDS over the lane's worlds
   std::vector<uint32_t> pending_free; // released while a frame may still use them
@@ -180,6 +180,7 @@ int steel_server_admit(steel_server* s, const steel_world_desc* w, uint32_t lane
   if (!w->header || !w->statF || !w->statU || !w->poses0 || !w->vel0 || !w->art0) return fail(STEEL_BAD_ARGUMENT, "null world array");
   Lane& L = s->lanes[lane];
   uint32_t S = w->header[W_SUBSTEPS];
+  if (L.members.empty()) L.substeps = 0;  // an empty lane takes the substep count of the next world it admits
   if (L.substeps && L.substeps != S) return fail(STEEL_BAD_ARGUMENT, "lane %u steps %u substeps per frame, the world %u", lane, L.substeps, S);
   Sizes sz;
   int rc = world_sizes(w->header, sz);
@@ -296,7 +297,7 @@ int steel_server_read(steel_server* s, uint32_t id, uint32_t buf, size_t byte_of
   int k = buf_kind(buf);
   if (it == s->worlds.end() || k < 0) return fail(STEEL_BAD_ARGUMENT, "unknown world %u or buffer %u", id, buf);
   const ServerWorld& W = it->second;
-  if (byte_off + bytes > W.ksize[k]) return fail(STEEL_BAD_ARGUMENT, "read of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, W.ksize[k]);
+  if (byte_off > W.ksize[k] || bytes > W.ksize[k] - byte_off) return fail(STEEL_BAD_ARGUMENT, "read of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, W.ksize[k]);
   HIPCHK(hipSetDevice(s->dev->ordinal));
   HIPCHK(hipStreamSynchronize(s->lanes[W.lane].stream));
   HIPCHK(hipMemcpy(dst, s->pool + W.off + W.koff[k] + byte_off, bytes, hipMemcpyDeviceToHost));
@@ -309,7 +310,7 @@ int steel_server_write(steel_server* s, uint32_t id, uint32_t buf, size_t byte_o
   int k = buf_kind(buf);
   if (it == s->worlds.end() || k < 0) return fail(STEEL_BAD_ARGUMENT, "unknown world %u or buffer %u", id, buf);
   const ServerWorld& W = it->second;
-  if (byte_off + bytes > W.ksize[k]) return fail(STEEL_BAD_ARGUMENT, "write of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, W.ksize[k]);
+  if (byte_off > W.ksize[k] || bytes > W.ksize[k] - byte_off) return fail(STEEL_BAD_ARGUMENT, "write of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, W.ksize[k]);
   HIPCHK(hipSetDevice(s->dev->ordinal));
   HIPCHK(hipStreamSynchronize(s->lanes[W.lane].stream));
   HIPCHK(hipMemcpy(s->pool + W.off + W.koff[k] + byte_off, src, bytes, hipMemcpyHostToDevice));

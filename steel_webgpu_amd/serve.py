@@ -27,6 +27,10 @@ LANE_EDGES = (100, 600)
 LANE_BLOCKS = (64, 128, 256)
 
 
+class AdmitRefused(RuntimeError):
+    """The library refused to admit a world (steel_server_admit returned a status other than out-of-memory)."""
+
+
 def lane_of(pw, edges=LANE_EDGES):
     return sum(pw.NB >= e for e in edges)
 
@@ -54,12 +58,16 @@ class Server:
         return out.value
 
     def admit(self, pw, lane):
-        """World id, or None when the pool cannot hold the world now (nothing was allocated)."""
+        """World id, or None when the pool cannot hold the world now (nothing was allocated).  Raises
+        AdmitRefused when the library refuses the world itself (a world it cannot step, or a substep count
+        other than that of the lane's current worlds)."""
         arr, keep = _sim._descs([pw])
         wid = ctypes.c_uint32()
         rc = self.lib.steel_server_admit(self.handle, arr, lane, ctypes.byref(wid))
         if rc == _sim.STEEL_OUT_OF_MEMORY:
             return None
+        if rc in (_sim.STEEL_UNSUPPORTED, _sim.STEEL_BAD_ARGUMENT):
+            raise AdmitRefused(_sim.last_error())
         _sim._check(rc)
         self._sizes[wid.value] = pw
         return wid.value
@@ -167,7 +175,17 @@ def run(requests, server, readback=("poses",), clock=time.perf_counter, log=None
             queue.append(pending.popleft())
         while queue:
             r = queue[0]
-            wid = server.admit(r.world, r.lane)
+            if members[r.lane] and members[r.lane][0].world.h("W_SUBSTEPS") != r.world.h("W_SUBSTEPS"):
+                deferrals += 1             # its lane steps another substep count: wait until the lane drains
+                break
+            try:
+                wid = server.admit(r.world, r.lane)
+            except AdmitRefused:
+                rejected += 1              # a world the library cannot step
+                queue.popleft()
+                r.finished = now
+                finished += 1
+                continue
             if wid is None:
                 if not any(members):
                     rejected += 1          # larger than the whole pool: it can never be admitted

@@ -97,6 +97,11 @@ def load_library(path=LIB_PATH):
     return lib
 
 
+def last_error():
+    """The library's message for the calling thread's last failed call."""
+    return _lib.steel_last_error().decode()
+
+
 def _check(rc):
     if rc != STEEL_OK:
         msg = _lib.steel_last_error().decode()
