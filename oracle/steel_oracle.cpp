@@ -3687,6 +3687,32 @@ template <class T> struct World {
     counters[0] = 0; counters[1] = 0;
   }
 
+  // Reduction inputs and survivors of the last prepare, per sorted contact (the OracleOut red/redf export).
+  // The survivors are the contacts S_KEPT lists; the listing is S_CORDER over the units before compaction.
+  void export_reduction(uint32_t* red, float* redf) const {
+    uint32_t nc = std::min(counters[1], MAXC);
+    for (uint32_t c = 0; c < nc; c++) {
+      for (int k = 0; k < 8; k++) red[c * 8 + k] = 0;
+      red[c * 8] = 0xffffffffu; red[c * 8 + 7] = c;
+      for (int k = 0; k < 4; k++) redf[c * 4 + k] = 0.f;
+    }
+    uint32_t nu = S_META[0];
+    uint32_t nl = 0;
+    for (uint32_t u = 0; u < nu; u++) nl += S_USIZE_TMP[u];
+    nl = std::min(nl, MAXC);
+    std::vector<uint32_t> unit_start(nu + 1, 0);
+    for (uint32_t u = 0; u < nu; u++) unit_start[u + 1] = unit_start[u] + S_USIZE_TMP[u];
+    for (uint32_t p = 0, u = 0; p < nl; p++) {
+      while (u + 1 < nu && unit_start[u + 1] <= p) u++;
+      uint32_t c = S_CORDER[p];
+      BPC g = bp_contact(c);
+      red[c * 8] = u; red[c * 8 + 1] = p; red[c * 8 + 2] = g.k[0]; red[c * 8 + 3] = g.k[1]; red[c * 8 + 4] = g.k[2];
+      red[c * 8 + 5] = g.pk;
+      redf[c * 4] = (float)g.pos.x; redf[c * 4 + 1] = (float)g.pos.y; redf[c * 4 + 2] = (float)g.gap;
+    }
+    for (uint32_t u = 0; u < nu; u++)  // unit u's survivors sit at its listing start (S_OSTART) in S_KEPT
+      for (uint32_t j = 0; j < S_USIZE[u] && S_OSTART[u] + j < MAXC; j++) red[S_KEPT[S_OSTART[u] + j] * 8 + 6] = 1;
+  }
   void alloc_scratch() {
     pairs.reserve(2 * MAXP);
     craw.reserve(MAXC);
@@ -3724,6 +3750,11 @@ struct OracleOut {
   uint32_t* nrows;
   uint32_t* meta;       // [8]: nu, ncol, nc, ndense, ...
   float* rows;          // [MAXC * ROWS * 4]: contact rows after prepare (initial impulses included)
+  // Body-pair reduction inputs and result per sorted contact c (wgsl.js:1459-1503, 2029-2105), or null:
+  // red[c*8..]: unit (0xffffffff when the contact has no unit), listing position p, group keys k0, k1, k2,
+  // position identity pk, kept (1/0), and the contact's index again; redf[c*4..]: pos.x, pos.y, gap, 0.
+  uint32_t* red;        // [MAXC * 8]
+  float* redf;          // [MAXC * 4]
 };
 
 template <class T>
@@ -3782,6 +3813,7 @@ static int run_substeps(const uint32_t* header, const float* statF, const uint32
       }
       for (int i = 0; i < 8; i++) dbg->meta[i] = w.S_META[i];
     }
+    if (dbg && dbg->red) w.export_reduction(dbg->red, dbg->redf);
     if (dbg && dbg->rows)
       for (size_t i = 0; i < (size_t)w.MAXC * w.ROWS; i++) {
         dbg->rows[i * 4] = (float)w.rows[i].x; dbg->rows[i * 4 + 1] = (float)w.rows[i].y;
@@ -3866,6 +3898,65 @@ void oracle_gsup_f64(const uint32_t* header, const float* statF, const uint32_t*
   World<double> w = geometry_world(header, statF, statU);
   V3<double> p = w.gsup(w.geo(s), V3<double>{d[0], d[1], d[2]});
   out[0] = p.x; out[1] = p.y; out[2] = p.z;
+}
+// Featherstone probes on a world's current state, for the pins against an independent composite-rigid-body
+// mass matrix.  Runs FK and the articulated-body factor of every articulation as prepare does
+// (artwgsl.js:98-369), then:
+//   mode 0: link a0's unit-impulse response at its COM (artwgsl.js:431-510): out[0..35] = W2 (unsplit),
+//           out[36..71] = W with the path's split counts in[0..path length) (in = null: all 1);
+//   mode 1: dof_hinv(link a0, its dof a1) (artwgsl.js:513-518): out[0];
+//   mode 2: art_solve_gen(articulation a0, tau = in[0..NDA)) (artwgsl.js:557-596): out[0..NDA);
+//   mode 3: add_gen_jac(link a0, point in[0..2], directions in[3..5], in[6..8], in[9..11]) (artwgsl.js:525-540),
+//           relative to the articulation's first dof: out[0..3*NDA), zeroed first;
+//   mode 4: every link's frame after FK: out[l*10..] = position (3), rotation quaternion xyzw (4), COM (3).
+// poses, vel and art are doubles and are not written back except through art (FK and factor outputs).
+int oracle_art_probe_f64(const uint32_t* header, const float* statF, const uint32_t* statU, double* poses, double* vel, double* art,
+                         uint32_t mode, uint32_t a0, uint32_t a1, const double* in, double* out) {
+  World<double> w;
+  w.load_header(header);
+  w.statF.resize(header[W_NSTATF]);
+  for (uint32_t i = 0; i < header[W_NSTATF]; i++)
+    w.statF[i] = {(double)statF[i * 4], (double)statF[i * 4 + 1], (double)statF[i * 4 + 2], (double)statF[i * 4 + 3]};
+  w.statU = statU;
+  w.poses = (V4<double>*)poses;
+  w.vel = (V4<double>*)vel;
+  w.art = art;
+  w.alloc_scratch();
+  for (uint32_t a = 0; a < w.NA; a++) { w.art_fk(a); w.art_factor(a); }
+  if (mode == 0) {
+    if (a0 >= w.NL) return -1;
+    std::vector<uint32_t> path(w.MAXPATH), cnt(w.MAXPATH, 1u);
+    uint32_t np = w.link_path(a0, path.data());
+    if (in) for (uint32_t i = 0; i < np; i++) cnt[i] = (uint32_t)in[i];
+    w.link_response(a0, cnt.data());
+    for (uint32_t k = 0; k < 36; k++) { out[k] = art[w.lo(a0) + L_W2 + k]; out[36 + k] = art[w.lo(a0) + L_W + k]; }
+  } else if (mode == 1) {
+    if (a0 >= w.NL || a1 >= w.link_ndof(a0)) return -1;
+    out[0] = w.dof_hinv(a0, a1);
+  } else if (mode == 2) {
+    if (a0 >= w.NA) return -1;
+    std::vector<double> y(w.NDA, 0.0);
+    w.art_solve_gen(a0, in, y.data());
+    for (uint32_t k = 0; k < w.NDA; k++) out[k] = y[k];
+  } else if (mode == 3) {
+    if (a0 >= w.NL) return -1;
+    int b = w.link_body(a0);
+    uint32_t ai[4];
+    w.art_info((uint32_t)w.body_art(b), ai);
+    for (uint32_t k = 0; k < 3 * w.NDA; k++) out[k] = 0;
+    w.add_gen_jac(a0, V3<double>{in[0], in[1], in[2]}, 1.0, V3<double>{in[3], in[4], in[5]}, V3<double>{in[6], in[7], in[8]},
+                  V3<double>{in[9], in[10], in[11]}, ai[2], out);
+  } else if (mode == 4) {
+    for (uint32_t l = 0; l < w.NL; l++) {
+      V3<double> p = w.ld3(w.lo(l) + L_POS), c = w.ld3(w.lo(l) + L_COM);
+      V4<double> q = w.link_rot(l);
+      double* o = out + l * 10;
+      o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = q.x; o[4] = q.y; o[5] = q.z; o[6] = q.w; o[7] = c.x; o[8] = c.y; o[9] = c.z;
+    }
+  } else {
+    return -1;
+  }
+  return 0;
 }
 uint32_t oracle_hash_u32(uint32_t x) { return hash_u32(x); }
 float oracle_sinf(float x) { return sinf_c(x); }

@@ -9,7 +9,7 @@ ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
 
 
 class Dbg(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in ("pairs", "npairs", "contacts", "ncontacts", "rowinfo", "nrows", "meta", "rows")]
+    _fields_ = [(n, ctypes.c_void_p) for n in ("pairs", "npairs", "contacts", "ncontacts", "rowinfo", "nrows", "meta", "rows", "red", "redf")]
 
 
 _lib = None
@@ -67,7 +67,8 @@ class OracleWorld:
         MAXC, MAXP = pw.h("W_MAXC"), pw.h("W_MAXP")
         self.dbg_bufs = dict(pairs=np.zeros(2 * MAXP, np.uint32), npairs=np.zeros(1, np.uint32), contacts=np.zeros(MAXC * 24, np.float32),
                              ncontacts=np.zeros(1, np.uint32), rowinfo=np.zeros(MAXC * 4, np.uint32), nrows=np.zeros(1, np.uint32),
-                             meta=np.zeros(8, np.uint32), rows=np.zeros(MAXC * pw.h("W_ROWS") * 4, np.float32))
+                             meta=np.zeros(8, np.uint32), rows=np.zeros(MAXC * pw.h("W_ROWS") * 4, np.float32),
+                             red=np.zeros(MAXC * 8, np.uint32), redf=np.zeros(MAXC * 4, np.float32))
         self.dbg = Dbg(**{k: P(v).value for k, v in self.dbg_bufs.items()})
 
     def substeps(self, n, debug=False):
@@ -110,6 +111,24 @@ def warm_table_map(buf, WH, parity):
     keys = buf[parity * WH:(parity + 1) * WH]
     vals = buf[2 * WH:].reshape(2 * WH, 4)[parity * WH:(parity + 1) * WH].view(np.float32)
     return {int(k): vals[i].copy() for i, k in enumerate(keys) if k}
+
+
+def art_probe(pw, mode, a0=0, a1=0, inp=None, nout=72, art=None, poses=None, vel=None):
+    """The FP64 oracle's Featherstone functions on a world's state (initial state by default): FK and factor of
+    every articulation, then mode 0 link response (W2, W), 1 dof_hinv, 2 art_solve_gen, 3 add_gen_jac, 4 link
+    frames (see oracle_art_probe_f64)."""
+    art = (pw.art0 if art is None else art).astype(np.float64).copy()
+    poses = (pw.poses0 if poses is None else poses).astype(np.float64).copy()
+    vel = (pw.vel0 if vel is None or not np.size(vel) else vel).astype(np.float64).copy()
+    if not vel.size:
+        vel = np.zeros(8)
+    statU = pw.statU if pw.statU.size else np.zeros(1, np.uint32)
+    out = np.zeros(nout)
+    inp_arr = None if inp is None else np.ascontiguousarray(inp, np.float64)
+    rc = lib().oracle_art_probe_f64(P(pw.header), P(pw.statF), P(statU), P(poses), P(vel), P(art), ctypes.c_uint32(mode),
+                                    ctypes.c_uint32(a0), ctypes.c_uint32(a1), None if inp_arr is None else P(inp_arr), P(out))
+    assert rc == 0
+    return out
 
 
 def mpr(pw, ha, hb, rq, t, r):
