@@ -137,6 +137,33 @@ int steel_server_write(steel_server* s, uint32_t id, uint32_t buf, size_t byte_o
 /* Pool bytes in use, pool capacity, admitted worlds. */
 int steel_server_info(steel_server* s, size_t* used, size_t* capacity, uint32_t* nworlds);
 
+/* ---- Step without allocation.  No counterpart in the reference: this is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * caller holds one world's static arrays and state in device memory (device pointers; each array has the size
+ * steel_batch_buffer_size reports for that world, header/statF/statU the sizes of steel_world_desc).  The step
+ * scratch is a fixed array of STEEL_NOALLOC_SCRATCH_BYTES per GPU inside the library, so a world whose scratch
+ * exceeds it is refused with STEEL_BAD_ARGUMENT.  Calls on one GPU share that array: calls that overlap on one
+ * GPU (other threads or streams) race.  The steps run on `stream` (a hipStream_t; null: the default stream),
+ * asynchronously, with the kernels and results of steel_batch_step on a batch of this one world. */
+#define STEEL_NOALLOC_SCRATCH_BYTES (16u << 20)
+typedef struct steel_world_device_arrays {
+  const uint32_t* header;  /* device copies of the static arrays of steel_world_desc */
+  const float* statF;
+  const uint32_t* statU;
+  float* poses;            /* state, as the STEEL_BUF_* buffers of the same name */
+  float* vel;
+  float* art;
+  uint32_t* anchor_keys;
+  uint32_t* anchor_vals;
+  uint32_t* warm;
+  uint32_t* counters;      /* zero at reset, like stats, anchors and warm */
+  uint32_t* stats;
+} steel_world_device_arrays;
+/* `host` gives the world's header and statF in host memory (read for sizing only; the other pointers of
+ * `host` are not read). */
+int steel_world_step_noalloc(steel_device* dev, const steel_world_desc* host, const steel_world_device_arrays* arrays,
+                             uint32_t frames, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,8 @@ int steel_device_open(int ordinal, steel_device** out) {
     delete d;
     return fail(STEEL_UNSUPPORTED, "device %d is %s; this library is built for gfx950", ordinal, arch.c_str());
   }
+  int rc = steel_noalloc_init(d);
+  if (rc) { delete d; return rc; }
   *out = d;
   return STEEL_OK;
 }

@@ -116,4 +116,8 @@ static constexpr int K_ZEROED[] = {K_AKEY, K_AVAL, K_CTR, K_STATS, K_TW, K_IMP, 
 struct steel_device {
   int ordinal;
   hipDeviceProp_t prop;
+  unsigned char* noalloc_scratch = nullptr;  // this device's address of steel_noalloc.hip's scratch array
 };
+
+// Takes the device address of the no-allocation entry point's scratch array on the current device.
+int steel_noalloc_init(steel_device* dev);

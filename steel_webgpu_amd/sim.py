@@ -29,7 +29,7 @@ EXPORTS = ["steel_last_error", "steel_device_open", "steel_device_close", "steel
            "steel_batch_device_bytes", "steel_estimate_bytes", "steel_batch_step_timed", "steel_batch_profile_kernels",
            "steel_server_create", "steel_server_destroy", "steel_server_configure_lane", "steel_server_world_bytes",
            "steel_server_admit", "steel_server_release", "steel_server_step_lane", "steel_server_lane_poll", "steel_server_read", "steel_server_write",
-           "steel_server_info"]
+           "steel_server_info", "steel_world_step_noalloc"]
 
 
 class UnsupportedError(RuntimeError):
@@ -38,6 +38,12 @@ class UnsupportedError(RuntimeError):
 
 class SteelError(RuntimeError):
     pass
+
+
+class WorldDeviceArrays(ctypes.Structure):
+    """steel_world_device_arrays: device pointers of one world's static arrays and state."""
+    _fields_ = [(n, ctypes.c_void_p) for n in ("header", "statF", "statU", "poses", "vel", "art", "anchor_keys", "anchor_vals",
+                                                "warm", "counters", "stats")]
 
 
 class WorldDesc(ctypes.Structure):
@@ -88,6 +94,7 @@ def load_library(path=LIB_PATH):
         "steel_server_read": ([V, U32, U32, SZ, SZ, V], ctypes.c_int),
         "steel_server_write": ([V, U32, U32, SZ, SZ, V], ctypes.c_int),
         "steel_server_info": ([V, ctypes.POINTER(SZ), ctypes.POINTER(SZ), ctypes.POINTER(U32)], ctypes.c_int),
+        "steel_world_step_noalloc": ([V, ctypes.POINTER(WorldDesc), ctypes.POINTER(WorldDeviceArrays), U32, V], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
