@@ -3,17 +3,23 @@
 
 Workload (the batching configuration): 4096 worlds of uneven size, each a floating-base 12-joint quadruped plus
 K ~ Geometric(0.1) <= 64 boxes on a ground plane, FP32, dt = 1/200 s, 4 substeps, 12 iterations, all worlds in
-one launch per kernel with the frame's substep loop captured in a hipGraph; 1000 frames from rest.
+one launch per kernel with the frame's substep loop captured in a hipGraph; the configuration runs 1000 frames
+from rest.
 
-value       batched world-steps/s: worlds x frames / wall time of frames 0..K-1 after a reset, state resident
-            in HBM (warmup frames run before the reset)
+window      the timed frames: all 1000 frames when --steps >= 1000, else the last --steps frames of the 1000
+            (frames 1000-K..999, after an untimed advance from rest), where the batch has settled
+value       batched world-steps/s: worlds x K / wall time of the window, state resident in HBM
 plain       world-steps/s of the plain per-world step (one world per launch, per-substep launches, as the
-            reference dispatches), frames 0..K-1 of a few of the same worlds
+            reference dispatches) over the same window of a few of the same worlds
 latency     p50/p99 of the batched frame time (HIP events per frame)
-roofline    dominant kernel: algorithmic bytes per launch / average launch time, both over a replay of the
-            same K frames (each kernel bracketed by HIP events on the library's stream), against 8 TB/s
-cpu_baseline  the FP32 CPU oracle (port of the reference step) over the same frames of a sample of the
-            worlds, one world at a time per process on the host cores
+roofline    dominant kernel: algorithmic bytes per launch / average launch time, both over a replay of the same
+            window (each kernel bracketed by HIP events on the library's stream), against 8 TB/s; `traffic` is the
+            counter-measured HBM bytes per launch of a committed profile of the same window, or null
+cpu_baseline  the FP32 CPU oracle (a port of the reference step) over the same window of a sample of the worlds,
+            std::threads over the host cores this process may use (at most 16, the GPU box's share)
+parity      the sample worlds' state after the window, GPU against the CPU oracle, bit for bit
+fp64_error  one-step error of FP32 against the FP64 oracle from the settled state of 16 sample worlds, per
+            quantity (largest difference over the largest FP64 magnitude)
 """
 import argparse
 import json
@@ -30,6 +36,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "world-steps/s batched vs plain per-world (FP32) + p50/p99 step latency"
 HBM_PEAK_GBS = 8000.0
+CONFIG_FRAMES = 1000
+# A world's state: everything a step reads that is not static (the step's other tables are rebuilt every substep).
+STATE = ("poses", "vel", "art", "anchor_keys", "anchor_vals", "warm", "counters", "stats")
 
 
 def _pack_world(i):
@@ -120,9 +129,10 @@ def hull_bytes(pw):
     return out
 
 
-def algorithmic_bytes(batch, worlds):
-    """Per-launch compulsory bytes of each kernel, summed over worlds and averaged over the substeps since the
-    last reset (stats words 12-15: sums of pairs, kept contacts, rows; substep count).
+def algorithmic_bytes(batch, worlds, base=None):
+    """Per-launch compulsory bytes of each kernel, summed over worlds and averaged over the substeps of the
+    window: stats words 12-15 (running sums of pairs, kept contacts, rows; substep count) now minus `base`
+    (the same words at the window's start; None: since the reset).
 
     collide  288 NS + 296 NP + Hp + 304 NC      (shape pose/AABB, pair record + two shape records, hull data of
                                                  hull pairs, contact write + sort read/key/write)
@@ -134,6 +144,9 @@ def algorithmic_bytes(batch, worlds):
     tot = np.zeros(3)
     for i, pw in enumerate(worlds):
         st, _ = batch.stats_raw(i)
+        st = st.astype(np.int64)
+        if base is not None:
+            st[12:16] -= base[i][12:16]
         nsub = max(int(st[15]), 1)
         NP, NC, NR = st[12] / nsub, st[13] / nsub, st[14] / nsub
         NS, NB, NL, ND = pw.NS, pw.NB, pw.NL, pw.ND
@@ -159,63 +172,119 @@ def algorithmic_bytes(batch, worlds):
     return tot
 
 
-def _cpu_worker(args):
-    idx, frames = args
+def cpu_threads():
+    """Host threads for the CPU legs: the cores this process may run on, at most 16 (the GPU box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(worlds, f0, k, threads, log):
+    """FP32 CPU oracle over the window: the sample worlds advance frames 0..f0-1 untimed, then step frames
+    f0..f0+k-1 timed, on `threads` std::threads in one process.  Returns the rate, the oracle worlds (their
+    state at frame f0+k) and a description."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_bind import OracleWorld
-    n = 0
-    t = 0.0
-    for i in idx:
-        pw = _pack_world(i)
-        o = OracleWorld(pw)
-        t0 = time.perf_counter()
-        o.frames(frames)
-        t += time.perf_counter() - t0
-        n += frames
-    return n, t
-
-
-def cpu_baseline(nworlds, frames, procs):
-    """FP32 CPU oracle (the port of the reference step) on `procs` processes; returns the aggregate rate
-    (all world-steps / the busiest process's stepping time), the step count and the wall time."""
-    chunks = [list(range(k, nworlds, procs)) for k in range(procs)]
-    ctx = mp.get_context("fork")
+    from oracle_bind import OracleWorld, run_batch
+    ows = [OracleWorld(pw) for pw in worlds]
     t0 = time.perf_counter()
-    with ctx.Pool(procs) as pool:
-        res = pool.map(_cpu_worker, [(c, frames) for c in chunks])
-    wall = time.perf_counter() - t0
-    steps = sum(r[0] for r in res)
-    busy = max(r[1] for r in res)
-    return steps / busy, steps, wall
+    wall, busy = run_batch(ows, f0, k, threads)
+    total = time.perf_counter() - t0
+    rate = len(ows) * k / wall
+    log(f"cpu baseline: {rate:.1f} world-steps/s on {threads} threads ({len(ows)} worlds x frames {f0}..{f0 + k - 1}, "
+        f"{wall:.2f}s timed, {sum(busy):.1f} CPU-s; {total:.1f}s with the advance)")
+    return rate, ows, wall, sum(busy)
 
 
-def traffic_from_profile(kernel):
-    """HBM bytes per launch of k_<kernel> on this workload, from the committed counter summary (its
-    `bytes_per_launch:` line: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel factor)."""
+def fp64_reference(ows, log):
+    """One-step error of FP32 against the FP64 oracle from each world's current state, per quantity: the largest
+    absolute difference over the worlds, and the norm-wise relative error (largest difference over the largest
+    FP64 magnitude of the world) as its median and maximum over the worlds; plus the single-thread FP32 and FP64
+    step rates."""
+    from oracle_bind import one_step_errors
+    per, t32, t64 = {}, 0.0, 0.0
+    for o in ows:
+        e, walls = one_step_errors(o)
+        t32 += walls[0]
+        t64 += walls[1]
+        for q, v in e.items():
+            per.setdefault(q, []).append((v["abs"], v["rel"]))
+    err = {}
+    for q, vals in per.items():
+        a = np.asarray(vals)
+        err[q] = {"max_abs": float(f"{a[:, 0].max():.3g}"), "median_rel": float(f"{np.median(a[:, 1]):.3g}"),
+                  "max_rel": float(f"{a[:, 1].max():.3g}")}
+    log(f"fp64 one-step error over {len(ows)} worlds (median / max relative): " +
+        ", ".join(f"{q} {v['median_rel']:.1e}/{v['max_rel']:.1e}" for q, v in err.items()))
+    return err, len(ows) / t32, len(ows) / t64
+
+
+def traffic_from_profile(kernel, window):
+    """HBM bytes per launch of k_<kernel> from the committed counter summary of this workload (its
+    `bytes_per_launch:` line, FETCH_SIZE + WRITE_SIZE per dispatch scaled by the copy-kernel factor), when its
+    `window:` line names the same frames as this run; None otherwise."""
     path = os.path.join(ROOT, "profiles", f"k_{kernel}_quadruped4096_traffic.md")
     if not os.path.exists(path):
-        return None
+        return None, None
+    got, win = None, None
     with open(path) as f:
         for line in f:
             if line.startswith("bytes_per_launch:"):
-                return int(line.split(":")[1])
-    return None
+                got = int(line.split(":")[1])
+            if line.startswith("window:"):
+                win = line.split(":", 1)[1].strip()
+    return (got, win) if win == window else (None, win)
+
+
+def save_states(batch, idx):
+    return [{n: batch.read(i, n, dtype=np.uint8) for n in STATE} for i in idx]
+
+
+def restore_states(batch, idx, states):
+    for i, st in zip(idx, states):
+        for n in STATE:
+            batch.write(i, n, st[n])
+
+
+def write_state_file(path, states):
+    arrs = {}
+    for n in STATE:
+        arrs[n] = np.concatenate([st[n] for st in states])
+        arrs[n + "_len"] = np.asarray([st[n].size for st in states], np.int64)
+    np.savez(path, **arrs)
+
+
+def read_state_file(path):
+    z = np.load(path, allow_pickle=False)
+    out = None
+    for n in STATE:
+        lens = z[n + "_len"]
+        parts = np.split(z[n], np.cumsum(lens)[:-1])
+        if out is None:
+            out = [{} for _ in parts]
+        for st, p in zip(out, parts):
+            st[n] = p
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000, help="timed frames, from rest (the configuration's 1000 steps)")
-    ap.add_argument("--warmup", type=int, default=20, help="untimed frames before the reset that starts the timed run")
+    ap.add_argument("--steps", type=int, default=CONFIG_FRAMES,
+                    help="timed frames: the configuration's 1000, or the last K of them after an untimed advance")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed frames before the reset that starts the run")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--plain-worlds", type=int, default=3)
-    ap.add_argument("--cpu-worlds", type=int, default=48)
+    ap.add_argument("--cpu-worlds", type=int, default=0, help="CPU sample worlds (0: enough for about 4000 timed world-steps, 48..256)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="prepare/solve workgroup size (0: autotune)")
     ap.add_argument("--worlds-cache", default=None, help="pickle file to load packed worlds from / save them to")
     ap.add_argument("--pack-only", action="store_true", help="pack the worlds into --worlds-cache and exit")
     ap.add_argument("--profile", action="store_true",
-                    help="profiler run: batched leg only (no autotune, plain or CPU legs), then a 1 GiB copy for counter scaling")
+                    help="profiler run: batched leg over the window only (no autotune, plain or CPU legs), then a 1 GiB copy")
+    ap.add_argument("--state-out", default=None, help="write every world's state at the window's start to this .npz and exit")
+    ap.add_argument("--state-in", default=None, help="start the window from this .npz (written by --state-out) instead of advancing")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--workload", default="batch", choices=["batch", "serving"])
     ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
@@ -227,12 +296,13 @@ def main():
     if args.profile:
         args.no_cpu = True
         args.plain_worlds = 0
+        args.warmup = 0
         args.block = args.block or 64
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (worlds shard whole across GPUs with no "
               f"per-step exchange, so N GPUs would run N independent batches).", file=sys.stderr)
         sys.exit(2)
-    procs = max(1, min(16, os.cpu_count() or 1))
+    procs = cpu_threads()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     if args.workload == "serving":
         res = serving(args, procs, log)
@@ -247,85 +317,132 @@ def main():
             "per_rate": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in res]}))
         return
     K = args.steps
+    F0 = CONFIG_FRAMES - K if K < CONFIG_FRAMES else 0
+    window = f"{F0}..{F0 + K - 1}"
     # Host-side work that uses worker processes runs before anything touches the GPU.
     t0 = time.time()
     worlds = build_worlds(args.worlds, procs, args.worlds_cache)
     if args.pack_only:
         return
-    log(f"built {len(worlds)} worlds in {time.time() - t0:.1f}s; bodies/world mean {np.mean([w.NB for w in worlds]):.1f} "
-        f"max {max(w.NB for w in worlds)}")
-    cpu = None
+    nw = len(worlds)
+    log(f"built {nw} worlds in {time.time() - t0:.1f}s; bodies/world mean {np.mean([w.NB for w in worlds]):.1f} "
+        f"max {max(w.NB for w in worlds)}; window frames {window}")
+    cpu = ows = None
     if not args.no_cpu:
-        rate, steps, cwall = cpu_baseline(args.cpu_worlds, K, procs)
+        ncw = args.cpu_worlds or int(min(256, max(48, -(-4000 // K))))
+        rate, ows, cwall, cbusy = cpu_baseline(worlds[:ncw], F0, K, procs, log)
         cpu = {"value": round(rate, 1), "unit": "world-steps/s", "cores": procs, "kind": "port",
-               "sample": f"worlds 0..{args.cpu_worlds - 1} of the {args.worlds}-world quadruped batch, frames 0..{K - 1} from rest "
-                         f"(the timed window), FP32 oracle, one world at a time per process ({steps} world-steps, {cwall:.1f}s wall)"}
-        log(f"cpu baseline: {cpu['value']} world-steps/s on {procs} processes")
+               "sample": f"worlds 0..{ncw - 1} of the {nw}-world quadruped batch over the timed window (frames {window}, "
+                         f"advanced untimed from rest), FP32 oracle, std::threads in one process ({ncw * K} world-steps, "
+                         f"{cwall:.2f}s wall, {cbusy:.1f} CPU-s)"}
     from steel_webgpu_amd import sim
     dev = sim.Device(0)
     log(f"device {dev.name}, {dev.num_cus} CUs, {dev.total_mem / 2**30:.0f} GiB")
     graph = not args.no_graph
-    batch = sim.Batch(worlds, dev, block=256, graph=graph)
+    batch = sim.Batch(worlds, dev, block=64, graph=graph)
     log(f"batch resident: {batch.device_bytes() / 2**30:.2f} GiB")
-    # Autotuner: prepare/solve workgroup size by measured frame time over the first frames.
+    S = worlds[0].h("W_SUBSTEPS")
+    everyone = list(range(nw))
+    if args.warmup:
+        batch.step(args.warmup)
+    batch.reset()
+    # The window's starting state: advanced from rest (graph replays) or read from a state file.
+    t0 = time.perf_counter()
+    if args.state_in:
+        start = read_state_file(args.state_in)
+        restore_states(batch, everyone, start)
+        log(f"window start loaded from {args.state_in}")
+    else:
+        for f in range(0, F0, 100):
+            batch.step(min(100, F0 - f))
+        batch.sync()
+        start = save_states(batch, everyone) if F0 else None
+        log(f"advanced frames 0..{F0 - 1} untimed in {time.perf_counter() - t0:.1f}s")
+    if args.state_out:
+        write_state_file(args.state_out, start if start is not None else save_states(batch, everyone))
+        log(f"window start written to {args.state_out}")
+        batch.destroy()
+        dev.close()
+        return
+
+    def to_start():
+        batch.reset()
+        if start is not None:
+            restore_states(batch, everyone, start)
+
+    # Autotuner: prepare/solve workgroup size by measured frame time over the window's first frames.
     block = args.block
     if block == 0:
         best = None
         for bs in (64, 128, 256):
             batch.configure(bs, graph)
-            batch.reset()
+            to_start()
             ms = float(np.sum(batch.step_timed(10)))
-            log(f"  autotune block {bs}: {ms / 10:.2f} ms/frame over frames 0-9")
+            log(f"  autotune block {bs}: {ms / 10:.2f} ms/frame over frames {F0}..{F0 + 9}")
             if best is None or ms < best[1]:
                 best = (bs, ms)
         block = best[0]
     batch.configure(block, graph)
-    nw = len(worlds)
-    S = worlds[0].h("W_SUBSTEPS")
     CH = 100
     value = ms_per_step = None
     frame_ms = np.zeros(0)
+    plain_start = []
+    parity = None
     if not args.profile:
-        batch.step(args.warmup)
-        batch.reset()
+        to_start()
+        pidx = [i * (nw // max(1, args.plain_worlds)) for i in range(min(args.plain_worlds, nw))]
+        plain_start = save_states(batch, pidx) if start is not None else []
         batch.sync()
         parts = []
         t0 = time.perf_counter()
         for f0 in range(0, K, CH):
             parts.append(batch.step_timed(min(CH, K - f0)))
-            log(f"  timed frames {f0}..{f0 + len(parts[-1]) - 1}: {np.mean(parts[-1]):.3f} ms/frame")
+            log(f"  timed frames {F0 + f0}..{F0 + f0 + len(parts[-1]) - 1}: {np.mean(parts[-1]):.3f} ms/frame")
         batch.sync()
         wall = time.perf_counter() - t0
         frame_ms = np.concatenate(parts)
         value = nw * K / wall
         ms_per_step = wall / K * 1e3
-        log(f"batched: {value:.0f} world-steps/s, {ms_per_step:.3f} ms/frame over frames 0..{K - 1} (block {block})")
-    # Per-kernel times on a replay of the same frames (the step is bitwise deterministic), each kernel bracketed
-    # by HIP events on the library's stream; the algorithmic bytes are averaged over the same substeps.
-    batch.reset()
+        log(f"batched: {value:.0f} world-steps/s, {ms_per_step:.3f} ms/frame over frames {window} (block {block})")
+        if ows is not None:
+            same = 0
+            for i, o in enumerate(ows):
+                eq = all(np.array_equal(batch.read(i, n).view(np.uint32), getattr(o, n)[:batch.read(i, n).size].view(np.uint32))
+                         for n in ("poses", "vel", "art"))
+                same += int(eq)
+            parity = {"worlds": len(ows), "bitexact": same, "frames": f"0..{F0 + K - 1}",
+                      "compared": "poses, velocities, articulation state after the window, GPU batch vs FP32 oracle"}
+            log(f"parity: {same} of {len(ows)} sample worlds bit-exact against the FP32 oracle after frame {F0 + K - 1}")
+    # Per-kernel times on a replay of the same window (the step is bitwise deterministic), each kernel bracketed
+    # by HIP events on the library's stream; the algorithmic bytes are the same window's.
+    to_start()
+    base = [batch.stats_raw(i)[0].astype(np.int64) for i in everyone]
     acc = np.zeros(3)
     for f0 in range(0, K, CH):
         n = min(CH, K - f0)
         km = batch.profile_kernels(n * S)
         acc += np.asarray(km) * n
-        log(f"  kernel replay frames {f0}..{f0 + n - 1}: collide {km[0]:.3f} prepare {km[1]:.3f} solve {km[2]:.3f} ms/launch")
+        log(f"  kernel replay frames {F0 + f0}..{F0 + f0 + n - 1}: collide {km[0]:.3f} prepare {km[1]:.3f} solve {km[2]:.3f} ms/launch")
     kms = list(acc / K)
-    algo = algorithmic_bytes(batch, worlds)
+    algo = algorithmic_bytes(batch, worlds, base)
     names = ["collide", "prepare", "solve"]
     k = int(np.argmax(kms))
     achieved = algo[k] / (kms[k] * 1e-3) / 1e9
-    overflow = sum(1 for i in range(nw) if batch.stats_raw(i)[0][1] >= worlds[i].h("W_MAXC"))
+    overflow = sum(1 for i in everyone if batch.stats_raw(i)[0][1] >= worlds[i].h("W_MAXC"))
     log(f"kernels ms/launch collide {kms[0]:.3f} prepare {kms[1]:.3f} solve {kms[2]:.3f}; algorithmic MB/launch "
         f"{algo[0] / 1e6:.1f} {algo[1] / 1e6:.1f} {algo[2] / 1e6:.1f}; worlds at contact capacity: {overflow}")
     batch.destroy()
     if args.profile:
         copy_calibration()
+        dev.close()
         log("profile run done")
         return
-    # Plain per-world step: one world per batch, one launch per kernel per substep (no graph), same frames.
+    # Plain per-world step: one world per batch, one launch per kernel per substep (no graph), same window.
     plain_rates, plain_ms = [], []
-    for i in range(min(args.plain_worlds, nw)):
-        b1 = sim.Batch([worlds[i * (nw // max(1, args.plain_worlds))]], dev, block=64, graph=False)
+    for j, i in enumerate(pidx):
+        b1 = sim.Batch([worlds[i]], dev, block=64, graph=False)
+        if plain_start:
+            restore_states(b1, [0], [plain_start[j]])
         b1.sync()
         t1 = time.perf_counter()
         fm = b1.step_timed(K)
@@ -334,23 +451,35 @@ def main():
         plain_rates.append(K / dtp)
         plain_ms.extend(fm.tolist())
         b1.destroy()
-        log(f"  plain world {i * (nw // max(1, args.plain_worlds))}: {K / dtp:.1f} world-steps/s")
+        log(f"  plain world {i}: {K / dtp:.1f} world-steps/s")
+    dev.close()
     plain = float(np.mean(plain_rates))
     log(f"plain per-world: {plain:.1f} world-steps/s ({np.median(plain_ms):.3f} ms/frame GPU)")
+    fp64 = None
+    if ows is not None:
+        err, r32, r64 = fp64_reference(ows[:16], log)
+        fp64 = {"one_step_rel_error": err, "state": f"after frame {F0 + K - 1} of worlds 0..{min(16, len(ows)) - 1}",
+                "cpu_fp64_reference_world_steps_per_s": round(r64, 1), "cpu_fp32_single_thread_world_steps_per_s": round(r32, 1),
+                "note": "FP64 oracle: the error reference, timed on one thread; not a baseline"}
+    traffic, twin = traffic_from_profile(names[k], window)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "world-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded PCG32 scenes)",
         "config": {"workload": "quadruped_batch", "worlds": nw, "fps": 200, "substeps": S, "iterations": worlds[0].h("W_ITERS"),
-                   "frames": f"0..{K - 1} from rest", "block": block, "graph": graph},
+                   "frames": window + (f" (frames 0..{F0 - 1} of the 1000-frame run advanced untimed)" if F0 else " from rest"),
+                   "block": block, "graph": graph},
         "plain_world_steps_per_s": round(plain, 2),
         "speedup_vs_plain": round(value / plain, 1),
         "latency_ms": {"p50": round(float(np.percentile(frame_ms, 50)), 4), "p99": round(float(np.percentile(frame_ms, 99)), 4),
                        "plain_p50": round(float(np.percentile(plain_ms, 50)), 4), "plain_p99": round(float(np.percentile(plain_ms, 99)), 4)},
         "roofline": {"bound": "hbm", "kernel": names[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(names[k]),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_window": twin if traffic is not None else f"none for frames {window} (profile holds {twin})",
                      "kernel_ms": [round(x, 4) for x in kms], "algorithmic_bytes_per_launch": [int(x) for x in algo]},
         "cpu_baseline": cpu,
+        "parity": parity,
+        "fp64_error": fp64,
     }
     print(json.dumps(out))
 

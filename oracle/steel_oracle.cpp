@@ -12,7 +12,10 @@
 #include "omath.h"
 #include "../include/steel_world.h"
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdio>
+#include <thread>
 #include <vector>
 
 namespace oracle {
@@ -3874,6 +3877,88 @@ void oracle_warm_export_f32(uint32_t WH, const void* warm, uint32_t* out) {
     o[0] = f2u(s[i].v.x); o[1] = f2u(s[i].v.y); o[2] = f2u(s[i].v.z); o[3] = f2u(s[i].v.w);
   }
 }
+// One world's float-oracle buffers (as oracle_substeps_f32 takes them), for the threaded batch runner.
+struct OracleWorldPtrs {
+  const uint32_t* header;
+  const float* statF;
+  const uint32_t* statU;
+  float* poses;
+  float* vel;
+  float* art;
+  uint32_t* akey;
+  void* aval;
+  void* warm;
+  uint32_t* clock;
+  uint32_t* stats;
+};
+
+// CPU baseline runner: n worlds on `nthreads` std::threads (worlds dealt from a shared counter).  Every world
+// first steps `frames_untimed` frames; after all threads have finished that phase, every world steps
+// `frames_timed` frames, and timed_wall_s receives the wall time of that second phase (first world start to
+// last world end).  busy_s[t] receives thread t's stepping time in the timed phase.
+int oracle_run_batch_f32(uint32_t n, const OracleWorldPtrs* w, uint32_t frames_untimed, uint32_t frames_timed, uint32_t nthreads,
+                         double* timed_wall_s, double* busy_s) {
+  if (nthreads == 0) nthreads = 1;
+  auto phase = [&](uint32_t frames, bool timed) {
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < nthreads; t++)
+      th.emplace_back([&, t]() {
+        double busy = 0;
+        for (;;) {
+          uint32_t i = next.fetch_add(1);
+          if (i >= n) break;
+          const OracleWorldPtrs& x = w[i];
+          auto t0 = std::chrono::steady_clock::now();
+          run_substeps<float>(x.header, x.statF, x.statU, x.poses, x.vel, x.art, x.akey, (AnchorVal<float>*)x.aval, (WarmSlot<float>*)x.warm,
+                              x.clock, frames * x.header[W_SUBSTEPS], x.stats, nullptr);
+          busy += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        if (timed && busy_s) busy_s[t] = busy;
+      });
+    for (auto& x : th) x.join();
+  };
+  if (frames_untimed) phase(frames_untimed, false);
+  auto t0 = std::chrono::steady_clock::now();
+  if (frames_timed) phase(frames_timed, true);
+  if (timed_wall_s) *timed_wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+// One-step error reference: from the float oracle state of a world (poses, vel, art, anchors, warm table, clock),
+// steps `nsub` substeps in float in place and, from the same state widened to double, in double; the double
+// results go to poses64 / vel64 / art64.  wall_s[0] / wall_s[1] receive the float and double stepping times.
+int oracle_step_pair_f32_f64(const OracleWorldPtrs* w, uint32_t nsub, double* poses64, double* vel64, double* art64, double* wall_s) {
+  const uint32_t* h = w->header;
+  size_t np = (size_t)h[W_NPOSES] * 4, nv = (size_t)std::max(h[W_NB], 1u) * 8, na = h[W_NART];
+  for (size_t i = 0; i < np; i++) poses64[i] = w->poses[i];
+  for (size_t i = 0; i < nv; i++) vel64[i] = w->vel[i];
+  for (size_t i = 0; i < na; i++) art64[i] = w->art[i];
+  uint32_t AH = h[W_AH], WH = h[W_WH];
+  std::vector<uint32_t> akey(w->akey, w->akey + 2 * AH);
+  std::vector<AnchorVal<double>> aval(2 * AH);
+  const AnchorVal<float>* af = (const AnchorVal<float>*)w->aval;
+  auto v3 = [](V3<float> x) { return V3<double>{x.x, x.y, x.z}; };
+  for (uint32_t i = 0; i < 2 * AH; i++) aval[i] = {af[i].pair, af[i].k, v3(af[i].ll), v3(af[i].el), v3(af[i].il), v3(af[i].nl)};
+  std::vector<WarmSlot<double>> warm(std::max(2 * WH, 1u));
+  const WarmSlot<float>* wf = (const WarmSlot<float>*)w->warm;
+  for (uint32_t i = 0; i < 2 * WH; i++) warm[i] = {wf[i].key, V4<double>{wf[i].v.x, wf[i].v.y, wf[i].v.z, wf[i].v.w}};
+  uint32_t clock64[4], stats64[16];
+  for (int i = 0; i < 4; i++) clock64[i] = w->clock[i];
+  for (int i = 0; i < 16; i++) stats64[i] = w->stats[i];
+  auto t0 = std::chrono::steady_clock::now();
+  run_substeps<float>(h, w->statF, w->statU, w->poses, w->vel, w->art, w->akey, (AnchorVal<float>*)w->aval, (WarmSlot<float>*)w->warm,
+                      w->clock, nsub, w->stats, nullptr);
+  auto t1 = std::chrono::steady_clock::now();
+  run_substeps<double>(h, w->statF, w->statU, poses64, vel64, art64, akey.data(), aval.data(), warm.data(), clock64, nsub, stats64, nullptr);
+  auto t2 = std::chrono::steady_clock::now();
+  if (wall_s) {
+    wall_s[0] = std::chrono::duration<double>(t1 - t0).count();
+    wall_s[1] = std::chrono::duration<double>(t2 - t1).count();
+  }
+  return 0;
+}
+
 // Known-answer hooks for the oracle's own pinning tests.
 // World of static data only (geometry queries): header, statF and statU of a packed world.
 static World<double> geometry_world(const uint32_t* header, const float* statF, const uint32_t* statU) {

@@ -44,6 +44,10 @@ def P(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+class WorldPtrs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("header", "statF", "statU", "poses", "vel", "art", "akey", "aval", "warm", "clock", "stats")]
+
+
 class OracleWorld:
     """One packed world stepped by the CPU oracle (float = parity oracle, double = error reference)."""
 
@@ -81,6 +85,13 @@ class OracleWorld:
     def frames(self, n):
         self.substeps(n * self.pw.h("W_SUBSTEPS"))
 
+    def ptrs(self):
+        """The float oracle's buffers of this world as the C runners take them."""
+        assert not self.f64
+        pw = self.pw
+        return WorldPtrs(*(P(a).value for a in (pw.header, pw.statF, self.statU, self.poses, self.vel, self.art, self.akey, self.aval,
+                                                self.warm, self.count, self.stats)))
+
     def anchors_u32(self):
         assert not self.f64
         AH = self.pw.h("W_AH")
@@ -104,6 +115,67 @@ class OracleWorld:
         if self.f64:
             raise NotImplementedError
         return warm_table_map(self.warm_u32(), WH, parity)
+
+
+def run_batch(worlds, frames_untimed, frames_timed, nthreads):
+    """Step float-oracle worlds on nthreads std::threads (oracle_run_batch_f32): all worlds frames_untimed
+    frames, then all worlds frames_timed frames; returns (wall seconds of the timed phase, busy seconds per
+    thread)."""
+    L = lib()
+    arr = (WorldPtrs * len(worlds))(*[w.ptrs() for w in worlds])
+    wall = ctypes.c_double()
+    busy = (ctypes.c_double * max(nthreads, 1))()
+    L.oracle_run_batch_f32(ctypes.c_uint32(len(worlds)), arr, ctypes.c_uint32(frames_untimed), ctypes.c_uint32(frames_timed),
+                           ctypes.c_uint32(nthreads), ctypes.byref(wall), busy)
+    return wall.value, list(busy)
+
+
+def step_pair(o, nsub):
+    """From float-oracle world o's current state, nsub substeps in float (o, in place) and in double (returned as
+    (poses, vel, art) float64 arrays), plus the float and double wall times."""
+    pw = o.pw
+    p64 = np.zeros(pw.h("W_NPOSES") * 4)
+    v64 = np.zeros(max(pw.NB, 1) * 8)
+    a64 = np.zeros(pw.h("W_NART"))
+    walls = np.zeros(2)
+    ptr = o.ptrs()
+    lib().oracle_step_pair_f32_f64(ctypes.byref(ptr), ctypes.c_uint32(nsub), P(p64), P(v64), P(a64), P(walls))
+    return p64, v64, a64, walls
+
+
+def state_quantities(pw, poses, vel, art):
+    """Per-quantity views of a world's state: body COM positions and orientations, body twists (free bodies
+    from vel, articulated bodies from their link twists), joint coordinates q and velocities qd."""
+    from steel_webgpu_amd import pack as _pack
+    NB = pw.NB
+    P_ = np.asarray(poses, np.float64)[:NB * 8].reshape(NB, 8) if NB else np.zeros((0, 8))
+    V = np.asarray(vel, np.float64)[:NB * 8].reshape(NB, 8).copy() if NB else np.zeros((0, 8))
+    A = pw.A
+    art = np.asarray(art, np.float64)
+    for l, lk in enumerate(pw.links):
+        o = A["A_LINK"] + l * _pack.LB + _pack.L_V
+        b = lk["body"]
+        V[b] = [art[o + 3], art[o + 4], art[o + 5], 0, art[o], art[o + 1], art[o + 2], 0]
+    return {"position": P_[:, 0:3], "orientation": P_[:, 4:8], "linear_velocity": V[:, 0:3], "angular_velocity": V[:, 4:7],
+            "q": art[A["A_Q"]:A["A_Q"] + pw.h("W_NQ")], "qd": art[A["A_QD"]:A["A_QD"] + pw.ND]}
+
+
+def one_step_errors(o, nsub=None):
+    """Error of one float step against the double step from the same state: per quantity, the largest absolute
+    difference and that over the largest double magnitude (norm-wise relative error).  Steps o in place."""
+    pw = o.pw
+    nsub = nsub or pw.h("W_SUBSTEPS")
+    p64, v64, a64, walls = step_pair(o, nsub)
+    q32 = state_quantities(pw, o.poses, o.vel, o.art)
+    q64 = state_quantities(pw, p64, v64, a64)
+    out = {}
+    for k in q32:
+        if q64[k].size == 0:
+            continue
+        d = float(np.abs(q32[k] - q64[k]).max())
+        m = float(np.abs(q64[k]).max())
+        out[k] = {"abs": d, "rel": d / m if m > 0 else 0.0}
+    return out, walls
 
 
 def warm_table_map(buf, WH, parity):
