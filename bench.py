@@ -431,6 +431,13 @@ def main():
     overflow = sum(1 for i in everyone if batch.stats_raw(i)[0][1] >= worlds[i].h("W_MAXC"))
     log(f"kernels ms/launch collide {kms[0]:.3f} prepare {kms[1]:.3f} solve {kms[2]:.3f}; algorithmic MB/launch "
         f"{algo[0] / 1e6:.1f} {algo[1] / 1e6:.1f} {algo[2] / 1e6:.1f}; worlds at contact capacity: {overflow}")
+    if args.profile:
+        import hashlib
+        h = hashlib.sha256()
+        for i in everyone:
+            for n in ("poses", "vel", "art"):
+                h.update(batch.read(i, n, dtype=np.uint8).tobytes())
+        log(f"state after the window: sha256 {h.hexdigest()[:16]}")
     batch.destroy()
     if args.profile:
         copy_calibration()

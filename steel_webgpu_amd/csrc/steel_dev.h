@@ -33,7 +33,7 @@ struct WorldDev {
 struct SchedLayout {
   uint32_t HKEY, HCNT, CSLOT, CLOCAL, UKEY, USIZE, USTART, CSTART, META, CORDER, RK0, RK1, RK2, KEPT, OSTART, LEAD, SLOT, ROWOF,
       RUNIT, PCOL, PORD, UTC, UTP, RCON, UWOFF, WALK, HEADS, DSLOT, WCOL, WMASK, WCOUNT, LINKHIT, JSPLIT, SLOTT, NPAIRL, UF, UFF, WWIN,
-      WSLOT, TOTAL;
+      WSLOT, FPX, TOTAL;
 };
 
 __host__ __device__ inline SchedLayout sched_layout(uint32_t MAXC, uint32_t MAXU, uint32_t UH, uint32_t NCOL, uint32_t NB, uint32_t NCOLW,
@@ -80,6 +80,7 @@ __host__ __device__ inline SchedLayout sched_layout(uint32_t MAXC, uint32_t MAXU
   o = (o + 1) & ~1u;
   s.WWIN = o; o += 2 * WH;                                 // warm table: winning row order per slot (uint64)
   s.WSLOT = o; o += WH ? MAXC + MAXDENSE : 0;              // warm table: slot of each row
+  s.FPX = o; o += MAXC * 4;                                // prepare: each listed contact's reduction inputs (float4)
   s.TOTAL = o;
   return s;
 }

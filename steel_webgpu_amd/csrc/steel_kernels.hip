@@ -444,6 +444,14 @@ struct Prep {
     r.pos = pos; r.gap = c0.w; r.pk = (qx << 15) | (qy << 1) | 1u;
     return r;
   }
+  // bp_value with the six directions' cosines and sines taken from trig[2k], trig[2k + 1] (computed once per launch
+  // as cos_c / sin_c of k * 60 degrees, the values bp_value computes).
+  static DEV uint2 bp_value_t(f2 pos, float gap, uint32_t pk, uint32_t q, const float* trig) {
+    if (q == 6) return make_uint2(fflip_u(fbits(-gap)), pk);
+    if (q > 6 && gap > 0.0f) return make_uint2(0, 0);
+    uint32_t k = q > 6 ? q - 7 : q;
+    return make_uint2(fflip_u(fbits(dot(pos, mk2(trig[2 * k], trig[2 * k + 1])))), pk);
+  }
   DEV uint2 bp_value(const BPC& g, uint32_t q) const {  // wgsl.js:1498-1503
     if (q == 6) return make_uint2(fflip_u(fbits(-g.gap)), g.pk);
     if (q > 6 && g.gap > 0.0f) return make_uint2(0, 0);
@@ -1009,9 +1017,15 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   extern __shared__ uint32_t s_pdyn[];                 // coloring: color masks, then first-pending slots
   __shared__ uint32_t s_tree[(BS / 64) * 64];        // factor: link tree of each wave's articulation
   __shared__ float s_pab[(BS / 64) * 512];           // factor: children's bias forces
+  __shared__ float s_trig[12];                       // reduction: cos, sin of k * 60 degrees, k = 0..5
   uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
   const WorldDev wd = worlds[wid];
+  if (threadIdx.x < 6) {
+    const float ang = (float)threadIdx.x * 1.0471975511965976f;
+    s_trig[2 * threadIdx.x] = cos_c(ang);
+    s_trig[2 * threadIdx.x + 1] = sin_c(ang);
+  }
   load_header(Hs, wd);
   Prep P;
   P.w = make_world(Hs, wd);
@@ -1307,9 +1321,13 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   }
   __syncthreads();
   const uint32_t nl = min(tot, MAXC);
+  // Each listed contact's reduction inputs, once: group keys to RK0-2, (pos, gap, position identity) to the
+  // solve's eigenbasis table (unused until the solve).
+  float4* bpx = (float4*)(sched + S.FPX);
   for (uint32_t p = tid; p < nl; p += BS) {
     Prep::BPC g = P.bp_contact(sched[S.CORDER + p]);
     sched[S.RK0 + p] = g.k0; sched[S.RK1 + p] = g.k1; sched[S.RK2 + p] = g.k2;
+    bpx[p] = make_float4(g.pos.x, g.pos.y, g.gap, ubits(g.pk));
   }
   for (uint32_t sp = tid; sp < nu; sp += BS) sched[S.OSTART + sp] = sched[S.USTART + sp];
   __syncthreads();
@@ -1324,23 +1342,29 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   }
   __syncthreads();
   for (uint32_t p = tid; p < nl; p += BS) {
-    Prep::BPC g = P.bp_contact(sched[S.CORDER + p]);
+    const float4 g = bpx[p];
     uint32_t sl = S.SLOT + sched[S.LEAD + p] * 26;
-    for (uint32_t q = 0; q < 13; q++) { uint2 v = P.bp_value(g, q); if (v.y != 0) atomicMax(&sched[sl + q * 2], v.x); }
+    for (uint32_t q = 0; q < 13; q++) {
+      uint2 v = Prep::bp_value_t(mk2(g.x, g.y), g.z, fbits(g.w), q, s_trig);
+      if (v.y != 0) atomicMax(&sched[sl + q * 2], v.x);
+    }
   }
   __syncthreads();
   for (uint32_t p = tid; p < nl; p += BS) {
-    Prep::BPC g = P.bp_contact(sched[S.CORDER + p]);
+    const float4 g = bpx[p];
     uint32_t sl = S.SLOT + sched[S.LEAD + p] * 26;
-    for (uint32_t q = 0; q < 13; q++) { uint2 v = P.bp_value(g, q); if (v.y != 0 && v.x == sched[sl + q * 2]) atomicMax(&sched[sl + q * 2 + 1], v.y); }
+    for (uint32_t q = 0; q < 13; q++) {
+      uint2 v = Prep::bp_value_t(mk2(g.x, g.y), g.z, fbits(g.w), q, s_trig);
+      if (v.y != 0 && v.x == sched[sl + q * 2]) atomicMax(&sched[sl + q * 2 + 1], v.y);
+    }
   }
   __syncthreads();
   for (uint32_t p = tid; p < nl; p += BS) {
-    Prep::BPC g = P.bp_contact(sched[S.CORDER + p]);
+    const float4 g = bpx[p];
     uint32_t sl = S.SLOT + sched[S.LEAD + p] * 26;
     bool win = false;
     for (uint32_t q = 0; q < 13; q++) {
-      uint2 v = P.bp_value(g, q);
+      uint2 v = Prep::bp_value_t(mk2(g.x, g.y), g.z, fbits(g.w), q, s_trig);
       if (v.y != 0 && v.x == sched[sl + q * 2] && v.y == sched[sl + q * 2 + 1]) win = true;
     }
     sched[S.HEADS + p] = (win || w.h(W_COMPL)) ? 1u : 0u;  // compliant contacts stay whole
@@ -2532,6 +2556,267 @@ struct Solver {
 
 // Dense rows of every articulation, one wave each in rounds, then the twists of the articulations they moved
 // (wgsl.js:3366-3377 dense_phase).
+// ---- The default-path row solve with explicit address spaces.  Solver::solve_unit<false> reached its pointers
+// through the Solver object (on the stack, since the object's address escapes to out-of-line members) and used
+// flat memory operations, so every access waited for every outstanding load; here the row pointers are global,
+// the twists and impulses LDS (worlds whose twists fit) or global, all carried in registers, and the row's
+// records are loaded before anything is stored.  Same arithmetic, in the same order, as solve_unit<false>.
+#define AS_GLOBAL __attribute__((address_space(1)))
+#define AS_LDS __attribute__((address_space(3)))
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+template <bool TWL> struct TwSpace { typedef AS_GLOBAL v4f T; };
+template <> struct TwSpace<true> { typedef AS_LDS v4f T; };
+DEV float4 to4(v4f v) { return make_float4(v.x, v.y, v.z, v.w); }
+DEV v4f tov(float4 v) { v4f r = {v.x, v.y, v.z, v.w}; return r; }
+DEV f3 xyzv(v4f v) { return mk3(v.x, v.y, v.z); }
+
+template <bool TWL>
+struct RowCtx {
+  const AS_GLOBAL v4f* rows;
+  typename TwSpace<TWL>::T* tw;
+  typename TwSpace<TWL>::T* imp;
+  AS_GLOBAL float* uload;   // art + A_ULOAD
+  const AS_GLOBAL v4u* slots;
+  uint32_t ROWS, PO;
+  float omega, beta_anchor, dt;
+  bool extra, patch, vpass;
+};
+
+template <bool TWL>
+DEV RowCtx<TWL> row_ctx(const Solver& sv) {
+  const W& w = sv.w;
+  RowCtx<TWL> c;
+  c.rows = (const AS_GLOBAL v4f*)w.d.rows;
+  c.tw = (typename TwSpace<TWL>::T*)sv.tw;
+  c.imp = (typename TwSpace<TWL>::T*)sv.imp;
+  c.uload = (AS_GLOBAL float*)(w.d.art + w.h(W_A_ULOAD));
+  c.slots = (const AS_GLOBAL v4u*)(w.d.sched + w.S.SLOTT);
+  c.ROWS = w.ROWS();
+  c.PO = w.PO();
+  c.omega = w.hf(W_F_OMEGA);
+  c.beta_anchor = w.hf(W_F_ANCHOR_BETA);
+  c.dt = w.hf(W_DT);
+  c.extra = w.h(W_EXTRA) != 0;
+  c.patch = w.h(W_PATCH) != 0;
+  c.vpass = sv.vpass;
+  return c;
+}
+
+// Accumulated link impulses.  Free bodies' entries are written too: nothing reads them (only link bodies' are
+// propagated), which saves the body-kind lookup.
+template <bool TWL>
+DEV void row_add_imp(const RowCtx<TWL>& c, int b, f3 f, f3 t) {
+  if (b < 0) return;
+  const v4f i0 = c.imp[(uint32_t)b * 2], i1 = c.imp[(uint32_t)b * 2 + 1];
+  c.imp[(uint32_t)b * 2] = tov(make_float4(i0.x + f.x, i0.y + f.y, i0.z + f.z, i0.w + 0.0f));
+  c.imp[(uint32_t)b * 2 + 1] = tov(make_float4(i1.x + t.x, i1.y + t.y, i1.z + t.z, i1.w + 0.0f));
+}
+
+template <bool TWL>
+DEV void solve_row(const RowCtx<TWL>& c, uint32_t us, uint32_t r, int a, int b, uint32_t mode) {  // wgsl.js:3033-3231
+  const AS_GLOBAL v4f* R = c.rows + (size_t)r * c.ROWS;
+  const uint32_t PO = c.PO;
+  // Every record the row reads, first.
+  const float4 r0 = to4(R[0]), r1 = to4(R[1]), r2 = to4(R[2]), r5 = to4(R[5]);
+  const f3 ra = xyzv(R[3]), rb = xyzv(R[4]);
+  float4 lam = to4(R[6]);
+  const f3 na0 = xyzv(R[7]), na1 = xyzv(R[8]), nb0 = xyzv(R[9]), nb1 = xyzv(R[10]);
+  const f3 t1a0 = xyzv(R[11]), t1a1 = xyzv(R[12]), t1b0 = xyzv(R[13]), t1b1 = xyzv(R[14]);
+  const f3 t2a0 = xyzv(R[15]), t2a1 = xyzv(R[16]), t2b0 = xyzv(R[17]), t2b1 = xyzv(R[18]);
+  float4 x0 = make_float4(0, 0, 0, 0), x1 = x0, x2 = x0;
+  if (c.extra) { x0 = to4(R[PO]); x1 = to4(R[PO + 1]); x2 = to4(R[PO + 2]); }
+  const f3 n = xyz(r0), t1 = xyz(r1), t2 = xyz(r2);
+  f3 ta[2], tb[2];
+  if (a >= 0) { ta[0] = xyzv(c.tw[(uint32_t)a * 2]); ta[1] = xyzv(c.tw[(uint32_t)a * 2 + 1]); } else { ta[0] = mk3(0, 0, 0); ta[1] = ta[0]; }
+  if (b >= 0) { tb[0] = xyzv(c.tw[(uint32_t)b * 2]); tb[1] = xyzv(c.tw[(uint32_t)b * 2 + 1]); } else { tb[0] = mk3(0, 0, 0); tb[1] = tb[0]; }
+  const f3 fra = c.extra ? xyz(x0) : ra, frb = c.extra ? xyz(x1) : rb;
+  AS_GLOBAL v4f* Rw = (AS_GLOBAL v4f*)R;
+  if (mode == 0) {
+    const float dn = lam.x, dt1 = lam.y, dt2 = lam.z;
+    ta[0] += na0 * dn; ta[1] += na1 * dn; tb[0] += nb0 * dn; tb[1] += nb1 * dn;
+    ta[0] += t1a0 * dt1; ta[1] += t1a1 * dt1; tb[0] += t1b0 * dt1; tb[1] += t1b1 * dt1;
+    ta[0] += t2a0 * dt2; ta[1] += t2a1 * dt2; tb[0] += t2b0 * dt2; tb[1] += t2b1 * dt2;
+    const f3 fn = n * dn, ft = t1 * dt1 + t2 * dt2;
+    row_add_imp(c, a, -fn - ft, -cross(ra, fn) - cross(fra, ft));
+    row_add_imp(c, b, fn + ft, cross(rb, fn) + cross(frb, ft));
+  } else {
+    const bool anchor = c.extra && c.patch && x0.w == 2.0f;
+    const float uload0 = (c.extra && c.patch && x0.w != 1.0f) ? c.uload[us] : 0.0f;
+    const f3 vrel = (tb[0] + cross(tb[1], rb)) - (ta[0] + cross(ta[1], ra));
+    const float jv = dot(n, vrel);
+    const float wgt = c.vpass ? 1.0f : r5.w;
+    const float res = jv + r1.w;
+    const float nl = mx(lam.x + c.omega * (-res * r0.w * wgt - (1.0f - wgt) * lam.x), 0.0f);
+    const float dn = nl - lam.x;
+    lam.x = nl;
+    ta[0] += na0 * dn; ta[1] += na1 * dn;
+    tb[0] += nb0 * dn; tb[1] += nb1 * dn;
+    row_add_imp(c, a, -n * dn, -cross(ra, n * dn));
+    row_add_imp(c, b, n * dn, cross(rb, n * dn));
+    float lim = r2.w * nl;
+    f2 bias = mk2(0, 0);
+    const float role = c.extra ? x0.w : 1.0f;
+    if (c.extra && c.patch && role != 1.0f) {
+      const float load = uload0 + dn;
+      c.uload[us] = load;
+      if (anchor) {
+        lim = r2.w * load;
+        const f3 e = xyz(x2);
+        if (!c.vpass) bias = mk2(c.beta_anchor * dot(t1, e) / c.dt, c.beta_anchor * dot(t2, e) / c.dt);
+        Rw[PO + 2] = tov(make_float4(x2.x, x2.y, x2.z, lim));
+      }
+    }
+    if (role != 0.0f) {
+      const f3 vr2 = (tb[0] + cross(tb[1], frb)) - (ta[0] + cross(ta[1], fra));
+      const float4 ki = r5;
+      const float v1 = dot(t1, vr2) + bias.x, v2 = dot(t2, vr2) + bias.y;
+      const float kd = ki.x * ki.z - ki.y * ki.y;
+      const f2 lt = Solver::friction_pair_solve(ki.z / kd, -ki.y / kd, ki.x / kd, v1, v2, lam.y, lam.z, lim, c.omega);
+      const float dt1 = lt.x - lam.y, dt2 = lt.y - lam.z;
+      lam.y = lt.x; lam.z = lt.y;
+      ta[0] += t1a0 * dt1 + t2a0 * dt2;
+      ta[1] += t1a1 * dt1 + t2a1 * dt2;
+      tb[0] += t1b0 * dt1 + t2b0 * dt2;
+      tb[1] += t1b1 * dt1 + t2b1 * dt2;
+      const f3 f = t1 * dt1 + t2 * dt2;
+      row_add_imp(c, a, -f, -cross(fra, f));
+      row_add_imp(c, b, f, cross(frb, f));
+    }
+    Rw[6] = tov(lam);
+  }
+  if (a >= 0) { c.tw[(uint32_t)a * 2] = tov(F4(ta[0], 0.0f)); c.tw[(uint32_t)a * 2 + 1] = tov(F4(ta[1], 0.0f)); }
+  if (b >= 0) { c.tw[(uint32_t)b * 2] = tov(F4(tb[0], 0.0f)); c.tw[(uint32_t)b * 2 + 1] = tov(F4(tb[1], 0.0f)); }
+}
+
+// ---- Articulation propagation of the sweep with the link mirror addressed as LDS (the Solver's members are generic
+// pointers, so its out-of-line passes used flat accesses that wait for all outstanding memory).  Same arithmetic, in
+// the same order, as Solver::art_contact_impulses and Solver::art_apply.
+template <bool TWL>
+struct ArtLds {
+  AS_LDS float* lm;
+  const AS_LDS v4u* lmeta;
+  const AS_LDS v4u* ltree;
+  AS_LDS float* lcon;
+  AS_LDS float* laqd;
+  typename TwSpace<TWL>::T* tw;
+  typename TwSpace<TWL>::T* imp;
+  const AS_GLOBAL v4u* art_info;   // statU + SU_ART
+  uint32_t MP;
+  DEV int mpar(uint32_t l) const { return (int)lmeta[l].x; }
+  DEV uint32_t mnd(uint32_t l) const { return lmeta[l].y & 0xffu; }
+  DEV uint32_t mdof(uint32_t l) const { return lmeta[l].z; }
+  DEV uint32_t mbody(uint32_t l) const { return lmeta[l].w; }
+  DEV AS_LDS float* ml(uint32_t l) const { return lm + (lmeta[l].y >> 8); }
+  static DEV SVd msv(const AS_LDS float* p) { return mksv(mk3(p[0], p[1], p[2]), mk3(p[3], p[4], p[5])); }
+  static DEV void mst(AS_LDS float* p, SVd v) { p[0] = v.a.x; p[1] = v.a.y; p[2] = v.a.z; p[3] = v.l.x; p[4] = v.l.y; p[5] = v.l.z; }
+  DEV SVd mS(uint32_t l, uint32_t k) const { return msv(ml(l) + k * 6); }
+  DEV SVd mU(uint32_t l, uint32_t k) const { return msv(ml(l) + 6 * mnd(l) + k * 6); }
+  DEV float mDI(uint32_t l, uint32_t r, uint32_t c) const { uint32_t n = mnd(l); return ml(l)[12 * n + r * n + c]; }
+  DEV AS_LDS float* mP(uint32_t l) const { uint32_t n = mnd(l); return ml(l) + 12 * n + n * n; }
+  DEV AS_LDS float* mDV(uint32_t l) const { return mP(l) + 6; }
+  DEV AS_LDS float* mV(uint32_t l) const { return mP(l) + 12; }
+  DEV AS_LDS float* mUU(uint32_t l) const { return mP(l) + 18; }
+  DEV AS_LDS float* mTAU(uint32_t l) const { return mP(l) + 18 + mnd(l); }
+  DEV f3 mcom(uint32_t l) const { const AS_LDS float* p = mP(l) + 18 + 2 * mnd(l); return mk3(p[0], p[1], p[2]); }
+  DEV f3 mr(uint32_t l) const {
+    int par = mpar(l);
+    if (par < 0) return mk3(0, 0, 0);
+    return mcom(l) - mcom((uint32_t)par);
+  }
+  DEV void set_link_twist(uint32_t l, SVd v) const {
+    uint32_t b = mbody(l);
+    tw[b * 2] = tov(F4(v.l, 0.0f)); tw[b * 2 + 1] = tov(F4(v.a, 0.0f));
+  }
+  DEV void apply(uint32_t a, uint32_t lane) const {  // wgsl.js:3254-3299
+    const v4u ai = art_info[a];
+    const bool has = lane < ai.y;
+    const uint32_t l = ai.x + lane, dl = has ? ltree[l].x : 0xffffffffu;
+    Solver::wave_sync();
+    for (int d = (int)MP - 1; d >= 0; d--) {
+      if (dl == (uint32_t)d) {
+        uint32_t n = mnd(l);
+        AS_LDS float *UU = mUU(l), *TAU = mTAU(l);
+        SVd p = msv(mP(l));
+        float u[6];
+        for (uint32_t r = 0; r < n; r++) { u[r] = TAU[r] - sdot(mS(l, r), p); UU[r] = u[r]; }
+        if (mpar(l) >= 0) {
+          SVd pp = p;
+          for (uint32_t r = 0; r < n; r++) {
+            float du = 0.0f;
+            for (uint32_t c = 0; c < n; c++) du += mDI(l, r, c) * u[c];
+            pp = sadd(pp, sscale(mU(l, r), du));
+          }
+          mst(lcon + l * 8, fshift(pp, mr(l)));
+        }
+      }
+      Solver::wave_sync();
+      if (d > 0 && dl == (uint32_t)(d - 1)) {
+        uint32_t c = ltree[l].y;
+        if (c != 0xffffffffu) {
+          SVd P = msv(mP(l));
+          for (; c != 0xffffffffu; c = ltree[c].z) P = sadd(P, msv(lcon + c * 8));
+          mst(mP(l), P);
+        }
+      }
+      Solver::wave_sync();
+    }
+    for (uint32_t d = 0; d < MP; d++) {
+      if (dl == d) {
+        int par = mpar(l);
+        SVd dv = sv0();
+        if (par >= 0) dv = mshift(msv(mDV((uint32_t)par)), mr(l));
+        uint32_t n = mnd(l);
+        const AS_LDS float* UU = mUU(l);
+        float dq[6];
+        for (uint32_t r = 0; r < n; r++) {
+          dq[r] = 0.0f;
+          for (uint32_t c = 0; c < n; c++) dq[r] += mDI(l, r, c) * (UU[c] - sdot(mU(l, c), dv));
+        }
+        for (uint32_t r = 0; r < n; r++) { dv = sadd(dv, sscale(mS(l, r), dq[r])); laqd[mdof(l) + r] += dq[r]; }
+        mst(mDV(l), dv);
+        SVd v = sadd(msv(mV(l)), dv);
+        mst(mV(l), v);
+        set_link_twist(l, v);
+        mst(mP(l), sv0());
+        AS_LDS float* TAU = mTAU(l);
+        for (uint32_t r = 0; r < n; r++) TAU[r] = 0.0f;
+      }
+      Solver::wave_sync();
+    }
+  }
+  DEV void contact_impulses(uint32_t a, uint32_t lane) const {  // wgsl.js:3380-3394
+    const v4u ai = art_info[a];
+    bool hit = false;
+    if (lane < ai.y) {
+      uint32_t l = ai.x + lane, b = mbody(l);
+      const v4f i0 = imp[b * 2], i1 = imp[b * 2 + 1];
+      f3 f = xyzv(i0), t = xyzv(i1);
+      if (vne0(f) || vne0(t)) {
+        hit = true;
+        mst(mP(l), mksv(-t, -f));
+        imp[b * 2] = tov(make_float4(0, 0, 0, 0)); imp[b * 2 + 1] = tov(make_float4(0, 0, 0, 0));
+      }
+    }
+    if (__ballot(hit) != 0) apply(a, lane);
+  }
+};
+
+template <bool TWL>
+DEV ArtLds<TWL> art_lds(const Solver& sv) {
+  ArtLds<TWL> m;
+  m.lm = (AS_LDS float*)sv.lm;
+  m.lmeta = (const AS_LDS v4u*)sv.lmeta;
+  m.ltree = (const AS_LDS v4u*)sv.ltree;
+  m.lcon = (AS_LDS float*)sv.lcon;
+  m.laqd = (AS_LDS float*)sv.laqd;
+  m.tw = (typename TwSpace<TWL>::T*)sv.tw;
+  m.imp = (typename TwSpace<TWL>::T*)sv.imp;
+  m.art_info = (const AS_GLOBAL v4u*)(sv.w.d.statU + sv.w.h(W_SU_ART));
+  m.MP = sv.w.h(W_MAXPATH);
+  return m;
+}
+
 template <int BS>
 DEV void dense_phase(const Solver& sv, uint32_t mode, uint32_t nd) {
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, NA = sv.w.NA();
@@ -2545,7 +2830,8 @@ DEV void dense_phase(const Solver& sv, uint32_t mode, uint32_t nd) {
   }
 }
 
-template <int BS>
+// FAST: the default-path rows go through solve_row with the twists in LDS (TWL) or global memory.
+template <int BS, bool FAST = false, bool TWL = false>
 DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint32_t nd) {
   const W& w = sv.w;
   const uint32_t tid = threadIdx.x, NA = w.NA(), NCOL = w.NCOL(), PO = w.PO();
@@ -2553,8 +2839,26 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
   const SchedLayout& S = w.S;
   const uint32_t lane = tid & 63, wave = tid >> 6;
   const bool ext = w.h(W_FMODE) || w.h(W_ANG) || w.h(W_TORS) || w.h(W_COMPL);
-  for (uint32_t a = wave; a < NA; a += BS / 64) sv.art_propagate_rows(a, lane);
+  // Out-of-line calls spill the caller's live registers: call only for articulations with joint rows.
+  for (uint32_t a = wave; a < NA; a += BS / 64)
+    if (!FAST || w.d.art[w.h(W_A_RLN) + a] != 0.0f) sv.art_propagate_rows(a, lane);
   if (w.h(W_PATCH) && mode == 1) {
+    if (FAST) {
+      const AS_GLOBAL uint32_t* ust = (const AS_GLOBAL uint32_t*)(sched + S.USTART);
+      const AS_GLOBAL uint32_t* usz = (const AS_GLOBAL uint32_t*)(sched + S.USIZE);
+      const AS_GLOBAL float* rw = (const AS_GLOBAL float*)w.d.rows;
+      AS_GLOBAL float* ul = (AS_GLOBAL float*)(w.d.art + w.h(W_A_ULOAD));
+      const uint32_t RW = w.ROWS() * 4;
+      for (uint32_t us = tid; us < nu; us += BS) {
+        const uint32_t s0 = ust[us], n = usz[us];
+        float load = 0.0f;
+        for (uint32_t m = s0; m < s0 + n; m++) {
+          const float role = rw[(size_t)m * RW + PO * 4 + 3], ln = rw[(size_t)m * RW + 6 * 4];
+          if (role != 1.0f) load += ln;
+        }
+        ul[us] = load;
+      }
+    } else
     for (uint32_t us = tid; us < nu; us += BS) {
       uint32_t s0 = sched[S.USTART + us];
       float load = 0.0f;
@@ -2563,8 +2867,20 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
     }
   }
   __syncthreads();
-  dense_phase<BS>(sv, 1, nd);
+  if (!FAST || nd) dense_phase<BS>(sv, 1, nd);
   const uint4* slots = (const uint4*)(sched + S.SLOTT);
+  if (FAST && !ext) {
+    const RowCtx<TWL> rc = row_ctx<TWL>(sv);
+    const AS_GLOBAL uint32_t* cst = (const AS_GLOBAL uint32_t*)(sched + S.CSTART);
+    const uint32_t nc = min(NCOL, ncol);
+    uint32_t s = cst[0];
+    for (uint32_t c = 0; c < nc; c++) {
+      const uint32_t e = cst[c + 1];
+      for (uint32_t u = s + tid; u < e; u += BS) { const v4u sl = rc.slots[u]; solve_row<TWL>(rc, sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
+      s = e;
+      __syncthreads();
+    }
+  } else
   for (uint32_t c = 0; c < NCOL && c < ncol; c++) {
     uint32_t s = sched[S.CSTART + c], e = sched[S.CSTART + c + 1];
     if (ext)
@@ -2582,6 +2898,14 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
     }
   }
   __syncthreads();
+  if (FAST) {
+    const ArtLds<TWL> am = art_lds<TWL>(sv);
+    const bool vlim = w.h(W_VLIM_ON) != 0;
+    for (uint32_t a = wave; a < NA; a += BS / 64) {
+      am.contact_impulses(a, lane);
+      if (vlim && w.d.art[w.h(W_A_AVL) + a] > 0.5f) sv.art_velocity_limits(a, lane);
+    }
+  } else
   for (uint32_t a = wave; a < NA; a += BS / 64) {
     sv.art_contact_impulses(a, lane);
     sv.art_velocity_limits(a, lane);
@@ -2854,12 +3178,12 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   for (uint32_t a = tid >> 6; a < NA; a += BS / 64) sv.art_twists(a, tid & 63);
   __syncthreads();
   dense_phase<BS>(sv, 0, nd);
-  sweep<BS>(sv, 0, ncol, nu, nd);
+  sweep<BS, true, TWL>(sv, 0, ncol, nu, nd);
   const uint32_t iters = w.h(W_ITERS);
   f2 kb = mk2(0.0f, 0.0f);
   for (uint32_t it = 0; it < iters; it++) {
     if (NEST) kb = momentum_step<BS>(sv, ncol, it, kb);
-    sweep<BS>(sv, 1, ncol, nu, nd);
+    sweep<BS, true, TWL>(sv, 1, ncol, nu, nd);
   }
   if (w.h(W_VELITERS)) velocity_pass<BS>(sv, ncol, nu, nd);
   // Mutable mirror fields back to the link blocks (p, dv, v, u, tau; S, U, D^-1, COM are unchanged).

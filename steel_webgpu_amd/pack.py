@@ -126,7 +126,7 @@ def sched_layout(MAXC, MAXU, UH, NCOL, NB, NCOLW, NL, WH=0, MAXDENSE=0):
              ("UTC", MAXU), ("UTP", MAXU), ("RCON", MAXC), ("UWOFF", UH), ("WALK", MAXC), ("HEADS", MAXC + 1), ("DSLOT", MAXC + 1),
              ("WCOL", MAXC), ("WMASK", max(NB, 1) * NCOLW), ("WCOUNT", NCOL + 2), ("LINKHIT", max(NB, 1)), ("JSPLIT", max(NL, 1)),
              ("SLOTT", MAXC * 4), ("NPAIRL", 4), ("UF", max(NB, 1) + max(NL, 1)), ("UFF", max(NB, 1) + max(NL, 1)),
-             ("WWIN", 2 * WH), ("WSLOT", (MAXC + MAXDENSE) if WH else 0)]
+             ("WWIN", 2 * WH), ("WSLOT", (MAXC + MAXDENSE) if WH else 0), ("FPX", MAXC * 4)]
     out, o = {}, 0
     for k, n in names:
         if k == "WWIN":

@@ -99,54 +99,204 @@ def qmat(q):
 _HULL_CACHE: dict = {}
 
 
-def convexHull(points):
-    """Convex hull of a point cloud: {'verts': [[x,y,z]..], 'faces': [{'n','d','v'}..]}.
+def _sub(a, b):
+    return [a[0] - b[0], a[1] - b[1], a[2] - b[2]]
 
-    Outward polygon faces with coplanar triangles merged and vertices counter-clockwise about the
-    normal (plane n.x = d), the layout builder.js:58-173 produces.  Qhull gives the facets; merging
-    and loop ordering are done here.
-    """
+
+def _add(a, b):
+    return [a[0] + b[0], a[1] + b[1], a[2] + b[2]]
+
+
+def _scale(a, s):
+    return [a[0] * s, a[1] * s, a[2] * s]
+
+
+def _dot(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+
+
+def _cross(a, b):
+    return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+
+
+def _hypot3(a):
+    """Math.hypot of three numbers as the JS engines compute it: scaled by the largest magnitude, squares summed
+    with Kahan compensation, then sqrt times the scale (so the hull planes round like the reference's)."""
+    m = max(abs(a[0]), abs(a[1]), abs(a[2]))
+    if m == 0.0:
+        return 0.0
+    total = comp = 0.0
+    for v in a:
+        x = abs(v) / m
+        summand = x * x - comp
+        pre = total + summand
+        comp = (pre - total) - summand
+        total = pre
+    return math.sqrt(total) * m
+
+
+class _Face:
+    __slots__ = ("v", "n", "d", "alive", "out", "vis", "group")
+
+    def __init__(self, P, a, b, c):
+        n = _cross(_sub(P[b], P[a]), _sub(P[c], P[a]))
+        n = _scale(n, 1.0 / _hypot3(n))
+        self.v, self.n, self.d = (a, b, c), n, _dot(n, P[a])
+        self.alive, self.out, self.vis, self.group = True, [], False, None
+
+
+def convexHull(points):
+    """Convex hull of a point cloud: {'verts': [[x,y,z]..], 'faces': [{'n','d','v'}..]}, outward polygon faces
+    (coplanar triangles merged, vertices counter-clockwise about the normal, plane n.x = d).
+
+    Follows the reference's incremental quickhull step for step (builder.js:61-173), in double precision like
+    the JS host, so a SceneBuilder scene gets the reference's vertex order, face order and plane floats (which
+    set feature ids and contact keys downstream): initial tetrahedron from extreme points; each point waits on
+    the face it is farthest outside (beyond eps); the first live face with waiting points grows the hull through
+    the faces its farthest point sees, breadth first over directed edges; connected coplanar triangles merge by a
+    flood fill, each plane keeping its first triangle's (n, d); each face's loop is the 2D monotone-chain hull of
+    its points; vertices are the used points in index order."""
     key = tuple(tuple(float(c) for c in p) for p in points)
     hit = _HULL_CACHE.get(key)
     if hit is not None:
         return hit
-    from scipy.spatial import ConvexHull as _QH
-    uniq = []
-    seen = set()
+    seen = {}
     for p in key:
-        if p not in seen:
-            seen.add(p)
-            uniq.append(p)
-    if len(uniq) < 4:
+        if p not in seen:  # -0.0 and 0.0 are one key, as in the reference's string keys
+            seen[p] = [p[0], p[1], p[2]]
+    P = list(seen.values())
+    if len(P) < 4:
         raise ValueError("a convex hull needs at least 4 distinct points")
-    P = np.asarray(uniq, dtype=np.float64)
-    ext = float(np.linalg.norm(P.max(0) - P.min(0)))
-    eps = 1e-7 * max(ext, 1e-12)
-    qh = _QH(P)
-    tris = qh.simplices
-    eqs = qh.equations  # n . x + off <= 0 inside; outward n
-    # Group coplanar facets: same outward normal and offset within tolerance.
-    groups = []
-    for t, e in zip(tris, eqs):
-        n = e[:3]
-        d = -e[3]
-        for g in groups:
-            if np.dot(g["n"], n) > 1 - 1e-9 and abs(g["d"] - d) <= 10 * eps:
-                g["ids"].update(int(i) for i in t)
-                break
-        else:
-            groups.append({"n": n, "d": d, "ids": set(int(i) for i in t)})
+    lo = [math.inf] * 3
+    hi = [-math.inf] * 3
+    for p in P:
+        lo = [min(lo[k], p[k]) for k in range(3)]
+        hi = [max(hi[k], p[k]) for k in range(3)]
+    eps = 1e-7 * max(_hypot3(_sub(hi, lo)), 1e-12)
+    i0 = 0
+    for i in range(1, len(P)):
+        if P[i][0] < P[i0][0]:
+            i0 = i
+
+    def far(fn):
+        best, bd = -1, -1.0
+        for i, p in enumerate(P):
+            d = fn(p)
+            if d > bd:
+                bd, best = d, i
+        return best, bd
+
+    i1, _ = far(lambda p: _hypot3(_sub(p, P[i0])))
+    e = _sub(P[i1], P[i0])
+    ne = _hypot3(e)
+    i2, d2 = far(lambda p: _hypot3(_cross(e, _sub(p, P[i0]))) / ne)
+    if d2 <= eps:
+        raise ValueError("convex hull points are collinear")
+    n0 = _cross(e, _sub(P[i2], P[i0]))
+    nn0 = _hypot3(n0)
+    i3, d3 = far(lambda p: abs(_dot(n0, _sub(p, P[i0]))) / nn0)
+    if d3 <= eps:
+        raise ValueError("convex hull points are coplanar")
+    center = _scale(_add(_add(P[i0], P[i1]), _add(P[i2], P[i3])), 0.25)
+    K = 1048576
+    edges = {}  # directed edge a * K + b -> face
+
+    def make(a, b, c):
+        f = _Face(P, a, b, c)
+        for k in range(3):
+            edges[f.v[k] * K + f.v[(k + 1) % 3]] = f
+        return f
+
+    def oriented(a, b, c):
+        f = _Face(P, a, b, c)
+        return (a, c, b) if _dot(f.n, center) - f.d > 0 else (a, b, c)
+
+    faces = [make(*oriented(*t)) for t in ((i0, i1, i2), (i0, i1, i3), (i0, i2, i3), (i1, i2, i3))]
+
+    def assign(pts, cands):
+        for i in pts:
+            best, bd = None, eps
+            for f in cands:
+                d = _dot(f.n, P[i]) - f.d
+                if d > bd:
+                    bd, best = d, f
+            if best is not None:
+                best.out.append(i)
+
+    assign([i for i in range(len(P)) if i not in (i0, i1, i2, i3)], faces)
+    for _ in range(4 * len(P) + 16):
+        f0 = next((f for f in faces if f.alive and f.out), None)
+        if f0 is None:
+            break
+        p, pd = f0.out[0], -math.inf
+        for i in f0.out:
+            d = _dot(f0.n, P[i]) - f0.d
+            if d > pd:
+                pd, p = d, i
+        f0.vis = True
+        visible, seen_f, horizon = [f0], {id(f0)}, []
+        k = 0
+        while k < len(visible):
+            f = visible[k]
+            k += 1
+            for ei in range(3):
+                a, b = f.v[ei], f.v[(ei + 1) % 3]
+                nb = edges.get(b * K + a)
+                if nb is None or id(nb) in seen_f:
+                    if nb is not None and not nb.vis:
+                        horizon.append((a, b))
+                    continue
+                if _dot(nb.n, P[p]) - nb.d > eps:
+                    nb.vis = True
+                    seen_f.add(id(nb))
+                    visible.append(nb)
+                else:
+                    horizon.append((a, b))
+        orphans = []
+        for f in visible:
+            f.alive = False
+            for kk in range(3):
+                ek = f.v[kk] * K + f.v[(kk + 1) % 3]
+                if edges.get(ek) is f:
+                    del edges[ek]
+            orphans.extend(i for i in f.out if i != p)
+        keep = [(a, b) for a, b in horizon if a * K + b not in edges]
+        fresh = [make(a, b, p) for a, b in keep]
+        assign(orphans, fresh)
+        faces = [f for f in faces if f.alive] + fresh
+    # Coplanar triangles merge into polygons by a flood fill across shared edges; a plane keeps its first
+    # triangle's (n, d).
+    planes = []
+    for f in faces:
+        if f.group is not None:
+            continue
+        pl = {"n": f.n, "d": f.d, "ids": {}}
+        f.group = pl
+        stack = [f]
+        while stack:
+            g = stack.pop()
+            for v in g.v:
+                pl["ids"].setdefault(v, None)  # insertion-ordered set
+            for kk in range(3):
+                nb = edges.get(g.v[(kk + 1) % 3] * K + g.v[kk])
+                if (nb is not None and nb.group is None and _dot(nb.n, pl["n"]) > 0
+                        and all(abs(_dot(pl["n"], P[v]) - pl["d"]) <= 10 * eps for v in nb.v)):
+                    nb.group = pl
+                    stack.append(nb)
+        planes.append(pl)
     loops = []
-    for g in groups:
-        n = g["n"] / np.linalg.norm(g["n"])
-        ids = sorted(g["ids"])
-        c = P[ids].mean(0)
-        ref = np.array([1.0, 0, 0]) if abs(n[0]) < 0.9 else np.array([0, 1.0, 0])
-        u = np.cross(n, ref)
-        u /= np.linalg.norm(u)
-        w = np.cross(n, u)
-        pts = sorted(((i, float(np.dot(P[i] - c, u)), float(np.dot(P[i] - c, w))) for i in ids),
-                     key=lambda q: (q[1], q[2]))
+    for pl in planes:
+        ids = list(pl["ids"])
+        acc = [0.0, 0.0, 0.0]
+        for i in ids:
+            acc = _add(acc, P[i])
+        c = _scale(acc, 1.0 / len(ids))
+        n = pl["n"]
+        ref = [1.0, 0.0, 0.0] if abs(n[0]) < 0.9 else [0.0, 1.0, 0.0]
+        u = _cross(n, ref)
+        u = _scale(u, 1.0 / _hypot3(u))
+        w = _cross(n, u)
+        pts = sorted(((i, _dot(_sub(P[i], c), u), _dot(_sub(P[i], c), w)) for i in ids), key=lambda q: (q[1], q[2]))
 
         def turn(o, a, b):
             return (a[1] - o[1]) * (b[2] - o[2]) - (a[2] - o[2]) * (b[1] - o[1])
@@ -162,11 +312,11 @@ def convexHull(points):
 
         ring = half(pts) + half(list(reversed(pts)))
         if len(ring) >= 3:
-            loops.append({"n": [float(x) for x in n], "d": float(np.dot(n, P[ring[0][0]])), "ids": [q[0] for q in ring]})
+            loops.append({"n": n, "d": pl["d"], "ids": [q[0] for q in ring]})
     used = sorted({i for lp in loops for i in lp["ids"]})
     remap = {v: i for i, v in enumerate(used)}
-    out = {"verts": [list(map(float, P[v])) for v in used],
-           "faces": [{"n": lp["n"], "d": lp["d"], "v": [remap[i] for i in lp["ids"]]} for lp in loops]}
+    out = {"verts": [list(P[v]) for v in used],
+           "faces": [{"n": list(lp["n"]), "d": lp["d"], "v": [remap[i] for i in lp["ids"]]} for lp in loops]}
     _HULL_CACHE[key] = out
     return out
 

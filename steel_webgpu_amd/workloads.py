@@ -59,11 +59,13 @@ class PCG32:
 SEED = 0x57EE1000
 
 
-def _place(rng, lo, hi, radius, placed, tries=1000):
-    """Uniform position in the box [lo, hi], re-drawn while its bounding sphere overlaps a placed one or
-    reaches below the ground (z = 0)."""
+def _place(rng, lo, hi, radius, placed, tries=1000, axis_clear=0.0):
+    """Uniform position in the box [lo, hi], re-drawn while its bounding sphere overlaps a placed one, reaches
+    below the ground (z = 0), or comes within axis_clear of the vertical axis x = y = 0."""
     for _ in range(tries):
         p = [rng.uniform(lo[k], hi[k]) for k in range(3)]
+        if axis_clear and p[0] ** 2 + p[1] ** 2 < (axis_clear + radius) ** 2:
+            continue
         if p[2] >= radius and all((p[0] - q[0]) ** 2 + (p[1] - q[1]) ** 2 + (p[2] - q[2]) ** 2 >= (radius + r) ** 2 for q, r in placed):
             placed.append((p, radius))
             return p
@@ -152,7 +154,7 @@ def arm_hulls(seed=SEED + 1, nlinks=32, nhulls=256, fps=240, substeps=8, iterati
         b.addJointRevolute(parent, body, axis=axis, parentXform=[0, 0, prev_len if parent >= 0 else 0.0, 0, 0, 0, 1],
                            limitLower=-math.pi / 2, limitUpper=math.pi / 2)
         parent, prev_len, z = body, L, z + L
-    placed = [([0.0, 0.0, 0.5 * z], 0.25)]
+    placed = []
     for _ in range(nhulls):
         r = rng.uniform(0.04, 0.12)
         npts = min(8 + int(rng.uniform(0, 9)), 16)
@@ -160,7 +162,8 @@ def arm_hulls(seed=SEED + 1, nlinks=32, nhulls=256, fps=240, substeps=8, iterati
         for _ in range(npts):
             v = rng.unit_vec()
             pts.append([r * v[0], r * v[1], r * v[2]])
-        p = _place(rng, (-0.9, -0.9, 0.15), (0.9, 0.9, 2.5), r, placed)
+        # clear of the standing arm (capsules of radius 0.03 on the vertical axis) by 2 cm
+        p = _place(rng, (-0.9, -0.9, 0.15), (0.9, 0.9, 2.5), r, placed, axis_clear=0.05)
         body = b.addBody(xform=p + rng.unit_quat())
         b.addShapeConvexHull(body, vertices=pts, mu=0.6, density=1000.0)
     return b.finalize(solver=dict(substeps=substeps, iterations=iterations))

@@ -4,9 +4,10 @@ re-running the oracle's approach to it.
 
     python tests/golden/make_checkpoints.py
 
-Each checkpoint is tests/golden/checkpoints/<name>_f<frame>.npz holding the state arrays (poses, vel, art, anchor
-keys and values in the kernels' layout, the frame clock and stats words) and the frame number; the world itself is
-regenerated from its seeded workload function."""
+Each checkpoint is tests/golden/checkpoints/<name>_f<frame>.npz holding the packed world (header, statF, statU and the
+initial state: the packing computes hull planes with library float code whose last bits may differ between hosts, so
+the world travels with its state) and the state arrays (poses, vel, art, anchor keys and values in the kernels'
+layout, the frame clock and stats words) with the frame number."""
 import os
 import sys
 
@@ -37,8 +38,21 @@ def save(name):
     o.frames(frames)
     assert o.warm.size <= 1 or pw.h("W_WH") == 0, "warm-start tables are not checkpointed"
     np.savez_compressed(checkpoint_path(name), frame=np.int64(frames), poses=o.poses, vel=o.vel, art=o.art, akey=o.akey, aval=o.aval,
-                        clock=o.count, stats=o.stats)
+                        clock=o.count, stats=o.stats, **{"world_" + k: getattr(pw, k) for k in WORLD_FIELDS})
     print(f"{name}: frame {frames}, contacts max {int(o.stats[1])}, dense rows max {int(o.stats[7])}")
+
+
+WORLD_FIELDS = ("header", "statF", "statU", "poses0", "vel0", "art0")
+
+
+def world(name):
+    """The checkpoint's packed world (enough for Batch and the oracle)."""
+    z = np.load(checkpoint_path(name), allow_pickle=False)
+    pw = pack.PackedWorld()
+    for k in WORLD_FIELDS:
+        setattr(pw, k, z["world_" + k])
+    pw.NB, pw.NS, pw.NA, pw.NL, pw.ND = (pw.h(f) for f in ("W_NB", "W_NS", "W_NA", "W_NL", "W_ND"))
+    return pw
 
 
 def load_into(o, name):

@@ -478,7 +478,7 @@ def test_arm_among_hulls_from_a_settled_checkpoint_bitexact_with_dense_rows(sim)
     floor and on the arm): 20 more frames on the GPU and on the oracle match bit for bit, state, stats and anchors,
     and the arm's self-contacts run as dense rows in that window."""
     import make_checkpoints
-    pw = pack.pack_scene(workloads.arm_hulls())
+    pw = make_checkpoints.world("arm_hulls")
     o = OracleWorld(pw)
     frame = make_checkpoints.load_into(o, "arm_hulls")
     assert frame == 400

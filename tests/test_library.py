@@ -122,6 +122,68 @@ def test_convex_hull_faces_are_outward_ccw_planes():
         assert area @ n > 0  # counter-clockwise about the outward normal
 
 
+def test_convex_hull_of_a_tetrahedron_traced_by_hand():
+    """The reference quickhull (builder.js:61-173) on the corner tetrahedron, traced by hand: i0 = 0 (smallest x,
+    first), i1 = 1 (first farthest from it), i2 = 2 (first farthest from that line), i3 = 3; the initial faces are
+    (0,2,1), (0,1,3), (0,3,2) (the first and third flipped to face away from the centroid) and (1,2,3); each is its
+    own plane, keeping its triangle's (n, d); each loop is the monotone chain over (u, w) coordinates starting at
+    the smallest u."""
+    h = scene_mod.convexHull([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    assert h["verts"] == [[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]
+    s3 = 1 / np.sqrt(3)
+    want = [((0, 0, -1), 0.0, [2, 1, 0]), ((0, -1, 0), 0.0, [1, 3, 0]), ((-1, 0, 0), 0.0, [3, 2, 0]), ((s3, s3, s3), s3, [3, 1, 2])]
+    assert len(h["faces"]) == 4
+    for f, (n, d, v) in zip(h["faces"], want):
+        np.testing.assert_allclose(f["n"], n, atol=1e-15)
+        assert abs(f["d"] - d) < 1e-15 and f["v"] == v
+
+
+def test_convex_hull_of_a_cube_traced_by_hand():
+    """The unit cube in the builder's corner order (x, then y, then z): the tetrahedron (0, 7, 1, 2), then the
+    points 4, 3, 5, 6 grow the hull in that order, and the flood fill meets the faces' planes in the order -x, -y,
+    -z, +y, +z, +x; the -x face's loop is 3, 2, 0, 1 (traced through the monotone chain)."""
+    P = [[x, y, z] for x in (0.0, 1.0) for y in (0.0, 1.0) for z in (0.0, 1.0)]
+    h = scene_mod.convexHull(P)
+    assert h["verts"] == P
+    normals = [tuple(f["n"]) for f in h["faces"]]
+    assert normals == [(-1.0, 0.0, 0.0), (0.0, -1.0, 0.0), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0)]
+    assert [f["d"] for f in h["faces"]] == [0.0, 0.0, 0.0, 1.0, 1.0, 1.0]
+    assert h["faces"][0]["v"] == [3, 2, 0, 1]
+    V = np.array(P)
+    for f in h["faces"]:
+        ring = V[f["v"]]
+        assert len(ring) == 4 and np.allclose(ring @ np.array(f["n"]), f["d"])
+        area = sum(np.cross(ring[i] - ring[0], ring[i + 1] - ring[0]) for i in range(1, len(ring) - 1))
+        assert area @ np.array(f["n"]) == pytest.approx(2.0)  # twice the unit face area, counter-clockwise
+
+
+def test_convex_hull_invariants_on_random_clouds():
+    """Every input point is inside or on the hull, each face is a convex CCW polygon on its plane, and the
+    polyhedron is closed (V - E + F = 2, every edge shared by exactly two faces in opposite directions)."""
+    rng = np.random.default_rng(9)
+    for trial in range(20):
+        n = int(rng.integers(5, 40))
+        pts = rng.normal(size=(n, 3)) * rng.uniform(0.05, 2.0, size=3)
+        if trial % 3 == 0:
+            pts = np.round(pts * 4) / 4  # lattice points: coplanar faces to merge
+            if np.linalg.matrix_rank(pts - pts.mean(0)) < 3:
+                continue
+        h = scene_mod.convexHull(pts.tolist())
+        V = np.array(h["verts"])
+        edges = {}
+        for f in h["faces"]:
+            nrm, d = np.array(f["n"]), f["d"]
+            assert np.all(pts @ nrm - d <= 1e-9 * (1 + np.abs(pts).max()))
+            ring = f["v"]
+            for i in range(len(ring)):
+                a, b = ring[i], ring[(i + 1) % len(ring)]
+                assert (a, b) not in edges
+                edges[(a, b)] = True
+        for a, b in edges:
+            assert (b, a) in edges
+        assert len(V) - len(edges) // 2 + len(h["faces"]) == 2
+
+
 def test_scene_folder_round_trip(tmp_path):
     sc = workloads.boxes64(n=5)
     sc.save(str(tmp_path))
