@@ -686,6 +686,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     pw.dofMap = dofMap
     pw.qMap = qMap
     pw.A = A
+    pw.SF, pw.SU, pw.P = SF, SU, P
     pw.opts = o
     pw.solver = SV
     pw.fps = fps
