@@ -71,7 +71,7 @@ def _pack_request(args):
     i, nmax, hmax = args
     from steel_webgpu_amd import pack, workloads
     sc, h = workloads.serving_request(i, nmax=nmax, hmax=hmax)
-    pw = pack.pack_scene(sc, capacity_policy="compact")
+    pw = pack.pack_scene(sc, capacity_policy="scaled")
     pw.scene = None
     return pw, h
 
@@ -94,13 +94,93 @@ def serving(args, procs, log):
         arr = serve.poisson_arrivals(len(packed), rate, rng)
         reqs = [serve.Request(rid=i, arrival=arr[i], world=pw, horizon=h, lane=serve.lane_of(pw)) for i, (pw, h) in enumerate(packed)]
         srv = serve.Server(budget_bytes=int(args.budget_gb * 2**30), device=dev)
-        m = serve.run(reqs, srv, readback=())
+        m = serve.run(reqs, srv, readback=(), log=log)
         srv.destroy()
         m["rate"] = rate
         log(f"  rate {rate}/s: {m['world_steps_per_s']:.0f} world-steps/s, step latency p50 {m['step_latency_ms']['p50']:.2f} ms "
             f"p99 {m['step_latency_ms']['p99']:.2f} ms, queue wait p99 {m['queue_wait_ms']['p99']:.1f} ms, {m['wall_s']:.1f}s")
         results.append(m)
-    return results
+    # Plain per-world reference: sample requests stepped alone, one launch per kernel per substep.
+    frames, secs = 0, 0.0
+    for i in range(0, len(packed), max(1, len(packed) // args.plain_worlds)) if args.plain_worlds else []:
+        pw, _ = packed[i]
+        b1 = sim.Batch([pw], dev, block=serve.LANE_BLOCKS[serve.lane_of(pw)], graph=False)
+        b1.sync()
+        t1 = time.perf_counter()
+        b1.step_timed(20)
+        b1.sync()
+        secs += time.perf_counter() - t1
+        frames += 20
+        b1.destroy()
+    plain = frames / secs if secs else None
+    if plain:
+        log(f"  plain per-world over {frames // 20} sample requests x 20 frames: {plain:.1f} world-steps/s")
+    dev.close()
+    return results, plain
+
+
+PLAIN_CONFIGS = (
+    ("boxes64", "ground plane + 64 boxes, fps 240, 4 substeps", 500),
+    ("arm_hulls", "32-link revolute arm among 256 hulls in a walled bin, fps 240, 8 substeps", 1000),
+)
+
+
+def plain_worlds(args, log):
+    """The single-world configurations: each world stepped alone on the GPU over its whole run (one launch per
+    kernel per substep, HIP events per frame), and by the FP32 oracle on one thread over a bounded prefix of the
+    same run (frames in chunks of 25 until --cpu-seconds have passed).  The GPU state after that prefix is
+    compared with the oracle's bit for bit, and the oracle then takes one FP32 and one FP64 step from it."""
+    from steel_webgpu_amd import pack, workloads
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import OracleWorld, one_step_errors
+    cases = []
+    for name, desc, frames in PLAIN_CONFIGS:
+        pw = pack.pack_scene(getattr(workloads, name)(), capacity_policy="scaled")
+        o = OracleWorld(pw)
+        done, t0 = 0, time.perf_counter()
+        while done < frames and time.perf_counter() - t0 < args.cpu_seconds:
+            n = min(25, frames - done)
+            o.frames(n)
+            done += n
+        cpu_s = time.perf_counter() - t0
+        log(f"{name}: FP32 oracle, one thread: frames 0..{done - 1} in {cpu_s:.1f}s ({done / cpu_s:.1f} world-steps/s)")
+        cases.append((name, desc, frames, pw, o, done, cpu_s))
+    from steel_webgpu_amd import sim
+    dev = sim.Device(0)
+    out = []
+    from steel_webgpu_amd import serve
+    for name, desc, frames, pw, o, done, cpu_s in cases:
+        block = serve.LANE_BLOCKS[serve.lane_of(pw)]
+        b = sim.Batch([pw], dev, block=block, graph=False)
+        b.sync()
+        parts, wall, same = [], 0.0, None
+        cuts = sorted(set(range(0, frames, 100)) | {done, frames})
+        for f0, f1 in zip(cuts[:-1], cuts[1:]):
+            t0 = time.perf_counter()
+            parts.append(b.step_timed(f1 - f0))
+            b.sync()
+            wall += time.perf_counter() - t0
+            log(f"  {name} frames {f0}..{f1 - 1}: {np.mean(parts[-1]):.3f} ms/frame")
+            if f1 == done:
+                same = all(np.array_equal(b.read(0, q).view(np.uint32), getattr(o, q)[:b.read(0, q).size].view(np.uint32))
+                           for q in ("poses", "vel", "art"))
+        st, _ = b.stats_raw(0)
+        b.destroy()
+        fm = np.concatenate(parts)
+        err, walls = one_step_errors(o)
+        log(f"{name}: GPU plain {frames / wall:.1f} world-steps/s ({np.median(fm):.3f} ms/frame p50, {np.percentile(fm, 99):.3f} p99); "
+            f"block {block}; bit-exact with the oracle after frame {done - 1}: {same}; last substep {int(st[9])} pairs")
+        out.append({"workload": name, "description": desc, "frames": frames, "bodies": pw.NB, "shapes": pw.NS, "block": block,
+                    "gpu_world_steps_per_s": round(frames / wall, 2),
+                    "gpu_frame_ms": {"p50": round(float(np.median(fm)), 4), "p99": round(float(np.percentile(fm, 99)), 4)},
+                    "cpu_fp32_one_thread": {"world_steps_per_s": round(done / cpu_s, 2), "frames": f"0..{done - 1}",
+                                            "seconds": round(cpu_s, 2)},
+                    "parity": {"bitexact": bool(same), "after_frame": done - 1,
+                               "compared": "poses, velocities, articulation state, GPU vs FP32 oracle"},
+                    "fp64_one_step_error": {q: {"abs": float(f"{v['abs']:.3g}"), "rel": float(f"{v['rel']:.3g}")} for q, v in err.items()},
+                    "cpu_fp64_one_step_s": round(float(walls[1]), 4)})
+    dev.close()
+    return out
 
 
 def copy_calibration(nbytes=1 << 30, reps=4):
@@ -286,7 +366,8 @@ def main():
     ap.add_argument("--state-out", default=None, help="write every world's state at the window's start to this .npz and exit")
     ap.add_argument("--state-in", default=None, help="start the window from this .npz (written by --state-out) instead of advancing")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--workload", default="batch", choices=["batch", "serving"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "serving", "plain"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="plain: FP32 oracle time budget per configuration")
     ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
     ap.add_argument("--rates", type=float, nargs="+", default=[200.0, 1000.0, 5000.0], help="serving: Poisson rates (req/s)")
     ap.add_argument("--nmax", type=int, default=2000, help="serving: largest request body count")
@@ -304,8 +385,14 @@ def main():
         sys.exit(2)
     procs = cpu_threads()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.workload == "plain":
+        res = plain_worlds(args, log)
+        print(json.dumps({"metric": "plain single-world world-steps/s (FP32) + p50/p99 frame time", "unit": "world-steps/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded PCG32 scenes)",
+                          "per_config": res}))
+        return
     if args.workload == "serving":
-        res = serving(args, procs, log)
+        res, plain = serving(args, procs, log)
         tot = sum(r["world_steps"] for r in res)
         wall = sum(r["wall_s"] for r in res)
         print(json.dumps({
@@ -314,6 +401,8 @@ def main():
             "config": {"workload": "serving", "requests_per_rate": args.requests, "rates": args.rates, "nmax": args.nmax,
                        "hmax": args.hmax, "budget_gb": args.budget_gb, "fps": 240, "substeps": 4, "iterations": 12,
                        "lanes": "NB<100 / <600 / >=600 bodies, blocks 64/128/256"},
+            "plain_world_steps_per_s": round(plain, 2) if plain else None,
+            "served_vs_plain": {str(r["rate"]): round(r["world_steps_per_s"] / plain, 1) for r in res} if plain else None,
             "per_rate": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in res]}))
         return
     K = args.steps
@@ -370,18 +459,15 @@ def main():
         if start is not None:
             restore_states(batch, everyone, start)
 
-    # Autotuner: prepare/solve workgroup size by measured frame time over the window's first frames.
-    block = args.block
+    # The library's autotuner: workgroup size x narrowphase grid, timed over the window's first frames from its
+    # starting state (restored after each trial), cached per device.
+    block, tune = args.block, None
     if block == 0:
-        best = None
-        for bs in (64, 128, 256):
-            batch.configure(bs, graph)
-            to_start()
-            ms = float(np.sum(batch.step_timed(10)))
-            log(f"  autotune block {bs}: {ms / 10:.2f} ms/frame over frames {F0}..{F0 + 9}")
-            if best is None or ms < best[1]:
-                best = (bs, ms)
-        block = best[0]
+        to_start()
+        tune = batch.autotune(frames=10)
+        block = tune["block"]
+        log(f"  autotune: block {block}, narrowphase {tune['narrow_per_cu']} workgroups/CU "
+            f"({tune['trials']} schedules x 10 frames from frame {F0}; {tune['frame_ms']:.2f} ms/frame)")
     batch.configure(block, graph)
     CH = 100
     value = ms_per_step = None
@@ -475,7 +561,7 @@ def main():
         "data": "synthetic (seeded PCG32 scenes)",
         "config": {"workload": "quadruped_batch", "worlds": nw, "fps": 200, "substeps": S, "iterations": worlds[0].h("W_ITERS"),
                    "frames": window + (f" (frames 0..{F0 - 1} of the 1000-frame run advanced untimed)" if F0 else " from rest"),
-                   "block": block, "graph": graph},
+                   "block": block, "graph": graph, "autotune": tune},
         "plain_world_steps_per_s": round(plain, 2),
         "speedup_vs_plain": round(value / plain, 1),
         "latency_ms": {"p50": round(float(np.percentile(frame_ms, 50)), 4), "p99": round(float(np.percentile(frame_ms, 99)), 4),

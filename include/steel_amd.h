@@ -103,6 +103,23 @@ int steel_batch_buffer_size(steel_batch* b, uint32_t world, uint32_t buf, size_t
 int steel_batch_stats(steel_batch* b, uint32_t world, uint32_t out[16], double* gpu_ms);
 /* Launch options: workgroup size of prepare/solve (64, 128 or 256) and hipGraph capture on/off. */
 int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph);
+
+/* The launch schedule the autotuner picked for a batch. */
+typedef struct steel_tune_result {
+  uint32_t block;          /* prepare/solve workgroup size: 64, 128 or 256 */
+  uint32_t narrow_per_cu;  /* k_narrow workgroups (4 waves each) per compute unit: 2, 4 or 8 */
+  uint32_t from_cache;     /* 1: the device's cache held this batch's signature, nothing was measured */
+  uint32_t trials;         /* schedules measured */
+  float frame_ms;          /* the chosen schedule's mean frame time over the trial (0 when cached) */
+} steel_tune_result;
+#define STEEL_TUNE_REMEASURE 1u  /* measure even when the device's cache holds the batch's signature */
+/* Autotuner (no counterpart in the reference, whose dispatch sizes are fixed in sim.js:648-690): times each
+ * schedule (workgroup size x narrowphase grid) over `frames` frames from the batch's current state, restoring
+ * the state after every trial, and keeps the fastest; the batch's state is unchanged on return (the step's
+ * results do not depend on the schedule).  The choice is cached per device under the batch's signature
+ * (world count, body/shape/link totals and maxima, substeps); with STEEL_TUNE_CACHE naming a file, the cache
+ * is also read from and appended to that file, so later processes on the same device model skip the trials. */
+int steel_batch_autotune(steel_batch* b, uint32_t frames, uint32_t flags, steel_tune_result* out);
 /* Device bytes the batch holds (admission by HBM budget). */
 int steel_batch_device_bytes(steel_batch* b, size_t* bytes);
 /* Device bytes a batch of these worlds would hold, without creating it. */

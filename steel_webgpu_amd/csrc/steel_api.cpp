@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <cstdlib>
+#include <mutex>
 #include "steel_runtime.h"
 
 thread_local std::string g_err;
@@ -45,7 +47,7 @@ struct steel_batch {
   WorldDev* d_worlds = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  std::vector<uint32_t> headers;  // n * 128
+  std::vector<uint32_t> headers;  // n * STEEL_WORLD_HEADER_WORDS
 };
 
 extern "C" {
@@ -353,6 +355,132 @@ int steel_batch_stats(steel_batch* b, uint32_t world, uint32_t out[16], double* 
       *gpu_ms = ms;
     }
   }
+  return STEEL_OK;
+}
+
+// ---- Autotuner
+// The batch's signature: what decides which schedule is fastest (the worlds' sizes and the substep count).
+static std::string tune_signature(const steel_batch* b) {
+  uint64_t nb = 0, ns = 0, nl = 0, mb = 0, ms = 0, ml = 0;
+  for (uint32_t i = 0; i < b->n; i++) {
+    const uint32_t* h = &b->headers[(size_t)i * STEEL_WORLD_HEADER_WORDS];
+    nb += h[W_NB]; ns += h[W_NS]; nl += h[W_NL];
+    mb = std::max<uint64_t>(mb, h[W_NB]); ms = std::max<uint64_t>(ms, h[W_NS]); ml = std::max<uint64_t>(ml, h[W_NL]);
+  }
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "n%u_b%llu_%llu_s%llu_%llu_l%llu_%llu_S%u", b->n, (unsigned long long)nb, (unsigned long long)mb,
+                (unsigned long long)ns, (unsigned long long)ms, (unsigned long long)nl, (unsigned long long)ml, b->substeps);
+  return buf;
+}
+
+static std::string tune_device_key(const steel_device* d) {
+  char buf[160];
+  std::snprintf(buf, sizeof buf, "%s_%dcu", d->prop.gcnArchName, d->prop.multiProcessorCount);
+  for (char* c = buf; *c; c++) if (*c == ' ' || *c == ':') *c = '_';
+  return buf;
+}
+
+// Persistent cache (STEEL_TUNE_CACHE): one line per entry, "<device key> <signature> <block> <narrow per CU>".
+static std::mutex g_tune_file_mu;
+static void tune_file_load(steel_device* d) {
+  const char* path = std::getenv("STEEL_TUNE_CACHE");
+  if (!path || !*path || d->tune_file_loaded) return;
+  d->tune_file_loaded = true;
+  std::lock_guard<std::mutex> lk(g_tune_file_mu);
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  const std::string dk = tune_device_key(d);
+  char key[160], sig[256];
+  unsigned blk, npc;
+  while (std::fscanf(f, "%159s %255s %u %u", key, sig, &blk, &npc) == 4)
+    if (dk == key && (blk == 64 || blk == 128 || blk == 256) && (npc == 2 || npc == 4 || npc == 8)) d->tune_cache[sig] = {blk, npc};
+  std::fclose(f);
+}
+static void tune_file_append(const steel_device* d, const std::string& sig, uint32_t blk, uint32_t npc) {
+  const char* path = std::getenv("STEEL_TUNE_CACHE");
+  if (!path || !*path) return;
+  std::lock_guard<std::mutex> lk(g_tune_file_mu);
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;
+  std::fprintf(f, "%s %s %u %u\n", tune_device_key(d).c_str(), sig.c_str(), blk, npc);
+  std::fclose(f);
+}
+
+static void batch_set_schedule(steel_batch* b, uint32_t block, uint32_t narrow_groups) {
+  if (b->gexec && (block != b->block || narrow_groups != b->narrow_groups)) {
+    (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr;
+    (void)hipGraphDestroy(b->graph); b->graph = nullptr;
+  }
+  b->block = block;
+  b->narrow_groups = narrow_groups;
+}
+
+// The state a frame carries to the next (everything else is rebuilt every substep).
+static constexpr int K_STATE[] = {K_POSES, K_VEL, K_ART, K_AKEY, K_AVAL, K_CTR, K_STATS, K_WARM};
+
+int steel_batch_autotune(steel_batch* b, uint32_t frames, uint32_t flags, steel_tune_result* out) {
+  if (!b || !out) return fail(STEEL_BAD_ARGUMENT, "null argument");
+  if (frames == 0 || frames > 1000) return fail(STEEL_BAD_ARGUMENT, "trial frames must be 1..1000");
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  *out = steel_tune_result{};
+  const uint32_t cus = (uint32_t)b->dev->prop.multiProcessorCount;
+  const std::string sig = tune_signature(b);
+  tune_file_load(b->dev);
+  if (!(flags & STEEL_TUNE_REMEASURE)) {
+    auto it = b->dev->tune_cache.find(sig);
+    if (it != b->dev->tune_cache.end()) {
+      batch_set_schedule(b, it->second.first, cus * it->second.second);
+      *out = steel_tune_result{it->second.first, it->second.second, 1u, 0u, 0.0f};
+      return STEEL_OK;
+    }
+  }
+  // Snapshot of the state, restored after each trial and at the end.
+  size_t tot = 0;
+  for (int k : K_STATE) tot += b->total[k];
+  char* snap = nullptr;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (hipMalloc(&snap, tot) != hipSuccess) return fail(STEEL_OUT_OF_MEMORY, "autotune: no room for a %zu-byte state snapshot", tot);
+  auto copy_state = [&](bool save) -> hipError_t {
+    size_t o = 0;
+    for (int k : K_STATE) {
+      hipError_t e = save ? hipMemcpyAsync(snap + o, b->arena[k], b->total[k], hipMemcpyDeviceToDevice, b->stream)
+                          : hipMemcpyAsync(b->arena[k], snap + o, b->total[k], hipMemcpyDeviceToDevice, b->stream);
+      if (e != hipSuccess) return e;
+      o += b->total[k];
+    }
+    return hipStreamSynchronize(b->stream);
+  };
+  const uint32_t block0 = b->block, groups0 = b->narrow_groups;
+  int rc = STEEL_OK;
+  if (copy_state(true) != hipSuccess) rc = fail(STEEL_ERROR, "autotune: state snapshot failed");
+  uint32_t best_b = block0, best_g = groups0 / (cus ? cus : 1);
+  float best = 0.0f;
+  std::vector<float> fm(frames);
+  static const uint32_t blocks[] = {64, 128, 256}, per_cu[] = {2, 4, 8};
+  for (uint32_t bs : blocks)
+    for (uint32_t g : per_cu) {
+      if (rc) break;
+      batch_set_schedule(b, bs, cus * g);
+      // one untimed frame captures the graph and warms the caches; then the timed trial
+      rc = steel_batch_step(b, 1);
+      if (!rc) rc = steel_batch_step_timed(b, frames, fm.data());
+      if (!rc && copy_state(false) != hipSuccess) rc = fail(STEEL_ERROR, "autotune: state restore failed");
+      if (rc) break;
+      float ms = 0.0f;
+      for (float v : fm) ms += v;
+      ms /= (float)frames;
+      out->trials++;
+      if (best == 0.0f || ms < best) { best = ms; best_b = bs; best_g = g; }
+    }
+  if (!rc && copy_state(false) != hipSuccess) rc = fail(STEEL_ERROR, "autotune: state restore failed");
+  (void)hipFree(snap);
+  if (rc) { batch_set_schedule(b, block0, groups0); return rc; }
+  batch_set_schedule(b, best_b, cus * best_g);
+  b->dev->tune_cache[sig] = {best_b, best_g};
+  tune_file_append(b->dev, sig, best_b, best_g);
+  out->block = best_b;
+  out->narrow_per_cu = best_g;
+  out->frame_ms = best;
   return STEEL_OK;
 }
 

@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <map>
+#include <utility>
 #include "steel_dev.h"
 #include "steel_launch.h"
 #include "../../include/steel_amd.h"
@@ -117,6 +119,8 @@ struct steel_device {
   int ordinal;
   hipDeviceProp_t prop;
   unsigned char* noalloc_scratch = nullptr;  // this device's address of steel_noalloc.hip's scratch array
+  std::map<std::string, std::pair<uint32_t, uint32_t>> tune_cache;  // autotuner: signature -> (block, narrow per CU)
+  bool tune_file_loaded = false;
 };
 
 // Takes the device address of the no-allocation entry point's scratch array on the current device.

@@ -102,11 +102,24 @@ def capacities(NS, NB, NA, policy="reference"):
     shrinks each to the exact bound where it cannot (pairs <= NS(NS-1)/2, contacts <= 5 per pair).
     "compact" additionally bounds contacts by 16 per shape, for batches of many small worlds; a world
     that exceeds it reports the overflow in its stats.
+    "scaled" keeps the reference's sizes as a floor and grows them with the world (24 pairs, 32 contacts and 12
+    units per shape), for worlds whose resting contact graph outgrows the fixed sizes: 256 hulls settling in a
+    2 m bin with the default 0.1 m gap reach about 3000 touching body pairs, and the units past the reference's
+    2048 would get no rows, so their bodies sink through the floor (the reference reports that overflow in
+    stats word 3).
     """
     maxp = max(1, min(16384, NS * (NS - 1) // 2))
     maxc = max(1, min(8192, 5 * maxp))
     if policy == "compact":
         maxc = max(1, min(maxc, max(256, 16 * NS)))
+    if policy == "scaled":
+        maxp = max(1, min(max(16384, 24 * NS), NS * (NS - 1) // 2))
+        maxc = max(1, min(max(8192, 32 * NS), 5 * maxp))
+        maxu = min(max(2048, 12 * NS), maxc)
+        uh = min(max(4096, _next_pow2(2 * maxu)), maxc)
+        ah = _next_pow2(max(64, 2 * maxu))
+        maxdense = min(512, maxc) if NA > 0 else 1
+        return maxp, maxc, maxu, uh, ah, maxdense
     maxu = min(2048, maxc)
     uh = min(4096, maxc)
     ah = min(4096, _next_pow2(max(64, 2 * maxu)))

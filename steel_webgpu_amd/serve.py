@@ -169,8 +169,13 @@ def run(requests, server, readback=("poses",), clock=time.perf_counter, log=None
     finished = 0
     total = len(requests)
     t0 = clock()
+    next_log = 10.0
     while finished < total:
         now = clock() - t0
+        if log is not None and now >= next_log:
+            log(f"    {now:.0f}s: {finished} of {total} requests finished, {sum(len(m) for m in members)} worlds resident, "
+                f"{len(queue)} waiting, {steps} world-steps")
+            next_log += 10.0
         while pending and pending[0].arrival <= now:
             queue.append(pending.popleft())
         while queue:

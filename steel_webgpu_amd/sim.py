@@ -29,7 +29,7 @@ EXPORTS = ["steel_last_error", "steel_device_open", "steel_device_close", "steel
            "steel_batch_device_bytes", "steel_estimate_bytes", "steel_batch_step_timed", "steel_batch_profile_kernels",
            "steel_server_create", "steel_server_destroy", "steel_server_configure_lane", "steel_server_world_bytes",
            "steel_server_admit", "steel_server_release", "steel_server_step_lane", "steel_server_lane_poll", "steel_server_read", "steel_server_write",
-           "steel_server_info", "steel_world_step_noalloc"]
+           "steel_server_info", "steel_world_step_noalloc", "steel_batch_autotune"]
 
 
 class UnsupportedError(RuntimeError):
@@ -44,6 +44,15 @@ class WorldDeviceArrays(ctypes.Structure):
     """steel_world_device_arrays: device pointers of one world's static arrays and state."""
     _fields_ = [(n, ctypes.c_void_p) for n in ("header", "statF", "statU", "poses", "vel", "art", "anchor_keys", "anchor_vals",
                                                 "warm", "counters", "stats")]
+
+
+class TuneResult(ctypes.Structure):
+    """steel_tune_result: the schedule the autotuner picked."""
+    _fields_ = [("block", ctypes.c_uint32), ("narrow_per_cu", ctypes.c_uint32), ("from_cache", ctypes.c_uint32),
+                ("trials", ctypes.c_uint32), ("frame_ms", ctypes.c_float)]
+
+
+STEEL_TUNE_REMEASURE = 1
 
 
 class WorldDesc(ctypes.Structure):
@@ -95,6 +104,7 @@ def load_library(path=LIB_PATH):
         "steel_server_write": ([V, U32, U32, SZ, SZ, V], ctypes.c_int),
         "steel_server_info": ([V, ctypes.POINTER(SZ), ctypes.POINTER(SZ), ctypes.POINTER(U32)], ctypes.c_int),
         "steel_world_step_noalloc": ([V, ctypes.POINTER(WorldDesc), ctypes.POINTER(WorldDeviceArrays), U32, V], ctypes.c_int),
+        "steel_batch_autotune": ([V, U32, U32, ctypes.POINTER(TuneResult)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -182,6 +192,14 @@ class Batch:
 
     def configure(self, block=256, graph=True):
         _check(self.lib.steel_batch_configure(self.handle, block, 1 if graph else 0))
+
+    def autotune(self, frames=5, remeasure=False):
+        """steel_batch_autotune: the fastest (workgroup size, narrowphase grid) over `frames` frames from the current
+        state, which is restored afterwards; cached per device (and in the file STEEL_TUNE_CACHE names)."""
+        r = TuneResult()
+        _check(self.lib.steel_batch_autotune(self.handle, frames, STEEL_TUNE_REMEASURE if remeasure else 0, ctypes.byref(r)))
+        return {"block": r.block, "narrow_per_cu": r.narrow_per_cu, "from_cache": bool(r.from_cache), "trials": r.trials,
+                "frame_ms": round(float(r.frame_ms), 4)}
 
     def step(self, frames=1):
         _check(self.lib.steel_batch_step(self.handle, frames))

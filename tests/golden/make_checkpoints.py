@@ -23,7 +23,7 @@ from steel_webgpu_amd import pack, workloads  # noqa: E402
 
 # name -> (world, frames from rest)
 CHECKPOINTS = {
-    "arm_hulls": (lambda: pack.pack_scene(workloads.arm_hulls()), 400),
+    "arm_hulls": (lambda: pack.pack_scene(workloads.arm_hulls(), capacity_policy="scaled"), 400),
 }
 
 

@@ -74,3 +74,36 @@ def test_lane_takes_a_new_substep_count_once_empty():
         srv.read(wb, "poses")
     finally:
         srv.destroy()
+
+
+def test_autotune_leaves_the_state_and_the_trajectory_unchanged_and_caches_per_device():
+    """steel_batch_autotune measures every schedule from the batch's current state and restores it: the state right
+    after tuning equals the state before, and the tuned batch's next frames equal an untuned batch's bit for bit.
+    A second call with the same signature is answered from the device's cache without measuring."""
+    from steel_webgpu_amd import sim
+    worlds = [pack.pack_scene(workloads.quadruped_world(i), capacity_policy="compact") for i in (0, 3, 7)]
+    dev = sim.Device(0)
+    try:
+        a = sim.Batch(worlds, dev, block=64)
+        ref = sim.Batch(worlds, dev, block=64)
+        a.step(10)
+        ref.step(10)
+        before = [[a.read(i, n, dtype=np.uint8) for n in ("poses", "vel", "art", "anchor_keys", "anchor_vals", "counters", "stats")]
+                  for i in range(len(worlds))]
+        t = a.autotune(frames=2, remeasure=True)
+        assert t["trials"] == 9 and not t["from_cache"] and t["block"] in (64, 128, 256) and t["narrow_per_cu"] in (2, 4, 8)
+        after = [[a.read(i, n, dtype=np.uint8) for n in ("poses", "vel", "art", "anchor_keys", "anchor_vals", "counters", "stats")]
+                 for i in range(len(worlds))]
+        assert all(np.array_equal(x, y) for bx, by in zip(before, after) for x, y in zip(bx, by))
+        a.step(10)
+        ref.step(10)
+        for i in range(len(worlds)):
+            for n in ("poses", "vel", "art"):
+                assert np.array_equal(a.read(i, n, dtype=np.uint8), ref.read(i, n, dtype=np.uint8)), (i, n)
+        c = sim.Batch(worlds, dev, block=64)
+        t2 = c.autotune(frames=2)
+        assert t2["from_cache"] and t2["trials"] == 0 and (t2["block"], t2["narrow_per_cu"]) == (t["block"], t["narrow_per_cu"])
+        for b in (a, ref, c):
+            b.destroy()
+    finally:
+        dev.close()

@@ -96,6 +96,21 @@ def test_capacities_are_exact_bounds():
     assert c[1] == 400 and c[3] <= c[1]
 
 
+@pytest.mark.parametrize("ns", [2, 10, 65, 291, 700, 2000, 5600])
+def test_scaled_capacities_floor_at_the_reference_and_grow_with_the_world(ns):
+    """The scaled sizes never go below the reference's (where the world can reach them), grow with the shape
+    count, keep each hash table within the contact capacity and the anchor table a power of two."""
+    ref = pack.capacities(ns, ns, 1)
+    maxp, maxc, maxu, uh, ah, md = pack.capacities(ns, ns, 1, "scaled")
+    assert maxp >= ref[0] and maxc >= ref[1] and maxu >= ref[2] and uh >= ref[3] and ah >= ref[4] and md == ref[5]
+    assert maxp <= ns * (ns - 1) // 2 and maxc <= 5 * maxp and maxu <= maxc and uh <= maxc
+    assert ah & (ah - 1) == 0 and ah >= 2 * maxu
+    if ns >= 700:  # past the reference's fixed sizes
+        assert maxu == 12 * ns and maxc == 32 * ns and maxp == 24 * ns
+    pw = pack.pack_scene(workloads.arm_hulls(nlinks=4, nhulls=8), capacity_policy="scaled")
+    assert pw.h("W_MAXU") <= pw.h("W_MAXC") and pw.h("W_UH") <= pw.h("W_MAXC")
+
+
 def test_unsupported_options_raise():
     with pytest.raises(pack.UnsupportedError):
         pack.pack_scene(workloads.boxes64(n=4), opts={"solver": {"row_schedule": "contact_then_internal"}})

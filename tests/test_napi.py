@@ -77,7 +77,7 @@ def test_node_steps_two_worlds_bitexact_with_the_python_host(tmp_path):
     """Two packed worlds created, stepped (12 frames) and read back through the addon under Node match the same
     batch stepped through the ctypes host bit for bit (poses, velocities, articulation state)."""
     from steel_webgpu_amd import sim
-    pws = [pack.pack_scene(workloads.quadruped_world(2)), pack.pack_scene(workloads.arm_hulls(nlinks=6, nhulls=10))]
+    pws = [pack.pack_scene(workloads.quadruped_world(2)), pack.pack_scene(workloads.arm_hulls(nlinks=6, nhulls=10, fps=200, substeps=4))]
     for i, pw in enumerate(pws):
         for n, dt in (("header", np.uint32), ("statF", np.float32), ("statU", np.uint32), ("poses0", np.float32), ("vel0", np.float32),
                       ("art0", np.float32)):
