@@ -86,6 +86,10 @@ enum steel_world_field {
   /* enable_torsional_rolling_friction: articulated contacts solve (t1, t2, spin, roll1, roll2) in Steel's elliptic
      cone block (u32 flag), the row offset of its 5x5 response (u32), the creep speed (f32) */
   W_ANG_ART, W_EO, W_F_CREEP,
+  /* collision-filter format (u32): 0 the reference's NS x NS bitset at SU_FILTER (bit i * NS + j); 1 a sorted list per
+     shape, for worlds whose bitset would be large: offsets[NS + 1], then for each shape i the shapes j > i it must not
+     pair with, ascending */
+  W_FILT_LIST,
   W_FIELD_COUNT
 };
 

@@ -68,7 +68,7 @@ template <class T> struct World {
   uint32_t NB, NS, NH, NA, NL, ND, NQ, NDA, JM, MAXPATH, MAXLP, MAXP, MAXC, MAXU, UH, AH, MAXDENSE, NCOLW, NCOL, ROWS, PO, PATCH, EXTRA,
       FCOMB, RCOMB, REST, ITERS, DHDR, DPS;
   uint32_t SF_BODY, SF_SHAPE, SF_HAABB, SF_VERTS, SF_FPLANE, SF_LINK, SF_DOF;
-  uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER;
+  uint32_t SU_HULL, SU_FACE, SU_FV, SU_HEDGE, SU_EDGE, SU_ART, SU_FILTER, FILT_LIST;
   uint32_t P_BODY, P_SHAPE, P_AABB, P_PREV;
   uint32_t A_Q, A_QD, A_QDH, A_TGT, A_TGTV, A_LIM, A_DRV, A_JF, A_BF, A_FRIC, A_HINV, A_LINK, A_LG, A_RL, A_RLN, A_ULOAD, A_DENSE, A_VLG, A_AVL;
   uint32_t WH, NEST, MO, TORS, TO, VELITERS, A_SNAP, COMPL, CO, DRIVE_ROWS, FUSE_VLIM, VEL_DRIVE_ACTIVE, NLOOP, SF_LOOP, ANG, CTRL_KIND, CTRL_A, CTRL_B, SF_CTRL, FMODE, FMO, ANG_ART, EO;
@@ -101,9 +101,13 @@ template <class T> struct World {
   std::vector<Contact<T>> craw, contacts;
   std::vector<V4<T>> rows;      // MAXC*ROWS + 2*MAXC slot table
   // sched sections (sim.js:442-444), one vector each
-  std::vector<uint32_t> S_HKEY, S_HCNT, S_SLOT2U, S_CSLOT, S_CLOCAL, S_USIZE_TMP, S_UKEY, S_USIZE, S_USTART, S_CSTART, S_META, S_CORDER,
+  // Body-pair keys (b + 1, a + 1) as one 64-bit word: the order of the reference's (a + 1) | (b + 1) << 16
+  // (wgsl.js:1517-1524) for any body count.
+  std::vector<uint64_t> S_HKEY, S_UKEY, wkey;
+  std::vector<uint32_t> S_HCNT, S_SLOT2U, S_CSLOT, S_CLOCAL, S_USIZE_TMP, S_USIZE, S_USTART, S_CSTART, S_META, S_CORDER,
       S_RK0, S_RK1, S_RK2, S_KEPT, S_OSTART, S_LEAD, S_SLOT, S_ROWOF, S_RUNIT, S_PCOL, S_PORD, S_UTC, S_UTP, S_RCON, S_UWOFF, S_WALK;
-  std::vector<uint32_t> wkey, wcol, wmask, wcount, wlinkhit, wjsplit, uf, uff;
+  std::vector<uint64_t> wutp;   // coloring: each unit's largest (color + 1, walk index)
+  std::vector<uint32_t> wcol, wmask, wcount, wlinkhit, wjsplit, uf, uff;
   uint32_t wnu, wnwalk, ndc;
   std::vector<V4<T>> tw, imp;
   std::vector<T> aqd;
@@ -120,6 +124,7 @@ template <class T> struct World {
     SF_LINK = h[W_SF_LINK]; SF_DOF = h[W_SF_DOF];
     SU_HULL = h[W_SU_HULL]; SU_FACE = h[W_SU_FACE]; SU_FV = h[W_SU_FV]; SU_HEDGE = h[W_SU_HEDGE]; SU_EDGE = h[W_SU_EDGE]; SU_ART = h[W_SU_ART];
     SU_FILTER = h[W_SU_FILTER];
+    FILT_LIST = h[W_FILT_LIST];
     P_BODY = h[W_P_BODY]; P_SHAPE = h[W_P_SHAPE]; P_AABB = h[W_P_AABB]; P_PREV = h[W_P_PREV];
     A_Q = h[W_A_Q]; A_QD = h[W_A_QD]; A_QDH = h[W_A_QDH]; A_TGT = h[W_A_TGT]; A_TGTV = h[W_A_TGTV]; A_LIM = h[W_A_LIM]; A_DRV = h[W_A_DRV];
     A_JF = h[W_A_JF]; A_BF = h[W_A_BF]; A_FRIC = h[W_A_FRIC]; A_HINV = h[W_A_HINV]; A_LINK = h[W_A_LINK]; A_LG = h[W_A_LG]; A_RL = h[W_A_RL];
@@ -259,22 +264,43 @@ template <class T> struct World {
   }
 
   // ============================ broadphase (wgsl.js:154-177) ============================
+  // Worlds of more than 4096 shapes find the same pair set by a sweep over the shapes sorted by AABB min x (the
+  // candidates of shape i are the later shapes whose min x is at most i's max x); only the emission order differs.
   void broad() {
     pairs.clear();
     counters[0] = 0;
+    if (NS > 4096) {
+      std::vector<uint32_t> ord(NS);
+      for (uint32_t i = 0; i < NS; i++) ord[i] = i;
+      std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return poses[P_AABB + x * 2].x < poses[P_AABB + y * 2].x; });
+      for (uint32_t u = 0; u < NS; u++)
+        for (uint32_t v = u + 1; v < NS; v++) {
+          if (poses[P_AABB + ord[v] * 2].x > poses[P_AABB + ord[u] * 2 + 1].x) break;
+          broad_test(std::min(ord[u], ord[v]), std::max(ord[u], ord[v]));
+        }
+      return;
+    }
     for (uint32_t i = 0; i < NS; i++)
-      for (uint32_t j = i + 1; j < NS; j++) {
+      for (uint32_t j = i + 1; j < NS; j++) broad_test(i, j);
+  }
+  void broad_test(uint32_t i, uint32_t j) {
+      {
         int bi = ti(statF[SF_SHAPE + i * 7].x), bj = ti(statF[SF_SHAPE + j * 7].x);
-        if (bi == bj) continue;
-        if (bi < 0 && bj < 0) continue;
+        if (bi == bj) return;
+        if (bi < 0 && bj < 0) return;
         bool fi = bi < 0 || body_kind(bi) == 3 || (body_flags(bi) & 1) != 0;
         bool fj = bj < 0 || body_kind(bj) == 3 || (body_flags(bj) & 1) != 0;
-        if (fi && fj) continue;
-        uint32_t k = i * NS + j;
-        if ((statU[SU_FILTER + (k >> 5)] & (1u << (k & 31))) != 0) continue;
+        if (fi && fj) return;
+        if (FILT_LIST) {  // shape i's sorted list of filtered partners j > i
+          const uint32_t* L = statU + SU_FILTER + NS + 1;
+          if (std::binary_search(L + statU[SU_FILTER + i], L + statU[SU_FILTER + i + 1], j)) return;
+        } else {
+          uint64_t k = (uint64_t)i * NS + j;
+          if ((statU[SU_FILTER + (uint32_t)(k >> 5)] & (1u << (uint32_t)(k & 31))) != 0) return;
+        }
         V3<T> lo_i = xyz(poses[P_AABB + i * 2]), hi_i = xyz(poses[P_AABB + i * 2 + 1]);
         V3<T> lo_j = xyz(poses[P_AABB + j * 2]), hi_j = xyz(poses[P_AABB + j * 2 + 1]);
-        if (lo_i.x > hi_j.x || lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.x > hi_i.x || lo_j.y > hi_i.y || lo_j.z > hi_i.z) continue;
+        if (lo_i.x > hi_j.x || lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.x > hi_i.x || lo_j.y > hi_i.y || lo_j.z > hi_i.z) return;
         uint32_t slot = counters[0]++;
         if (slot < MAXP) { pairs.push_back(i); pairs.push_back(j); }
       }
@@ -1140,21 +1166,22 @@ template <class T> struct World {
   // ============================ contact sort (wgsl.js:3895-3980) ============================
   void csort() {
     uint32_t nc = std::min(counters[1], MAXC);
-    struct K { uint32_t k[4]; };
+    // (b + 1, a + 1) in place of the 16-bit packed pair (a + 1) | (b + 1) << 16: the same order, for any body count
+    struct K { uint32_t k[5]; };
     std::vector<K> keys(nc);
     for (uint32_t c = 0; c < nc; c++) {
       const Contact<T>& r = craw[c];
-      uint32_t pair = (uint32_t)(ti(r.q[2].x) + 1) | ((uint32_t)(ti(r.q[2].y) + 1) << 16);
+      uint32_t pa = (uint32_t)(ti(r.q[2].x) + 1), pb = (uint32_t)(ti(r.q[2].y) + 1);
       uint32_t feat = tu(r.q[2].z) | (tu(r.q[2].w) << 16);
       uint32_t bits = hash_u32(hash_u32(hash_u32(fbits(r.q[0].x)) ^ fbits(r.q[0].y)) ^ fbits(r.q[0].z)) ^ fbits(r.q[0].w);
-      keys[c] = {{pair, feat, bits, c}};
+      keys[c] = {{pb, pa, feat, bits, c}};
     }
     std::sort(keys.begin(), keys.end(), [](const K& a, const K& b) {
-      for (int i = 0; i < 4; i++) if (a.k[i] != b.k[i]) return a.k[i] < b.k[i];
+      for (int i = 0; i < 5; i++) if (a.k[i] != b.k[i]) return a.k[i] < b.k[i];
       return false;
     });
     contacts.resize(nc);
-    for (uint32_t i = 0; i < nc; i++) contacts[i] = craw[keys[i].k[3]];
+    for (uint32_t i = 0; i < nc; i++) contacts[i] = craw[keys[i].k[4]];
   }
 
   // ============================ articulation (artwgsl.js) ============================
@@ -1633,9 +1660,12 @@ template <class T> struct World {
     if (body_kind(b) == 2) { uint32_t o = lo((uint32_t)body_link(b)) + L_V; out[0] = ld3(o + 3); out[1] = ld3(o); }
   }
   T row_weight(T phi) const { if (phi > T(0) || reg <= T(0)) return T(1); return T(1) / (T(1) + reg); }
-  uint32_t contact_pair(uint32_t c) const {
+  static uint64_t pair_key(int a, int b) { return ((uint64_t)(uint32_t)(b + 1) << 32) | (uint32_t)(a + 1); }
+  static int key_a(uint64_t k) { return (int)(uint32_t)k - 1; }
+  static int key_b(uint64_t k) { return (int)(uint32_t)(k >> 32) - 1; }
+  uint64_t contact_pair(uint32_t c) const {
     const V4<T>& m = contacts[c].q[2];
-    return (uint32_t)(ti(m.x) + 1) | ((uint32_t)(ti(m.y) + 1) << 16);
+    return pair_key(ti(m.x), ti(m.y));
   }
   uint32_t unit_of(uint32_t p, uint32_t nu) const {
     uint32_t lo_ = 0, hi = nu;
@@ -1715,8 +1745,8 @@ template <class T> struct World {
     uint32_t st = S_USTART[sp];
     uint32_t n = std::min(S_USIZE[sp], MAXC - std::min(st, MAXC));
     if (n == 0) return;
-    uint32_t pair = S_UKEY[sp];
-    int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+    uint64_t pair = S_UKEY[sp];
+    int a = key_a(pair), b = key_b(pair);
     uint32_t L = st * ROWS;
     bool art_u = (a >= 0 && body_kind(a) >= 2) || (b >= 0 && body_kind(b) >= 2);
     uint32_t seed = st;
@@ -1747,8 +1777,10 @@ template <class T> struct World {
     uint32_t st = S_USTART[sp];
     uint32_t n = std::min(S_USIZE[sp], MAXC - std::min(st, MAXC));
     if (n == 0) return;
-    uint32_t pair = S_UKEY[sp];
-    int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+    uint64_t key = S_UKEY[sp];
+    int a = key_a(key), b = key_b(key);
+    // Anchor-table pair id: the reference's (a + 1) | (b + 1) << 16, taken modulo 2^32 past 65535 bodies.
+    uint32_t pair = (uint32_t)(a + 1) | ((uint32_t)(b + 1) << 16);
     V3<T> ca{0, 0, 0}, cb{0, 0, 0};
     V4<T> qa{0, 0, 0, 1};
     if (a >= 0) { ca = xyz(poses[P_BODY + (uint32_t)a * 2]); qa = poses[P_BODY + (uint32_t)a * 2 + 1]; }
@@ -2033,7 +2065,7 @@ template <class T> struct World {
   void prepare() {
     uint32_t ROWSN = ROWS;
     // Phase 0
-    std::fill(S_HKEY.begin(), S_HKEY.end(), 0u);
+    std::fill(S_HKEY.begin(), S_HKEY.end(), 0ull);
     std::fill(S_HCNT.begin(), S_HCNT.end(), 0u);
     if (PATCH) for (uint32_t i = 0; i < AH; i++) akey[(1 - parity) * AH + i] = 0;
     for (uint32_t i = 0; i < WH; i++) wtab[(1 - parity) * WH + i].key = 0;
@@ -2135,7 +2167,7 @@ template <class T> struct World {
     for (uint32_t c = 0; c < nc; c++) {
       const V4<T>& m = contacts[c].q[2];
       int a = ti(m.x), b = ti(m.y);
-      uint32_t pk = contact_pair(c);
+      uint64_t pk = contact_pair(c);
       uint32_t cls = 0xfffffffeu;  // no rows: both nodes asleep (or static)
       if (!pair_asleep(a, b)) {
         if (same_art(a, b)) {
@@ -2169,7 +2201,7 @@ template <class T> struct World {
     // Phase 2
     uint32_t u = 0, woff = 0;
     for (uint32_t h = 0; h < UH; h++) {
-      uint32_t key = S_HKEY[h];
+      uint64_t key = S_HKEY[h];
       if (key == 0) continue;
       if (u < MAXU) { wkey[u] = key; S_USIZE_TMP[u] = S_HCNT[h]; }
       S_SLOT2U[h] = u;
@@ -2408,7 +2440,7 @@ template <class T> struct World {
     for (uint32_t c = 0; c < nc; c++) S_ROWOF[c] = 0xffffffffu;
     std::fill(wmask.begin(), wmask.end(), 0u);
     std::fill(wcount.begin(), wcount.end(), 0u);
-    for (uint32_t sp = 0; sp < nu; sp++) { S_UTC[sp] = 0; S_UTP[sp] = 0; }
+    for (uint32_t sp = 0; sp < nu; sp++) { S_UTC[sp] = 0; wutp[sp] = 0; }
     for (uint32_t row = 0; row < totp; row++) {
       uint32_t sp = unit_of(row, nu);
       uint32_t c = S_KEPT[S_OSTART[sp] + row - S_USTART[sp]];
@@ -2421,27 +2453,27 @@ template <class T> struct World {
       for (uint32_t i = 0; i < n; i++) {
         uint32_t wc = S_WALK[base + i];
         uint32_t row = wc < MAXC ? S_ROWOF[std::min(wc, MAXC - 1)] : 0xffffffffu;
-        wkey[i] = row < totp ? S_UKEY[S_RUNIT[std::min(row, MAXC - 1)]] : 0xffffffffu;
+        wkey[i] = row < totp ? S_UKEY[S_RUNIT[std::min(row, MAXC - 1)]] : ~0ull;
       }
       for (uint32_t i = 0; i < n; i++) {
-        uint32_t key = wkey[i];
-        if (key != 0xffffffffu) wcol[i] = first_free_color((int)(key & 0xffffu) - 1, (int)(key >> 16) - 1);
+        uint64_t key = wkey[i];
+        if (key != ~0ull) wcol[i] = first_free_color(key_a(key), key_b(key));
       }
       for (uint32_t i = 0; i < n; i++) {
-        if (wkey[i] == 0xffffffffu) continue;
+        if (wkey[i] == ~0ull) continue;
         uint32_t c = S_WALK[base + i];
         uint32_t row = S_ROWOF[c];
         uint32_t col = wcol[i];
         S_PCOL[row] = col;
         wcount[col]++;
-        uint32_t& utp = S_UTP[S_RUNIT[row]];
-        utp = std::max(utp, ((col + 1) << 16) | c);
+        uint64_t& utp = wutp[S_RUNIT[row]];
+        utp = std::max(utp, ((uint64_t)(col + 1) << 32) | c);
       }
     }
     for (uint32_t sp = 0; sp < nu; sp++) {
-      uint32_t packed = S_UTP[sp];
-      S_UTC[sp] = packed >> 16;
-      S_UTP[sp] = packed != 0 ? S_ROWOF[packed & 0xffffu] : 0xffffffffu;
+      uint64_t packed = wutp[sp];
+      S_UTC[sp] = (uint32_t)(packed >> 32);
+      S_UTP[sp] = packed != 0 ? S_ROWOF[(uint32_t)packed] : 0xffffffffu;
     }
     uint32_t acc = 0, ncol = 0;
     for (uint32_t c = 0; c <= NCOL; c++) {
@@ -2670,8 +2702,8 @@ template <class T> struct World {
     for (uint32_t b = 0; b < NB; b++) { st3(A_SNAP + b * 8, xyz(tw[b * 2])); st3(A_SNAP + b * 8 + 4, xyz(tw[b * 2 + 1])); }
     for (uint32_t i = 0; i < ND; i++) art[A_SNAP + NB * 8 + i] = aqd[i];
     for (uint32_t us = 0; us < nu; us++) {
-      uint32_t key = S_UKEY[us];
-      int a = (int)(key & 0xffffu) - 1, b = (int)(key >> 16) - 1;
+      uint64_t key = S_UKEY[us];
+      int a = key_a(key), b = key_b(key);
       V3<T> ta[2], tb[2];
       load_tw(a, ta); load_tw(b, tb);
       uint32_t s0 = S_USTART[us];
@@ -3572,8 +3604,9 @@ template <class T> struct World {
     uint32_t nslot = std::min(S_CSTART[NCOL + 1], MAXC);
     std::vector<uint32_t> slot(MAXC * 4, 0);
     for (uint32_t u = 0; u < nslot; u++) {
-      uint32_t row = S_PORD[u], us = S_RUNIT[row], key = S_UKEY[us];
-      slot[u * 4] = row; slot[u * 4 + 1] = us; slot[u * 4 + 2] = (uint32_t)((int)(key & 0xffffu) - 1); slot[u * 4 + 3] = (uint32_t)((int)(key >> 16) - 1);
+      uint32_t row = S_PORD[u], us = S_RUNIT[row];
+      uint64_t key = S_UKEY[us];
+      slot[u * 4] = row; slot[u * 4 + 1] = us; slot[u * 4 + 2] = (uint32_t)key_a(key); slot[u * 4 + 3] = (uint32_t)key_b(key);
     }
     for (uint32_t a = 0; a < NA; a++) art_twists(a);
     dense_phase(0, 0);
@@ -3642,8 +3675,9 @@ template <class T> struct World {
       for (uint32_t i = 0; i < nc; i++) {
         uint32_t o = i * ROWS;
         if (rows[o + PO].w != T(2)) continue;
-        uint32_t pair = get_u32(rows[o + PO + 3].w);
-        int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+        uint32_t pair = get_u32(rows[o + PO + 3].w);  // anchor-table pair id; the bodies from the row's unit
+        uint64_t ukey = S_UKEY[S_RUNIT[i]];
+        int a = key_a(ukey), b = key_b(ukey);
         V3<T> fra = xyz(rows[o + PO]), frb = xyz(rows[o + PO + 1]);
         uint32_t k = tu(rows[o + PO + 1].w);
         V4<T> ex = rows[o + PO + 2];
@@ -3720,11 +3754,11 @@ template <class T> struct World {
     pairs.reserve(2 * MAXP);
     craw.reserve(MAXC);
     rows.assign((size_t)MAXC * ROWS + 2 * MAXC, V4<T>{0, 0, 0, 0});
-    auto z = [](std::vector<uint32_t>& v, size_t n) { v.assign(n, 0u); };
+    auto z = [](auto& v, size_t n) { v.assign(n, 0u); };
     z(S_HKEY, UH); z(S_HCNT, UH); z(S_SLOT2U, UH); z(S_CSLOT, MAXC); z(S_CLOCAL, MAXC); z(S_USIZE_TMP, MAXU); z(S_UKEY, MAXU);
     z(S_USIZE, MAXU); z(S_USTART, MAXU + 1); z(S_CSTART, NCOL + 2); z(S_META, 8); z(S_CORDER, MAXC); z(S_RK0, MAXC); z(S_RK1, MAXC);
     z(S_RK2, MAXC); z(S_KEPT, MAXC); z(S_OSTART, MAXU); z(S_LEAD, MAXC); z(S_SLOT, (size_t)MAXC * 26); z(S_ROWOF, MAXC); z(S_RUNIT, MAXC);
-    z(S_PCOL, MAXC); z(S_PORD, MAXC); z(S_UTC, MAXU); z(S_UTP, MAXU); z(S_RCON, MAXC); z(S_UWOFF, UH); z(S_WALK, MAXC);
+    z(S_PCOL, MAXC); z(S_PORD, MAXC); z(S_UTC, MAXU); z(S_UTP, MAXU); z(wutp, MAXU); z(S_RCON, MAXC); z(S_UWOFF, UH); z(S_WALK, MAXC);
     z(wkey, MAXU); z(wcol, MAXU); z(wmask, (size_t)std::max(NB, 1u) * NCOLW); z(wcount, NCOL + 2); z(wlinkhit, std::max(NB, 1u));
     z(wjsplit, std::max(NL, 1u));
     z(uf, NB + NA); z(uff, NB + NA);
@@ -3812,7 +3846,10 @@ static int run_substeps(const uint32_t* header, const float* statF, const uint32
         dbg->rowinfo[r * 4] = tu(m.z) | (tu(m.w) << 16);
         dbg->rowinfo[r * 4 + 1] = w.S_PCOL[r];
         dbg->rowinfo[r * 4 + 2] = w.S_RUNIT[r];
-        dbg->rowinfo[r * 4 + 3] = w.S_UKEY[w.S_RUNIT[r]];
+        {  // the reference's 16-bit packing (a + 1) | (b + 1) << 16 (worlds of fewer than 65536 bodies)
+          uint64_t k = w.S_UKEY[w.S_RUNIT[r]];
+          dbg->rowinfo[r * 4 + 3] = (uint32_t)(w.key_a(k) + 1) | ((uint32_t)(w.key_b(k) + 1) << 16);
+        }
       }
       for (int i = 0; i < 8; i++) dbg->meta[i] = w.S_META[i];
     }

@@ -94,7 +94,7 @@ int steel_estimate_bytes(uint32_t nworlds, const steel_world_desc* worlds, size_
   size_t tot = 0;
   for (uint32_t i = 0; i < nworlds; i++) {
     Sizes s;
-    int rc = world_sizes(worlds[i].header, s);
+    int rc = world_sizes(worlds[i].header, s, worlds[i].statF);
     if (rc) return rc;
     for (int k = 0; k < K_COUNT; k++) tot += align256(s.b[k]);
   }
@@ -153,7 +153,7 @@ int steel_batch_create(steel_device* dev, uint32_t nworlds, const steel_world_de
     if (!w.header || !w.statF || !w.statU || !w.poses0 || !w.vel0 || !w.art0) { delete b; return fail(STEEL_BAD_ARGUMENT, "world %u: null array", i); }
     if (w.header[W_SUBSTEPS] != b->substeps) { delete b; return fail(STEEL_BAD_ARGUMENT, "world %u: all worlds of a batch step the same substep count", i); }
     Sizes s;
-    int rc = world_sizes(w.header, s);
+    int rc = world_sizes(w.header, s, w.statF);
     if (rc) { delete b; return rc; }
     for (int k = 0; k < K_COUNT; k++) { b->off[k][i] = b->total[k]; b->size[k][i] = s.b[k]; b->total[k] += align256(s.b[k]); }
     std::memcpy(&b->headers[(size_t)i * STEEL_WORLD_HEADER_WORDS], w.header, STEEL_WORLD_HEADER_WORDS * 4);
@@ -193,7 +193,7 @@ int steel_batch_create(steel_device* dev, uint32_t nworlds, const steel_world_de
   for (uint32_t i = 0; i < nworlds; i++) {
     char* base[K_COUNT];
     for (int k = 0; k < K_COUNT; k++) base[k] = b->arena[k] + b->off[k][i];
-    wd[i] = world_dev_at(base);
+    wd[i] = world_dev_at(base, b->size[K_MIRROR][i] > 0);
   }
   b->narrow_groups = steel_narrow_groups((uint32_t)dev->prop.multiProcessorCount);
   if (hipMalloc(&b->pair_offs, sizeof(uint32_t) * (nworlds + 1)) != hipSuccess ||

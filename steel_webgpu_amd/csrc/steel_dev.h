@@ -27,11 +27,12 @@ struct WorldDev {
   float4* imp;             // [2*NB] accumulated link impulses
   float* aqd;              // [ND] solver joint velocities
   uint32_t* warm;          // contact warm start: keys [2*WH] (two parities), then values [2*WH*4] (float bits)
+  float4* mscr;            // k_solve's link records and mirror in global memory, for a world too large for its LDS (else null)
 };
 
 // Offsets of the schedule tables inside WorldDev::sched (uint32 words), from the world's capacities.
 struct SchedLayout {
-  uint32_t HKEY, HCNT, CSLOT, CLOCAL, UKEY, USIZE, USTART, CSTART, META, CORDER, RK0, RK1, RK2, KEPT, OSTART, LEAD, SLOT, ROWOF,
+  uint32_t HKEY, UKEY, UTPW, HCNT, CSLOT, CLOCAL, USIZE, USTART, CSTART, META, CORDER, RK0, RK1, RK2, KEPT, OSTART, LEAD, SLOT, ROWOF,
       RUNIT, PCOL, PORD, UTC, UTP, RCON, UWOFF, WALK, HEADS, DSLOT, WCOL, WMASK, WCOUNT, LINKHIT, JSPLIT, SLOTT, NPAIRL, UF, UFF, WWIN,
       WSLOT, FPX, TOTAL;
 };
@@ -40,11 +41,12 @@ __host__ __device__ inline SchedLayout sched_layout(uint32_t MAXC, uint32_t MAXU
                                                     uint32_t NL, uint32_t WH, uint32_t MAXDENSE) {
   SchedLayout s;
   uint32_t o = 0;
-  s.HKEY = o; o += UH;
+  s.HKEY = o; o += 2 * UH;                                 // run heads' body-pair keys (uint64, pair_key)
+  s.UKEY = o; o += 2 * MAXU;                               // units' body-pair keys (uint64)
+  s.UTPW = o; o += 2 * MAXU;                               // coloring: each unit's largest (color + 1, walk index) (uint64)
   s.HCNT = o; o += UH;
   s.CSLOT = o; o += MAXC;
   s.CLOCAL = o; o += MAXC;
-  s.UKEY = o; o += MAXU;
   s.USIZE = o; o += MAXU;
   s.USTART = o; o += MAXU + 1;
   s.CSTART = o; o += NCOL + 2;

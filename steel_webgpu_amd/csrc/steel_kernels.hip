@@ -161,8 +161,19 @@ DEV uint32_t shape_tag(const W& w, uint32_t s) {
 DEV bool pair_allowed(const W& w, uint32_t i, uint32_t j, uint32_t tag_i, uint32_t tag_j) {
   if ((tag_i & 0x7fffffffu) == (tag_j & 0x7fffffffu)) return false;
   if ((tag_i & tag_j) >> 31) return false;
-  uint32_t k = i * w.NS() + j;
-  return (w.su(w.h(W_SU_FILTER) + (k >> 5)) & (1u << (k & 31))) == 0;
+  const uint32_t F = w.h(W_SU_FILTER);
+  if (w.h(W_FILT_LIST)) {  // shape i's sorted list of filtered partners j > i
+    uint32_t lo_ = w.su(F + i), hi_ = w.su(F + i + 1);
+    const uint32_t L = F + w.NS() + 1;
+    while (lo_ < hi_) {
+      const uint32_t m = (lo_ + hi_) >> 1, v = w.su(L + m);
+      if (v == j) return false;
+      if (v < j) lo_ = m + 1; else hi_ = m;
+    }
+    return true;
+  }
+  const uint64_t k = (uint64_t)i * w.NS() + j;  // past 65536 shapes the bit index needs 64 bits
+  return (w.su(F + (uint32_t)(k >> 5)) & (1u << (uint32_t)(k & 31))) == 0;
 }
 
 // Waves per SIMD each kernel is compiled for (a VGPR budget: 512 / waves).  Every phase of a world is a
@@ -360,16 +371,29 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_tie = 0;
   const uint32_t nc = min(wd.ctr[1], w.MAXC());
+  // Key (body pair, feature): one 64-bit radix key while body ids + 1 fit 16 bits (the reference's packing);
+  // past that, the feature pass then a stable pass over the 64-bit pair key.
+  const bool wide = w.NB() > 0xffffu;
+  auto feat_of = [&](uint32_t c) { const float4 m = wd.craw[c * 6 + 2]; return tu(m.z) | (tu(m.w) << 16); };
+  auto pair_of = [&](uint32_t c) { const float4 m = wd.craw[c * 6 + 2]; return pair_key(ti(m.x), ti(m.y)); };
   for (uint32_t c = tid; c < nc; c += BS) {
     float4 m = wd.craw[c * 6 + 2];
-    uint32_t pair = (uint32_t)(ti(m.x) + 1) | ((uint32_t)(ti(m.y) + 1) << 16);
-    uint32_t feat = tu(m.z) | (tu(m.w) << 16);
-    wd.skey[c] = ((uint64_t)pair << 32) | feat;
+    const uint32_t feat = tu(m.z) | (tu(m.w) << 16);
+    wd.skey[c] = wide ? (uint64_t)feat : ((uint64_t)((uint32_t)(ti(m.x) + 1) | ((uint32_t)(ti(m.y) + 1) << 16)) << 32) | feat;
     wd.sval[c] = c;
   }
   __syncthreads();
   block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), nc, s_hist, s_cnt, s_red);
-  for (uint32_t i = tid; i + 1 < nc; i += BS) if (wd.skey[i] == wd.skey[i + 1]) s_tie = 1;
+  if (wide) {
+    for (uint32_t i = tid; i < nc; i += BS) wd.skey[i] = pair_of(wd.sval[i]);
+    __syncthreads();
+    block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), nc, s_hist, s_cnt, s_red);
+  }
+  // equal (pair, feature) of sorted positions i and j
+  auto same = [&](uint32_t i, uint32_t j) {
+    return wd.skey[i] == wd.skey[j] && (!wide || feat_of(wd.sval[i]) == feat_of(wd.sval[j]));
+  };
+  for (uint32_t i = tid; i + 1 < nc; i += BS) if (same(i, i + 1)) s_tie = 1;
   __syncthreads();
   if (s_tie && tid == 0) {
     auto bits_of = [&](uint32_t c) {
@@ -379,9 +403,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     for (uint32_t i = 1; i < nc; i++) {
       uint64_t k = wd.skey[i];
       uint32_t v = wd.sval[i];
-      uint32_t bv = bits_of(v);
+      uint32_t bv = bits_of(v), fv = wide ? feat_of(v) : 0u;
       int j = (int)i - 1;
-      while (j >= 0 && wd.skey[j] == k && (bits_of(wd.sval[j]) > bv || (bits_of(wd.sval[j]) == bv && wd.sval[j] > v))) {
+      while (j >= 0 && wd.skey[j] == k && (!wide || feat_of(wd.sval[j]) == fv) &&
+             (bits_of(wd.sval[j]) > bv || (bits_of(wd.sval[j]) == bv && wd.sval[j] > v))) {
         wd.skey[j + 1] = wd.skey[j]; wd.sval[j + 1] = wd.sval[j]; j--;
       }
       wd.skey[j + 1] = k; wd.sval[j + 1] = v;
@@ -401,7 +426,7 @@ struct Prep {
   W w;
   uint32_t parity;
   DEV float4 con(uint32_t c, uint32_t q) const { return w.d.contacts[c * 6 + q]; }
-  DEV uint32_t contact_pair(uint32_t c) const { float4 m = con(c, 2); return (uint32_t)(ti(m.x) + 1) | ((uint32_t)(ti(m.y) + 1) << 16); }
+  DEV uint64_t contact_pair(uint32_t c) const { float4 m = con(c, 2); return pair_key(ti(m.x), ti(m.y)); }
   DEV uint32_t unit_of(uint32_t p, uint32_t nu) const {
     const uint32_t* US = w.sch(w.S.USTART);
     uint32_t lo_ = 0, hi = nu;
@@ -510,8 +535,8 @@ struct Prep {
     const uint32_t st = w.sch(w.S.USTART)[sp];
     const uint32_t n = min(w.sch(w.S.USIZE)[sp], MAXC - min(st, MAXC));
     if (n == 0) return;
-    const uint32_t pair = w.sch(w.S.UKEY)[sp];
-    const int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+    const uint64_t pair = w.sch64(w.S.UKEY)[sp];
+    const int a = key_a(pair), b = key_b(pair);
     const bool art_u = (a >= 0 && w.body_kind(a) >= 2) || (b >= 0 && w.body_kind(b) >= 2);
     uint32_t seed = st;
     for (uint32_t i = st; i < st + n; i++) if (row(i, 1).w < row(seed, 1).w) seed = i;
@@ -543,8 +568,10 @@ struct Prep {
     uint32_t st = US[sp];
     uint32_t n = min(w.sch(w.S.USIZE)[sp], MAXC - min(st, MAXC));
     if (n == 0) return;
-    uint32_t pair = w.sch(w.S.UKEY)[sp];
-    int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+    const uint64_t key = w.sch64(w.S.UKEY)[sp];
+    int a = key_a(key), b = key_b(key);
+    // Anchor-table pair id: the reference's (a + 1) | (b + 1) << 16, taken modulo 2^32 past 65535 bodies.
+    const uint32_t pair = (uint32_t)(a + 1) | ((uint32_t)(b + 1) << 16);
     f3 ca = mk3(0, 0, 0), cb = mk3(0, 0, 0);
     q4 qa = mk4(0, 0, 0, 1);
     if (a >= 0) { ca = w.pose_c(a); qa = w.pose_q(a); }
@@ -1011,7 +1038,8 @@ template <int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t cap) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ uint32_t s_w[BS / 64];
-  __shared__ uint32_t s_wr[PREP_CH], s_wk[PREP_CH];  // coloring: row and body-pair key of a chunk of walk entries
+  __shared__ uint32_t s_wr[PREP_CH];                 // coloring: row of each walk entry of a chunk
+  __shared__ uint64_t s_wk[PREP_CH];                 // coloring: body-pair key of each walk entry of a chunk
   __shared__ uint32_t s_wc[PREP_WC];                  // coloring: per-color counts and offsets when NCOL + 2 fits
   __shared__ uint32_t s_chg;                          // sleep islands: a contact joined two islands this pass
   extern __shared__ uint32_t s_pdyn[];                 // coloring: color masks, then first-pending slots
@@ -1038,7 +1066,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   uint32_t* sched = wd.sched;
   float* art_ = wd.art;
   // ---- Phase 0
-  for (uint32_t i = tid; i < UH; i += BS) { sched[S.HKEY + i] = 0; sched[S.HCNT + i] = 0; }
+  for (uint32_t i = tid; i < UH; i += BS) { w.sch64(S.HKEY)[i] = 0; sched[S.HCNT + i] = 0; }
   if (PATCH) for (uint32_t i = tid; i < AH; i += BS) wd.akey[(1 - P.parity) * AH + i] = 0;
   if (const uint32_t WH = w.h(W_WH)) {  // the table this substep's solve writes, and its winners
     for (uint32_t i = tid; i < WH; i += BS) wd.warm[(1 - P.parity) * WH + i] = 0;
@@ -1237,7 +1265,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   for (uint32_t c = tid; c < nc; c += BS) {
     float4 m = P.con(c, 2);
     int a = ti(m.x), b = ti(m.y);
-    uint32_t pk = P.contact_pair(c);
+    const uint64_t pk = P.contact_pair(c);
     uint32_t cls = 0xfffffffeu, head = 0, dense = 0;  // no rows: both nodes asleep (or static)
     if (!w.pair_asleep(a, b)) {
       if (w.same_art(a, b)) {
@@ -1263,10 +1291,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     uint32_t unit = sched[S.HEADS + c] + cls;  // inclusive count of heads
     uint32_t h = unit - 1;
     if (h >= UH) { sched[S.CSLOT + c] = 0xfffffffeu; continue; }
-    uint32_t pk = P.contact_pair(c);
+    const uint64_t pk = P.contact_pair(c);
     uint32_t run0 = c;
     while (run0 > 0 && P.contact_pair(run0 - 1) == pk) run0--;
-    if (c == run0) sched[S.HKEY + h] = pk;
+    if (c == run0) w.sch64(S.HKEY)[h] = pk;
     atomicAdd(&sched[S.HCNT + h], 1u);
     sched[S.CSLOT + c] = h;
     sched[S.CLOCAL + c] = c - run0;
@@ -1282,7 +1310,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   __syncthreads();
   const uint32_t nwalk = block_scan_array<BS>(sched + S.UWOFF, nh, s_w);
   const uint32_t nu = min(nh, MAXU);
-  for (uint32_t h = tid; h < nu; h += BS) { sched[S.UKEY + h] = sched[S.HKEY + h]; sched[S.USIZE + h] = sched[S.HCNT + h]; }
+  for (uint32_t h = tid; h < nu; h += BS) { w.sch64(S.UKEY)[h] = w.sch64(S.HKEY)[h]; sched[S.USIZE + h] = sched[S.HCNT + h]; }
   for (uint32_t l = tid; l < NL; l += BS) {
     if (sched[S.LINKHIT + (uint32_t)w.link_body(l)] == 0) continue;
     int j = (int)l;
@@ -1399,7 +1427,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   // ---- Phase 6c: first-fit coloring in walk order (serial by definition, wgsl.js:2282-2363)
   const uint32_t totp = total;
   for (uint32_t c = tid; c < nc; c += BS) sched[S.ROWOF + c] = 0xffffffffu;
-  for (uint32_t sp = tid; sp < nu; sp += BS) { sched[S.UTC + sp] = 0; sched[S.UTP + sp] = 0; }
+  for (uint32_t sp = tid; sp < nu; sp += BS) { sched[S.UTC + sp] = 0; w.sch64(S.UTPW)[sp] = 0; }
   __syncthreads();
   for (uint32_t r = tid; r < totp; r += BS) {
     uint32_t sp = P.unit_of(r, nu);
@@ -1430,7 +1458,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       uint32_t wc = sched[S.WALK + c0 + i];
       uint32_t r = wc < MAXC ? sched[S.ROWOF + min(wc, MAXC - 1)] : 0xffffffffu;
       s_wr[i] = r;
-      s_wk[i] = r < totp ? sched[S.UKEY + sched[S.RUNIT + min(r, MAXC - 1)]] : 0u;
+      s_wk[i] = r < totp ? w.sch64(S.UKEY)[sched[S.RUNIT + min(r, MAXC - 1)]] : 0ull;
     }
     __syncthreads();
     if (mask_lds) {
@@ -1440,8 +1468,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
           const uint32_t i = i0 + lane;
           const uint32_t r = i < n ? s_wr[i] : 0xffffffffu;
           bool pending = r < totp;
-          const uint32_t key = pending ? s_wk[i] : 0u;
-          const int a = (int)(key & 0xffffu) - 1, b = (int)(key >> 16) - 1;
+          const uint64_t key = pending ? s_wk[i] : 0ull;
+          const int a = key_a(key), b = key_b(key);
           while (__ballot(pending) != 0) {
             if (pending) { if (a >= 0) OWN[a] = 64u; if (b >= 0) OWN[b] = 64u; }
             wave_sync();
@@ -1459,8 +1487,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       for (uint32_t i = 0; i < n; i++) {
         uint32_t r = s_wr[i];
         if (r >= totp) continue;
-        uint32_t key = s_wk[i];
-        sched[S.PCOL + r] = Prep::first_free_color(sched + S.WMASK, NCOLW, NCOL, (int)(key & 0xffffu) - 1, (int)(key >> 16) - 1);
+        const uint64_t key = s_wk[i];
+        sched[S.PCOL + r] = Prep::first_free_color(sched + S.WMASK, NCOLW, NCOL, key_a(key), key_b(key));
       }
     }
   }
@@ -1471,13 +1499,13 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     if (r >= totp) continue;
     uint32_t col = sched[S.PCOL + r];
     atomicAdd(&WC[col], 1u);
-    atomicMax(&sched[S.UTP + sched[S.RUNIT + r]], ((col + 1) << 16) | wc);
+    atomicMax((unsigned long long*)(w.sch64(S.UTPW) + sched[S.RUNIT + r]), ((unsigned long long)(col + 1) << 32) | wc);
   }
   __syncthreads();
   for (uint32_t sp = tid; sp < nu; sp += BS) {
-    uint32_t packed = sched[S.UTP + sp];
-    sched[S.UTC + sp] = packed >> 16;
-    sched[S.UTP + sp] = packed != 0 ? sched[S.ROWOF + (packed & 0xffffu)] : 0xffffffffu;
+    const uint64_t packed = w.sch64(S.UTPW)[sp];
+    sched[S.UTC + sp] = (uint32_t)(packed >> 32);
+    sched[S.UTP + sp] = packed != 0 ? sched[S.ROWOF + (uint32_t)packed] : 0xffffffffu;
   }
   if (tid == 0) {
     uint32_t acc = 0, ncol = 0;
@@ -3024,8 +3052,8 @@ __device__ __noinline__ void velocity_pass(const Solver& sv, uint32_t ncol, uint
   for (uint32_t b = tid; b < NB; b += BS) { w.st3(SN + b * 8, xyz(sv.tw[b * 2])); w.st3(SN + b * 8 + 4, xyz(sv.tw[b * 2 + 1])); }
   for (uint32_t i = tid; i < ND; i += BS) art_[SN + NB * 8 + i] = sv.laqd[i];
   for (uint32_t us = tid; us < nu; us += BS) {
-    const uint32_t key = sched[S.UKEY + us];
-    const int a = (int)(key & 0xffffu) - 1, b = (int)(key >> 16) - 1;
+    const uint64_t key = *(const uint64_t*)(sched + S.UKEY + 2 * us);
+    const int a = key_a(key), b = key_b(key);
     f3 ta[2], tb[2];
     sv.load_tw(a, ta); sv.load_tw(b, tb);
     const uint32_t s0 = sched[S.USTART + us];
@@ -3052,7 +3080,9 @@ __device__ __noinline__ void velocity_pass(const Solver& sv, uint32_t ncol, uint
   for (uint32_t it = 0; it < VI; it++) sweep<BS>(vs, 1, ncol, nu, nd);
 }
 
-template <int BS, bool TWL>
+// MGL: the launch holds a world whose link records and mirror live in global memory (WorldDev::mscr); every world of
+// the launch then runs the generic-pointer sweep, the others keeping their mirror in LDS.
+template <int BS, bool TWL, bool MGL = false>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_SOLVE))) k_solve(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ uint32_t s_w[BS / 64];
@@ -3076,7 +3106,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   // Dynamic LDS (steel_solve_lds_bytes of the launch's largest world): link records, joint velocities,
   // with TWL the body twists and impulses, then the link mirror.
   extern __shared__ float4 s_dyn4[];
-  char* lds = (char*)s_dyn4;
+  char* lds = (MGL && wd.mscr) ? (char*)wd.mscr : (char*)s_dyn4;
   sv.lmeta = (uint4*)lds;
   lds += (size_t)NL * 16;
   sv.ltree = (uint4*)lds;
@@ -3171,19 +3201,20 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const uint32_t nslot = min(sched[S.CSTART + NCOL + 1], MAXC);
   uint4* slots = (uint4*)(sched + S.SLOTT);
   for (uint32_t u = tid; u < nslot; u += BS) {
-    uint32_t r = sched[S.PORD + u], us = sched[S.RUNIT + r], key = sched[S.UKEY + us];
-    slots[u] = make_uint4(r, us, (uint32_t)((int)(key & 0xffffu) - 1), (uint32_t)((int)(key >> 16) - 1));
+    const uint32_t r = sched[S.PORD + u], us = sched[S.RUNIT + r];
+    const uint64_t key = w.sch64(S.UKEY)[us];
+    slots[u] = make_uint4(r, us, (uint32_t)key_a(key), (uint32_t)key_b(key));
   }
   __syncthreads();
   for (uint32_t a = tid >> 6; a < NA; a += BS / 64) sv.art_twists(a, tid & 63);
   __syncthreads();
   dense_phase<BS>(sv, 0, nd);
-  sweep<BS, true, TWL>(sv, 0, ncol, nu, nd);
+  sweep<BS, !MGL, TWL>(sv, 0, ncol, nu, nd);
   const uint32_t iters = w.h(W_ITERS);
   f2 kb = mk2(0.0f, 0.0f);
   for (uint32_t it = 0; it < iters; it++) {
     if (NEST) kb = momentum_step<BS>(sv, ncol, it, kb);
-    sweep<BS, true, TWL>(sv, 1, ncol, nu, nd);
+    sweep<BS, !MGL, TWL>(sv, 1, ncol, nu, nd);
   }
   if (w.h(W_VELITERS)) velocity_pass<BS>(sv, ncol, nu, nd);
   // Mutable mirror fields back to the link blocks (p, dv, v, u, tau; S, U, D^-1, COM are unchanged).
@@ -3260,8 +3291,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     uint32_t ab = (1 - sv.parity) * AH;
     for (uint32_t i = tid; i < nrows; i += BS) {
       if (sv.row(i, PO).w != 2.0f) continue;
-      uint32_t pair = fbits(sv.row(i, PO + 3).w);
-      int a = (int)(pair & 0xffffu) - 1, b = (int)(pair >> 16) - 1;
+      const uint32_t pair = fbits(sv.row(i, PO + 3).w);  // anchor-table pair id; the bodies from the row's unit
+      const uint64_t ukey = w.sch64(S.UKEY)[sched[S.RUNIT + i]];
+      const int a = key_a(ukey), b = key_b(ukey);
       f3 fra = xyz(sv.row(i, PO)), frb = xyz(sv.row(i, PO + 1));
       uint32_t k = tu(sv.row(i, PO + 1).w);
       float4 ex = sv.row(i, PO + 2);
@@ -3355,7 +3387,11 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
     else hipLaunchKernelGGL((k_prepare<256>), grid, dim3(256), pb, stream, worlds, nworlds, pw);
   }
   if (marks) (void)hipEventRecord(marks[2], stream);
-  if (stages & STAGE_SOLVE) {
+  if ((stages & STAGE_SOLVE) && sl.mgl) {
+    if (block == 64) hipLaunchKernelGGL((k_solve<64, false, true>), grid, dim3(64), solve_lds, stream, worlds, nworlds);
+    else if (block == 128) hipLaunchKernelGGL((k_solve<128, false, true>), grid, dim3(128), solve_lds, stream, worlds, nworlds);
+    else hipLaunchKernelGGL((k_solve<256, false, true>), grid, dim3(256), solve_lds, stream, worlds, nworlds);
+  } else if (stages & STAGE_SOLVE) {
     if (block == 64) {
       if (twl) hipLaunchKernelGGL((k_solve<64, true>), grid, dim3(64), solve_lds, stream, worlds, nworlds);
       else hipLaunchKernelGGL((k_solve<64, false>), grid, dim3(64), solve_lds, stream, worlds, nworlds);

@@ -36,18 +36,23 @@ static constexpr uint32_t STEEL_PREP_LDS_MAX_WORDS = 16384;
 // in LDS; twists go to LDS when that still lets four waves per SIMD stay resident (160 KB of LDS per CU
 // shared by 16 waves: 16 / (block / 64) workgroups).  k_broad: the largest world's shape count, capped.
 // k_prepare: the largest world's coloring words, capped.
+// A world whose k_solve LDS (without the twists) exceeds STEEL_SOLVE_LDS_MAX keeps its link records and mirror in
+// global memory (WorldDev::mscr, solve_lds_bytes of it); a launch holding one runs the solve's global-mirror
+// instantiation (mgl), in which every world solves through the generic-pointer passes.
 struct LaunchLds {
   size_t notw = 0, tw = 0;
   uint32_t ns = 0, cw = 0;
+  bool mgl = false;
   void add(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, uint32_t NS, uint32_t NCOLW, bool nest) {
-    notw = notw > solve_lds_bytes(NL, ND, mfloats, NB, false, nest) ? notw : solve_lds_bytes(NL, ND, mfloats, NB, false, nest);
-    tw = tw > solve_lds_bytes(NL, ND, mfloats, NB, true, nest) ? tw : solve_lds_bytes(NL, ND, mfloats, NB, true, nest);
     ns = ns > NS ? ns : NS;
     uint32_t c = (NB > 0 ? NB : 1) * (NCOLW + 1);
     cw = cw > c ? cw : c;
+    if (solve_lds_bytes(NL, ND, mfloats, NB, false, nest) > STEEL_SOLVE_LDS_MAX) { mgl = true; return; }
+    notw = notw > solve_lds_bytes(NL, ND, mfloats, NB, false, nest) ? notw : solve_lds_bytes(NL, ND, mfloats, NB, false, nest);
+    tw = tw > solve_lds_bytes(NL, ND, mfloats, NB, true, nest) ? tw : solve_lds_bytes(NL, ND, mfloats, NB, true, nest);
   }
   static size_t budget(uint32_t block) { return (size_t)160 * 1024 / (16 / (block / 64)) - 1024; }
-  bool tw_lds(uint32_t block) const { return tw <= budget(block); }
+  bool tw_lds(uint32_t block) const { return !mgl && tw <= budget(block); }
   size_t bytes(uint32_t block) const { return tw_lds(block) ? tw : notw; }
   uint32_t broad_cap() const { return ns < STEEL_BROAD_LDS_MAX ? ns : STEEL_BROAD_LDS_MAX; }
   size_t broad_bytes() const { return (size_t)(broad_cap() > 0 ? broad_cap() : 1) * 36 + 16; }

@@ -82,7 +82,7 @@ extern "C" int steel_world_step_noalloc(steel_device* dev, const steel_world_des
     return fail(STEEL_BAD_ARGUMENT, "the world needs %zu bytes of step scratch; steel_world_step_noalloc holds %zu", L.total,
                 (size_t)STEEL_NOALLOC_SCRATCH_BYTES);
   LaunchLds sl;
-  rc = world_solve_lds(host->header, host->statF, sl);
+  rc = world_solve_lds(host->header, host->statF, sl, false);
   if (rc) return rc;
   if (frames == 0) return STEEL_OK;
   HIPCHK(hipSetDevice(dev->ordinal));

@@ -24,11 +24,13 @@ int fail(int code, const char* fmt, ...);
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 enum Kind { K_HDR, K_STATF, K_STATU, K_POSES, K_VEL, K_ART, K_AKEY, K_AVAL, K_CTR, K_STATS, K_PAIRS, K_CRAW, K_CONTACTS, K_ROWS, K_SCHED,
-            K_SKEY, K_SVAL, K_TW, K_IMP, K_AQD, K_POSES0, K_VEL0, K_ART0, K_WARM, K_COUNT };
+            K_SKEY, K_SVAL, K_TW, K_IMP, K_AQD, K_POSES0, K_VEL0, K_ART0, K_WARM, K_MIRROR, K_COUNT };
 
 struct Sizes { size_t b[K_COUNT]; };
 
-inline int world_sizes(const uint32_t* h, Sizes& s) {
+inline uint32_t world_mirror_floats(const uint32_t* h, const float* statF);
+// statF (when given) sizes k_solve's global link mirror of a world too large for its LDS (K_MIRROR).
+inline int world_sizes(const uint32_t* h, Sizes& s, const float* statF = nullptr) {
   uint32_t NB = h[W_NB], NS = h[W_NS], ND = h[W_ND], MAXC = h[W_MAXC], MAXP = h[W_MAXP], AH = h[W_AH], ROWS = h[W_ROWS];
   if (MAXC < NS || MAXC == 0) return fail(STEEL_BAD_ARGUMENT, "world capacity MAXC=%u is below its shape count %u", MAXC, NS);
   if (h[W_NDA] > 64 || h[W_MAXLP] > 64) return fail(STEEL_BAD_ARGUMENT, "articulations of more than 64 dofs or links are not supported");
@@ -59,6 +61,11 @@ inline int world_sizes(const uint32_t* h, Sizes& s) {
   s.b[K_VEL0] = s.b[K_VEL];
   s.b[K_ART0] = s.b[K_ART];
   s.b[K_WARM] = std::max((size_t)h[W_WH] * 10 * 4, (size_t)4);
+  s.b[K_MIRROR] = 0;
+  if (statF) {
+    size_t mb = solve_lds_bytes(h[W_NL], h[W_ND], world_mirror_floats(h, statF), NB, false, h[W_NEST] != 0);
+    if (mb > STEEL_SOLVE_LDS_MAX) s.b[K_MIRROR] = mb;
+  }
   return STEEL_OK;
 }
 
@@ -69,8 +76,8 @@ inline int buf_kind(uint32_t buf) {
   return buf < STEEL_BUF_COUNT ? map[buf] : -1;
 }
 
-// Pointers of one world's buffers, given the base address of each kind's slice.
-inline WorldDev world_dev_at(char* const* base) {
+// Pointers of one world's buffers, given the base address of each kind's slice (mirror: the world has a K_MIRROR slice).
+inline WorldDev world_dev_at(char* const* base, bool mirror = false) {
   WorldDev x;
   x.hdr = (const uint32_t*)base[K_HDR];
   x.statF = (const float4*)base[K_STATF];
@@ -93,6 +100,7 @@ inline WorldDev world_dev_at(char* const* base) {
   x.imp = (float4*)base[K_IMP];
   x.aqd = (float*)base[K_AQD];
   x.warm = (uint32_t*)base[K_WARM];
+  x.mscr = mirror ? (float4*)base[K_MIRROR] : nullptr;
   return x;
 }
 
@@ -103,10 +111,10 @@ inline uint32_t world_mirror_floats(const uint32_t* h, const float* statF) {
   for (uint32_t l = 0; l < h[W_NL]; l++) t += mirror_floats((uint32_t)statF[((size_t)h[W_SF_LINK] + l * 6 + 1) * 4 + 3]);
   return t;
 }
-// Adds the world to sl; refuses worlds whose mirror does not fit k_solve's LDS.
-inline int world_solve_lds(const uint32_t* h, const float* statF, LaunchLds& sl) {
+// Adds the world to sl; with global_mirror false, refuses worlds whose mirror does not fit k_solve's LDS.
+inline int world_solve_lds(const uint32_t* h, const float* statF, LaunchLds& sl, bool global_mirror = true) {
   uint32_t mf = world_mirror_floats(h, statF);
-  if (solve_lds_bytes(h[W_NL], h[W_ND], mf, h[W_NB], false, h[W_NEST] != 0) > STEEL_SOLVE_LDS_MAX)
+  if (!global_mirror && solve_lds_bytes(h[W_NL], h[W_ND], mf, h[W_NB], false, h[W_NEST] != 0) > STEEL_SOLVE_LDS_MAX)
     return fail(STEEL_UNSUPPORTED, "a world of %u articulation links exceeds the solve kernel's LDS link mirror", h[W_NL]);
   sl.add(h[W_NL], h[W_ND], mf, h[W_NB], h[W_NS], h[W_NCOLW], h[W_NEST] != 0);
   return STEEL_OK;

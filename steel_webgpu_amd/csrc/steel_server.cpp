@@ -186,7 +186,7 @@ int steel_server_admit(steel_server* s, const steel_world_desc* w, uint32_t lane
   int rc = world_sizes(w->header, sz);
   if (rc) return rc;
   LaunchLds probe;
-  rc = world_solve_lds(w->header, w->statF, probe);
+  rc = world_solve_lds(w->header, w->statF, probe, false);
   if (rc) return rc;
   ServerWorld W;
   W.lane = lane;

@@ -36,6 +36,12 @@ DEV SVd crf(SVd v, SVd f) { return mksv(cross(v.a, f.a) + cross(v.l, f.l), cross
 DEV SVd mshift(SVd m, f3 r) { return mksv(m.a, m.l + cross(m.a, r)); }
 DEV SVd fshift(SVd f, f3 r) { return mksv(f.a + cross(r, f.l), f.l); }
 
+// Body-pair key of a contact or unit: (b + 1, a + 1) as one 64-bit word, so keys order by b, then a, as the
+// reference's 16-bit packing (a + 1) | (b + 1) << 16 does (wgsl.js:1517-1524, 3906-3912), for any body count.
+DEV uint64_t pair_key(int a, int b) { return ((uint64_t)(uint32_t)(b + 1) << 32) | (uint32_t)(a + 1); }
+DEV int key_a(uint64_t k) { return (int)(uint32_t)k - 1; }
+DEV int key_b(uint64_t k) { return (int)(uint32_t)(k >> 32) - 1; }
+
 struct W {
   const uint32_t* H;  // header copy in LDS
   WorldDev d;
@@ -61,6 +67,7 @@ struct W {
   DEV float4 sf(uint32_t i) const { return d.statF[i]; }
   DEV uint32_t su(uint32_t i) const { return d.statU[i]; }
   DEV uint32_t* sch(uint32_t off) const { return d.sched + off; }
+  DEV uint64_t* sch64(uint32_t off) const { return (uint64_t*)(d.sched + off); }  // even offsets (HKEY, UKEY, UTPW)
 
   // ---- body records
   DEV uint32_t bo(int b) const { return H[W_SF_BODY] + (uint32_t)b * 8; }

@@ -127,13 +127,17 @@ def capacities(NS, NB, NA, policy="reference"):
     return maxp, maxc, maxu, uh, ah, maxdense
 
 
+# Shape count above which a world's listed collision filters are kept as sorted lists (W_FILT_LIST).
+FILT_LIST_SHAPES = 16384
+
 # Contact warm-start table slots per parity (sim.js:396 WH).
 WARM_SLOTS = 16384
 
 
 def sched_layout(MAXC, MAXU, UH, NCOL, NB, NCOLW, NL, WH=0, MAXDENSE=0):
     """Word offsets of the kernels' schedule tables (mirror of sched_layout in csrc/steel_dev.h)."""
-    names = [("HKEY", UH), ("HCNT", UH), ("CSLOT", MAXC), ("CLOCAL", MAXC), ("UKEY", MAXU), ("USIZE", MAXU), ("USTART", MAXU + 1),
+    names = [("HKEY", 2 * UH), ("UKEY", 2 * MAXU), ("UTPW", 2 * MAXU), ("HCNT", UH), ("CSLOT", MAXC), ("CLOCAL", MAXC), ("USIZE", MAXU),
+             ("USTART", MAXU + 1),
              ("CSTART", NCOL + 2), ("META", 8), ("CORDER", MAXC), ("RK0", MAXC), ("RK1", MAXC), ("RK2", MAXC), ("KEPT", MAXC),
              ("OSTART", MAXU), ("LEAD", MAXC), ("SLOT", MAXC * 26), ("ROWOF", MAXC), ("RUNIT", MAXC), ("PCOL", MAXC), ("PORD", MAXC),
              ("UTC", MAXU), ("UTP", MAXU), ("RCON", MAXC), ("UWOFF", UH), ("WALK", MAXC), ("HEADS", MAXC + 1), ("DSLOT", MAXC + 1),
@@ -325,44 +329,57 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     SU["SU_EDGE"] = uoff; uoff += nEdges * 4
     SU["SU_ART"] = uoff; uoff += max(NA, 1) * 4
     SU["SU_ACON"] = uoff; uoff += max(NA, 1)
-    SU["SU_FILTER"] = uoff; uoff += (NS * NS + 31) // 32
-    nstatu = uoff
-    statU = np.zeros(max(nstatu, 1), np.uint32)
-    index = {}
-    for i, sh in enumerate(S["shapes"]):
-        if sh.get("hull") or sh.get("plane"):
-            index[i] = len(index)
-    filt = statU[SU["SU_FILTER"]:]
-
-    def setbit(k):
-        filt[k >> 5] |= np.uint32(1 << (k & 31))
-
-    for a0, b0 in S.get("filters", []):
-        a, b = index.get(a0), index.get(b0)
-        if a is None or b is None:
-            continue
-        setbit(a * NS + b)
-        setbit(b * NS + a)
 
     def groupsMeet(ga, gb):
         return ga != 0 and gb != 0 and ((ga == gb or gb < 0) if ga > 0 else ga != gb)
 
+    index = {}
+    for i, sh in enumerate(S["shapes"]):
+        if sh.get("hull") or sh.get("plane"):
+            index[i] = len(index)
+    fl = [(index.get(a0), index.get(b0)) for a0, b0 in S.get("filters", [])]
+    fl = [(a, b) for a, b in fl if a is not None and b is not None and a != b]
+    # World and group rules (sim.js:244-255).  Both are symmetric in the two shapes, so they are decided once per
+    # pair of (world, group) classes; a class pair that does not meet filters all of its shape pairs.
+    cls = {}
     for a in range(NS):
-        A = shapes[a]
-        wa = A.get("world", -1)
-        wa = -1 if wa is None else wa
-        ga = A.get("group", -1)
-        ga = -1 if ga is None else ga
-        for b in range(a + 1, NS):
-            B = shapes[b]
-            wb = B.get("world", -1)
-            wb = -1 if wb is None else wb
-            gb = B.get("group", -1)
-            gb = -1 if gb is None else gb
-            meet = (wa < 0 or wb < 0 or wa == wb) and groupsMeet(ga, gb)
-            if not meet:
-                setbit(a * NS + b)
-                setbit(b * NS + a)
+        wa, ga = shapes[a].get("world", -1), shapes[a].get("group", -1)
+        cls.setdefault((-1 if wa is None else wa, -1 if ga is None else ga), []).append(a)
+    keys = sorted(cls)
+    apart = [(x, y) for x in keys for y in keys if x <= y and not ((x[0] < 0 or y[0] < 0 or x[0] == y[0]) and groupsMeet(x[1], y[1]))]
+    # Past FILT_LIST_SHAPES shapes, a world whose filters are only listed pairs keeps them as per-shape sorted lists
+    # instead of the NS x NS bitset (about 1.3 GB at 100k shapes).
+    filt_list = NS > FILT_LIST_SHAPES and not apart
+    if filt_list:
+        lists = [[] for _ in range(NS)]
+        for a, b in sorted({(min(a, b), max(a, b)) for a, b in fl}):
+            lists[a].append(b)
+        offs = np.zeros(NS + 1, np.int64)
+        offs[1:] = np.cumsum([len(x) for x in lists])
+        SU["SU_FILTER"] = uoff; uoff += NS + 1 + int(offs[-1])
+    else:
+        SU["SU_FILTER"] = uoff; uoff += (NS * NS + 31) // 32
+    nstatu = uoff
+    statU = np.zeros(max(nstatu, 1), np.uint32)
+    filt = statU[SU["SU_FILTER"]:]
+    if filt_list:
+        filt[:NS + 1] = offs
+        filt[NS + 1:NS + 1 + int(offs[-1])] = [j for x in lists for j in x]
+
+    def setbits(a, b):  # pairs (a, b) and (b, a); bit index a * NS + b needs 64 bits past 65536 shapes
+        a, b = np.asarray(a, np.int64), np.asarray(b, np.int64)
+        for k in (a * NS + b, b * NS + a):
+            np.bitwise_or.at(filt, k >> 5, (np.uint32(1) << (k & 31).astype(np.uint32)))
+
+    if not filt_list:
+        if fl:
+            setbits([a for a, _ in fl], [b for _, b in fl])
+        for x, y in apart:
+            A, B = np.asarray(cls[x], np.int64), np.asarray(cls[y], np.int64)
+            aa, bb = np.meshgrid(A, B, indexing="ij")
+            aa, bb = aa.ravel(), bb.ravel()
+            keep = aa != bb
+            setbits(aa[keep], bb[keep])
 
     # Bodies (sim.js:257-305).
     bodyPose = np.zeros(NB * 8, np.float32)
@@ -626,6 +643,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_TGT_HELD", 1 if S.get("targets_held") else 0)
     H("W_SLEEP", 1 if (SV.get("enable_sleeping") or SO.get("enable_sleeping")) else 0)
     H("W_NMIMIC", len(mimics))
+    H("W_FILT_LIST", 1 if filt_list else 0)
     H("W_WH", WARM_SLOTS if warm else 0)
     H("W_NEST", 1 if nest else 0)
     H("W_MO", MO)

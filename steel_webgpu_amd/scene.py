@@ -555,6 +555,10 @@ class SceneBuilder:
         rootof = [rootOf(b) for b in range(NB)]
         roots = sorted(set(rootof))
         jroot = [rootOf(j["child"]) for j in self.joints]
+        byroot = {}  # each root's tree joints in joint order
+        for i in range(len(self.joints)):
+            if i not in loop:
+                byroot.setdefault(jroot[i], []).append(i)
         for a, r in enumerate(roots):
             if parentJoint[r] < 0:
                 implicit.append(dict(type=JT["FREE"], parent=-1, child=r, Xp=IDENTITY, Xc=IDENTITY,
@@ -562,9 +566,8 @@ class SceneBuilder:
                                      label=f"{self.bodies[r]['label']}_free", dofs=[dict(DOF_DEFAULTS) for _ in range(6)], mimic=None))
                 order.append(-len(implicit))
                 art.append(a)
-            for i, j in enumerate(self.joints):
-                if i in loop or jroot[i] != r:
-                    continue
+            for i in byroot.get(r, ()):
+                j = self.joints[i]
                 order.append(i)
                 art.append(a)
                 placed.add(j["child"])
@@ -643,10 +646,13 @@ class SceneBuilder:
             pose[j["child"]] = tmul(tmul(tmul(Xwp, j["Xp"]), T), tinv(j["Xc"]))
 
         bodies = dict(count=NB, mass=[], inertia=[], com=[], q=pose, twist=[], label=[], kinematic=[], no_gravity=[])
+        shapes_of = {}  # each body's shapes in shape order
+        for s in self.shapes:
+            shapes_of.setdefault(s["body"], []).append(s)
         for i, b in enumerate(self.bodies):
             m, c, parts = 0.0, [0.0, 0.0, 0.0], []
-            for s in self.shapes:
-                if s["body"] != i or s["type"] == GT["PLANE"] or not ((s["o"].get("mass") or 0) > 0 or s["o"]["density"] > 0):
+            for s in shapes_of.get(i, ()):
+                if s["type"] == GT["PLANE"] or not ((s["o"].get("mass") or 0) > 0 or s["o"]["density"] > 0):
                     continue
                 p = massProps(s["type"], s["size"], s["hull"])
                 rho = s["o"]["mass"] / p["m"] if (s["o"].get("mass") or 0) > 0 else s["o"]["density"]

@@ -4,6 +4,8 @@ Scenes (fps / substeps / steps as in the benchmark definitions):
   boxes64        ground plane + 64 boxes, half extents U[0.05,0.2]^3, density 1000, mu 0.6; 240 / 4 / 500
   arm_hulls      32-link fixed-base revolute capsule chain among 256 convex hulls in a walled 2x2 m bin; 240 / 8 / 1000
   quadruped(i)   floating-base 12-joint quadruped + K~Geom(0.1)<=64 boxes, world i of a batch; 200 / 4 / 1000
+  large_world    one world of 100k bodies (boxes and 12-point sphere hulls, sizes U[0.03,0.1]) on a jittered 3D lattice
+                 over a 20x20 m floor, plus 256 quadrupeds; 240 / 4 / 300
 Gravity is -z and the ground is z = 0, so the vertical range of each spawn box is its z range.
 """
 from __future__ import annotations
@@ -12,7 +14,9 @@ import math
 
 import numpy as np
 
-from .scene import SceneBuilder
+from .scene import GT, SceneBuilder
+
+GT_BOX = GT["BOX"]
 
 
 class PCG32:
@@ -275,3 +279,110 @@ def gait_path(frames=120, dofs=18, free=6, amp=0.3, hz=1.5, seed=0):
     path = amp * np.sin(2 * np.pi * hz * t[:, None] + ph[None, :])
     path[:, :free] = 0.0
     return path
+
+
+# --- the large single world -----------------------------------------------------------------------------------
+LARGE_PITCH = 0.36   # lattice cell (m): two bodies of bounding radius <= 0.1 sqrt(3) never overlap with 5 mm jitter
+LARGE_GAP = 0.02     # contact gap (m) of the lattice bodies: the shipped scenes' range (0-0.02), see large_world
+
+
+def _hull_job(pts):
+    from . import scene as _scene
+    return _scene.convexHull(pts)
+
+
+def large_world(seed=SEED + 4, nbodies=100000, nquads=256, extent=20.0, gap=LARGE_GAP, fps=240, substeps=4, iterations=12,
+                workers=1):
+    """One large world: `nbodies` bodies, alternately boxes (half extents U[0.03, 0.1] per axis) and convex hulls of 12
+    points on a sphere of radius U[0.03, 0.1], density 1000, mu 0.6, random orientations, at rest on a jittered 3D
+    lattice (LARGE_PITCH cells, +-5 mm) over an extent x extent m floor, filled layer by layer from z = 0.2 m; and
+    `nquads` floating-base quadrupeds standing on a square grid over the same floor, whose lattice cells below 1.1 m
+    are left empty.  The lattice bodies use a `gap` contact gap (the builder's default 0.1 m, against bodies of 3-10 cm,
+    would make every lattice neighbour a speculative contact pair); the quadrupeds keep the default.  `workers` > 1
+    computes the shapes' convex hulls in that many processes first (the same hulls: SceneBuilder finds them cached)."""
+    rng = PCG32(seed, 0)
+    b = SceneBuilder(fps=fps)
+    b.addGroundPlane(mu=0.6)
+    qs = max(1, int(math.ceil(math.sqrt(nquads)))) if nquads else 1
+    qpitch = extent / qs
+    quads = []
+    for k in range(nquads):
+        qy, qx = divmod(k, qs)
+        x0, y0 = (qx + 0.5) * qpitch - 0.5 * extent, (qy + 0.5) * qpitch - 0.5 * extent
+        _add_quadruped(b, x0, y0, rng)
+        quads.append((x0, y0))
+    side = max(1, int(extent / LARGE_PITCH))
+    x_cells = [(i + 0.5) * LARGE_PITCH - 0.5 * side * LARGE_PITCH for i in range(side)]
+
+    def blocked(x, y, z):
+        if z >= 1.1:
+            return False
+        cx, cy = int(math.floor((x + 0.5 * extent) / qpitch)), int(math.floor((y + 0.5 * extent) / qpitch))
+        for ix in (cx - 1, cx, cx + 1):
+            for iy in (cy - 1, cy, cy + 1):
+                k = iy * qs + ix
+                if 0 <= ix < qs and 0 <= iy < qs and k < nquads:
+                    qx, qy = quads[k]
+                    if abs(x - qx) < 0.45 + 0.5 * LARGE_PITCH and abs(y - qy) < 0.3 + 0.5 * LARGE_PITCH:
+                        return True
+        return False
+
+    bodies = []  # (pose, box half extents or None, hull points or None), drawn in lattice order
+    layer = 0
+    while len(bodies) < nbodies:
+        z = 0.2 + layer * LARGE_PITCH
+        for y in x_cells:
+            for x in x_cells:
+                if len(bodies) >= nbodies:
+                    break
+                if blocked(x, y, z):
+                    continue
+                p = [x + rng.uniform(-0.005, 0.005), y + rng.uniform(-0.005, 0.005), z + rng.uniform(-0.005, 0.005)]
+                pose = p + rng.unit_quat()
+                if len(bodies) % 2 == 0:
+                    bodies.append((pose, [rng.uniform(0.03, 0.1) for _ in range(3)], None))
+                else:
+                    r = rng.uniform(0.03, 0.1)
+                    bodies.append((pose, None, [[r * c for c in rng.unit_vec()] for _ in range(12)]))
+        layer += 1
+    if workers > 1:
+        from . import scene as _scene
+        jobs = [_scene.shapePoints(GT_BOX, h) if h is not None else pts for _, h, pts in bodies]
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            hulls = pool.map(_hull_job, jobs, chunksize=256)
+        for pts, hull in zip(jobs, hulls):
+            _scene._HULL_CACHE[tuple(tuple(float(c) for c in p) for p in pts)] = hull
+    for pose, h, pts in bodies:
+        body = b.addBody(xform=pose)
+        if h is not None:
+            b.addShapeBox(body, hx=h[0], hy=h[1], hz=h[2], mu=0.6, density=1000.0, gap=gap)
+        else:
+            b.addShapeConvexHull(body, vertices=pts, mu=0.6, density=1000.0, gap=gap)
+    return b.finalize(solver=dict(substeps=substeps, iterations=iterations))
+
+
+def large_world_small(seed=SEED + 4, nbodies=10000, nquads=25):
+    """The large world's construction at a tenth of its bodies (same lattice density: the floor scaled by
+    sqrt(1/10)), for parity tests the oracle finishes in seconds."""
+    return large_world(seed=seed, nbodies=nbodies, nquads=nquads, extent=20.0 * math.sqrt(nbodies / 100000.0))
+
+
+def offset_stack(nfar=0, seed=SEED + 5, fps=240, substeps=4, iterations=12):
+    """Two boxes stacked on the ground, preceded by `nfar` free bodies without shapes (1 kg, falling far from the
+    stack).  The stack's bodies are numbered nfar and nfar + 1 while its shapes keep the numbers 1 and 2, so with
+    nfar >= 65535 its contact keys, units, colors and anchors need body ids past 16 bits, while everything keyed by
+    shape (feature hashes) is unchanged: its trajectory is the same for every nfar."""
+    rng = PCG32(seed, 0)
+    b = SceneBuilder(fps=fps)
+    b.addGroundPlane(mu=0.6)
+    for i in range(nfar):
+        b.addBody(xform=[100.0 + (i % 300), float(i // 300), 50.0, 0, 0, 0, 1], mass=1.0, com=[0.0, 0.0, 0.0],
+                  inertia=[0.01, 0, 0, 0, 0.01, 0, 0, 0, 0.01])
+    h = [[rng.uniform(0.05, 0.2) for _ in range(3)] for _ in range(2)]
+    q = rng.unit_quat()
+    lower = b.addBody(xform=[0.0, 0.0, h[0][2] + 0.01, 0, 0, 0.0998, 0.995])
+    b.addShapeBox(lower, hx=h[0][0], hy=h[0][1], hz=h[0][2], mu=0.6, density=1000.0)
+    upper = b.addBody(xform=[0.02, -0.01, 2 * h[0][2] + h[1][2] + 0.05] + q)
+    b.addShapeBox(upper, hx=h[1][0], hy=h[1][1], hz=h[1][2], mu=0.6, density=1000.0)
+    return b.finalize(solver=dict(substeps=substeps, iterations=iterations))
