@@ -103,6 +103,13 @@ int steel_batch_buffer_size(steel_batch* b, uint32_t world, uint32_t buf, size_t
 int steel_batch_stats(steel_batch* b, uint32_t world, uint32_t out[16], double* gpu_ms);
 /* Launch options: workgroup size of prepare/solve (64, 128 or 256) and hipGraph capture on/off. */
 int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph);
+/* Device-wide stepping of one large world (Sim.step, sim.js:692-701, for a world past the reference's limits of
+ * 256 articulations, 16-bit body ids and one prepare/solve workgroup, sim.js:159, :663): every stage but the
+ * narrowphase runs as one cooperative launch over the whole device instead of one workgroup.  The batch must
+ * hold one world whose solve keeps its link mirror in global memory, without sleeping, contact warm start,
+ * Nesterov momentum or velocity iterations (STEEL_UNSUPPORTED otherwise); results are those of the one-workgroup
+ * step, bit for bit.  enable 0 returns to one workgroup per stage.  Frames are not captured as graphs. */
+int steel_batch_set_wide(steel_batch* b, int enable);
 
 /* The launch schedule the autotuner picked for a batch. */
 typedef struct steel_tune_result {

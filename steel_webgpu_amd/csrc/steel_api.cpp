@@ -48,6 +48,9 @@ struct steel_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<uint32_t> headers;  // n * STEEL_WORLD_HEADER_WORDS
+  bool wide = false;               // one large world over the whole device (steel_batch_set_wide)
+  uint32_t wide_groups = 0;        // workgroups of its cooperative launches
+  uint32_t* grid_scratch = nullptr;
 };
 
 extern "C" {
@@ -109,6 +112,7 @@ static void batch_free(steel_batch* b) {
   for (int k = 0; k < K_COUNT; k++) if (b->arena[k]) (void)hipFree(b->arena[k]);
   if (b->d_worlds) (void)hipFree(b->d_worlds);
   if (b->pair_offs) (void)hipFree(b->pair_offs);
+  if (b->grid_scratch) (void)hipFree(b->grid_scratch);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -230,8 +234,44 @@ int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph) {
   return STEEL_OK;
 }
 
+// One substep's launches: one workgroup per world and stage, or for a wide batch the device-wide launches.
+static hipError_t launch_substep(steel_batch* b, uint32_t stages, hipEvent_t* marks = nullptr) {
+  if (b->wide) return steel_launch_substep_grid(b->d_worlds, b->wide_groups, b->grid_scratch, b->pair_offs, b->narrow_groups, b->stream, stages, marks);
+  return steel_launch_substep(b->d_worlds, b->n, b->block, b->sl, b->pair_offs, b->narrow_groups, b->stream, stages, marks);
+}
+
+// Device-wide stepping stays off until its launches run clean on the GPU.
+static constexpr bool kWideEnabled = false;
+static bool steel_wide_enabled() { return kWideEnabled; }
+
 static int launch_frame(steel_batch* b) {
-  for (uint32_t s = 0; s < b->substeps; s++) HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->sl, b->pair_offs, b->narrow_groups, b->stream, STAGE_ALL));
+  for (uint32_t s = 0; s < b->substeps; s++) HIPCHK(launch_substep(b, STAGE_ALL));
+  return STEEL_OK;
+}
+
+int steel_batch_set_wide(steel_batch* b, int enable) {
+  if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
+  HIPCHK(hipSetDevice(b->dev->ordinal));
+  if (!enable) { b->wide = false; return STEEL_OK; }
+  if (!steel_wide_enabled())
+    return fail(STEEL_UNSUPPORTED, "device-wide stepping is switched off in this build: its cooperative launches faulted the GPU "
+                                   "and the cause is not yet found");
+  const uint32_t* h = b->headers.data();
+  if (b->n != 1) return fail(STEEL_UNSUPPORTED, "device-wide stepping takes a batch of one world, not %u", b->n);
+  if (b->size[K_MIRROR][0] == 0) return fail(STEEL_UNSUPPORTED, "the world fits one solve workgroup; step it as a batch");
+  if (h[W_SLEEP] || h[W_WH] || h[W_NEST] || h[W_VELITERS])
+    return fail(STEEL_UNSUPPORTED, "device-wide stepping does not support sleeping, contact warm start, Nesterov momentum or velocity iterations");
+  if (!b->grid_scratch) {
+    HIPCHK(steel_grid_groups((uint32_t)b->dev->prop.multiProcessorCount, &b->wide_groups));
+    const size_t bytes = grid_scratch_words(b->wide_groups) * 4;
+    HIPCHK(hipMalloc(&b->grid_scratch, bytes));
+    HIPCHK(hipMemset(b->grid_scratch, 0, bytes));
+  }
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr;
+    (void)hipGraphDestroy(b->graph); b->graph = nullptr;
+  }
+  b->wide = true;
   return STEEL_OK;
 }
 
@@ -239,7 +279,8 @@ int steel_batch_step(steel_batch* b, uint32_t frames) {
   if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
   HIPCHK(hipSetDevice(b->dev->ordinal));
   if (frames == 0) return STEEL_OK;
-  if (b->use_graph && !b->gexec) {
+  const bool graph = b->use_graph && !b->wide;
+  if (graph && !b->gexec) {
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = launch_frame(b);
     hipGraph_t g = nullptr;
@@ -251,7 +292,7 @@ int steel_batch_step(steel_batch* b, uint32_t frames) {
   }
   HIPCHK(hipEventRecord(b->ev0, b->stream));
   for (uint32_t f = 0; f < frames; f++) {
-    if (b->use_graph) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    if (graph) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
     else { int rc = launch_frame(b); if (rc) return rc; }
   }
   HIPCHK(hipEventRecord(b->ev1, b->stream));
@@ -262,7 +303,7 @@ int steel_batch_step(steel_batch* b, uint32_t frames) {
 int steel_batch_substep_stages(steel_batch* b, uint32_t stages) {
   if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
   HIPCHK(hipSetDevice(b->dev->ordinal));
-  HIPCHK(steel_launch_substep(b->d_worlds, b->n, b->block, b->sl, b->pair_offs, b->narrow_groups, b->stream, stages));
+  HIPCHK(launch_substep(b, stages));
   return STEEL_OK;
 }
 
@@ -292,7 +333,7 @@ int steel_batch_profile_kernels(steel_batch* b, uint32_t substeps, double kernel
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   int rc = STEEL_OK;
   for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)
-    if (steel_launch_substep(b->d_worlds, b->n, b->block, b->sl, b->pair_offs, b->narrow_groups, b->stream, STAGE_ALL, &ev[4 * s]) != hipSuccess) rc = fail(STEEL_ERROR, "launch failed");
+    if (launch_substep(b, STAGE_ALL, &ev[4 * s]) != hipSuccess) rc = fail(STEEL_ERROR, "launch failed");
   if (rc == STEEL_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = fail(STEEL_ERROR, "sync failed");
   for (int k = 0; k < 3; k++) kernel_ms[k] = 0.0;
   for (uint32_t s = 0; s < substeps && rc == STEEL_OK; s++)
@@ -305,11 +346,20 @@ int steel_batch_profile_kernels(steel_batch* b, uint32_t substeps, double kernel
   return rc;
 }
 
+// A wide batch's barriers record a wait they gave up on (grid_barrier); its results are then not the step's.
+static int check_wide(steel_batch* b) {
+  if (!b->grid_scratch) return STEEL_OK;
+  uint32_t flag = 0;
+  HIPCHK(hipMemcpy(&flag, b->grid_scratch + 2, 4, hipMemcpyDeviceToHost));
+  if (flag) return fail(STEEL_ERROR, "a device-wide launch's workgroups did not all reach a barrier");
+  return STEEL_OK;
+}
+
 int steel_batch_sync(steel_batch* b) {
   if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
   HIPCHK(hipSetDevice(b->dev->ordinal));
   HIPCHK(hipStreamSynchronize(b->stream));
-  return STEEL_OK;
+  return check_wide(b);
 }
 
 
@@ -328,6 +378,7 @@ int steel_batch_read(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_o
   if (byte_off > b->size[k][world] || bytes > b->size[k][world] - byte_off) return fail(STEEL_BAD_ARGUMENT, "read of %zu bytes at %zu past the buffer's %zu", bytes, byte_off, b->size[k][world]);
   HIPCHK(hipSetDevice(b->dev->ordinal));
   HIPCHK(hipStreamSynchronize(b->stream));
+  if (int rc = check_wide(b)) return rc;
   HIPCHK(hipMemcpy(dst, b->arena[k] + b->off[k][world] + byte_off, bytes, hipMemcpyDeviceToHost));
   return STEEL_OK;
 }

@@ -6,6 +6,7 @@
 // shapes/broad/narrow/csort (wgsl.js:117-1215, 3895-3980), prepare (1787-2494) and solve (3633-3891).
 #include "steel_collide.cuh"
 #include "steel_launch.h"
+#include "steel_team.cuh"
 
 // ------------------------------------------------------------------------------------------------
 // Block helpers
@@ -188,38 +189,28 @@ DEV bool pair_allowed(const W& w, uint32_t i, uint32_t j, uint32_t tag_i, uint32
 #define STEEL_WPE_SOLVE 4
 #endif
 
-// Shapes and broadphase, one workgroup per world: sort shapes by the order-preserving integer of AABB min x,
-// then sweep the sorted list; pair (i, j) survives iff it passes the reference's all-pairs predicate
-// (wgsl.js:154-177), tested in full for every candidate the sweep reaches.  Worlds of up to `cap` shapes
-// sweep from LDS.
-template <int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_broad(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t cap) {
-  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
-  __shared__ uint32_t s_hist[256];
-  __shared__ uint32_t s_cnt[(BS / 64) * 256];
-  __shared__ uint64_t s_red[2 * (BS / 64)];
-  __shared__ uint32_t s_np;
-  __shared__ uint32_t s_w[BS / 64];
-  // Dynamic LDS for worlds of up to `cap` shapes: the sorted AABBs (lo.w holds the shape id) and each sorted
-  // shape's first candidate in the flattened candidate list (cap + 1 words).
-  extern __shared__ float4 s_box[];
-  float4* s_lo = s_box;
-  float4* s_hi = s_box + cap;
-  uint32_t* s_run = (uint32_t*)(s_box + 2 * cap);
-  uint32_t wid = blockIdx.x;
-  if (wid >= nworlds) return;
-  const WorldDev wd = worlds[wid];
+// Shapes and broadphase of one world by a team: sort shapes by the order-preserving integer of AABB min x, then
+// sweep the sorted list; pair (i, j) survives iff it passes the reference's all-pairs predicate (wgsl.js:154-177),
+// tested in full for every candidate the sweep reaches.  A block team sweeps worlds of up to `cap` shapes from LDS
+// (s_box) and larger ones shape by shape from global memory; a grid team keeps the sorted bounds and runs in the
+// world's contact buffers (free until the narrowphase) and deals the flattened candidates out over the launch.
+// np: the team's pair counter (LDS for a block, global for a grid).
+template <int BS, class TM>
+DEV void broad_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t cap, float4* s_box, uint32_t* np) {
   load_header(Hs, wd);
   W w = make_world(Hs, wd);
-  const uint32_t tid = threadIdx.x, NS = w.NS();
-  if (tid == 0) s_np = 0;
-  for (uint32_t s = tid; s < NS; s += BS) shape_pose(w, s);
-  __syncthreads();
+  const uint32_t tid = tm.tid, NT = tm.NT, NS = w.NS();
+  if (tid == 0) *np = 0;
+  for (uint32_t s = tid; s < NS; s += NT) shape_pose(w, s);
+  tm.sync();
   const uint32_t PA = w.h(W_P_AABB);
-  for (uint32_t s = tid; s < NS; s += BS) { wd.skey[s] = fflip_u(fbits(wd.poses[PA + s * 2].x)); wd.sval[s] = s; }
-  __syncthreads();
-  block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), NS, s_hist, s_cnt, s_red);
-  const bool lds = NS <= cap;
+  for (uint32_t s = tid; s < NS; s += NT) { wd.skey[s] = fflip_u(fbits(wd.poses[PA + s * 2].x)); wd.sval[s] = s; }
+  tm.sync();
+  tm.sort(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), NS);
+  const bool flat = TM::GRID || NS <= cap;
+  float4* s_lo = TM::GRID ? wd.craw : s_box;
+  float4* s_hi = s_lo + (TM::GRID ? NS : cap);
+  uint32_t* s_run = TM::GRID ? (uint32_t*)wd.contacts : (uint32_t*)(s_box + 2 * cap);
   const uint32_t MAXP = w.MAXP();
   // Candidate (i, j), j after i in x order with min x_j <= max x_i.  The x intervals overlap (min x_i <= min
   // x_j <= max x_j), so the AABB test is y and z, then the rest of the predicate.
@@ -228,22 +219,22 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     if (lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.y > hi_i.y || lo_j.z > hi_i.z) return;
     uint32_t a = min(si, sj), b = max(si, sj);
     if (!pair_allowed(w, a, b, tag_i, tag_j)) return;
-    uint32_t slot = atomicAdd(&s_np, 1u);
+    uint32_t slot = atomicAdd(np, 1u);
     if (slot < MAXP) { wd.pairs[slot * 2] = a; wd.pairs[slot * 2 + 1] = b; }
   };
-  if (lds) {
-    for (uint32_t i = tid; i < NS; i += BS) {
+  if (flat) {
+    for (uint32_t i = tid; i < NS; i += NT) {
       uint32_t si = wd.sval[i];
       float4 lo = wd.poses[PA + si * 2], hi = wd.poses[PA + si * 2 + 1];
       lo.w = ubits(si);
       hi.w = ubits(shape_tag(w, si));
       s_lo[i] = lo; s_hi[i] = hi;
     }
-    __syncthreads();
+    tm.sync();
     // Run length of each sorted shape (binary search for the first later min x above its max x), then the
     // candidates are flattened by an exclusive scan and dealt out to the lanes in chunks of 8, so one shape
-    // with a long run (a ground plane spans every x) is shared by the whole workgroup.
-    for (uint32_t i = tid; i < NS; i += BS) {
+    // with a long run (a ground plane spans every x) is shared by the whole team.
+    for (uint32_t i = tid; i < NS; i += NT) {
       const float hx = s_hi[i].x;
       uint32_t lo_ = i + 1, hi_ = NS;
       while (lo_ < hi_) {
@@ -252,12 +243,12 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       }
       s_run[i] = lo_ - i - 1;
     }
-    __syncthreads();
-    const uint32_t T = block_scan_array<BS>(s_run, NS, s_w);
+    tm.sync();
+    const uint32_t T = tm.scan(s_run, NS);
     if (tid == 0) s_run[NS] = T;
-    __syncthreads();
+    tm.sync();
     constexpr uint32_t CH = 8;
-    for (uint32_t c = tid; c < (T + CH - 1) / CH; c += BS) {
+    for (uint32_t c = tid; c < (T + CH - 1) / CH; c += NT) {
       const uint32_t k0 = c * CH, k1 = min(k0 + CH, T);
       uint32_t lo_ = 0, hi_ = NS - 1;  // largest i with s_run[i] <= k0 (s_run[0] = 0, s_run[NS] = T > k0)
       while (lo_ < hi_) {
@@ -277,7 +268,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       }
     }
   } else {
-    for (uint32_t i = tid; i < NS; i += BS) {
+    for (uint32_t i = tid; i < NS; i += NT) {
       const uint32_t si = wd.sval[i];
       const float4 lo_i = wd.poses[PA + si * 2], hi_i = wd.poses[PA + si * 2 + 1];
       const uint32_t tag_i = shape_tag(w, si);
@@ -291,8 +282,33 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       }
     }
   }
-  __syncthreads();
-  if (tid == 0) { wd.ctr[0] = s_np; wd.ctr[1] = 0; }
+  tm.sync();
+  if (tid == 0) { wd.ctr[0] = *np; wd.ctr[1] = 0; }
+}
+
+// One world per workgroup.  Dynamic LDS for worlds of up to `cap` shapes: the sorted AABBs (lo.w holds the shape
+// id) and each sorted shape's first candidate in the flattened candidate list (cap + 1 words).
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_broad(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t cap) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ TeamLds<BS> tl;
+  __shared__ SortLds<BS> sl;
+  __shared__ uint32_t s_np;
+  extern __shared__ float4 s_box[];
+  const uint32_t wid = blockIdx.x;
+  if (wid >= nworlds) return;
+  const BlockTeam<BS> tm{threadIdx.x, &tl, &sl};
+  broad_world<BS>(tm, worlds[wid], Hs, cap, s_box, &s_np);
+}
+
+// One large world over a cooperative launch (g: grid team scratch; g[3] counts the pairs).
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_broad_grid(const WorldDev* __restrict__ world, uint32_t* g) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ TeamLds<BS> tl;
+  __shared__ SortLds<BS> sl;
+  const GridTeam<BS> tm{gridDim.x * BS, blockIdx.x * BS + threadIdx.x, &tl, &sl, g};
+  broad_world<BS>(tm, world[0], Hs, 0, nullptr, g + 3);
 }
 
 // Exclusive prefix of the worlds' pair counts (min(ctr[0], MAXP)) into offs[0..n], one block.
@@ -353,49 +369,41 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEEL_
   }
 }
 
-// Canonical contact order, one workgroup per world: (body pair, feature key) by the LDS radix sort; equal
-// pairs of those (a feature-hash collision) are ordered by (point bits, emission index) afterwards, as the
-// reference's key does (wgsl.js:3906-3980).
-template <int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_csort(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
-  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
-  __shared__ uint32_t s_hist[256];
-  __shared__ uint32_t s_cnt[(BS / 64) * 256];
-  __shared__ uint64_t s_red[2 * (BS / 64)];
-  __shared__ uint32_t s_tie;
-  uint32_t wid = blockIdx.x;
-  if (wid >= nworlds) return;
-  const WorldDev wd = worlds[wid];
+// Canonical contact order of one world by a team: (body pair, feature key) by the radix sort; equal pairs of
+// those (a feature-hash collision) are ordered by (point bits, emission index) afterwards, as the reference's key
+// does (wgsl.js:3906-3980).  tie: the team's tie flag (LDS for a block, global for a grid).
+template <int BS, class TM>
+DEV void csort_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t* tie) {
   load_header(Hs, wd);
   W w = make_world(Hs, wd);
-  const uint32_t tid = threadIdx.x;
-  if (tid == 0) s_tie = 0;
+  const uint32_t tid = tm.tid, NT = tm.NT;
+  if (tid == 0) *tie = 0;
   const uint32_t nc = min(wd.ctr[1], w.MAXC());
   // Key (body pair, feature): one 64-bit radix key while body ids + 1 fit 16 bits (the reference's packing);
   // past that, the feature pass then a stable pass over the 64-bit pair key.
   const bool wide = w.NB() > 0xffffu;
   auto feat_of = [&](uint32_t c) { const float4 m = wd.craw[c * 6 + 2]; return tu(m.z) | (tu(m.w) << 16); };
   auto pair_of = [&](uint32_t c) { const float4 m = wd.craw[c * 6 + 2]; return pair_key(ti(m.x), ti(m.y)); };
-  for (uint32_t c = tid; c < nc; c += BS) {
+  for (uint32_t c = tid; c < nc; c += NT) {
     float4 m = wd.craw[c * 6 + 2];
     const uint32_t feat = tu(m.z) | (tu(m.w) << 16);
     wd.skey[c] = wide ? (uint64_t)feat : ((uint64_t)((uint32_t)(ti(m.x) + 1) | ((uint32_t)(ti(m.y) + 1) << 16)) << 32) | feat;
     wd.sval[c] = c;
   }
-  __syncthreads();
-  block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), nc, s_hist, s_cnt, s_red);
+  tm.sync();
+  tm.sort(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), nc);
   if (wide) {
-    for (uint32_t i = tid; i < nc; i += BS) wd.skey[i] = pair_of(wd.sval[i]);
-    __syncthreads();
-    block_radix_sort<BS>(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), nc, s_hist, s_cnt, s_red);
+    for (uint32_t i = tid; i < nc; i += NT) wd.skey[i] = pair_of(wd.sval[i]);
+    tm.sync();
+    tm.sort(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), nc);
   }
   // equal (pair, feature) of sorted positions i and j
   auto same = [&](uint32_t i, uint32_t j) {
     return wd.skey[i] == wd.skey[j] && (!wide || feat_of(wd.sval[i]) == feat_of(wd.sval[j]));
   };
-  for (uint32_t i = tid; i + 1 < nc; i += BS) if (same(i, i + 1)) s_tie = 1;
-  __syncthreads();
-  if (s_tie && tid == 0) {
+  for (uint32_t i = tid; i + 1 < nc; i += NT) if (same(i, i + 1)) *tie = 1;
+  tm.sync();
+  if (*tie && tid == 0) {
     auto bits_of = [&](uint32_t c) {
       float4 p = wd.craw[c * 6];
       return hash_u32(hash_u32(hash_u32(fbits(p.x)) ^ fbits(p.y)) ^ fbits(p.z)) ^ fbits(p.w);
@@ -412,11 +420,33 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       wd.skey[j + 1] = k; wd.sval[j + 1] = v;
     }
   }
-  __syncthreads();
-  for (uint32_t i = tid; i < nc * 6; i += BS) {
+  tm.sync();
+  for (uint32_t i = tid; i < nc * 6; i += NT) {
     uint32_t r = i / 6, q = i % 6;
     wd.contacts[r * 6 + q] = wd.craw[wd.sval[r] * 6 + q];
   }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_csort(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ TeamLds<BS> tl;
+  __shared__ SortLds<BS> sl;
+  __shared__ uint32_t s_tie;
+  const uint32_t wid = blockIdx.x;
+  if (wid >= nworlds) return;
+  const BlockTeam<BS> tm{threadIdx.x, &tl, &sl};
+  csort_world<BS>(tm, worlds[wid], Hs, &s_tie);
+}
+
+// One large world over a cooperative launch (g: grid team scratch; g[3] is the tie flag).
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_csort_grid(const WorldDev* __restrict__ world, uint32_t* g) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ TeamLds<BS> tl;
+  __shared__ SortLds<BS> sl;
+  const GridTeam<BS> tm{gridDim.x * BS, blockIdx.x * BS + threadIdx.x, &tl, &sl, g};
+  csort_world<BS>(tm, world[0], Hs, g + 3);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1031,24 +1061,32 @@ struct Prep {
 
 // LDS of prepare's coloring phase (words).
 static constexpr uint32_t PREP_CH = 256, PREP_WC = 258;
+// Walk entries each thread of the first workgroup holds per window when the color masks are in global memory.
+static constexpr uint32_t PREP_CPER = 4;
 
-// cap: words of dynamic LDS, which hold a world's per-body color masks and first-pending slots when
-// NB * (NCOLW + 1) fits (steel_prep_lds_words).
+// LDS of prepare's phases, per workgroup.
 template <int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t cap) {
-  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
-  __shared__ uint32_t s_w[BS / 64];
-  __shared__ uint32_t s_wr[PREP_CH];                 // coloring: row of each walk entry of a chunk
-  __shared__ uint64_t s_wk[PREP_CH];                 // coloring: body-pair key of each walk entry of a chunk
-  __shared__ uint32_t s_wc[PREP_WC];                  // coloring: per-color counts and offsets when NCOL + 2 fits
-  __shared__ uint32_t s_chg;                          // sleep islands: a contact joined two islands this pass
-  extern __shared__ uint32_t s_pdyn[];                 // coloring: color masks, then first-pending slots
-  __shared__ uint32_t s_tree[(BS / 64) * 64];        // factor: link tree of each wave's articulation
-  __shared__ float s_pab[(BS / 64) * 512];           // factor: children's bias forces
-  __shared__ float s_trig[12];                       // reduction: cos, sin of k * 60 degrees, k = 0..5
-  uint32_t wid = blockIdx.x;
-  if (wid >= nworlds) return;
-  const WorldDev wd = worlds[wid];
+struct PrepLds {
+  uint32_t wr[PREP_CH];              // coloring: row of each walk entry of a chunk
+  uint64_t wk[PREP_CH];              // coloring: body-pair key of each walk entry of a chunk
+  uint32_t wc[PREP_WC];              // coloring: per-color counts and offsets when NCOL + 2 fits
+  uint32_t chg;                      // sleep islands: a contact joined two islands this pass
+  uint32_t tree[(BS / 64) * 64];     // factor: link tree of each wave's articulation
+  float pab[(BS / 64) * 512];        // factor: children's bias forces
+  float trig[12];                    // reduction: cos, sin of k * 60 degrees, k = 0..5
+};
+
+// Prepare of one world by a team.  cap: words of dynamic LDS (s_pdyn), which hold a block team's per-body color
+// masks and first-pending slots when NB * (NCOLW + 1) fits (steel_prep_lds_words); a grid team keeps them in
+// global memory.
+template <int BS, class TM>
+DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t cap, uint32_t* s_pdyn, PrepLds<BS>* pl) {
+  uint32_t* s_wr = pl->wr;
+  uint64_t* s_wk = pl->wk;
+  uint32_t* s_wc = pl->wc;
+  uint32_t* s_tree = pl->tree;
+  float* s_pab = pl->pab;
+  float* s_trig = pl->trig;
   if (threadIdx.x < 6) {
     const float ang = (float)threadIdx.x * 1.0471975511965976f;
     s_trig[2 * threadIdx.x] = cos_c(ang);
@@ -1059,23 +1097,23 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   P.w = make_world(Hs, wd);
   const W& w = P.w;
   const SchedLayout& S = w.S;
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = tm.tid, NT = tm.NT;
   P.parity = wd.ctr[2] & 1u;
   const uint32_t NB = w.NB(), NA = w.NA(), NL = w.NL(), ND = w.ND(), MAXC = w.MAXC(), MAXU = w.MAXU(), UH = w.UH(), AH = w.AH();
   const uint32_t NCOL = w.NCOL(), NCOLW = w.h(W_NCOLW), PATCH = w.h(W_PATCH);
   uint32_t* sched = wd.sched;
   float* art_ = wd.art;
   // ---- Phase 0
-  for (uint32_t i = tid; i < UH; i += BS) { w.sch64(S.HKEY)[i] = 0; sched[S.HCNT + i] = 0; }
-  if (PATCH) for (uint32_t i = tid; i < AH; i += BS) wd.akey[(1 - P.parity) * AH + i] = 0;
+  for (uint32_t i = tid; i < UH; i += NT) { w.sch64(S.HKEY)[i] = 0; sched[S.HCNT + i] = 0; }
+  if (PATCH) for (uint32_t i = tid; i < AH; i += NT) wd.akey[(1 - P.parity) * AH + i] = 0;
   if (const uint32_t WH = w.h(W_WH)) {  // the table this substep's solve writes, and its winners
-    for (uint32_t i = tid; i < WH; i += BS) wd.warm[(1 - P.parity) * WH + i] = 0;
-    for (uint32_t i = tid; i < 2 * WH; i += BS) sched[S.WWIN + i] = 0;
+    for (uint32_t i = tid; i < WH; i += NT) wd.warm[(1 - P.parity) * WH + i] = 0;
+    for (uint32_t i = tid; i < 2 * WH; i += NT) sched[S.WWIN + i] = 0;
   }
-  for (uint32_t i = tid; i < max(NB, 1u); i += BS) sched[S.LINKHIT + i] = 0;
-  for (uint32_t i = tid; i < max(NB, 1u) * NCOLW; i += BS) sched[S.WMASK + i] = 0;
-  for (uint32_t i = tid; i < NCOL + 2; i += BS) sched[S.WCOUNT + i] = 0;
-  for (uint32_t i = tid; i < max(NL, 1u); i += BS) sched[S.JSPLIT + i] = 0;
+  for (uint32_t i = tid; i < max(NB, 1u); i += NT) sched[S.LINKHIT + i] = 0;
+  for (uint32_t i = tid; i < max(NB, 1u) * NCOLW; i += NT) sched[S.WMASK + i] = 0;
+  for (uint32_t i = tid; i < NCOL + 2; i += NT) sched[S.WCOUNT + i] = 0;
+  for (uint32_t i = tid; i < max(NL, 1u); i += NT) sched[S.JSPLIT + i] = 0;
   if (tid == 0) sched[S.META + 3] = 0;
   const float dt = w.hf(W_DT);
   if (w.h(W_SLEEP)) {
@@ -1096,31 +1134,31 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       }
       return n;
     };
-    for (uint32_t n = tid; n < NN; n += BS) { UF[n] = n; UFF[n] = 0; }
+    for (uint32_t n = tid; n < NN; n += NT) { UF[n] = n; UFF[n] = 0; }
     if (tid == 0) sched[S.META + 5] = 0;
-    __syncthreads();
-    for (uint32_t c = tid; c < nc0; c += BS) {
+    tm.sync();
+    for (uint32_t c = tid; c < nc0; c += NT) {
       float4 m = P.con(c, 2);
       int na = w.body_node(ti(m.x)), nb = w.body_node(ti(m.y));
       if ((na < 0 || nb < 0) && max(na, nb) >= 0 && P.con(c, 0).w <= 1e-5f) atomicOr(&UFF[(uint32_t)max(na, nb)], 1u);
     }
     for (;;) {
-      if (tid == 0) s_chg = 0;
-      __syncthreads();
-      for (uint32_t c = tid; c < nc0; c += BS) {
+      if (tid == 0) pl->chg = 0;
+      tm.sync();
+      for (uint32_t c = tid; c < nc0; c += NT) {
         float4 m = P.con(c, 2);
         int na = w.body_node(ti(m.x)), nb = w.body_node(ti(m.y));
         if (na < 0 || nb < 0) continue;
         uint32_t ra = find((uint32_t)na), rb = find((uint32_t)nb);
-        if (ra != rb) { atomicMin(&UF[max(ra, rb)], min(ra, rb)); s_chg = 1; }
+        if (ra != rb) { atomicMin(&UF[max(ra, rb)], min(ra, rb)); pl->chg = 1; }
       }
-      __syncthreads();
-      for (uint32_t n = tid; n < NN; n += BS) atomicMin(&UF[n], find(n));
-      __syncthreads();
-      if (s_chg == 0) break;
-      __syncthreads();
+      tm.sync();
+      for (uint32_t n = tid; n < NN; n += NT) atomicMin(&UF[n], find(n));
+      tm.sync();
+      if (pl->chg == 0) break;
+      tm.sync();
     }
-    for (uint32_t n = tid; n < NN; n += BS) {
+    for (uint32_t n = tid; n < NN; n += NT) {
       bool moving = false, fixed = false, veto = false;
       if (n < NB) {
         if (w.body_kind((int)n) != 1) continue;
@@ -1138,8 +1176,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       uint32_t f = (fixed ? 1u : 0u) | ((moving || veto) ? 2u : 0u) | (UFF[n] & 1u);
       atomicOr(&UFF[UF[n]], f);
     }
-    __syncthreads();
-    for (uint32_t n = tid; n < NN; n += BS) {
+    tm.sync();
+    for (uint32_t n = tid; n < NN; n += NT) {
       if (n < NB && w.body_kind((int)n) != 1) continue;
       uint32_t r = UF[n], f = __atomic_load_n(&UFF[r], __ATOMIC_RELAXED);
       float age = art_[ASLP + n * 4];
@@ -1147,14 +1185,14 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       art_[ASLP + n * 4] = age;
       if (age < QUIET) atomicOr(&UFF[r], 4u);
     }
-    __syncthreads();
-    for (uint32_t n = tid; n < NN; n += BS) {
+    tm.sync();
+    for (uint32_t n = tid; n < NN; n += NT) {
       if (n < NB && w.body_kind((int)n) != 1) continue;
       bool s = (UFF[UF[n]] & 4u) == 0;
       art_[ASLP + n * 4 + 1] = s ? 1.0f : 0.0f;
       if (s) atomicAdd(&sched[S.META + 5], 1u);
     }
-    __syncthreads();
+    tm.sync();
   }
   // Recorded target paths (60 Hz samples), interpolated at every substep at its midpoint half a frame back
   // (or held per frame), plus the integral term of the first dofs (wgsl.js:1800-1820).  Off once the user
@@ -1166,7 +1204,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       const float phase = (float)__hiloint2double((int)wd.ctr[5], (int)wd.ctr[4]);
       const uint32_t PF = w.h(W_SF_PATH) * 4, NI = w.h(W_INTEGRAL_COUNT);
       const float* path = (const float*)wd.statF;
-      for (uint32_t i = tid; i < driven; i += BS) {
+      for (uint32_t i = tid; i < driven; i += NT) {
         float ramp = (phase + ((float)sub + 0.5f) * dt) * 60.0f - 0.5f;
         float frame = clampf_(w.h(W_TGT_HELD) ? phase * 60.0f : ramp, 0.0f, (float)(frames - 1u));
         uint32_t fa = (uint32_t)frame, fb = min(fa + 1u, frames - 1u);
@@ -1182,7 +1220,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
         }
         art_[w.h(W_A_TGT) + i] = value;
       }
-      __syncthreads();
+      tm.sync();
     }
   }
   if (w.h(W_CTRL_KIND) == 1) {
@@ -1191,9 +1229,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       const float4 cc = w.sf(w.h(W_SF_CTRL));
       art_[w.h(W_A_TGTV) + w.h(W_CTRL_B)] = cc.y * clampf_(art_[w.h(W_A_Q) + w.h(W_CTRL_A)] / cc.x, 0.0f, 1.0f);
     }
-    __syncthreads();
+    tm.sync();
   }
-  for (uint32_t a = tid; a < NA; a += BS) {
+  for (uint32_t a = tid; a < NA; a += NT) {
     uint4 ai = w.art_info(a);
     if (w.h(W_VLIM_ON)) {
       // Velocity-limit prescale: the articulation's scalar dof velocities scaled so the most exceeded limit is
@@ -1236,19 +1274,19 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     art_[w.h(W_A_RLN) + a] = (float)n;
   }
   // FK and articulated-body factor, one wave per articulation, tree levels in parallel lanes.
-  for (uint32_t base = 0; base < NA; base += BS / 64) {
+  for (uint32_t base = 0; base < NA; base += NT / 64) {
     const uint32_t a = base + (tid >> 6), k = tid & 63;
     if (a < NA) {
-      uint32_t* tree = s_tree + (tid >> 6) * 64;
+      uint32_t* tree = s_tree + (threadIdx.x >> 6) * 64;
       w.art_tree(a, tree);
       w.art_fk_lv(a, k < w.art_info(a).y ? (tree[k] & 0xffu) : 0xffu);
-      w.art_factor_lv(a, tree, s_pab + (tid >> 6) * 512);
+      w.art_factor_lv(a, tree, s_pab + (threadIdx.x >> 6) * 512);
     }
   }
   if (w.h(W_VLIM_FREE)) {
     // Free-body velocity limits: per axis, whether the body's speed is near its limit (wgsl.js:1878-1887).
     const float frac = w.hf(W_F_VLIM_FRAC);
-    for (uint32_t b = tid; b < NB; b += BS) {
+    for (uint32_t b = tid; b < NB; b += NT) {
       if (w.body_kind((int)b) != 1) continue;
       q4 lm = w.body_limits((int)b);
       f3 v = xyz(wd.vel[b * 2]), wv = xyz(wd.vel[b * 2 + 1]);
@@ -1259,10 +1297,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       }
     }
   }
-  __syncthreads();
+  tm.sync();
   // ---- Phase 1: classes, run heads and dense slots
   const uint32_t nc = min(wd.ctr[1], MAXC);
-  for (uint32_t c = tid; c < nc; c += BS) {
+  for (uint32_t c = tid; c < nc; c += NT) {
     float4 m = P.con(c, 2);
     int a = ti(m.x), b = ti(m.y);
     const uint64_t pk = P.contact_pair(c);
@@ -1282,10 +1320,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     sched[S.HEADS + c] = head;
     sched[S.DSLOT + c] = dense;
   }
-  __syncthreads();
-  const uint32_t nruns = block_scan_array<BS>(sched + S.HEADS, nc, s_w);
-  const uint32_t ndc = block_scan_array<BS>(sched + S.DSLOT, nc, s_w);
-  for (uint32_t c = tid; c < nc; c += BS) {
+  tm.sync();
+  const uint32_t nruns = tm.scan(sched + S.HEADS, nc);
+  const uint32_t ndc = tm.scan(sched + S.DSLOT, nc);
+  for (uint32_t c = tid; c < nc; c += NT) {
     uint32_t cls = sched[S.CSLOT + c];
     if (cls >= 0xfffffffeu) continue;
     uint32_t unit = sched[S.HEADS + c] + cls;  // inclusive count of heads
@@ -1303,26 +1341,26 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     if (a >= 0 && w.body_kind(a) == 2) sched[S.LINKHIT + (uint32_t)a] = 1;
     if (b >= 0 && w.body_kind(b) == 2) sched[S.LINKHIT + (uint32_t)b] = 1;
   }
-  __syncthreads();
+  tm.sync();
   // ---- Phase 2: units are the run slots in order (unit u = slot h); walk offsets
   const uint32_t nh = min(nruns, UH);
-  for (uint32_t h = tid; h < nh; h += BS) sched[S.UWOFF + h] = sched[S.HCNT + h];
-  __syncthreads();
-  const uint32_t nwalk = block_scan_array<BS>(sched + S.UWOFF, nh, s_w);
+  for (uint32_t h = tid; h < nh; h += NT) sched[S.UWOFF + h] = sched[S.HCNT + h];
+  tm.sync();
+  const uint32_t nwalk = tm.scan(sched + S.UWOFF, nh);
   const uint32_t nu = min(nh, MAXU);
-  for (uint32_t h = tid; h < nu; h += BS) { w.sch64(S.UKEY)[h] = w.sch64(S.HKEY)[h]; sched[S.USIZE + h] = sched[S.HCNT + h]; }
-  for (uint32_t l = tid; l < NL; l += BS) {
+  for (uint32_t h = tid; h < nu; h += NT) { w.sch64(S.UKEY)[h] = w.sch64(S.HKEY)[h]; sched[S.USIZE + h] = sched[S.HCNT + h]; }
+  for (uint32_t l = tid; l < NL; l += NT) {
     if (sched[S.LINKHIT + (uint32_t)w.link_body(l)] == 0) continue;
     int j = (int)l;
     while (j >= 0) { atomicAdd(&sched[S.JSPLIT + (uint32_t)j], 1u); j = w.link_parent((uint32_t)j); }
   }
-  __syncthreads();
+  tm.sync();
   // Responses of contact-bearing links: six lanes per link (one per unit impulse), ten links per wave.
-  for (uint32_t l0 = (tid >> 6) * 10; l0 < NL; l0 += (BS / 64) * 10) {
+  for (uint32_t l0 = (tid >> 6) * 10; l0 < NL; l0 += (NT / 64) * 10) {
     const uint32_t lane = tid & 63, l = l0 + lane / 6, e = lane % 6;
     if (lane < 60 && l < NL && sched[S.LINKHIT + (uint32_t)w.link_body(l)] != 0) w.link_response_e(l, e, lane - e, sched + S.JSPLIT);
   }
-  for (uint32_t dd = tid; dd < ND; dd += BS) {
+  for (uint32_t dd = tid; dd < ND; dd += NT) {
     float4 lim = w.dof_limit(dd);
     if (lim.z > 0.5f || w.dof_passive(dd).y > 0.0f || w.dof_passive(dd).z >= 0.0f || w.sf(w.h(W_SF_DOF) + dd * 5 + 3).w > 0.0f ||
         (w.h(W_DRIVE_ROWS) && w.driven_dof(dd))) {
@@ -1330,36 +1368,36 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       art_[w.h(W_A_HINV) + dd] = w.dof_hinv(l, dd - w.link_dof(l));
     }
   }
-  for (uint32_t c = tid; c < nc; c += BS) {
+  for (uint32_t c = tid; c < nc; c += NT) {
     uint32_t h = sched[S.CSLOT + c];
     if (h >= UH) continue;
     sched[S.WALK + sched[S.UWOFF + h] + sched[S.CLOCAL + c]] = h >= MAXU ? 0xffffffffu : c;
   }
   if (tid == 0) { sched[S.META + 0] = nu; sched[S.META + 2] = nc; }
-  for (uint32_t i = tid; i < nu; i += BS) sched[S.USTART + i] = sched[S.USIZE + i];
-  __syncthreads();
-  uint32_t tot = block_scan_array<BS>(sched + S.USTART, nu, s_w);
+  for (uint32_t i = tid; i < nu; i += NT) sched[S.USTART + i] = sched[S.USIZE + i];
+  tm.sync();
+  uint32_t tot = tm.scan(sched + S.USTART, nu);
   if (tid == 0) sched[S.USTART + nu] = tot;
-  __syncthreads();
+  tm.sync();
   // ---- Phase 5: listing and body-pair reduction
-  for (uint32_t c = tid; c < nc; c += BS) {
+  for (uint32_t c = tid; c < nc; c += NT) {
     uint32_t slot = sched[S.CSLOT + c];
     if (slot >= 0xfffffffeu || slot >= MAXU) continue;
     sched[S.CORDER + sched[S.USTART + slot] + sched[S.CLOCAL + c]] = c;
   }
-  __syncthreads();
+  tm.sync();
   const uint32_t nl = min(tot, MAXC);
   // Each listed contact's reduction inputs, once: group keys to RK0-2, (pos, gap, position identity) to the
   // solve's eigenbasis table (unused until the solve).
   float4* bpx = (float4*)(sched + S.FPX);
-  for (uint32_t p = tid; p < nl; p += BS) {
+  for (uint32_t p = tid; p < nl; p += NT) {
     Prep::BPC g = P.bp_contact(sched[S.CORDER + p]);
     sched[S.RK0 + p] = g.k0; sched[S.RK1 + p] = g.k1; sched[S.RK2 + p] = g.k2;
     bpx[p] = make_float4(g.pos.x, g.pos.y, g.gap, ubits(g.pk));
   }
-  for (uint32_t sp = tid; sp < nu; sp += BS) sched[S.OSTART + sp] = sched[S.USTART + sp];
-  __syncthreads();
-  for (uint32_t p = tid; p < nl; p += BS) {
+  for (uint32_t sp = tid; sp < nu; sp += NT) sched[S.OSTART + sp] = sched[S.USTART + sp];
+  tm.sync();
+  for (uint32_t p = tid; p < nl; p += NT) {
     uint32_t st = sched[S.USTART + P.unit_of(p, nu)];
     uint32_t k0 = sched[S.RK0 + p], k1 = sched[S.RK1 + p], k2 = sched[S.RK2 + p];
     uint32_t lead = p;
@@ -1368,8 +1406,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     sched[S.LEAD + p] = lead;
     for (uint32_t q = 0; q < 26; q++) sched[S.SLOT + p * 26 + q] = 0;
   }
-  __syncthreads();
-  for (uint32_t p = tid; p < nl; p += BS) {
+  tm.sync();
+  for (uint32_t p = tid; p < nl; p += NT) {
     const float4 g = bpx[p];
     uint32_t sl = S.SLOT + sched[S.LEAD + p] * 26;
     for (uint32_t q = 0; q < 13; q++) {
@@ -1377,8 +1415,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       if (v.y != 0) atomicMax(&sched[sl + q * 2], v.x);
     }
   }
-  __syncthreads();
-  for (uint32_t p = tid; p < nl; p += BS) {
+  tm.sync();
+  for (uint32_t p = tid; p < nl; p += NT) {
     const float4 g = bpx[p];
     uint32_t sl = S.SLOT + sched[S.LEAD + p] * 26;
     for (uint32_t q = 0; q < 13; q++) {
@@ -1386,8 +1424,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       if (v.y != 0 && v.x == sched[sl + q * 2]) atomicMax(&sched[sl + q * 2 + 1], v.y);
     }
   }
-  __syncthreads();
-  for (uint32_t p = tid; p < nl; p += BS) {
+  tm.sync();
+  for (uint32_t p = tid; p < nl; p += NT) {
     const float4 g = bpx[p];
     uint32_t sl = S.SLOT + sched[S.LEAD + p] * 26;
     bool win = false;
@@ -1397,8 +1435,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     }
     sched[S.HEADS + p] = (win || w.h(W_COMPL)) ? 1u : 0u;  // compliant contacts stay whole
   }
-  __syncthreads();
-  for (uint32_t sp = tid; sp < nu; sp += BS) {
+  tm.sync();
+  for (uint32_t sp = tid; sp < nu; sp += NT) {
     uint32_t n = sched[S.USIZE + sp], st = sched[S.USTART + sp], kept = 0;
     for (uint32_t j = 0; j < n; j++) {
       if (st + j >= MAXC) break;
@@ -1406,62 +1444,64 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     }
     sched[S.USIZE + sp] = kept;
   }
-  __syncthreads();
-  for (uint32_t i = tid; i < nu; i += BS) sched[S.USTART + i] = sched[S.USIZE + i];
-  __syncthreads();
-  tot = block_scan_array<BS>(sched + S.USTART, nu, s_w);
+  tm.sync();
+  for (uint32_t i = tid; i < nu; i += NT) sched[S.USTART + i] = sched[S.USIZE + i];
+  tm.sync();
+  tot = tm.scan(sched + S.USTART, nu);
   if (tid == 0) sched[S.USTART + nu] = tot;
-  __syncthreads();
+  tm.sync();
   // ---- Phase 6: rows, friction patches
   const uint32_t total = min(tot, MAXC);
-  for (uint32_t idx = tid; idx < total; idx += BS) P.build_row(idx, nu);
-  __syncthreads();
+  for (uint32_t idx = tid; idx < total; idx += NT) P.build_row(idx, nu);
+  tm.sync();
   if (w.h(W_TORS)) {
-    for (uint32_t sp = tid; sp < nu; sp += BS) P.build_torsion(sp);
-    __syncthreads();
+    for (uint32_t sp = tid; sp < nu; sp += NT) P.build_torsion(sp);
+    tm.sync();
   }
   if (PATCH) {
-    for (uint32_t sp = tid; sp < nu; sp += BS) P.build_patch(sp);
-    __syncthreads();
+    for (uint32_t sp = tid; sp < nu; sp += NT) P.build_patch(sp);
+    tm.sync();
   }
   // ---- Phase 6c: first-fit coloring in walk order (serial by definition, wgsl.js:2282-2363)
   const uint32_t totp = total;
-  for (uint32_t c = tid; c < nc; c += BS) sched[S.ROWOF + c] = 0xffffffffu;
-  for (uint32_t sp = tid; sp < nu; sp += BS) { sched[S.UTC + sp] = 0; w.sch64(S.UTPW)[sp] = 0; }
-  __syncthreads();
-  for (uint32_t r = tid; r < totp; r += BS) {
+  for (uint32_t c = tid; c < nc; c += NT) sched[S.ROWOF + c] = 0xffffffffu;
+  for (uint32_t sp = tid; sp < nu; sp += NT) { sched[S.UTC + sp] = 0; w.sch64(S.UTPW)[sp] = 0; }
+  tm.sync();
+  for (uint32_t r = tid; r < totp; r += NT) {
     uint32_t sp = P.unit_of(r, nu);
     uint32_t c = sched[S.KEPT + sched[S.OSTART + sp] + r - sched[S.USTART + sp]];
     sched[S.RUNIT + r] = sp; sched[S.RCON + r] = c;
     if (c < nc) sched[S.ROWOF + c] = r;
   }
-  __syncthreads();
-  // First-fit in walk order.  Lanes first gather each walk entry's row and body pair into LDS.  With the
-  // masks in LDS, wave 0 colors 64 entries at a time in rounds: an entry is colored once no earlier entry
-  // still pending shares one of its bodies (the lowest pending lane of each body wins an atomicMin), so
-  // every entry sees exactly the masks the serial walk gives it and entries colored together touch disjoint
-  // bodies.  Otherwise lane 0 colors serially from global masks.  Per-color counts, unit tags (sums and
-  // maxima) and the color-ordered row list (a stable scatter) are then built in parallel.
+  tm.sync();
+  // First-fit in walk order.  With the masks in LDS (a block team whose world fits), lanes first gather each walk
+  // entry's row and body pair into LDS and wave 0 colors 64 entries at a time in rounds: an entry is colored once no
+  // earlier entry still pending shares one of its bodies (the lowest pending lane of each body wins an atomicMin), so
+  // every entry sees exactly the masks the serial walk gives it and entries colored together touch disjoint bodies.
+  // With the masks in global memory, one workgroup's lane 0 colors serially; a grid team's first workgroup does as
+  // wave 0 does over windows of PREP_CPER * BS entries, each thread holding PREP_CPER of them, with the first-pending
+  // slots in LINKHIT (free after the link responses).  Per-color counts, unit tags (sums and maxima) and the color-ordered row list (a stable scatter, or
+  // for a grid team a stable sort by color) are then built in parallel.
   const uint32_t nbm = max(NB, 1u);
-  const bool mask_lds = nbm * (NCOLW + 1) <= cap;
+  const bool mask_lds = !TM::GRID && nbm * (NCOLW + 1) <= cap;
   uint32_t* LM = s_pdyn;                // nbm * NCOLW color masks
   uint32_t* OWN = s_pdyn + nbm * NCOLW; // nbm first-pending-lane slots
-  if (mask_lds) for (uint32_t i = tid; i < nbm * NCOLW; i += BS) LM[i] = 0;
-  const bool wc_lds = NCOL + 2 <= PREP_WC;
+  if (mask_lds) for (uint32_t i = tid; i < nbm * NCOLW; i += NT) LM[i] = 0;
+  const bool wc_lds = !TM::GRID && NCOL + 2 <= PREP_WC;
   uint32_t* WC = wc_lds ? s_wc : sched + S.WCOUNT;
-  if (wc_lds) for (uint32_t i = tid; i < NCOL + 2; i += BS) s_wc[i] = 0;
+  if (wc_lds) for (uint32_t i = tid; i < NCOL + 2; i += NT) s_wc[i] = 0;
   const uint32_t ncu = min(nwalk, MAXC);
-  for (uint32_t c0 = 0; c0 < ncu; c0 += PREP_CH) {
-    const uint32_t n = min(PREP_CH, ncu - c0);
-    __syncthreads();
-    for (uint32_t i = tid; i < n; i += BS) {
-      uint32_t wc = sched[S.WALK + c0 + i];
-      uint32_t r = wc < MAXC ? sched[S.ROWOF + min(wc, MAXC - 1)] : 0xffffffffu;
-      s_wr[i] = r;
-      s_wk[i] = r < totp ? w.sch64(S.UKEY)[sched[S.RUNIT + min(r, MAXC - 1)]] : 0ull;
-    }
-    __syncthreads();
-    if (mask_lds) {
+  if (mask_lds) {
+    for (uint32_t c0 = 0; c0 < ncu; c0 += PREP_CH) {
+      const uint32_t n = min(PREP_CH, ncu - c0);
+      tm.sync();
+      for (uint32_t i = tid; i < n; i += NT) {
+        uint32_t wc = sched[S.WALK + c0 + i];
+        uint32_t r = wc < MAXC ? sched[S.ROWOF + min(wc, MAXC - 1)] : 0xffffffffu;
+        s_wr[i] = r;
+        s_wk[i] = r < totp ? w.sch64(S.UKEY)[sched[S.RUNIT + min(r, MAXC - 1)]] : 0ull;
+      }
+      tm.sync();
       if (tid < 64) {
         const uint32_t lane = tid;
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
@@ -1483,17 +1523,79 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
           }
         }
       }
-    } else if (tid == 0) {
-      for (uint32_t i = 0; i < n; i++) {
-        uint32_t r = s_wr[i];
-        if (r >= totp) continue;
-        const uint64_t key = s_wk[i];
-        sched[S.PCOL + r] = Prep::first_free_color(sched + S.WMASK, NCOLW, NCOL, key_a(key), key_b(key));
+    }
+  } else if (!TM::GRID) {
+    // masks in global memory, one workgroup: lane 0 colors serially, a chunk of walk entries at a time
+    for (uint32_t c0 = 0; c0 < ncu; c0 += PREP_CH) {
+      const uint32_t n = min(PREP_CH, ncu - c0);
+      tm.sync();
+      for (uint32_t i = tid; i < n; i += NT) {
+        uint32_t wc = sched[S.WALK + c0 + i];
+        uint32_t r = wc < MAXC ? sched[S.ROWOF + min(wc, MAXC - 1)] : 0xffffffffu;
+        s_wr[i] = r;
+        s_wk[i] = r < totp ? w.sch64(S.UKEY)[sched[S.RUNIT + min(r, MAXC - 1)]] : 0ull;
+      }
+      tm.sync();
+      if (tid == 0) {
+        for (uint32_t i = 0; i < n; i++) {
+          uint32_t r = s_wr[i];
+          if (r >= totp) continue;
+          const uint64_t key = s_wk[i];
+          sched[S.PCOL + r] = Prep::first_free_color(sched + S.WMASK, NCOLW, NCOL, key_a(key), key_b(key));
+        }
+      }
+    }
+  } else {
+    uint32_t* GO = sched + S.LINKHIT;   // first pending walk index of each body (0xffffffff: none)
+    uint32_t* GM = sched + S.WMASK;
+    for (uint32_t i = tid; i < nbm; i += NT) GO[i] = 0xffffffffu;
+    tm.sync();
+    if (blockIdx.x == 0) {
+      constexpr uint32_t PER = PREP_CPER;
+      const uint32_t lt = threadIdx.x;
+      for (uint32_t c0 = 0; c0 < ncu; c0 += PER * BS) {
+        uint32_t rr[PER];
+        int aa[PER], bb[PER];
+        uint32_t pend = 0;
+        for (uint32_t k = 0; k < PER; k++) {
+          const uint32_t i = c0 + k * BS + lt;
+          const uint32_t wc = i < ncu ? sched[S.WALK + i] : 0xffffffffu;
+          const uint32_t r = wc < MAXC ? sched[S.ROWOF + min(wc, MAXC - 1)] : 0xffffffffu;
+          const uint64_t key = r < totp ? w.sch64(S.UKEY)[sched[S.RUNIT + min(r, MAXC - 1)]] : 0ull;
+          rr[k] = r; aa[k] = key_a(key); bb[k] = key_b(key);
+          if (r < totp) pend |= 1u << k;
+        }
+        while (__syncthreads_or(pend != 0)) {
+          for (uint32_t k = 0; k < PER; k++)
+            if ((pend >> k) & 1u) {
+              const uint32_t i = c0 + k * BS + lt;
+              if (aa[k] >= 0) atomicMin(&GO[aa[k]], i);
+              if (bb[k] >= 0) atomicMin(&GO[bb[k]], i);
+            }
+          __syncthreads();
+          uint32_t bid = pend;
+          for (uint32_t k = 0; k < PER; k++)
+            if ((pend >> k) & 1u) {
+              const uint32_t i = c0 + k * BS + lt;
+              if ((aa[k] < 0 || __hip_atomic_load(&GO[aa[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == i) &&
+                  (bb[k] < 0 || __hip_atomic_load(&GO[bb[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == i)) {
+                sched[S.PCOL + rr[k]] = Prep::first_free_color(GM, NCOLW, NCOL, aa[k], bb[k]);
+                pend &= ~(1u << k);
+              }
+            }
+          __syncthreads();
+          for (uint32_t k = 0; k < PER; k++)
+            if ((bid >> k) & 1u) {
+              if (aa[k] >= 0) GO[aa[k]] = 0xffffffffu;
+              if (bb[k] >= 0) GO[bb[k]] = 0xffffffffu;
+            }
+          __syncthreads();
+        }
       }
     }
   }
-  __syncthreads();
-  for (uint32_t i = tid; i < ncu; i += BS) {
+  tm.sync();
+  for (uint32_t i = tid; i < ncu; i += NT) {
     uint32_t wc = sched[S.WALK + i];
     uint32_t r = wc < MAXC ? sched[S.ROWOF + min(wc, MAXC - 1)] : 0xffffffffu;
     if (r >= totp) continue;
@@ -1501,8 +1603,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     atomicAdd(&WC[col], 1u);
     atomicMax((unsigned long long*)(w.sch64(S.UTPW) + sched[S.RUNIT + r]), ((unsigned long long)(col + 1) << 32) | wc);
   }
-  __syncthreads();
-  for (uint32_t sp = tid; sp < nu; sp += BS) {
+  tm.sync();
+  for (uint32_t sp = tid; sp < nu; sp += NT) {
     const uint64_t packed = w.sch64(S.UTPW)[sp];
     sched[S.UTC + sp] = (uint32_t)(packed >> 32);
     sched[S.UTP + sp] = packed != 0 ? sched[S.ROWOF + (uint32_t)packed] : 0xffffffffu;
@@ -1517,10 +1619,16 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     sched[S.CSTART + NCOL + 1] = acc;
     sched[S.META + 1] = ncol;
   }
-  __syncthreads();
-  // Rows in increasing order within each color: wave 0 takes 64 rows at a time; a lane's slot is its color's
-  // running offset plus the number of lower lanes of the same color.
-  if (tid < 64) {
+  tm.sync();
+  // Rows in increasing order within each color.  A grid team sorts the rows by color (stable, so rows ascend
+  // within a color); in a block team wave 0 takes 64 rows at a time and a lane's slot is its color's running
+  // offset plus the number of lower lanes of the same color.
+  if constexpr (TM::GRID) {
+    for (uint32_t r = tid; r < totp; r += NT) { wd.skey[r] = sched[S.PCOL + r]; wd.sval[r] = r; }
+    tm.sync();
+    tm.sort(wd.skey, wd.sval, wd.skey + MAXC, wd.sval + MAXC, totp);
+    for (uint32_t i = tid; i < totp; i += NT) sched[S.PORD + i] = wd.sval[i];
+  } else if (tid < 64) {
     const uint32_t lane = tid;
     for (uint32_t r0 = 0; r0 < totp; r0 += 64) {
       const uint32_t r = r0 + lane;
@@ -1542,7 +1650,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       if (!wc_lds) __threadfence_block();
     }
   }
-  __syncthreads();
+  tm.sync();
   if (tid == 0) {
     // overflow color by contact index (wgsl.js:1740-1780)
     uint32_t o0 = min(sched[S.CSTART + NCOL], MAXC), m = min(sched[S.CSTART + NCOL + 1], MAXC) - o0;
@@ -1564,26 +1672,50 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   // ---- Phase 7: dense rows of same-articulation contacts (slot = rank in contact order), three lanes per
   // contact, 21 contacts per wave; RK0 (free after phase 5) maps slot -> contact.
   const uint32_t MAXDENSE = w.h(W_MAXDENSE);
-  for (uint32_t c = tid; c < nc; c += BS) {
+  for (uint32_t c = tid; c < nc; c += NT) {
     if (sched[S.CSLOT + c] != 0xffffffffu || sched[S.CLOCAL + c] == 0) continue;
     uint32_t slot = sched[S.DSLOT + c];
     if (slot < MAXDENSE) sched[S.RK0 + slot] = c;
   }
-  __syncthreads();
+  tm.sync();
   const uint32_t ndd = min(ndc, MAXDENSE);
-  for (uint32_t s0 = (tid >> 6) * 21; s0 < ndd; s0 += (BS / 64) * 21) {
+  for (uint32_t s0 = (tid >> 6) * 21; s0 < ndd; s0 += (NT / 64) * 21) {
     const uint32_t lane = tid & 63, sl = s0 + lane / 3, r = lane % 3;
     if (lane < 63 && sl < ndd) P.dense_row(sched[S.RK0 + sl], sl, r, lane - r);
   }
   // Loop closures, then mimic rows, in the slots after the contacts' (wgsl.js:2445-2492), a lane each.
   const uint32_t nloop = w.h(W_NLOOP), nlr = w.loop_rows();
-  for (uint32_t i = tid; i < nloop; i += BS) {
+  for (uint32_t i = tid; i < nloop; i += NT) {
     uint32_t slot = ndc;
     for (uint32_t k = 0; k < i; k++) slot += w.sf(w.h(W_SF_LOOP) + k * 5).w > 0.5f ? 2u : 1u;
     if (slot + (w.sf(w.h(W_SF_LOOP) + i * 5).w > 0.5f ? 1u : 0u) < MAXDENSE) P.loop_row(i, slot);
   }
-  for (uint32_t i = tid; i < w.h(W_NMIMIC); i += BS)
+  for (uint32_t i = tid; i < w.h(W_NMIMIC); i += NT)
     if (ndc + nlr + i < MAXDENSE) P.mimic_row(i, ndc + nlr + i);
+}
+
+
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t cap) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ TeamLds<BS> tl;
+  __shared__ PrepLds<BS> pl;
+  extern __shared__ uint32_t s_pdyn[];
+  const uint32_t wid = blockIdx.x;
+  if (wid >= nworlds) return;
+  const BlockTeam<BS> tm{threadIdx.x, &tl, nullptr};
+  prepare_world<BS>(tm, worlds[wid], Hs, cap, s_pdyn, &pl);
+}
+
+// One large world over a cooperative launch (g: grid team scratch).
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_prepare_grid(const WorldDev* __restrict__ world, uint32_t* g) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ TeamLds<BS> tl;
+  __shared__ PrepLds<BS> pl;
+  __shared__ SortLds<BS> sl;
+  const GridTeam<BS> tm{gridDim.x * BS, blockIdx.x * BS + threadIdx.x, &tl, &sl, g};
+  prepare_world<BS>(tm, world[0], Hs, 0, nullptr, &pl);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2845,30 +2977,30 @@ DEV ArtLds<TWL> art_lds(const Solver& sv) {
   return m;
 }
 
-template <int BS>
-DEV void dense_phase(const Solver& sv, uint32_t mode, uint32_t nd) {
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, NA = sv.w.NA();
-  for (uint32_t base = 0; base < NA; base += BS / 64) {
+template <int BS, class TM>
+DEV void dense_phase(const TM& tm, const Solver& sv, uint32_t mode, uint32_t nd) {
+  const uint32_t tid = tm.tid, NT = tm.NT, lane = tid & 63, wave = tid >> 6, NA = sv.w.NA();
+  for (uint32_t base = 0; base < NA; base += NT / 64) {
     const uint32_t a = base + wave;
     bool touched = false;
     if (a < NA) touched = sv.art_dense(a, mode, nd, lane);
-    __syncthreads();
+    tm.sync();
     if (a < NA && touched) sv.art_twists(a, lane);
-    __syncthreads();
+    tm.sync();
   }
 }
 
 // FAST: the default-path rows go through solve_row with the twists in LDS (TWL) or global memory.
-template <int BS, bool FAST = false, bool TWL = false>
-DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint32_t nd) {
+template <int BS, bool FAST = false, bool TWL = false, class TM>
+DEV void sweep(const TM& tm, const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint32_t nd) {
   const W& w = sv.w;
-  const uint32_t tid = threadIdx.x, NA = w.NA(), NCOL = w.NCOL(), PO = w.PO();
+  const uint32_t tid = tm.tid, NT = tm.NT, NA = w.NA(), NCOL = w.NCOL(), PO = w.PO();
   const uint32_t* sched = w.d.sched;
   const SchedLayout& S = w.S;
   const uint32_t lane = tid & 63, wave = tid >> 6;
   const bool ext = w.h(W_FMODE) || w.h(W_ANG) || w.h(W_TORS) || w.h(W_COMPL);
   // Out-of-line calls spill the caller's live registers: call only for articulations with joint rows.
-  for (uint32_t a = wave; a < NA; a += BS / 64)
+  for (uint32_t a = wave; a < NA; a += NT / 64)
     if (!FAST || w.d.art[w.h(W_A_RLN) + a] != 0.0f) sv.art_propagate_rows(a, lane);
   if (w.h(W_PATCH) && mode == 1) {
     if (FAST) {
@@ -2877,7 +3009,7 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
       const AS_GLOBAL float* rw = (const AS_GLOBAL float*)w.d.rows;
       AS_GLOBAL float* ul = (AS_GLOBAL float*)(w.d.art + w.h(W_A_ULOAD));
       const uint32_t RW = w.ROWS() * 4;
-      for (uint32_t us = tid; us < nu; us += BS) {
+      for (uint32_t us = tid; us < nu; us += NT) {
         const uint32_t s0 = ust[us], n = usz[us];
         float load = 0.0f;
         for (uint32_t m = s0; m < s0 + n; m++) {
@@ -2887,15 +3019,15 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
         ul[us] = load;
       }
     } else
-    for (uint32_t us = tid; us < nu; us += BS) {
+    for (uint32_t us = tid; us < nu; us += NT) {
       uint32_t s0 = sched[S.USTART + us];
       float load = 0.0f;
       for (uint32_t m = s0; m < s0 + sched[S.USIZE + us]; m++) if (sv.row(m, PO).w != 1.0f) load += sv.row(m, 6).x;
       w.d.art[w.h(W_A_ULOAD) + us] = load;
     }
   }
-  __syncthreads();
-  if (!FAST || nd) dense_phase<BS>(sv, 1, nd);
+  tm.sync();
+  if (!FAST || nd) dense_phase<BS>(tm, sv, 1, nd);
   const uint4* slots = (const uint4*)(sched + S.SLOTT);
   if (FAST && !ext) {
     const RowCtx<TWL> rc = row_ctx<TWL>(sv);
@@ -2904,18 +3036,18 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
     uint32_t s = cst[0];
     for (uint32_t c = 0; c < nc; c++) {
       const uint32_t e = cst[c + 1];
-      for (uint32_t u = s + tid; u < e; u += BS) { const v4u sl = rc.slots[u]; solve_row<TWL>(rc, sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
+      for (uint32_t u = s + tid; u < e; u += NT) { const v4u sl = rc.slots[u]; solve_row<TWL>(rc, sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
       s = e;
-      __syncthreads();
+      tm.sync();
     }
   } else
   for (uint32_t c = 0; c < NCOL && c < ncol; c++) {
     uint32_t s = sched[S.CSTART + c], e = sched[S.CSTART + c + 1];
     if (ext)
-      for (uint32_t u = s + tid; u < e; u += BS) { uint4 sl = slots[u]; sv.solve_unit<true>(sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
+      for (uint32_t u = s + tid; u < e; u += NT) { uint4 sl = slots[u]; sv.solve_unit<true>(sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
     else
-      for (uint32_t u = s + tid; u < e; u += BS) { uint4 sl = slots[u]; sv.solve_unit<false>(sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
-    __syncthreads();
+      for (uint32_t u = s + tid; u < e; u += NT) { uint4 sl = slots[u]; sv.solve_unit<false>(sl.y, sl.x, (int)sl.z, (int)sl.w, mode); }
+    tm.sync();
   }
   if (tid == 0) {
     uint32_t MAXC = w.MAXC();
@@ -2925,24 +3057,24 @@ DEV void sweep(const Solver& sv, uint32_t mode, uint32_t ncol, uint32_t nu, uint
       else sv.solve_unit<false>(sl.y, sl.x, (int)sl.z, (int)sl.w, mode);
     }
   }
-  __syncthreads();
+  tm.sync();
   if (FAST) {
     const ArtLds<TWL> am = art_lds<TWL>(sv);
     const bool vlim = w.h(W_VLIM_ON) != 0;
-    for (uint32_t a = wave; a < NA; a += BS / 64) {
+    for (uint32_t a = wave; a < NA; a += NT / 64) {
       am.contact_impulses(a, lane);
       if (vlim && w.d.art[w.h(W_A_AVL) + a] > 0.5f) sv.art_velocity_limits(a, lane);
     }
   } else
-  for (uint32_t a = wave; a < NA; a += BS / 64) {
+  for (uint32_t a = wave; a < NA; a += NT / 64) {
     sv.art_contact_impulses(a, lane);
     sv.art_velocity_limits(a, lane);
   }
   if (w.h(W_VLIM_FREE)) {
-    __syncthreads();
-    for (uint32_t b = tid; b < w.NB(); b += BS) sv.body_velocity_limits(b);
+    tm.sync();
+    for (uint32_t b = tid; b < w.NB(); b += NT) sv.body_velocity_limits(b);
   }
-  __syncthreads();
+  tm.sync();
 }
 
 // One Nesterov momentum step before position sweep `it` (wgsl.js:3527-3557).  Restart test: the sum over the
@@ -3077,24 +3209,20 @@ __device__ __noinline__ void velocity_pass(const Solver& sv, uint32_t ncol, uint
   __syncthreads();
   Solver vs = sv;  // the bias-free copy; k_solve's solver is never written after set-up
   vs.vpass = true;
-  for (uint32_t it = 0; it < VI; it++) sweep<BS>(vs, 1, ncol, nu, nd);
+  const BlockTeam<BS> tm{threadIdx.x, nullptr, nullptr};
+  for (uint32_t it = 0; it < VI; it++) sweep<BS>(tm, vs, 1, ncol, nu, nd);
 }
 
 // MGL: the launch holds a world whose link records and mirror live in global memory (WorldDev::mscr); every world of
 // the launch then runs the generic-pointer sweep, the others keeping their mirror in LDS.
-template <int BS, bool TWL, bool MGL = false>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_SOLVE))) k_solve(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
-  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
-  __shared__ uint32_t s_w[BS / 64];
-  uint32_t wid = blockIdx.x;
-  if (wid >= nworlds) return;
-  const WorldDev wd = worlds[wid];
+template <int BS, bool TWL, bool MGL, class TM>
+DEV void solve_world(const TM& tm, const WorldDev wd, uint32_t* Hs, float4* s_dyn4) {
   load_header(Hs, wd);
   Solver sv;
   sv.w = make_world(Hs, wd);
   const W& w = sv.w;
   const SchedLayout& S = w.S;
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = tm.tid, NT = tm.NT;
   sv.parity = wd.ctr[2] & 1u;
   const uint32_t NB = w.NB(), NA = w.NA(), NL = w.NL(), ND = w.ND(), MAXC = w.MAXC(), NCOL = w.NCOL(), AH = w.AH(), PO = w.PO();
   uint32_t* sched = wd.sched;
@@ -3105,7 +3233,6 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const uint32_t nd = min(sched[S.META + 3], w.h(W_MAXDENSE));
   // Dynamic LDS (steel_solve_lds_bytes of the launch's largest world): link records, joint velocities,
   // with TWL the body twists and impulses, then the link mirror.
-  extern __shared__ float4 s_dyn4[];
   char* lds = (MGL && wd.mscr) ? (char*)wd.mscr : (char*)s_dyn4;
   sv.lmeta = (uint4*)lds;
   lds += (size_t)NL * 16;
@@ -3129,28 +3256,28 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   sv.lm = (float*)lds;
   // Link records and mirror offsets (exclusive prefix of mirror_floats(ndof) over links).
   uint32_t* msz = (uint32_t*)sv.lm;
-  for (uint32_t l = tid; l < NL; l += BS) {
+  for (uint32_t l = tid; l < NL; l += NT) {
     uint32_t n = w.link_ndof(l);
     sv.lmeta[l] = make_uint4((uint32_t)w.link_parent(l), n, w.link_dof(l), (uint32_t)w.link_body(l));
     msz[l] = mirror_floats(n);
   }
-  __syncthreads();
-  const uint32_t MF = block_scan_array<BS>(msz, NL, s_w);
-  for (uint32_t l = tid; l < NL; l += BS) sv.lmeta[l].y |= msz[l] << 8;
-  __syncthreads();
+  tm.sync();
+  const uint32_t MF = tm.scan(msz, NL);
+  for (uint32_t l = tid; l < NL; l += NT) sv.lmeta[l].y |= msz[l] << 8;
+  tm.sync();
   // Link tree: depth by walking to the root; children lists in decreasing link order (built in link order).
-  for (uint32_t l = tid; l < NL; l += BS) {
+  for (uint32_t l = tid; l < NL; l += NT) {
     uint32_t d = 0;
     for (int p = sv.mpar(l); p >= 0; p = sv.mpar((uint32_t)p)) d++;
     sv.ltree[l] = make_uint4(d, 0xffffffffu, 0xffffffffu, 0u);
   }
-  __syncthreads();
+  tm.sync();
   if (tid == 0)
     for (uint32_t l = 0; l < NL; l++) {
       int p = sv.mpar(l);
       if (p >= 0) { sv.ltree[l].z = sv.ltree[p].y; sv.ltree[p].y = l; }
     }
-  __syncthreads();
+  tm.sync();
   const uint32_t JMw = w.JM();
   auto mirror_link = [&](uint32_t i) {  // link holding mirror float i (largest l with offset <= i)
     uint32_t lo = 0, hi = NL - 1;
@@ -3160,20 +3287,20 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     }
     return lo;
   };
-  for (uint32_t i = tid; i < MF; i += BS) {
+  for (uint32_t i = tid; i < MF; i += NT) {
     uint32_t l = mirror_link(i), n = sv.mnd(l), src = mirror_src(i - (sv.lmeta[l].y >> 8), n, JMw);
     sv.lm[i] = src < LB ? art_[w.lo(l) + src] : 0.0f;
   }
-  for (uint32_t i = tid; i < ND; i += BS) sv.laqd[i] = art_[w.h(W_A_QDH) + i];
+  for (uint32_t i = tid; i < ND; i += NT) sv.laqd[i] = art_[w.h(W_A_QDH) + i];
   if (w.h(W_SLEEP)) {  // asleep articulations start at rest (wgsl.js:3640-3646)
-    __syncthreads();
-    for (uint32_t a = tid; a < NA; a += BS) {
+    tm.sync();
+    for (uint32_t a = tid; a < NA; a += NT) {
       if (!w.node_asleep((int)(NB + a))) continue;
       uint4 ai = w.art_info(a);
       for (uint32_t dd = ai.z; dd < ai.z + ai.w; dd++) sv.laqd[dd] = 0.0f;
     }
   }
-  for (uint32_t b = tid; b < NB; b += BS) {
+  for (uint32_t b = tid; b < NB; b += NT) {
     uint32_t kind = w.body_kind((int)b);
     sv.imp[b * 2] = make_float4(0, 0, 0, 0); sv.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
     // Bodies other than free ones: zero (static bodies keep it; link bodies get theirs from art_twists below).
@@ -3196,42 +3323,42 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     }
     sv.tw[b * 2] = F4(v, 0.0f); sv.tw[b * 2 + 1] = F4(wv, 0.0f);
   }
-  for (uint32_t i = tid; i < 2 * ND; i += BS) art_[w.h(W_A_LIM) + i] = 0.0f;
-  for (uint32_t i = tid; i < ND; i += BS) { art_[w.h(W_A_FRIC) + i] = 0.0f; art_[w.h(W_A_DRV) + i] = 0.0f; }
+  for (uint32_t i = tid; i < 2 * ND; i += NT) art_[w.h(W_A_LIM) + i] = 0.0f;
+  for (uint32_t i = tid; i < ND; i += NT) { art_[w.h(W_A_FRIC) + i] = 0.0f; art_[w.h(W_A_DRV) + i] = 0.0f; }
   const uint32_t nslot = min(sched[S.CSTART + NCOL + 1], MAXC);
   uint4* slots = (uint4*)(sched + S.SLOTT);
-  for (uint32_t u = tid; u < nslot; u += BS) {
+  for (uint32_t u = tid; u < nslot; u += NT) {
     const uint32_t r = sched[S.PORD + u], us = sched[S.RUNIT + r];
     const uint64_t key = w.sch64(S.UKEY)[us];
     slots[u] = make_uint4(r, us, (uint32_t)key_a(key), (uint32_t)key_b(key));
   }
-  __syncthreads();
-  for (uint32_t a = tid >> 6; a < NA; a += BS / 64) sv.art_twists(a, tid & 63);
-  __syncthreads();
-  dense_phase<BS>(sv, 0, nd);
-  sweep<BS, !MGL, TWL>(sv, 0, ncol, nu, nd);
+  tm.sync();
+  for (uint32_t a = tid >> 6; a < NA; a += NT / 64) sv.art_twists(a, tid & 63);
+  tm.sync();
+  dense_phase<BS>(tm, sv, 0, nd);
+  sweep<BS, !MGL, TWL>(tm, sv, 0, ncol, nu, nd);
   const uint32_t iters = w.h(W_ITERS);
   f2 kb = mk2(0.0f, 0.0f);
   for (uint32_t it = 0; it < iters; it++) {
-    if (NEST) kb = momentum_step<BS>(sv, ncol, it, kb);
-    sweep<BS, !MGL, TWL>(sv, 1, ncol, nu, nd);
+    if constexpr (!TM::GRID) if (NEST) kb = momentum_step<BS>(sv, ncol, it, kb);
+    sweep<BS, !MGL, TWL>(tm, sv, 1, ncol, nu, nd);
   }
-  if (w.h(W_VELITERS)) velocity_pass<BS>(sv, ncol, nu, nd);
+  if constexpr (!TM::GRID) if (w.h(W_VELITERS)) velocity_pass<BS>(sv, ncol, nu, nd);
   // Mutable mirror fields back to the link blocks (p, dv, v, u, tau; S, U, D^-1, COM are unchanged).
-  for (uint32_t i = tid; i < MF; i += BS) {
+  for (uint32_t i = tid; i < MF; i += NT) {
     uint32_t l = mirror_link(i), n = sv.mnd(l), f = i - (sv.lmeta[l].y >> 8);
     if (mirror_mutable(f, n)) art_[w.lo(l) + mirror_src(f, n, JMw)] = sv.lm[i];
   }
-  __syncthreads();
+  tm.sync();
   const uint32_t nrows = sched[S.USTART + nu];
-  if (w.h(W_WH)) warm_store<BS>(sv, nrows);
+  if constexpr (!TM::GRID) if (w.h(W_WH)) warm_store<BS>(sv, nrows);
   const bool PATCH = w.h(W_PATCH) != 0;
   const uint32_t PB = w.h(W_P_BODY), PP = w.h(W_P_PREV);
   if (PATCH) {
-    for (uint32_t b = tid; b < NB * 2; b += BS) wd.poses[PP + b] = wd.poses[PB + b];
-    __syncthreads();
+    for (uint32_t b = tid; b < NB * 2; b += NT) wd.poses[PP + b] = wd.poses[PB + b];
+    tm.sync();
   }
-  for (uint32_t b = tid; b < NB; b += BS) {
+  for (uint32_t b = tid; b < NB; b += NT) {
     if (w.body_kind((int)b) != 1) continue;
     if (w.node_asleep((int)b)) { wd.vel[b * 2] = make_float4(0, 0, 0, 0); wd.vel[b * 2 + 1] = make_float4(0, 0, 0, 0); continue; }
     f3 v = xyz(sv.tw[b * 2]), wv = xyz(sv.tw[b * 2 + 1]);
@@ -3244,19 +3371,19 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     wd.poses[PB + b * 2] = F4(c, 0.0f);
     wd.poses[PB + b * 2 + 1] = F4(q);
   }
-  for (uint32_t i = tid; i < ND; i += BS) {
+  for (uint32_t i = tid; i < ND; i += NT) {
     art_[w.h(W_A_QD) + i] = sv.laqd[i];
     if (w.h(W_VELITERS)) sv.laqd[i] = art_[w.h(W_A_SNAP) + NB * 8 + i];
   }
-  __syncthreads();
-  for (uint32_t l = tid; l < NL; l += BS) {
+  tm.sync();
+  for (uint32_t l = tid; l < NL; l += NT) {
     if (w.link_type(l) != JT_FREE || w.link_parent(l) >= 0) continue;
     float k = 1.0f - w.body_limits(w.link_body(l)).x * dt;
     uint32_t dd = w.h(W_A_QD) + w.link_dof(l);
     art_[dd + 3] *= k; art_[dd + 4] *= k; art_[dd + 5] *= k;
   }
   // Joint positions (per link), then FK and body poses, one wave per articulation.
-  for (uint32_t base = 0; base < NA; base += BS / 64) {
+  for (uint32_t base = 0; base < NA; base += NT / 64) {
     const uint32_t a = base + (tid >> 6), k = tid & 63;
     if (a >= NA) continue;
     const uint4 ai = w.art_info(a);
@@ -3287,9 +3414,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
     }
   }
   if (PATCH) {
-    __syncthreads();
+    tm.sync();
     uint32_t ab = (1 - sv.parity) * AH;
-    for (uint32_t i = tid; i < nrows; i += BS) {
+    for (uint32_t i = tid; i < nrows; i += NT) {
       if (sv.row(i, PO).w != 2.0f) continue;
       const uint32_t pair = fbits(sv.row(i, PO + 3).w);  // anchor-table pair id; the bodies from the row's unit
       const uint64_t ukey = w.sch64(S.UKEY)[sched[S.RUNIT + i]];
@@ -3332,7 +3459,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       }
     }
   }
-  __syncthreads();
+  tm.sync();
   if (tid == 0) {
     uint32_t* st = wd.stats;
     st[1] = max(st[1], wd.ctr[1]);
@@ -3358,6 +3485,27 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
       wd.ctr[5] = (uint32_t)__double2hiint(ph);
     }
   }
+}
+
+template <int BS, bool TWL, bool MGL = false>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_SOLVE))) k_solve(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ TeamLds<BS> tl;
+  extern __shared__ float4 s_dyn4[];
+  const uint32_t wid = blockIdx.x;
+  if (wid >= nworlds) return;
+  const BlockTeam<BS> tm{threadIdx.x, &tl, nullptr};
+  solve_world<BS, TWL, MGL>(tm, worlds[wid], Hs, s_dyn4);
+}
+
+// One large world over a cooperative launch: twists, impulses and the link mirror in global memory (g: grid team
+// scratch).
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_SOLVE))) k_solve_grid(const WorldDev* __restrict__ world, uint32_t* g) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ TeamLds<BS> tl;
+  const GridTeam<BS> tm{gridDim.x * BS, blockIdx.x * BS + threadIdx.x, &tl, nullptr, g};
+  solve_world<BS, false, true>(tm, world[0], Hs, nullptr);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -3402,6 +3550,57 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
       if (twl) hipLaunchKernelGGL((k_solve<256, true>), grid, dim3(256), solve_lds, stream, worlds, nworlds);
       else hipLaunchKernelGGL((k_solve<256, false>), grid, dim3(256), solve_lds, stream, worlds, nworlds);
     }
+  }
+  if (marks) (void)hipEventRecord(marks[3], stream);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// One large world over the whole device: cooperative launches of the stage bodies with a grid team.
+// ------------------------------------------------------------------------------------------------
+static constexpr int GRID_BS = 512;
+
+hipError_t steel_grid_groups(uint32_t cus, uint32_t* groups) {
+  const void* ks[] = {(const void*)k_broad_grid<GRID_BS>, (const void*)k_csort_grid<GRID_BS>, (const void*)k_prepare_grid<GRID_BS>,
+                      (const void*)k_solve_grid<GRID_BS>};
+  int m = 1 << 20;
+  for (const void* k : ks) {
+    int x = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&x, k, GRID_BS, 0);
+    if (e != hipSuccess) return e;
+    m = x < m ? x : m;
+  }
+  if (m < 1) return hipErrorInvalidConfiguration;
+  uint32_t gr = (uint32_t)m * cus;
+  *groups = gr < GRID_MAX_GROUPS ? gr : GRID_MAX_GROUPS;
+  return hipSuccess;
+}
+
+hipError_t steel_launch_substep_grid(const WorldDev* world, uint32_t groups, uint32_t* g, uint32_t* pair_offs, uint32_t narrow_groups,
+                                     hipStream_t stream, uint32_t stages, hipEvent_t* marks) {
+  void* args[] = {(void*)&world, (void*)&g};
+  const dim3 grid(groups), block(GRID_BS);
+  hipError_t e = hipSuccess;
+  if (marks) (void)hipEventRecord(marks[0], stream);
+  if (stages & STAGE_COLLIDE) {
+    e = hipLaunchCooperativeKernel((const void*)k_broad_grid<GRID_BS>, grid, block, args, 0, stream);
+    if (e != hipSuccess) return e;
+    if (stages & STAGE_NARROW) {
+      hipLaunchKernelGGL((k_pair_offsets<1024>), dim3(1), dim3(1024), 0, stream, world, 1u, pair_offs);
+      hipLaunchKernelGGL(k_narrow, dim3(narrow_groups), dim3(256), 0, stream, world, 1u, (const uint32_t*)pair_offs);
+      e = hipLaunchCooperativeKernel((const void*)k_csort_grid<GRID_BS>, grid, block, args, 0, stream);
+      if (e != hipSuccess) return e;
+    }
+  }
+  if (marks) (void)hipEventRecord(marks[1], stream);
+  if (stages & STAGE_PREPARE) {
+    e = hipLaunchCooperativeKernel((const void*)k_prepare_grid<GRID_BS>, grid, block, args, 0, stream);
+    if (e != hipSuccess) return e;
+  }
+  if (marks) (void)hipEventRecord(marks[2], stream);
+  if (stages & STAGE_SOLVE) {
+    e = hipLaunchCooperativeKernel((const void*)k_solve_grid<GRID_BS>, grid, block, args, 0, stream);
+    if (e != hipSuccess) return e;
   }
   if (marks) (void)hipEventRecord(marks[3], stream);
   return hipGetLastError();

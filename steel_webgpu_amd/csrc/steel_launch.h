@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "steel_dev.h"
+#include "steel_team.cuh"
 
 enum : uint32_t {
   STAGE_COLLIDE = 1u,   // shapes + broadphase (+ narrowphase and contact sort with STAGE_NARROW)
@@ -68,3 +69,10 @@ inline uint32_t steel_narrow_groups(uint32_t cus) { return cus * 4; }
 // marks (optional): four events recorded before collide, prepare, solve and after solve.
 hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32_t block, const LaunchLds& sl, uint32_t* pair_offs,
                                 uint32_t narrow_groups, hipStream_t stream, uint32_t stages, hipEvent_t* marks = nullptr);
+
+// One large world over the whole device (cooperative launches of the stage kernels, GRID_BS threads per workgroup).
+// steel_grid_groups: workgroups per launch that can all be resident on a device of `cus` compute units.
+// g: grid team scratch of grid_scratch_words(groups) words, zeroed once.
+hipError_t steel_grid_groups(uint32_t cus, uint32_t* groups);
+hipError_t steel_launch_substep_grid(const WorldDev* world, uint32_t groups, uint32_t* g, uint32_t* pair_offs, uint32_t narrow_groups,
+                                     hipStream_t stream, uint32_t stages, hipEvent_t* marks = nullptr);

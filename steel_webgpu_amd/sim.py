@@ -29,7 +29,7 @@ EXPORTS = ["steel_last_error", "steel_device_open", "steel_device_close", "steel
            "steel_batch_device_bytes", "steel_estimate_bytes", "steel_batch_step_timed", "steel_batch_profile_kernels",
            "steel_server_create", "steel_server_destroy", "steel_server_configure_lane", "steel_server_world_bytes",
            "steel_server_admit", "steel_server_release", "steel_server_step_lane", "steel_server_lane_poll", "steel_server_read", "steel_server_write",
-           "steel_server_info", "steel_world_step_noalloc", "steel_batch_autotune"]
+           "steel_server_info", "steel_world_step_noalloc", "steel_batch_autotune", "steel_batch_set_wide"]
 
 
 class UnsupportedError(RuntimeError):
@@ -88,6 +88,7 @@ def load_library(path=LIB_PATH):
         "steel_batch_buffer_size": ([V, U32, U32, ctypes.POINTER(SZ)], ctypes.c_int),
         "steel_batch_stats": ([V, U32, ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "steel_batch_configure": ([V, U32, ctypes.c_int], ctypes.c_int),
+        "steel_batch_set_wide": ([V, ctypes.c_int], ctypes.c_int),
         "steel_batch_device_bytes": ([V, ctypes.POINTER(SZ)], ctypes.c_int),
         "steel_estimate_bytes": ([U32, ctypes.POINTER(WorldDesc), ctypes.POINTER(SZ)], ctypes.c_int),
         "steel_batch_step_timed": ([V, U32, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
@@ -192,6 +193,10 @@ class Batch:
 
     def configure(self, block=256, graph=True):
         _check(self.lib.steel_batch_configure(self.handle, block, 1 if graph else 0))
+
+    def set_wide(self, enable=True):
+        """steel_batch_set_wide: step this batch's one large world with device-wide (cooperative) launches."""
+        _check(self.lib.steel_batch_set_wide(self.handle, 1 if enable else 0))
 
     def autotune(self, frames=5, remeasure=False):
         """steel_batch_autotune: the fastest (workgroup size, narrowphase grid) over `frames` frames from the current

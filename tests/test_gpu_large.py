@@ -126,3 +126,57 @@ def test_body_ids_past_16_bits_bitexact(sim):
     assert np.array_equal(_bits(stack_w), _bits(stack_s))
     assert b.stats_raw(0)[0][1] > 0
     b.destroy()
+
+
+@pytest.mark.skip(reason="device-wide stepping is switched off: its cooperative launches faulted the GPU twice and the "
+                         "cause is not yet found")
+def test_device_wide_step_equals_the_one_workgroup_step(sim, lattice):
+    """The lattice stepped by device-wide cooperative launches (grid teams) matches the one-workgroup-per-stage step
+    bit for bit, from rest and from a piled state, and its last substep's pairs, contacts and colors match the
+    oracle's."""
+    pw = lattice
+    S = pw.h("W_SUBSTEPS")
+    a = sim.Batch([pw], block=256, graph=False)
+    b = sim.Batch([pw], block=256, graph=False)
+    b.set_wide(True)
+    for k in range(2):
+        a.step(5)
+        b.step(5)
+        for q in ("poses", "vel", "art"):
+            assert np.array_equal(_bits(a.read(0, q)), _bits(b.read(0, q))), (k, q)
+    for _ in range(3):
+        a.step(100)
+    state = {q: a.read(0, q, dtype=np.uint8) for q in ("poses", "vel", "art", "anchor_keys", "anchor_vals", "counters", "stats")}
+    for q, v in state.items():
+        b.write(0, q, v)
+    o = OracleWorld(pw)
+    o.poses[:] = a.read(0, "poses")
+    o.vel[:2 * pw.NB * 4] = a.read(0, "vel")
+    o.art[:] = a.read(0, "art")
+    o.akey[:] = a.read(0, "anchor_keys", dtype=np.uint32)
+    o.aval[:] = a.read(0, "anchor_vals", dtype=np.uint8)
+    o.count[:4] = a.read(0, "counters", dtype=np.uint32)[2:6]
+    a.step(2)
+    b.step(2)
+    for q in ("poses", "vel", "art"):
+        assert np.array_equal(_bits(a.read(0, q)), _bits(b.read(0, q))), q
+    o.frames(1)
+    o.substeps(S - 1)
+    o.substeps(1, debug=True)
+    assert _same_state(b, o, pw)
+    npairs, nc, nr = _compare_last_substep(b, o, pw)
+    print(f"device-wide, lattice frame 302: {npairs} pairs, {nc} contacts, {nr} rows", file=sys.stderr)
+    a.destroy()
+    b.destroy()
+
+
+def test_device_wide_refuses_what_it_does_not_step(sim, lattice):
+    small = pack.pack_scene(workloads.quadruped_world(2))
+    b = sim.Batch([small])
+    with pytest.raises(sim.UnsupportedError):
+        b.set_wide(True)
+    b.destroy()
+    b = sim.Batch([lattice, small])
+    with pytest.raises(sim.UnsupportedError):
+        b.set_wide(True)
+    b.destroy()
