@@ -185,16 +185,15 @@ def plain_worlds(args, log):
 
 
 LARGE_FRAMES = 300
-LARGE_STATE = ("poses", "vel", "art", "anchor_keys", "anchor_vals", "counters", "stats")
 
 
 def large_world(args, procs, log):
     """The large single world (100k boxes and 12-point hulls on a jittered lattice over a 20 x 20 m floor, 256
-    quadrupeds; fps 240, 4 substeps, 300 frames).  Device-wide stepping (every stage but the narrowphase one
-    cooperative launch over the device) advances frames 0..F0-1 untimed and times the window F0..299; the
-    one-workgroup-per-stage step times --large-plain-frames frames from the window's starting state, and the two
-    are compared bit for bit after them.  Per-kernel times and algorithmic bytes come from a replay of the window.
-    The FP32 oracle times frames from the same state on one host thread for about --cpu-seconds."""
+    quadrupeds; fps 240, 4 substeps, 300 frames), from rest: frames 0..K-1 (K = --steps, at most 300) stepped one
+    workgroup per stage with 1024 threads (the value) and with 256 threads (the same per-world kernels at the batch's
+    size), compared bit for bit; device-wide stepping when the library has it on.  Per-kernel times and algorithmic
+    bytes come from a replay of the same frames.  The FP32 oracle steps the same frames on one host thread for about
+    --cpu-seconds."""
     from steel_webgpu_amd import pack, sim, workloads
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_bind import OracleWorld
@@ -205,98 +204,80 @@ def large_world(args, procs, log):
     log(f"large world: {pw.NB} bodies, {pw.NS} shapes, {pw.NA} articulations, MAXC {pw.h('W_MAXC')}; built and packed in "
         f"{time.time() - t0:.1f}s")
     K = min(args.steps, LARGE_FRAMES)
-    F0 = LARGE_FRAMES - K
     S = pw.h("W_SUBSTEPS")
     dev = sim.Device(0)
-    b = sim.Batch([pw], dev, block=256, graph=False)
-    b.set_wide(True)
-    log(f"resident {b.device_bytes() / 2**30:.2f} GiB")
-    t0 = time.perf_counter()
-    for f in range(0, F0, 20):
-        b.step(min(20, F0 - f))
+    legs, finals = {}, {}
+    for block in (1024, 256):
+        b = sim.Batch([pw], dev, block=block, graph=False)
+        if block == 1024:
+            log(f"resident {b.device_bytes() / 2**30:.2f} GiB")
         b.sync()
+        t0 = time.perf_counter()
+        fm = b.step_timed(K)
+        b.sync()
+        wall = time.perf_counter() - t0
         st = b.stats_raw(0)[0]
-        log(f"  advanced to frame {min(f + 20, F0) - 1}: {st[9]} pairs, {st[5]} contacts, {st[10]} rows, {st[6]} colors "
-            f"({time.perf_counter() - t0:.1f}s)")
-    start = {q: b.read(0, q, dtype=np.uint8) for q in LARGE_STATE}
-    parts = []
-    t0 = time.perf_counter()
-    for f0 in range(0, K, 10):
-        parts.append(b.step_timed(min(10, K - f0)))
-        log(f"  timed frames {F0 + f0}..{F0 + f0 + len(parts[-1]) - 1}: {np.mean(parts[-1]):.2f} ms/frame")
-    b.sync()
-    wall = time.perf_counter() - t0
-    fm = np.concatenate(parts)
-    wide_rate = K / wall
-    st = b.stats_raw(0)[0]
-    counts = {"pairs": int(st[9]), "contacts": int(st[5]), "rows": int(st[10]), "colors": int(st[6]), "units": int(st[11])}
-    log(f"device-wide: {wide_rate:.2f} world-steps/s ({np.median(fm):.2f} ms/frame p50); last substep {counts}")
-    # per-kernel replay of the window
-    for q, v in start.items():
-        b.write(0, q, v)
-    base = [b.stats_raw(0)[0].astype(np.int64)]
-    acc = np.zeros(3)
-    for f0 in range(0, K, 10):
-        n = min(10, K - f0)
-        acc += np.asarray(b.profile_kernels(n * S)) * n
-    kms = list(acc / K)
-    algo = algorithmic_bytes(b, [pw], base)
+        counts = {"pairs": int(st[9]), "contacts": int(st[5]), "rows": int(st[10]), "colors": int(st[6]), "units": int(st[11])}
+        legs[block] = {"world_steps_per_s": K / wall, "frame_ms": fm, "counts": counts}
+        finals[block] = [b.read(0, q, dtype=np.uint32) for q in ("poses", "vel", "art")]
+        log(f"one workgroup of {block} threads per stage: {K / wall:.3f} world-steps/s over frames 0..{K - 1} "
+            f"({np.median(fm):.1f} ms/frame p50); last substep {counts}")
+        if block == 1024:
+            b.reset()
+            base = [b.stats_raw(0)[0].astype(np.int64)]
+            acc = np.zeros(3)
+            for f in range(K):
+                acc += np.asarray(b.profile_kernels(S))
+            kms = list(acc / K)
+            algo = algorithmic_bytes(b, [pw], base)
+            log(f"kernels ms/launch collide {kms[0]:.3f} prepare {kms[1]:.3f} solve {kms[2]:.3f}; algorithmic MB/launch "
+                f"{algo[0] / 1e6:.2f} {algo[1] / 1e6:.2f} {algo[2] / 1e6:.2f}")
+        b.destroy()
+    same = all(np.array_equal(x, y) for x, y in zip(finals[1024], finals[256]))
+    wide = None
+    b = sim.Batch([pw], dev, block=256, graph=False)
+    try:
+        b.set_wide(True)
+        t0 = time.perf_counter()
+        fmw = b.step_timed(K)
+        b.sync()
+        wide = K / (time.perf_counter() - t0)
+        log(f"device-wide: {wide:.3f} world-steps/s")
+    except sim.UnsupportedError as e:
+        log(f"device-wide: not available ({e})")
+    b.destroy()
+    dev.close()
     names = ["collide", "prepare", "solve"]
     k = int(np.argmax(kms))
     achieved = algo[k] / (kms[k] * 1e-3) / 1e9
-    log(f"kernels ms/launch collide {kms[0]:.3f} prepare {kms[1]:.3f} solve {kms[2]:.3f}; algorithmic MB/launch "
-        f"{algo[0] / 1e6:.1f} {algo[1] / 1e6:.1f} {algo[2] / 1e6:.1f}")
-    # one workgroup per stage from the window's start, and the device-wide step over the same frames
-    KP = max(1, min(args.large_plain_frames, K))
-    p1 = sim.Batch([pw], dev, block=256, graph=False)
-    for q, v in start.items():
-        p1.write(0, q, v)
-    p1.sync()
-    t0 = time.perf_counter()
-    pm = p1.step_timed(KP)
-    p1.sync()
-    plain_rate = KP / (time.perf_counter() - t0)
-    for q, v in start.items():
-        b.write(0, q, v)
-    b.step(KP)
-    same = all(np.array_equal(p1.read(0, q, dtype=np.uint32), b.read(0, q, dtype=np.uint32)) for q in ("poses", "vel", "art"))
-    log(f"one workgroup per stage: {plain_rate:.3f} world-steps/s over frames {F0}..{F0 + KP - 1} "
-        f"({np.median(pm):.1f} ms/frame); bit-exact with the device-wide step: {same}")
-    p1.destroy()
-    b.destroy()
-    dev.close()
-    # CPU: the FP32 oracle from the window's state, one thread
     o = OracleWorld(pw)
-    o.poses[:] = start["poses"].view(np.float32)
-    o.vel[:start["vel"].size // 4] = start["vel"].view(np.float32)
-    o.art[:] = start["art"].view(np.float32)
-    o.akey[:] = start["anchor_keys"].view(np.uint32)
-    o.aval[:] = start["anchor_vals"]
-    o.count[:4] = start["counters"].view(np.uint32)[2:6]
     done, t0 = 0, time.perf_counter()
     while done < K and (done == 0 or time.perf_counter() - t0 < args.cpu_seconds):
         o.frames(1)
         done += 1
     cpu_s = time.perf_counter() - t0
-    log(f"FP32 oracle, one thread: {done} frames in {cpu_s:.1f}s ({done / cpu_s:.4f} world-steps/s)")
+    ok = np.array_equal(o.poses.view(np.uint32), finals[1024][0]) if done == K else None
+    log(f"FP32 oracle, one thread: {done} frames in {cpu_s:.1f}s ({done / cpu_s:.4f} world-steps/s)"
+        + (f"; bit-exact with the GPU after frame {K - 1}: {ok}" if ok is not None else ""))
+    L = legs[1024]
     return {
-        "metric": "large single world: world-steps/s, device-wide vs one workgroup per stage (FP32)", "value": round(wide_rate, 3),
-        "unit": "world-steps/s", "n_gpus": 1, "steps": K, "warmup": F0, "ms_per_step": round(wall / K * 1e3, 3),
-        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG32 scene)",
+        "metric": "large single world: world-steps/s (FP32)", "value": round(L["world_steps_per_s"], 4), "unit": "world-steps/s",
+        "n_gpus": 1, "steps": K, "warmup": 0, "ms_per_step": round(1e3 / L["world_steps_per_s"], 2), "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG32 scene)",
         "config": {"workload": "large_world", "bodies": pw.NB, "shapes": pw.NS, "articulations": pw.NA, "fps": 240, "substeps": S,
-                   "iterations": pw.h("W_ITERS"), "frames": f"{F0}..{LARGE_FRAMES - 1} (frames 0..{F0 - 1} advanced untimed)",
-                   "last_substep": counts},
-        "plain_world_steps_per_s": round(plain_rate, 4), "plain_frames": f"{F0}..{F0 + KP - 1}",
-        "speedup_vs_plain": round(wide_rate / plain_rate, 1),
-        "parity": {"bitexact_device_wide_vs_one_workgroup": bool(same), "frames": KP},
-        "latency_ms": {"p50": round(float(np.median(fm)), 3), "p99": round(float(np.percentile(fm, 99)), 3),
-                       "plain_p50": round(float(np.median(pm)), 3)},
-        "roofline": {"bound": "hbm", "kernel": names[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel_ms": [round(x, 4) for x in kms], "algorithmic_bytes_per_launch": [int(x) for x in algo]},
+                   "iterations": pw.h("W_ITERS"), "frames": f"0..{K - 1} from rest", "path": "one workgroup of 1024 threads per stage",
+                   "last_substep": L["counts"]},
+        "plain_world_steps_per_s": round(legs[256]["world_steps_per_s"], 4),
+        "plain": "the same per-world kernels at the batch's 256 threads per stage",
+        "speedup_vs_plain": round(L["world_steps_per_s"] / legs[256]["world_steps_per_s"], 2),
+        "device_wide_world_steps_per_s": wide,
+        "parity": {"bitexact_1024_vs_256_threads": bool(same), "bitexact_vs_fp32_oracle": ok, "frames": f"0..{K - 1}"},
+        "latency_ms": {"p50": round(float(np.median(L["frame_ms"])), 2), "p99": round(float(np.percentile(L["frame_ms"], 99)), 2)},
+        "roofline": {"bound": "hbm", "kernel": names[k], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel_ms": [round(x, 3) for x in kms], "algorithmic_bytes_per_launch": [int(x) for x in algo]},
         "cpu_baseline": {"value": round(done / cpu_s, 5), "unit": "world-steps/s", "cores": 1, "kind": "port",
-                         "sample": f"frames {F0}..{F0 + done - 1} of the large world from the window's GPU state, FP32 oracle, "
-                                   f"one thread ({cpu_s:.1f}s)"},
+                         "sample": f"frames 0..{done - 1} of the large world from rest, FP32 oracle, one thread ({cpu_s:.1f}s)"},
     }
 
 
@@ -486,7 +467,6 @@ def main():
     ap.add_argument("--workload", default="batch", choices=["batch", "serving", "plain", "large"])
     ap.add_argument("--large-bodies", type=int, default=100000, help="large: lattice bodies")
     ap.add_argument("--large-quads", type=int, default=256, help="large: quadrupeds")
-    ap.add_argument("--large-plain-frames", type=int, default=2, help="large: frames timed one workgroup per stage")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="plain: FP32 oracle time budget per configuration")
     ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
     ap.add_argument("--rates", type=float, nargs="+", default=[200.0, 1000.0, 5000.0], help="serving: Poisson rates (req/s)")

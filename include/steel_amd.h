@@ -101,7 +101,8 @@ int steel_batch_write(steel_batch* b, uint32_t world, uint32_t buf, size_t byte_
 int steel_batch_buffer_size(steel_batch* b, uint32_t world, uint32_t buf, size_t* bytes);
 /* Sim.stats (sim.js:822-834): the 16 stats words of one world and the GPU time of the last step. */
 int steel_batch_stats(steel_batch* b, uint32_t world, uint32_t out[16], double* gpu_ms);
-/* Launch options: workgroup size of prepare/solve (64, 128 or 256) and hipGraph capture on/off. */
+/* Launch options: workgroup size of prepare/solve (64, 128 or 256; 1024 for batches of large worlds, whose solve then
+ * keeps every world's link mirror behind generic pointers) and hipGraph capture on/off. */
 int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph);
 /* Device-wide stepping of one large world (Sim.step, sim.js:692-701, for a world past the reference's limits of
  * 256 articulations, 16-bit body ids and one prepare/solve workgroup, sim.js:159, :663): every stage but the

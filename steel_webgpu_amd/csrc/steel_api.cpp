@@ -224,7 +224,7 @@ int steel_batch_destroy(steel_batch* b) {
 
 int steel_batch_configure(steel_batch* b, uint32_t block, int use_graph) {
   if (!b) return fail(STEEL_BAD_ARGUMENT, "null batch");
-  if (block != 64 && block != 128 && block != 256) return fail(STEEL_BAD_ARGUMENT, "block must be 64, 128 or 256");
+  if (block != 64 && block != 128 && block != 256 && block != 1024) return fail(STEEL_BAD_ARGUMENT, "block must be 64, 128, 256 or 1024");
   if (b->gexec && (block != b->block || (use_graph != 0) != b->use_graph)) {
     (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr;
     (void)hipGraphDestroy(b->graph); b->graph = nullptr;

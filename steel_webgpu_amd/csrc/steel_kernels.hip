@@ -191,9 +191,9 @@ DEV bool pair_allowed(const W& w, uint32_t i, uint32_t j, uint32_t tag_i, uint32
 
 // Shapes and broadphase of one world by a team: sort shapes by the order-preserving integer of AABB min x, then
 // sweep the sorted list; pair (i, j) survives iff it passes the reference's all-pairs predicate (wgsl.js:154-177),
-// tested in full for every candidate the sweep reaches.  A block team sweeps worlds of up to `cap` shapes from LDS
-// (s_box) and larger ones shape by shape from global memory; a grid team keeps the sorted bounds and runs in the
-// world's contact buffers (free until the narrowphase) and deals the flattened candidates out over the launch.
+// tested in full for every candidate the sweep reaches.  A block team keeps the sorted bounds and runs of worlds of
+// up to `cap` shapes in LDS (s_box); larger worlds, and a grid team, keep them in the world's contact buffers (free
+// until the narrowphase).  The flattened candidates are dealt out over the team.
 // np: the team's pair counter (LDS for a block, global for a grid).
 template <int BS, class TM>
 DEV void broad_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t cap, float4* s_box, uint32_t* np) {
@@ -207,10 +207,10 @@ DEV void broad_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t cap
   for (uint32_t s = tid; s < NS; s += NT) { wd.skey[s] = fflip_u(fbits(wd.poses[PA + s * 2].x)); wd.sval[s] = s; }
   tm.sync();
   tm.sort(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), NS);
-  const bool flat = TM::GRID || NS <= cap;
-  float4* s_lo = TM::GRID ? wd.craw : s_box;
-  float4* s_hi = s_lo + (TM::GRID ? NS : cap);
-  uint32_t* s_run = TM::GRID ? (uint32_t*)wd.contacts : (uint32_t*)(s_box + 2 * cap);
+  const bool glob = TM::GRID || NS > cap;
+  float4* s_lo = glob ? wd.craw : s_box;
+  float4* s_hi = s_lo + (glob ? NS : cap);
+  uint32_t* s_run = glob ? (uint32_t*)wd.contacts : (uint32_t*)(s_box + 2 * cap);
   const uint32_t MAXP = w.MAXP();
   // Candidate (i, j), j after i in x order with min x_j <= max x_i.  The x intervals overlap (min x_i <= min
   // x_j <= max x_j), so the AABB test is y and z, then the rest of the predicate.
@@ -222,64 +222,48 @@ DEV void broad_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t cap
     uint32_t slot = atomicAdd(np, 1u);
     if (slot < MAXP) { wd.pairs[slot * 2] = a; wd.pairs[slot * 2 + 1] = b; }
   };
-  if (flat) {
-    for (uint32_t i = tid; i < NS; i += NT) {
-      uint32_t si = wd.sval[i];
-      float4 lo = wd.poses[PA + si * 2], hi = wd.poses[PA + si * 2 + 1];
-      lo.w = ubits(si);
-      hi.w = ubits(shape_tag(w, si));
-      s_lo[i] = lo; s_hi[i] = hi;
+  for (uint32_t i = tid; i < NS; i += NT) {
+    uint32_t si = wd.sval[i];
+    float4 lo = wd.poses[PA + si * 2], hi = wd.poses[PA + si * 2 + 1];
+    lo.w = ubits(si);
+    hi.w = ubits(shape_tag(w, si));
+    s_lo[i] = lo; s_hi[i] = hi;
+  }
+  tm.sync();
+  // Run length of each sorted shape (binary search for the first later min x above its max x), then the
+  // candidates are flattened by an exclusive scan and dealt out to the lanes in chunks of 8, so one shape
+  // with a long run (a ground plane spans every x) is shared by the whole team.
+  for (uint32_t i = tid; i < NS; i += NT) {
+    const float hx = s_hi[i].x;
+    uint32_t lo_ = i + 1, hi_ = NS;
+    while (lo_ < hi_) {
+      uint32_t m = (lo_ + hi_) >> 1;
+      if (s_lo[m].x > hx) hi_ = m; else lo_ = m + 1;
     }
-    tm.sync();
-    // Run length of each sorted shape (binary search for the first later min x above its max x), then the
-    // candidates are flattened by an exclusive scan and dealt out to the lanes in chunks of 8, so one shape
-    // with a long run (a ground plane spans every x) is shared by the whole team.
-    for (uint32_t i = tid; i < NS; i += NT) {
-      const float hx = s_hi[i].x;
-      uint32_t lo_ = i + 1, hi_ = NS;
-      while (lo_ < hi_) {
-        uint32_t m = (lo_ + hi_) >> 1;
-        if (s_lo[m].x > hx) hi_ = m; else lo_ = m + 1;
-      }
-      s_run[i] = lo_ - i - 1;
+    s_run[i] = lo_ - i - 1;
+  }
+  tm.sync();
+  const uint32_t T = tm.scan(s_run, NS);
+  if (tid == 0) s_run[NS] = T;
+  tm.sync();
+  constexpr uint32_t CH = 8;
+  for (uint32_t c = tid; c < (T + CH - 1) / CH; c += NT) {
+    const uint32_t k0 = c * CH, k1 = min(k0 + CH, T);
+    uint32_t lo_ = 0, hi_ = NS - 1;  // largest i with s_run[i] <= k0 (s_run[0] = 0, s_run[NS] = T > k0)
+    while (lo_ < hi_) {
+      uint32_t m = (lo_ + hi_ + 1) >> 1;
+      if (s_run[m] <= k0) lo_ = m; else hi_ = m - 1;
     }
-    tm.sync();
-    const uint32_t T = tm.scan(s_run, NS);
-    if (tid == 0) s_run[NS] = T;
-    tm.sync();
-    constexpr uint32_t CH = 8;
-    for (uint32_t c = tid; c < (T + CH - 1) / CH; c += NT) {
-      const uint32_t k0 = c * CH, k1 = min(k0 + CH, T);
-      uint32_t lo_ = 0, hi_ = NS - 1;  // largest i with s_run[i] <= k0 (s_run[0] = 0, s_run[NS] = T > k0)
-      while (lo_ < hi_) {
-        uint32_t m = (lo_ + hi_ + 1) >> 1;
-        if (s_run[m] <= k0) lo_ = m; else hi_ = m - 1;
+    uint32_t i = lo_;
+    float4 lo_i = s_lo[i], hi_i = s_hi[i];
+    for (uint32_t k = k0; k < k1; k++) {
+      if (s_run[i + 1] <= k) {
+        do { i++; } while (s_run[i + 1] <= k);
+        lo_i = s_lo[i]; hi_i = s_hi[i];
       }
-      uint32_t i = lo_;
-      float4 lo_i = s_lo[i], hi_i = s_hi[i];
-      for (uint32_t k = k0; k < k1; k++) {
-        if (s_run[i + 1] <= k) {
-          do { i++; } while (s_run[i + 1] <= k);
-          lo_i = s_lo[i]; hi_i = s_hi[i];
-        }
-        const uint32_t j = i + 1 + (k - s_run[i]);
-        const float4 lo_j = s_lo[j], hi_j = s_hi[j];
-        test(lo_i, hi_i, fbits(lo_i.w), fbits(hi_i.w), lo_j, hi_j, fbits(lo_j.w), fbits(hi_j.w));
-      }
-    }
-  } else {
-    for (uint32_t i = tid; i < NS; i += NT) {
-      const uint32_t si = wd.sval[i];
-      const float4 lo_i = wd.poses[PA + si * 2], hi_i = wd.poses[PA + si * 2 + 1];
-      const uint32_t tag_i = shape_tag(w, si);
-      for (uint32_t j = i + 1; j < NS; j++) {
-        const uint32_t sj = wd.sval[j];
-        const float4 lo_j = wd.poses[PA + sj * 2];
-        if (lo_j.x > hi_i.x) break;
-        const float4 hi_j = wd.poses[PA + sj * 2 + 1];
-        if (lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.y > hi_i.y || lo_j.z > hi_i.z) continue;
-        test(lo_i, hi_i, si, tag_i, lo_j, hi_j, sj, shape_tag(w, sj));
-      }
+      const uint32_t j = i + 1 + (k - s_run[i]);
+      const float4 lo_j = s_lo[j], hi_j = s_hi[j];
+      test(lo_i, hi_i, fbits(lo_i.w), fbits(hi_i.w), lo_j, hi_j, fbits(lo_j.w), fbits(hi_j.w));
     }
   }
   tm.sync();
@@ -3532,10 +3516,14 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
     const size_t pb = (size_t)pw * 4;
     if (block == 64) hipLaunchKernelGGL((k_prepare<64>), grid, dim3(64), pb, stream, worlds, nworlds, pw);
     else if (block == 128) hipLaunchKernelGGL((k_prepare<128>), grid, dim3(128), pb, stream, worlds, nworlds, pw);
+    else if (block == 1024) hipLaunchKernelGGL((k_prepare<1024>), grid, dim3(1024), pb, stream, worlds, nworlds, pw);
     else hipLaunchKernelGGL((k_prepare<256>), grid, dim3(256), pb, stream, worlds, nworlds, pw);
   }
   if (marks) (void)hipEventRecord(marks[2], stream);
-  if ((stages & STAGE_SOLVE) && sl.mgl) {
+  if ((stages & STAGE_SOLVE) && block == 1024) {
+    // sixteen waves per world, for large worlds: the global-mirror instantiation serves every world
+    hipLaunchKernelGGL((k_solve<1024, false, true>), grid, dim3(1024), sl.notw, stream, worlds, nworlds);
+  } else if ((stages & STAGE_SOLVE) && sl.mgl) {
     if (block == 64) hipLaunchKernelGGL((k_solve<64, false, true>), grid, dim3(64), solve_lds, stream, worlds, nworlds);
     else if (block == 128) hipLaunchKernelGGL((k_solve<128, false, true>), grid, dim3(128), solve_lds, stream, worlds, nworlds);
     else hipLaunchKernelGGL((k_solve<256, false, true>), grid, dim3(256), solve_lds, stream, worlds, nworlds);
