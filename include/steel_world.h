@@ -90,6 +90,10 @@ enum steel_world_field {
      shape, for worlds whose bitset would be large: offsets[NS + 1], then for each shape i the shapes j > i it must not
      pair with, ascending */
   W_FILT_LIST,
+  /* cell broadphase (f32; 0: the min-x sweep): shapes whose AABB extent is at most this on every axis are binned
+     in a uniform grid of cells sorted by Morton code, the rest are tested against every shape; the pair set is the
+     reference's either way */
+  W_CELL,
   W_FIELD_COUNT
 };
 

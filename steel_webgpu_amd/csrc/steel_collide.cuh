@@ -36,7 +36,7 @@ DEV f3 shfl3(f3 v, int src) { return mk3(__shfl(v.x, src, 64), __shfl(v.y, src, 
 struct ManifoldLds { f2 A[10], B[10]; int idx[4]; Mand m; };
 
 struct Collider {
-  W w;
+  WC w;
   uint32_t* s_nc;      // the world's contact counter
   ManifoldLds* mlds;   // this wave's LDS work area
 

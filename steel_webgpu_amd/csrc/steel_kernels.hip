@@ -117,7 +117,7 @@ DEV void load_header(uint32_t* Hs, const WorldDev& wd) {
 // ------------------------------------------------------------------------------------------------
 // k_collide
 // ------------------------------------------------------------------------------------------------
-DEV void shape_pose(const W& w, uint32_t s) {  // wgsl.js:117-150
+DEV void shape_pose(const WC& w, uint32_t s) {  // wgsl.js:117-150
   uint32_t o = w.so(s);
   float4 r0 = w.sf(o);
   int body = ti(r0.x);
@@ -150,7 +150,7 @@ DEV void shape_pose(const W& w, uint32_t s) {  // wgsl.js:117-150
 
 // Shape tag for the pair predicate: body index + 1 in bits 0-30, bit 31 set when the shape cannot move
 // (static shape, or static or kinematic body).
-DEV uint32_t shape_tag(const W& w, uint32_t s) {
+DEV uint32_t shape_tag(const WC& w, uint32_t s) {
   int b = ti(w.sf(w.so(s)).x);
   bool fixed = b < 0 || w.body_kind(b) == 3 || (w.body_flags(b) & 1) != 0;
   return (uint32_t)(b + 1) | (fixed ? 0x80000000u : 0u);
@@ -159,7 +159,7 @@ DEV uint32_t shape_tag(const W& w, uint32_t s) {
 // The reference's pair predicate (wgsl.js:158-172) for shapes i < j with overlapping AABBs: different
 // bodies, not both unmovable, not filtered (explicit list, joint parent/child, world and group rules, all in
 // the filter bitset).
-DEV bool pair_allowed(const W& w, uint32_t i, uint32_t j, uint32_t tag_i, uint32_t tag_j) {
+DEV bool pair_allowed(const WC& w, uint32_t i, uint32_t j, uint32_t tag_i, uint32_t tag_j) {
   if ((tag_i & 0x7fffffffu) == (tag_j & 0x7fffffffu)) return false;
   if ((tag_i & tag_j) >> 31) return false;
   const uint32_t F = w.h(W_SU_FILTER);
@@ -189,20 +189,178 @@ DEV bool pair_allowed(const W& w, uint32_t i, uint32_t j, uint32_t tag_i, uint32
 #define STEEL_WPE_SOLVE 4
 #endif
 
+// ---- cell broadphase (worlds with a W_CELL bound) ----
+// A shape is binned when its AABB extent on every axis is at most the bound HB (a NaN extent is not).  With H the
+// largest binned extent and cells of side Hc > H, two binned shapes whose AABBs overlap have min corners less than
+// Hc apart on every axis, so their cells differ by at most one on each axis; cell coordinates are counted from the
+// least binned cell and clamped to 21 bits, which keeps neighbours neighbours.  Cells are sorted by Morton code and
+// found through an open-addressing table; each binned shape tests the shapes of its own cell that follow it and of
+// the 13 cells after its own in (z, y, x) order, the shapes not binned test every shape.  Every candidate passes the
+// reference's full test (wgsl.js:154-177), so the pair set is the reference's.
+DEV bool cell_binned(const float4& lo, const float4& hi, float HB) {
+  return hi.x - lo.x <= HB && hi.y - lo.y <= HB && hi.z - lo.z <= HB;
+}
+DEV int cell_floor(float x, double Hc) {  // floor(x / Hc) clamped to [-2^30, 2^30), in fp64 (error far below the margin)
+  return (int)floor(fmin(fmax((double)x / Hc, -1073741824.0), 1073741823.0));
+}
+DEV uint64_t morton_spread(uint32_t v) {  // bit i of v (21 bits) to bit 3i
+  uint64_t x = v & 0x1fffffu;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+DEV uint32_t morton_compact(uint64_t x) {  // inverse of morton_spread
+  x &= 0x1249249249249249ull;
+  x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+  x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+  x = (x ^ (x >> 8)) & 0x1f0000ff0000ffull;
+  x = (x ^ (x >> 16)) & 0x1f00000000ffffull;
+  x = (x ^ (x >> 32)) & 0x1fffffull;
+  return (uint32_t)x;
+}
+DEV uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) { return morton_spread(x) | morton_spread(y) << 1 | morton_spread(z) << 2; }
+static constexpr uint64_t CELL_NONE = 1ull << 63;  // key of a shape not binned: after every Morton code
+// Table of 2^lg entries of two words: Morton code + 1 (0: empty), then first | end << 32 of the cell's sorted run.
+// Reads go to the device-coherent level (the keys were placed by atomics).
+DEV uint32_t cell_slot(uint64_t m, uint32_t lg) { return (uint32_t)(m ^ (m >> lg) ^ (m >> (2 * lg))) & ((1u << lg) - 1u); }
+DEV void cell_insert(uint64_t* T, uint32_t lg, uint64_t m, uint32_t first, uint32_t end) {
+  for (uint32_t e = cell_slot(m, lg);; e = (e + 1) & ((1u << lg) - 1u)) {
+    if (atomicCAS((unsigned long long*)&T[2 * e], 0ull, (unsigned long long)(m + 1)) == 0ull) {
+      atomicExch((unsigned long long*)&T[2 * e + 1], (unsigned long long)((uint64_t)first | (uint64_t)end << 32));
+      return;
+    }
+  }
+}
+DEV bool cell_find(const uint64_t* T, uint32_t lg, uint64_t m, uint32_t& first, uint32_t& end) {
+  for (uint32_t e = cell_slot(m, lg);; e = (e + 1) & ((1u << lg) - 1u)) {
+    const uint64_t k = __hip_atomic_load(&T[2 * e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == 0) return false;
+    if (k == m + 1) {
+      const uint64_t v = __hip_atomic_load(&T[2 * e + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      first = (uint32_t)v; end = (uint32_t)(v >> 32);
+      return true;
+    }
+  }
+}
+
+// aux: the team's words (LDS for a block, global for a grid), set by the caller before its first barrier:
+// [0] pair count 0, [1] largest binned extent 0, [2..4] least cell per axis 0xffffffff, [5] binned count 0.
+template <int BS, class TM>
+DEV void broad_cells(const TM& tm, const WC& w, const WorldDev& wd, uint32_t* aux, float HB) {
+  const uint32_t tid = tm.tid, NT = tm.NT, NS = w.NS(), PA = w.h(W_P_AABB), MAXP = w.MAXP();
+  for (uint32_t s = tid; s < NS; s += NT) {
+    const float4 lo = wd.poses[PA + s * 2], hi = wd.poses[PA + s * 2 + 1];
+    if (cell_binned(lo, hi, HB)) atomicMax(&aux[1], fbits(fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z)));
+  }
+  tm.sync();
+  const float H = ubits(aux[1]);
+  const double Hc = (double)(H > 0.0f ? H : HB) * 1.0078125;
+  for (uint32_t s = tid; s < NS; s += NT) {
+    const float4 lo = wd.poses[PA + s * 2], hi = wd.poses[PA + s * 2 + 1];
+    if (!cell_binned(lo, hi, HB)) continue;
+    atomicMin(&aux[2], (uint32_t)(cell_floor(lo.x, Hc) + (1 << 30)));
+    atomicMin(&aux[3], (uint32_t)(cell_floor(lo.y, Hc) + (1 << 30)));
+    atomicMin(&aux[4], (uint32_t)(cell_floor(lo.z, Hc) + (1 << 30)));
+    atomicAdd(&aux[5], 1u);
+  }
+  tm.sync();
+  const uint32_t ox = aux[2], oy = aux[3], oz = aux[4], nsm = aux[5];
+  for (uint32_t s = tid; s < NS; s += NT) {
+    const float4 lo = wd.poses[PA + s * 2], hi = wd.poses[PA + s * 2 + 1];
+    uint64_t key = CELL_NONE;
+    if (cell_binned(lo, hi, HB))
+      key = morton3(min((uint32_t)(cell_floor(lo.x, Hc) + (1 << 30)) - ox, 0x1fffffu),
+                    min((uint32_t)(cell_floor(lo.y, Hc) + (1 << 30)) - oy, 0x1fffffu),
+                    min((uint32_t)(cell_floor(lo.z, Hc) + (1 << 30)) - oz, 0x1fffffu));
+    wd.skey[s] = key; wd.sval[s] = s;
+  }
+  tm.sync();
+  tm.sort(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), NS);
+  // Sorted bounds (lo.w: shape id, hi.w: pair tag) in the contact buffer, the cell table in the other one (at most
+  // 4 NS entries of 16 bytes; both buffers hold 96 MAXC bytes and MAXC >= NS).
+  float4* s_lo = wd.craw;
+  float4* s_hi = s_lo + NS;
+  uint64_t* T = (uint64_t*)wd.contacts;
+  uint32_t lg = 1;
+  while ((1u << lg) < 2 * nsm) lg++;
+  for (uint32_t p = tid; p < NS; p += NT) {
+    const uint32_t si = wd.sval[p];
+    float4 lo = wd.poses[PA + si * 2], hi = wd.poses[PA + si * 2 + 1];
+    lo.w = ubits(si);
+    hi.w = ubits(shape_tag(w, si));
+    s_lo[p] = lo; s_hi[p] = hi;
+  }
+  for (uint32_t e = tid; e < (1u << lg); e += NT) T[2 * e] = 0;
+  tm.sync();
+  for (uint32_t p = tid; p < nsm; p += NT) {
+    const uint64_t m = wd.skey[p];
+    if (p > 0 && wd.skey[p - 1] == m) continue;
+    uint32_t q = p + 1;
+    while (q < nsm && wd.skey[q] == m) q++;
+    cell_insert(T, lg, m, p, q);
+  }
+  tm.sync();
+  auto test = [&](const float4& lo_i, const float4& hi_i, const float4& lo_j, const float4& hi_j) {
+    if (lo_i.x > hi_j.x || lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.x > hi_i.x || lo_j.y > hi_i.y || lo_j.z > hi_i.z) return;
+    const uint32_t si = fbits(lo_i.w), sj = fbits(lo_j.w), a = min(si, sj), b = max(si, sj);
+    if (!pair_allowed(w, a, b, fbits(hi_i.w), fbits(hi_j.w))) return;
+    const uint32_t slot = atomicAdd(&aux[0], 1u);
+    if (slot < MAXP) { wd.pairs[slot * 2] = a; wd.pairs[slot * 2 + 1] = b; }
+  };
+  for (uint32_t p = tid; p < nsm; p += NT) {
+    const uint64_t m = wd.skey[p];
+    const uint32_t cx = morton_compact(m), cy = morton_compact(m >> 1), cz = morton_compact(m >> 2);
+    const float4 lo_p = s_lo[p], hi_p = s_hi[p];
+    for (uint32_t t = 0; t < 14; t++) {  // own cell, then the 13 cells after it: dz = 1 (9), dz = 0 dy = 1 (3), dx = 1
+      const int dz = t == 0 ? 0 : (t <= 9 ? 1 : 0);
+      const int dy = t == 0 ? 0 : (t <= 9 ? (int)((t - 1) / 3) - 1 : (t <= 12 ? 1 : 0));
+      const int dx = t == 0 ? 0 : (t <= 9 ? (int)((t - 1) % 3) - 1 : (t <= 12 ? (int)(t - 10) - 1 : 1));
+      const uint32_t nx = cx + (uint32_t)dx, ny = cy + (uint32_t)dy, nz = cz + (uint32_t)dz;
+      if ((nx | ny | nz) > 0x1fffffu) continue;
+      uint32_t q0 = p + 1, q1 = 0;
+      if (t == 0) {
+        q1 = q0;
+        while (q1 < nsm && wd.skey[q1] == m) q1++;
+      } else if (!cell_find(T, lg, morton3(nx, ny, nz), q0, q1)) {
+        continue;
+      }
+      for (uint32_t q = q0; q < q1; q++) test(lo_p, hi_p, s_lo[q], s_hi[q]);
+    }
+  }
+  for (uint32_t b = nsm; b < NS; b++) {  // shapes not binned, against every shape (a pair of two of them once)
+    const float4 lo_b = s_lo[b], hi_b = s_hi[b];
+    for (uint32_t q = tid; q < NS; q += NT)
+      if (q < nsm || q > b) test(lo_b, hi_b, s_lo[q], s_hi[q]);
+  }
+  tm.sync();
+  if (tid == 0) { wd.ctr[0] = aux[0]; wd.ctr[1] = 0; }
+}
+
 // Shapes and broadphase of one world by a team: sort shapes by the order-preserving integer of AABB min x, then
 // sweep the sorted list; pair (i, j) survives iff it passes the reference's all-pairs predicate (wgsl.js:154-177),
 // tested in full for every candidate the sweep reaches.  A block team keeps the sorted bounds and runs of worlds of
 // up to `cap` shapes in LDS (s_box); larger worlds, and a grid team, keep them in the world's contact buffers (free
 // until the narrowphase).  The flattened candidates are dealt out over the team.
-// np: the team's pair counter (LDS for a block, global for a grid).
+// np: the team's pair counter and, for the cell broadphase, five more words (LDS for a block, global for a grid).
 template <int BS, class TM>
 DEV void broad_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t cap, float4* s_box, uint32_t* np) {
   load_header(Hs, wd);
-  W w = make_world(Hs, wd);
+  WC w = make_world<false>(Hs, wd);
   const uint32_t tid = tm.tid, NT = tm.NT, NS = w.NS();
-  if (tid == 0) *np = 0;
+  const float HB = w.hf(W_CELL);
+  if (tid == 0) {
+    np[0] = 0;
+    if (HB > 0.0f) { np[1] = 0; np[2] = np[3] = np[4] = 0xffffffffu; np[5] = 0; }
+  }
   for (uint32_t s = tid; s < NS; s += NT) shape_pose(w, s);
   tm.sync();
+  if (HB > 0.0f) {
+    broad_cells<BS>(tm, w, wd, np, HB);
+    return;
+  }
   const uint32_t PA = w.h(W_P_AABB);
   for (uint32_t s = tid; s < NS; s += NT) { wd.skey[s] = fflip_u(fbits(wd.poses[PA + s * 2].x)); wd.sval[s] = s; }
   tm.sync();
@@ -277,15 +435,15 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ TeamLds<BS> tl;
   __shared__ SortLds<BS> sl;
-  __shared__ uint32_t s_np;
+  __shared__ uint32_t s_np[6];
   extern __shared__ float4 s_box[];
   const uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
   const BlockTeam<BS> tm{threadIdx.x, &tl, &sl};
-  broad_world<BS>(tm, worlds[wid], Hs, cap, s_box, &s_np);
+  broad_world<BS>(tm, worlds[wid], Hs, cap, s_box, s_np);
 }
 
-// One large world over a cooperative launch (g: grid team scratch; g[3] counts the pairs).
+// One large world over a cooperative launch (g: grid team scratch; g[3..8] the pair counter and cell words).
 template <int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_broad_grid(const WorldDev* __restrict__ world, uint32_t* g) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
@@ -315,9 +473,21 @@ __global__ void __launch_bounds__(BS) k_pair_offsets(const WorldDev* __restrict_
 // expensive pairs from all worlds, and a world with many pairs is spread over many CUs.  Contacts take slots
 // from the world's counter ctr[1]; their order is made canonical by k_csort.
 static constexpr uint32_t NARROW_LDS_OFFS = 4097;
+// A wave's view of a world whose header it copies into its own LDS slot hs (faster than reading the header from
+// global memory through the pair loop).  Called by a whole wave; the slot's earlier reads are complete before the
+// copy overwrites it (a wave's LDS operations execute in order).
+DEV WC wave_world(uint32_t* hs, const WorldDev& wd, uint32_t lane) {
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t k = lane; k < STEEL_WORLD_HEADER_WORDS; k += 64) hs[k] = wd.hdr[k];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return make_world<false>(hs, wd);
+}
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_COLLIDE))) k_narrow(const WorldDev* __restrict__ worlds, uint32_t nworlds, const uint32_t* __restrict__ offs) {
   __shared__ uint32_t s_offs[NARROW_LDS_OFFS];
   __shared__ ManifoldLds s_man[4];
+  __shared__ uint32_t s_hdr[4][STEEL_WORLD_HEADER_WORDS];  // each wave's copy of its current world's header
   const bool lds_offs = nworlds + 1 <= NARROW_LDS_OFFS;
   if (lds_offs)
     for (uint32_t i = threadIdx.x; i <= nworlds; i += 256) s_offs[i] = offs[i];
@@ -338,14 +508,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEEL_
     uint32_t w = lo, wend = O[w + 1];
     WorldDev wd = worlds[w];
     Collider col;
-    col.w = make_world(wd.hdr, wd);
+    col.w = wave_world(s_hdr[threadIdx.x >> 6], wd, lane);
     col.s_nc = wd.ctr + 1;
     col.mlds = s_man + (threadIdx.x >> 6);
     for (uint32_t i = i0; i < i1; i++) {
       if (i >= wend) {
         do { w++; wend = O[w + 1]; } while (i >= wend);
         wd = worlds[w];
-        col.w = make_world(wd.hdr, wd);
+        col.w = wave_world(s_hdr[threadIdx.x >> 6], wd, lane);
         col.s_nc = wd.ctr + 1;
       }
       col.narrow_pair(i - O[w], lane);
@@ -359,7 +529,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEEL_
 template <int BS, class TM>
 DEV void csort_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t* tie) {
   load_header(Hs, wd);
-  W w = make_world(Hs, wd);
+  WC w = make_world<false>(Hs, wd);
   const uint32_t tid = tm.tid, NT = tm.NT;
   if (tid == 0) *tie = 0;
   const uint32_t nc = min(wd.ctr[1], w.MAXC());
@@ -3503,11 +3673,15 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
   dim3 grid(nworlds);
   if (marks) (void)hipEventRecord(marks[0], stream);
   if (stages & STAGE_COLLIDE) {
-    hipLaunchKernelGGL((k_broad<256>), grid, dim3(256), sl.broad_bytes(), stream, worlds, nworlds, sl.broad_cap());
+    // Workgroups of 1024 (large worlds) keep the broadphase's sorted bounds in global memory (cap 0): 1024 threads'
+    // sort LDS and a 4096-shape bounds cache would not fit one CU's LDS together.
+    if (block == 1024) hipLaunchKernelGGL((k_broad<1024>), grid, dim3(1024), 16, stream, worlds, nworlds, 0u);
+    else hipLaunchKernelGGL((k_broad<256>), grid, dim3(256), sl.broad_bytes(), stream, worlds, nworlds, sl.broad_cap());
     if (stages & STAGE_NARROW) {
       hipLaunchKernelGGL((k_pair_offsets<1024>), dim3(1), dim3(1024), 0, stream, worlds, nworlds, pair_offs);
       hipLaunchKernelGGL(k_narrow, dim3(narrow_groups), dim3(256), 0, stream, worlds, nworlds, (const uint32_t*)pair_offs);
-      hipLaunchKernelGGL((k_csort<256>), grid, dim3(256), 0, stream, worlds, nworlds);
+      if (block == 1024) hipLaunchKernelGGL((k_csort<1024>), grid, dim3(1024), 0, stream, worlds, nworlds);
+      else hipLaunchKernelGGL((k_csort<256>), grid, dim3(256), 0, stream, worlds, nworlds);
     }
   }
   if (marks) (void)hipEventRecord(marks[1], stream);

@@ -47,9 +47,9 @@ struct BlockTeam {
 };
 
 // Grid team scratch in global memory (words): [0] arrivals, [1] barrier generation, [2] set when a barrier gave up
-// waiting, [3] a flag word of the bodies, [4..7] reserved, then GRID_PART words of per-workgroup partials and
-// 256 * gridDim.x radix counts.
-static constexpr uint32_t GRID_HDR = 8, GRID_PART = 2048, GRID_MAX_GROUPS = 512;
+// waiting, [3..10] words of the bodies (counters, flags, reductions), [11..15] reserved, then GRID_PART words of
+// per-workgroup partials and 256 * gridDim.x radix counts.
+static constexpr uint32_t GRID_HDR = 16, GRID_PART = 2048, GRID_MAX_GROUPS = 512;
 __host__ __device__ inline size_t grid_scratch_words(uint32_t groups) { return GRID_HDR + GRID_PART + 256 * (size_t)groups; }
 
 // Barrier over the workgroups of a cooperative launch.  Writes before it are visible to every workgroup after it

@@ -42,35 +42,43 @@ DEV uint64_t pair_key(int a, int b) { return ((uint64_t)(uint32_t)(b + 1) << 32)
 DEV int key_a(uint64_t k) { return (int)(uint32_t)k - 1; }
 DEV int key_b(uint64_t k) { return (int)(uint32_t)(k >> 32) - 1; }
 
-struct W {
+// A world's view: its header (LDS) and buffers.  SG: header words, uniform across a wave, are moved into scalar
+// registers as they are read (a gain in the prepare and solve kernels; the collision kernels run faster leaving
+// that to the compiler).
+template <bool SG>
+struct WV {
   const uint32_t* H;  // header copy in LDS
   WorldDev d;
   SchedLayout S;
 
-  DEV uint32_t h(uint32_t k) const { return H[k]; }
-  DEV float hf(uint32_t k) const { return ubits(H[k]); }
-  DEV uint32_t NB() const { return H[W_NB]; }
-  DEV uint32_t NS() const { return H[W_NS]; }
-  DEV uint32_t NA() const { return H[W_NA]; }
-  DEV uint32_t NL() const { return H[W_NL]; }
-  DEV uint32_t ND() const { return H[W_ND]; }
-  DEV uint32_t NDA() const { return H[W_NDA]; }
-  DEV uint32_t JM() const { return H[W_JM]; }
-  DEV uint32_t MAXC() const { return H[W_MAXC]; }
-  DEV uint32_t MAXP() const { return H[W_MAXP]; }
-  DEV uint32_t MAXU() const { return H[W_MAXU]; }
-  DEV uint32_t UH() const { return H[W_UH]; }
-  DEV uint32_t AH() const { return H[W_AH]; }
-  DEV uint32_t NCOL() const { return H[W_NCOLW] * 32; }
-  DEV uint32_t ROWS() const { return H[W_ROWS]; }
-  DEV uint32_t PO() const { return H[W_PO]; }
+  DEV uint32_t hu(uint32_t k) const {
+    if constexpr (SG) return __builtin_amdgcn_readfirstlane(H[k]);
+    else return H[k];
+  }
+  DEV uint32_t h(uint32_t k) const { return hu(k); }
+  DEV float hf(uint32_t k) const { return ubits(hu(k)); }
+  DEV uint32_t NB() const { return hu(W_NB); }
+  DEV uint32_t NS() const { return hu(W_NS); }
+  DEV uint32_t NA() const { return hu(W_NA); }
+  DEV uint32_t NL() const { return hu(W_NL); }
+  DEV uint32_t ND() const { return hu(W_ND); }
+  DEV uint32_t NDA() const { return hu(W_NDA); }
+  DEV uint32_t JM() const { return hu(W_JM); }
+  DEV uint32_t MAXC() const { return hu(W_MAXC); }
+  DEV uint32_t MAXP() const { return hu(W_MAXP); }
+  DEV uint32_t MAXU() const { return hu(W_MAXU); }
+  DEV uint32_t UH() const { return hu(W_UH); }
+  DEV uint32_t AH() const { return hu(W_AH); }
+  DEV uint32_t NCOL() const { return hu(W_NCOLW) * 32; }
+  DEV uint32_t ROWS() const { return hu(W_ROWS); }
+  DEV uint32_t PO() const { return hu(W_PO); }
   DEV float4 sf(uint32_t i) const { return d.statF[i]; }
   DEV uint32_t su(uint32_t i) const { return d.statU[i]; }
   DEV uint32_t* sch(uint32_t off) const { return d.sched + off; }
   DEV uint64_t* sch64(uint32_t off) const { return (uint64_t*)(d.sched + off); }  // even offsets (HKEY, UKEY, UTPW)
 
   // ---- body records
-  DEV uint32_t bo(int b) const { return H[W_SF_BODY] + (uint32_t)b * 8; }
+  DEV uint32_t bo(int b) const { return hu(W_SF_BODY) + (uint32_t)b * 8; }
   DEV float body_inv_mass(int b) const { return sf(bo(b)).x; }
   DEV uint32_t body_kind(int b) const { return tu(sf(bo(b)).y); }
   DEV int body_art(int b) const { return ti(sf(bo(b)).z); }
@@ -82,12 +90,12 @@ struct W {
     if (b < 0) return -1;
     uint32_t k = body_kind(b);
     if (k == 1) return b;
-    if (k == 2 || k == 3) return (int)H[W_NB] + body_art(b);
+    if (k == 2 || k == 3) return (int)hu(W_NB) + body_art(b);
     return -1;
   }
-  DEV bool node_asleep(int n) const { return H[W_SLEEP] && n >= 0 && d.art[H[W_A_SLP] + (uint32_t)n * 4 + 1] > 0.5f; }
+  DEV bool node_asleep(int n) const { return hu(W_SLEEP) && n >= 0 && d.art[hu(W_A_SLP) + (uint32_t)n * 4 + 1] > 0.5f; }
   DEV bool pair_asleep(int a, int b) const {
-    if (!H[W_SLEEP]) return false;
+    if (!hu(W_SLEEP)) return false;
     int na = body_node(a), nb = body_node(b);
     return (na < 0 || node_asleep(na)) && (nb < 0 || node_asleep(nb));
   }
@@ -105,41 +113,41 @@ struct W {
   DEV m3 body_inertia(int b) const { return rows_T(bo(b) + 2); }
   DEV m3 body_inv_inertia(int b) const { return rows_T(bo(b) + 5); }
   // ---- hull records
-  DEV f3 hv(uint32_t h_, uint32_t i) const { return xyz(sf(H[W_SF_VERTS] + su(H[W_SU_HULL] + h_ * 4) + i)); }
-  DEV uint32_t hnv(uint32_t h_) const { return su(H[W_SU_HULL] + h_ * 4 + 1); }
-  DEV uint32_t hfo(uint32_t h_) const { return su(H[W_SU_HULL] + h_ * 4 + 2); }
-  DEV uint32_t hnf(uint32_t h_) const { return su(H[W_SU_HULL] + h_ * 4 + 3); }
-  DEV q4 fplane(uint32_t f) const { return Q(sf(H[W_SF_FPLANE] + f)); }
-  DEV uint32_t fstart(uint32_t f) const { return su(H[W_SU_FACE] + f * 2); }
-  DEV uint32_t fcount(uint32_t f) const { return su(H[W_SU_FACE] + f * 2 + 1); }
-  DEV uint32_t fvert(uint32_t k) const { return su(H[W_SU_FV] + k); }
-  DEV uint32_t hne(uint32_t h_) const { return su(H[W_SU_HEDGE] + h_ * 2 + 1); }
+  DEV f3 hv(uint32_t h_, uint32_t i) const { return xyz(sf(hu(W_SF_VERTS) + su(hu(W_SU_HULL) + h_ * 4) + i)); }
+  DEV uint32_t hnv(uint32_t h_) const { return su(hu(W_SU_HULL) + h_ * 4 + 1); }
+  DEV uint32_t hfo(uint32_t h_) const { return su(hu(W_SU_HULL) + h_ * 4 + 2); }
+  DEV uint32_t hnf(uint32_t h_) const { return su(hu(W_SU_HULL) + h_ * 4 + 3); }
+  DEV q4 fplane(uint32_t f) const { return Q(sf(hu(W_SF_FPLANE) + f)); }
+  DEV uint32_t fstart(uint32_t f) const { return su(hu(W_SU_FACE) + f * 2); }
+  DEV uint32_t fcount(uint32_t f) const { return su(hu(W_SU_FACE) + f * 2 + 1); }
+  DEV uint32_t fvert(uint32_t k) const { return su(hu(W_SU_FV) + k); }
+  DEV uint32_t hne(uint32_t h_) const { return su(hu(W_SU_HEDGE) + h_ * 2 + 1); }
   DEV uint4 hedge(uint32_t h_, uint32_t k) const {
-    uint32_t o = H[W_SU_EDGE] + (su(H[W_SU_HEDGE] + h_ * 2) + k) * 4;
+    uint32_t o = hu(W_SU_EDGE) + (su(hu(W_SU_HEDGE) + h_ * 2) + k) * 4;
     return make_uint4(su(o), su(o + 1), su(o + 2), su(o + 3));
   }
-  DEV uint32_t so(uint32_t s) const { return H[W_SF_SHAPE] + s * 7; }
+  DEV uint32_t so(uint32_t s) const { return hu(W_SF_SHAPE) + s * 7; }
   DEV q4 shape_prim(uint32_t s) const { return Q(sf(so(s) + 3)); }
   DEV uint32_t geo_type(uint32_t s) const { return tu(sf(so(s) + 6).x); }
   // ---- articulation records
-  DEV uint32_t lk(uint32_t l) const { return H[W_SF_LINK] + l * 6; }
+  DEV uint32_t lk(uint32_t l) const { return hu(W_SF_LINK) + l * 6; }
   DEV uint32_t link_type(uint32_t l) const { return tu(sf(lk(l)).x); }
   DEV int link_parent(uint32_t l) const { return ti(sf(lk(l)).y); }
   DEV uint32_t link_dof(uint32_t l) const { return tu(sf(lk(l)).z); }
   DEV int link_body(uint32_t l) const { return ti(sf(lk(l)).w); }
   DEV uint32_t link_ndof(uint32_t l) const { return tu(sf(lk(l) + 1).w); }
   DEV uint32_t link_qs(uint32_t l) const { return tu(sf(lk(l) + 3).w); }
-  DEV float4 dof_drive(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5); }
+  DEV float4 dof_drive(uint32_t dd) const { return sf(hu(W_SF_DOF) + dd * 5); }
   DEV bool driven_dof(uint32_t dd) const { const float4 dr = dof_drive(dd); return dr.y > 0.0f || dr.z > 0.0f; }
   // drive_mode rows: a driven dof's velocity limit is left to the sweep clamp (artwgsl.js:54)
-  DEV bool fused_dof(uint32_t dd) const { return H[W_DRIVE_ROWS] && H[W_FUSE_VLIM] && driven_dof(dd); }
-  DEV f3 dof_axis(uint32_t dd) const { return xyz(sf(H[W_SF_DOF] + dd * 5 + 3)); }
+  DEV bool fused_dof(uint32_t dd) const { return hu(W_DRIVE_ROWS) && hu(W_FUSE_VLIM) && driven_dof(dd); }
+  DEV f3 dof_axis(uint32_t dd) const { return xyz(sf(hu(W_SF_DOF) + dd * 5 + 3)); }
   DEV uint32_t link_nlin(uint32_t l) const { return tu(sf(lk(l) + 5).w); }  // D6: leading translation dofs
-  DEV float4 dof_limit(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 1); }
-  DEV float4 dof_passive(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 2); }
-  DEV bool scalar_dof(uint32_t dd) const { return sf(H[W_SF_DOF] + dd * 5 + 2).w > 0.5f; }
-  DEV uint4 art_info(uint32_t a) const { uint32_t o = H[W_SU_ART] + a * 4; return make_uint4(su(o), su(o + 1), su(o + 2), su(o + 3)); }
-  DEV uint32_t lo(uint32_t l) const { return H[W_A_LINK] + l * LB; }
+  DEV float4 dof_limit(uint32_t dd) const { return sf(hu(W_SF_DOF) + dd * 5 + 1); }
+  DEV float4 dof_passive(uint32_t dd) const { return sf(hu(W_SF_DOF) + dd * 5 + 2); }
+  DEV bool scalar_dof(uint32_t dd) const { return sf(hu(W_SF_DOF) + dd * 5 + 2).w > 0.5f; }
+  DEV uint4 art_info(uint32_t a) const { uint32_t o = hu(W_SU_ART) + a * 4; return make_uint4(su(o), su(o + 1), su(o + 2), su(o + 3)); }
+  DEV uint32_t lo(uint32_t l) const { return hu(W_A_LINK) + l * LB; }
   DEV float* art() const { return d.art; }
   DEV f3 ld3(uint32_t o) const { return mk3(d.art[o], d.art[o + 1], d.art[o + 2]); }
   DEV void st3(uint32_t o, f3 v) const { d.art[o] = v.x; d.art[o + 1] = v.y; d.art[o + 2] = v.z; }
@@ -160,8 +168,8 @@ struct W {
     return ld3(lo(l) + L_COM) - ld3(lo((uint32_t)par) + L_COM);
   }
   DEV q4 link_rot(uint32_t l) const { f3 r = ld3(lo(l) + L_ROT); return mk4(r, d.art[lo(l) + L_ROT + 3]); }
-  DEV q4 pose_q(int b) const { return Q(d.poses[H[W_P_BODY] + (uint32_t)b * 2 + 1]); }
-  DEV f3 pose_c(int b) const { return xyz(d.poses[H[W_P_BODY] + (uint32_t)b * 2]); }
+  DEV q4 pose_q(int b) const { return Q(d.poses[hu(W_P_BODY) + (uint32_t)b * 2 + 1]); }
+  DEV f3 pose_c(int b) const { return xyz(d.poses[hu(W_P_BODY) + (uint32_t)b * 2]); }
 
   // ---- Featherstone (artwgsl.js), one thread per articulation
   // Forward kinematics of one link from its parent's pose (artwgsl.js:98-187).
@@ -179,7 +187,7 @@ struct W {
       q4 mq = mk4(0, 0, 0, 1);
       f3 mp = mk3(0, 0, 0);
       uint32_t t = link_type(l), qs = link_qs(l);
-      uint32_t AQ = H[W_A_Q];
+      uint32_t AQ = hu(W_A_Q);
       float* art_ = d.art;
       if (t == JT_REVOLUTE) mq = qaxis(axis, art_[AQ + qs]);
       if (t == JT_PRISMATIC) mp = axis * art_[AQ + qs];
@@ -256,7 +264,7 @@ struct W {
   }
   // FK of articulation a, one wave (lane k: link ai.x + k at depth dl), root level first.
   DEVN void art_fk_lv(uint32_t a, uint32_t dl) const {
-    const uint32_t MP = H[W_MAXPATH], l = art_info(a).x + (threadIdx.x & 63);
+    const uint32_t MP = hu(W_MAXPATH), l = art_info(a).x + (threadIdx.x & 63);
     wave_sync();
     for (uint32_t d = 0; d < MP; d++) {
       if (dl == d) fk_link(l);
@@ -288,7 +296,7 @@ struct W {
   // art_factor_lv.  Pass 1 (root first): velocity, bias, rigid inertia and bias force of the link.
   DEV void factor_fwd_link(uint32_t l) const {
     float* art_ = d.art;
-    uint32_t AQD = H[W_A_QD], ABF = H[W_A_BF];
+    uint32_t AQD = hu(W_A_QD), ABF = hu(W_A_BF);
     f3 g = mk3(hf(W_GX), hf(W_GY), hf(W_GZ));
     uint32_t L = lo(l);
     int par = link_parent(l);
@@ -336,7 +344,7 @@ struct W {
   // returns the bias force pa the parent folds in (meaningless for a root).
   DEV SVd factor_bwd_link(uint32_t l) const {
     float* art_ = d.art;
-    uint32_t JMv = JM(), AQD = H[W_A_QD], AJF = H[W_A_JF];
+    uint32_t JMv = JM(), AQD = hu(W_A_QD), AJF = hu(W_A_JF);
     float dt = hf(W_DT);
     uint32_t L = lo(l);
     uint32_t n = link_ndof(l);
@@ -353,13 +361,13 @@ struct W {
       D[r * JMv + r] += dt * damp;
       float tau = -damp * art_[AQD + dd] + art_[AJF + dd];
       // Passive spring, explicit (artwgsl.js:298-300).
-      float4 sp = sf(H[W_SF_DOF] + dd * 5 + 4);
-      if (sp.x != 0.0f) tau += sp.x * (sp.y - art_[H[W_A_Q] + link_qs(l) + r]);
+      float4 sp = sf(hu(W_SF_DOF) + dd * 5 + 4);
+      if (sp.x != 0.0f) tau += sp.x * (sp.y - art_[hu(W_A_Q) + link_qs(l) + r]);
       // Implicit PD drive toward (A_TGT, A_TGTV), saturated on the implicit torque (artwgsl.js:301-313).
-      if (!H[W_DRIVE_ROWS] && (dr.y > 0.0f || dr.z > 0.0f)) {
-        float q = art_[H[W_A_Q] + link_qs(l) + r], qd = art_[AQD + dd];
-        float err = art_[H[W_A_TGT] + dd] - q;
-        float tv = art_[H[W_A_TGTV] + dd];
+      if (!hu(W_DRIVE_ROWS) && (dr.y > 0.0f || dr.z > 0.0f)) {
+        float q = art_[hu(W_A_Q) + link_qs(l) + r], qd = art_[AQD + dd];
+        float err = art_[hu(W_A_TGT) + dd] - q;
+        float tv = art_[hu(W_A_TGTV) + dd];
         float m = mx(D[r * JMv + r], 1e-9f);
         float qdn = (m * qd + dt * (dr.y * err + dr.z * tv)) / (m + dt * dr.z + dt * dt * dr.y);
         float ti = dr.y * (err - dt * qdn) + dr.z * (tv - qdn);
@@ -406,7 +414,7 @@ struct W {
   // Pass 3 (root first): accelerations and the free joint velocities qd_hat.
   DEV void factor_acc_link(uint32_t l) const {
     float* art_ = d.art;
-    uint32_t AQD = H[W_A_QD], AQDH = H[W_A_QDH];
+    uint32_t AQD = hu(W_A_QD), AQDH = hu(W_A_QDH);
     float dt = hf(W_DT);
     f3 g = mk3(hf(W_GX), hf(W_GY), hf(W_GZ));
     uint32_t L = lo(l);
@@ -425,14 +433,14 @@ struct W {
         acc = sadd(acc, sscale(ldS(l, r), qdd));
       }
       stsv(L + L_A, acc);
-      if (H[W_JFL]) {
+      if (hu(W_JFL)) {
         // Load-dependent joint friction: the joint's transmitted force and torque under the free dynamics
         // (articulated force at the child) raise its friction bound (artwgsl.js:361-367).
         SVd f = sadd(imul(L + L_IA, acc), ldsv(L + L_PA));
         float fm = len(f.l) + len(f.a);
         for (uint32_t r = 0; r < n; r++) {
           uint32_t dd = link_dof(l) + r;
-          art_[H[W_A_JFL] + dd] = sf(H[W_SF_DOF] + dd * 5 + 3).w * fm;
+          art_[hu(W_A_JFL) + dd] = sf(hu(W_SF_DOF) + dd * 5 + 3).w * fm;
         }
       }
     }
@@ -442,7 +450,7 @@ struct W {
   // folded into a parent in decreasing link order, the order of the serial pass, so results match it bit
   // for bit.
   DEVN void art_factor_lv(uint32_t a, const uint32_t* tree, float* pabuf) const {
-    const uint32_t lane = threadIdx.x & 63, MP = H[W_MAXPATH];
+    const uint32_t lane = threadIdx.x & 63, MP = hu(W_MAXPATH);
     const uint4 ai = art_info(a);
     const bool has = lane < ai.y;
     const uint32_t l = ai.x + lane, dl = has ? (tree[lane] & 0xffu) : 0xffu;
@@ -492,7 +500,7 @@ struct W {
     wave_sync();
   }
   DEV uint32_t link_path(uint32_t l0, uint32_t* path) const {
-    uint32_t n = 0, MP = H[W_MAXPATH];
+    uint32_t n = 0, MP = hu(W_MAXPATH);
     int l = (int)l0;
     while (!(l < 0 || n >= MP)) { path[n++] = (uint32_t)l; l = link_parent((uint32_t)l); }
     return n;
@@ -682,7 +690,7 @@ struct W {
   // Dense rows of the loop closures: one slot per closure, two for a revolute one (wgsl.js:2367-2368).
   DEV uint32_t loop_rows() const {
     uint32_t n = 0;
-    for (uint32_t k = 0; k < H[W_NLOOP]; k++) n += sf(H[W_SF_LOOP] + k * 5).w > 0.5f ? 2u : 1u;
+    for (uint32_t k = 0; k < hu(W_NLOOP); k++) n += sf(hu(W_SF_LOOP) + k * 5).w > 0.5f ? 2u : 1u;
     return n;
   }
   DEV void add_gen_jac(uint32_t x, f3 p, float sgn, f3 d0, f3 d1, f3 d2, uint32_t ds, float* J) const {
@@ -758,10 +766,14 @@ struct W {
   }
 };
 
-DEV W make_world(const uint32_t* Hs, const WorldDev& wd) {
-  W w;
+typedef WV<true> W;    // prepare and solve
+typedef WV<false> WC;  // broadphase, narrowphase, contact sort
+
+template <bool SG = true>
+DEV WV<SG> make_world(const uint32_t* Hs, const WorldDev& wd) {
+  WV<SG> w;
   w.H = Hs;
   w.d = wd;
-  w.S = sched_layout(Hs[W_MAXC], Hs[W_MAXU], Hs[W_UH], Hs[W_NCOLW] * 32, Hs[W_NB], Hs[W_NCOLW], Hs[W_NL], Hs[W_WH], Hs[W_MAXDENSE]);
+  w.S = sched_layout(w.hu(W_MAXC), w.hu(W_MAXU), w.hu(W_UH), w.hu(W_NCOLW) * 32, w.hu(W_NB), w.hu(W_NCOLW), w.hu(W_NL), w.hu(W_WH), w.hu(W_MAXDENSE));
   return w;
 }

@@ -129,6 +129,23 @@ def capacities(NS, NB, NA, policy="reference"):
 
 # Shape count above which a world's listed collision filters are kept as sorted lists (W_FILT_LIST).
 FILT_LIST_SHAPES = 16384
+# Shape count above which the broadphase bins shapes in a uniform grid of cells (W_CELL) instead of sweeping on min x.
+CELL_SHAPES = 4096
+
+
+def cell_bound(statF, SF, NS):
+    """Largest AABB extent the cell broadphase bins: a shape's AABB extent on any axis is at most
+    2 (|hull half extents| + gap) whatever its rotation; planes and shapes over four times the median of that bound
+    are left out (they are tested against every shape), so a few large static shapes do not coarsen the cells."""
+    rec = statF[SF["SF_SHAPE"] * 4:(SF["SF_SHAPE"] + NS * 7) * 4].reshape(NS, 7, 4)
+    hull = rec[:, 0, 1].astype(np.int64)
+    hh = statF[SF["SF_HAABB"] * 4:].reshape(-1, 4)[2 * hull + 1, :3].astype(np.float64)
+    b = 2.0 * (np.sqrt((hh * hh).sum(1)) + np.maximum(rec[:, 0, 3].astype(np.float64), 0.0))
+    b = b[rec[:, 3, 0] != 4.0]
+    if len(b) == 0:
+        return 0.0
+    b = b[b <= 4.0 * np.median(b)]
+    return float(b.max()) if len(b) else 0.0
 
 # Contact warm-start table slots per parity (sim.js:396 WH).
 WARM_SLOTS = 16384
@@ -644,6 +661,7 @@ def pack_scene(scene, opts=None, capacity_policy="reference", capacity_override=
     H("W_SLEEP", 1 if (SV.get("enable_sleeping") or SO.get("enable_sleeping")) else 0)
     H("W_NMIMIC", len(mimics))
     H("W_FILT_LIST", 1 if filt_list else 0)
+    HF("W_CELL", cell_bound(statF, SF, NS) if NS > CELL_SHAPES else 0.0)
     H("W_WH", WARM_SLOTS if warm else 0)
     H("W_NEST", 1 if nest else 0)
     H("W_MO", MO)

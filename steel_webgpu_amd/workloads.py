@@ -386,3 +386,26 @@ def offset_stack(nfar=0, seed=SEED + 5, fps=240, substeps=4, iterations=12):
     upper = b.addBody(xform=[0.02, -0.01, 2 * h[0][2] + h[1][2] + 0.05] + q)
     b.addShapeBox(upper, hx=h[1][0], hy=h[1][1], hz=h[1][2], mu=0.6, density=1000.0)
     return b.finalize(solver=dict(substeps=substeps, iterations=iterations))
+
+
+def cell_mix(seed=SEED + 6, n=4600, far=False, fps=240, substeps=4, iterations=12):
+    """A world past the cell broadphase's shape count with shapes it bins and shapes it does not: `n` free boxes of
+    half extents U[0.03, 0.15] at random poses in an 8 x 8 x 6 m region (overlaps allowed), six large free boxes and
+    three large static boxes (tested against every shape), and the ground plane.  With `far`, two overlapping pairs
+    of boxes sit 5e6 m out on either side of x, so the cell coordinates of everything else are clamped."""
+    rng = PCG32(seed, 0)
+    b = SceneBuilder(fps=fps)
+    b.addGroundPlane(mu=0.6)
+    for k in range(3):
+        b.addShapeBox(-1, hx=3.0, hy=0.5 + k, hz=0.4, mu=0.6, xform=[-4.0 + 4.0 * k, 2.0, 0.4, 0, 0, 0, 1])
+    for i in range(n + 6):
+        big = i >= n
+        hs = [rng.uniform(0.5, 0.8) if big else rng.uniform(0.03, 0.15) for _ in range(3)]
+        p = [rng.uniform(-4.0, 4.0), rng.uniform(-4.0, 4.0), rng.uniform(0.2, 6.0)]
+        body = b.addBody(xform=p + rng.unit_quat())
+        b.addShapeBox(body, hx=hs[0], hy=hs[1], hz=hs[2], mu=0.6, density=500.0)
+    if far:
+        for x in (5e6, 5e6 + 0.15, -5e6, -5e6 - 0.15):
+            body = b.addBody(xform=[x, 0.0, 5.0, 0, 0, 0, 1])
+            b.addShapeBox(body, hx=0.1, hy=0.1, hz=0.1, mu=0.6, density=500.0)
+    return b.finalize(solver=dict(substeps=substeps, iterations=iterations))

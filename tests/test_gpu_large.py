@@ -180,3 +180,41 @@ def test_device_wide_refuses_what_it_does_not_step(sim, lattice):
     with pytest.raises(sim.UnsupportedError):
         b.set_wide(True)
     b.destroy()
+
+
+@pytest.mark.parametrize("far", [False, True], ids=["cluster", "far_clamped"])
+def test_cell_broadphase_mixed_sizes_bitexact(sim, far):
+    """The cell broadphase on a world of 4600 small boxes at random overlapping poses, large free and static boxes
+    it does not bin, and the ground plane (with `far`, boxes 5e6 m out clamp every cell coordinate): after one frame
+    and after ten, the pair set, sorted contacts, colors and state equal the oracle's."""
+    pw = pack.pack_scene(workloads.cell_mix(far=far), capacity_policy="scaled")
+    assert pw.NS > pack.CELL_SHAPES and pw.hf("W_CELL") > 0
+    S = pw.h("W_SUBSTEPS")
+    b = sim.Batch([pw], block=256, graph=False)
+    o = OracleWorld(pw)
+    for frames in (1, 9):
+        b.step(frames)
+        if frames > 1:
+            o.frames(frames - 1)
+        o.substeps(S - 1)
+        o.substeps(1, debug=True)
+        assert _same_state(b, o, pw)
+        npairs, nc, nr = _compare_last_substep(b, o, pw)
+        print(f"cell mix far={far}: {npairs} pairs, {nc} contacts, {nr} rows", file=sys.stderr)
+        assert npairs > 4600
+    b.destroy()
+
+
+def test_lattice_1024_thread_workgroups_equal_256(sim, lattice):
+    """Workgroups of 1024 threads (broadphase bounds in global memory, 1024-thread sorts) step the lattice bit for
+    bit as workgroups of 256 do, over 10 frames."""
+    pw = lattice
+    a = sim.Batch([pw], block=256, graph=False)
+    b = sim.Batch([pw], block=1024, graph=False)
+    a.step(10)
+    b.step(10)
+    for name in ("poses", "vel", "art"):
+        assert np.array_equal(_bits(a.read(0, name)), _bits(b.read(0, name))), name
+    assert np.array_equal(a.stats_raw(0)[0], b.stats_raw(0)[0])
+    a.destroy()
+    b.destroy()

@@ -74,3 +74,33 @@ def test_body_ids_past_16_bits_leave_the_stack_unchanged():
         pw_ = ow.poses.reshape(-1, 4)[wide.h("W_P_BODY") + 2 * 65600:wide.h("W_P_BODY") + 2 * 65602]
         assert np.array_equal(_bits(ps), _bits(pw_))
     assert os_.stats[1] > 0 and ow.stats[1] == os_.stats[1]  # the stack's contacts, and no others
+
+
+def test_cell_bound_bins_the_small_shapes_only():
+    """W_CELL is set past CELL_SHAPES shapes only, bounds every small box's AABB extent under any rotation
+    (2 (|half extents| + gap)), and stays below the large boxes' bound so that they are tested against every shape."""
+    pw = pack.pack_scene(workloads.cell_mix(n=4200), capacity_policy="scaled")
+    small = pack.pack_scene(workloads.offset_stack(0))
+    assert small.hf("W_CELL") == 0.0
+    HB = pw.hf("W_CELL")
+    assert HB > 0
+    sf = pw.statF
+    rec = sf[pw.h("W_SF_SHAPE") * 4:(pw.h("W_SF_SHAPE") + pw.NS * 7) * 4].reshape(pw.NS, 7, 4)
+    hh = sf[pw.h("W_SF_HAABB") * 4:].reshape(-1, 4)[2 * rec[:, 0, 1].astype(np.int64) + 1, :3].astype(np.float64)
+    bound = 2 * (np.linalg.norm(hh, axis=1) + rec[:, 0, 3])
+    dyn = rec[:, 0, 0] >= 0
+    # the 4200 small boxes (half extents <= 0.15 and the default gap) are binned, the 6 large free ones are not
+    half = hh.max(1)
+    assert np.all(bound[dyn & (half <= 0.15 + 1e-6)] <= HB + 1e-6)
+    assert np.all(bound[dyn & (half >= 0.5)] > HB)
+    # a rotated box's AABB extent never exceeds the bound: corners of a box with the largest small half extents
+    rng = np.random.default_rng(1)
+    h = np.array([0.15, 0.15, 0.15])
+    for _ in range(200):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        x, y, z, w = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        assert np.all(2 * (np.abs(R) @ h) <= 2 * np.linalg.norm(h) + 1e-12)
