@@ -399,16 +399,16 @@ def fp64_reference(ows, log):
 
 
 def traffic_from_profile(kernel, window):
-    """HBM bytes per launch of k_<kernel> from the committed counter summary of this workload (its
-    `bytes_per_launch:` line, FETCH_SIZE + WRITE_SIZE per dispatch scaled by the copy-kernel factor), when its
-    `window:` line names the same frames as this run; None otherwise."""
-    path = os.path.join(ROOT, "profiles", f"k_{kernel}_quadruped4096_traffic.md")
+    """HBM bytes per launch of a stage (collide, prepare or solve) from the committed counter summary of this workload
+    (its `bytes_per_launch <stage>:` lines, FETCH_SIZE + WRITE_SIZE per dispatch scaled by the copy-kernel factor),
+    when its `window:` line names the same frames as this run; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "quadruped4096_traffic.md")
     if not os.path.exists(path):
         return None, None
     got, win = None, None
     with open(path) as f:
         for line in f:
-            if line.startswith("bytes_per_launch:"):
+            if line.startswith(f"bytes_per_launch {kernel}:"):
                 got = int(line.split(":")[1])
             if line.startswith("window:"):
                 win = line.split(":", 1)[1].strip()

@@ -60,8 +60,9 @@ struct LaunchLds {
   uint32_t prep_words() const { return cw < STEEL_PREP_LDS_MAX_WORDS ? cw : STEEL_PREP_LDS_MAX_WORDS; }
 };
 
-// Workgroups of k_narrow per launch for a device of `cus` compute units (four 4-wave groups per CU).
-inline uint32_t steel_narrow_groups(uint32_t cus) { return cus * 4; }
+// Workgroups of k_narrow per launch for a device of `cus` compute units: eight 4-wave groups per CU, the autotuner's
+// choice on the 4096-world quadruped batch (more waves in flight hide the hull loads).
+inline uint32_t steel_narrow_groups(uint32_t cus) { return cus * 8; }
 
 // sl: dynamic LDS over the launch's worlds.
 // pair_offs: device scratch of nworlds + 1 words (per-world pair offsets for k_narrow).
