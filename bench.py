@@ -545,8 +545,13 @@ def main():
         restore_states(batch, everyone, start)
         log(f"window start loaded from {args.state_in}")
     else:
-        for f in range(0, F0, 100):
-            batch.step(min(100, F0 - f))
+        # Under a profiler, frames are advanced one at a time with a sync each: runs that queued 100 frames of
+        # kernels per call ended in a SIGSEGV on a profiler thread.
+        chunk = 1 if args.profile else 100
+        for f in range(0, F0, chunk):
+            batch.step(min(chunk, F0 - f))
+            if args.profile:
+                batch.sync()
         batch.sync()
         start = save_states(batch, everyone) if F0 else None
         log(f"advanced frames 0..{F0 - 1} untimed in {time.perf_counter() - t0:.1f}s")

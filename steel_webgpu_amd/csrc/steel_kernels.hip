@@ -8,6 +8,9 @@
 #include "steel_launch.h"
 #include "steel_team.cuh"
 
+#define AS_GLOBAL __attribute__((address_space(1)))
+#define AS_LDS __attribute__((address_space(3)))
+
 // ------------------------------------------------------------------------------------------------
 // Block helpers
 // ------------------------------------------------------------------------------------------------
@@ -2607,22 +2610,38 @@ struct Solver {
       wave_sync();
     }
   }
+  // Partner of lane ^ K within each 32-lane half (the pairing of __shfl_xor(x, K, 32)): ds_swizzle's bit mode for
+  // K = 16, 8, 4 and a DPP quad permutation for K = 2, 1, so the sums below use no LDS round trips.
+  template <int K>
+  static DEV float xor32(float x) {
+    if constexpr (K == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+    else if constexpr (K == 2) return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, false));
+    else return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), (K << 10) | 0x1F));
+  }
   // subgroupAdd of the reference's 32-lane path (wgsl.js:3305), order fixed as an xor butterfly over strides
   // 16, 8, 4, 2, 1 within each 32-lane half of the wave; every lane ends with the same bits (x + y == y + x).
   static DEV float lane_sum32(float x) {
-    x = x + __shfl_xor(x, 16, 32);
-    x = x + __shfl_xor(x, 8, 32);
-    x = x + __shfl_xor(x, 4, 32);
-    x = x + __shfl_xor(x, 2, 32);
-    x = x + __shfl_xor(x, 1, 32);
+    x = x + xor32<16>(x);
+    x = x + xor32<8>(x);
+    x = x + xor32<4>(x);
+    x = x + xor32<2>(x);
+    x = x + xor32<1>(x);
     return x;
   }
+  // A dense row's uniform fields and this lane's entries of its three J and Y rows (dofs lane and lane + 32).
+  struct DenseRec {
+    float bil, mu, rhs, meff, k6, k7, k8, lx, ly, lz, wgt;
+    float j0[6], j1[6];  // J rows 0-2 then Y rows 0-2
+  };
   // Dense same-articulation rows, one wave per articulation (wgsl.js:3306-3364, subgroup form): lane l < 32
   // owns dofs l and l + 32 and keeps their joint velocities in registers for the whole pass; rows run in
-  // order (Gauss-Seidel).  Called by all 64 lanes of the wave with the same (a, mode, nd).
+  // order (Gauss-Seidel).  The wave finds the articulation's rows 64 at a time (a ballot over their owners) and
+  // loads each row's record while the row before it is solved.  Called by all 64 lanes of the wave with the same
+  // (a, mode, nd).
   DEVN bool art_dense(uint32_t a, uint32_t mode, uint32_t nd, uint32_t lane) const {
-    uint4 ai = w.art_info(a);
-    float* art_ = w.d.art;
+    const uint4 ai = w.art_info(a);
+    const AS_GLOBAL float* ar = (const AS_GLOBAL float*)w.d.art;
+    AS_GLOBAL float* aw = (AS_GLOBAL float*)w.d.art;
     float* aqd = laqd;
     const uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR), DPS = w.h(W_DPS), AD = w.h(W_A_DENSE);
     const float omega = w.hf(W_F_OMEGA);
@@ -2630,66 +2649,85 @@ struct Solver {
     const bool own0 = lane < 32 && lane < n, own1 = lane < 32 && lane + 32 < n;
     float q0 = own0 ? aqd[ai.z + lane] : 0.0f, q1 = own1 ? aqd[ai.z + lane + 32] : 0.0f;
     bool touched = false;
-    for (uint32_t i = 0; i < nd; i++) {
+    auto fetch = [&](uint32_t i, DenseRec& r) {
+      const uint32_t o = AD + i * DPS, jo = o + DHDR, yo = o + DHDR + 3 * NDA;
+      r.bil = ar[o + 21]; r.mu = ar[o + 3]; r.rhs = ar[o + 4]; r.meff = ar[o + 5];
+      r.k6 = ar[o + 6]; r.k7 = ar[o + 7]; r.k8 = ar[o + 8];
+      r.lx = ar[o + 9]; r.ly = ar[o + 10]; r.lz = ar[o + 11];
+      r.wgt = vpass ? 1.0f : ar[o + 24];
+      for (uint32_t k = 0; k < 3; k++) {
+        r.j0[k] = own0 ? ar[jo + k * NDA + lane] : 0.0f;
+        r.j0[3 + k] = own0 ? ar[yo + k * NDA + lane] : 0.0f;
+        r.j1[k] = own1 ? ar[jo + k * NDA + lane + 32] : 0.0f;
+        r.j1[3 + k] = own1 ? ar[yo + k * NDA + lane + 32] : 0.0f;
+      }
+    };
+    auto solve = [&](uint32_t i, const DenseRec& r) {
       const uint32_t o = AD + i * DPS;
-      if (tu(art_[o]) != a) continue;
-      touched = true;
-      const uint32_t jo = o + DHDR, yo = o + DHDR + 3 * NDA;
-      const float lx = art_[o + 9], ly = art_[o + 10], lz = art_[o + 11];
-      if (art_[o + 21] > 0.5f) {
+      if (r.bil > 0.5f) {
         // Bilateral rows (mimics): unclamped, one at a time; no warm impulse (wgsl.js:3319-3332).
-        if (mode == 0) continue;
-        float nlam[3] = {lx, ly, lz};
-        const uint32_t nb = tu(art_[o + 21]);
-        for (uint32_t r = 0; r < nb && r < 3; r++) {
-          const float jr0 = own0 ? art_[jo + r * NDA + lane] : 0.0f, jr1 = own1 ? art_[jo + r * NDA + lane + 32] : 0.0f;
-          const float yr0 = own0 ? art_[yo + r * NDA + lane] : 0.0f, yr1 = own1 ? art_[yo + r * NDA + lane + 32] : 0.0f;
+        if (mode == 0) return;
+        float nlam[3] = {r.lx, r.ly, r.lz};
+        const float rhs3[3] = {r.rhs, ar[o + 22], ar[o + 23]}, m3_[3] = {r.meff, r.k6, r.k7};
+        const uint32_t nb = tu(r.bil);
+#pragma unroll
+        for (uint32_t k = 0; k < 3; k++) {
+          if (k >= nb) break;
           float part = 0.0f;
-          if (own0) part += jr0 * q0;
-          if (own1) part += jr1 * q1;
+          if (own0) part += r.j0[k] * q0;
+          if (own1) part += r.j1[k] * q1;
           const float jv = lane_sum32(part);
-          const float rhs = vpass ? 0.0f : (r == 0 ? art_[o + 4] : (r == 1 ? art_[o + 22] : art_[o + 23]));
-          const float dl = -omega * (jv + rhs) * art_[o + 5 + r];
-          nlam[r] += dl;
-          if (own0) q0 += yr0 * dl;
-          if (own1) q1 += yr1 * dl;
+          const float rhs = vpass ? 0.0f : rhs3[k];
+          const float dl = -omega * (jv + rhs) * m3_[k];
+          nlam[k] += dl;
+          if (own0) q0 += r.j0[3 + k] * dl;
+          if (own1) q1 += r.j1[3 + k] * dl;
         }
-        if (lane == 0) { art_[o + 9] = nlam[0]; art_[o + 10] = nlam[1]; art_[o + 11] = nlam[2]; }
-        continue;
-      }
-      float ja0 = 0, jb0 = 0, jc0 = 0, ya0 = 0, yb0 = 0, yc0 = 0, ja1 = 0, jb1 = 0, jc1 = 0, ya1 = 0, yb1 = 0, yc1 = 0;
-      if (own0) {
-        ja0 = art_[jo + lane]; jb0 = art_[jo + NDA + lane]; jc0 = art_[jo + 2 * NDA + lane];
-        ya0 = art_[yo + lane]; yb0 = art_[yo + NDA + lane]; yc0 = art_[yo + 2 * NDA + lane];
-      }
-      if (own1) {
-        ja1 = art_[jo + lane + 32]; jb1 = art_[jo + NDA + lane + 32]; jc1 = art_[jo + 2 * NDA + lane + 32];
-        ya1 = art_[yo + lane + 32]; yb1 = art_[yo + NDA + lane + 32]; yc1 = art_[yo + 2 * NDA + lane + 32];
+        if (lane == 0) { aw[o + 9] = nlam[0]; aw[o + 10] = nlam[1]; aw[o + 11] = nlam[2]; }
+        return;
       }
       if (mode == 0) {
-        if (own0) q0 += ya0 * lx + yb0 * ly + yc0 * lz;
-        if (own1) q1 += ya1 * lx + yb1 * ly + yc1 * lz;
-        continue;
+        if (own0) q0 += r.j0[3] * r.lx + r.j0[4] * r.ly + r.j0[5] * r.lz;
+        if (own1) q1 += r.j1[3] * r.lx + r.j1[4] * r.ly + r.j1[5] * r.lz;
+        return;
       }
-      const float rhs = art_[o + 4], meff = art_[o + 5], wgt = vpass ? 1.0f : art_[o + 24], mu = art_[o + 3];
-      const float k6 = art_[o + 6], k7 = art_[o + 7], k8 = art_[o + 8];
       float part = 0.0f;
-      if (own0) part += ja0 * q0;
-      if (own1) part += ja1 * q1;
+      if (own0) part += r.j0[0] * q0;
+      if (own1) part += r.j1[0] * q1;
       const float jv = lane_sum32(part);
-      const float nl = mx(lx + omega * (-(jv + rhs) * meff * wgt - (1.0f - wgt) * lx), 0.0f);
-      const float dn = nl - lx;
+      const float nl = mx(r.lx + omega * (-(jv + r.rhs) * r.meff * r.wgt - (1.0f - r.wgt) * r.lx), 0.0f);
+      const float dn = nl - r.lx;
       float p1 = 0.0f, p2 = 0.0f;
-      if (own0) { q0 += ya0 * dn; p1 += jb0 * q0; p2 += jc0 * q0; }
-      if (own1) { q1 += ya1 * dn; p1 += jb1 * q1; p2 += jc1 * q1; }
+      if (own0) { q0 += r.j0[3] * dn; p1 += r.j0[1] * q0; p2 += r.j0[2] * q0; }
+      if (own1) { q1 += r.j1[3] * dn; p1 += r.j1[1] * q1; p2 += r.j1[2] * q1; }
       const float v1 = lane_sum32(p1), v2 = lane_sum32(p2);
-      const float lim = mu * nl;
-      const float kd = k6 * k8 - k7 * k7;
-      f2 lt = friction_pair_solve(k8 / kd, -k7 / kd, k6 / kd, v1, v2, ly, lz, lim, omega);
-      const float d1 = lt.x - ly, d2 = lt.y - lz;
-      if (own0) q0 += yb0 * d1 + yc0 * d2;
-      if (own1) q1 += yb1 * d1 + yc1 * d2;
-      if (lane == 0) { art_[o + 9] = nl; art_[o + 10] = lt.x; art_[o + 11] = lt.y; }
+      const float lim = r.mu * nl;
+      const float kd = r.k6 * r.k8 - r.k7 * r.k7;
+      f2 lt = friction_pair_solve(r.k8 / kd, -r.k7 / kd, r.k6 / kd, v1, v2, r.ly, r.lz, lim, omega);
+      const float d1 = lt.x - r.ly, d2 = lt.y - r.lz;
+      if (own0) q0 += r.j0[4] * d1 + r.j0[5] * d2;
+      if (own1) q1 += r.j1[4] * d1 + r.j1[5] * d2;
+      if (lane == 0) { aw[o + 9] = nl; aw[o + 10] = lt.x; aw[o + 11] = lt.y; }
+    };
+    DenseRec cur, nxt;
+    for (uint32_t base = 0; base < nd; base += 64) {
+      const uint32_t ii = base + lane;
+      const uint32_t owner = ii < nd ? tu(ar[AD + ii * DPS]) : 0xffffffffu;
+      uint64_t m = __ballot(owner == a);
+      if (m == 0) continue;
+      touched = true;
+      uint32_t i = base + (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      fetch(i, cur);
+      for (;;) {
+        const bool more = m != 0;
+        const uint32_t inext = more ? base + (uint32_t)__builtin_ctzll(m) : 0u;
+        if (more) { m &= m - 1; fetch(inext, nxt); }
+        solve(i, cur);
+        if (!more) break;
+        i = inext;
+        cur = nxt;
+      }
     }
     if (touched) {
       if (own0) aqd[ai.z + lane] = q0;
@@ -2875,8 +2913,6 @@ struct Solver {
 // flat memory operations, so every access waited for every outstanding load; here the row pointers are global,
 // the twists and impulses LDS (worlds whose twists fit) or global, all carried in registers, and the row's
 // records are loaded before anything is stored.  Same arithmetic, in the same order, as solve_unit<false>.
-#define AS_GLOBAL __attribute__((address_space(1)))
-#define AS_LDS __attribute__((address_space(3)))
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 template <bool TWL> struct TwSpace { typedef AS_GLOBAL v4f T; };
