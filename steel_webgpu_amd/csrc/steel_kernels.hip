@@ -2244,16 +2244,29 @@ struct Solver {
     const f2 tf = clip_cone(lt + tangent_step(ut0 + gnt * (ln - lam.x), g1), mu * ln);
     return mk3(ln, tf.x, tf.y);
   }
-  static DEV f2 friction_pair_solve(float a0, float c0, float d0, float r0, float r1, float old0, float old1, float radius, float omega) {
-    if (radius <= 0.0f) return mk2(0, 0);
-    float scale = mx(mx(a0, d0), 1e-20f);
-    float a = a0 / scale, c = c0 / scale, d = d0 / scale;
-    float largest = 0.5f * (a + d + sqrtf((a - d) * (a - d) + 4.0f * c * c));
-    float smallest = largest > 0.0f ? mx((a * d - c * c) / largest, 0.0f) : 0.0f;
-    float vx = c, vy = largest - a;
-    if (a >= d) { vx = largest - d; vy = c; }
+  // The part of the friction pair solve that depends on the row's constants only: the scaled 2x2 response's
+  // eigenvalues and eigenvector.
+  struct FricConst { float scale, largest, smallest, vx, vy; };
+  static DEV FricConst friction_const(float a0, float c0, float d0) {
+    FricConst k;
+    k.scale = mx(mx(a0, d0), 1e-20f);
+    float a = a0 / k.scale, c = c0 / k.scale, d = d0 / k.scale;
+    k.largest = 0.5f * (a + d + sqrtf((a - d) * (a - d) + 4.0f * c * c));
+    k.smallest = k.largest > 0.0f ? mx((a * d - c * c) / k.largest, 0.0f) : 0.0f;
+    float vx = c, vy = k.largest - a;
+    if (a >= d) { vx = k.largest - d; vy = c; }
     float vn = sqrtf(vx * vx + vy * vy);
     if (vn > 0.0f) { vx /= vn; vy /= vn; } else { vx = 1.0f; vy = 0.0f; }
+    k.vx = vx; k.vy = vy;
+    return k;
+  }
+  static DEV f2 friction_pair_solve(float a0, float c0, float d0, float r0, float r1, float old0, float old1, float radius, float omega) {
+    if (radius <= 0.0f) return mk2(0, 0);
+    return friction_pair_apply(friction_const(a0, c0, d0), r0, r1, old0, old1, radius, omega);
+  }
+  static DEV f2 friction_pair_apply(const FricConst& fk, float r0, float r1, float old0, float old1, float radius, float omega) {
+    if (radius <= 0.0f) return mk2(0, 0);
+    const float scale = fk.scale, largest = fk.largest, smallest = fk.smallest, vx = fk.vx, vy = fk.vy;
     float r0r = (vx * r0 + vy * r1) / scale;
     float r1r = (-vy * r0 + vx * r1) / scale;
     f2 res = mk2(old0, old1);
@@ -2610,22 +2623,18 @@ struct Solver {
       wave_sync();
     }
   }
-  // Partner of lane ^ K within each 32-lane half (the pairing of __shfl_xor(x, K, 32)): ds_swizzle's bit mode for
-  // K = 16, 8, 4 and a DPP quad permutation for K = 2, 1, so the sums below use no LDS round trips.
-  template <int K>
-  static DEV float xor32(float x) {
-    if constexpr (K == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
-    else if constexpr (K == 2) return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, false));
-    else return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), (K << 10) | 0x1F));
-  }
   // subgroupAdd of the reference's 32-lane path (wgsl.js:3305), order fixed as an xor butterfly over strides
   // 16, 8, 4, 2, 1 within each 32-lane half of the wave; every lane ends with the same bits (x + y == y + x).
+  // All strides stay in VALU: v_permlane16_swap exchanges the 16-lane rows of each half (the sum of the two
+  // swapped registers is x + partner in every lane), DPP row_ror:8 gives lane ^ 8, row_half_mirror then a
+  // reversed quad gives lane ^ 4 (7 - (i ^ 3) within 8 lanes), and quad permutes give lane ^ 2 and lane ^ 1.
   static DEV float lane_sum32(float x) {
-    x = x + xor32<16>(x);
-    x = x + xor32<8>(x);
-    x = x + xor32<4>(x);
-    x = x + xor32<2>(x);
-    x = x + xor32<1>(x);
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    x = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    x = x + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x128, 0xF, 0xF, false));
+    x = x + __int_as_float(__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x141, 0xF, 0xF, false), 0x1B, 0xF, 0xF, false));
+    x = x + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, false));
+    x = x + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
     return x;
   }
   // A dense row's uniform fields and this lane's entries of its three J and Y rows (dofs lane and lane + 32).
@@ -2662,7 +2671,7 @@ struct Solver {
         r.j1[3 + k] = own1 ? ar[yo + k * NDA + lane + 32] : 0.0f;
       }
     };
-    auto solve = [&](uint32_t i, const DenseRec& r) {
+    auto solve = [&](uint32_t i, const DenseRec& r, const FricConst& fl, uint32_t base) {
       const uint32_t o = AD + i * DPS;
       if (r.bil > 0.5f) {
         // Bilateral rows (mimics): unclamped, one at a time; no warm impulse (wgsl.js:3319-3332).
@@ -2702,8 +2711,14 @@ struct Solver {
       if (own1) { q1 += r.j1[3] * dn; p1 += r.j1[1] * q1; p2 += r.j1[2] * q1; }
       const float v1 = lane_sum32(p1), v2 = lane_sum32(p2);
       const float lim = r.mu * nl;
-      const float kd = r.k6 * r.k8 - r.k7 * r.k7;
-      f2 lt = friction_pair_solve(r.k8 / kd, -r.k7 / kd, r.k6 / kd, v1, v2, r.ly, r.lz, lim, omega);
+      FricConst fk;  // the row's constants, computed by lane i - base of the chunk
+      const int src = (int)(i - base);
+      fk.scale = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fl.scale), src));
+      fk.largest = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fl.largest), src));
+      fk.smallest = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fl.smallest), src));
+      fk.vx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fl.vx), src));
+      fk.vy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fl.vy), src));
+      f2 lt = friction_pair_apply(fk, v1, v2, r.ly, r.lz, lim, omega);
       const float d1 = lt.x - r.ly, d2 = lt.y - r.lz;
       if (own0) q0 += r.j0[4] * d1 + r.j0[5] * d2;
       if (own1) q1 += r.j1[4] * d1 + r.j1[5] * d2;
@@ -2716,6 +2731,15 @@ struct Solver {
       uint64_t m = __ballot(owner == a);
       if (m == 0) continue;
       touched = true;
+      // Each lane computes the friction constants of its row of the chunk (they depend on the row's K only),
+      // the same expressions the serial row solve would evaluate, once per pass instead of once per row.
+      FricConst fl = {1.0f, 0.0f, 0.0f, 1.0f, 0.0f};
+      if (owner == a) {
+        const uint32_t o = AD + ii * DPS;
+        const float k6 = ar[o + 6], k7 = ar[o + 7], k8 = ar[o + 8];
+        const float kd = k6 * k8 - k7 * k7;
+        fl = friction_const(k8 / kd, -k7 / kd, k6 / kd);
+      }
       uint32_t i = base + (uint32_t)__builtin_ctzll(m);
       m &= m - 1;
       fetch(i, cur);
@@ -2723,7 +2747,7 @@ struct Solver {
         const bool more = m != 0;
         const uint32_t inext = more ? base + (uint32_t)__builtin_ctzll(m) : 0u;
         if (more) { m &= m - 1; fetch(inext, nxt); }
-        solve(i, cur);
+        solve(i, cur, fl, base);
         if (!more) break;
         i = inext;
         cur = nxt;
