@@ -1229,7 +1229,7 @@ struct PrepLds {
   uint32_t wc[PREP_WC];              // coloring: per-color counts and offsets when NCOL + 2 fits
   uint32_t chg;                      // sleep islands: a contact joined two islands this pass
   uint32_t tree[(BS / 64) * 64];     // factor: link tree of each wave's articulation
-  float pab[(BS / 64) * 512];        // factor: children's bias forces
+  alignas(16) float pab[(BS / 64) * 512];  // factor: children's bias forces; then the link mirror (PrepMirror)
   float trig[12];                    // reduction: cos, sin of k * 60 degrees, k = 0..5
 };
 
@@ -1438,6 +1438,36 @@ DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t c
       w.art_tree(a, tree);
       w.art_fk_lv(a, k < w.art_info(a).y ? (tree[k] & 0xffu) : 0xffu);
       w.art_factor_lv(a, tree, s_pab + (threadIdx.x >> 6) * 512);
+    }
+  }
+  tm.sync();
+  // The factored link data the unit-impulse solves read (responses, dof_hinv, dense rows) in LDS when it fits the
+  // factor's scratch (no later phase writes S, U, D^-1 or the link COMs); identical values, shorter loads.
+  if constexpr (!TM::GRID) {
+    const uint32_t cap = (BS / 64) * 512;
+    uint32_t need = NL * 4;
+    for (uint32_t l = 0; l < NL && need <= cap; l++) need += prep_mirror_floats(w.link_ndof(l));
+    if (need <= cap) {
+      AS_LDS float* mf = (AS_LDS float*)pl->pab;
+      AS_LDS pm_u4* mm = (AS_LDS pm_u4*)pl->pab;
+      for (uint32_t l = tid; l < NL; l += NT) {
+        uint32_t off = NL * 4;
+        for (uint32_t k = 0; k < l; k++) off += prep_mirror_floats(w.link_ndof(k));
+        const uint32_t n = w.link_ndof(l);
+        mm[l] = pm_u4{(uint32_t)w.link_parent(l), n, w.link_dof(l), off};
+        for (uint32_t k = 0; k < n; k++) {
+          const SVd sk = w.ldS(l, k), uk = w.ldU(l, k);
+          mf[off + k * 6] = sk.a.x; mf[off + k * 6 + 1] = sk.a.y; mf[off + k * 6 + 2] = sk.a.z;
+          mf[off + k * 6 + 3] = sk.l.x; mf[off + k * 6 + 4] = sk.l.y; mf[off + k * 6 + 5] = sk.l.z;
+          mf[off + 6 * n + k * 6] = uk.a.x; mf[off + 6 * n + k * 6 + 1] = uk.a.y; mf[off + 6 * n + k * 6 + 2] = uk.a.z;
+          mf[off + 6 * n + k * 6 + 3] = uk.l.x; mf[off + 6 * n + k * 6 + 4] = uk.l.y; mf[off + 6 * n + k * 6 + 5] = uk.l.z;
+          for (uint32_t c = 0; c < n; c++) mf[off + 12 * n + k * n + c] = w.ldDI(l, k, c);
+        }
+        const f3 r = w.link_r_blocks(l);
+        mf[off + 12 * n + n * n] = r.x; mf[off + 12 * n + n * n + 1] = r.y; mf[off + 12 * n + n * n + 2] = r.z;
+      }
+      tm.sync();
+      P.w.pm = PrepMirror{(const AS_LDS float*)mf, (const AS_LDS pm_u4*)mm};
     }
   }
   if (w.h(W_VLIM_FREE)) {
@@ -3102,6 +3132,25 @@ struct ArtLds {
     uint32_t b = mbody(l);
     tw[b * 2] = tov(F4(v.l, 0.0f)); tw[b * 2 + 1] = tov(F4(v.a, 0.0f));
   }
+  // Link twists from the joint velocities, root first (Solver::art_twists with LDS-typed accesses).
+  DEV void twists(uint32_t a, uint32_t lane) const {  // wgsl.js:3240-3251
+    const v4u ai = art_info[a];
+    const bool has = lane < ai.y;
+    const uint32_t l = ai.x + lane, dl = has ? ltree[l].x : 0xffffffffu;
+    Solver::wave_sync();
+    for (uint32_t d = 0; d < MP; d++) {
+      if (dl == d) {
+        const int par = mpar(l);
+        SVd v = sv0();
+        if (par >= 0) v = mshift(msv(mV((uint32_t)par)), mr(l));
+        const uint32_t n = mnd(l), dof = mdof(l);
+        for (uint32_t r = 0; r < n; r++) v = sadd(v, sscale(mS(l, r), laqd[dof + r]));
+        mst(mV(l), v);
+        set_link_twist(l, v);
+      }
+      Solver::wave_sync();
+    }
+  }
   DEV void apply(uint32_t a, uint32_t lane) const {  // wgsl.js:3254-3299
     const v4u ai = art_info[a];
     const bool has = lane < ai.y;
@@ -3191,7 +3240,7 @@ DEV ArtLds<TWL> art_lds(const Solver& sv) {
   return m;
 }
 
-template <int BS, class TM>
+template <int BS, bool FAST = false, bool TWL = false, class TM>
 DEV void dense_phase(const TM& tm, const Solver& sv, uint32_t mode, uint32_t nd) {
   const uint32_t tid = tm.tid, NT = tm.NT, lane = tid & 63, wave = tid >> 6, NA = sv.w.NA();
   for (uint32_t base = 0; base < NA; base += NT / 64) {
@@ -3199,7 +3248,10 @@ DEV void dense_phase(const TM& tm, const Solver& sv, uint32_t mode, uint32_t nd)
     bool touched = false;
     if (a < NA) touched = sv.art_dense(a, mode, nd, lane);
     tm.sync();
-    if (a < NA && touched) sv.art_twists(a, lane);
+    if (a < NA && touched) {
+      if constexpr (FAST) art_lds<TWL>(sv).twists(a, lane);
+      else sv.art_twists(a, lane);
+    }
     tm.sync();
   }
 }
@@ -3241,7 +3293,7 @@ DEV void sweep(const TM& tm, const Solver& sv, uint32_t mode, uint32_t ncol, uin
     }
   }
   tm.sync();
-  if (!FAST || nd) dense_phase<BS>(tm, sv, 1, nd);
+  if (!FAST || nd) dense_phase<BS, FAST, TWL>(tm, sv, 1, nd);
   const uint4* slots = (const uint4*)(sched + S.SLOTT);
   if (FAST && !ext) {
     const RowCtx<TWL> rc = row_ctx<TWL>(sv);
@@ -3547,9 +3599,12 @@ DEV void solve_world(const TM& tm, const WorldDev wd, uint32_t* Hs, float4* s_dy
     slots[u] = make_uint4(r, us, (uint32_t)key_a(key), (uint32_t)key_b(key));
   }
   tm.sync();
-  for (uint32_t a = tid >> 6; a < NA; a += NT / 64) sv.art_twists(a, tid & 63);
+  for (uint32_t a = tid >> 6; a < NA; a += NT / 64) {
+    if constexpr (!MGL) art_lds<TWL>(sv).twists(a, tid & 63);
+    else sv.art_twists(a, tid & 63);
+  }
   tm.sync();
-  dense_phase<BS>(tm, sv, 0, nd);
+  dense_phase<BS, !MGL, TWL>(tm, sv, 0, nd);
   sweep<BS, !MGL, TWL>(tm, sv, 0, ncol, nu, nd);
   const uint32_t iters = w.h(W_ITERS);
   f2 kb = mk2(0.0f, 0.0f);

@@ -45,11 +45,21 @@ DEV int key_b(uint64_t k) { return (int)(uint32_t)(k >> 32) - 1; }
 // A world's view: its header (LDS) and buffers.  SG: header words, uniform across a wave, are moved into scalar
 // registers as they are read (a gain in the prepare and solve kernels; the collision kernels run faster leaving
 // that to the compiler).
+// Prepare's LDS mirror of the factored link data an articulation's solves read (null: read the link blocks).
+// Per link: (parent, ndof, first dof, mirror offset), and at the offset S (6n), U (6n), D^-1 (n x n), link_r (3).
+typedef uint32_t pm_u4 __attribute__((ext_vector_type(4)));
+struct PrepMirror {
+  const __attribute__((address_space(3))) float* f;
+  const __attribute__((address_space(3))) pm_u4* meta;
+};
+DEV uint32_t prep_mirror_floats(uint32_t n) { return 12 * n + n * n + 3; }
+
 template <bool SG>
 struct WV {
   const uint32_t* H;  // header copy in LDS
   WorldDev d;
   SchedLayout S;
+  PrepMirror pm{nullptr, nullptr};
 
   DEV uint32_t hu(uint32_t k) const {
     if constexpr (SG) return __builtin_amdgcn_readfirstlane(H[k]);
@@ -132,10 +142,10 @@ struct WV {
   // ---- articulation records
   DEV uint32_t lk(uint32_t l) const { return hu(W_SF_LINK) + l * 6; }
   DEV uint32_t link_type(uint32_t l) const { return tu(sf(lk(l)).x); }
-  DEV int link_parent(uint32_t l) const { return ti(sf(lk(l)).y); }
-  DEV uint32_t link_dof(uint32_t l) const { return tu(sf(lk(l)).z); }
+  DEV int link_parent(uint32_t l) const { return pm.meta ? (int)pm.meta[l].x : ti(sf(lk(l)).y); }
+  DEV uint32_t link_dof(uint32_t l) const { return pm.meta ? pm.meta[l].z : tu(sf(lk(l)).z); }
   DEV int link_body(uint32_t l) const { return ti(sf(lk(l)).w); }
-  DEV uint32_t link_ndof(uint32_t l) const { return tu(sf(lk(l) + 1).w); }
+  DEV uint32_t link_ndof(uint32_t l) const { return pm.meta ? pm.meta[l].y : tu(sf(lk(l) + 1).w); }
   DEV uint32_t link_qs(uint32_t l) const { return tu(sf(lk(l) + 3).w); }
   DEV float4 dof_drive(uint32_t dd) const { return sf(hu(W_SF_DOF) + dd * 5); }
   DEV bool driven_dof(uint32_t dd) const { const float4 dr = dof_drive(dd); return dr.y > 0.0f || dr.z > 0.0f; }
@@ -159,13 +169,28 @@ struct WV {
     m3 A = ldm(o), B = ldm(o + 9), C = ldm(o + 18);
     return mksv(mul(A, m.a) + mul(B, m.l), mul(tr(B), m.a) + mul(C, m.l));
   }
-  DEV SVd ldS(uint32_t l, uint32_t k) const { return ldsv(lo(l) + L_S + k * 6); }
-  DEV SVd ldU(uint32_t l, uint32_t k) const { return ldsv(lo(l) + L_U + k * 6); }
-  DEV float ldDI(uint32_t l, uint32_t r, uint32_t c) const { return d.art[lo(l) + L_DI + r * JM() + c]; }
-  DEV f3 link_r(uint32_t l) const {
-    int par = link_parent(l);
+  DEV SVd pmsv(uint32_t o) const { const auto* p = pm.f + o; return mksv(mk3(p[0], p[1], p[2]), mk3(p[3], p[4], p[5])); }
+  DEV SVd ldS(uint32_t l, uint32_t k) const {
+    if (pm.meta) return pmsv(pm.meta[l].w + k * 6);
+    return ldsv(lo(l) + L_S + k * 6);
+  }
+  DEV SVd ldU(uint32_t l, uint32_t k) const {
+    if (pm.meta) { const pm_u4 m = pm.meta[l]; return pmsv(m.w + 6 * m.y + k * 6); }
+    return ldsv(lo(l) + L_U + k * 6);
+  }
+  DEV float ldDI(uint32_t l, uint32_t r, uint32_t c) const {
+    if (pm.meta) { const pm_u4 m = pm.meta[l]; return pm.f[m.w + 12 * m.y + r * m.y + c]; }
+    return d.art[lo(l) + L_DI + r * JM() + c];
+  }
+  // COM(l) - COM(parent), zero at a root.
+  DEV f3 link_r_blocks(uint32_t l) const {
+    int par = ti(sf(lk(l)).y);
     if (par < 0) return mk3(0, 0, 0);
     return ld3(lo(l) + L_COM) - ld3(lo((uint32_t)par) + L_COM);
+  }
+  DEV f3 link_r(uint32_t l) const {
+    if (pm.meta) { const pm_u4 m = pm.meta[l]; const auto* p = pm.f + m.w + 12 * m.y + m.y * m.y; return mk3(p[0], p[1], p[2]); }
+    return link_r_blocks(l);
   }
   DEV q4 link_rot(uint32_t l) const { f3 r = ld3(lo(l) + L_ROT); return mk4(r, d.art[lo(l) + L_ROT + 3]); }
   DEV q4 pose_q(int b) const { return Q(d.poses[hu(W_P_BODY) + (uint32_t)b * 2 + 1]); }
