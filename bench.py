@@ -94,7 +94,7 @@ def serving(args, procs, log):
         rng = workloads.PCG32(workloads.SEED + 5, int(rate))
         arr = serve.poisson_arrivals(len(packed), rate, rng)
         reqs = [serve.Request(rid=i, arrival=arr[i], world=pw, horizon=h, lane=serve.lane_of(pw)) for i, (pw, h) in enumerate(packed)]
-        srv = serve.Server(budget_bytes=int(args.budget_gb * 2**30), device=dev)
+        srv = serve.Server(budget_bytes=int(args.budget_gb * 2**30), device=dev, blocks=tuple(args.lane_blocks))
         m = serve.run(reqs, srv, readback=(), log=log)
         srv.destroy()
         m["rate"] = rate
@@ -105,7 +105,7 @@ def serving(args, procs, log):
     frames, secs = 0, 0.0
     for i in range(0, len(packed), max(1, len(packed) // args.plain_worlds)) if args.plain_worlds else []:
         pw, _ = packed[i]
-        b1 = sim.Batch([pw], dev, block=serve.LANE_BLOCKS[serve.lane_of(pw)], graph=False)
+        b1 = sim.Batch([pw], dev, block=args.lane_blocks[serve.lane_of(pw)], graph=False)
         b1.sync()
         t1 = time.perf_counter()
         b1.step_timed(20)
@@ -473,7 +473,11 @@ def main():
     ap.add_argument("--nmax", type=int, default=2000, help="serving: largest request body count")
     ap.add_argument("--hmax", type=int, default=1000, help="serving: longest horizon (frames)")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="serving: HBM budget of the pool")
+    ap.add_argument("--lane-blocks", type=int, nargs=3, default=None, help="serving: workgroup size of each lane")
     args = ap.parse_args()
+    if args.lane_blocks is None:
+        from steel_webgpu_amd import serve
+        args.lane_blocks = list(serve.LANE_BLOCKS)
     if args.profile:
         args.no_cpu = True
         args.plain_worlds = 0
@@ -503,7 +507,7 @@ def main():
             "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded PCG32 scenes)",
             "config": {"workload": "serving", "requests_per_rate": args.requests, "rates": args.rates, "nmax": args.nmax,
                        "hmax": args.hmax, "budget_gb": args.budget_gb, "fps": 240, "substeps": 4, "iterations": 12,
-                       "lanes": "NB<100 / <600 / >=600 bodies, blocks 64/128/256"},
+                       "lanes": "NB<100 / <600 / >=600 bodies, blocks " + "/".join(str(x) for x in args.lane_blocks)},
             "plain_world_steps_per_s": round(plain, 2) if plain else None,
             "served_vs_plain": {str(r["rate"]): round(r["world_steps_per_s"] / plain, 1) for r in res} if plain else None,
             "per_rate": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in res]}))

@@ -141,7 +141,7 @@ int steel_estimate_bytes(uint32_t nworlds, const steel_world_desc* worlds, size_
 typedef struct steel_server steel_server;
 int steel_server_create(steel_device* dev, size_t budget_bytes, uint32_t nlanes, steel_server** out);
 int steel_server_destroy(steel_server* s);
-/* Workgroup size (64, 128 or 256) of a lane's prepare/solve kernels. */
+/* Workgroup size (64, 128, 256 or 1024) of a lane's stage kernels. */
 int steel_server_configure_lane(steel_server* s, uint32_t lane, uint32_t block);
 /* Pool bytes one world takes. */
 int steel_server_world_bytes(const steel_world_desc* w, size_t* bytes);

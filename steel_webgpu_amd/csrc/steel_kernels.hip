@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
 struct Prep {
   W w;
   uint32_t parity;
-  DEV float4 con(uint32_t c, uint32_t q) const { return w.d.contacts[c * 6 + q]; }
+  DEV float4 con(uint32_t c, uint32_t q) const { const gv4f v = ((const AS_GLOBAL gv4f*)w.d.contacts)[c * 6 + q]; return make_float4(v.x, v.y, v.z, v.w); }
   DEV uint64_t contact_pair(uint32_t c) const { float4 m = con(c, 2); return pair_key(ti(m.x), ti(m.y)); }
   DEV uint32_t unit_of(uint32_t p, uint32_t nu) const {
     const uint32_t* US = w.sch(w.S.USTART);
@@ -687,8 +687,8 @@ struct Prep {
     return NCOL;
   }
   DEV uint32_t anchor_hash(uint32_t pair, uint32_t k) const { return max(hash_u32(pair * 2u + k), 1u); }
-  DEV float4 row(uint32_t r, uint32_t k) const { return w.d.rows[r * w.ROWS() + k]; }
-  DEV void set_row(uint32_t r, uint32_t k, float4 v) const { w.d.rows[r * w.ROWS() + k] = v; }
+  DEV float4 row(uint32_t r, uint32_t k) const { const gv4f v = ((const AS_GLOBAL gv4f*)w.d.rows)[r * w.ROWS() + k]; return make_float4(v.x, v.y, v.z, v.w); }
+  DEV void set_row(uint32_t r, uint32_t k, float4 v) const { ((AS_GLOBAL gv4f*)w.d.rows)[r * w.ROWS() + k] = gv4f{v.x, v.y, v.z, v.w}; }
   // Contact warm start (wgsl.js:2175-2190, 2421-2431): the previous substep's impulses stored under `key`, scaled by
   // the warm scale; the tangent part projected on (t1, t2) within the friction cone mu * ln.
   DEV void warm_lookup(uint32_t key, float mu, f3 t1, f3 t2, float& ln, f2& lt) const {
@@ -1068,7 +1068,7 @@ struct Prep {
   DEVN void put_bilateral(uint32_t o, uint32_t art_id, const float* J, uint32_t nb, f3 rhs) const {
     const uint4 ai = w.art_info(art_id);
     const uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR);
-    float* art_ = w.d.art;
+    AS_GLOBAL float* art_ = (AS_GLOBAL float*)w.d.art;
     for (uint32_t r = 0; r < nb; r++) {
       float tau[64], y[64];
       SVd ps[64], dvs[64];
@@ -1128,7 +1128,7 @@ struct Prep {
     const uint32_t art_id = tu(M0.x);
     const uint4 ai = w.art_info(art_id);
     const uint32_t NDA = w.NDA(), DHDR = w.h(W_DHDR);
-    float* art_ = w.d.art;
+    AS_GLOBAL float* art_ = (AS_GLOBAL float*)w.d.art;
     const uint32_t o = w.h(W_A_DENSE) + slot * w.h(W_DPS);
     float tau[64], y[64];
     SVd ps[64], dvs[64];
@@ -1169,7 +1169,7 @@ struct Prep {
     for (uint32_t k = 0; k < 3 * NDA; k++) J[k] = 0.0f;
     w.add_gen_jac((uint32_t)w.body_link(b), p, 1.0f, n, t1, t2, ai.z, J);
     w.add_gen_jac((uint32_t)w.body_link(a), p, -1.0f, n, t1, t2, ai.z, J);
-    float* art_ = w.d.art;
+    AS_GLOBAL float* art_ = (AS_GLOBAL float*)w.d.art;
     uint32_t o = w.h(W_A_DENSE) + slot * w.h(W_DPS);
     SVd ps[64], dvs[64];
     float uu[6 * 64];
@@ -1259,7 +1259,7 @@ DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t c
   const uint32_t NB = w.NB(), NA = w.NA(), NL = w.NL(), ND = w.ND(), MAXC = w.MAXC(), MAXU = w.MAXU(), UH = w.UH(), AH = w.AH();
   const uint32_t NCOL = w.NCOL(), NCOLW = w.h(W_NCOLW), PATCH = w.h(W_PATCH);
   uint32_t* sched = wd.sched;
-  float* art_ = wd.art;
+  AS_GLOBAL float* art_ = (AS_GLOBAL float*)wd.art;
   // ---- Phase 0
   for (uint32_t i = tid; i < UH; i += NT) { w.sch64(S.HKEY)[i] = 0; sched[S.HCNT + i] = 0; }
   if (PATCH) for (uint32_t i = tid; i < AH; i += NT) wd.akey[(1 - P.parity) * AH + i] = 0;
@@ -1985,7 +1985,7 @@ struct Solver {
       return F4(xyz(row(i, 1)) * lam.y + xyz(row(i, 2)) * lam.z, lam.x);  // (tangent, normal): reordered on store
     }
     const uint32_t o = w.h(W_A_DENSE) + (i - nc) * w.h(W_DPS);
-    const float* art_ = w.d.art;
+    const AS_GLOBAL float* art_ = (const AS_GLOBAL float*)w.d.art;
     return F4(w.ld3(o + 12) * art_[o + 10] + w.ld3(o + 15) * art_[o + 11], art_[o + 9]);
   }
   DEV unsigned long long warm_order(float x, uint32_t i) const {
@@ -2353,7 +2353,7 @@ struct Solver {
   DEVN void solve_unit(uint32_t us, uint32_t r, int a, int b, uint32_t mode) const {  // wgsl.js:3033-3231
     const uint32_t PO = w.PO();
     const bool EXTRA = w.h(W_EXTRA) != 0, PATCH = w.h(W_PATCH) != 0;
-    float* art_ = w.d.art;
+    AS_GLOBAL float* art_ = (AS_GLOBAL float*)w.d.art;
     f3 ta[2], tb[2];
     load_tw(a, ta); load_tw(b, tb);
     float4 r0 = row(r, 0), r1 = row(r, 1), r2 = row(r, 2);
@@ -2807,7 +2807,7 @@ struct Solver {
   // uniform) and writes the same values; lane 0 sets the link's tau before the wave's art_apply.
   DEVN void art_propagate_rows(uint32_t a, uint32_t lane) const {
     uint4 ai = w.art_info(a);
-    float* art_ = w.d.art;
+    AS_GLOBAL float* art_ = (AS_GLOBAL float*)w.d.art;
     uint32_t n = tu(art_[w.h(W_A_RLN) + a]);
     float dt = w.hf(W_DT), beta = w.hf(W_BETA), maxd = w.hf(W_MAX_DEPEN), omega = w.hf(W_F_OMEGA);
     for (uint32_t i = 0; i < n; i++) {
@@ -2864,7 +2864,7 @@ struct Solver {
   // Velocity limits, last in a sweep: a stateless push of each active scalar dof back onto |qd| <= limit
   // through the articulation (wgsl.js:3413-3419, 3462-3467); a wave per articulation, as the limit rows.
   DEVN void art_velocity_limits(uint32_t a, uint32_t lane) const {
-    float* art_ = w.d.art;
+    AS_GLOBAL float* art_ = (AS_GLOBAL float*)w.d.art;
     if (!(w.h(W_VLIM_ON) && art_[w.h(W_A_AVL) + a] > 0.5f)) return;
     uint4 ai = w.art_info(a);
     for (uint32_t k = 0; k < ai.y; k++) {
@@ -2888,7 +2888,7 @@ struct Solver {
   // inertia), stateless (wgsl.js:3586-3610).
   DEV void body_velocity_limits(uint32_t b) const {
     if (w.body_kind((int)b) != 1 || (w.body_flags((int)b) & 1) != 0) return;
-    const float* art_ = w.d.art;
+    const AS_GLOBAL float* art_ = (const AS_GLOBAL float*)w.d.art;
     q4 lm = w.body_limits((int)b);
     const uint32_t g = w.h(W_A_VLG) + w.ND() + b * 6;
     if (lm.y > 0.0f) {
@@ -3443,7 +3443,7 @@ __device__ __noinline__ void velocity_pass(const Solver& sv, uint32_t ncol, uint
   const SchedLayout& S = w.S;
   const uint32_t tid = threadIdx.x, NB = w.NB(), ND = w.ND(), MAXC = w.MAXC();
   const uint32_t* sched = wd.sched;
-  float* art_ = wd.art;
+  AS_GLOBAL float* art_ = (AS_GLOBAL float*)wd.art;
   const uint32_t VI = w.h(W_VELITERS);
   const uint32_t SN = w.h(W_A_SNAP), ROWS = w.ROWS();
   const float dt = w.hf(W_DT);
@@ -3492,7 +3492,7 @@ DEV void solve_world(const TM& tm, const WorldDev wd, uint32_t* Hs, float4* s_dy
   sv.parity = wd.ctr[2] & 1u;
   const uint32_t NB = w.NB(), NA = w.NA(), NL = w.NL(), ND = w.ND(), MAXC = w.MAXC(), NCOL = w.NCOL(), AH = w.AH(), PO = w.PO();
   uint32_t* sched = wd.sched;
-  float* art_ = wd.art;
+  AS_GLOBAL float* art_ = (AS_GLOBAL float*)wd.art;
   const float dt = w.hf(W_DT);
   const f3 g = mk3(w.hf(W_GX), w.hf(W_GY), w.hf(W_GZ));
   const uint32_t nu = sched[S.META], ncol = min(sched[S.META + 1], NCOL);

@@ -159,7 +159,7 @@ int steel_server_destroy(steel_server* s) {
 
 int steel_server_configure_lane(steel_server* s, uint32_t lane, uint32_t block) {
   if (!s || lane >= s->lanes.size()) return fail(STEEL_BAD_ARGUMENT, "bad lane %u", lane);
-  if (block != 64 && block != 128 && block != 256) return fail(STEEL_BAD_ARGUMENT, "block must be 64, 128 or 256");
+  if (block != 64 && block != 128 && block != 256 && block != 1024) return fail(STEEL_BAD_ARGUMENT, "block must be 64, 128, 256 or 1024");
   s->lanes[lane].block = block;
   return STEEL_OK;
 }

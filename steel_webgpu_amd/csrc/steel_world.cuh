@@ -45,6 +45,11 @@ DEV int key_b(uint64_t k) { return (int)(uint32_t)(k >> 32) - 1; }
 // A world's view: its header (LDS) and buffers.  SG: header words, uniform across a wave, are moved into scalar
 // registers as they are read (a gain in the prepare and solve kernels; the collision kernels run faster leaving
 // that to the compiler).
+#ifndef AS_GLOBAL
+#define AS_GLOBAL __attribute__((address_space(1)))
+#endif
+typedef float gv4f __attribute__((ext_vector_type(4)));
+
 // Prepare's LDS mirror of the factored link data an articulation's solves read (null: read the link blocks).
 // Per link: (parent, ndof, first dof, mirror offset), and at the offset S (6n), U (6n), D^-1 (n x n), link_r (3).
 typedef uint32_t pm_u4 __attribute__((ext_vector_type(4)));
@@ -82,8 +87,17 @@ struct WV {
   DEV uint32_t NCOL() const { return hu(W_NCOLW) * 32; }
   DEV uint32_t ROWS() const { return hu(W_ROWS); }
   DEV uint32_t PO() const { return hu(W_PO); }
-  DEV float4 sf(uint32_t i) const { return d.statF[i]; }
-  DEV uint32_t su(uint32_t i) const { return d.statU[i]; }
+  // World buffers are global memory: typed loads and stores (not flat ones, which wait on LDS traffic too).
+  DEV AS_GLOBAL float* ga() const { return (AS_GLOBAL float*)d.art; }
+  // (The collision kernels keep generic loads for the static records: they measured faster that way.)
+  DEV float4 sf(uint32_t i) const {
+    if constexpr (SG) { const gv4f v = ((const AS_GLOBAL gv4f*)d.statF)[i]; return make_float4(v.x, v.y, v.z, v.w); }
+    else return d.statF[i];
+  }
+  DEV uint32_t su(uint32_t i) const {
+    if constexpr (SG) return ((const AS_GLOBAL uint32_t*)d.statU)[i];
+    else return d.statU[i];
+  }
   DEV uint32_t* sch(uint32_t off) const { return d.sched + off; }
   DEV uint64_t* sch64(uint32_t off) const { return (uint64_t*)(d.sched + off); }  // even offsets (HKEY, UKEY, UTPW)
 
@@ -103,7 +117,7 @@ struct WV {
     if (k == 2 || k == 3) return (int)hu(W_NB) + body_art(b);
     return -1;
   }
-  DEV bool node_asleep(int n) const { return hu(W_SLEEP) && n >= 0 && d.art[hu(W_A_SLP) + (uint32_t)n * 4 + 1] > 0.5f; }
+  DEV bool node_asleep(int n) const { return hu(W_SLEEP) && n >= 0 && ga()[hu(W_A_SLP) + (uint32_t)n * 4 + 1] > 0.5f; }
   DEV bool pair_asleep(int a, int b) const {
     if (!hu(W_SLEEP)) return false;
     int na = body_node(a), nb = body_node(b);
@@ -159,8 +173,8 @@ struct WV {
   DEV uint4 art_info(uint32_t a) const { uint32_t o = hu(W_SU_ART) + a * 4; return make_uint4(su(o), su(o + 1), su(o + 2), su(o + 3)); }
   DEV uint32_t lo(uint32_t l) const { return hu(W_A_LINK) + l * LB; }
   DEV float* art() const { return d.art; }
-  DEV f3 ld3(uint32_t o) const { return mk3(d.art[o], d.art[o + 1], d.art[o + 2]); }
-  DEV void st3(uint32_t o, f3 v) const { d.art[o] = v.x; d.art[o + 1] = v.y; d.art[o + 2] = v.z; }
+  DEV f3 ld3(uint32_t o) const { const AS_GLOBAL float* a = ga(); return mk3(a[o], a[o + 1], a[o + 2]); }
+  DEV void st3(uint32_t o, f3 v) const { AS_GLOBAL float* a = ga(); a[o] = v.x; a[o + 1] = v.y; a[o + 2] = v.z; }
   DEV SVd ldsv(uint32_t o) const { return mksv(ld3(o), ld3(o + 3)); }
   DEV void stsv(uint32_t o, SVd s) const { st3(o, s.a); st3(o + 3, s.l); }
   DEV m3 ldm(uint32_t o) const { m3 m; m.c[0] = ld3(o); m.c[1] = ld3(o + 3); m.c[2] = ld3(o + 6); return m; }
@@ -180,7 +194,7 @@ struct WV {
   }
   DEV float ldDI(uint32_t l, uint32_t r, uint32_t c) const {
     if (pm.meta) { const pm_u4 m = pm.meta[l]; return pm.f[m.w + 12 * m.y + r * m.y + c]; }
-    return d.art[lo(l) + L_DI + r * JM() + c];
+    return ga()[lo(l) + L_DI + r * JM() + c];
   }
   // COM(l) - COM(parent), zero at a root.
   DEV f3 link_r_blocks(uint32_t l) const {
@@ -192,9 +206,13 @@ struct WV {
     if (pm.meta) { const pm_u4 m = pm.meta[l]; const auto* p = pm.f + m.w + 12 * m.y + m.y * m.y; return mk3(p[0], p[1], p[2]); }
     return link_r_blocks(l);
   }
-  DEV q4 link_rot(uint32_t l) const { f3 r = ld3(lo(l) + L_ROT); return mk4(r, d.art[lo(l) + L_ROT + 3]); }
-  DEV q4 pose_q(int b) const { return Q(d.poses[hu(W_P_BODY) + (uint32_t)b * 2 + 1]); }
-  DEV f3 pose_c(int b) const { return xyz(d.poses[hu(W_P_BODY) + (uint32_t)b * 2]); }
+  DEV q4 link_rot(uint32_t l) const { f3 r = ld3(lo(l) + L_ROT); return mk4(r, ga()[lo(l) + L_ROT + 3]); }
+  DEV float4 pose4(uint32_t i) const {
+    if constexpr (SG) { const gv4f v = ((const AS_GLOBAL gv4f*)d.poses)[i]; return make_float4(v.x, v.y, v.z, v.w); }
+    else return d.poses[i];
+  }
+  DEV q4 pose_q(int b) const { return Q(pose4(hu(W_P_BODY) + (uint32_t)b * 2 + 1)); }
+  DEV f3 pose_c(int b) const { return xyz(pose4(hu(W_P_BODY) + (uint32_t)b * 2)); }
 
   // ---- Featherstone (artwgsl.js), one thread per articulation
   // Forward kinematics of one link from its parent's pose (artwgsl.js:98-187).
@@ -213,7 +231,7 @@ struct WV {
       f3 mp = mk3(0, 0, 0);
       uint32_t t = link_type(l), qs = link_qs(l);
       uint32_t AQ = hu(W_A_Q);
-      float* art_ = d.art;
+      AS_GLOBAL float* art_ = ga();
       if (t == JT_REVOLUTE) mq = qaxis(axis, art_[AQ + qs]);
       if (t == JT_PRISMATIC) mp = axis * art_[AQ + qs];
       if (t == JT_BALL) mq = nrm4(mk4(art_[AQ + qs], art_[AQ + qs + 1], art_[AQ + qs + 2], art_[AQ + qs + 3]));
@@ -320,7 +338,7 @@ struct WV {
   // Articulated-body factor (artwgsl.js:210-369), split into per-link steps run level by level by
   // art_factor_lv.  Pass 1 (root first): velocity, bias, rigid inertia and bias force of the link.
   DEV void factor_fwd_link(uint32_t l) const {
-    float* art_ = d.art;
+    AS_GLOBAL float* art_ = ga();
     uint32_t AQD = hu(W_A_QD), ABF = hu(W_A_BF);
     f3 g = mk3(hf(W_GX), hf(W_GY), hf(W_GZ));
     uint32_t L = lo(l);
@@ -368,7 +386,7 @@ struct WV {
   // Pass 2 (leaves first), after the link's children were folded in: U, D^-1, u and the reduced inertia;
   // returns the bias force pa the parent folds in (meaningless for a root).
   DEV SVd factor_bwd_link(uint32_t l) const {
-    float* art_ = d.art;
+    AS_GLOBAL float* art_ = ga();
     uint32_t JMv = JM(), AQD = hu(W_A_QD), AJF = hu(W_A_JF);
     float dt = hf(W_DT);
     uint32_t L = lo(l);
@@ -438,7 +456,7 @@ struct WV {
   }
   // Pass 3 (root first): accelerations and the free joint velocities qd_hat.
   DEV void factor_acc_link(uint32_t l) const {
-    float* art_ = d.art;
+    AS_GLOBAL float* art_ = ga();
     uint32_t AQD = hu(W_A_QD), AQDH = hu(W_A_QDH);
     float dt = hf(W_DT);
     f3 g = mk3(hf(W_GX), hf(W_GY), hf(W_GZ));
@@ -617,7 +635,7 @@ struct WV {
       Wm[0 * 6 + e] = dv[e].l.x; Wm[1 * 6 + e] = dv[e].l.y; Wm[2 * 6 + e] = dv[e].l.z;
       Wm[3 * 6 + e] = dv[e].a.x; Wm[4 * 6 + e] = dv[e].a.y; Wm[5 * 6 + e] = dv[e].a.z;
     }
-    float* art_ = d.art;
+    AS_GLOBAL float* art_ = ga();
     for (uint32_t k = 0; k < 36; k++) art_[lo(l) + L_W2 + k] = Wm[k];
     for (uint32_t i = 0; i < np; i++) {
       float extra = (float)jsplit[path[i]] - 1.0f;
@@ -678,7 +696,7 @@ struct WV {
       dv = dd;
     }
     float col[6] = {dv.l.x, dv.l.y, dv.l.z, dv.a.x, dv.a.y, dv.a.z};  // column e of W2
-    float* art_ = d.art;
+    AS_GLOBAL float* art_ = ga();
     for (uint32_t k = 0; k < 6; k++) art_[lo(l) + L_W2 + k * 6 + e] = col[k];
     for (uint32_t i = 0; i < np; i++) {
       float extra = (float)jsplit[path[i]] - 1.0f;
@@ -779,14 +797,14 @@ struct WV {
     }
     uint32_t wo = lo((uint32_t)body_link(b)) + L_W;
     float x[6] = {f.x, f.y, f.z, t.x, t.y, t.z}, o[6];
-    for (uint32_t r = 0; r < 6; r++) { float s = 0.0f; for (uint32_t c = 0; c < 6; c++) s += d.art[wo + r * 6 + c] * x[c]; o[r] = s; }
+    for (uint32_t r = 0; r < 6; r++) { float s = 0.0f; for (uint32_t c = 0; c < 6; c++) s += ga()[wo + r * 6 + c] * x[c]; o[r] = s; }
     out[0] = mk3(o[0], o[1], o[2]); out[1] = mk3(o[3], o[4], o[5]);
   }
   DEV void body_response_unsplit(int b, f3 f, f3 t, f3 out[2]) const {
     if (b < 0 || body_kind(b) != 2) { body_response(b, f, t, out); return; }
     uint32_t wo = lo((uint32_t)body_link(b)) + L_W2;
     float x[6] = {f.x, f.y, f.z, t.x, t.y, t.z}, o[6];
-    for (uint32_t r = 0; r < 6; r++) { float s = 0.0f; for (uint32_t c = 0; c < 6; c++) s += d.art[wo + r * 6 + c] * x[c]; o[r] = s; }
+    for (uint32_t r = 0; r < 6; r++) { float s = 0.0f; for (uint32_t c = 0; c < 6; c++) s += ga()[wo + r * 6 + c] * x[c]; o[r] = s; }
     out[0] = mk3(o[0], o[1], o[2]); out[1] = mk3(o[3], o[4], o[5]);
   }
 };

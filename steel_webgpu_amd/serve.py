@@ -24,7 +24,7 @@ from . import sim as _sim
 
 # Lane (size class) by body count: small worlds, medium worlds, large worlds; workgroup size per lane.
 LANE_EDGES = (100, 600)
-LANE_BLOCKS = (64, 128, 256)
+LANE_BLOCKS = (64, 128, 1024)
 
 
 class AdmitRefused(RuntimeError):
