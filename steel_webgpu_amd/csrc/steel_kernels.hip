@@ -1220,6 +1220,10 @@ struct Prep {
 static constexpr uint32_t PREP_CH = 256, PREP_WC = 258;
 // Walk entries each thread of the first workgroup holds per window when the color masks are in global memory.
 static constexpr uint32_t PREP_CPER = 4;
+// Walk entries up to which one workgroup whose color masks are in global memory colors them serially on one lane;
+// past it the workgroup colors windows of entries in parallel rounds (the same colors: an entry is colored once no
+// earlier pending entry shares one of its bodies).
+static constexpr uint32_t PREP_SERIAL_COLOR = 256;
 
 // LDS of prepare's phases, per workgroup.
 template <int BS>
@@ -1665,9 +1669,10 @@ DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t c
   // entry's row and body pair into LDS and wave 0 colors 64 entries at a time in rounds: an entry is colored once no
   // earlier entry still pending shares one of its bodies (the lowest pending lane of each body wins an atomicMin), so
   // every entry sees exactly the masks the serial walk gives it and entries colored together touch disjoint bodies.
-  // With the masks in global memory, one workgroup's lane 0 colors serially; a grid team's first workgroup does as
-  // wave 0 does over windows of PREP_CPER * BS entries, each thread holding PREP_CPER of them, with the first-pending
-  // slots in LINKHIT (free after the link responses).  Per-color counts, unit tags (sums and maxima) and the color-ordered row list (a stable scatter, or
+  // With the masks in global memory, a workgroup with up to PREP_SERIAL_COLOR entries colors them serially on lane 0;
+  // a larger world's workgroup, or a grid team's first workgroup, does as wave 0 does over windows of PREP_CPER * BS
+  // entries, each thread holding PREP_CPER of them, with the first-pending slots in LINKHIT (free after the link
+  // responses).  Per-color counts, unit tags (sums and maxima) and the color-ordered row list (a stable scatter, or
   // for a grid team a stable sort by color) are then built in parallel.
   const uint32_t nbm = max(NB, 1u);
   const bool mask_lds = !TM::GRID && nbm * (NCOLW + 1) <= cap;
@@ -1711,7 +1716,7 @@ DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t c
         }
       }
     }
-  } else if (!TM::GRID) {
+  } else if (!TM::GRID && ncu <= PREP_SERIAL_COLOR) {
     // masks in global memory, one workgroup: lane 0 colors serially, a chunk of walk entries at a time
     for (uint32_t c0 = 0; c0 < ncu; c0 += PREP_CH) {
       const uint32_t n = min(PREP_CH, ncu - c0);
@@ -1737,7 +1742,7 @@ DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t c
     uint32_t* GM = sched + S.WMASK;
     for (uint32_t i = tid; i < nbm; i += NT) GO[i] = 0xffffffffu;
     tm.sync();
-    if (blockIdx.x == 0) {
+    if (!TM::GRID || blockIdx.x == 0) {
       constexpr uint32_t PER = PREP_CPER;
       const uint32_t lt = threadIdx.x;
       for (uint32_t c0 = 0; c0 < ncu; c0 += PER * BS) {
@@ -3538,11 +3543,14 @@ DEV void solve_world(const TM& tm, const WorldDev wd, uint32_t* Hs, float4* s_dy
     sv.ltree[l] = make_uint4(d, 0xffffffffu, 0xffffffffu, 0u);
   }
   tm.sync();
-  if (tid == 0)
-    for (uint32_t l = 0; l < NL; l++) {
+  // Children lists, a thread per articulation (a link's parent is in its articulation), links in order.
+  for (uint32_t a = tid; a < NA; a += NT) {
+    const uint4 ai = w.art_info(a);
+    for (uint32_t l = ai.x; l < ai.x + ai.y; l++) {
       int p = sv.mpar(l);
       if (p >= 0) { sv.ltree[l].z = sv.ltree[p].y; sv.ltree[p].y = l; }
     }
+  }
   tm.sync();
   const uint32_t JMw = w.JM();
   auto mirror_link = [&](uint32_t i) {  // link holding mirror float i (largest l with offset <= i)
