@@ -1240,7 +1240,9 @@ struct PrepLds {
 // Prepare of one world by a team.  cap: words of dynamic LDS (s_pdyn), which hold a block team's per-body color
 // masks and first-pending slots when NB * (NCOLW + 1) fits (steel_prep_lds_words); a grid team keeps them in
 // global memory.
-template <int BS, class TM>
+// PART: 0 the whole prepare; 1 up to the forward kinematics and factor, 2 from there on (a large world's FK and
+// factor run between them in k_art_factor, a workgroup per articulation over the whole device).
+template <int BS, class TM, int PART = 0>
 DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t cap, uint32_t* s_pdyn, PrepLds<BS>* pl) {
   uint32_t* s_wr = pl->wr;
   uint64_t* s_wk = pl->wk;
@@ -1264,6 +1266,7 @@ DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t c
   const uint32_t NCOL = w.NCOL(), NCOLW = w.h(W_NCOLW), PATCH = w.h(W_PATCH);
   uint32_t* sched = wd.sched;
   AS_GLOBAL float* art_ = (AS_GLOBAL float*)wd.art;
+  if constexpr (PART != 2) {
   // ---- Phase 0
   for (uint32_t i = tid; i < UH; i += NT) { w.sch64(S.HKEY)[i] = 0; sched[S.HCNT + i] = 0; }
   if (PATCH) for (uint32_t i = tid; i < AH; i += NT) wd.akey[(1 - P.parity) * AH + i] = 0;
@@ -1434,7 +1437,10 @@ DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t c
     }
     art_[w.h(W_A_RLN) + a] = (float)n;
   }
+  }
+  if constexpr (PART == 1) return;
   // FK and articulated-body factor, one wave per articulation, tree levels in parallel lanes.
+  if constexpr (PART == 0)
   for (uint32_t base = 0; base < NA; base += NT / 64) {
     const uint32_t a = base + (tid >> 6), k = tid & 63;
     if (a < NA) {
@@ -1887,7 +1893,7 @@ DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t c
 }
 
 
-template <int BS>
+template <int BS, int PART = 0>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_prepare(const WorldDev* __restrict__ worlds, uint32_t nworlds, uint32_t cap) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ TeamLds<BS> tl;
@@ -1896,7 +1902,26 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
   const BlockTeam<BS> tm{threadIdx.x, &tl, nullptr};
-  prepare_world<BS>(tm, worlds[wid], Hs, cap, s_pdyn, &pl);
+  prepare_world<BS, BlockTeam<BS>, PART>(tm, worlds[wid], Hs, cap, s_pdyn, &pl);
+}
+
+// Forward kinematics and the articulated-body factor of world 0's articulations, one 64-thread workgroup per
+// articulation (a, a + gridDim.x, ...): the same per-articulation passes prepare runs on its waves, spread over the
+// device for a world of many articulations.  Articulations share no links, so their order does not matter.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_PREPARE))) k_art_factor(const WorldDev* __restrict__ world) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  __shared__ uint32_t tree[64];
+  __shared__ float pab[512];
+  const WorldDev wd = world[0];
+  load_header(Hs, wd);
+  const W w = make_world(Hs, wd);
+  const uint32_t NA = w.NA(), k = threadIdx.x;
+  for (uint32_t a = blockIdx.x; a < NA; a += gridDim.x) {
+    w.art_tree(a, tree);
+    w.art_fk_lv(a, k < w.art_info(a).y ? (tree[k] & 0xffu) : 0xffu);
+    w.art_factor_lv(a, tree, pab);
+    __syncthreads();
+  }
 }
 
 // One large world over a cooperative launch (g: grid team scratch).
@@ -3813,7 +3838,12 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
     const size_t pb = (size_t)pw * 4;
     if (block == 64) hipLaunchKernelGGL((k_prepare<64>), grid, dim3(64), pb, stream, worlds, nworlds, pw);
     else if (block == 128) hipLaunchKernelGGL((k_prepare<128>), grid, dim3(128), pb, stream, worlds, nworlds, pw);
-    else if (block == 1024) hipLaunchKernelGGL((k_prepare<1024>), grid, dim3(1024), pb, stream, worlds, nworlds, pw);
+    else if (block == 1024 && nworlds == 1) {
+      // One large world: its many articulations' kinematics and factor over the device, between prepare's halves.
+      hipLaunchKernelGGL((k_prepare<1024, 1>), grid, dim3(1024), pb, stream, worlds, nworlds, pw);
+      hipLaunchKernelGGL(k_art_factor, dim3(1024), dim3(64), 0, stream, worlds);
+      hipLaunchKernelGGL((k_prepare<1024, 2>), grid, dim3(1024), pb, stream, worlds, nworlds, pw);
+    } else if (block == 1024) hipLaunchKernelGGL((k_prepare<1024>), grid, dim3(1024), pb, stream, worlds, nworlds, pw);
     else hipLaunchKernelGGL((k_prepare<256>), grid, dim3(256), pb, stream, worlds, nworlds, pw);
   }
   if (marks) (void)hipEventRecord(marks[2], stream);
