@@ -249,28 +249,51 @@ DEV bool cell_find(const uint64_t* T, uint32_t lg, uint64_t m, uint32_t& first, 
   }
 }
 
-// aux: the team's words (LDS for a block, global for a grid), set by the caller before its first barrier:
-// [0] pair count 0, [1] largest binned extent 0, [2..4] least cell per axis 0xffffffff, [5] binned count 0.
-template <int BS, class TM>
-DEV void broad_cells(const TM& tm, const WC& w, const WorldDev& wd, uint32_t* aux, float HB) {
-  const uint32_t tid = tm.tid, NT = tm.NT, NS = w.NS(), PA = w.h(W_P_AABB), MAXP = w.MAXP();
+// The cell broadphase's phases, each a loop over its items by thread `tid` of `NT`; a team runs them with a barrier
+// between phases, the device-wide form (a large world alone in its batch) as one launch each.  aux: the phases'
+// words, set before the first: [0] pair count 0, [1] largest binned extent 0, [2..4] least cell per axis
+// 0xffffffff, [5] binned count 0.
+DEV double cell_side(const uint32_t* aux, float HB) { const float H = ubits(aux[1]); return (double)(H > 0.0f ? H : HB) * 1.0078125; }
+DEV uint32_t cell_lg(uint32_t nsm) { uint32_t lg = 1; while ((1u << lg) < 2 * nsm) lg++; return lg; }
+// Each thread reduces over its shapes, then each wave (all lanes reach the end of the loop) before one lane updates
+// the shared words: a word in global memory is then updated once per wave, not once per shape.
+DEV uint32_t wave_max_u(uint32_t x) { for (int o = 32; o >= 1; o >>= 1) x = max(x, (uint32_t)__shfl_xor((int)x, o, 64)); return x; }
+DEV uint32_t wave_min_u(uint32_t x) { for (int o = 32; o >= 1; o >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, o, 64)); return x; }
+DEV uint32_t wave_sum_u(uint32_t x) { for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64); return x; }
+DEV void cells_extent(uint32_t tid, uint32_t NT, const WC& w, const WorldDev& wd, uint32_t* aux, float HB) {
+  const uint32_t NS = w.NS(), PA = w.h(W_P_AABB);
+  uint32_t m = 0;  // extents are >= 0, so their bits order as the floats do
   for (uint32_t s = tid; s < NS; s += NT) {
     const float4 lo = wd.poses[PA + s * 2], hi = wd.poses[PA + s * 2 + 1];
-    if (cell_binned(lo, hi, HB)) atomicMax(&aux[1], fbits(fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z)));
+    if (cell_binned(lo, hi, HB)) m = max(m, fbits(fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z)));
   }
-  tm.sync();
-  const float H = ubits(aux[1]);
-  const double Hc = (double)(H > 0.0f ? H : HB) * 1.0078125;
+  m = wave_max_u(m);
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(&aux[1], m);
+}
+DEV void cells_origin(uint32_t tid, uint32_t NT, const WC& w, const WorldDev& wd, uint32_t* aux, float HB) {
+  const uint32_t NS = w.NS(), PA = w.h(W_P_AABB);
+  const double Hc = cell_side(aux, HB);
+  uint32_t mx_ = 0xffffffffu, my_ = 0xffffffffu, mz_ = 0xffffffffu, n = 0;
   for (uint32_t s = tid; s < NS; s += NT) {
     const float4 lo = wd.poses[PA + s * 2], hi = wd.poses[PA + s * 2 + 1];
     if (!cell_binned(lo, hi, HB)) continue;
-    atomicMin(&aux[2], (uint32_t)(cell_floor(lo.x, Hc) + (1 << 30)));
-    atomicMin(&aux[3], (uint32_t)(cell_floor(lo.y, Hc) + (1 << 30)));
-    atomicMin(&aux[4], (uint32_t)(cell_floor(lo.z, Hc) + (1 << 30)));
-    atomicAdd(&aux[5], 1u);
+    mx_ = min(mx_, (uint32_t)(cell_floor(lo.x, Hc) + (1 << 30)));
+    my_ = min(my_, (uint32_t)(cell_floor(lo.y, Hc) + (1 << 30)));
+    mz_ = min(mz_, (uint32_t)(cell_floor(lo.z, Hc) + (1 << 30)));
+    n++;
   }
-  tm.sync();
-  const uint32_t ox = aux[2], oy = aux[3], oz = aux[4], nsm = aux[5];
+  mx_ = wave_min_u(mx_); my_ = wave_min_u(my_); mz_ = wave_min_u(mz_); n = wave_sum_u(n);
+  if ((threadIdx.x & 63) == 0 && n) {
+    atomicMin(&aux[2], mx_); atomicMin(&aux[3], my_); atomicMin(&aux[4], mz_);
+    atomicAdd(&aux[5], n);
+  }
+}
+// vary (device-wide form): or and and of the keys, for the sort's digit skipping.
+DEV void cells_keys(uint32_t tid, uint32_t NT, const WC& w, const WorldDev& wd, const uint32_t* aux, float HB, uint64_t* vary) {
+  const uint32_t NS = w.NS(), PA = w.h(W_P_AABB);
+  const double Hc = cell_side(aux, HB);
+  const uint32_t ox = aux[2], oy = aux[3], oz = aux[4];
+  uint64_t kor = 0, kand = ~0ull;
   for (uint32_t s = tid; s < NS; s += NT) {
     const float4 lo = wd.poses[PA + s * 2], hi = wd.poses[PA + s * 2 + 1];
     uint64_t key = CELL_NONE;
@@ -279,16 +302,26 @@ DEV void broad_cells(const TM& tm, const WC& w, const WorldDev& wd, uint32_t* au
                     min((uint32_t)(cell_floor(lo.y, Hc) + (1 << 30)) - oy, 0x1fffffu),
                     min((uint32_t)(cell_floor(lo.z, Hc) + (1 << 30)) - oz, 0x1fffffu));
     wd.skey[s] = key; wd.sval[s] = s;
+    kor |= key; kand &= key;
   }
-  tm.sync();
-  tm.sort(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), NS);
-  // Sorted bounds (lo.w: shape id, hi.w: pair tag) in the contact buffer, the cell table in the other one (at most
-  // 4 NS entries of 16 bytes; both buffers hold 96 MAXC bytes and MAXC >= NS).
+  if (vary) {
+    for (int o = 32; o >= 1; o >>= 1) {
+      kor |= (uint64_t)__shfl_xor((unsigned long long)kor, o, 64);
+      kand &= (uint64_t)__shfl_xor((unsigned long long)kand, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicOr((unsigned long long*)&vary[0], (unsigned long long)kor);
+      atomicAnd((unsigned long long*)&vary[1], (unsigned long long)kand);
+    }
+  }
+}
+// Sorted bounds (lo.w: shape id, hi.w: pair tag) in the raw contact buffer, the cell table (at most 4 NS entries of
+// 16 bytes) cleared in the contact buffer; both buffers hold 96 MAXC bytes and MAXC >= NS.
+DEV void cells_gather(uint32_t tid, uint32_t NT, const WC& w, const WorldDev& wd, const uint32_t* aux) {
+  const uint32_t NS = w.NS(), PA = w.h(W_P_AABB), lg = cell_lg(aux[5]);
   float4* s_lo = wd.craw;
   float4* s_hi = s_lo + NS;
   uint64_t* T = (uint64_t*)wd.contacts;
-  uint32_t lg = 1;
-  while ((1u << lg) < 2 * nsm) lg++;
   for (uint32_t p = tid; p < NS; p += NT) {
     const uint32_t si = wd.sval[p];
     float4 lo = wd.poses[PA + si * 2], hi = wd.poses[PA + si * 2 + 1];
@@ -297,7 +330,10 @@ DEV void broad_cells(const TM& tm, const WC& w, const WorldDev& wd, uint32_t* au
     s_lo[p] = lo; s_hi[p] = hi;
   }
   for (uint32_t e = tid; e < (1u << lg); e += NT) T[2 * e] = 0;
-  tm.sync();
+}
+DEV void cells_insert(uint32_t tid, uint32_t NT, const WorldDev& wd, const uint32_t* aux) {
+  const uint32_t nsm = aux[5], lg = cell_lg(nsm);
+  uint64_t* T = (uint64_t*)wd.contacts;
   for (uint32_t p = tid; p < nsm; p += NT) {
     const uint64_t m = wd.skey[p];
     if (p > 0 && wd.skey[p - 1] == m) continue;
@@ -305,14 +341,30 @@ DEV void broad_cells(const TM& tm, const WC& w, const WorldDev& wd, uint32_t* au
     while (q < nsm && wd.skey[q] == m) q++;
     cell_insert(T, lg, m, p, q);
   }
-  tm.sync();
-  auto test = [&](const float4& lo_i, const float4& hi_i, const float4& lo_j, const float4& hi_j) {
-    if (lo_i.x > hi_j.x || lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.x > hi_i.x || lo_j.y > hi_i.y || lo_j.z > hi_i.z) return;
-    const uint32_t si = fbits(lo_i.w), sj = fbits(lo_j.w), a = min(si, sj), b = max(si, sj);
-    if (!pair_allowed(w, a, b, fbits(hi_i.w), fbits(hi_j.w))) return;
-    const uint32_t slot = atomicAdd(&aux[0], 1u);
+}
+// The reference's test of one candidate; the pairs that pass are appended by their wave with one counter update
+// (slots in lane order), so a counter in global memory is not hit once per pair.  Called by every active lane.
+DEV void cell_test(const WC& w, const WorldDev& wd, uint32_t* np, uint32_t MAXP, const float4& lo_i, const float4& hi_i,
+                   const float4& lo_j, const float4& hi_j) {
+  const uint32_t si = fbits(lo_i.w), sj = fbits(lo_j.w), a = min(si, sj), b = max(si, sj);
+  const bool hit = !(lo_i.x > hi_j.x || lo_i.y > hi_j.y || lo_i.z > hi_j.z || lo_j.x > hi_i.x || lo_j.y > hi_i.y || lo_j.z > hi_i.z) &&
+                   pair_allowed(w, a, b, fbits(hi_i.w), fbits(hi_j.w));
+  const unsigned long long m = __ballot(hit);
+  if (m == 0) return;
+  const uint32_t lane = threadIdx.x & 63, leader = (uint32_t)__ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(np, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader, 64);
+  if (hit) {
+    const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     if (slot < MAXP) { wd.pairs[slot * 2] = a; wd.pairs[slot * 2 + 1] = b; }
-  };
+  }
+}
+DEV void cells_pairs(uint32_t tid, uint32_t NT, const WC& w, const WorldDev& wd, uint32_t* aux) {
+  const uint32_t NS = w.NS(), MAXP = w.MAXP(), nsm = aux[5], lg = cell_lg(nsm);
+  const float4* s_lo = wd.craw;
+  const float4* s_hi = s_lo + NS;
+  const uint64_t* T = (const uint64_t*)wd.contacts;
   for (uint32_t p = tid; p < nsm; p += NT) {
     const uint64_t m = wd.skey[p];
     const uint32_t cx = morton_compact(m), cy = morton_compact(m >> 1), cz = morton_compact(m >> 2);
@@ -330,14 +382,37 @@ DEV void broad_cells(const TM& tm, const WC& w, const WorldDev& wd, uint32_t* au
       } else if (!cell_find(T, lg, morton3(nx, ny, nz), q0, q1)) {
         continue;
       }
-      for (uint32_t q = q0; q < q1; q++) test(lo_p, hi_p, s_lo[q], s_hi[q]);
+      for (uint32_t q = q0; q < q1; q++) cell_test(w, wd, &aux[0], MAXP, lo_p, hi_p, s_lo[q], s_hi[q]);
     }
   }
+}
+DEV void cells_big(uint32_t tid, uint32_t NT, const WC& w, const WorldDev& wd, uint32_t* aux) {
+  const uint32_t NS = w.NS(), MAXP = w.MAXP(), nsm = aux[5];
+  const float4* s_lo = wd.craw;
+  const float4* s_hi = s_lo + NS;
   for (uint32_t b = nsm; b < NS; b++) {  // shapes not binned, against every shape (a pair of two of them once)
     const float4 lo_b = s_lo[b], hi_b = s_hi[b];
     for (uint32_t q = tid; q < NS; q += NT)
-      if (q < nsm || q > b) test(lo_b, hi_b, s_lo[q], s_hi[q]);
+      if (q < nsm || q > b) cell_test(w, wd, &aux[0], MAXP, lo_b, hi_b, s_lo[q], s_hi[q]);
   }
+}
+
+template <int BS, class TM>
+DEV void broad_cells(const TM& tm, const WC& w, const WorldDev& wd, uint32_t* aux, float HB) {
+  const uint32_t tid = tm.tid, NT = tm.NT;
+  cells_extent(tid, NT, w, wd, aux, HB);
+  tm.sync();
+  cells_origin(tid, NT, w, wd, aux, HB);
+  tm.sync();
+  cells_keys(tid, NT, w, wd, aux, HB, nullptr);
+  tm.sync();
+  tm.sort(wd.skey, wd.sval, wd.skey + w.MAXC(), wd.sval + w.MAXC(), w.NS());
+  cells_gather(tid, NT, w, wd, aux);
+  tm.sync();
+  cells_insert(tid, NT, wd, aux);
+  tm.sync();
+  cells_pairs(tid, NT, w, wd, aux);
+  cells_big(tid, NT, w, wd, aux);
   tm.sync();
   if (tid == 0) { wd.ctr[0] = aux[0]; wd.ctr[1] = 0; }
 }
@@ -454,6 +529,146 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   __shared__ SortLds<BS> sl;
   const GridTeam<BS> tm{gridDim.x * BS, blockIdx.x * BS + threadIdx.x, &tl, &sl, g};
   broad_world<BS>(tm, world[0], Hs, 0, nullptr, g + 3);
+}
+
+// ---- Device-wide cell broadphase of a large world alone in its batch (world 0): the phases above, a launch each,
+// and a stable LSD radix sort of the Morton keys by digit-count, scan and scatter launches (digits no key varies in
+// skipped).  Scratch: the contact buffer past the cell table (which takes at most 64 NS of its 96 MAXC bytes):
+// [0..15] the phases' words, [16..19] the keys' or and and, [64..] 256 digit counts per sort chunk.
+static constexpr uint32_t CELLW_BS = 256, CELLW_MAXG = 128;
+DEV uint32_t* cellw_scratch(const WorldDev& wd, uint32_t MAXC) { return (uint32_t*)(wd.contacts + 4 * (size_t)MAXC); }
+DEV uint32_t cellw_chunk(uint32_t NS) {  // keys per sort chunk: a multiple of the block, at most CELLW_MAXG chunks
+  uint32_t ch = 4 * CELLW_BS;
+  while ((NS + ch - 1) / ch > CELLW_MAXG) ch += CELLW_BS;
+  return ch;
+}
+#define CELLW_VIEW                                          \
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];         \
+  const WorldDev wd = world[0];                             \
+  load_header(Hs, wd);                                      \
+  const WC w = make_world<false>(Hs, wd);                   \
+  const float HB = w.hf(W_CELL);                            \
+  uint32_t* aux = cellw_scratch(wd, w.MAXC());              \
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, NT = gridDim.x * blockDim.x;
+__global__ void __launch_bounds__(64) k_cellw_init(const WorldDev* __restrict__ world) {
+  CELLW_VIEW
+  (void)HB; (void)NT;
+  if (tid == 0) {
+    aux[0] = 0; aux[1] = 0; aux[2] = aux[3] = aux[4] = 0xffffffffu; aux[5] = 0;
+    uint64_t* vary = (uint64_t*)(aux + 16);
+    vary[0] = 0; vary[1] = ~0ull;
+  }
+}
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_pose(const WorldDev* __restrict__ world) {
+  CELLW_VIEW
+  for (uint32_t s = tid; s < w.NS(); s += NT) shape_pose(w, s);
+  cells_extent(tid, NT, w, wd, aux, HB);  // the same thread reads back the boxes it wrote
+}
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_origin(const WorldDev* __restrict__ world) { CELLW_VIEW cells_origin(tid, NT, w, wd, aux, HB); }
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_keys(const WorldDev* __restrict__ world) {
+  CELLW_VIEW
+  cells_keys(tid, NT, w, wd, aux, HB, (uint64_t*)(aux + 16));
+}
+// The pass on bits [sh, sh + 8): whether any key varies there, and whether the keys are in (skey, sval) before it
+// (an even number of earlier passes ran) or in the second halves.
+DEV bool cellw_pass(const uint32_t* aux, uint32_t sh, bool& in_first) {
+  const uint64_t* vary = (const uint64_t*)(aux + 16);
+  const uint64_t v = vary[0] ^ vary[1];
+  uint32_t before = 0;
+  for (uint32_t d = 0; d < sh; d += 8) before += ((v >> d) & 0xffull) != 0;
+  in_first = (before & 1u) == 0;
+  return ((v >> sh) & 0xffull) != 0;
+}
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_count(const WorldDev* __restrict__ world, uint32_t sh) {
+  CELLW_VIEW
+  (void)HB; (void)tid; (void)NT;
+  __shared__ uint32_t hist[256];
+  bool in_first;
+  if (!cellw_pass(aux, sh, in_first)) return;
+  const uint32_t NS = w.NS(), MAXC = w.MAXC(), ch = cellw_chunk(NS), G = (NS + ch - 1) / ch, b = blockIdx.x;
+  if (b >= G) return;
+  const uint64_t* src = in_first ? wd.skey : wd.skey + MAXC;
+  for (uint32_t d = threadIdx.x; d < 256; d += CELLW_BS) hist[d] = 0;
+  __syncthreads();
+  const uint32_t c0 = b * ch, c1 = min(c0 + ch, NS);
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += CELLW_BS) atomicAdd(&hist[(uint32_t)(src[i] >> sh) & 0xffu], 1u);
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < 256; d += CELLW_BS) aux[64 + d * G + b] = hist[d];
+}
+__global__ void __launch_bounds__(1024) k_cellw_scan(const WorldDev* __restrict__ world, uint32_t sh) {
+  CELLW_VIEW
+  (void)HB; (void)tid; (void)NT;
+  __shared__ uint32_t s_w[1024 / 64];
+  bool in_first;
+  if (!cellw_pass(aux, sh, in_first)) return;
+  const uint32_t NS = w.NS(), ch = cellw_chunk(NS), G = (NS + ch - 1) / ch;
+  block_scan_array<1024>(aux + 64, 256 * G, s_w);
+}
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_scatter(const WorldDev* __restrict__ world, uint32_t sh) {
+  CELLW_VIEW
+  (void)HB; (void)tid; (void)NT;
+  constexpr uint32_t NW = CELLW_BS / 64;
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t wcnt[NW * 256];
+  bool in_first;
+  if (!cellw_pass(aux, sh, in_first)) return;
+  const uint32_t NS = w.NS(), MAXC = w.MAXC(), ch = cellw_chunk(NS), G = (NS + ch - 1) / ch, b = blockIdx.x;
+  if (b >= G) return;
+  const uint64_t* src = in_first ? wd.skey : wd.skey + MAXC;
+  const uint32_t* srcv = in_first ? wd.sval : wd.sval + MAXC;
+  uint64_t* dst = in_first ? wd.skey + MAXC : wd.skey;
+  uint32_t* dstv = in_first ? wd.sval + MAXC : wd.sval;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t d = threadIdx.x; d < 256; d += CELLW_BS) hist[d] = aux[64 + d * G + b];
+  for (uint32_t i = threadIdx.x; i < NW * 256; i += CELLW_BS) wcnt[i] = 0;
+  __syncthreads();
+  const uint32_t c0 = b * ch, c1 = min(c0 + ch, NS);
+  for (uint32_t base = c0; base < c1; base += CELLW_BS) {  // tiles keep their order; lanes rank by ballots
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < c1;
+    const uint64_t key = valid ? src[i] : 0;
+    const uint32_t val = valid ? srcv[i] : 0;
+    const uint32_t dg = (uint32_t)(key >> sh) & 0xffu;
+    unsigned long long peers = __ballot(valid);
+    for (uint32_t k = 0; k < 8; k++) {
+      unsigned long long m = __ballot((dg >> k) & 1u);
+      peers &= ((dg >> k) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    if (valid && rank == 0) wcnt[wave * 256 + dg] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = hist[dg] + rank;
+      for (uint32_t w2 = 0; w2 < wave; w2++) pos += wcnt[w2 * 256 + dg];
+      dst[pos] = key; dstv[pos] = val;
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < 256; d += CELLW_BS) {
+      uint32_t sum = 0;
+      for (uint32_t w2 = 0; w2 < NW; w2++) { sum += wcnt[w2 * 256 + d]; wcnt[w2 * 256 + d] = 0; }
+      hist[d] += sum;
+    }
+    __syncthreads();
+  }
+}
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_sortfix(const WorldDev* __restrict__ world) {
+  CELLW_VIEW
+  (void)HB;
+  bool last_first;  // where the keys were before the last digit's pass, and whether that pass ran
+  const bool last = cellw_pass(aux, 56, last_first);
+  const bool sorted_in_first = last ? !last_first : last_first;
+  if (sorted_in_first) return;
+  const uint32_t MAXC = w.MAXC();
+  for (uint32_t i = tid; i < w.NS(); i += NT) { wd.skey[i] = wd.skey[MAXC + i]; wd.sval[i] = wd.sval[MAXC + i]; }
+}
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_gather(const WorldDev* __restrict__ world) { CELLW_VIEW (void)HB; cells_gather(tid, NT, w, wd, aux); }
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_insert(const WorldDev* __restrict__ world) { CELLW_VIEW (void)HB; cells_insert(tid, NT, wd, aux); }
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_pairs(const WorldDev* __restrict__ world) { CELLW_VIEW (void)HB; cells_pairs(tid, NT, w, wd, aux); }
+__global__ void __launch_bounds__(CELLW_BS) k_cellw_big(const WorldDev* __restrict__ world) { CELLW_VIEW (void)HB; cells_big(tid, NT, w, wd, aux); }
+__global__ void __launch_bounds__(64) k_cellw_finish(const WorldDev* __restrict__ world) {
+  CELLW_VIEW
+  (void)HB; (void)NT;
+  if (tid == 0) { wd.ctr[0] = aux[0]; wd.ctr[1] = 0; }
 }
 
 // Exclusive prefix of the worlds' pair counts (min(ctr[0], MAXP)) into offs[0..n], one block.
@@ -3823,7 +4038,25 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
   if (stages & STAGE_COLLIDE) {
     // Workgroups of 1024 (large worlds) keep the broadphase's sorted bounds in global memory (cap 0): 1024 threads'
     // sort LDS and a 4096-shape bounds cache would not fit one CU's LDS together.
-    if (block == 1024) hipLaunchKernelGGL((k_broad<1024>), grid, dim3(1024), 16, stream, worlds, nworlds, 0u);
+    if (block == 1024 && nworlds == 1 && sl.cell) {
+      // A large world alone in its batch: the cell broadphase's phases over the device, a launch each.
+      const dim3 g(1024), bs(CELLW_BS);
+      hipLaunchKernelGGL(k_cellw_init, dim3(1), dim3(64), 0, stream, worlds);
+      hipLaunchKernelGGL(k_cellw_pose, g, bs, 0, stream, worlds);
+      hipLaunchKernelGGL(k_cellw_origin, g, bs, 0, stream, worlds);
+      hipLaunchKernelGGL(k_cellw_keys, g, bs, 0, stream, worlds);
+      for (uint32_t sh = 0; sh < 64; sh += 8) {
+        hipLaunchKernelGGL(k_cellw_count, dim3(CELLW_MAXG), bs, 0, stream, worlds, sh);
+        hipLaunchKernelGGL(k_cellw_scan, dim3(1), dim3(1024), 0, stream, worlds, sh);
+        hipLaunchKernelGGL(k_cellw_scatter, dim3(CELLW_MAXG), bs, 0, stream, worlds, sh);
+      }
+      hipLaunchKernelGGL(k_cellw_sortfix, g, bs, 0, stream, worlds);
+      hipLaunchKernelGGL(k_cellw_gather, g, bs, 0, stream, worlds);
+      hipLaunchKernelGGL(k_cellw_insert, g, bs, 0, stream, worlds);
+      hipLaunchKernelGGL(k_cellw_pairs, g, bs, 0, stream, worlds);
+      hipLaunchKernelGGL(k_cellw_big, g, bs, 0, stream, worlds);
+      hipLaunchKernelGGL(k_cellw_finish, dim3(1), dim3(64), 0, stream, worlds);
+    } else if (block == 1024) hipLaunchKernelGGL((k_broad<1024>), grid, dim3(1024), 16, stream, worlds, nworlds, 0u);
     else hipLaunchKernelGGL((k_broad<256>), grid, dim3(256), sl.broad_bytes(), stream, worlds, nworlds, sl.broad_cap());
     if (stages & STAGE_NARROW) {
       hipLaunchKernelGGL((k_pair_offsets<1024>), dim3(1), dim3(1024), 0, stream, worlds, nworlds, pair_offs);

@@ -44,8 +44,10 @@ struct LaunchLds {
   size_t notw = 0, tw = 0;
   uint32_t ns = 0, cw = 0;
   bool mgl = false;
-  void add(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, uint32_t NS, uint32_t NCOLW, bool nest) {
+  bool cell = false;  // some world has a cell broadphase bound (W_CELL)
+  void add(uint32_t NL, uint32_t ND, uint32_t mfloats, uint32_t NB, uint32_t NS, uint32_t NCOLW, bool nest, bool has_cell = false) {
     ns = ns > NS ? ns : NS;
+    cell = cell || has_cell;
     uint32_t c = (NB > 0 ? NB : 1) * (NCOLW + 1);
     cw = cw > c ? cw : c;
     if (solve_lds_bytes(NL, ND, mfloats, NB, false, nest) > STEEL_SOLVE_LDS_MAX) { mgl = true; return; }

@@ -116,7 +116,7 @@ inline int world_solve_lds(const uint32_t* h, const float* statF, LaunchLds& sl,
   uint32_t mf = world_mirror_floats(h, statF);
   if (!global_mirror && solve_lds_bytes(h[W_NL], h[W_ND], mf, h[W_NB], false, h[W_NEST] != 0) > STEEL_SOLVE_LDS_MAX)
     return fail(STEEL_UNSUPPORTED, "a world of %u articulation links exceeds the solve kernel's LDS link mirror", h[W_NL]);
-  sl.add(h[W_NL], h[W_ND], mf, h[W_NB], h[W_NS], h[W_NCOLW], h[W_NEST] != 0);
+  sl.add(h[W_NL], h[W_ND], mf, h[W_NB], h[W_NS], h[W_NCOLW], h[W_NEST] != 0, h[W_CELL] != 0);
   return STEEL_OK;
 }
 

@@ -182,15 +182,17 @@ def test_device_wide_refuses_what_it_does_not_step(sim, lattice):
     b.destroy()
 
 
+@pytest.mark.parametrize("block", [256, 1024], ids=["workgroup", "device_wide_broadphase"])
 @pytest.mark.parametrize("far", [False, True], ids=["cluster", "far_clamped"])
-def test_cell_broadphase_mixed_sizes_bitexact(sim, far):
+def test_cell_broadphase_mixed_sizes_bitexact(sim, far, block):
     """The cell broadphase on a world of 4600 small boxes at random overlapping poses, large free and static boxes
     it does not bin, and the ground plane (with `far`, boxes 5e6 m out clamp every cell coordinate): after one frame
-    and after ten, the pair set, sorted contacts, colors and state equal the oracle's."""
+    and after ten, the pair set, sorted contacts, colors and state equal the oracle's.  With 1024-thread workgroups
+    the world, alone in its batch, takes the device-wide broadphase (a launch per phase, a multi-launch radix sort)."""
     pw = pack.pack_scene(workloads.cell_mix(far=far), capacity_policy="scaled")
     assert pw.NS > pack.CELL_SHAPES and pw.hf("W_CELL") > 0
     S = pw.h("W_SUBSTEPS")
-    b = sim.Batch([pw], block=256, graph=False)
+    b = sim.Batch([pw], block=block, graph=False)
     o = OracleWorld(pw)
     for frames in (1, 9):
         b.step(frames)
