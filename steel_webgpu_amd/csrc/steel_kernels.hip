@@ -3377,6 +3377,65 @@ struct ArtLds {
     uint32_t b = mbody(l);
     tw[b * 2] = tov(F4(v.l, 0.0f)); tw[b * 2 + 1] = tov(F4(v.a, 0.0f));
   }
+  // Joint-limit, friction and drive rows (Solver::art_propagate_rows with LDS-typed accesses).
+  DEV void propagate_rows(const Solver& sv, uint32_t a, uint32_t lane) const {
+    const W& w = sv.w;
+    const bool vpass = sv.vpass;
+    const v4u ai = art_info[a];
+    AS_GLOBAL float* art_ = (AS_GLOBAL float*)w.d.art;
+    uint32_t n = tu(art_[w.h(W_A_RLN) + a]);
+    float dt = w.hf(W_DT), beta = w.hf(W_BETA), maxd = w.hf(W_MAX_DEPEN), omega = w.hf(W_F_OMEGA);
+    for (uint32_t i = 0; i < n; i++) {
+      uint32_t dd = tu(art_[w.h(W_A_RL) + ai.z + i]);
+      float4 lim = w.dof_limit(dd);
+      uint32_t l = tu(lim.w), r = dd - w.link_dof(l);
+      float hinv = mx(art_[w.h(W_A_HINV) + dd], 1e-12f);
+      const float4 dr = w.dof_drive(dd);
+      if (w.h(W_DRIVE_ROWS) && (dr.y > 0.0f || dr.z > 0.0f) && !(vpass && !w.h(W_VEL_DRIVE_ACTIVE))) {
+        // drive_mode rows (wgsl.js:3425-3438): a PhysX-style drive row, cold-started each step, not relaxed.
+        const float q = art_[w.h(W_A_Q) + w.link_qs(l) + r];
+        const float pd = vpass ? dt * art_[w.h(W_A_SNAP) + w.NB() * 8 + dd] : 0.0f;
+        const float ca = dt * (dt * dr.y + dr.z);
+        const float x = 1.0f / (1.0f + ca * hinv);
+        const float bias = x * dt * dr.z * art_[w.h(W_A_TGTV) + dd] + dr.y * x * dt * (art_[w.h(W_A_TGT) + dd] - q - pd);
+        const float cap = dr.w > 0.0f ? dr.w * dt : 1e20f;
+        const float lam = art_[w.h(W_A_DRV) + dd];
+        const float nl = clampf_(lam * (1.0f - x) - x * ca * laqd[dd] + bias, -cap, cap);
+        if (nl != lam) {
+          art_[w.h(W_A_DRV) + dd] = nl;
+          if (lane == 0) mTAU(l)[r] = nl - lam;
+          apply(a, lane);
+        }
+      }
+      // Joint friction: a velocity row with |impulse| <= friction * dt (plus the load term).
+      float fr = w.dof_passive(dd).y;
+      if (w.h(W_JFL)) fr += art_[w.h(W_A_JFL) + dd];
+      if (fr > 0.0f) {
+        float lam = art_[w.h(W_A_FRIC) + dd];
+        float nl = clampf_(lam - omega * laqd[dd] / hinv, -fr * dt, fr * dt);
+        if (nl != lam) {
+          art_[w.h(W_A_FRIC) + dd] = nl;
+          if (lane == 0) mTAU(l)[r] = nl - lam;
+          apply(a, lane);
+        }
+      }
+      if (art_[w.h(W_A_LG) + dd] > 0.5f) {
+        float q = art_[w.h(W_A_Q) + w.link_qs(l) + r];
+        for (uint32_t side = 0; side < 2; side++) {
+          float sg = side == 0 ? 1.0f : -1.0f;
+          float gap = side == 0 ? q - lim.x : lim.y - q;
+          float lam = art_[w.h(W_A_LIM) + 2 * dd + side];
+          float rhs = gap / dt;
+          if (gap < 0.0f) rhs = vpass ? 0.0f : mx(beta * gap / dt, -maxd);
+          float nl = mx(lam - omega * (sg * laqd[dd] + rhs) / hinv, 0.0f);
+          if (nl == lam) continue;
+          art_[w.h(W_A_LIM) + 2 * dd + side] = nl;
+          if (lane == 0) mTAU(l)[r] = sg * (nl - lam);
+          apply(a, lane);
+        }
+      }
+    }
+  }
   // Link twists from the joint velocities, root first (Solver::art_twists with LDS-typed accesses).
   DEV void twists(uint32_t a, uint32_t lane) const {  // wgsl.js:3240-3251
     const v4u ai = art_info[a];
@@ -3512,7 +3571,8 @@ DEV void sweep(const TM& tm, const Solver& sv, uint32_t mode, uint32_t ncol, uin
   const bool ext = w.h(W_FMODE) || w.h(W_ANG) || w.h(W_TORS) || w.h(W_COMPL);
   // Out-of-line calls spill the caller's live registers: call only for articulations with joint rows.
   for (uint32_t a = wave; a < NA; a += NT / 64)
-    if (!FAST || w.d.art[w.h(W_A_RLN) + a] != 0.0f) sv.art_propagate_rows(a, lane);
+    if (!FAST) sv.art_propagate_rows(a, lane);
+    else if (w.d.art[w.h(W_A_RLN) + a] != 0.0f) art_lds<TWL>(sv).propagate_rows(sv, a, lane);
   if (w.h(W_PATCH) && mode == 1) {
     if (FAST) {
       const AS_GLOBAL uint32_t* ust = (const AS_GLOBAL uint32_t*)(sched + S.USTART);
