@@ -1659,10 +1659,10 @@ DEV void prepare_world(const TM& tm, const WorldDev wd, uint32_t* Hs, uint32_t c
   for (uint32_t base = 0; base < NA; base += NT / 64) {
     const uint32_t a = base + (tid >> 6), k = tid & 63;
     if (a < NA) {
-      uint32_t* tree = s_tree + (threadIdx.x >> 6) * 64;
+      AS_LDS uint32_t* tree = (AS_LDS uint32_t*)(s_tree + (threadIdx.x >> 6) * 64);
       w.art_tree(a, tree);
       w.art_fk_lv(a, k < w.art_info(a).y ? (tree[k] & 0xffu) : 0xffu);
-      w.art_factor_lv(a, tree, s_pab + (threadIdx.x >> 6) * 512);
+      w.art_factor_lv(a, tree, (AS_LDS float*)(s_pab + (threadIdx.x >> 6) * 512));
     }
   }
   tm.sync();
@@ -2132,9 +2132,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const W w = make_world(Hs, wd);
   const uint32_t NA = w.NA(), k = threadIdx.x;
   for (uint32_t a = blockIdx.x; a < NA; a += gridDim.x) {
-    w.art_tree(a, tree);
+    w.art_tree(a, (AS_LDS uint32_t*)tree);
     w.art_fk_lv(a, k < w.art_info(a).y ? (tree[k] & 0xffu) : 0xffu);
-    w.art_factor_lv(a, tree, pab);
+    w.art_factor_lv(a, (const AS_LDS uint32_t*)tree, (AS_LDS float*)pab);
     __syncthreads();
   }
 }

@@ -48,6 +48,9 @@ DEV int key_b(uint64_t k) { return (int)(uint32_t)(k >> 32) - 1; }
 #ifndef AS_GLOBAL
 #define AS_GLOBAL __attribute__((address_space(1)))
 #endif
+#ifndef AS_LDS
+#define AS_LDS __attribute__((address_space(3)))
+#endif
 typedef float gv4f __attribute__((ext_vector_type(4)));
 
 // Prepare's LDS mirror of the factored link data an articulation's solves read (null: read the link blocks).
@@ -492,7 +495,7 @@ struct WV {
   // highest-index child (0xff: none), 16-23 the next lower sibling; pabuf: 8 floats per link.  Children are
   // folded into a parent in decreasing link order, the order of the serial pass, so results match it bit
   // for bit.
-  DEVN void art_factor_lv(uint32_t a, const uint32_t* tree, float* pabuf) const {
+  DEVN void art_factor_lv(uint32_t a, const AS_LDS uint32_t* tree, AS_LDS float* pabuf) const {
     const uint32_t lane = threadIdx.x & 63, MP = hu(W_MAXPATH);
     const uint4 ai = art_info(a);
     const bool has = lane < ai.y;
@@ -505,13 +508,13 @@ struct WV {
     for (int dd = (int)MP - 1; dd >= 0; dd--) {
       if (dl == (uint32_t)dd) {
         SVd pa = factor_bwd_link(l);
-        float* pb = pabuf + lane * 8;
+        AS_LDS float* pb = pabuf + lane * 8;
         pb[0] = pa.a.x; pb[1] = pa.a.y; pb[2] = pa.a.z; pb[3] = pa.l.x; pb[4] = pa.l.y; pb[5] = pa.l.z;
       }
       wave_sync();
       if (dd > 0 && dl == (uint32_t)(dd - 1)) {
         for (uint32_t c = (tree[lane] >> 8) & 0xffu; c != 0xffu; c = (tree[c] >> 16) & 0xffu) {
-          const float* pb = pabuf + c * 8;
+          const AS_LDS float* pb = pabuf + c * 8;
           factor_fold_child(l, ai.x + c, mksv(mk3(pb[0], pb[1], pb[2]), mk3(pb[3], pb[4], pb[5])));
         }
       }
@@ -523,7 +526,7 @@ struct WV {
     }
   }
   // tree of articulation a for art_factor_lv (one wave; lane k fills entry k).
-  DEVN void art_tree(uint32_t a, uint32_t* tree) const {
+  DEVN void art_tree(uint32_t a, AS_LDS uint32_t* tree) const {
     const uint32_t lane = threadIdx.x & 63;
     const uint4 ai = art_info(a);
     if (lane < ai.y) {
