@@ -686,7 +686,7 @@ __global__ void __launch_bounds__(BS) k_pair_offsets(const WorldDev* __restrict_
 }
 
 // Narrowphase over every candidate pair of every world: one wave64 per pair (wgsl.js:1063-1195).  The
-// pairs are laid end to end in world order (offs) and cut into chunks of CH = clamp(total / waves, 1, 4)
+// pairs are laid end to end in world order (offs) and cut into chunks of CH = clamp(total / waves, 1, 8)
 // consecutive pairs; wave g takes chunks g, g + G, g + 2G, ... so every wave draws a similar mix of cheap and
 // expensive pairs from all worlds, and a world with many pairs is spread over many CUs.  Contacts take slots
 // from the world's counter ctr[1]; their order is made canonical by k_csort.
@@ -714,7 +714,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEEL_
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t G = gridDim.x * 4, g = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t total = O[nworlds];
-  const uint32_t CH = max(1u, min(4u, total / G));
+  const uint32_t CH = max(1u, min(8u, total / G));
   const uint32_t nch = (total + CH - 1) / CH;
   for (uint32_t c = g; c < nch; c += G) {
     const uint32_t i0 = c * CH, i1 = min(i0 + CH, total);
