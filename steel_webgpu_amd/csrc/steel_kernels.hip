@@ -3786,7 +3786,35 @@ __device__ __noinline__ void velocity_pass(const Solver& sv, uint32_t ncol, uint
 
 // MGL: the launch holds a world whose link records and mirror live in global memory (WorldDev::mscr); every world of
 // the launch then runs the generic-pointer sweep, the others keeping their mirror in LDS.
-template <int BS, bool TWL, bool MGL, class TM>
+// Predictor of body b (wgsl.js solve prologue): impulses and twists start at zero; a free body's twist is its
+// velocity with gravity, applied forces and the gyroscopic term.
+DEV void predict_body(const Solver& sv, const WorldDev& wd, uint32_t b, f3 g, float dt) {
+  const W& w = sv.w;
+  uint32_t kind = w.body_kind((int)b);
+  sv.imp[b * 2] = make_float4(0, 0, 0, 0); sv.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
+  // Bodies other than free ones: zero (static bodies keep it; link bodies get theirs from art_twists below).
+  sv.tw[b * 2] = make_float4(0, 0, 0, 0); sv.tw[b * 2 + 1] = make_float4(0, 0, 0, 0);
+  if (kind != 1) return;
+  uint32_t fl = w.body_flags((int)b);
+  f3 v = xyz(wd.vel[b * 2]) + ((w.node_asleep((int)b) || fl != 0) ? mk3(0, 0, 0) : g * dt);
+  f3 wv = xyz(wd.vel[b * 2 + 1]);
+  uint32_t bfo = w.h(W_A_BF) + b * 8;
+  if ((fl & 1) == 0 && (vne0(w.ld3(bfo)) || vne0(w.ld3(bfo + 4)))) {
+    m3 R = qmat(w.pose_q((int)b));
+    v += w.ld3(bfo) * w.body_inv_mass((int)b) * dt;
+    wv += mul(R, mul(w.body_inv_inertia((int)b), mul(tr(R), w.ld3(bfo + 4)))) * dt;
+  }
+  m3 I = w.body_inertia((int)b);
+  bool iso = I.c[0].x == I.c[1].y && I.c[1].y == I.c[2].z && I.c[0].y == 0.0f && I.c[0].z == 0.0f && I.c[1].z == 0.0f;
+  if ((fl & 1) == 0 && !iso && vne0(wv)) {
+    m3 R = qmat(w.pose_q((int)b));
+    wv = mul(R, sv.gyroscopic_velocity(I, mul(tr(R), wv), dt));
+  }
+  sv.tw[b * 2] = F4(v, 0.0f); sv.tw[b * 2 + 1] = F4(wv, 0.0f);
+}
+
+// PRED: the body predictor runs here (false: k_solve_pred ran it over the device first).
+template <int BS, bool TWL, bool MGL, class TM, bool PRED = true>
 DEV void solve_world(const TM& tm, const WorldDev wd, uint32_t* Hs, float4* s_dyn4) {
   load_header(Hs, wd);
   Solver sv;
@@ -3874,29 +3902,8 @@ DEV void solve_world(const TM& tm, const WorldDev wd, uint32_t* Hs, float4* s_dy
       for (uint32_t dd = ai.z; dd < ai.z + ai.w; dd++) sv.laqd[dd] = 0.0f;
     }
   }
-  for (uint32_t b = tid; b < NB; b += NT) {
-    uint32_t kind = w.body_kind((int)b);
-    sv.imp[b * 2] = make_float4(0, 0, 0, 0); sv.imp[b * 2 + 1] = make_float4(0, 0, 0, 0);
-    // Bodies other than free ones: zero (static bodies keep it; link bodies get theirs from art_twists below).
-    sv.tw[b * 2] = make_float4(0, 0, 0, 0); sv.tw[b * 2 + 1] = make_float4(0, 0, 0, 0);
-    if (kind != 1) continue;
-    uint32_t fl = w.body_flags((int)b);
-    f3 v = xyz(wd.vel[b * 2]) + ((w.node_asleep((int)b) || fl != 0) ? mk3(0, 0, 0) : g * dt);
-    f3 wv = xyz(wd.vel[b * 2 + 1]);
-    uint32_t bfo = w.h(W_A_BF) + b * 8;
-    if ((fl & 1) == 0 && (vne0(w.ld3(bfo)) || vne0(w.ld3(bfo + 4)))) {
-      m3 R = qmat(w.pose_q((int)b));
-      v += w.ld3(bfo) * w.body_inv_mass((int)b) * dt;
-      wv += mul(R, mul(w.body_inv_inertia((int)b), mul(tr(R), w.ld3(bfo + 4)))) * dt;
-    }
-    m3 I = w.body_inertia((int)b);
-    bool iso = I.c[0].x == I.c[1].y && I.c[1].y == I.c[2].z && I.c[0].y == 0.0f && I.c[0].z == 0.0f && I.c[1].z == 0.0f;
-    if ((fl & 1) == 0 && !iso && vne0(wv)) {
-      m3 R = qmat(w.pose_q((int)b));
-      wv = mul(R, sv.gyroscopic_velocity(I, mul(tr(R), wv), dt));
-    }
-    sv.tw[b * 2] = F4(v, 0.0f); sv.tw[b * 2 + 1] = F4(wv, 0.0f);
-  }
+  if (PRED)
+    for (uint32_t b = tid; b < NB; b += NT) predict_body(sv, wd, b, g, dt);
   for (uint32_t i = tid; i < 2 * ND; i += NT) art_[w.h(W_A_LIM) + i] = 0.0f;
   for (uint32_t i = tid; i < ND; i += NT) { art_[w.h(W_A_FRIC) + i] = 0.0f; art_[w.h(W_A_DRV) + i] = 0.0f; }
   const uint32_t nslot = min(sched[S.CSTART + NCOL + 1], MAXC);
@@ -4064,7 +4071,7 @@ DEV void solve_world(const TM& tm, const WorldDev wd, uint32_t* Hs, float4* s_dy
   }
 }
 
-template <int BS, bool TWL, bool MGL = false>
+template <int BS, bool TWL, bool MGL = false, bool PRED = true>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_WPE_SOLVE))) k_solve(const WorldDev* __restrict__ worlds, uint32_t nworlds) {
   __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
   __shared__ TeamLds<BS> tl;
@@ -4072,7 +4079,23 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STEEL_W
   const uint32_t wid = blockIdx.x;
   if (wid >= nworlds) return;
   const BlockTeam<BS> tm{threadIdx.x, &tl, nullptr};
-  solve_world<BS, TWL, MGL>(tm, worlds[wid], Hs, s_dyn4);
+  solve_world<BS, TWL, MGL, BlockTeam<BS>, PRED>(tm, worlds[wid], Hs, s_dyn4);
+}
+
+// The body predictor of world 0 over the device (a large world alone in its batch, twists and impulses in its
+// buffers), before its k_solve<1024, false, true, false>.
+__global__ void __launch_bounds__(256) k_solve_pred(const WorldDev* __restrict__ world) {
+  __shared__ uint32_t Hs[STEEL_WORLD_HEADER_WORDS];
+  const WorldDev wd = world[0];
+  load_header(Hs, wd);
+  Solver sv;
+  sv.w = make_world(Hs, wd);
+  sv.tw = wd.tw;
+  sv.imp = wd.imp;
+  const float dt = sv.w.hf(W_DT);
+  const f3 g = mk3(sv.w.hf(W_GX), sv.w.hf(W_GY), sv.w.hf(W_GZ));
+  const uint32_t NB = sv.w.NB();
+  for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < NB; b += gridDim.x * 256) predict_body(sv, wd, b, g, dt);
 }
 
 // One large world over a cooperative launch: twists, impulses and the link mirror in global memory (g: grid team
@@ -4142,7 +4165,12 @@ hipError_t steel_launch_substep(const WorldDev* worlds, uint32_t nworlds, uint32
   if (marks) (void)hipEventRecord(marks[2], stream);
   if ((stages & STAGE_SOLVE) && block == 1024) {
     // sixteen waves per world, for large worlds: the global-mirror instantiation serves every world
-    hipLaunchKernelGGL((k_solve<1024, false, true>), grid, dim3(1024), sl.notw, stream, worlds, nworlds);
+    if (nworlds == 1) {  // one large world: its body predictor over the device first
+      hipLaunchKernelGGL(k_solve_pred, dim3(512), dim3(256), 0, stream, worlds);
+      hipLaunchKernelGGL((k_solve<1024, false, true, false>), grid, dim3(1024), sl.notw, stream, worlds, nworlds);
+    } else {
+      hipLaunchKernelGGL((k_solve<1024, false, true>), grid, dim3(1024), sl.notw, stream, worlds, nworlds);
+    }
   } else if ((stages & STAGE_SOLVE) && sl.mgl) {
     if (block == 64) hipLaunchKernelGGL((k_solve<64, false, true>), grid, dim3(64), solve_lds, stream, worlds, nworlds);
     else if (block == 128) hipLaunchKernelGGL((k_solve<128, false, true>), grid, dim3(128), solve_lds, stream, worlds, nworlds);
